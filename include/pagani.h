@@ -1,0 +1,182 @@
+/*
+ * pagani.h -- C ABI of libpagani.so, the MI355X (gfx950) implementation of the
+ * PAGANI network-construction-and-metrics path.
+ *
+ * One context per host thread; a context owns a HIP stream and a device arena
+ * that is reused across calls.  Every pointer argument may be host memory or
+ * device memory (hipMalloc / torch CUDA tensors): the library detects which
+ * and stages host buffers through the arena.  Functions never exit or prompt;
+ * they return a pg_status and leave a message in pg_last_error().
+ *
+ * Each entry point names the reference interface it replaces.  Reference path
+ * prefixes: U/ = src/BNAPlatform-win64-cuda7.0-20151118/src/,
+ *           W/ = src/BNAPlatform-weighted-network-win64-cuda7.0-20151118/src/.
+ */
+#ifndef PAGANI_H
+#define PAGANI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PAGANI_ABI_VERSION 1
+
+typedef enum pg_status {
+    PG_OK = 0,
+    PG_EINVAL = 1,        /* bad argument */
+    PG_ENOMEM = 2,        /* device allocation failed */
+    PG_EDEVICE = 3,       /* HIP runtime error */
+    PG_EFORMAT = 4,       /* malformed CSR (row[n] != nnz, unsorted, ...) */
+    PG_ETHRESH = 5,       /* threshold out of range / selection mismatch */
+    PG_ENOTSUP = 6,       /* size beyond what this build handles */
+    PG_ESTATE = 7         /* call order (e.g. pg_corr_csr before pg_corr_build) */
+} pg_status;
+
+typedef struct pg_ctx pg_ctx;
+
+/* Create a context on HIP device `device` (replaces the device pick of
+ * U/BC/CUBC/CUBC.cpp:17-38 and findCudaDevice in W modularity_GPU.cu:409). */
+pg_status pg_create(pg_ctx **ctx, int device);
+void pg_destroy(pg_ctx *ctx);
+const char *pg_last_error(const pg_ctx *ctx);
+const char *pg_status_string(pg_status s);
+int pg_abi_version(void);
+/* Wait for all work queued by this context. */
+pg_status pg_synchronize(pg_ctx *ctx);
+
+/* ------------------------------------------------------------ construction */
+
+typedef enum pg_prec {
+    PG_PREC_FP32 = 0,     /* FP32 Gram on MFMA (v_mfma_f32_32x32x2_f32) */
+    PG_PREC_BF16X3 = 1    /* split z = hi + lo (bf16); hi.hi + hi.lo + lo.hi on MFMA */
+} pg_prec;
+
+typedef enum pg_thr_kind {
+    PG_THR_R = 0,         /* keep r > thr (weighted: r > thr - 1e-6) */
+    PG_THR_SPARSITY = 1   /* keep the k largest r, k from the sparsity percent */
+} pg_thr_kind;
+
+typedef enum pg_layout {
+    PG_BOLD_TIME_MAJOR = 0,   /* bold[l*N + i]: the reference's BOLD gather order */
+    PG_BOLD_VOXEL_MAJOR = 1   /* bold[i*T + l] */
+} pg_layout;
+
+typedef struct pg_corr_opts {
+    int32_t thr_kind;     /* pg_thr_kind */
+    float thr;            /* r threshold, or sparsity percent in (0, 100] */
+    int32_t weighted;     /* 1: CSR values = r (float of the FP64 r) */
+    int32_t prec;         /* pg_prec */
+    int32_t layout;       /* pg_layout of the bold argument */
+} pg_corr_opts;
+
+typedef struct pg_corr_info {
+    int32_t n;            /* voxels */
+    int32_t t;            /* time points */
+    int64_t nnz;          /* 2 x undirected edges = row[n] */
+    float r_thresh;       /* r threshold as the reference names the file */
+    float s_thresh;       /* sparsity percent as the reference names the file */
+    double t_star;        /* FP64 k-th largest r (sparsity mode), else thr */
+    double band;          /* proven bound on |r_fast - r_fp64| used for rechecks */
+    int64_t band_pairs;   /* pairs recomputed in FP64 */
+    int64_t band_flips;   /* rechecked pairs whose fast-path decision was wrong */
+    int64_t gram_passes;  /* Gram sweeps run (1 r-mode, 2 sparsity, +1 on overflow) */
+    double ms_zscore, ms_gram, ms_select, ms_csr;  /* device time per stage */
+} pg_corr_info;
+
+/* Voxel time series -> thresholded CSR adjacency, on the device.
+ * Replaces CorMat_gpu (U/CorMat/CUCorMat/CUCORMAT/CorMat.cu:18) together with
+ * post_block (U CorMat.cpp:775), FormFullGraph (U CorMat.cpp:633,
+ * W CorMat.cpp:647), FormFullGraph_s (U CorMat.cpp:693) and FormCSRGraph
+ * (U CorMat.cpp:839, W CorMat.cpp:853).  The CSR stays in the context until
+ * the next pg_corr_build; read it with pg_corr_csr or pg_corr_csr_view. */
+pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, int32_t t,
+                        const pg_corr_opts *opts, pg_corr_info *info);
+
+/* Copy the last CSR out: row[n+1], col[nnz], val[nnz] (val may be NULL; it is
+ * only written for weighted builds).  Host or device destinations. */
+pg_status pg_corr_csr(pg_ctx *ctx, int32_t *row, int32_t *col, float *val);
+
+/* Device pointers of the last CSR, owned by the context (val NULL unless weighted). */
+pg_status pg_corr_csr_view(pg_ctx *ctx, const int32_t **row, const int32_t **col,
+                           const float **val);
+
+/* FP64 z-scored series of the last build, voxel-major [n][t] (device pointer). */
+pg_status pg_corr_z64_view(pg_ctx *ctx, const double **z64);
+
+/* ----------------------------------------------------------------- metrics */
+/* CSR arguments: row[n+1] (row[0] = 0, row[n] = nnz), col[nnz] ascending within
+ * each row, symmetric, no self loops; val[nnz] FP32 weights where used. */
+
+/* Degree (U/Degree/Degree/Degree.cpp:84-91).  deg: exact int32 (may be NULL);
+ * deg_ref: the float the reference writes, (float)R[k+1] - R[k] evaluated in
+ * float (may be NULL). */
+pg_status pg_degree(pg_ctx *ctx, const int32_t *row, int32_t n, int32_t *deg, float *deg_ref);
+
+/* Strength: FP32 running sum of each CSR row (W/Degree/Degree/Degree.cpp:84-88). */
+pg_status pg_strength(pg_ctx *ctx, const int32_t *row, const float *val, int32_t n, float *s);
+
+/* Triangles through each vertex by sorted-adjacency intersection (the count
+ * U/Cp/Cp/Cp.cpp:102-108 forms with a dense lookup). */
+pg_status pg_triangles(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
+                       int64_t nnz, int64_t *tri);
+
+/* Binary clustering coefficient, Cp = (double)T/k/(k-1)*2 as float
+ * (replaces Cp, U/Cp/Cp/Cp.cpp:21-115).  mean: FP64 mean of cp (may be NULL). */
+pg_status pg_cp_binary(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
+                       int64_t nnz, float *cp, double *mean);
+
+typedef enum pg_cp_kind { PG_CP_BARRAT = 1, PG_CP_ONNELA = 2 } pg_cp_kind;
+
+/* Weighted clustering in FP64: Barrat (W/Cp/Cp/Cp_1.cpp:24-131) or Onnela
+ * (W/Cp/Cp/Cp_2.cpp:23-129, W/Cp_GPU/CUCP/Cp.cu:111).  wmax_mode 0 normalises
+ * Onnela weights by the largest weight, 1 by the largest of val[0..n-1]
+ * (the reference CPU tool).  mean may be NULL. */
+pg_status pg_cp_weighted(pg_ctx *ctx, const int32_t *row, const int32_t *col,
+                         const float *val, int32_t n, int64_t nnz, int32_t kind,
+                         int32_t wmax_mode, double *cp, double *mean);
+
+/* Brandes betweenness, exact: uint64 path counts, FP64 dependencies, summed
+ * over ordered pairs and unnormalised like the reference.  sources == NULL
+ * means every vertex.  Replaces CUBC (U/BC/CUBC/CUBC.cpp:13) and
+ * Betweenness_GPU_node_pred (U/BC/CUBC/BC_GPU.h:9). */
+pg_status pg_bc(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
+                int64_t nnz, const int32_t *sources, int64_t nsrc, double *bc);
+
+/* BFS hop levels for nsrc sources: level[s*n + v], -1 where unreachable
+ * (the BFS of U/BFS_MulCPU/BFS_MulCPU/APSP_BFS.cpp:150-192). */
+pg_status pg_bfs(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
+                 int64_t nnz, const int32_t *sources, int64_t nsrc, int32_t *level);
+
+/* Blocked Floyd-Warshall closure of a dense row-major n x n FP32 distance
+ * matrix, in place (+inf = no edge).  Replaces cuAPSP (W/Lp/CUBFW_Lp/CUBFW_Lp/
+ * CUBFW.cpp:19) and the tile kernels of CUBFW_ph1/2/3.cu.  block: tile edge
+ * (0 = default 64); results are bit-identical to the blocked algorithm of
+ * W BFW.cpp:40-108 run with the same tile edge. */
+pg_status pg_fw(pg_ctx *ctx, float *d, int32_t n, int32_t block);
+
+/* Dense distance matrix from a CSR (W/Lp/CUBFW_Lp/CUBFW_Lp/FormBlock.cpp:45-75):
+ * kind 0: 1/w (the reference), 1: 1 - w, 2: 1 (hops); +inf where no edge, 0 on
+ * the diagonal.  d is n x n row-major. */
+pg_status pg_fw_init(pg_ctx *ctx, const int32_t *row, const int32_t *col, const float *val,
+                     int32_t n, int64_t nnz, int32_t kind, float *d);
+
+typedef enum pg_mod_kmode {
+    PG_MOD_STRENGTH = 0,  /* k = strength, 2m = sum of weights (binary if val NULL) */
+    PG_MOD_BINARY_K = 1   /* k = degree, 2m = nnz, A weighted: the reference's tool */
+} pg_mod_kmode;
+
+/* Newman leading-eigenvector modularity by recursive bisection (replaces
+ * Partition_GPU, W/Newman_Modularity_GPU/CUmodularity/modularity_GPU.cu:405).
+ * labels: 1..K.  q, lambda0 (round-0 leading eigenvalue), eig0 (round-0
+ * vector, n doubles) and rounds may be NULL. */
+pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_t *col,
+                        const float *val, int32_t n, int64_t nnz, int32_t kmode,
+                        int32_t *labels, double *q, double *lambda0, double *eig0,
+                        int32_t *rounds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PAGANI_H */

@@ -1,0 +1,660 @@
+/*
+ * oracle.c -- FP64 CPU oracle for the PAGANI construction-and-metrics path.
+ *
+ * TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load liboracle.so, and only as the checker.  Nothing in
+ * the product links it.
+ *
+ * Reference path prefixes:
+ *   U/ = src/BNAPlatform-win64-cuda7.0-20151118/src/
+ *   W/ = src/BNAPlatform-weighted-network-win64-cuda7.0-20151118/src/
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   orc_zscore          closed forms (mean 0, unit L2 norm, z of an affine
+ *                       series), numpy FP64 of the same expression
+ *   orc_r / corr_csr_*  numpy corrcoef FP64 (1e-12), r(x, a x + b) = sign(a),
+ *                       brute-force all-pairs threshold with injected exact
+ *                       ties (duplicate voxels) for the sparsity tie rule
+ *   orc_sparsity_k      the reference's float expression evaluated with numpy
+ *                       float32 scalars
+ *   orc_degree          R differences; the float expression with numpy float32
+ *   orc_strength        numpy float32 sequential sum
+ *   orc_triangles/cp    networkx 3.4.2 triangles/clustering fixtures, complete
+ *                       graph (Cp = 1), star (Cp = 0), trace(A^3)/6
+ *   orc_cp_weighted     networkx clustering(weight=) fixtures (Onnela), brute
+ *                       force dense Python loops (Barrat)
+ *   orc_bc              networkx betweenness x2 fixtures, closed forms on paths
+ *                       and stars, the compiled reference Betweenness_CPU
+ *   orc_bfs             networkx shortest_path_length fixtures
+ *   orc_fw_blocked      the compiled reference BFW_C (bit-exact), networkx
+ *                       floyd_warshall_numpy, unit weights == BFS hops
+ *   orc_modularity      disjoint cliques split exactly, Q against networkx
+ *                       modularity, leading vector against numpy eigh
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------- z-score */
+
+/* U CorMat/CUCorMat/CUCORMAT/CorMat.cu:34-53: per-voxel mean and
+ * sqrt(sum (x-mu)^2) in FP64, z = (x - mu) / norm.  A zero-variance voxel gives
+ * 0/0 there, which post_block (U CorMat.cpp:834) turns into r = 0; z = 0 here
+ * yields the same r = 0 directly. */
+void orc_zscore(const float *bold, int64_t n, int t, double *z64, float *z32)
+{
+    for (int64_t i = 0; i < n; i++) {
+        double s1 = 0.0, s2 = 0.0;
+        for (int l = 0; l < t; l++) s1 += (double)bold[(int64_t)l * n + i];
+        s1 /= t;
+        for (int l = 0; l < t; l++) {
+            double d = (double)bold[(int64_t)l * n + i] - s1;
+            double p = d * d;
+            s2 += p;
+        }
+        s2 = sqrt(s2);
+        for (int l = 0; l < t; l++) {
+            double z = (s2 > 0.0) ? ((double)bold[(int64_t)l * n + i] - s1) / s2 : 0.0;
+            if (z64) z64[i * t + l] = z;
+            if (z32) z32[i * t + l] = (float)z;
+        }
+    }
+}
+
+/* r = sum_t z_i[t] z_j[t], sequential, each product rounded before the add
+ * (the file is built with -ffp-contract=off).  Clamped to [-1, 1]: the
+ * reference drops |r| > 1 from rounding (U CorMat.cpp:834). */
+double orc_r(const double *z64, int t, int64_t i, int64_t j)
+{
+    const double *a = z64 + i * t, *b = z64 + j * t;
+    double s = 0.0;
+    for (int l = 0; l < t; l++) {
+        double p = a[l] * b[l];
+        s += p;
+    }
+    if (s > 1.0) s = 1.0;
+    if (s < -1.0) s = -1.0;
+    return s;
+}
+
+/* U CorMat.cpp:646 tests r > thr; W CorMat.cpp:310 passes thr - 1e-6 (double)
+ * into a float parameter, so the weighted threshold is rounded to float. */
+float orc_r_threshold_eff(float thr, int weighted)
+{
+    if (!weighted) return thr;
+    return (float)((double)thr - 1e-6);
+}
+
+/* U CorMat.cpp:311-313:  Clength = (long long) 2*M1*s/100.0; Clength += Clength%2.
+ * 2*M1 (long long) times a float is a float product; /100.0 is double. */
+int64_t orc_sparsity_k(int64_t n, float s_percent)
+{
+    int64_t m1 = (n - 1) * n / 2;
+    float prod = (float)(2 * m1) * s_percent;
+    int64_t clen = (int64_t)((double)prod / 100.0);
+    clen += clen % 2;
+    return clen / 2;
+}
+
+int64_t orc_corr_count_r(const double *z64, int64_t n, int t, float thr_eff)
+{
+    int64_t cnt = 0;
+    const double th = (double)thr_eff;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : cnt)
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = i + 1; j < n; j++)
+            if (orc_r(z64, t, i, j) > th) cnt++;
+    return 2 * cnt;
+}
+
+/* FormFullGraph + FormCSRGraph (U CorMat.cpp:633-653, 839-857; weighted V from
+ * W CorMat.cpp:853-885): columns ascending, both halves, no self loops. */
+void orc_corr_csr_r(const double *z64, int64_t n, int t, float thr_eff,
+                    int32_t *row, int32_t *col, float *val)
+{
+    const double th = (double)thr_eff;
+    int64_t pos = 0;
+    row[0] = 0;
+    for (int64_t i = 0; i < n; i++) {
+        for (int64_t j = 0; j < n; j++) {
+            if (j == i) continue;
+            double r = (i < j) ? orc_r(z64, t, i, j) : orc_r(z64, t, j, i);
+            if (r > th) {
+                col[pos] = (int32_t)j;
+                if (val) val[pos] = (float)r;
+                pos++;
+            }
+        }
+        row[i + 1] = (int32_t)pos;
+    }
+}
+
+/* ------------------------------------------------------- sparsity select */
+
+typedef struct { double r; int64_t idx; } rpair;
+
+static void swapd(double *a, int64_t i, int64_t j) { double x = a[i]; a[i] = a[j]; a[j] = x; }
+
+/* k-th largest (1-based) of a[0..m) by quickselect on a scratch copy. */
+static double kth_largest(double *a, int64_t m, int64_t k)
+{
+    int64_t lo = 0, hi = m - 1, want = k - 1;
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    while (lo < hi) {
+        st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+        int64_t p = lo + (int64_t)(st % (uint64_t)(hi - lo + 1));
+        double pv = a[p];
+        /* three-way partition, descending: [> pv][== pv][< pv] */
+        int64_t lt = lo, i = lo, gt = hi;
+        while (i <= gt) {
+            if (a[i] > pv) swapd(a, lt++, i++);
+            else if (a[i] < pv) swapd(a, i, gt--);
+            else i++;
+        }
+        if (want < lt) hi = lt - 1;
+        else if (want > gt) lo = gt + 1;
+        else return pv;
+    }
+    return a[lo];
+}
+
+/* FormFullGraph_s (U CorMat.cpp:693-773): t* = k-th largest r; edges are all
+ * r > t*, then ties r == t* in row-major (i<j) order until k edges (the
+ * reference pushes ties on a stack and pops the newest when over count). */
+double orc_corr_csr_s(const double *z64, int64_t n, int t, int64_t k,
+                      int32_t *row, int32_t *col, float *val)
+{
+    int64_t m1 = (n - 1) * n / 2;
+    double *all = (double *)malloc(sizeof(double) * (size_t)(m1 > 0 ? m1 : 1));
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; i++) {
+        int64_t base = i * n - i * (i + 1) / 2; /* packed index of (i, i+1) */
+        for (int64_t j = i + 1; j < n; j++) all[base + (j - i - 1)] = orc_r(z64, t, i, j);
+    }
+    double tstar = 1.0;
+    unsigned char *edge = NULL;
+    if (k > 0 && k <= m1) {
+        double *cp = (double *)malloc(sizeof(double) * (size_t)m1);
+        memcpy(cp, all, sizeof(double) * (size_t)m1);
+        tstar = kth_largest(cp, m1, k);
+        free(cp);
+        int64_t greater = 0;
+        for (int64_t q = 0; q < m1; q++) greater += all[q] > tstar;
+        int64_t ties_left = k - greater;
+        edge = (unsigned char *)calloc((size_t)m1, 1);
+        for (int64_t q = 0; q < m1; q++) {
+            if (all[q] > tstar) edge[q] = 1;
+            else if (all[q] == tstar && ties_left > 0) { edge[q] = 1; ties_left--; }
+        }
+    }
+    int64_t pos = 0;
+    row[0] = 0;
+    for (int64_t i = 0; i < n; i++) {
+        for (int64_t j = 0; j < n; j++) {
+            if (j == i || !edge) continue;
+            int64_t a = i < j ? i : j, b = i < j ? j : i;
+            int64_t q = a * n - a * (a + 1) / 2 + (b - a - 1);
+            if (edge[q]) {
+                col[pos] = (int32_t)j;
+                if (val) val[pos] = (float)all[q];
+                pos++;
+            }
+        }
+        row[i + 1] = (int32_t)pos;
+    }
+    free(edge);
+    free(all);
+    return tstar;
+}
+
+/* ----------------------------------------------------------------- degree */
+
+/* U Degree/Degree/Degree.cpp:87: Degree[k] = (float) R[k+1] - R[k], a float
+ * subtraction of two rounded row offsets (inexact once offsets pass 2^24). */
+void orc_degree(const int32_t *row, int64_t n, int32_t *deg, float *deg_ref)
+{
+    for (int64_t i = 0; i < n; i++) {
+        if (deg) deg[i] = row[i + 1] - row[i];
+        if (deg_ref) {
+            volatile float a = (float)row[i + 1];
+            volatile float b = (float)row[i];
+            deg_ref[i] = a - b;
+        }
+    }
+}
+
+/* W Degree/Degree/Degree.cpp:84-88: FP32 running sum in CSR order. */
+void orc_strength(const int32_t *row, const float *val, int64_t n, float *s)
+{
+    for (int64_t i = 0; i < n; i++) {
+        float acc = 0.0f;
+        for (int32_t e = row[i]; e < row[i + 1]; e++) acc += val[e];
+        s[i] = acc;
+    }
+}
+
+/* ------------------------------------------------------------- clustering */
+
+/* |N(a) ∩ N(b)| restricted to members > floor, by merging sorted rows. */
+static int64_t isect_count(const int32_t *col, int32_t a0, int32_t a1, int32_t b0, int32_t b1)
+{
+    int64_t c = 0;
+    while (a0 < a1 && b0 < b1) {
+        int32_t x = col[a0], y = col[b0];
+        if (x < y) a0++;
+        else if (y < x) b0++;
+        else { c++; a0++; b0++; }
+    }
+    return c;
+}
+
+/* Triangles through each vertex, T_v = #{(i<j) in N(v) : a_ij}, as counted by
+ * the pair loop of U Cp/Cp/Cp.cpp:102-108 (here by sorted intersection). */
+void orc_triangles(const int32_t *row, const int32_t *col, int64_t n, int64_t *tri)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t v = 0; v < n; v++) {
+        int64_t s = 0;
+        for (int32_t e = row[v]; e < row[v + 1]; e++) {
+            int32_t u = col[e];
+            s += isect_count(col, row[v], row[v + 1], row[u], row[u + 1]);
+        }
+        tri[v] = s / 2;
+    }
+}
+
+/* U Cp/Cp/Cp.cpp:95-111: Cp[k] = (double)T/k/(k-1)*2 stored as float, 0 for
+ * k < 2; returns the FP64 mean of the stored floats. */
+double orc_cp_binary(const int32_t *row, const int32_t *col, int64_t n, float *cp)
+{
+    int64_t *tri = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    orc_triangles(row, col, n, tri);
+    double mean = 0.0;
+    for (int64_t v = 0; v < n; v++) {
+        int64_t k = row[v + 1] - row[v];
+        float c = 0.0f;
+        if (k >= 2) c = (float)((double)tri[v] / (double)k / (double)(k - 1) * 2.0);
+        cp[v] = c;
+        mean += c;
+    }
+    free(tri);
+    return n > 0 ? mean / (double)n : 0.0;
+}
+
+/* weight of edge (a,b) by binary search of a's sorted row; 0 if absent. */
+static int find_w(const int32_t *row, const int32_t *col, const float *val,
+                  int32_t a, int32_t b, double *w)
+{
+    int32_t lo = row[a], hi = row[a + 1] - 1;
+    while (lo <= hi) {
+        int32_t mid = lo + (hi - lo) / 2;
+        if (col[mid] == b) { *w = (double)val[mid]; return 1; }
+        if (col[mid] < b) lo = mid + 1; else hi = mid - 1;
+    }
+    return 0;
+}
+
+/* Barrat (W Cp/Cp/Cp_1.cpp:109-127):
+ *   C_k = sum_{i<j in N(k), w_ij != 0} (w_ki + w_kj) / (deg_k - 1) / s_k
+ * Onnela (W Cp/Cp/Cp_2.cpp:117-125, weights divided by the maximum weight,
+ * W Cp/Cp/Cp_2.cpp:27-30 and W Cp_GPU/CUCP/Cp.cu:223-239):
+ *   C_k = 2 sum_{i<j in N(k), w_ij != 0} (w^_ij w^_ki w^_kj)^(1/3) / deg_k / (deg_k - 1)
+ * Both are 0 for deg_k < 2.  Returns the mean. */
+double orc_cp_weighted(const int32_t *row, const int32_t *col, const float *val,
+                       int64_t n, int kind, int wmax_mode, double *cp)
+{
+    double wmax = 0.0;
+    int64_t nnz = row[n];
+    int64_t lim = (wmax_mode == 1) ? (n < nnz ? n : nnz) : nnz;
+    if (lim > 0) wmax = val[0];
+    for (int64_t e = 1; e < lim; e++) if (val[e] > wmax) wmax = val[e];
+    double mean = 0.0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : mean)
+    for (int64_t k = 0; k < n; k++) {
+        int32_t b = row[k], e = row[k + 1];
+        int64_t deg = e - b;
+        if (deg < 2) { cp[k] = 0.0; continue; }
+        double sk = 0.0;
+        for (int32_t q = b; q < e; q++) sk += (double)val[q];
+        double acc = 0.0;
+        for (int32_t p = b; p < e; p++) {
+            int32_t i = col[p];
+            double wki = (double)val[p];
+            /* j in N(k) ∩ N(i), j > i */
+            int32_t a0 = p + 1, a1 = e, b0 = row[i], b1 = row[i + 1];
+            while (a0 < a1 && b0 < b1) {
+                int32_t x = col[a0], y = col[b0];
+                if (x < y) a0++;
+                else if (y < x) b0++;
+                else {
+                    double wij = (double)val[b0], wkj = (double)val[a0];
+                    if (wij != 0.0) {
+                        if (kind == 1) acc += wki + wkj;
+                        else acc += pow((wij / wmax) * (wki / wmax) * (wkj / wmax), 1.0 / 3.0);
+                    }
+                    a0++; b0++;
+                }
+            }
+        }
+        double c;
+        if (kind == 1) c = acc / (double)(deg - 1) / sk;
+        else c = 2.0 * acc / (double)deg / (double)(deg - 1);
+        cp[k] = c;
+        mean += c;
+    }
+    (void)find_w;
+    return n > 0 ? mean / (double)n : 0.0;
+}
+
+/* -------------------------------------------------------------------- BC */
+
+/* U BC/BC_CPU/BC_CPU/BC_CPU.cpp:22-75 with uint64 sigma and FP64 delta:
+ * BFS from s (queue, CSR order), predecessor lists in discovery order, then
+ * the stack in reverse BFS order: delta[v] += sigma[v]/sigma[w]*(1+delta[w]),
+ * BC[w] += delta[w] for w != s.  Summed over ordered pairs (no /2). */
+void orc_bc(const int32_t *row, const int32_t *col, int64_t n,
+            const int32_t *sources, int64_t nsrc, double *bc)
+{
+    int64_t nnz = row[n];
+    if (!sources) nsrc = n;
+    for (int64_t v = 0; v < n; v++) bc[v] = 0.0;
+#pragma omp parallel
+    {
+        int32_t *dist = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+        uint64_t *sig = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(n + 1));
+        double *del = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+        int32_t *pcnt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+        int32_t *pred = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz + 1));
+        int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+        double *part = (double *)calloc((size_t)(n + 1), sizeof(double));
+#pragma omp for schedule(static)
+        for (int64_t si = 0; si < nsrc; si++) {
+            int32_t s = sources ? sources[si] : (int32_t)si;
+            for (int64_t v = 0; v < n; v++) { dist[v] = -1; sig[v] = 0; del[v] = 0.0; pcnt[v] = 0; }
+            sig[s] = 1; dist[s] = 0;
+            int64_t head = 0, tail = 0;
+            order[tail++] = s;
+            while (head < tail) {
+                int32_t v = order[head++];
+                for (int32_t e = row[v]; e < row[v + 1]; e++) {
+                    int32_t w = col[e];
+                    if (dist[w] < 0) { dist[w] = dist[v] + 1; order[tail++] = w; }
+                    if (dist[w] == dist[v] + 1) {
+                        sig[w] += sig[v];
+                        pred[row[w] + pcnt[w]] = v;
+                        pcnt[w]++;
+                    }
+                }
+            }
+            for (int64_t q = tail - 1; q >= 0; q--) {
+                int32_t w = order[q];
+                for (int32_t p = 0; p < pcnt[w]; p++) {
+                    int32_t v = pred[row[w] + p];
+                    del[v] = del[v] + 1.0 * (double)sig[v] / (double)sig[w] * (1.0 + del[w]);
+                }
+                if (w != s) part[w] += del[w];
+            }
+        }
+#pragma omp critical
+        {
+            for (int64_t v = 0; v < n; v++) bc[v] += part[v];
+        }
+        free(dist); free(sig); free(del); free(pcnt); free(pred); free(order); free(part);
+    }
+}
+
+/* U BFS_MulCPU/BFS_MulCPU/APSP_BFS.cpp:163-190: queue BFS, dist[w] = dist[v]+1. */
+void orc_bfs(const int32_t *row, const int32_t *col, int64_t n, int32_t src, int32_t *level)
+{
+    int32_t *q = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    for (int64_t v = 0; v < n; v++) level[v] = -1;
+    int64_t head = 0, tail = 0;
+    level[src] = 0;
+    q[tail++] = src;
+    while (head < tail) {
+        int32_t v = q[head++];
+        for (int32_t e = row[v]; e < row[v + 1]; e++) {
+            int32_t w = col[e];
+            if (level[w] < 0) { level[w] = level[v] + 1; q[tail++] = w; }
+        }
+    }
+    free(q);
+}
+
+/* ----------------------------------------------------------- Floyd-Warshall */
+
+/* W Lp/CUBFW_Lp/CUBFW_Lp/BFW.cpp:26-38: for k, i, j: dst = min(dst, ik + kj). */
+static void fw_block(float *d, int64_t ld, int64_t b,
+                     int64_t dr, int64_t dc, int64_t ir, int64_t ic, int64_t kr, int64_t kc)
+{
+    for (int64_t k = 0; k < b; k++)
+        for (int64_t i = 0; i < b; i++) {
+            float ik = d[(ir + i) * ld + ic + k];
+            float *dst = d + (dr + i) * ld + dc;
+            const float *kj = d + (kr + k) * ld + kc;
+            for (int64_t j = 0; j < b; j++) {
+                float c = ik + kj[j];
+                if (c < dst[j]) dst[j] = c;
+            }
+        }
+}
+
+/* W BFW.cpp:40-108 (phases 1, 2 row then column, 3) on an n x n row-major
+ * matrix padded to a multiple of b with +inf (0 on the padded diagonal), as
+ * W FormBlock.cpp:45-75 pads.  Phase 3 tiles are independent. */
+void orc_fw_blocked(float *d, int64_t n, int64_t b)
+{
+    int64_t np = (n + b - 1) / b * b, nb = np / b;
+    float *m = (float *)malloc(sizeof(float) * (size_t)(np * np));
+    for (int64_t i = 0; i < np; i++)
+        for (int64_t j = 0; j < np; j++)
+            m[i * np + j] = (i < n && j < n) ? d[i * n + j] : (i == j ? 0.0f : INFINITY);
+    for (int64_t kb = 0; kb < nb; kb++) {
+        int64_t K = kb * b;
+        fw_block(m, np, b, K, K, K, K, K, K);
+        for (int64_t c = 0; c < nb; c++) if (c != kb) fw_block(m, np, b, K, c * b, K, K, K, c * b);
+        for (int64_t r = 0; r < nb; r++) if (r != kb) fw_block(m, np, b, r * b, K, r * b, K, K, K);
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int64_t r = 0; r < nb; r++)
+            for (int64_t c = 0; c < nb; c++)
+                if (r != kb && c != kb) fw_block(m, np, b, r * b, c * b, r * b, K, K, c * b);
+    }
+    for (int64_t i = 0; i < n; i++) memcpy(d + i * n, m + i * np, sizeof(float) * (size_t)n);
+    free(m);
+}
+
+/* W FormBlock.cpp:49-74 in row-major: +inf, d_ij from the edge weight, 0 on
+ * the diagonal.  kind 0: 1/w (the reference), 1: 1-w, 2: 1 (hops). */
+void orc_fw_init(const int32_t *row, const int32_t *col, const float *val, int64_t n,
+                 int kind, float *d)
+{
+    for (int64_t q = 0; q < n * n; q++) d[q] = INFINITY;
+    for (int64_t i = 0; i < n; i++)
+        for (int32_t e = row[i]; e < row[i + 1]; e++) {
+            float w = val ? val[e] : 1.0f;
+            float x = kind == 0 ? 1.0f / w : (kind == 1 ? 1.0f - w : 1.0f);
+            d[i * n + col[e]] = x;
+        }
+    for (int64_t i = 0; i < n; i++) d[i * n + i] = 0.0f;
+}
+
+/* -------------------------------------------------------------- modularity */
+
+typedef struct {
+    int64_t ng;          /* group size */
+    int32_t *g;          /* global ids, ascending */
+    int32_t *srow;       /* sub-CSR over members, local column ids */
+    int32_t *scol;
+    double *sval;
+    double *kg;          /* k_i of members */
+    double *sumbg;       /* k^(g)_i - k_i K_g / 2m */
+} mgroup;
+
+/* W Newman_Modularity_GPU/CUmodularity/Modularity_CPU.cpp:224-298 and
+ * modularity_GPU.cu:225-325: power iteration on B^(g) + beta I from
+ * v_i = global index, until ||v - v0|| or ||v + v0|| <= 1e-6 or 10000 steps;
+ * returns +-||(B + beta I) v0|| with the sign of v0 . v at the largest |v|. */
+static double lead_vector(const mgroup *G, double m2, double beta, double *v, double *v0, int *iters)
+{
+    int64_t ng = G->ng;
+    for (int64_t i = 0; i < ng; i++) v[i] = (double)G->g[i];
+    double err1 = 1.0, err2 = 1.0, vnorm = 0.0;
+    int it = 0;
+    while (err1 > 1e-6 && err2 > 1e-6 && it < 10000) {
+        for (int64_t i = 0; i < ng; i++) v0[i] = v[i];
+        double vk = 0.0;
+        for (int64_t i = 0; i < ng; i++) vk += v0[i] * G->kg[i];
+        for (int64_t i = 0; i < ng; i++) {
+            double acc = 0.0;
+            for (int32_t e = G->srow[i]; e < G->srow[i + 1]; e++) acc += v0[G->scol[e]] * G->sval[e];
+            acc -= vk / m2 * G->kg[i] + (G->sumbg[i] - beta) * v0[i];
+            v[i] = acc;
+        }
+        double s = 0.0;
+        for (int64_t i = 0; i < ng; i++) s += v[i] * v[i];
+        vnorm = sqrt(s);
+        for (int64_t i = 0; i < ng; i++) v[i] = v[i] / vnorm;
+        double e1 = 0.0, e2 = 0.0;
+        for (int64_t i = 0; i < ng; i++) {
+            double a = v[i] - v0[i], b = v[i] + v0[i];
+            e1 += a * a; e2 += b * b;
+        }
+        err1 = sqrt(e1); err2 = sqrt(e2);
+        it++;
+    }
+    int64_t mi = 0;
+    for (int64_t i = 0; i < ng; i++) if (fabs(v[i]) > fabs(v[mi])) mi = i;
+    if (iters) *iters += it;
+    return (v[mi] * v0[mi] > 0) ? vnorm : -vnorm;
+}
+
+static void node_k(const int32_t *row, const float *val, int64_t n, int kmode, double *k, double *m2)
+{
+    double s = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        double ki = 0.0;
+        if (kmode == 1 || !val) ki = (double)(row[i + 1] - row[i]);
+        else for (int32_t e = row[i]; e < row[i + 1]; e++) ki += (double)val[e];
+        k[i] = ki;
+        s += ki;
+    }
+    *m2 = s;
+}
+
+/* W modularity_GPU.cu:405-525 (Partition_GPU) with Sub_Partition
+ * (Modularity_CPU.cpp:164-208 / modularity_GPU.cu:329-397).  Groups are
+ * processed in label order Round = 0, 1, 2, ...; a split writes labels
+ * Max+1 (the larger side) and Max+2.  The negative-eigenvalue shift uses the
+ * reference's size rule: groups above 25000 nodes take the GPU test
+ * lambda < -0.01 (modularity_GPU.cu:369, 483), smaller ones lambda < 0
+ * (Modularity_CPU.cpp:183). */
+double orc_modularity(const int32_t *row, const int32_t *col, const float *val,
+                      int64_t n, int kmode, int32_t *labels, double *lambda_out,
+                      double *eig_out, int32_t *nrounds_out)
+{
+    double *k = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double m2 = 0.0;
+    node_k(row, val, n, kmode, k, &m2);
+    int32_t *res = (int32_t *)calloc((size_t)(n + 1), sizeof(int32_t));
+    int32_t *adj = (int32_t *)malloc(sizeof(int32_t) * (size_t)(2 * n + 2));
+    int32_t *loc = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    int64_t nnz = row[n];
+    mgroup G;
+    G.g = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    G.srow = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 2));
+    G.scol = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz + 1));
+    G.sval = (double *)malloc(sizeof(double) * (size_t)(nnz + 1));
+    G.kg = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    G.sumbg = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double *v = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double *v0 = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    int32_t round = 0, maxr = 0, index = 1, rounds = 0;
+    if (m2 <= 0.0) {
+        for (int64_t i = 0; i < n; i++) labels[i] = 1;
+        if (lambda_out) *lambda_out = 0.0;
+        if (nrounds_out) *nrounds_out = 0;
+        goto done;
+    }
+    for (int64_t i = 0; i < n; i++) loc[i] = -1;
+    while (round <= maxr) {
+        int64_t ng = 0;
+        double Kg = 0.0;
+        for (int64_t i = 0; i < n; i++) if (res[i] == round) { loc[i] = (int32_t)ng; G.g[ng++] = (int32_t)i; Kg += k[i]; }
+        if (ng > 0) {
+            rounds++;
+            G.ng = ng;
+            int64_t pos = 0;
+            G.srow[0] = 0;
+            for (int64_t a = 0; a < ng; a++) {
+                int32_t i = G.g[a];
+                double kin = 0.0;
+                for (int32_t e = row[i]; e < row[i + 1]; e++) {
+                    int32_t j = col[e];
+                    if (res[j] != round) continue;
+                    double w = (val && kmode != 2) ? (double)val[e] : 1.0;
+                    G.scol[pos] = loc[j];
+                    G.sval[pos] = w;
+                    kin += (kmode == 1 && val) ? 1.0 : w;
+                    pos++;
+                }
+                G.srow[a + 1] = (int32_t)pos;
+                G.kg[a] = k[i];
+                G.sumbg[a] = kin - k[i] * Kg / m2;
+            }
+            int its = 0;
+            double lam = lead_vector(&G, m2, 0.0, v, v0, &its);
+            int shift = (ng > 25000) ? (lam < -0.01) : (lam < 0.0);
+            if (shift) lam += lead_vector(&G, m2, -lam, v, v0, &its);
+            if (round == 0) {
+                if (lambda_out) *lambda_out = lam;
+                if (eig_out) for (int64_t a = 0; a < ng; a++) eig_out[G.g[a]] = v[a];
+            }
+            int64_t sp = 0, sn = 0;
+            for (int64_t a = 0; a < ng; a++) { sp += v[a] > 0; sn += v[a] <= 0; }
+            int split = lam > 0.01 && sp > 1 && sn > 1;
+            if (split) {
+                for (int64_t a = 0; a < ng; a++)
+                    res[G.g[a]] = maxr + 1 + (v[a] * ((double)(sp - sn) + 0.5) <= 0);
+                maxr += 2;
+            } else {
+                adj[round] = index++;
+            }
+            for (int64_t a = 0; a < ng; a++) loc[G.g[a]] = -1;
+        }
+        round++;
+    }
+    for (int64_t i = 0; i < n; i++) labels[i] = adj[res[i]];
+    if (nrounds_out) *nrounds_out = rounds;
+done:;
+    double q = orc_modularity_q(row, col, val, n, kmode, labels);
+    free(k); free(res); free(adj); free(loc);
+    free(G.g); free(G.srow); free(G.scol); free(G.sval); free(G.kg); free(G.sumbg);
+    free(v); free(v0);
+    return q;
+}
+
+/* W modularity_GPU.cu:500-509: Q = [sum_ij A_ij d(ci,cj) - sum_ij k_i k_j d / 2m] / 2m,
+ * the second sum taken per community (sum of K_c^2). */
+double orc_modularity_q(const int32_t *row, const int32_t *col, const float *val,
+                        int64_t n, int kmode, const int32_t *labels)
+{
+    double *k = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double m2 = 0.0;
+    node_k(row, val, n, kmode, k, &m2);
+    int32_t maxl = 0;
+    for (int64_t i = 0; i < n; i++) if (labels[i] > maxl) maxl = labels[i];
+    double *kc = (double *)calloc((size_t)maxl + 2, sizeof(double));
+    double in = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        kc[labels[i]] += k[i];
+        for (int32_t e = row[i]; e < row[i + 1]; e++)
+            if (labels[col[e]] == labels[i]) in += (kmode == 1 || !val) ? 1.0 : (double)val[e];
+    }
+    double s2 = 0.0;
+    for (int32_t c = 0; c <= maxl; c++) s2 += kc[c] * kc[c];
+    free(k); free(kc);
+    if (m2 <= 0.0) return 0.0;
+    return (in - s2 / m2) / m2;
+}

@@ -1,0 +1,232 @@
+// fw.hip -- blocked Floyd-Warshall APSP on MI355X (gfx950) VALUs.
+//
+// Replaces cuAPSP (W/Lp/CUBFW_Lp/CUBFW_Lp/CUBFW.cpp:19-168) and the phase
+// kernels of CUBFW_ph1.cu / CUBFW_ph2.cu / CUBFW_ph3.cu.  The whole padded
+// matrix stays resident in HBM (the reference streams every tile over PCIe each
+// round).  Per k-block round:
+//   phase 1: the diagonal tile, sequential k in LDS (one work-group);
+//   phase 2: row and column panel tiles, sequential k in LDS;
+//   phase 3: every other tile, min over k of (ik + kj), 8x8 (B=128) or 4x4
+//            (B=64) register micro-tiles over LDS panels.
+// Min-plus is not a dense contraction, so this runs on VALU (v_add_f32 +
+// v_min3_f32), not MFMA.  Each candidate path length is one FP32 add of the
+// same operands as the blocked CPU algorithm (W/Lp/CUBFW_Lp/CUBFW_Lp/BFW.cpp:
+// 40-108) with the same tile edge, and min is exact, so the result is
+// bit-identical to it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "pg_internal.hpp"
+
+namespace {
+
+__device__ __forceinline__ float relax(float cur, float cand) { return cand < cur ? cand : cur; }
+
+// Phase 1/2 tile update with sequential k.  role 0: diagonal tile (ik = kj =
+// self); role 1: row panel (ik = diag, kj = self); role 2: column panel
+// (ik = self, kj = diag).
+template <int B>
+__global__ __launch_bounds__(256) void fw_seq_tiles(float *__restrict__ d, int np, int kb, int phase)
+{
+    __shared__ float self[B][B + 1];
+    __shared__ float diag[B][B + 1];
+    const int tid = threadIdx.x;
+    int tr, tc, role;
+    const int nbk = np / B;
+    if (phase == 1) {
+        tr = kb;
+        tc = kb;
+        role = 0;
+    } else {
+        int t = blockIdx.x;  // 0 .. 2*(nbk-1)-1
+        int h = nbk - 1;
+        if (t < h) { tr = kb; tc = t < kb ? t : t + 1; role = 1; }
+        else { t -= h; tr = t < kb ? t : t + 1; tc = kb; role = 2; }
+    }
+    const long long r0 = (long long)tr * B, c0 = (long long)tc * B, k0 = (long long)kb * B;
+    for (int e = tid; e < B * B; e += 256) {
+        int i = e / B, j = e % B;
+        self[i][j] = d[(r0 + i) * np + c0 + j];
+        if (role != 0) diag[i][j] = d[(k0 + i) * np + k0 + j];
+    }
+    __syncthreads();
+    for (int k = 0; k < B; k++) {
+        for (int e = tid; e < B * B; e += 256) {
+            int i = e / B, j = e % B;
+            float ik, kj;
+            if (role == 0) { ik = self[i][k]; kj = self[k][j]; }
+            else if (role == 1) { ik = diag[i][k]; kj = self[k][j]; }
+            else { ik = self[i][k]; kj = diag[k][j]; }
+            self[i][j] = relax(self[i][j], ik + kj);
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < B * B; e += 256) {
+        int i = e / B, j = e % B;
+        d[(r0 + i) * np + c0 + j] = self[i][j];
+    }
+}
+
+// Phase 3: tile (r, c) = min(tile, min_k ik[i][k] + kj[k][j]) for r, c != kb.
+template <int B>
+__global__ __launch_bounds__(256) void fw_phase3(float *__restrict__ d, int np, int kb)
+{
+    constexpr int M = B / 16;  // micro-tile edge per thread (16 x 16 threads)
+    __shared__ float sa[B][B + 1];  // ik panel, [i][k]
+    __shared__ float sb[B][B + 4];  // kj panel, [k][j]
+    const int nbk = np / B;
+    int r = blockIdx.y, c = blockIdx.x;
+    r = r < kb ? r : r + 1;
+    c = c < kb ? c : c + 1;
+    if (r >= nbk || c >= nbk) return;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const long long r0 = (long long)r * B, c0 = (long long)c * B, k0 = (long long)kb * B;
+    for (int e = tid; e < B * B; e += 256) {
+        int i = e / B, j = e % B;
+        sa[i][j] = d[(r0 + i) * np + k0 + j];
+        sb[i][j] = d[(k0 + i) * np + c0 + j];
+    }
+    float acc[M][M];
+#pragma unroll
+    for (int a = 0; a < M; a++)
+#pragma unroll
+        for (int b = 0; b < M; b++) acc[a][b] = d[(r0 + ty * M + a) * np + c0 + tx * M + b];
+    __syncthreads();
+    for (int k = 0; k < B; k += 2) {
+        float x0[M], x1[M], y0[M], y1[M];
+#pragma unroll
+        for (int a = 0; a < M; a++) {
+            x0[a] = sa[ty * M + a][k];
+            x1[a] = sa[ty * M + a][k + 1];
+        }
+#pragma unroll
+        for (int b = 0; b < M; b++) {
+            y0[b] = sb[k][tx * M + b];
+            y1[b] = sb[k + 1][tx * M + b];
+        }
+#pragma unroll
+        for (int a = 0; a < M; a++)
+#pragma unroll
+            for (int b = 0; b < M; b++) acc[a][b] = fminf(acc[a][b], fminf(x0[a] + y0[b], x1[a] + y1[b]));
+    }
+#pragma unroll
+    for (int a = 0; a < M; a++)
+#pragma unroll
+        for (int b = 0; b < M; b++) d[(r0 + ty * M + a) * np + c0 + tx * M + b] = acc[a][b];
+}
+
+__global__ void fw_pad_in(const float *__restrict__ src, int n, float *__restrict__ dst, int np)
+{
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)np * np) return;
+    long long i = k / np, j = k % np;
+    dst[k] = (i < n && j < n) ? src[i * n + j] : (i == j ? 0.0f : INFINITY);
+}
+
+__global__ void fw_pad_out(const float *__restrict__ src, int np, float *__restrict__ dst, int n)
+{
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)n * n) return;
+    long long i = k / n, j = k % n;
+    dst[k] = src[i * np + j];
+}
+
+__global__ void fw_fill_inf(float *__restrict__ d, long long total)
+{
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < total) d[k] = INFINITY;
+}
+
+// W FormBlock.cpp:52-74 in row-major: edge distances, then 0 on the diagonal.
+__global__ void fw_from_csr(const int *__restrict__ row, const int *__restrict__ col,
+                            const float *__restrict__ val, int n, int kind, float *__restrict__ d)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int e = row[i]; e < row[i + 1]; e++) {
+        float w = val ? val[e] : 1.0f;
+        float x = kind == 0 ? 1.0f / w : (kind == 1 ? 1.0f - w : 1.0f);
+        d[(long long)i * n + col[e]] = x;
+    }
+    d[(long long)i * n + i] = 0.0f;
+}
+
+template <int B>
+pg_status fw_run(pg_ctx *ctx, float *dm, int np)
+{
+    hipStream_t st = ctx->stream;
+    const int nbk = np / B;
+    for (int kb = 0; kb < nbk; kb++) {
+        fw_seq_tiles<B><<<1, 256, 0, st>>>(dm, np, kb, 1);
+        if (nbk > 1) {
+            fw_seq_tiles<B><<<2 * (nbk - 1), 256, 0, st>>>(dm, np, kb, 2);
+            dim3 g(nbk - 1, nbk - 1);
+            fw_phase3<B><<<g, 256, 0, st>>>(dm, np, kb);
+        }
+    }
+    PG_KERNEL_CHECK(ctx);
+    return PG_OK;
+}
+
+}  // namespace
+
+extern "C" pg_status pg_fw(pg_ctx *ctx, float *d, int32_t n, int32_t block)
+{
+    if (!ctx) return PG_EINVAL;
+    if (!d || n < 0) return pg_fail(ctx, PG_EINVAL, "bad arguments");
+    if (block == 0) block = 128;
+    if (block != 64 && block != 128) return pg_fail(ctx, PG_EINVAL, "tile edge %d: use 64 or 128", block);
+    if (n == 0) return PG_OK;
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int np = (int)pg_cdiv(n, block) * block;
+    const long long nn = (long long)n * n, npp = (long long)np * np;
+    const bool dev = pg_is_device_ptr(d);
+    float *dm = nullptr;
+    float *din = d;
+    if (!dev) {
+        PG_TRY(pg_bufT(ctx, "fw.in", (size_t)nn, &din));
+        PG_HIP(ctx, hipMemcpyAsync(din, d, nn * 4, hipMemcpyHostToDevice, st));
+    }
+    if (np == n) {
+        dm = din;
+    } else {
+        PG_TRY(pg_bufT(ctx, "fw.pad", (size_t)npp, &dm));
+        fw_pad_in<<<(unsigned)pg_cdiv(npp, 256), 256, 0, st>>>(din, n, dm, np);
+    }
+    if (block == 64) PG_TRY(fw_run<64>(ctx, dm, np));
+    else PG_TRY(fw_run<128>(ctx, dm, np));
+    if (np != n) fw_pad_out<<<(unsigned)pg_cdiv(nn, 256), 256, 0, st>>>(dm, np, din, n);
+    PG_KERNEL_CHECK(ctx);
+    if (!dev) PG_HIP(ctx, hipMemcpyAsync(d, din, nn * 4, hipMemcpyDeviceToHost, st));
+    PG_HIP(ctx, hipStreamSynchronize(st));
+    return PG_OK;
+}
+
+extern "C" pg_status pg_fw_init(pg_ctx *ctx, const int32_t *row, const int32_t *col, const float *val,
+                                int32_t n, int64_t nnz, int32_t kind, float *d)
+{
+    if (!ctx) return PG_EINVAL;
+    if (!d || kind < 0 || kind > 2 || (kind != 2 && !val)) return pg_fail(ctx, PG_EINVAL, "bad arguments");
+    PG_TRY(pg_check_csr_args(ctx, row, col, n, nnz));
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    const int *drow, *dcol;
+    const float *dval = nullptr;
+    PG_TRY(pg_in(ctx, row, (size_t)n + 1, "in.row", &drow));
+    PG_TRY(pg_in(ctx, col, (size_t)nnz, "in.col", &dcol));
+    if (val) PG_TRY(pg_in(ctx, val, (size_t)nnz, "in.val", &dval));
+    const long long nn = (long long)n * n;
+    float *dd;
+    PG_TRY(pg_out(ctx, d, (size_t)nn, "out.fw", &dd));
+    hipStream_t st = ctx->stream;
+    if (nn > 0) {
+        fw_fill_inf<<<(unsigned)pg_cdiv(nn, 256), 256, 0, st>>>(dd, nn);
+        fw_from_csr<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(drow, dcol, dval, n, kind, dd);
+    }
+    PG_KERNEL_CHECK(ctx);
+    PG_TRY(pg_out_finish(ctx, d, dd, (size_t)nn));
+    PG_HIP(ctx, hipStreamSynchronize(st));
+    return PG_OK;
+}

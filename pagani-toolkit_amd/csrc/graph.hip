@@ -1,0 +1,667 @@
+// graph.hip -- per-node metrics on a CSR graph, MI355X (gfx950).
+//
+//  * degree / strength            (U/Degree/Degree/Degree.cpp, W Degree.cpp)
+//  * triangles, binary clustering (U/Cp/Cp/Cp.cpp:21-115) by sorted-adjacency
+//    intersection against an LDS bitmap of the vertex's neighbourhood
+//  * weighted clustering, Barrat and Onnela (W/Cp/Cp/Cp_1.cpp, Cp_2.cpp,
+//    W/Cp_GPU/CUCP/Cp.cu)
+//  * Brandes betweenness and BFS levels (U/BC/BC_CPU/BC_CPU/BC_CPU.cpp,
+//    U/BC/CUBC/BC_GPU_node_pred.cu, U/BFS_MulCPU/BFS_MulCPU/APSP_BFS.cpp):
+//    64 sources per batch, one per lane of a wave64; one wave per vertex pulls
+//    from its neighbours, whose per-source rows (uint64 path counts, FP64
+//    coefficients) are read coalesced.  Visited sets and hop levels mod 4 are
+//    kept as 64-bit per-vertex masks so the neighbour tests are scalar.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pg_internal.hpp"
+
+namespace {
+
+typedef unsigned long long u64;
+
+// ------------------------------------------------------------------ degree
+
+__global__ void degree_kernel(const int *__restrict__ row, int n, int *__restrict__ deg,
+                              float *__restrict__ deg_ref)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int a = row[i], b = row[i + 1];
+    if (deg) deg[i] = b - a;
+    // U Degree.cpp:87: (float) R[k+1] - R[k], both offsets rounded to float
+    if (deg_ref) deg_ref[i] = __fsub_rn((float)b, (float)a);
+}
+
+// W Degree.cpp:84-88: sequential FP32 sum in CSR order, one thread per row.
+__global__ void strength_kernel(const int *__restrict__ row, const float *__restrict__ val, int n,
+                                float *__restrict__ s)
+{
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.0f;
+    for (int e = row[i]; e < row[i + 1]; e++) acc += val[e];
+    s[i] = acc;
+}
+
+// -------------------------------------------------------------- clustering
+
+__device__ __forceinline__ int first_greater(const int *__restrict__ col, int lo, int hi, int key)
+{
+    // smallest e in [lo, hi) with col[e] > key
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (col[mid] <= key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <class T>
+__device__ T block_sum(T v, T *scratch)
+{
+    // fixed-order tree over 256 threads
+    const int tid = threadIdx.x;
+    scratch[tid] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) scratch[tid] += scratch[tid + off];
+        __syncthreads();
+    }
+    T r = scratch[0];
+    __syncthreads();
+    return r;
+}
+
+// One block per vertex u: T_u = #{(v, w) : v < w, v, w in N(u), (v, w) in E}.
+__global__ __launch_bounds__(256) void triangles_kernel(const int *__restrict__ row, const int *__restrict__ col,
+                                                        int n, long long *__restrict__ tri)
+{
+    extern __shared__ uint32_t bits[];
+    __shared__ long long red[256];
+    const int u = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int a = row[u], b = row[u + 1], d = b - a;
+    if (d < 2) {
+        if (tid == 0) tri[u] = 0;
+        return;
+    }
+    const int nw = (n + 31) >> 5;
+    for (int q = tid; q < nw; q += 256) bits[q] = 0u;
+    __syncthreads();
+    for (int e = a + tid; e < b; e += 256) {
+        int c = col[e];
+        atomicOr(&bits[c >> 5], 1u << (c & 31));
+    }
+    __syncthreads();
+    long long cnt = 0;
+    for (int k = wv; k < d; k += 4) {
+        const int v = col[a + k];
+        const int vb = row[v + 1];
+        int start = 0;
+        if (lane == 0) start = first_greater(col, row[v], vb, v);
+        start = __shfl(start, 0);
+        for (int e = start + lane; e < vb; e += 64) {
+            int w = col[e];
+            cnt += (bits[w >> 5] >> (w & 31)) & 1u;
+        }
+    }
+    long long s = block_sum<long long>(cnt, red);
+    if (tid == 0) tri[u] = s;
+}
+
+__global__ void cp_from_tri(const int *__restrict__ row, const long long *__restrict__ tri, int n,
+                            float *__restrict__ cp)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long k = row[i + 1] - row[i];
+    float c = 0.0f;
+    if (k >= 2) c = (float)((double)tri[i] / (double)k / (double)(k - 1) * 2.0);
+    cp[i] = c;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void mean_kernel(const T *__restrict__ x, int n, double *__restrict__ out)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += (double)x[i];
+    s = block_sum<double>(s, red);
+    if (threadIdx.x == 0) *out = n > 0 ? s / (double)n : 0.0;
+}
+
+__global__ __launch_bounds__(256) void max_kernel(const float *__restrict__ x, long long m, float *__restrict__ out)
+{
+    __shared__ float red[256];
+    float v = -INFINITY;
+    for (long long i = threadIdx.x; i < m; i += 256) v = fmaxf(v, x[i]);
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// Weighted clustering of vertex u (one block): bitmap + word prefix of N(u)
+// give the position of w in N(u), hence w_uw.
+__global__ __launch_bounds__(256) void cpw_kernel(const int *__restrict__ row, const int *__restrict__ col,
+                                                  const float *__restrict__ val, int n, int kind,
+                                                  const float *__restrict__ wmaxp, double *__restrict__ cp)
+{
+    extern __shared__ uint32_t sm[];
+    __shared__ double red[256];
+    __shared__ int part[256];
+    const int nw = (n + 31) >> 5;
+    uint32_t *bits = sm;
+    int *pre = (int *)(sm + nw);
+    const int u = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int a = row[u], b = row[u + 1], d = b - a;
+    if (d < 2) {
+        if (tid == 0) cp[u] = 0.0;
+        return;
+    }
+    for (int q = tid; q < nw; q += 256) bits[q] = 0u;
+    __syncthreads();
+    for (int e = a + tid; e < b; e += 256) {
+        int c = col[e];
+        atomicOr(&bits[c >> 5], 1u << (c & 31));
+    }
+    __syncthreads();
+    const int per = (nw + 255) / 256;
+    const int w0 = tid * per, w1 = min(nw, w0 + per);
+    int s = 0;
+    for (int q = w0; q < w1; q++) s += __popc(bits[q]);
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = part[tid] - s;
+    for (int q = w0; q < w1; q++) {
+        pre[q] = run;
+        run += __popc(bits[q]);
+    }
+    __syncthreads();
+    const double wmax = (double)*wmaxp;
+    double acc = 0.0, sk = 0.0;
+    for (int e = a + tid; e < b; e += 256) sk += (double)val[e];
+    for (int k = wv; k < d; k += 4) {
+        const int v = col[a + k];
+        const double wuv = (double)val[a + k];
+        const int vb = row[v + 1];
+        int start = 0;
+        if (lane == 0) start = first_greater(col, row[v], vb, v);
+        start = __shfl(start, 0);
+        for (int e = start + lane; e < vb; e += 64) {
+            const int w = col[e];
+            const uint32_t word = bits[w >> 5];
+            if ((word >> (w & 31)) & 1u) {
+                const double wvw = (double)val[e];
+                if (wvw != 0.0) {
+                    const int pos = pre[w >> 5] + __popc(word & ((1u << (w & 31)) - 1u));
+                    const double wuw = (double)val[a + pos];
+                    if (kind == PG_CP_BARRAT) acc += wuv + wuw;
+                    else acc += pow((wvw / wmax) * (wuv / wmax) * (wuw / wmax), 1.0 / 3.0);
+                }
+            }
+        }
+    }
+    acc = block_sum<double>(acc, red);
+    sk = block_sum<double>(sk, red);
+    if (tid == 0) {
+        double c;
+        if (kind == PG_CP_BARRAT) c = acc / (double)(d - 1) / sk;
+        else c = 2.0 * acc / (double)d / (double)(d - 1);
+        cp[u] = c;
+    }
+}
+
+// ------------------------------------------------------- betweenness / BFS
+
+constexpr int S = 64;  // sources per batch = lanes of a wave64
+constexpr int GATHER = 8;
+
+__global__ void bfs_init(int n, int nsrc, u64 *__restrict__ sigma, int *__restrict__ lev,
+                         u64 *__restrict__ vis, u64 *__restrict__ p0, u64 *__restrict__ p1)
+{
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)n * S) return;
+    sigma[k] = 0;
+    lev[k] = -1;
+    if ((k & (S - 1)) == 0) {
+        long long v = k / S;
+        vis[v] = nsrc >= S ? 0ull : (~0ull << nsrc);  // absent lanes count as visited
+        p0[v] = 0;
+        p1[v] = 0;
+    }
+}
+
+__global__ void bfs_seed(int nsrc, const int *__restrict__ srcs, u64 *__restrict__ sigma,
+                         int *__restrict__ lev, u64 *__restrict__ vis)
+{
+    int s = threadIdx.x;
+    if (s >= nsrc) return;
+    int v = srcs[s];
+    sigma[(long long)v * S + s] = 1;
+    lev[(long long)v * S + s] = 0;
+    atomicOr(&vis[v], 1ull << s);
+}
+
+// Level d -> d+1.  A vertex w not yet reached from source s collects
+// sigma[w][s] = sum of sigma[v][s] over neighbours v reached from s; in an
+// undirected graph those are exactly its level-d neighbours.
+__global__ __launch_bounds__(256) void bfs_forward(const int *__restrict__ row, const int *__restrict__ col, int n,
+                                                   int d, const u64 *__restrict__ visA, u64 *__restrict__ visB,
+                                                   u64 *__restrict__ sigma, int *__restrict__ lev,
+                                                   u64 *__restrict__ p0, u64 *__restrict__ p1,
+                                                   unsigned *__restrict__ progress)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= n) return;
+    const u64 vis = visA[w];
+    const u64 need = ~vis;
+    if (need == 0) {
+        if (lane == 0) visB[w] = vis;
+        return;
+    }
+    const int a = row[w], b = row[w + 1];
+    u64 acc = 0;
+    bool hit = false;
+    for (int base = a; base < b; base += 64) {
+        const int e = base + lane;
+        int v = 0;
+        u64 m = 0;
+        if (e < b) {
+            v = col[e];
+            m = visA[v] & need;
+        }
+        u64 any = __ballot(m != 0);
+        while (any) {
+            u64 g[GATHER];
+            bool gh[GATHER];
+#pragma unroll
+            for (int q = 0; q < GATHER; q++) {
+                g[q] = 0;
+                gh[q] = false;
+                if (any) {
+                    const int sl = __ffsll((long long)any) - 1;
+                    any &= any - 1;
+                    const int vv = __shfl(v, sl);
+                    const u64 mm = __shfl(m, sl);
+                    if ((mm >> lane) & 1ull) {
+                        g[q] = sigma[(long long)vv * S + lane];
+                        gh[q] = true;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < GATHER; q++) {
+                acc += g[q];
+                hit |= gh[q];
+            }
+        }
+    }
+    const bool fresh = ((need >> lane) & 1ull) && hit;
+    const u64 nbits = __ballot(fresh);
+    if (fresh) {
+        sigma[(long long)w * S + lane] = acc;
+        lev[(long long)w * S + lane] = d + 1;
+    }
+    if (lane == 0) {
+        visB[w] = vis | nbits;
+        if (nbits) {
+            if ((d + 1) & 1) p0[w] |= nbits;
+            if ((d + 1) & 2) p1[w] |= nbits;
+            atomicAdd(progress, 1u);
+        }
+    }
+}
+
+__global__ void bc_coef_init(long long total, const u64 *__restrict__ sigma, double *__restrict__ coef)
+{
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    u64 s = sigma[k];
+    coef[k] = s ? 1.0 / (double)s : 0.0;
+}
+
+// Level d: delta[v][s] = sigma[v][s] * sum over children w (level d+1) of
+// (1 + delta[w][s]) / sigma[w][s]; coef[v][s] = (1 + delta[v][s]) / sigma[v][s].
+// BC[v] += sum over sources s != v of delta[v][s] (fixed lane order).
+__global__ __launch_bounds__(256) void bc_backward(const int *__restrict__ row, const int *__restrict__ col, int n,
+                                                   int d, const u64 *__restrict__ vis, const u64 *__restrict__ p0,
+                                                   const u64 *__restrict__ p1, const u64 *__restrict__ sigma,
+                                                   const int *__restrict__ lev, double *__restrict__ coef,
+                                                   const int *__restrict__ srcs, double *__restrict__ bc)
+{
+    const int lane = threadIdx.x & 63;
+    const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (v >= n) return;
+    const bool own = lev[(long long)v * S + lane] == d;
+    const u64 ownm = __ballot(own);
+    if (ownm == 0) return;
+    const u64 x0 = ((d + 1) & 1) ? ~0ull : 0ull;
+    const u64 x1 = ((d + 1) & 2) ? ~0ull : 0ull;
+    const int a = row[v], b = row[v + 1];
+    double acc = 0.0;
+    for (int base = a; base < b; base += 64) {
+        const int e = base + lane;
+        int w = 0;
+        u64 m = 0;
+        if (e < b) {
+            w = col[e];
+            m = ~(p0[w] ^ x0) & ~(p1[w] ^ x1) & vis[w] & ownm;
+        }
+        u64 any = __ballot(m != 0);
+        while (any) {
+            double g[GATHER];
+#pragma unroll
+            for (int q = 0; q < GATHER; q++) {
+                g[q] = 0.0;
+                if (any) {
+                    const int sl = __ffsll((long long)any) - 1;
+                    any &= any - 1;
+                    const int ww = __shfl(w, sl);
+                    const u64 mm = __shfl(m, sl);
+                    if ((mm >> lane) & 1ull) g[q] = coef[(long long)ww * S + lane];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < GATHER; q++) acc += g[q];
+        }
+    }
+    double contrib = 0.0;
+    if (own) {
+        const double sg = (double)sigma[(long long)v * S + lane];
+        const double dl = sg * acc;
+        coef[(long long)v * S + lane] = (1.0 + dl) / sg;
+        if (srcs[lane] != v) contrib = dl;
+    }
+    // fixed-order butterfly over the wave
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off);
+    if (lane == 0) bc[v] += contrib;
+}
+
+__global__ void bfs_export(int n, int nsrc, const int *__restrict__ lev, int *__restrict__ out)
+{
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)n * nsrc) return;
+    long long s = k / n, v = k % n;
+    out[k] = lev[v * S + s];
+}
+
+struct BatchBufs {
+    u64 *sigma, *visA, *visB, *p0, *p1;
+    int *lev, *srcs;
+    double *coef;
+    unsigned *progress;
+};
+
+pg_status batch_bufs(pg_ctx *ctx, int n, BatchBufs &B, bool need_coef)
+{
+    const size_t ns = (size_t)n * S;
+    PG_TRY(pg_bufT(ctx, "bc.sigma", ns, &B.sigma));
+    PG_TRY(pg_bufT(ctx, "bc.lev", ns, &B.lev));
+    PG_TRY(pg_bufT(ctx, "bc.visA", (size_t)n + 1, &B.visA));
+    PG_TRY(pg_bufT(ctx, "bc.visB", (size_t)n + 1, &B.visB));
+    PG_TRY(pg_bufT(ctx, "bc.p0", (size_t)n + 1, &B.p0));
+    PG_TRY(pg_bufT(ctx, "bc.p1", (size_t)n + 1, &B.p1));
+    PG_TRY(pg_bufT(ctx, "bc.srcs", S, &B.srcs));
+    PG_TRY(pg_bufT(ctx, "bc.progress", 4, &B.progress));
+    B.coef = nullptr;
+    if (need_coef) PG_TRY(pg_bufT(ctx, "bc.coef", ns, &B.coef));
+    return PG_OK;
+}
+
+// Forward sweep of one batch; returns the deepest level reached and leaves
+// the final visited masks in *vis_final.
+pg_status run_forward(pg_ctx *ctx, const int *row, const int *col, int n, int nsrc, BatchBufs &B,
+                      int *maxlev, u64 **vis_final)
+{
+    hipStream_t st = ctx->stream;
+    const long long ns = (long long)n * S;
+    bfs_init<<<(unsigned)pg_cdiv(ns, 256), 256, 0, st>>>(n, nsrc, B.sigma, B.lev, B.visA, B.p0, B.p1);
+    bfs_seed<<<1, S, 0, st>>>(nsrc, B.srcs, B.sigma, B.lev, B.visA);
+    PG_KERNEL_CHECK(ctx);
+    u64 *va = B.visA, *vb = B.visB;
+    int d = 0;
+    for (;; d++) {
+        PG_HIP(ctx, hipMemsetAsync(B.progress, 0, 4, st));
+        bfs_forward<<<(unsigned)pg_cdiv(n, 4), 256, 0, st>>>(row, col, n, d, va, vb, B.sigma, B.lev, B.p0,
+                                                             B.p1, B.progress);
+        PG_KERNEL_CHECK(ctx);
+        unsigned prog = 0;
+        PG_HIP(ctx, hipMemcpyAsync(&prog, B.progress, 4, hipMemcpyDeviceToHost, st));
+        PG_HIP(ctx, hipStreamSynchronize(st));
+        std::swap(va, vb);
+        if (prog == 0 || d > n) break;
+    }
+    *maxlev = d;  // levels 0..d hold vertices
+    *vis_final = va;
+    return PG_OK;
+}
+
+}  // namespace
+
+extern "C" pg_status pg_degree(pg_ctx *ctx, const int32_t *row, int32_t n, int32_t *deg, float *deg_ref)
+{
+    if (!ctx) return PG_EINVAL;
+    if (!row || n < 0) return pg_fail(ctx, PG_EINVAL, "bad arguments");
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    const int *drow;
+    int *ddeg;
+    float *dref;
+    PG_TRY(pg_in(ctx, row, (size_t)n + 1, "in.row", &drow));
+    PG_TRY(pg_out(ctx, deg, (size_t)n, "out.deg", &ddeg));
+    PG_TRY(pg_out(ctx, deg_ref, (size_t)n, "out.degref", &dref));
+    if (n > 0) degree_kernel<<<(unsigned)pg_cdiv(n, 256), 256, 0, ctx->stream>>>(drow, n, ddeg, dref);
+    PG_KERNEL_CHECK(ctx);
+    PG_TRY(pg_out_finish(ctx, deg, ddeg, (size_t)n));
+    PG_TRY(pg_out_finish(ctx, deg_ref, dref, (size_t)n));
+    PG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PG_OK;
+}
+
+extern "C" pg_status pg_strength(pg_ctx *ctx, const int32_t *row, const float *val, int32_t n, float *s)
+{
+    if (!ctx) return PG_EINVAL;
+    if (!row || !val || !s || n < 0) return pg_fail(ctx, PG_EINVAL, "bad arguments");
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    int nnz = 0;
+    if (pg_is_device_ptr(row)) PG_HIP(ctx, hipMemcpy(&nnz, row + n, 4, hipMemcpyDeviceToHost));
+    else nnz = row[n];
+    const int *drow;
+    const float *dval;
+    float *ds;
+    PG_TRY(pg_in(ctx, row, (size_t)n + 1, "in.row", &drow));
+    PG_TRY(pg_in(ctx, val, (size_t)nnz, "in.val", &dval));
+    PG_TRY(pg_out(ctx, s, (size_t)n, "out.s", &ds));
+    if (n > 0) strength_kernel<<<(unsigned)pg_cdiv(n, 256), 256, 0, ctx->stream>>>(drow, dval, n, ds);
+    PG_KERNEL_CHECK(ctx);
+    PG_TRY(pg_out_finish(ctx, s, ds, (size_t)n));
+    PG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PG_OK;
+}
+
+namespace {
+pg_status stage_csr(pg_ctx *ctx, const int32_t *row, const int32_t *col, const float *val, int32_t n,
+                    int64_t nnz, const int **drow, const int **dcol, const float **dval)
+{
+    PG_TRY(pg_check_csr_args(ctx, row, col, n, nnz));
+    PG_TRY(pg_in(ctx, row, (size_t)n + 1, "in.row", drow));
+    PG_TRY(pg_in(ctx, col, (size_t)nnz, "in.col", dcol));
+    if (dval) PG_TRY(pg_in(ctx, val, (size_t)nnz, "in.val", dval));
+    return PG_OK;
+}
+}  // namespace
+
+extern "C" pg_status pg_triangles(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
+                                  int64_t nnz, int64_t *tri)
+{
+    if (!ctx) return PG_EINVAL;
+    if (!tri) return pg_fail(ctx, PG_EINVAL, "tri is NULL");
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    const int *drow, *dcol;
+    PG_TRY(stage_csr(ctx, row, col, nullptr, n, nnz, &drow, &dcol, nullptr));
+    const int nw = (n + 31) / 32;
+    if ((size_t)nw * 4 + 4096 > ctx->lds_per_block) return pg_fail(ctx, PG_ENOTSUP, "n too large for LDS bitmap");
+    long long *dtri;
+    PG_TRY(pg_out(ctx, (long long *)tri, (size_t)n, "out.tri", &dtri));
+    if (n > 0) triangles_kernel<<<(unsigned)n, 256, (size_t)nw * 4, ctx->stream>>>(drow, dcol, n, dtri);
+    PG_KERNEL_CHECK(ctx);
+    PG_TRY(pg_out_finish(ctx, (long long *)tri, dtri, (size_t)n));
+    PG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PG_OK;
+}
+
+extern "C" pg_status pg_cp_binary(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
+                                  int64_t nnz, float *cp, double *mean)
+{
+    if (!ctx) return PG_EINVAL;
+    if (!cp) return pg_fail(ctx, PG_EINVAL, "cp is NULL");
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    const int *drow, *dcol;
+    PG_TRY(stage_csr(ctx, row, col, nullptr, n, nnz, &drow, &dcol, nullptr));
+    const int nw = (n + 31) / 32;
+    if ((size_t)nw * 4 + 4096 > ctx->lds_per_block) return pg_fail(ctx, PG_ENOTSUP, "n too large for LDS bitmap");
+    long long *dtri;
+    float *dcp;
+    double *dmean;
+    PG_TRY(pg_bufT(ctx, "cp.tri", (size_t)n + 1, &dtri));
+    PG_TRY(pg_bufT(ctx, "cp.mean", 1, &dmean));
+    PG_TRY(pg_out(ctx, cp, (size_t)n, "out.cp", &dcp));
+    hipStream_t st = ctx->stream;
+    if (n > 0) {
+        triangles_kernel<<<(unsigned)n, 256, (size_t)nw * 4, st>>>(drow, dcol, n, dtri);
+        cp_from_tri<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(drow, dtri, n, dcp);
+    }
+    mean_kernel<float><<<1, 256, 0, st>>>(dcp, n, dmean);
+    PG_KERNEL_CHECK(ctx);
+    PG_TRY(pg_out_finish(ctx, cp, dcp, (size_t)n));
+    if (mean) PG_HIP(ctx, hipMemcpyAsync(mean, dmean, 8, pg_is_device_ptr(mean) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+    PG_HIP(ctx, hipStreamSynchronize(st));
+    return PG_OK;
+}
+
+extern "C" pg_status pg_cp_weighted(pg_ctx *ctx, const int32_t *row, const int32_t *col, const float *val,
+                                    int32_t n, int64_t nnz, int32_t kind, int32_t wmax_mode, double *cp,
+                                    double *mean)
+{
+    if (!ctx) return PG_EINVAL;
+    if (!cp || !val) return pg_fail(ctx, PG_EINVAL, "cp/val is NULL");
+    if (kind != PG_CP_BARRAT && kind != PG_CP_ONNELA) return pg_fail(ctx, PG_EINVAL, "unknown kind %d", kind);
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    const int *drow, *dcol;
+    const float *dval = nullptr;
+    PG_TRY(stage_csr(ctx, row, col, val, n, nnz, &drow, &dcol, &dval));
+    const int nw = (n + 31) / 32;
+    if ((size_t)nw * 8 + 4096 > ctx->lds_per_block) return pg_fail(ctx, PG_ENOTSUP, "n too large for LDS bitmap");
+    double *dcp, *dmean;
+    float *dwmax;
+    PG_TRY(pg_bufT(ctx, "cpw.wmax", 1, &dwmax));
+    PG_TRY(pg_bufT(ctx, "cpw.mean", 1, &dmean));
+    PG_TRY(pg_out(ctx, cp, (size_t)n, "out.cpw", &dcp));
+    hipStream_t st = ctx->stream;
+    long long lim = wmax_mode == 1 ? std::min<long long>(n, nnz) : nnz;
+    max_kernel<<<1, 256, 0, st>>>(dval, lim, dwmax);
+    if (n > 0) cpw_kernel<<<(unsigned)n, 256, (size_t)nw * 8, st>>>(drow, dcol, dval, n, kind, dwmax, dcp);
+    mean_kernel<double><<<1, 256, 0, st>>>(dcp, n, dmean);
+    PG_KERNEL_CHECK(ctx);
+    PG_TRY(pg_out_finish(ctx, cp, dcp, (size_t)n));
+    if (mean) PG_HIP(ctx, hipMemcpyAsync(mean, dmean, 8, pg_is_device_ptr(mean) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+    PG_HIP(ctx, hipStreamSynchronize(st));
+    return PG_OK;
+}
+
+extern "C" pg_status pg_bc(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n, int64_t nnz,
+                           const int32_t *sources, int64_t nsrc, double *bc)
+{
+    if (!ctx) return PG_EINVAL;
+    if (!bc) return pg_fail(ctx, PG_EINVAL, "bc is NULL");
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    const int *drow, *dcol;
+    PG_TRY(stage_csr(ctx, row, col, nullptr, n, nnz, &drow, &dcol, nullptr));
+    std::vector<int32_t> hsrc;
+    if (sources) {
+        if (nsrc < 0) return pg_fail(ctx, PG_EINVAL, "nsrc < 0");
+        hsrc.resize((size_t)nsrc);
+        if (pg_is_device_ptr(sources)) PG_HIP(ctx, hipMemcpy(hsrc.data(), sources, nsrc * 4, hipMemcpyDeviceToHost));
+        else std::memcpy(hsrc.data(), sources, nsrc * 4);
+    } else {
+        nsrc = n;
+        hsrc.resize((size_t)n);
+        for (int i = 0; i < n; i++) hsrc[i] = i;
+    }
+    for (auto s : hsrc)
+        if (s < 0 || s >= n) return pg_fail(ctx, PG_EINVAL, "source %d out of range", s);
+    double *dbc;
+    PG_TRY(pg_out(ctx, bc, (size_t)n, "out.bc", &dbc));
+    hipStream_t st = ctx->stream;
+    if (n > 0) PG_HIP(ctx, hipMemsetAsync(dbc, 0, (size_t)n * 8, st));
+    BatchBufs B;
+    PG_TRY(batch_bufs(ctx, n, B, true));
+    const long long ns = (long long)n * S;
+    for (long long b0 = 0; b0 < nsrc; b0 += S) {
+        const int nb = (int)std::min<long long>(S, nsrc - b0);
+        int pad[S];
+        for (int s = 0; s < S; s++) pad[s] = s < nb ? hsrc[b0 + s] : -1;
+        PG_HIP(ctx, hipMemcpyAsync(B.srcs, pad, sizeof pad, hipMemcpyHostToDevice, st));
+        int maxlev = 0;
+        u64 *vis = nullptr;
+        PG_TRY(run_forward(ctx, drow, dcol, n, nb, B, &maxlev, &vis));
+        bc_coef_init<<<(unsigned)pg_cdiv(ns, 256), 256, 0, st>>>(ns, B.sigma, B.coef);
+        for (int d = maxlev - 1; d >= 1; d--)
+            bc_backward<<<(unsigned)pg_cdiv(n, 4), 256, 0, st>>>(drow, dcol, n, d, vis, B.p0, B.p1, B.sigma,
+                                                                 B.lev, B.coef, B.srcs, dbc);
+        PG_KERNEL_CHECK(ctx);
+    }
+    PG_TRY(pg_out_finish(ctx, bc, dbc, (size_t)n));
+    PG_HIP(ctx, hipStreamSynchronize(st));
+    return PG_OK;
+}
+
+extern "C" pg_status pg_bfs(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n, int64_t nnz,
+                            const int32_t *sources, int64_t nsrc, int32_t *level)
+{
+    if (!ctx) return PG_EINVAL;
+    if (!level || !sources || nsrc < 0) return pg_fail(ctx, PG_EINVAL, "bad arguments");
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    const int *drow, *dcol;
+    PG_TRY(stage_csr(ctx, row, col, nullptr, n, nnz, &drow, &dcol, nullptr));
+    std::vector<int32_t> hsrc((size_t)nsrc);
+    if (pg_is_device_ptr(sources)) PG_HIP(ctx, hipMemcpy(hsrc.data(), sources, nsrc * 4, hipMemcpyDeviceToHost));
+    else std::memcpy(hsrc.data(), sources, nsrc * 4);
+    for (auto s : hsrc)
+        if (s < 0 || s >= n) return pg_fail(ctx, PG_EINVAL, "source %d out of range", s);
+    int *dlev;
+    PG_TRY(pg_out(ctx, level, (size_t)n * nsrc, "out.level", &dlev));
+    BatchBufs B;
+    PG_TRY(batch_bufs(ctx, n, B, false));
+    hipStream_t st = ctx->stream;
+    for (long long b0 = 0; b0 < nsrc; b0 += S) {
+        const int nb = (int)std::min<long long>(S, nsrc - b0);
+        int pad[S];
+        for (int s = 0; s < S; s++) pad[s] = s < nb ? hsrc[b0 + s] : -1;
+        PG_HIP(ctx, hipMemcpyAsync(B.srcs, pad, sizeof pad, hipMemcpyHostToDevice, st));
+        int maxlev = 0;
+        u64 *vis = nullptr;
+        PG_TRY(run_forward(ctx, drow, dcol, n, nb, B, &maxlev, &vis));
+        bfs_export<<<(unsigned)pg_cdiv((long long)n * nb, 256), 256, 0, st>>>(n, nb, B.lev, dlev + b0 * n);
+        PG_KERNEL_CHECK(ctx);
+    }
+    PG_TRY(pg_out_finish(ctx, level, dlev, (size_t)n * nsrc));
+    PG_HIP(ctx, hipStreamSynchronize(st));
+    return PG_OK;
+}

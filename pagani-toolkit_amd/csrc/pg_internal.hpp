@@ -1,0 +1,109 @@
+// pg_internal.hpp -- context, device arena and helpers shared by the
+// libpagani translation units.  gfx950 (MI355X) only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+
+#include "pagani.h"
+
+struct PgBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct pg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 256;
+    size_t lds_per_block = 65536;
+    std::string err;
+    std::map<std::string, PgBuf> bufs;  // named grow-only device buffers
+    hipEvent_t ev[8] = {};
+
+    // result of the last pg_corr_build
+    int32_t cn = 0, ct = 0;
+    int64_t cnnz = -1;
+    bool cweighted = false;
+    int32_t *c_row = nullptr, *c_col = nullptr;
+    float *c_val = nullptr;
+    double *c_z64 = nullptr;
+};
+
+pg_status pg_fail(pg_ctx *ctx, pg_status st, const char *fmt, ...);
+
+#define PG_HIP(ctx, expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return pg_fail((ctx), e_ == hipErrorOutOfMemory ? PG_ENOMEM : PG_EDEVICE,      \
+                           "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__,       \
+                           __LINE__);                                                      \
+    } while (0)
+
+#define PG_TRY(expr)                 \
+    do {                             \
+        pg_status s_ = (expr);       \
+        if (s_ != PG_OK) return s_;  \
+    } while (0)
+
+#define PG_KERNEL_CHECK(ctx) PG_HIP(ctx, hipGetLastError())
+
+// Grow-only named device buffer; contents are not preserved on growth.
+pg_status pg_buf(pg_ctx *ctx, const char *name, size_t bytes, void **out);
+template <class T>
+inline pg_status pg_bufT(pg_ctx *ctx, const char *name, size_t count, T **out)
+{
+    void *p = nullptr;
+    pg_status s = pg_buf(ctx, name, count * sizeof(T) + 16, &p);
+    *out = (T *)p;
+    return s;
+}
+
+bool pg_is_device_ptr(const void *p);
+
+// Device view of an input: device pointers pass through, host pointers are
+// copied into the named buffer.
+template <class T>
+inline pg_status pg_in(pg_ctx *ctx, const T *p, size_t count, const char *name, const T **dev)
+{
+    if (p == nullptr || count == 0 || pg_is_device_ptr(p)) {
+        *dev = p;
+        return PG_OK;
+    }
+    T *d = nullptr;
+    PG_TRY(pg_bufT(ctx, name, count, &d));
+    PG_HIP(ctx, hipMemcpyAsync(d, p, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    *dev = d;
+    return PG_OK;
+}
+
+// Device destination for an output: device pointers pass through, host ones
+// get a staging buffer copied back by pg_out_finish.
+template <class T>
+inline pg_status pg_out(pg_ctx *ctx, T *p, size_t count, const char *name, T **dev)
+{
+    if (p == nullptr || count == 0 || pg_is_device_ptr(p)) {
+        *dev = p;
+        return PG_OK;
+    }
+    return pg_bufT(ctx, name, count, dev);
+}
+
+template <class T>
+inline pg_status pg_out_finish(pg_ctx *ctx, T *host, const T *dev, size_t count)
+{
+    if (host == nullptr || count == 0 || host == dev) return PG_OK;
+    PG_HIP(ctx, hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    return PG_OK;
+}
+
+pg_status pg_check_csr_args(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
+                            int64_t nnz);
+
+inline int64_t pg_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }

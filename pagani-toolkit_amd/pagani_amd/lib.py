@@ -1,0 +1,260 @@
+"""ctypes binding of libpagani.so.
+
+Mirrors the entry points of include/pagani.h one to one; each method names
+the reference interface it stands in for (see the header).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libpagani.so")
+
+PREC_FP32, PREC_BF16X3 = 0, 1
+THR_R, THR_SPARSITY = 0, 1
+LAYOUT_TIME_MAJOR, LAYOUT_VOXEL_MAJOR = 0, 1
+CP_BARRAT, CP_ONNELA = 1, 2
+MOD_STRENGTH, MOD_BINARY_K = 0, 1
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "out of device memory", 3: "device error",
+          4: "malformed input", 5: "threshold error", 6: "not supported", 7: "bad call order"}
+
+
+class PgError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"pagani status {status} ({STATUS.get(status, '?')}): {msg}")
+        self.status = status
+
+
+class _Opts(C.Structure):
+    _fields_ = [("thr_kind", C.c_int32), ("thr", C.c_float), ("weighted", C.c_int32),
+                ("prec", C.c_int32), ("layout", C.c_int32)]
+
+
+class _Info(C.Structure):
+    _fields_ = [("n", C.c_int32), ("t", C.c_int32), ("nnz", C.c_int64),
+                ("r_thresh", C.c_float), ("s_thresh", C.c_float), ("t_star", C.c_double),
+                ("band", C.c_double), ("band_pairs", C.c_int64), ("band_flips", C.c_int64),
+                ("gram_passes", C.c_int64), ("ms_zscore", C.c_double), ("ms_gram", C.c_double),
+                ("ms_select", C.c_double), ("ms_csr", C.c_double)]
+
+
+@dataclass
+class CorrInfo:
+    n: int
+    t: int
+    nnz: int
+    r_thresh: float
+    s_thresh: float
+    t_star: float
+    band: float
+    band_pairs: int
+    band_flips: int
+    gram_passes: int
+    ms_zscore: float
+    ms_gram: float
+    ms_select: float
+    ms_csr: float
+
+
+_LIB = None
+_VP = C.c_void_p
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "pg_create": (C.c_int, [C.POINTER(_VP), C.c_int]),
+    "pg_destroy": (None, [_VP]),
+    "pg_last_error": (C.c_char_p, [_VP]),
+    "pg_status_string": (C.c_char_p, [C.c_int]),
+    "pg_abi_version": (C.c_int, []),
+    "pg_synchronize": (C.c_int, [_VP]),
+    "pg_corr_build": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.POINTER(_Opts), C.POINTER(_Info)]),
+    "pg_corr_csr": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "pg_corr_csr_view": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP)]),
+    "pg_corr_z64_view": (C.c_int, [_VP, C.POINTER(_VP)]),
+    "pg_degree": (C.c_int, [_VP, _VP, C.c_int32, _VP, _VP]),
+    "pg_strength": (C.c_int, [_VP, _VP, _VP, C.c_int32, _VP]),
+    "pg_triangles": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP]),
+    "pg_cp_binary": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP]),
+    "pg_cp_weighted": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_int32, C.c_int32, _VP, _VP]),
+    "pg_bc": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, C.c_int64, _VP]),
+    "pg_bfs": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, C.c_int64, _VP]),
+    "pg_fw": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32]),
+    "pg_fw_init": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_int32, _VP]),
+    "pg_modularity": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_int32, _VP, _VP, _VP, _VP, _VP]),
+}
+
+
+def library():
+    """Load libpagani.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built; run python pagani-toolkit_amd/build.py")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def _ptr(x):
+    """Host numpy array or torch tensor -> void*."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        if not x.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays must be C-contiguous")
+        return x.ctypes.data_as(_VP)
+    if hasattr(x, "data_ptr"):
+        if not x.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+        return _VP(x.data_ptr())
+    raise TypeError(f"unsupported array type {type(x)!r}")
+
+
+def _len(x):
+    return int(x.numel()) if hasattr(x, "numel") else int(x.size)
+
+
+class Context:
+    """One pg_ctx: a HIP stream plus a reusable device arena."""
+
+    def __init__(self, device: int = 0):
+        self._lib = library()
+        h = _VP()
+        st = self._lib.pg_create(C.byref(h), device)
+        if st != 0:
+            raise PgError(st, "pg_create failed (no usable HIP device?)")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st):
+        if st != 0:
+            msg = self._lib.pg_last_error(self._h)
+            raise PgError(st, msg.decode() if msg else "")
+
+    def synchronize(self):
+        self._check(self._lib.pg_synchronize(self._h))
+
+    # ------------------------------------------------------- construction
+    def corr_build(self, bold, n: int, t: int, thr: float, *, sparsity: bool = False,
+                   weighted: bool = False, prec: int = PREC_BF16X3,
+                   layout: int = LAYOUT_TIME_MAJOR) -> CorrInfo:
+        """CorMat_gpu + post_block + FormFullGraph(_s) + FormCSRGraph."""
+        o = _Opts(THR_SPARSITY if sparsity else THR_R, thr, int(weighted), prec, layout)
+        info = _Info()
+        self._check(self._lib.pg_corr_build(self._h, _ptr(bold), n, t, C.byref(o), C.byref(info)))
+        return CorrInfo(**{f: getattr(info, f) for f, _ in _Info._fields_})
+
+    def corr_csr(self, n: int, nnz: int, weighted: bool = False):
+        row = np.empty(n + 1, np.int32)
+        col = np.empty(max(nnz, 1), np.int32)
+        val = np.empty(max(nnz, 1), np.float32) if weighted else None
+        self._check(self._lib.pg_corr_csr(self._h, _ptr(row), _ptr(col), _ptr(val)))
+        return row, col[:nnz], (val[:nnz] if weighted else None)
+
+    def corr_csr_view(self):
+        r, c, v = _VP(), _VP(), _VP()
+        self._check(self._lib.pg_corr_csr_view(self._h, C.byref(r), C.byref(c), C.byref(v)))
+        return r.value, c.value, v.value
+
+    def corr_z64_view(self):
+        z = _VP()
+        self._check(self._lib.pg_corr_z64_view(self._h, C.byref(z)))
+        return z.value
+
+    # ------------------------------------------------------------ metrics
+    def degree(self, row, deg=None, deg_ref=None):
+        n = _len(row) - 1
+        if deg is None and deg_ref is None:
+            deg, deg_ref = np.empty(n, np.int32), np.empty(n, np.float32)
+        self._check(self._lib.pg_degree(self._h, _ptr(row), n, _ptr(deg), _ptr(deg_ref)))
+        return deg, deg_ref
+
+    def strength(self, row, val, out=None):
+        n = _len(row) - 1
+        out = np.empty(n, np.float32) if out is None else out
+        self._check(self._lib.pg_strength(self._h, _ptr(row), _ptr(val), n, _ptr(out)))
+        return out
+
+    def triangles(self, row, col, out=None):
+        n = _len(row) - 1
+        out = np.empty(n, np.int64) if out is None else out
+        self._check(self._lib.pg_triangles(self._h, _ptr(row), _ptr(col), n, _len(col), _ptr(out)))
+        return out
+
+    def cp_binary(self, row, col, out=None):
+        n = _len(row) - 1
+        out = np.empty(n, np.float32) if out is None else out
+        mean = C.c_double(0.0)
+        self._check(self._lib.pg_cp_binary(self._h, _ptr(row), _ptr(col), n, _len(col), _ptr(out),
+                                           C.cast(C.pointer(mean), _VP)))
+        return out, mean.value
+
+    def cp_weighted(self, row, col, val, kind: int = CP_ONNELA, wmax_mode: int = 0, out=None):
+        n = _len(row) - 1
+        out = np.empty(n, np.float64) if out is None else out
+        mean = C.c_double(0.0)
+        self._check(self._lib.pg_cp_weighted(self._h, _ptr(row), _ptr(col), _ptr(val), n, _len(col),
+                                             kind, wmax_mode, _ptr(out), C.cast(C.pointer(mean), _VP)))
+        return out, mean.value
+
+    def bc(self, row, col, sources=None, out=None):
+        n = _len(row) - 1
+        out = np.empty(n, np.float64) if out is None else out
+        ns = 0 if sources is None else _len(sources)
+        self._check(self._lib.pg_bc(self._h, _ptr(row), _ptr(col), n, _len(col), _ptr(sources), ns, _ptr(out)))
+        return out
+
+    def bfs(self, row, col, sources, out=None):
+        n = _len(row) - 1
+        ns = _len(sources)
+        out = np.empty((ns, n), np.int32) if out is None else out
+        self._check(self._lib.pg_bfs(self._h, _ptr(row), _ptr(col), n, _len(col), _ptr(sources), ns, _ptr(out)))
+        return out
+
+    def fw(self, d, block: int = 0):
+        """Closure in place; d is an n x n float32 array/tensor."""
+        n = int(d.shape[0])
+        self._check(self._lib.pg_fw(self._h, _ptr(d), n, block))
+        return d
+
+    def fw_init(self, row, col, val, kind: int, out=None):
+        n = _len(row) - 1
+        out = np.empty((n, n), np.float32) if out is None else out
+        self._check(self._lib.pg_fw_init(self._h, _ptr(row), _ptr(col), _ptr(val), n, _len(col), kind, _ptr(out)))
+        return out
+
+    def modularity(self, row, col, val=None, kmode: int = MOD_STRENGTH):
+        n = _len(row) - 1
+        labels = np.empty(n, np.int32)
+        eig = np.empty(n, np.float64)
+        q, lam, nr = C.c_double(0), C.c_double(0), C.c_int32(0)
+        self._check(self._lib.pg_modularity(self._h, _ptr(row), _ptr(col), _ptr(val), n, _len(col), kmode,
+                                            _ptr(labels), C.cast(C.pointer(q), _VP),
+                                            C.cast(C.pointer(lam), _VP), _ptr(eig),
+                                            C.cast(C.pointer(nr), _VP)))
+        return labels, q.value, lam.value, eig, nr.value

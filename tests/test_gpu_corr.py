@@ -1,0 +1,163 @@
+"""GPU parity: voxel series -> CSR (libpagani) against the FP64 oracle.
+
+Edge sets, row offsets and (weighted) values must be bit-identical to the
+oracle for both the FP32 and the BF16x3 Gram.
+"""
+import numpy as np
+import pytest
+
+from conftest import hip_d2h, say
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PRECS = [0, 1]  # FP32, BF16X3
+
+
+def _data(n, t, k, seed):
+    from pagani_amd import synth
+    return synth.bold(seed, n, t, k)
+
+
+def _build(ctx, x, thr, sparsity=False, weighted=False, prec=1, layout=0):
+    t, n = x.shape
+    if layout == 1:
+        x = np.ascontiguousarray(x.T)
+    info = ctx.corr_build(x, n, t, thr, sparsity=sparsity, weighted=weighted, prec=prec, layout=layout)
+    row, col, val = ctx.corr_csr(n, info.nnz, weighted)
+    return info, row, col, val
+
+
+def _assert_same(row, col, val, orow, ocol, oval):
+    np.testing.assert_array_equal(row, orow)
+    np.testing.assert_array_equal(col, ocol)
+    if oval is not None:
+        np.testing.assert_array_equal(val.view(np.uint32), oval.view(np.uint32))
+
+
+def test_zscore_matches_oracle_bitwise(ctx):
+    x = _data(300, 173, 7, 11)
+    say("zscore n=300 t=173")
+    info, *_ = _build(ctx, x, 0.5)
+    z = np.frombuffer(hip_d2h(ctx.corr_z64_view(), 300 * 173 * 8), np.float64).reshape(300, 173)
+    z64, _ = O.zscore(x)
+    np.testing.assert_array_equal(z.view(np.uint64), z64.view(np.uint64))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("thr", [0.4, 0.15])
+def test_rmode_bit_exact(ctx, prec, weighted, thr):
+    x = _data(700, 200, 10, 3)
+    z64, _ = O.zscore(x)
+    say(f"rmode prec={prec} weighted={weighted} thr={thr}")
+    info, row, col, val = _build(ctx, x, thr, weighted=weighted, prec=prec)
+    orow, ocol, oval = O.corr_csr_r(z64, thr, weighted)
+    _assert_same(row, col, val, orow, ocol, oval)
+    assert info.nnz == orow[-1]
+    m1 = 700 * 699 // 2
+    assert info.s_thresh == np.float32(100.0 * info.nnz / m1 / 2)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("s", [1.0, 7.5])
+def test_smode_bit_exact(ctx, prec, weighted, s):
+    x = _data(640, 150, 8, 5)
+    z64, _ = O.zscore(x)
+    say(f"smode prec={prec} weighted={weighted} s={s}")
+    info, row, col, val = _build(ctx, x, s, sparsity=True, weighted=weighted, prec=prec)
+    orow, ocol, oval, tstar = O.corr_csr_s(z64, s, weighted)
+    _assert_same(row, col, val, orow, ocol, oval)
+    assert info.t_star == tstar
+    assert info.nnz == 2 * O.sparsity_k(640, s)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_smode_ties_row_major(ctx, prec):
+    # duplicated voxels give bit-identical r against every other voxel: the
+    # k-th largest value is tied many times and the earliest ties must be kept
+    x = _data(400, 120, 6, 9)
+    for dst, src in [(50, 10), (51, 10), (300, 10), (201, 77), (399, 77)]:
+        x[:, dst] = x[:, src]
+    z64, _ = O.zscore(x)
+    n = 400
+    s = _sparsity_splitting_a_tie(z64, n)
+    orow, ocol, _, tstar = O.corr_csr_s(z64, s)
+    say(f"ties prec={prec} s={s}")
+    info, row, col, _ = _build(ctx, x, s, sparsity=True, prec=prec)
+    np.testing.assert_array_equal(row, orow)
+    np.testing.assert_array_equal(col, ocol)
+    assert info.t_star == tstar
+
+
+def _sparsity_splitting_a_tie(z64, n):
+    """A sparsity percent whose k-th largest r is tied and the cut splits the tie."""
+    iu = np.triu_indices(n, 1)
+    r = np.clip((z64 @ z64.T)[iu], -1, 1)  # ordering only; exact r via the oracle below
+    order = np.argsort(-r, kind="stable")
+    exact = np.array([O.r_pair(z64, int(iu[0][q]), int(iu[1][q])) for q in order[: r.size // 4]])
+    for s in np.linspace(0.3, 24.0, 800, dtype=np.float32):
+        k = O.sparsity_k(n, float(s))
+        if k < 2 or k > exact.size:
+            continue
+        t = exact[k - 1]
+        g, e = int((exact > t).sum()), int((exact == t).sum())
+        if e > 1 and g < k < g + e:
+            return float(s)
+    raise AssertionError("no sparsity splits a tie group")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_voxel_major_layout_and_zero_variance(ctx, prec):
+    x = _data(333, 97, 5, 21)
+    x[:, 17] = 3.25  # constant series: r = 0 with everything
+    z64, _ = O.zscore(x)
+    say(f"layout1 prec={prec}")
+    info, row, col, _ = _build(ctx, x, 0.3, prec=prec, layout=1)
+    orow, ocol, _ = O.corr_csr_r(z64, 0.3)
+    np.testing.assert_array_equal(row, orow)
+    np.testing.assert_array_equal(col, ocol)
+    assert row[18] == row[17]
+    info, row, col, _ = _build(ctx, x, -0.2, prec=prec)  # negative threshold: r=0 > -0.2
+    orow, ocol, _ = O.corr_csr_r(z64, -0.2)
+    np.testing.assert_array_equal(row, orow)
+    np.testing.assert_array_equal(col, ocol)
+
+
+def test_tiny_inputs(ctx):
+    for n in (1, 2, 3, 129):
+        x = _data(n, 40, 2, 4)
+        z64, _ = O.zscore(x)
+        say(f"tiny n={n}")
+        info, row, col, _ = _build(ctx, x, 0.0)
+        orow, ocol, _ = O.corr_csr_r(z64, 0.0)
+        np.testing.assert_array_equal(row, orow)
+        np.testing.assert_array_equal(col, ocol)
+
+
+def test_bad_arguments(ctx):
+    import pagani_amd
+    x = _data(10, 20, 2, 1)
+    with pytest.raises(pagani_amd.PgError):
+        ctx.corr_build(x, 10, 20, 1.5)  # |r threshold| > 1
+    with pytest.raises(pagani_amd.PgError):
+        ctx.corr_build(x, 10, 20, 0.0, sparsity=True)  # sparsity must be > 0
+    with pytest.raises(pagani_amd.PgError):
+        ctx.corr_build(x, 10, 1, 0.2)  # T < 2
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_plain_workload_n5000(ctx, prec):
+    """The CPU-runnable plain workload: N=5000, T=200, K=10, r > 0.4."""
+    from pagani_amd import synth
+    w = synth.WORKLOADS["plain_n5000_t200"]
+    x = synth.bold(w["seed"], w["n"], w["t"], w["k"])
+    z64, _ = O.zscore(x)
+    say(f"n5000 prec={prec}")
+    info, row, col, _ = _build(ctx, x, 0.4, prec=prec)
+    orow, ocol, _ = O.corr_csr_r(z64, 0.4)
+    np.testing.assert_array_equal(row, orow)
+    np.testing.assert_array_equal(col, ocol)
+    say(f"  nnz={info.nnz} band_pairs={info.band_pairs} flips={info.band_flips} "
+        f"gram_ms={info.ms_gram:.3f} csr_ms={info.ms_csr:.3f}")

@@ -1,0 +1,202 @@
+"""GPU parity: per-node metrics (libpagani) against the FP64 oracle.
+
+Bit-exact: degree (int and the reference's float), strength, triangles, binary
+Cp, BFS hop levels, Floyd-Warshall.  Tolerance (relative, stated per test):
+betweenness, weighted clustering, modularity.
+"""
+import numpy as np
+import pytest
+
+from conftest import say
+from helpers import cliques, complete_graph, path_graph, random_graph, star_graph
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BC_RTOL = 1e-9      # FP64 dependencies, different summation order
+CPW_RTOL = 1e-9     # FP64 sums in a different order, pow on device vs host
+Q_ATOL = 1e-9
+
+
+@pytest.fixture(scope="module")
+def plain_graph(ctx):
+    """CSR of the N=5000, T=200, K=10, r > 0.4 workload (built on the GPU)."""
+    from pagani_amd import synth
+    w = synth.WORKLOADS["plain_n5000_t200"]
+    x = synth.bold(w["seed"], w["n"], w["t"], w["k"])
+    info = ctx.corr_build(x, w["n"], w["t"], 0.4)
+    row, col, _ = ctx.corr_csr(w["n"], info.nnz)
+    return row, col
+
+
+@pytest.fixture(scope="module")
+def weighted_graph(ctx):
+    from pagani_amd import synth
+    x = synth.bold(7, 1500, 180, 12)
+    info = ctx.corr_build(x, 1500, 180, 0.3, weighted=True)
+    return ctx.corr_csr(1500, info.nnz, True)
+
+
+def _graphs():
+    return {
+        "path": path_graph(40),
+        "star": star_graph(33),
+        "complete": complete_graph(20),
+        "random": random_graph(300, 0.05, 1)[:2],
+        "sparse_disconnected": random_graph(500, 0.004, 2)[:2],
+        "cliques": cliques([12, 9, 15, 7]),
+    }
+
+
+def test_degree_and_reference_float(ctx, plain_graph):
+    row, col = plain_graph
+    say("degree")
+    deg, dref = ctx.degree(row)
+    od, odr = O.degree(row)
+    np.testing.assert_array_equal(deg, od)
+    np.testing.assert_array_equal(dref.view(np.uint32), odr.view(np.uint32))
+    # offsets past 2^24: the reference's float subtraction rounds
+    big = np.array([0, 16777217, 33554439, 33554440], np.int32)
+    d2, r2 = ctx.degree(big)
+    o2, or2 = O.degree(big)
+    np.testing.assert_array_equal(d2, o2)
+    np.testing.assert_array_equal(r2.view(np.uint32), or2.view(np.uint32))
+
+
+def test_strength_bit_exact(ctx, weighted_graph):
+    row, col, val = weighted_graph
+    say("strength")
+    s = ctx.strength(row, val)
+    np.testing.assert_array_equal(s.view(np.uint32), O.strength(row, val).view(np.uint32))
+
+
+@pytest.mark.parametrize("name", list(_graphs()))
+def test_triangles_cp_small(ctx, name):
+    row, col = _graphs()[name][:2]
+    say(f"triangles {name}")
+    np.testing.assert_array_equal(ctx.triangles(row, col), O.triangles(row, col))
+    cp, mean = ctx.cp_binary(row, col)
+    ocp, omean = O.cp_binary(row, col)
+    np.testing.assert_array_equal(cp.view(np.uint32), ocp.view(np.uint32))
+    assert abs(mean - omean) <= 1e-12 * max(1.0, abs(omean))
+
+
+def test_triangles_cp_plain(ctx, plain_graph):
+    row, col = plain_graph
+    say("triangles plain")
+    np.testing.assert_array_equal(ctx.triangles(row, col), O.triangles(row, col))
+    cp, mean = ctx.cp_binary(row, col)
+    ocp, omean = O.cp_binary(row, col)
+    np.testing.assert_array_equal(cp.view(np.uint32), ocp.view(np.uint32))
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("wmax_mode", [0, 1])
+def test_cp_weighted(ctx, weighted_graph, kind, wmax_mode):
+    row, col, val = weighted_graph
+    say(f"cpw kind={kind} wmax={wmax_mode}")
+    cp, mean = ctx.cp_weighted(row, col, val, kind, wmax_mode)
+    ocp, omean = O.cp_weighted(row, col, val, kind, wmax_mode)
+    np.testing.assert_allclose(cp, ocp, rtol=CPW_RTOL, atol=1e-14)
+    assert abs(mean - omean) <= CPW_RTOL * abs(omean)
+
+
+@pytest.mark.parametrize("name", list(_graphs()))
+def test_bc_small_all_sources(ctx, name):
+    row, col = _graphs()[name][:2]
+    say(f"bc {name}")
+    b = ctx.bc(row, col)
+    np.testing.assert_allclose(b, O.bc(row, col), rtol=BC_RTOL, atol=1e-9)
+
+
+def test_bc_closed_forms(ctx):
+    n = 41
+    row, col = path_graph(n)[:2]
+    b = ctx.bc(row, col)
+    i = np.arange(n)
+    np.testing.assert_allclose(b, 2.0 * i * (n - 1 - i), rtol=1e-12)
+    row, col = star_graph(n)[:2]
+    b = ctx.bc(row, col)
+    assert b[0] == (n - 1) * (n - 2)
+    assert np.all(b[1:] == 0)
+
+
+def test_bc_plain_sampled(ctx, plain_graph):
+    from pagani_amd import synth
+    row, col = plain_graph
+    src = synth.sample_sources(row.size - 1, 200)
+    say("bc plain 200 sources")
+    b = ctx.bc(row, col, src)
+    np.testing.assert_allclose(b, O.bc(row, col, src), rtol=BC_RTOL, atol=1e-9)
+
+
+def test_bfs_levels(ctx, plain_graph):
+    row, col = plain_graph
+    src = np.array([0, 5, 17, 4999, 2500] + list(range(100, 170)), np.int32)
+    say("bfs plain")
+    lev = ctx.bfs(row, col, src)
+    for k, s in enumerate(src):
+        np.testing.assert_array_equal(lev[k], O.bfs(row, col, int(s)))
+    row, col = _graphs()["sparse_disconnected"]
+    lev = ctx.bfs(row, col, np.array([0, 1, 2], np.int32))
+    for k in range(3):
+        np.testing.assert_array_equal(lev[k], O.bfs(row, col, k))
+
+
+@pytest.mark.parametrize("block", [64, 128])
+@pytest.mark.parametrize("n", [100, 300])
+def test_fw_bit_exact(ctx, weighted_graph, block, n):
+    row, col, val = weighted_graph
+    # induced subgraph on the first n voxels, d = 1/w as the reference
+    sub_r = [0]
+    sub_c, sub_v = [], []
+    for i in range(n):
+        a, b = row[i], row[i + 1]
+        m = col[a:b] < n
+        sub_c += col[a:b][m].tolist()
+        sub_v += val[a:b][m].tolist()
+        sub_r.append(len(sub_c))
+    r = np.array(sub_r, np.int32)
+    c = np.array(sub_c, np.int32)
+    v = np.array(sub_v, np.float32)
+    say(f"fw n={n} block={block}")
+    d = ctx.fw_init(r, c, v, 0)
+    np.testing.assert_array_equal(d.view(np.uint32), O.fw_init(r, c, v, 0).view(np.uint32))
+    got = ctx.fw(d.copy(), block)
+    want = O.fw_blocked(d, block)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_fw_unit_weights_equal_bfs(ctx):
+    row, col = random_graph(200, 0.03, 5)[:2]
+    d = ctx.fw_init(row, col, None, 2)
+    ctx.fw(d, 64)
+    for s in range(0, 200, 37):
+        lev = O.bfs(row, col, s).astype(np.float32)
+        lev[lev < 0] = np.inf
+        np.testing.assert_array_equal(d[s], lev)
+
+
+def test_modularity_cliques(ctx):
+    row, col = cliques([20, 20, 20, 20])[:2]
+    say("modularity cliques")
+    lab, q, lam, eig, nr = ctx.modularity(row, col)
+    olab, oq, olam, oeig, onr = O.modularity(row, col)
+    np.testing.assert_array_equal(lab, olab)
+    assert abs(q - oq) <= Q_ATOL
+    # four cliques come out as four modules
+    assert len(set(lab[i * 20] for i in range(4))) == 4
+    for i in range(4):
+        assert len(set(lab[i * 20:(i + 1) * 20])) == 1
+
+
+@pytest.mark.parametrize("kmode", [0, 1])
+def test_modularity_weighted(ctx, weighted_graph, kmode):
+    row, col, val = weighted_graph
+    say(f"modularity weighted kmode={kmode}")
+    lab, q, lam, eig, nr = ctx.modularity(row, col, val, kmode)
+    olab, oq, olam, oeig, onr = O.modularity(row, col, val, kmode)
+    assert abs(lam - olam) <= 1e-7 * abs(olam)
+    assert abs(q - oq) <= 1e-6
+    agree = (lab == olab).mean()
+    assert agree > 0.99, agree

@@ -83,6 +83,8 @@ typedef struct pg_corr_info {
     int64_t band_flips;   /* rechecked pairs whose fast-path decision was wrong */
     int64_t gram_passes;  /* Gram sweeps run (1 r-mode, 2 sparsity, +1 on overflow) */
     double ms_zscore, ms_gram, ms_select, ms_csr;  /* device time per stage */
+    double ms_gram_hist;  /* histogram Gram pass kernel alone (sparsity mode) */
+    double ms_gram_emit;  /* emitting Gram pass kernel alone (last launch) */
 } pg_corr_info;
 
 /* Voxel time series -> thresholded CSR adjacency, on the device.

@@ -109,6 +109,17 @@ int64_t orc_corr_count_r(const double *z64, int64_t n, int t, float thr_eff)
     return 2 * cnt;
 }
 
+int64_t orc_corr_count_rows(const double *z64, int64_t n, int t, float thr_eff, int64_t i0, int64_t i1)
+{
+    int64_t cnt = 0;
+    const double th = (double)thr_eff;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : cnt)
+    for (int64_t i = i0; i < i1; i++)
+        for (int64_t j = i + 1; j < n; j++)
+            if (orc_r(z64, t, i, j) > th) cnt++;
+    return cnt;
+}
+
 /* FormFullGraph + FormCSRGraph (U CorMat.cpp:633-653, 839-857; weighted V from
  * W CorMat.cpp:853-885): columns ascending, both halves, no self loops. */
 void orc_corr_csr_r(const double *z64, int64_t n, int t, float thr_eff,

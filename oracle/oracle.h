@@ -34,6 +34,10 @@ int64_t orc_sparsity_k(int64_t n, float s_percent);
 /* r-mode: count of undirected edges with r > thr_eff. */
 int64_t orc_corr_count_r(const double *z64, int64_t n, int t, float thr_eff);
 
+/* r-mode: undirected edge count over rows [i0, i1) only (pairs i < j); a
+ * bounded sample of the all-pairs work for CPU timing. */
+int64_t orc_corr_count_rows(const double *z64, int64_t n, int t, float thr_eff, int64_t i0, int64_t i1);
+
 /* r-mode: CSR.  row[n+1]; col/val sized by orc_corr_count_r()*2. val may be NULL. */
 void orc_corr_csr_r(const double *z64, int64_t n, int t, float thr_eff,
                     int32_t *row, int32_t *col, float *val);

@@ -41,6 +41,8 @@ def lib():
         L.orc_sparsity_k.restype = C.c_int64
         L.orc_corr_count_r.argtypes = [f64p, C.c_int64, C.c_int, C.c_float]
         L.orc_corr_count_r.restype = C.c_int64
+        L.orc_corr_count_rows.argtypes = [f64p, C.c_int64, C.c_int, C.c_float, C.c_int64, C.c_int64]
+        L.orc_corr_count_rows.restype = C.c_int64
         L.orc_corr_csr_r.argtypes = [f64p, C.c_int64, C.c_int, C.c_float, i32p, i32p, C.c_void_p]
         L.orc_corr_csr_s.argtypes = [f64p, C.c_int64, C.c_int, C.c_int64, i32p, i32p, C.c_void_p]
         L.orc_corr_csr_s.restype = C.c_double
@@ -104,6 +106,12 @@ def threshold_eff(thr: float, weighted: bool) -> float:
 
 def sparsity_k(n: int, s_percent: float) -> int:
     return int(lib().orc_sparsity_k(n, s_percent))
+
+
+def corr_count_rows(z64: np.ndarray, thr: float, i0: int, i1: int) -> int:
+    z64 = np.ascontiguousarray(z64)
+    n, t = z64.shape
+    return int(lib().orc_corr_count_rows(z64, n, t, thr, i0, i1))
 
 
 def corr_csr_r(z64: np.ndarray, thr: float, weighted: bool = False):

@@ -688,6 +688,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     ga.per = per;
     const size_t stage = prec == PREC_BF16X3 ? 4 * TILE * LDB * 2 : 2 * TILE * LDF * 4;
 
+    bool ran_hist = false, ran_emit = false;
     auto launch_gram = [&](int mode) -> pg_status {
         size_t shm = stage + (mode == MODE_HIST ? NBIN * 4 : 0);
         const void *fn;
@@ -698,6 +699,8 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             fn = mode == MODE_HIST ? (const void *)gram_kernel<PREC_F32, MODE_HIST>
                                    : (const void *)gram_kernel<PREC_F32, MODE_EMIT>;
         PG_HIP(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        const int e0 = mode == MODE_HIST ? 6 : 8;
+        PG_HIP(ctx, hipEventRecord(ctx->ev[e0], st));
         if (prec == PREC_BF16X3) {
             if (mode == MODE_HIST) gram_kernel<PREC_BF16X3, MODE_HIST><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
             else gram_kernel<PREC_BF16X3, MODE_EMIT><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
@@ -706,6 +709,9 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             else gram_kernel<PREC_F32, MODE_EMIT><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
         }
         PG_KERNEL_CHECK(ctx);
+        PG_HIP(ctx, hipEventRecord(ctx->ev[e0 + 1], st));
+        if (mode == MODE_HIST) ran_hist = true;
+        else ran_emit = true;
         inf.gram_passes++;
         return PG_OK;
     };
@@ -905,6 +911,8 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     inf.ms_select = ms;
     (void)hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
     inf.ms_csr = ms;
+    if (ran_hist && hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess) inf.ms_gram_hist = ms;
+    if (ran_emit && hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]) == hipSuccess) inf.ms_gram_emit = ms;
 
     ctx->cn = n;
     ctx->ct = t;

@@ -24,7 +24,7 @@ struct pg_ctx {
     size_t lds_per_block = 65536;
     std::string err;
     std::map<std::string, PgBuf> bufs;  // named grow-only device buffers
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[12] = {};
 
     // result of the last pg_corr_build
     int32_t cn = 0, ct = 0;

@@ -40,7 +40,8 @@ class _Info(C.Structure):
                 ("r_thresh", C.c_float), ("s_thresh", C.c_float), ("t_star", C.c_double),
                 ("band", C.c_double), ("band_pairs", C.c_int64), ("band_flips", C.c_int64),
                 ("gram_passes", C.c_int64), ("ms_zscore", C.c_double), ("ms_gram", C.c_double),
-                ("ms_select", C.c_double), ("ms_csr", C.c_double)]
+                ("ms_select", C.c_double), ("ms_csr", C.c_double),
+                ("ms_gram_hist", C.c_double), ("ms_gram_emit", C.c_double)]
 
 
 @dataclass
@@ -59,6 +60,8 @@ class CorrInfo:
     ms_gram: float
     ms_select: float
     ms_csr: float
+    ms_gram_hist: float
+    ms_gram_emit: float
 
 
 _LIB = None

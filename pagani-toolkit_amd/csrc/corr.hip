@@ -25,11 +25,14 @@
 namespace {
 
 constexpr int TILE = 128;      // output tile edge (rows and columns)
-constexpr int KC = 32;         // K (time) chunk staged in LDS
+#ifndef PG_GRAM_KC
+#define PG_GRAM_KC 64
+#endif
+constexpr int KC = PG_GRAM_KC; // K (time) chunk staged in LDS
 constexpr int THREADS = 256;   // 4 wave64s, each owns a 64 x 64 quarter
 constexpr int NBIN = 8192;     // r histogram bins over [-1, 1] (sparsity mode)
 constexpr int LDB = KC + 8;    // bf16 LDS row stride (elements), 16-byte aligned rows
-constexpr int LDF = KC + 4;    // fp32 LDS row stride (elements)
+constexpr int LDF = KC + 1;    // fp32 LDS row stride (elements): conflict-free column reads
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
@@ -57,6 +60,8 @@ struct GramArgs {
     long long band_cap;
     unsigned long long *band_cnt;  // [1]
     uint32_t *hist;        // [gridDim.x][NBIN]
+    float *seg_r;          // optional: r~ of each segment entry
+    long long sample_stride;  // > 0: visit only tiles t = k * stride (histogram sample)
 };
 
 __device__ __forceinline__ uint16_t f2bf(float f)
@@ -133,66 +138,153 @@ __global__ void zscore_write(const float *__restrict__ x, int layout, int n, int
 }
 
 // ------------------------------------------------------------ Gram tiles
+//
+// Schedule.  Work-groups are dispatched round-robin over the 8 XCDs, so
+// work-group g runs on XCD g % 8.  All work-groups of one XCD walk the same
+// 8 x 8 supertile of 128 x 128 tiles at a time: the 16 panels it touches (8
+// row blocks + 8 column blocks; 2 MB for BF16x3 at T = 256) stay in that XCD's
+// L2 while its CUs consume them, instead of every work-group streaming its own
+// panels from HBM / MALL.
+constexpr int SUPER = 8;
+#ifndef PG_GRAM_WG_PER_CU
+#define PG_GRAM_WG_PER_CU 2
+#endif
+constexpr int NXCD = 8;
 
-// Upper-triangle tile index -> (bi, bj), bi <= bj, row-major over bi.
-__device__ __forceinline__ void tile_coords(long long tid, int nb, int &bi, int &bj)
+// Upper-triangle index -> (r, c), r <= c, row-major over r, for an m x m grid.
+__device__ __forceinline__ void tri_coords(long long tid, int m, int &r, int &c)
 {
-    // start(bi) = bi*nb - bi*(bi-1)/2; solve start(bi) <= tid < start(bi+1)
-    double b = 2.0 * nb + 1.0;
-    int r = (int)floor((b - sqrt(b * b - 8.0 * (double)tid)) * 0.5);
-    if (r < 0) r = 0;
-    if (r > nb - 1) r = nb - 1;
-    while (r > 0 && (long long)r * nb - (long long)r * (r - 1) / 2 > tid) r--;
-    while (r < nb - 1 && (long long)(r + 1) * nb - (long long)(r + 1) * r / 2 <= tid) r++;
-    bi = r;
-    bj = r + (int)(tid - ((long long)r * nb - (long long)r * (r - 1) / 2));
+    // start(r) = r*m - r*(r-1)/2; solve start(r) <= tid < start(r+1)
+    double b = 2.0 * m + 1.0;
+    int x = (int)floor((b - sqrt(b * b - 8.0 * (double)tid)) * 0.5);
+    if (x < 0) x = 0;
+    if (x > m - 1) x = m - 1;
+    while (x > 0 && (long long)x * m - (long long)x * (x - 1) / 2 > tid) x--;
+    while (x < m - 1 && (long long)(x + 1) * m - (long long)(x + 1) * x / 2 <= tid) x++;
+    r = x;
+    c = x + (int)(tid - ((long long)x * m - (long long)x * (x - 1) / 2));
 }
 
-template <int PREC>
-__device__ __forceinline__ void load_chunk(const GramArgs &a, char *smem, int bi, int bj, int kc)
-{
-    const int tid = threadIdx.x;
-    if (PREC == PREC_BF16X3) {
-        uint16_t *sAh = (uint16_t *)smem;
-        uint16_t *sAl = sAh + TILE * LDB;
-        uint16_t *sBh = sAl + TILE * LDB;
-        uint16_t *sBl = sBh + TILE * LDB;
-        const uint16_t *gh = (const uint16_t *)a.p0;
-        const uint16_t *gl = (const uint16_t *)a.p1;
-        constexpr int PIECES = TILE * KC / 8;  // 16-byte pieces per panel chunk
-#pragma unroll
-        for (int it = 0; it < PIECES / THREADS; it++) {
-            int id = it * THREADS + tid;
-            int r = id / (KC / 8), c = (id % (KC / 8)) * 8;
-            long long ga = (long long)(bi * TILE + r) * a.ldp + kc * KC + c;
-            long long gb = (long long)(bj * TILE + r) * a.ldp + kc * KC + c;
-            u32x4 vah = *(const u32x4 *)(gh + ga);
-            u32x4 val = *(const u32x4 *)(gl + ga);
-            u32x4 vbh = *(const u32x4 *)(gh + gb);
-            u32x4 vbl = *(const u32x4 *)(gl + gb);
-            *(u32x4 *)(sAh + r * LDB + c) = vah;
-            *(u32x4 *)(sAl + r * LDB + c) = val;
-            *(u32x4 *)(sBh + r * LDB + c) = vbh;
-            *(u32x4 *)(sBl + r * LDB + c) = vbl;
+struct TileWalk {
+    int nb, nsb;        // tile blocks, supertile blocks per side
+    long long nst;      // supertiles in the upper triangle
+    int L;              // work-groups sharing this XCD's tile sequence
+    long long k;        // next position in this XCD's tile sequence
+    long long kend;     // positions available to this XCD
+    int xcd;
+
+    // This XCD's sequence is the concatenation of its supertiles (xcd,
+    // xcd + 8, ...), SUPER*SUPER slots each; work-group slot l takes
+    // positions l, l + L, l + 2L, ... so the XCD's work-groups move through
+    // the same supertile together.
+    long long stride, ntiles;
+
+    __device__ void init(int nb_, long long stride_)
+    {
+        nb = nb_;
+        nsb = (nb + SUPER - 1) / SUPER;
+        nst = (long long)nsb * (nsb + 1) / 2;
+        const int g = blockIdx.x, G = gridDim.x;
+        stride = stride_;
+        ntiles = (long long)nb * (nb + 1) / 2;
+        if (stride > 0) {  // sampled tiles, strided over the row-major upper triangle
+            k = g;
+            L = G;
+            kend = (ntiles + stride - 1) / stride;
+            xcd = 0;
+            return;
         }
-    } else {
-        float *sA = (float *)smem;
-        float *sB = sA + TILE * LDF;
-        const float *g = (const float *)a.p0;
-        constexpr int PIECES = TILE * KC / 4;
+        xcd = g % NXCD;
+        L = G / NXCD;
+        k = g / NXCD;
+        const long long mine = nst > xcd ? (nst - 1 - xcd) / NXCD + 1 : 0;
+        kend = mine * SUPER * SUPER;
+    }
+    __device__ bool next(int &bi, int &bj)
+    {
+        if (stride > 0) {
+            while (k < kend) {
+                const long long t = k * stride + stride / 2;
+                k += L;
+                if (t < ntiles) {
+                    tri_coords(t, nb, bi, bj);
+                    return true;
+                }
+            }
+            return false;
+        }
+        while (k < kend) {
+            const long long kk = k;
+            k += L;
+            const long long s = xcd + (kk / (SUPER * SUPER)) * NXCD;
+            const int q = (int)(kk % (SUPER * SUPER));
+            int sr, sc;
+            tri_coords(s, nsb, sr, sc);
+            const int ti = sr * SUPER + q / SUPER, tj = sc * SUPER + q % SUPER;
+            if (ti <= tj && tj < nb) {
+                bi = ti;
+                bj = tj;
+                return true;
+            }
+        }
+        return false;
+    }
+};
+
+// Global -> register staging of one K chunk of the row and column panels.
+template <int PREC>
+struct Stage {
+    static constexpr int VEC = PREC == PREC_BF16X3 ? 8 : 4;         // elements per 16 B
+    static constexpr int PIECES = TILE * KC / VEC;                  // per panel array
+    static constexpr int ITERS = PIECES / THREADS;
+    static constexpr int NARR = PREC == PREC_BF16X3 ? 2 : 1;        // hi/lo or f32
+    u32x4 a[NARR][ITERS], b[NARR][ITERS];
+
+    __device__ __forceinline__ void load(const GramArgs &g, int bi, int bj, int kc)
+    {
+        const int tid = threadIdx.x;
 #pragma unroll
-        for (int it = 0; it < PIECES / THREADS; it++) {
-            int id = it * THREADS + tid;
-            int r = id / (KC / 4), c = (id % (KC / 4)) * 4;
-            long long ga = (long long)(bi * TILE + r) * a.ldp + kc * KC + c;
-            long long gb = (long long)(bj * TILE + r) * a.ldp + kc * KC + c;
-            u32x4 va = *(const u32x4 *)(g + ga);
-            u32x4 vb = *(const u32x4 *)(g + gb);
-            *(u32x4 *)(sA + r * LDF + c) = va;
-            *(u32x4 *)(sB + r * LDF + c) = vb;
+        for (int it = 0; it < ITERS; it++) {
+            const int id = it * THREADS + tid;
+            const int r = id / (KC / VEC), c = (id % (KC / VEC)) * VEC;
+            const long long ga = (long long)(bi * TILE + r) * g.ldp + kc * KC + c;
+            const long long gb = (long long)(bj * TILE + r) * g.ldp + kc * KC + c;
+            if (PREC == PREC_BF16X3) {
+                a[0][it] = *(const u32x4 *)((const uint16_t *)g.p0 + ga);
+                a[NARR - 1][it] = *(const u32x4 *)((const uint16_t *)g.p1 + ga);
+                b[0][it] = *(const u32x4 *)((const uint16_t *)g.p0 + gb);
+                b[NARR - 1][it] = *(const u32x4 *)((const uint16_t *)g.p1 + gb);
+            } else {
+                a[0][it] = *(const u32x4 *)((const float *)g.p0 + ga);
+                b[0][it] = *(const u32x4 *)((const float *)g.p0 + gb);
+            }
         }
     }
-}
+    __device__ __forceinline__ void store(char *smem) const
+    {
+        const int tid = threadIdx.x;
+#pragma unroll
+        for (int it = 0; it < ITERS; it++) {
+            const int id = it * THREADS + tid;
+            const int r = id / (KC / VEC), c = (id % (KC / VEC)) * VEC;
+            if (PREC == PREC_BF16X3) {
+                uint16_t *s = (uint16_t *)smem;
+                *(u32x4 *)(s + 0 * TILE * LDB + r * LDB + c) = a[0][it];
+                *(u32x4 *)(s + 1 * TILE * LDB + r * LDB + c) = a[NARR - 1][it];
+                *(u32x4 *)(s + 2 * TILE * LDB + r * LDB + c) = b[0][it];
+                *(u32x4 *)(s + 3 * TILE * LDB + r * LDB + c) = b[NARR - 1][it];
+            } else {
+                // odd row stride: 4-byte stores
+                float *s = (float *)smem;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    s[r * LDF + c + e] = __uint_as_float(a[0][it][e]);
+                    s[TILE * LDF + r * LDF + c + e] = __uint_as_float(b[0][it][e]);
+                }
+            }
+        }
+    }
+};
 
 template <int PREC>
 __device__ __forceinline__ void mfma_chunk(const char *smem, f32x16 (&acc)[2][2])
@@ -217,14 +309,22 @@ __device__ __forceinline__ void mfma_chunk(const char *smem, f32x16 (&acc)[2][2]
                 bh[b] = *(const bf16x8 *)(sBh + rb);
                 bl[b] = *(const bf16x8 *)(sBl + rb);
             }
+            // small cross terms first, independent accumulators interleaved
 #pragma unroll
             for (int bm = 0; bm < 2; bm++)
 #pragma unroll
-                for (int bn = 0; bn < 2; bn++) {
+                for (int bn = 0; bn < 2; bn++)
                     acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[bm], bh[bn], acc[bm][bn], 0, 0, 0);
+#pragma unroll
+            for (int bm = 0; bm < 2; bm++)
+#pragma unroll
+                for (int bn = 0; bn < 2; bn++)
                     acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[bm], bl[bn], acc[bm][bn], 0, 0, 0);
+#pragma unroll
+            for (int bm = 0; bm < 2; bm++)
+#pragma unroll
+                for (int bn = 0; bn < 2; bn++)
                     acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[bm], bh[bn], acc[bm][bn], 0, 0, 0);
-                }
         }
     } else {
         const float *sA = (const float *)smem;
@@ -252,10 +352,10 @@ __device__ __forceinline__ unsigned lane_rank(unsigned long long m)
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-// Persistent over a contiguous range of upper-triangle tiles (neighbouring
-// tiles share the row panel bi, which stays in the XCD's L2).
+// Persistent: each work-group walks its tiles (TileWalk), prefetching the next
+// K chunk into registers while the MFMAs consume the current one from LDS.
 template <int PREC, int MODE>
-__global__ __launch_bounds__(THREADS, 2) void gram_kernel(GramArgs a)
+__global__ __launch_bounds__(THREADS, PG_GRAM_WG_PER_CU) void gram_kernel(GramArgs a)
 {
     extern __shared__ __align__(16) char smem[];
     constexpr int STAGE_BYTES = PREC == PREC_BF16X3 ? 4 * TILE * LDB * 2 : 2 * TILE * LDF * 4;
@@ -268,13 +368,15 @@ __global__ __launch_bounds__(THREADS, 2) void gram_kernel(GramArgs a)
     if (MODE == MODE_HIST)
         for (int b = tid; b < NBIN; b += THREADS) s_hist[b] = 0;
 
-    const long long t0 = (long long)blockIdx.x * a.per;
-    const long long t1 = min(a.ntiles, t0 + a.per);
-    int bi = 0, bj = 0;
-    if (t0 < t1) tile_coords(t0, a.nb, bi, bj);
+    TileWalk walk;
+    walk.init(a.nb, a.sample_stride);
     int2 *myseg = a.seg + (long long)blockIdx.x * a.seg_cap;
+    int bi = 0, bj = 0;
+    bool have = walk.next(bi, bj);
+    Stage<PREC> pre;
+    if (have) pre.load(a, bi, bj, 0);
 
-    for (long long tt = t0; tt < t1; tt++) {
+    while (have) {
         f32x16 acc[2][2];
 #pragma unroll
         for (int bm = 0; bm < 2; bm++)
@@ -282,16 +384,24 @@ __global__ __launch_bounds__(THREADS, 2) void gram_kernel(GramArgs a)
             for (int bn = 0; bn < 2; bn++)
 #pragma unroll
                 for (int q = 0; q < 16; q++) acc[bm][bn][q] = 0.0f;
-
+        int nbi = 0, nbj = 0;
+        bool nhave = false;
         for (int kc = 0; kc < a.kchunks; kc++) {
             __syncthreads();
-            load_chunk<PREC>(a, smem, bi, bj, kc);
+            pre.store(smem);
             __syncthreads();
+            if (kc + 1 < a.kchunks) {
+                pre.load(a, bi, bj, kc + 1);
+            } else {
+                nhave = walk.next(nbi, nbj);
+                if (nhave) pre.load(a, nbi, nbj, 0);
+            }
             mfma_chunk<PREC>(smem, acc);
         }
 
         // epilogue: 32x32 block layout, lane holds column lane%32 and rows
         // (q/4)*8 + (lane/32)*4 + q%4 of each block.
+        const bool interior = bi < bj && (bj + 1) * TILE <= a.n;
 #pragma unroll
         for (int bm = 0; bm < 2; bm++)
 #pragma unroll
@@ -301,46 +411,82 @@ __global__ __launch_bounds__(THREADS, 2) void gram_kernel(GramArgs a)
                     const float v = acc[bm][bn][q];
                     const int i = bi * TILE + wm * 64 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
                     const int j = bj * TILE + wn * 64 + bn * 32 + (lane & 31);
-                    const bool valid = (j < a.n) && (i < j);
                     if (MODE == MODE_HIST) {
-                        if (valid) {
+                        if (interior || ((j < a.n) && (i < j))) {
                             int b = (int)((v + 1.0f) * (NBIN * 0.5f));
                             b = b < 0 ? 0 : (b >= NBIN ? NBIN - 1 : b);
                             atomicAdd(&s_hist[b], 1u);
                         }
-                    } else {
-                        const bool e = valid && (v > a.upper);
-                        const bool c = valid && !e && (v >= a.lower);
-                        const unsigned long long me = __ballot(e);
-                        if (me) {
-                            unsigned long long base = 0;
-                            if (lane == 0) base = atomicAdd(&s_seg, (unsigned long long)__popcll(me));
-                            base = __shfl(base, 0);
-                            if (e) {
-                                unsigned long long p = base + lane_rank(me);
-                                if ((long long)p < a.seg_cap) myseg[p] = make_int2(i, j);
-                            }
+                    }
+                }
+        if (MODE == MODE_EMIT) {
+            // pass 1: count edges and rechecks with one compare per value and
+            // scalar popcounts of the ballots; pass 2 writes them after one
+            // LDS reservation per wave and tile.
+            unsigned ne = 0, nc = 0;
+#pragma unroll
+            for (int bm = 0; bm < 2; bm++)
+#pragma unroll
+                for (int bn = 0; bn < 2; bn++)
+#pragma unroll
+                    for (int q = 0; q < 16; q++) {
+                        const float v = acc[bm][bn][q];
+                        const int i = bi * TILE + wm * 64 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
+                        const int j = bj * TILE + wn * 64 + bn * 32 + (lane & 31);
+                        bool hit = v >= a.lower;
+                        if (!interior) hit = hit && (j < a.n) && (i < j);
+                        const unsigned long long mh = __ballot(hit);
+                        if (mh) {
+                            const unsigned long long me = __ballot(hit && v > a.upper);
+                            ne += __popcll(me);
+                            nc += __popcll(mh & ~me);
                         }
-                        const unsigned long long mc = __ballot(c);
-                        if (mc) {
-                            unsigned long long base = 0;
-                            if (lane == 0) base = atomicAdd(a.band_cnt, (unsigned long long)__popcll(mc));
-                            base = __shfl(base, 0);
-                            if (c) {
-                                unsigned long long p = base + lane_rank(mc);
+                    }
+            if (ne + nc) {
+                unsigned long long be = 0, bc = 0;
+                if (lane == 0) {
+                    if (ne) be = atomicAdd(&s_seg, (unsigned long long)ne);
+                    if (nc) bc = atomicAdd(a.band_cnt, (unsigned long long)nc);
+                }
+                be = __shfl(be, 0);
+                bc = __shfl(bc, 0);
+#pragma unroll
+                for (int bm = 0; bm < 2; bm++)
+#pragma unroll
+                    for (int bn = 0; bn < 2; bn++)
+#pragma unroll
+                        for (int q = 0; q < 16; q++) {
+                            const float v = acc[bm][bn][q];
+                            const int i = bi * TILE + wm * 64 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
+                            const int j = bj * TILE + wn * 64 + bn * 32 + (lane & 31);
+                            bool hit = v >= a.lower;
+                            if (!interior) hit = hit && (j < a.n) && (i < j);
+                            const unsigned long long mh = __ballot(hit);
+                            if (mh == 0ull) continue;
+                            const bool e = hit && v > a.upper;
+                            const unsigned long long me = __ballot(e);
+                            const unsigned long long mc = mh & ~me;
+                            if (e) {
+                                const unsigned long long p = be + lane_rank(me);
+                                if ((long long)p < a.seg_cap) {
+                                    myseg[p] = make_int2(i, j);
+                                    if (a.seg_r) a.seg_r[(long long)blockIdx.x * a.seg_cap + p] = v;
+                                }
+                            } else if (hit) {
+                                const unsigned long long p = bc + lane_rank(mc);
                                 if ((long long)p < a.band_cap) {
                                     a.band[p] = make_int2(i, j);
                                     a.band_r[p] = v;
                                 }
                             }
+                            be += __popcll(me);
+                            bc += __popcll(mc);
                         }
-                    }
-                }
-        // next tile in row-major upper-triangle order
-        if (++bj == a.nb) {
-            bi++;
-            bj = bi;
+            }
         }
+        bi = nbi;
+        bj = nbj;
+        have = nhave;
     }
     __syncthreads();
     if (MODE == MODE_HIST) {
@@ -418,7 +564,14 @@ struct EdgeSrc {
     int nseg;
     const int2 *extra;
     const unsigned long long *extra_cnt;
+    const float *seg_r;   // optional filter: keep segment entries with r~ > keep_above
+    float keep_above;
 };
+
+__device__ __forceinline__ bool src_keep(const EdgeSrc &s, int g, long long k)
+{
+    return !(s.seg_r && g < s.nseg && s.seg_r[(long long)g * s.seg_cap + k] <= s.keep_above);
+}
 
 __device__ __forceinline__ long long src_count(const EdgeSrc &s, int g)
 {
@@ -429,118 +582,340 @@ __device__ __forceinline__ int2 src_edge(const EdgeSrc &s, int g, long long k)
     return g < s.nseg ? s.seg[(long long)g * s.seg_cap + k] : s.extra[k];
 }
 
-// One block per source segment (the last block takes the accepted rechecks).
-__global__ void degree_count(EdgeSrc s, int *__restrict__ deg)
+// Two-level counting sort into CSR.  Both directions of every kept edge go to
+// the bucket of 128 rows holding their source row: per-segment bucket counts
+// in LDS, a 2-D exclusive scan (segment-major inside each bucket) that fixes
+// every segment's output range, a scatter of packed 32-bit entries
+// (row-in-bucket << 25 | column), then one work-group per bucket derives the
+// row offsets and sorts each row by LDS-bitmap rank.  No global atomics.
+constexpr int BSH = 7;
+constexpr int BROWS = 1 << BSH;
+constexpr unsigned COLMASK = (1u << 25) - 1u;
+
+__global__ __launch_bounds__(256) void bucket_count(EdgeSrc s, int nbk, unsigned *__restrict__ cnt)
 {
+    extern __shared__ unsigned sc[];
+    for (int b = threadIdx.x; b < nbk; b += 256) sc[b] = 0u;
+    __syncthreads();
     const int g = blockIdx.x;
     const long long c = src_count(s, g);
-    for (long long k = threadIdx.x; k < c; k += blockDim.x) {
-        int2 e = src_edge(s, g, k);
-        atomicAdd(&deg[e.x], 1);
-        atomicAdd(&deg[e.y], 1);
+    for (long long k = threadIdx.x; k < c; k += 256) {
+        if (!src_keep(s, g, k)) continue;
+        const int2 e = src_edge(s, g, k);
+        atomicAdd(&sc[e.x >> BSH], 1u);
+        atomicAdd(&sc[e.y >> BSH], 1u);
     }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbk; b += 256) cnt[(long long)g * nbk + b] = sc[b];
 }
 
-// Exclusive scan of deg[0..n) into row[0..n], single block of 1024 threads.
-__global__ __launch_bounds__(1024) void scan_rows(const int *__restrict__ deg, int n,
-                                                   int *__restrict__ row, int *__restrict__ cur)
+// off[g][b] = bstart[b] + sum_{g' < g} cnt[g'][b];  bstart = exclusive scan of
+// the bucket totals (bstart[nbk] = total entries).
+__global__ __launch_bounds__(1024) void bucket_scan(const unsigned *__restrict__ cnt, int nsrc, int nbk,
+                                                    long long *__restrict__ off, long long *__restrict__ bstart)
 {
     __shared__ long long part[1024];
     const int tid = threadIdx.x;
-    const long long per = (n + 1023) / 1024;
-    const long long a = tid * per, b = min((long long)n, a + per);
+    const int per = (nbk + 1023) / 1024;
+    const int b0 = tid * per, b1 = min(nbk, b0 + per);
     long long s = 0;
-    for (long long i = a; i < b; i++) s += deg[i];
+    for (int b = b0; b < b1; b++)
+        for (int g = 0; g < nsrc; g++) s += cnt[(long long)g * nbk + b];
     part[tid] = s;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        long long v = tid >= off ? part[tid - off] : 0;
+    for (int o = 1; o < 1024; o <<= 1) {
+        long long v = tid >= o ? part[tid - o] : 0;
         __syncthreads();
         part[tid] += v;
         __syncthreads();
     }
     long long run = part[tid] - s;
-    for (long long i = a; i < b; i++) {
-        row[i] = (int)run;
-        cur[i] = 0;
-        run += deg[i];
+    for (int b = b0; b < b1; b++) {
+        bstart[b] = run;
+        for (int g = 0; g < nsrc; g++) {
+            off[(long long)g * nbk + b] = run;
+            run += cnt[(long long)g * nbk + b];
+        }
     }
-    if (tid == 1023) row[n] = (int)part[1023];
+    if (tid == 1023) bstart[nbk] = part[1023];
 }
 
-__global__ void scatter_edges(EdgeSrc s, const int *__restrict__ row, int *__restrict__ cur,
-                              int *__restrict__ ctmp, float *__restrict__ vtmp,
-                              const double *__restrict__ z, int t)
+__global__ __launch_bounds__(256) void bucket_scatter(EdgeSrc s, int nbk, const long long *__restrict__ off,
+                                                      unsigned *__restrict__ ent, float *__restrict__ entv,
+                                                      const double *__restrict__ z, int t)
 {
+    extern __shared__ unsigned sc[];  // per-bucket cursors
+    for (int b = threadIdx.x; b < nbk; b += 256) sc[b] = 0u;
+    __syncthreads();
     const int g = blockIdx.x;
+    const long long *o = off + (long long)g * nbk;
     const long long c = src_count(s, g);
-    for (long long k = threadIdx.x; k < c; k += blockDim.x) {
-        int2 e = src_edge(s, g, k);
-        int p = atomicAdd(&cur[e.x], 1);
-        int q = atomicAdd(&cur[e.y], 1);
-        ctmp[row[e.x] + p] = e.y;
-        ctmp[row[e.y] + q] = e.x;
-        if (vtmp) {
-            float r = (float)r64_pair(z, t, e.x, e.y);
-            vtmp[row[e.x] + p] = r;
-            vtmp[row[e.y] + q] = r;
+    for (long long k = threadIdx.x; k < c; k += 256) {
+        if (!src_keep(s, g, k)) continue;
+        const int2 e = src_edge(s, g, k);
+        const long long pf = o[e.x >> BSH] + atomicAdd(&sc[e.x >> BSH], 1u);
+        const long long pr = o[e.y >> BSH] + atomicAdd(&sc[e.y >> BSH], 1u);
+        ent[pf] = ((unsigned)(e.x & (BROWS - 1)) << 25) | (unsigned)e.y;
+        ent[pr] = ((unsigned)(e.y & (BROWS - 1)) << 25) | (unsigned)e.x;
+        if (entv) {
+            const float r = (float)r64_pair(z, t, e.x, e.y);
+            entv[pf] = r;
+            entv[pr] = r;
         }
     }
 }
 
-// Sort each row's columns ascending: mark columns in an LDS bitmap over
-// [0, n), prefix-count the words, then place each entry at its rank.
-__global__ __launch_bounds__(256) void sort_rows(const int *__restrict__ row, const int *__restrict__ ctmp,
-                                                 const float *__restrict__ vtmp, int n,
-                                                 int *__restrict__ col, float *__restrict__ val)
+// One work-group per bucket: row counts -> row offsets, scatter into the
+// rows, then each wave sorts whole rows (nbm LDS bitmaps over [0, n)).
+__global__ __launch_bounds__(256) void bucket_rows(const long long *__restrict__ bstart, const unsigned *__restrict__ ent,
+                                                   const float *__restrict__ entv, int n, int nbk, int nbm,
+                                                   int *__restrict__ row, int *__restrict__ ctmp, float *__restrict__ vtmp,
+                                                   int *__restrict__ col, float *__restrict__ val)
 {
     extern __shared__ uint32_t sm[];
-    __shared__ int s_part[256];
-    const int nw = (n + 31) >> 5;
-    uint32_t *bits = sm;
-    uint32_t *pre = sm + nw;
-    const int r = blockIdx.x, tid = threadIdx.x;
-    const int a = row[r], b = row[r + 1];
-    const int d = b - a;
-    if (d == 0) return;
-    if (d == 1) {
-        if (tid == 0) {
-            col[a] = ctmp[a];
-            if (val) val[a] = vtmp[a];
+    __shared__ unsigned rc[BROWS], rb[BROWS + 1];
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const long long e0 = bstart[b], e1 = bstart[b + 1];
+    if (tid < BROWS) rc[tid] = 0u;
+    __syncthreads();
+    for (long long k = e0 + tid; k < e1; k += 256) atomicAdd(&rc[ent[k] >> 25], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        unsigned run = 0;
+        for (int r = 0; r < BROWS; r++) {
+            rb[r] = run;
+            run += rc[r];
         }
+        rb[BROWS] = run;
+    }
+    __syncthreads();
+    const int r0 = b * BROWS;
+    if (tid < BROWS && r0 + tid < n) row[r0 + tid] = (int)(e0 + rb[tid]);
+    if (b == nbk - 1 && tid == 0) row[n] = (int)e1;
+    if (tid < BROWS) rc[tid] = 0u;
+    __syncthreads();
+    for (long long k = e0 + tid; k < e1; k += 256) {
+        const unsigned x = ent[k];
+        const unsigned lr = x >> 25;
+        const long long p = e0 + rb[lr] + atomicAdd(&rc[lr], 1u);
+        ctmp[p] = (int)(x & COLMASK);
+        if (vtmp) vtmp[p] = entv[k];
+    }
+    __syncthreads();
+    // sort rows: wave wv < nbm takes rows wv, wv + nbm, ...
+    if (wv >= nbm) return;
+    const int nw = (n + 31) >> 5;
+    uint32_t *bits = sm + (long long)wv * 2 * nw;
+    uint32_t *pre = bits + nw;
+    for (int lr = wv; lr < BROWS && r0 + lr < n; lr += nbm) {
+        const long long a = e0 + rb[lr], z1 = e0 + rb[lr + 1];
+        const long long d = z1 - a;
+        if (d == 0) continue;
+        if (d == 1) {
+            if (lane == 0) {
+                col[a] = ctmp[a];
+                if (val) val[a] = vtmp[a];
+            }
+            continue;
+        }
+        for (int q = lane; q < nw; q += 64) bits[q] = 0u;
+        __builtin_amdgcn_wave_barrier();
+        for (long long e = a + lane; e < z1; e += 64) {
+            const int c = ctmp[e];
+            atomicOr(&bits[c >> 5], 1u << (c & 31));
+        }
+        __builtin_amdgcn_wave_barrier();
+        // prefix popcount over words: per-lane contiguous ranges + wave scan
+        const int per = (nw + 63) / 64;
+        const int w0 = lane * per, w1 = min(nw, w0 + per);
+        int sacc = 0;
+        for (int q = w0; q < w1; q++) sacc += __popc(bits[q]);
+        int incl = sacc;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        int run = incl - sacc;
+        for (int q = w0; q < w1; q++) {
+            pre[q] = run;
+            run += __popc(bits[q]);
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (long long e = a + lane; e < z1; e += 64) {
+            const int c = ctmp[e];
+            const int pos = pre[c >> 5] + __popc(bits[c >> 5] & ((1u << (c & 31)) - 1u));
+            col[a + pos] = c;
+            if (val) val[a + pos] = vtmp[e];
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ------------------------------------------------------ sparsity selection
+
+constexpr int NBIN2 = 16384;   // fine histogram over the emitted set
+
+// Histogram of r~ over the kept segment entries, bins over [lo, lo + NBIN2/scale).
+__global__ __launch_bounds__(256) void emitted_hist(const float *__restrict__ seg_r, long long seg_cap,
+                                                    const unsigned long long *__restrict__ seg_cnt, int nseg,
+                                                    float lo, float scale, uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t h[NBIN2];
+    for (int b = threadIdx.x; b < NBIN2; b += 256) h[b] = 0;
+    __syncthreads();
+    for (int g = blockIdx.x; g < nseg; g += gridDim.x) {
+        const long long c = min((long long)seg_cnt[g], seg_cap);
+        const float *r = seg_r + (long long)g * seg_cap;
+        for (long long k = threadIdx.x; k < c; k += 256) {
+            int b = (int)((r[k] - lo) * scale);
+            b = b < 0 ? 0 : (b >= NBIN2 ? NBIN2 - 1 : b);
+            atomicAdd(&h[b], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t *o = out + (long long)blockIdx.x * NBIN2;
+    for (int b = threadIdx.x; b < NBIN2; b += 256) o[b] = h[b];
+}
+
+__global__ void hist_sum(const uint32_t *__restrict__ h, int nrow, int nbin, unsigned long long *__restrict__ out)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbin) return;
+    unsigned long long s = 0;
+    for (int g = 0; g < nrow; g++) s += h[(long long)g * nbin + b];
+    out[b] = s;
+}
+
+// Split the emitted set: r~ > hi counts as a certain edge (kept by the CSR
+// filter), lo <= r~ <= hi goes to the FP64 candidate list.
+__global__ __launch_bounds__(256) void window_kernel(const int2 *__restrict__ seg, const float *__restrict__ seg_r,
+                                                     long long seg_cap, const unsigned long long *__restrict__ seg_cnt,
+                                                     int nseg, float lo, float hi, int2 *__restrict__ cand,
+                                                     float *__restrict__ cand_r, long long cand_cap,
+                                                     unsigned long long *__restrict__ cand_cnt,
+                                                     unsigned long long *__restrict__ definite)
+{
+    __shared__ unsigned long long s_def;
+    if (threadIdx.x == 0) s_def = 0;
+    __syncthreads();
+    unsigned long long mine = 0;
+    for (int g = blockIdx.x; g < nseg; g += gridDim.x) {
+        const long long c = min((long long)seg_cnt[g], seg_cap);
+        const long long base = (long long)g * seg_cap;
+        for (long long k = threadIdx.x; k < c; k += 256) {
+            const float r = seg_r[base + k];
+            if (r > hi) mine++;
+            else if (r >= lo) {
+                unsigned long long p = atomicAdd(cand_cnt, 1ull);
+                if ((long long)p < cand_cap) {
+                    cand[p] = seg[base + k];
+                    cand_r[p] = r;
+                }
+            }
+        }
+    }
+    atomicAdd(&s_def, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(definite, s_def);
+}
+
+// Keys ordered like the reference's selection: r64 descending, then row-major.
+__device__ __forceinline__ unsigned long long key_r_desc(double x)
+{
+    unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    b = (b >> 63) ? ~b : (b | 0x8000000000000000ull);  // ascending order
+    return ~b;                                        // descending
+}
+
+// m-th smallest (1-based) key among the entries passing `want`, by 8-bit
+// radix select in one work-group.  Returns the key; *rank_left gets m minus
+// the number of strictly smaller keys.
+template <class KeyF, class WantF>
+__device__ unsigned long long radix_select(long long c, long long m, KeyF key, WantF want,
+                                           unsigned *hist, unsigned long long *sh, long long *rank_left)
+{
+    unsigned long long prefix = 0, mask = 0;
+    long long mr = m;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        for (int d = threadIdx.x; d < 256; d += blockDim.x) hist[d] = 0;
+        __syncthreads();
+        for (long long q = threadIdx.x; q < c; q += blockDim.x) {
+            if (!want(q)) continue;
+            const unsigned long long kq = key(q);
+            if ((kq & mask) == prefix) atomicAdd(&hist[(kq >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long cum = 0;
+            int dsel = 255;
+            for (int d = 0; d < 256; d++) {
+                if (cum + (long long)hist[d] >= mr) { dsel = d; break; }
+                cum += hist[d];
+            }
+            sh[0] = (unsigned long long)dsel;
+            sh[1] = (unsigned long long)(mr - cum);
+        }
+        __syncthreads();
+        prefix |= sh[0] << shift;
+        mask |= 255ull << shift;
+        mr = (long long)sh[1];
+        __syncthreads();
+    }
+    *rank_left = mr;
+    return prefix;
+}
+
+// Exact selection (U CorMat.cpp:693-773 on FP64 r): need = k - definite
+// candidates, all with r > t* plus the earliest ties in row-major order.
+__global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ cand, const double *__restrict__ r64,
+                                                      const float *__restrict__ rfast,
+                                                      const unsigned long long *__restrict__ cand_cnt, long long cand_cap,
+                                                      long long k, const unsigned long long *__restrict__ dsrc, int dn,
+                                                      int2 *__restrict__ extra, unsigned long long *__restrict__ extra_cnt,
+                                                      double *__restrict__ tstar, unsigned long long *__restrict__ flips,
+                                                      int *__restrict__ err)
+{
+    __shared__ unsigned hist[256];
+    __shared__ unsigned long long sh[2];
+    __shared__ unsigned long long s_def;
+    if (threadIdx.x == 0) s_def = 0;
+    __syncthreads();
+    unsigned long long dpart = 0;
+    for (int g = threadIdx.x; g < dn; g += blockDim.x) dpart += dsrc[g];
+    atomicAdd(&s_def, dpart);
+    __syncthreads();
+    const long long c = (long long)min((unsigned long long)cand_cap, *cand_cnt);
+    const long long need = k - (long long)s_def;
+    if (need < 1 || need > c || (long long)*cand_cnt > cand_cap) {
+        if (threadIdx.x == 0) *err = 1;
         return;
     }
-    for (int q = tid; q < nw; q += 256) bits[q] = 0u;
+    long long ties_left = 0;
+    const unsigned long long k1 = radix_select(
+        c, need, [&](long long q) { return key_r_desc(r64[q]); }, [](long long) { return true; }, hist, sh,
+        &ties_left);
+    long long dummy = 0;
+    const unsigned long long k2 = radix_select(
+        c, ties_left,
+        [&](long long q) { return ((unsigned long long)(unsigned)cand[q].x << 32) | (unsigned)cand[q].y; },
+        [&](long long q) { return key_r_desc(r64[q]) == k1; }, hist, sh, &dummy);
+    // t* is the r64 carrying key k1
+    for (long long q = threadIdx.x; q < c; q += blockDim.x)
+        if (key_r_desc(r64[q]) == k1) tstar[0] = r64[q];
     __syncthreads();
-    for (int e = a + tid; e < b; e += 256) {
-        int c = ctmp[e];
-        atomicOr(&bits[c >> 5], 1u << (c & 31));
+    const double ts = tstar[0];
+    unsigned long long fl = 0;
+    for (long long q = threadIdx.x; q < c; q += blockDim.x) {
+        const unsigned long long kq = key_r_desc(r64[q]);
+        const unsigned long long k2q = ((unsigned long long)(unsigned)cand[q].x << 32) | (unsigned)cand[q].y;
+        if (kq < k1 || (kq == k1 && k2q <= k2)) {
+            unsigned long long p = atomicAdd(extra_cnt, 1ull);
+            extra[p] = cand[q];
+        }
+        fl += ((double)rfast[q] > ts) != (r64[q] > ts);
     }
-    __syncthreads();
-    const int per = (nw + 255) / 256;
-    const int w0 = tid * per, w1 = min(nw, w0 + per);
-    int s = 0;
-    for (int q = w0; q < w1; q++) s += __popc(bits[q]);
-    s_part[tid] = s;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        int v = tid >= off ? s_part[tid - off] : 0;
-        __syncthreads();
-        s_part[tid] += v;
-        __syncthreads();
-    }
-    int run = s_part[tid] - s;
-    for (int q = w0; q < w1; q++) {
-        pre[q] = run;
-        run += __popc(bits[q]);
-    }
-    __syncthreads();
-    for (int e = a + tid; e < b; e += 256) {
-        int c = ctmp[e];
-        int pos = pre[c >> 5] + __popc(bits[c >> 5] & ((1u << (c & 31)) - 1u));
-        col[a + pos] = c;
-        if (val) val[a + pos] = vtmp[e];
-    }
+    atomicAdd(flips, fl);
 }
 
 // ------------------------------------------------------------------ host
@@ -581,11 +956,6 @@ float round_down_f(double x)
     if ((double)f > x) f = std::nextafter(f, -INFINITY);
     return f;
 }
-
-struct HostCand {
-    double r;
-    int i, j;
-};
 
 }  // namespace
 
@@ -634,7 +1004,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         inf.t_star = (double)thr_eff;
     }
 
-    // ---- buffers
+    // ---- z-score and MFMA panels
     const int nb = (int)pg_cdiv(std::max(n, 1), TILE);
     const int npad = nb * TILE;
     const int ldp = (int)pg_cdiv(t, KC) * KC;
@@ -668,11 +1038,11 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     }
     PG_HIP(ctx, hipEventRecord(ctx->ev[1], st));
 
-    // ---- Gram launch geometry
+    // ---- Gram launch geometry: PG_GRAM_WG_PER_CU work-groups per CU, a
+    // multiple of the XCD count (see TileWalk)
     const long long ntiles = (long long)nb * (nb + 1) / 2;
-    long long nwg = std::min<long long>((long long)ctx->num_cu * 2, std::max<long long>(ntiles, 1));
-    const long long per = pg_cdiv(std::max<long long>(ntiles, 1), nwg);
-    nwg = pg_cdiv(std::max<long long>(ntiles, 1), per);
+    long long nwg = (long long)ctx->num_cu * PG_GRAM_WG_PER_CU;
+    nwg = std::max<long long>(NXCD, nwg / NXCD * NXCD);
     const double band = error_band(prec, t);
     inf.band = band;
 
@@ -685,7 +1055,6 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     ga.n = n;
     ga.nb = nb;
     ga.ntiles = ntiles;
-    ga.per = per;
     const size_t stage = prec == PREC_BF16X3 ? 4 * TILE * LDB * 2 : 2 * TILE * LDF * 4;
 
     bool ran_hist = false, ran_emit = false;
@@ -712,184 +1081,301 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         PG_HIP(ctx, hipEventRecord(ctx->ev[e0 + 1], st));
         if (mode == MODE_HIST) ran_hist = true;
         else ran_emit = true;
-        inf.gram_passes++;
+        if (ga.sample_stride == 0) inf.gram_passes++;
         return PG_OK;
     };
 
-    // ---- sparsity mode: histogram pass to bracket the k-th largest r~
-    double lo_edge = 0.0, hi_edge = 0.0;
-    float ms_hist = 0.0f;
-    if (smode && kwant > 0 && n > 1) {
+    // counters: [0] band/candidate count, [1] accepted/selected count,
+    // [2] flips, [3] definite (sparsity single pass), [4] error flag
+    unsigned long long *counters = nullptr, *seg_cnt = nullptr;
+    PG_TRY(pg_bufT(ctx, "corr.counters", 16, &counters));
+    PG_TRY(pg_bufT(ctx, "corr.segcnt", (size_t)nwg, &seg_cnt));
+    double *tstar_d = nullptr;
+    PG_TRY(pg_bufT(ctx, "corr.tstar", 2, &tstar_d));
+    std::vector<unsigned long long> hseg(nwg);
+    unsigned long long hcnt[16];
+
+    // One emitting Gram pass, re-run with larger buffers on overflow.
+    // store_r keeps r~ of every segment entry.
+    int2 *seg = nullptr, *bandl = nullptr;
+    float *band_r = nullptr, *seg_r = nullptr;
+    long long seg_cap = 0, band_cap = 0;
+    auto emit_pass = [&](long long seg_cap0, long long band_cap0, bool store_r) -> pg_status {
+        seg_cap = seg_cap0;
+        band_cap = band_cap0;
+        for (int attempt = 0; attempt < 3; attempt++) {
+            PG_TRY(pg_bufT(ctx, "corr.seg", (size_t)(nwg * seg_cap), &seg));
+            if (store_r) PG_TRY(pg_bufT(ctx, "corr.segr", (size_t)(nwg * seg_cap), &seg_r));
+            else seg_r = nullptr;
+            PG_TRY(pg_bufT(ctx, "corr.band", (size_t)band_cap, &bandl));
+            PG_TRY(pg_bufT(ctx, "corr.bandr", (size_t)band_cap, &band_r));
+            PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
+            PG_HIP(ctx, hipMemsetAsync(seg_cnt, 0, nwg * sizeof(unsigned long long), st));
+            ga.seg = seg;
+            ga.seg_r = seg_r;
+            ga.seg_cap = seg_cap;
+            ga.seg_cnt = seg_cnt;
+            ga.band = bandl;
+            ga.band_r = band_r;
+            ga.band_cap = band_cap;
+            ga.band_cnt = counters + 0;
+            ga.sample_stride = 0;
+            if (n > 1) PG_TRY(launch_gram(MODE_EMIT));
+            PG_HIP(ctx, hipMemcpyAsync(hseg.data(), seg_cnt, nwg * 8, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipStreamSynchronize(st));
+            long long mx = 0;
+            for (auto c : hseg) mx = std::max<long long>(mx, (long long)c);
+            if (mx <= seg_cap && (long long)hcnt[0] <= band_cap) return PG_OK;
+            if (attempt == 2) break;
+            seg_cap = std::max(seg_cap, mx + mx / 8 + 4096);
+            band_cap = std::max(band_cap, (long long)hcnt[0] + (long long)hcnt[0] / 8 + 4096);
+        }
+        return pg_fail(ctx, PG_ENOMEM, "edge buffers overflowed repeatedly");
+    };
+
+    // Histogram pass over all tiles (stride 0) or a strided sample of tiles.
+    std::vector<unsigned long long> h(NBIN);
+    auto hist_pass = [&](long long stride) -> pg_status {
         uint32_t *hist;
         unsigned long long *htot;
         PG_TRY(pg_bufT(ctx, "corr.hist", (size_t)nwg * NBIN, &hist));
         PG_TRY(pg_bufT(ctx, "corr.htot", NBIN, &htot));
         ga.hist = hist;
+        ga.sample_stride = stride;
         PG_TRY(launch_gram(MODE_HIST));
+        ga.sample_stride = 0;
         hist_reduce<<<NBIN / 256, 256, 0, st>>>(hist, (int)nwg, htot);
         PG_KERNEL_CHECK(ctx);
-        std::vector<unsigned long long> h(NBIN);
         PG_HIP(ctx, hipMemcpyAsync(h.data(), htot, NBIN * 8, hipMemcpyDeviceToHost, st));
         PG_HIP(ctx, hipStreamSynchronize(st));
-        long long above = 0;
-        int bstar = -1;
-        for (int b = NBIN - 1; b >= 0; b--) {
-            if (above + (long long)h[b] >= kwant) { bstar = b; break; }
-            above += (long long)h[b];
-        }
-        if (bstar < 0) return pg_fail(ctx, PG_ETHRESH, "histogram holds fewer than k=%lld pairs", kwant);
-        const double w = 2.0 / NBIN;
-        lo_edge = bstar * w - 1.0 - 1e-6;
-        hi_edge = (bstar + 1) * w - 1.0 + 1e-6;
-        ga.upper = round_up_f(hi_edge + 2.0 * band);
-        ga.lower = round_down_f(lo_edge - 2.0 * band);
-    } else if (smode) {
-        ga.upper = 2.0f;  // no edges requested
-        ga.lower = 2.0f;
-    } else {
+        return PG_OK;
+    };
+
+    int2 *extra = nullptr;
+    unsigned long long *extra_cnt = counters + 1, *flips = counters + 2;
+    EdgeSrc es;
+    std::memset(&es, 0, sizeof es);
+    float ms_hist = 0.0f;
+
+    if (!smode) {
+        // ---------------------------------------------------- r threshold
         ga.upper = round_up_f((double)thr_eff + band);
         ga.lower = round_down_f((double)thr_eff - band);
-    }
-    PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-
-    // ---- emission pass (re-run with larger buffers on overflow)
-    long long seg_cap;
-    if (smode) seg_cap = pg_cdiv(kwant, nwg) * 3 / 2 + 4096;
-    else seg_cap = pg_cdiv(std::max<long long>((long long)(m1 * 0.03), 1 << 20), nwg) + 4096;
-    long long band_cap = 1 << 22;
-    int2 *seg = nullptr, *bandl = nullptr;
-    float *band_r = nullptr;
-    unsigned long long *seg_cnt = nullptr, *counters = nullptr;
-    PG_TRY(pg_bufT(ctx, "corr.segcnt", (size_t)nwg, &seg_cnt));
-    PG_TRY(pg_bufT(ctx, "corr.counters", 8, &counters));
-    std::vector<unsigned long long> hseg(nwg);
-    unsigned long long hcnt[8];
-    for (int attempt = 0; attempt < 3; attempt++) {
-        PG_TRY(pg_bufT(ctx, "corr.seg", (size_t)(nwg * seg_cap), &seg));
-        PG_TRY(pg_bufT(ctx, "corr.band", (size_t)band_cap, &bandl));
-        PG_TRY(pg_bufT(ctx, "corr.bandr", (size_t)band_cap, &band_r));
-        PG_HIP(ctx, hipMemsetAsync(counters, 0, 8 * sizeof(unsigned long long), st));
-        PG_HIP(ctx, hipMemsetAsync(seg_cnt, 0, nwg * sizeof(unsigned long long), st));
-        ga.seg = seg;
-        ga.seg_cap = seg_cap;
-        ga.seg_cnt = seg_cnt;
-        ga.band = bandl;
-        ga.band_r = band_r;
-        ga.band_cap = band_cap;
-        ga.band_cnt = counters + 0;
-        if (n > 1 && !(smode && kwant == 0)) PG_TRY(launch_gram(MODE_EMIT));
-        PG_HIP(ctx, hipMemcpyAsync(hseg.data(), seg_cnt, nwg * 8, hipMemcpyDeviceToHost, st));
-        PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
-        PG_HIP(ctx, hipStreamSynchronize(st));
-        long long mx = 0;
-        for (auto c : hseg) mx = std::max<long long>(mx, (long long)c);
-        bool ok = mx <= seg_cap && (long long)hcnt[0] <= band_cap;
-        if (ok) break;
-        if (attempt == 2) return pg_fail(ctx, PG_ENOMEM, "edge buffers overflowed repeatedly");
-        seg_cap = std::max(seg_cap, mx + mx / 8 + 4096);
-        band_cap = std::max(band_cap, (long long)hcnt[0] + (long long)hcnt[0] / 8 + 4096);
-    }
-    PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
-    long long definite = 0;
-    for (auto c : hseg) definite += (long long)c;
-    const long long nband = (long long)hcnt[0];
-    inf.band_pairs = nband;
-
-    // ---- FP64 recheck
-    double *band64;
-    int2 *extra;
-    PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)std::max<long long>(nband, 1), &band64));
-    PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)std::max<long long>(nband, 1), &extra));
-    unsigned long long *extra_cnt = counters + 1, *flips = counters + 2;
-    if (nband > 0)
-        recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(bandl, counters + 0, band_cap, z64, t, band64);
-    if (!smode) {
-        if (nband > 0)
+        PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+        const long long seg0 = pg_cdiv(std::max<long long>((long long)(m1 * 0.03), 1 << 20), nwg) + 4096;
+        PG_TRY(emit_pass(seg0, 1 << 22, false));
+        PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+        long long definite = 0;
+        for (auto c : hseg) definite += (long long)c;
+        const long long nband = (long long)hcnt[0];
+        inf.band_pairs = nband;
+        double *band64;
+        PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)std::max<long long>(nband, 1), &band64));
+        PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)std::max<long long>(nband, 1), &extra));
+        if (nband > 0) {
+            recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(bandl, counters + 0, band_cap, z64, t, band64);
             accept_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(
                 bandl, band_r, band64, counters + 0, band_cap, (double)thr_eff, extra, extra_cnt, flips);
+        }
         PG_KERNEL_CHECK(ctx);
         PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
         PG_HIP(ctx, hipStreamSynchronize(st));
         inf.band_flips = (long long)hcnt[2];
-        long long nnz = 2 * (definite + (long long)hcnt[1]);
-        inf.nnz = nnz;
-        inf.s_thresh = (float)(m1 > 0 ? 100.0 * (double)nnz / (double)m1 / 2.0 : 0.0);
+        inf.nnz = 2 * (definite + (long long)hcnt[1]);
+        inf.s_thresh = (float)(m1 > 0 ? 100.0 * (double)inf.nnz / (double)m1 / 2.0 : 0.0);
+        es.seg = seg;
+        es.seg_cap = seg_cap;
+        es.seg_cnt = seg_cnt;
+        es.nseg = (int)nwg;
+    } else if (kwant == 0 || n < 2) {
+        // ---------------------------------------------------- nothing asked
+        PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+        PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
+        PG_HIP(ctx, hipMemsetAsync(seg_cnt, 0, nwg * sizeof(unsigned long long), st));
+        PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+        PG_TRY(pg_bufT(ctx, "corr.extra", 1, &extra));
+        PG_TRY(pg_bufT(ctx, "corr.seg", 1, &seg));
+        es.seg = seg;
+        es.seg_cap = 0;
+        es.seg_cnt = seg_cnt;
+        es.nseg = (int)nwg;
+        inf.nnz = 0;
+        inf.r_thresh = 1.0f;
+        inf.t_star = 1.0;
     } else {
-        // exact k-th largest among the candidates: order (r64 desc, i, j) is
-        // the reference's rule (all r > t*, then ties in row-major order)
-        long long need = kwant - definite;
-        long long nsel = 0;
-        if (kwant > 0) {
-            if (need < 1 || need > nband)
-                return pg_fail(ctx, PG_ETHRESH,
-                               "selection window invalid (k=%lld definite=%lld candidates=%lld)",
-                               kwant, definite, nband);
-            PG_KERNEL_CHECK(ctx);
-            std::vector<int2> hp(nband);
-            std::vector<double> hr(nband);
-            std::vector<float> hf(nband);
-            PG_HIP(ctx, hipMemcpyAsync(hp.data(), bandl, nband * sizeof(int2), hipMemcpyDeviceToHost, st));
-            PG_HIP(ctx, hipMemcpyAsync(hr.data(), band64, nband * 8, hipMemcpyDeviceToHost, st));
-            PG_HIP(ctx, hipMemcpyAsync(hf.data(), band_r, nband * 4, hipMemcpyDeviceToHost, st));
-            PG_HIP(ctx, hipStreamSynchronize(st));
-            std::vector<HostCand> c(nband);
-            for (long long q = 0; q < nband; q++) c[q] = {hr[q], hp[q].x, hp[q].y};
-            auto cmp = [](const HostCand &a, const HostCand &b) {
-                if (a.r != b.r) return a.r > b.r;
-                if (a.i != b.i) return a.i < b.i;
-                return a.j < b.j;
-            };
-            std::nth_element(c.begin(), c.begin() + (need - 1), c.end(), cmp);
-            const HostCand pivot = c[need - 1];
-            std::sort(c.begin(), c.begin() + need, cmp);
-            inf.t_star = pivot.r;
-            std::vector<int2> sel(need);
-            for (long long q = 0; q < need; q++) sel[q] = make_int2(c[q].i, c[q].j);
-            PG_HIP(ctx, hipMemcpyAsync(extra, sel.data(), need * sizeof(int2), hipMemcpyHostToDevice, st));
-            unsigned long long nn = (unsigned long long)need;
-            PG_HIP(ctx, hipMemcpyAsync(extra_cnt, &nn, 8, hipMemcpyHostToDevice, st));
-            // fast-path flips relative to t*: candidates whose r~ ordering
-            // against t* disagrees with r64's
-            long long fl = 0;
-            for (long long q = 0; q < nband; q++) fl += ((double)hf[q] > pivot.r) != (hr[q] > pivot.r);
-            inf.band_flips = fl;
-            nsel = need;
-            PG_HIP(ctx, hipStreamSynchronize(st));
+        // ---------------------------------------------------- sparsity
+        const double w = 2.0 / NBIN;
+        bool done = false;
+        // (1) sampled histogram -> conservative lower bound L on the k-th r~
+        const long long stride = std::max<long long>(1, std::min<long long>(50, ntiles / 512));
+        PG_TRY(hist_pass(stride));
+        unsigned long long msamp = 0;
+        for (auto c : h) msamp += c;
+        const double p = (double)kwant / (double)m1;
+        const double meff = std::max(1.0, (double)msamp / 64.0);  // pairs inside a tile are correlated
+        const double f = std::min(1.0, 1.15 * p + 6.0 * std::sqrt(p * (1.0 - p) / meff) + 1.0 / meff);
+        int bl = -1;
+        {
+            unsigned long long cum = 0;
+            for (int b = NBIN - 1; b >= 0; b--) {
+                cum += h[b];
+                if ((double)cum >= f * (double)msamp) { bl = b; break; }
+            }
         }
-        inf.nnz = 2 * (definite + nsel);
+        PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+        if (stride > 1 && bl > 0 && msamp > 0) {
+            const float lower = round_down_f(bl * w - 1.0 - 1e-6);
+            // (2) one emitting pass: every pair with r~ >= lower, with its r~
+            ga.lower = lower;
+            ga.upper = -4.0f;  // every hit is a segment entry
+            const double expect = std::min(1.0, f * 1.2 + 2.0 * w) * (double)m1;
+            PG_TRY(emit_pass(pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096, 1 << 10, true));
+            long long emitted = 0;
+            for (auto c : hseg) emitted += (long long)c;
+            if (emitted >= kwant) {
+                // (3) fine histogram of the emitted r~ -> bracket the k-th
+                const double top = 1.0 + 1e-6;
+                const float scale = (float)(NBIN2 / (top - (double)lower));
+                const int hb = (int)std::min<long long>(64, nwg);
+                uint32_t *h2;
+                unsigned long long *h2t;
+                PG_TRY(pg_bufT(ctx, "corr.h2", (size_t)hb * NBIN2, &h2));
+                PG_TRY(pg_bufT(ctx, "corr.h2t", NBIN2, &h2t));
+                emitted_hist<<<hb, 256, 0, st>>>(seg_r, seg_cap, seg_cnt, (int)nwg, lower, scale, h2);
+                hist_sum<<<NBIN2 / 256, 256, 0, st>>>(h2, hb, NBIN2, h2t);
+                PG_KERNEL_CHECK(ctx);
+                std::vector<unsigned long long> hh(NBIN2);
+                PG_HIP(ctx, hipMemcpyAsync(hh.data(), h2t, NBIN2 * 8, hipMemcpyDeviceToHost, st));
+                PG_HIP(ctx, hipStreamSynchronize(st));
+                int b2 = -1;
+                unsigned long long cum = 0;
+                for (int b = NBIN2 - 1; b >= 0; b--) {
+                    cum += hh[b];
+                    if ((long long)cum >= kwant) { b2 = b; break; }
+                }
+                const double w2 = 1.0 / (double)scale;
+                const double lo2 = (double)lower + b2 * w2 - 1e-6;
+                const double hi2 = (double)lower + (b2 + 1) * w2 + 1e-6;
+                const float L2 = round_down_f(lo2 - 2.0 * band);
+                const float U2 = round_up_f(hi2 + 2.0 * band);
+                // the window must lie inside the emitted range
+                if (b2 >= 0 && (double)L2 >= (double)lower) {
+                    long long ccap = 1024;
+                    const int blo = std::max(0, (int)(((double)L2 - lower) * scale) - 2);
+                    const int bhi = std::min(NBIN2 - 1, (int)(((double)U2 - lower) * scale) + 2);
+                    for (int b = blo; b <= bhi; b++) ccap += (long long)hh[b];
+                    int2 *cand;
+                    float *cand_r;
+                    double *cand64;
+                    PG_TRY(pg_bufT(ctx, "corr.band", (size_t)ccap, &cand));
+                    PG_TRY(pg_bufT(ctx, "corr.bandr", (size_t)ccap, &cand_r));
+                    PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)ccap, &cand64));
+                    PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)ccap, &extra));
+                    PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
+                    PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+                    window_kernel<<<(unsigned)std::min<long long>(nwg, 1024), 256, 0, st>>>(
+                        seg, seg_r, seg_cap, seg_cnt, (int)nwg, L2, U2, cand, cand_r, ccap, counters + 0,
+                        counters + 3);
+                    recheck_kernel<<<(unsigned)pg_cdiv(ccap, 256), 256, 0, st>>>(cand, counters + 0, ccap, z64, t, cand64);
+                    select_kernel<<<1, 1024, 0, st>>>(cand, cand64, cand_r, counters + 0, ccap, kwant, counters + 3, 1,
+                                                      extra, extra_cnt, tstar_d, flips, (int *)(counters + 4));
+                    PG_KERNEL_CHECK(ctx);
+                    PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
+                    PG_HIP(ctx, hipMemcpyAsync(&inf.t_star, tstar_d, 8, hipMemcpyDeviceToHost, st));
+                    PG_HIP(ctx, hipStreamSynchronize(st));
+                    if (((int *)hcnt)[8] == 0 &&
+                        (long long)hcnt[3] + (long long)hcnt[1] == kwant) {
+                        inf.band_pairs = (long long)hcnt[0];
+                        inf.band_flips = (long long)hcnt[2];
+                        es.seg = seg;
+                        es.seg_cap = seg_cap;
+                        es.seg_cnt = seg_cnt;
+                        es.nseg = (int)nwg;
+                        es.seg_r = seg_r;
+                        es.keep_above = U2;
+                        done = true;
+                    }
+                }
+            }
+        }
+        if (!done) {
+            // (fallback) exact bracket from a histogram of every pair, then an
+            // emitting pass with a definite/candidate split
+            PG_TRY(hist_pass(0));
+            long long above = 0;
+            int bstar = -1;
+            for (int b = NBIN - 1; b >= 0; b--) {
+                if (above + (long long)h[b] >= kwant) { bstar = b; break; }
+                above += (long long)h[b];
+            }
+            if (bstar < 0) return pg_fail(ctx, PG_ETHRESH, "histogram holds fewer than k=%lld pairs", kwant);
+            const double lo_edge = bstar * w - 1.0 - 1e-6;
+            const double hi_edge = (bstar + 1) * w - 1.0 + 1e-6;
+            ga.upper = round_up_f(hi_edge + 2.0 * band);
+            ga.lower = round_down_f(lo_edge - 2.0 * band);
+            PG_TRY(emit_pass(pg_cdiv(kwant, nwg) * 3 / 2 + 4096, 1 << 22, false));
+            PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+            const long long nband = (long long)hcnt[0];
+            double *band64;
+            PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)std::max<long long>(nband, 1), &band64));
+            PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)std::max<long long>(nband, 1), &extra));
+            if (nband > 0)
+                recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(bandl, counters + 0, band_cap, z64, t, band64);
+            select_kernel<<<1, 1024, 0, st>>>(bandl, band64, band_r, counters + 0, band_cap, kwant, seg_cnt, (int)nwg,
+                                              extra, extra_cnt, tstar_d, flips, (int *)(counters + 4));
+            PG_KERNEL_CHECK(ctx);
+            PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipMemcpyAsync(&inf.t_star, tstar_d, 8, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipStreamSynchronize(st));
+            if (((int *)hcnt)[8] != 0)
+                return pg_fail(ctx, PG_ETHRESH, "selection window invalid (k=%lld candidates=%lld)", kwant,
+                               (long long)hcnt[0]);
+            inf.band_pairs = nband;
+            inf.band_flips = (long long)hcnt[2];
+            es.seg = seg;
+            es.seg_cap = seg_cap;
+            es.seg_cnt = seg_cnt;
+            es.nseg = (int)nwg;
+        }
+        inf.nnz = 2 * kwant;
         inf.r_thresh = (float)inf.t_star;
-        if (kwant == 0) inf.r_thresh = 1.0f;
     }
+    es.extra = extra;
+    es.extra_cnt = extra_cnt;
     PG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
     if (inf.nnz > INT32_MAX) return pg_fail(ctx, PG_ENOTSUP, "nnz %lld exceeds int32 CSR", (long long)inf.nnz);
 
-    // ---- CSR build
-    int *deg, *row, *cur, *ctmp, *col;
-    float *vtmp = nullptr, *val = nullptr;
+    // ---- CSR build (bucketed counting sort)
+    const int nbk = (int)pg_cdiv(std::max(n, 1), BROWS);
+    const int nsrc = (int)nwg + 1;  // segments + the accepted/selected list
     const size_t nnzs = (size_t)std::max<long long>(inf.nnz, 1);
-    PG_TRY(pg_bufT(ctx, "corr.deg", (size_t)n + 1, &deg));
+    unsigned *bcnt, *ent;
+    long long *boff, *bstart;
+    int *row, *ctmp, *col;
+    float *entv = nullptr, *vtmp = nullptr, *val = nullptr;
+    PG_TRY(pg_bufT(ctx, "corr.bcnt", (size_t)nsrc * nbk, &bcnt));
+    PG_TRY(pg_bufT(ctx, "corr.boff", (size_t)nsrc * nbk, &boff));
+    PG_TRY(pg_bufT(ctx, "corr.bstart", (size_t)nbk + 1, &bstart));
+    PG_TRY(pg_bufT(ctx, "corr.ent", nnzs, &ent));
     PG_TRY(pg_bufT(ctx, "corr.row", (size_t)n + 2, &row));
-    PG_TRY(pg_bufT(ctx, "corr.cur", (size_t)n + 1, &cur));
     PG_TRY(pg_bufT(ctx, "corr.ctmp", nnzs, &ctmp));
     PG_TRY(pg_bufT(ctx, "corr.col", nnzs, &col));
     if (weighted) {
+        PG_TRY(pg_bufT(ctx, "corr.entv", nnzs, &entv));
         PG_TRY(pg_bufT(ctx, "corr.vtmp", nnzs, &vtmp));
         PG_TRY(pg_bufT(ctx, "corr.val", nnzs, &val));
     }
-    PG_HIP(ctx, hipMemsetAsync(deg, 0, ((size_t)n + 1) * 4, st));
-    EdgeSrc es;
-    es.seg = seg;
-    es.seg_cap = seg_cap;
-    es.seg_cnt = seg_cnt;
-    es.nseg = (int)nwg;
-    es.extra = extra;
-    es.extra_cnt = extra_cnt;
     if (n > 0) {
-        degree_count<<<(unsigned)(nwg + 1), 256, 0, st>>>(es, deg);
-        scan_rows<<<1, 1024, 0, st>>>(deg, n, row, cur);
-        scatter_edges<<<(unsigned)(nwg + 1), 256, 0, st>>>(es, row, cur, ctmp, vtmp, z64, t);
-        size_t shm = (size_t)nw * 8;
-        sort_rows<<<(unsigned)n, 256, shm, st>>>(row, ctmp, vtmp, n, col, val);
+        const size_t shb = (size_t)nbk * 4;
+        bucket_count<<<(unsigned)nsrc, 256, shb, st>>>(es, nbk, bcnt);
+        bucket_scan<<<1, 1024, 0, st>>>(bcnt, nsrc, nbk, boff, bstart);
+        bucket_scatter<<<(unsigned)nsrc, 256, shb, st>>>(es, nbk, boff, ent, entv, z64, t);
+        int nbm = 4;
+        while (nbm > 1 && (size_t)nbm * 8 * nw + 2048 > ctx->lds_per_block) nbm--;
+        const size_t shr = (size_t)nbm * 8 * nw;
+        PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shr));
+        bucket_rows<<<(unsigned)nbk, 256, shr, st>>>(bstart, ent, entv, n, nbk, nbm, row, ctmp, vtmp, col, val);
         PG_KERNEL_CHECK(ctx);
     } else {
         PG_HIP(ctx, hipMemsetAsync(row, 0, 4, st));

@@ -12,7 +12,7 @@ from dataclasses import dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libpagani.so")
+LIB_PATH = os.environ.get("PAGANI_LIB") or os.path.join(os.path.dirname(HERE), "lib", "libpagani.so")
 
 PREC_FP32, PREC_BF16X3 = 0, 1
 THR_R, THR_SPARSITY = 0, 1

@@ -74,6 +74,21 @@ def test_smode_bit_exact(ctx, prec, weighted, s):
 
 
 @pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("s", [1.0, 3.0])
+def test_smode_single_pass_large(ctx, prec, s):
+    """N large enough for the sampled single-pass sparsity path."""
+    x = _data(6000, 100, 12, 13)
+    z64, _ = O.zscore(x)
+    say(f"smode large prec={prec} s={s}")
+    info, row, col, _ = _build(ctx, x, s, sparsity=True, prec=prec)
+    orow, ocol, _, tstar = O.corr_csr_s(z64, s)
+    np.testing.assert_array_equal(row, orow)
+    np.testing.assert_array_equal(col, ocol)
+    assert info.t_star == tstar
+    assert info.gram_passes == 1  # no fallback to the two-pass path
+
+
+@pytest.mark.parametrize("prec", PRECS)
 def test_smode_ties_row_major(ctx, prec):
     # duplicated voxels give bit-identical r against every other voxel: the
     # k-th largest value is tied many times and the earliest ties must be kept

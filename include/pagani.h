@@ -164,6 +164,14 @@ pg_status pg_fw(pg_ctx *ctx, float *d, int32_t n, int32_t block);
 pg_status pg_fw_init(pg_ctx *ctx, const int32_t *row, const int32_t *col, const float *val,
                      int32_t n, int64_t nnz, int32_t kind, float *d);
 
+/* Nodal efficiency and characteristic path length from a closed distance
+ * matrix (W/Lp/CUBFW_Lp/CUBFW_Lp/CUBFW_Lp.cpp:70-113): for u != v with
+ * d_uv >= 0.1, e_u = sum of 1/d_uv (FP32) / (n-1), set to 0 when below 1/n;
+ * Lp = n (n-1) / sum of 1/d_uv (FP64 sum).  eff (n floats) and lp may be NULL;
+ * isolated (may be NULL) counts the zeroed e_u. */
+pg_status pg_fw_efficiency(pg_ctx *ctx, const float *d, int32_t n, float *eff, double *lp,
+                           int32_t *isolated);
+
 typedef enum pg_mod_kmode {
     PG_MOD_STRENGTH = 0,  /* k = strength, 2m = sum of weights (binary if val NULL) */
     PG_MOD_BINARY_K = 1   /* k = degree, 2m = nnz, A weighted: the reference's tool */

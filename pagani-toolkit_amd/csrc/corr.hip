@@ -24,12 +24,20 @@
 
 namespace {
 
-constexpr int TILE = 128;      // output tile edge (rows and columns)
-#ifndef PG_GRAM_KC
-#define PG_GRAM_KC 64
+#ifndef PG_GRAM_TM
+#define PG_GRAM_TM 256
 #endif
+#ifndef PG_GRAM_KC
+#define PG_GRAM_KC 32
+#endif
+constexpr int TM = PG_GRAM_TM;  // Gram tile rows (128 or 256)
+constexpr int TN = 128;         // Gram tile columns
+constexpr int RT = TM / TN;     // tile aspect ratio
+constexpr int WR = TM / 2;      // rows per wave (2 x 2 waves)
+constexpr int WC = TN / 2;      // columns per wave
+constexpr int BMW = WR / 32, BNW = WC / 32;  // 32 x 32 MFMA blocks per wave
 constexpr int KC = PG_GRAM_KC; // K (time) chunk staged in LDS
-constexpr int THREADS = 256;   // 4 wave64s, each owns a 64 x 64 quarter
+constexpr int THREADS = 256;   // 4 wave64s in a 2 x 2 arrangement
 constexpr int NBIN = 8192;     // r histogram bins over [-1, 1] (sparsity mode)
 constexpr int LDB = KC + 8;    // bf16 LDS row stride (elements), 16-byte aligned rows
 constexpr int LDF = KC + 1;    // fp32 LDS row stride (elements): conflict-free column reads
@@ -47,9 +55,10 @@ struct GramArgs {
     int ldp;               // panel row stride = padded T
     int kchunks;           // ldp / KC
     int n;                 // voxels
-    int nb;                // row blocks = npad / TILE
-    long long ntiles;      // nb (nb + 1) / 2 upper-triangle tiles
-    long long per;         // tiles per work-group
+    int nr, nc;            // row tiles (npad / TM) and column tiles (npad / TN)
+    const int2 *st_list;   // supertiles (SR, SC), host-built
+    long long nst;         // supertile count
+    long long sstep;       // > 0: visit every sstep-th supertile only (histogram sample)
     float upper;           // r~ > upper: edge without recheck
     float lower;           // lower <= r~ <= upper: FP64 recheck
     int2 *seg;             // [gridDim.x][seg_cap] private edge segments
@@ -61,7 +70,6 @@ struct GramArgs {
     unsigned long long *band_cnt;  // [1]
     uint32_t *hist;        // [gridDim.x][NBIN]
     float *seg_r;          // optional: r~ of each segment entry
-    long long sample_stride;  // > 0: visit only tiles t = k * stride (histogram sample)
 };
 
 __device__ __forceinline__ uint16_t f2bf(float f)
@@ -139,91 +147,60 @@ __global__ void zscore_write(const float *__restrict__ x, int layout, int n, int
 
 // ------------------------------------------------------------ Gram tiles
 //
+// Tiles are TM x TN (rows x columns) of the N x N Gram; a tile (R, C) covers
+// rows [R*TM, R*TM+TM) and columns [C*TN, C*TN+TN) and is needed when it holds
+// some i < j, i.e. C >= RT*R.  Four waves split a tile 2 x 2; each wave owns a
+// (TM/2) x (TN/2) block of 32 x 32 MFMA tiles.
+//
 // Schedule.  Work-groups are dispatched round-robin over the 8 XCDs, so
-// work-group g runs on XCD g % 8.  All work-groups of one XCD walk the same
-// 8 x 8 supertile of 128 x 128 tiles at a time: the 16 panels it touches (8
-// row blocks + 8 column blocks; 2 MB for BF16x3 at T = 256) stay in that XCD's
-// L2 while its CUs consume them, instead of every work-group streaming its own
-// panels from HBM / MALL.
+// work-group g runs on XCD g % 8.  Tiles are grouped in 8 x 8 supertiles
+// (listed by the host); the work-groups of one XCD walk the same supertile
+// together, so the panels it touches (8 row panels + 8 column panels) stay in
+// that XCD's L2 instead of each work-group streaming its own from HBM/MALL.
 constexpr int SUPER = 8;
+constexpr int NXCD = 8;
 #ifndef PG_GRAM_WG_PER_CU
 #define PG_GRAM_WG_PER_CU 2
 #endif
-constexpr int NXCD = 8;
-
-// Upper-triangle index -> (r, c), r <= c, row-major over r, for an m x m grid.
-__device__ __forceinline__ void tri_coords(long long tid, int m, int &r, int &c)
-{
-    // start(r) = r*m - r*(r-1)/2; solve start(r) <= tid < start(r+1)
-    double b = 2.0 * m + 1.0;
-    int x = (int)floor((b - sqrt(b * b - 8.0 * (double)tid)) * 0.5);
-    if (x < 0) x = 0;
-    if (x > m - 1) x = m - 1;
-    while (x > 0 && (long long)x * m - (long long)x * (x - 1) / 2 > tid) x--;
-    while (x < m - 1 && (long long)(x + 1) * m - (long long)(x + 1) * x / 2 <= tid) x++;
-    r = x;
-    c = x + (int)(tid - ((long long)x * m - (long long)x * (x - 1) / 2));
-}
 
 struct TileWalk {
-    int nb, nsb;        // tile blocks, supertile blocks per side
-    long long nst;      // supertiles in the upper triangle
-    int L;              // work-groups sharing this XCD's tile sequence
-    long long k;        // next position in this XCD's tile sequence
-    long long kend;     // positions available to this XCD
-    int xcd;
+    const int2 *st;     // supertile list (SR, SC)
+    long long nst;      // its length
+    long long sstep;    // visit list entries 0, sstep, 2 sstep, ... (sampling)
+    int nr, nc;         // row and column tiles
+    int L, xcd;
+    long long k, kend;  // position in this XCD's tile sequence
 
-    // This XCD's sequence is the concatenation of its supertiles (xcd,
-    // xcd + 8, ...), SUPER*SUPER slots each; work-group slot l takes
-    // positions l, l + L, l + 2L, ... so the XCD's work-groups move through
-    // the same supertile together.
-    long long stride, ntiles;
-
-    __device__ void init(int nb_, long long stride_)
+    // This XCD's sequence is the concatenation of its supertiles
+    // (xcd + 8 m) * sstep, SUPER*SUPER slots each; work-group slot l takes
+    // positions l, l + L, ... so the XCD's work-groups move together.
+    __device__ void init(const GramArgs &a)
     {
-        nb = nb_;
-        nsb = (nb + SUPER - 1) / SUPER;
-        nst = (long long)nsb * (nsb + 1) / 2;
+        st = a.st_list;
+        nst = a.nst;
+        sstep = a.sstep > 0 ? a.sstep : 1;
+        nr = a.nr;
+        nc = a.nc;
         const int g = blockIdx.x, G = gridDim.x;
-        stride = stride_;
-        ntiles = (long long)nb * (nb + 1) / 2;
-        if (stride > 0) {  // sampled tiles, strided over the row-major upper triangle
-            k = g;
-            L = G;
-            kend = (ntiles + stride - 1) / stride;
-            xcd = 0;
-            return;
-        }
         xcd = g % NXCD;
         L = G / NXCD;
         k = g / NXCD;
-        const long long mine = nst > xcd ? (nst - 1 - xcd) / NXCD + 1 : 0;
+        const long long mx = nst > 0 ? (nst - 1) / sstep : -1;
+        const long long mine = mx >= xcd ? (mx - xcd) / NXCD + 1 : 0;
         kend = mine * SUPER * SUPER;
     }
-    __device__ bool next(int &bi, int &bj)
+    __device__ bool next(int &R, int &C)
     {
-        if (stride > 0) {
-            while (k < kend) {
-                const long long t = k * stride + stride / 2;
-                k += L;
-                if (t < ntiles) {
-                    tri_coords(t, nb, bi, bj);
-                    return true;
-                }
-            }
-            return false;
-        }
         while (k < kend) {
             const long long kk = k;
             k += L;
-            const long long s = xcd + (kk / (SUPER * SUPER)) * NXCD;
+            const long long s = (xcd + (kk / (SUPER * SUPER)) * NXCD) * sstep;
             const int q = (int)(kk % (SUPER * SUPER));
-            int sr, sc;
-            tri_coords(s, nsb, sr, sc);
-            const int ti = sr * SUPER + q / SUPER, tj = sc * SUPER + q % SUPER;
-            if (ti <= tj && tj < nb) {
-                bi = ti;
-                bj = tj;
+            const int2 sc = st[s];
+            const int r = sc.x * SUPER + q / SUPER, c = sc.y * SUPER + q % SUPER;
+            if (r < nr && c < nc && c >= RT * r) {
+                R = r;
+                C = c;
                 return true;
             }
         }
@@ -231,117 +208,149 @@ struct TileWalk {
     }
 };
 
-// Global -> register staging of one K chunk of the row and column panels.
+// Global -> register staging of one K chunk of the row (TM) and column (TN) panels.
 template <int PREC>
 struct Stage {
-    static constexpr int VEC = PREC == PREC_BF16X3 ? 8 : 4;         // elements per 16 B
-    static constexpr int PIECES = TILE * KC / VEC;                  // per panel array
-    static constexpr int ITERS = PIECES / THREADS;
-    static constexpr int NARR = PREC == PREC_BF16X3 ? 2 : 1;        // hi/lo or f32
-    u32x4 a[NARR][ITERS], b[NARR][ITERS];
+    static constexpr int VEC = PREC == PREC_BF16X3 ? 8 : 4;   // elements per 16 B
+    static constexpr int IA = TM * KC / VEC / THREADS;        // 16-B pieces per thread, row panel
+    static constexpr int IB = TN * KC / VEC / THREADS;        // column panel
+    static constexpr int NARR = PREC == PREC_BF16X3 ? 2 : 1;  // hi/lo or f32
+    u32x4 a[NARR][IA], b[NARR][IB];
 
-    __device__ __forceinline__ void load(const GramArgs &g, int bi, int bj, int kc)
+    __device__ __forceinline__ static void piece(int it, int &r, int &c)
     {
-        const int tid = threadIdx.x;
+        const int id = it * THREADS + (int)threadIdx.x;
+        r = id / (KC / VEC);
+        c = (id % (KC / VEC)) * VEC;
+    }
+    __device__ __forceinline__ void load(const GramArgs &g, int R, int C, int kc)
+    {
 #pragma unroll
-        for (int it = 0; it < ITERS; it++) {
-            const int id = it * THREADS + tid;
-            const int r = id / (KC / VEC), c = (id % (KC / VEC)) * VEC;
-            const long long ga = (long long)(bi * TILE + r) * g.ldp + kc * KC + c;
-            const long long gb = (long long)(bj * TILE + r) * g.ldp + kc * KC + c;
+        for (int it = 0; it < IA; it++) {
+            int r, c;
+            piece(it, r, c);
+            const long long ga = (long long)(R * TM + r) * g.ldp + kc * KC + c;
             if (PREC == PREC_BF16X3) {
                 a[0][it] = *(const u32x4 *)((const uint16_t *)g.p0 + ga);
                 a[NARR - 1][it] = *(const u32x4 *)((const uint16_t *)g.p1 + ga);
+            } else {
+                a[0][it] = *(const u32x4 *)((const float *)g.p0 + ga);
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IB; it++) {
+            int r, c;
+            piece(it, r, c);
+            const long long gb = (long long)(C * TN + r) * g.ldp + kc * KC + c;
+            if (PREC == PREC_BF16X3) {
                 b[0][it] = *(const u32x4 *)((const uint16_t *)g.p0 + gb);
                 b[NARR - 1][it] = *(const u32x4 *)((const uint16_t *)g.p1 + gb);
             } else {
-                a[0][it] = *(const u32x4 *)((const float *)g.p0 + ga);
                 b[0][it] = *(const u32x4 *)((const float *)g.p0 + gb);
             }
         }
     }
     __device__ __forceinline__ void store(char *smem) const
     {
-        const int tid = threadIdx.x;
+        if (PREC == PREC_BF16X3) {
+            uint16_t *sAh = (uint16_t *)smem;
+            uint16_t *sAl = sAh + TM * LDB;
+            uint16_t *sBh = sAl + TM * LDB;
+            uint16_t *sBl = sBh + TN * LDB;
 #pragma unroll
-        for (int it = 0; it < ITERS; it++) {
-            const int id = it * THREADS + tid;
-            const int r = id / (KC / VEC), c = (id % (KC / VEC)) * VEC;
-            if (PREC == PREC_BF16X3) {
-                uint16_t *s = (uint16_t *)smem;
-                *(u32x4 *)(s + 0 * TILE * LDB + r * LDB + c) = a[0][it];
-                *(u32x4 *)(s + 1 * TILE * LDB + r * LDB + c) = a[NARR - 1][it];
-                *(u32x4 *)(s + 2 * TILE * LDB + r * LDB + c) = b[0][it];
-                *(u32x4 *)(s + 3 * TILE * LDB + r * LDB + c) = b[NARR - 1][it];
-            } else {
-                // odd row stride: 4-byte stores
-                float *s = (float *)smem;
+            for (int it = 0; it < IA; it++) {
+                int r, c;
+                piece(it, r, c);
+                *(u32x4 *)(sAh + r * LDB + c) = a[0][it];
+                *(u32x4 *)(sAl + r * LDB + c) = a[NARR - 1][it];
+            }
 #pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    s[r * LDF + c + e] = __uint_as_float(a[0][it][e]);
-                    s[TILE * LDF + r * LDF + c + e] = __uint_as_float(b[0][it][e]);
-                }
+            for (int it = 0; it < IB; it++) {
+                int r, c;
+                piece(it, r, c);
+                *(u32x4 *)(sBh + r * LDB + c) = b[0][it];
+                *(u32x4 *)(sBl + r * LDB + c) = b[NARR - 1][it];
+            }
+        } else {
+            // odd row stride (conflict-free column reads): 4-byte stores
+            float *sA = (float *)smem;
+            float *sB = sA + TM * LDF;
+#pragma unroll
+            for (int it = 0; it < IA; it++) {
+                int r, c;
+                piece(it, r, c);
+#pragma unroll
+                for (int e = 0; e < 4; e++) sA[r * LDF + c + e] = __uint_as_float(a[0][it][e]);
+            }
+#pragma unroll
+            for (int it = 0; it < IB; it++) {
+                int r, c;
+                piece(it, r, c);
+#pragma unroll
+                for (int e = 0; e < 4; e++) sB[r * LDF + c + e] = __uint_as_float(b[0][it][e]);
             }
         }
     }
 };
 
 template <int PREC>
-__device__ __forceinline__ void mfma_chunk(const char *smem, f32x16 (&acc)[2][2])
+__device__ __forceinline__ void mfma_chunk(const char *smem, f32x16 (&acc)[BMW][BNW])
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wm = w >> 1, wn = w & 1, lr = lane & 31;
     if (PREC == PREC_BF16X3) {
         const uint16_t *sAh = (const uint16_t *)smem;
-        const uint16_t *sAl = sAh + TILE * LDB;
-        const uint16_t *sBh = sAl + TILE * LDB;
-        const uint16_t *sBl = sBh + TILE * LDB;
+        const uint16_t *sAl = sAh + TM * LDB;
+        const uint16_t *sBh = sAl + TM * LDB;
+        const uint16_t *sBl = sBh + TN * LDB;
         const int lk = (lane >> 5) * 8;
 #pragma unroll
         for (int ks = 0; ks < KC / 16; ks++) {
-            bf16x8 ah[2], al[2], bh[2], bl[2];
+            bf16x8 ah[BMW], al[BMW], bh[BNW], bl[BNW];
 #pragma unroll
-            for (int b = 0; b < 2; b++) {
-                int ra = (wm * 64 + b * 32 + lr) * LDB + ks * 16 + lk;
-                int rb = (wn * 64 + b * 32 + lr) * LDB + ks * 16 + lk;
+            for (int b = 0; b < BMW; b++) {
+                const int ra = (wm * WR + b * 32 + lr) * LDB + ks * 16 + lk;
                 ah[b] = *(const bf16x8 *)(sAh + ra);
                 al[b] = *(const bf16x8 *)(sAl + ra);
+            }
+#pragma unroll
+            for (int b = 0; b < BNW; b++) {
+                const int rb = (wn * WC + b * 32 + lr) * LDB + ks * 16 + lk;
                 bh[b] = *(const bf16x8 *)(sBh + rb);
                 bl[b] = *(const bf16x8 *)(sBl + rb);
             }
-            // small cross terms first, independent accumulators interleaved
+            // small cross terms first; independent accumulators interleaved
 #pragma unroll
-            for (int bm = 0; bm < 2; bm++)
+            for (int bm = 0; bm < BMW; bm++)
 #pragma unroll
-                for (int bn = 0; bn < 2; bn++)
+                for (int bn = 0; bn < BNW; bn++)
                     acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[bm], bh[bn], acc[bm][bn], 0, 0, 0);
 #pragma unroll
-            for (int bm = 0; bm < 2; bm++)
+            for (int bm = 0; bm < BMW; bm++)
 #pragma unroll
-                for (int bn = 0; bn < 2; bn++)
+                for (int bn = 0; bn < BNW; bn++)
                     acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[bm], bl[bn], acc[bm][bn], 0, 0, 0);
 #pragma unroll
-            for (int bm = 0; bm < 2; bm++)
+            for (int bm = 0; bm < BMW; bm++)
 #pragma unroll
-                for (int bn = 0; bn < 2; bn++)
+                for (int bn = 0; bn < BNW; bn++)
                     acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[bm], bh[bn], acc[bm][bn], 0, 0, 0);
         }
     } else {
         const float *sA = (const float *)smem;
-        const float *sB = sA + TILE * LDF;
+        const float *sB = sA + TM * LDF;
         const int lk = lane >> 5;
 #pragma unroll
         for (int ks = 0; ks < KC / 2; ks++) {
-            float av[2], bv[2];
+            float av[BMW], bv[BNW];
 #pragma unroll
-            for (int b = 0; b < 2; b++) {
-                av[b] = sA[(wm * 64 + b * 32 + lr) * LDF + ks * 2 + lk];
-                bv[b] = sB[(wn * 64 + b * 32 + lr) * LDF + ks * 2 + lk];
-            }
+            for (int b = 0; b < BMW; b++) av[b] = sA[(wm * WR + b * 32 + lr) * LDF + ks * 2 + lk];
 #pragma unroll
-            for (int bm = 0; bm < 2; bm++)
+            for (int b = 0; b < BNW; b++) bv[b] = sB[(wn * WC + b * 32 + lr) * LDF + ks * 2 + lk];
 #pragma unroll
-                for (int bn = 0; bn < 2; bn++)
+            for (int bm = 0; bm < BMW; bm++)
+#pragma unroll
+                for (int bn = 0; bn < BNW; bn++)
                     acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[bm], bv[bn], acc[bm][bn], 0, 0, 0);
         }
     }
@@ -352,14 +361,19 @@ __device__ __forceinline__ unsigned lane_rank(unsigned long long m)
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
+template <int PREC>
+constexpr int stage_bytes()
+{
+    return PREC == PREC_BF16X3 ? (2 * TM + 2 * TN) * LDB * 2 : (TM + TN) * LDF * 4;
+}
+
 // Persistent: each work-group walks its tiles (TileWalk), prefetching the next
 // K chunk into registers while the MFMAs consume the current one from LDS.
 template <int PREC, int MODE>
 __global__ __launch_bounds__(THREADS, PG_GRAM_WG_PER_CU) void gram_kernel(GramArgs a)
 {
     extern __shared__ __align__(16) char smem[];
-    constexpr int STAGE_BYTES = PREC == PREC_BF16X3 ? 4 * TILE * LDB * 2 : 2 * TILE * LDF * 4;
-    uint32_t *s_hist = (uint32_t *)(smem + STAGE_BYTES);
+    uint32_t *s_hist = (uint32_t *)(smem + stage_bytes<PREC>());
     __shared__ unsigned long long s_seg;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -369,70 +383,69 @@ __global__ __launch_bounds__(THREADS, PG_GRAM_WG_PER_CU) void gram_kernel(GramAr
         for (int b = tid; b < NBIN; b += THREADS) s_hist[b] = 0;
 
     TileWalk walk;
-    walk.init(a.nb, a.sample_stride);
+    walk.init(a);
     int2 *myseg = a.seg + (long long)blockIdx.x * a.seg_cap;
-    int bi = 0, bj = 0;
-    bool have = walk.next(bi, bj);
+    int R = 0, C = 0;
+    bool have = walk.next(R, C);
     Stage<PREC> pre;
-    if (have) pre.load(a, bi, bj, 0);
+    if (have) pre.load(a, R, C, 0);
 
     while (have) {
-        f32x16 acc[2][2];
+        f32x16 acc[BMW][BNW];
 #pragma unroll
-        for (int bm = 0; bm < 2; bm++)
+        for (int bm = 0; bm < BMW; bm++)
 #pragma unroll
-            for (int bn = 0; bn < 2; bn++)
+            for (int bn = 0; bn < BNW; bn++)
 #pragma unroll
                 for (int q = 0; q < 16; q++) acc[bm][bn][q] = 0.0f;
-        int nbi = 0, nbj = 0;
+        int nR = 0, nC = 0;
         bool nhave = false;
         for (int kc = 0; kc < a.kchunks; kc++) {
             __syncthreads();
             pre.store(smem);
             __syncthreads();
-            if (kc + 1 < a.kchunks) {
-                pre.load(a, bi, bj, kc + 1);
-            } else {
-                nhave = walk.next(nbi, nbj);
-                if (nhave) pre.load(a, nbi, nbj, 0);
-            }
+            if (kc + 1 < a.kchunks) pre.load(a, R, C, kc + 1);
             mfma_chunk<PREC>(smem, acc);
         }
+        // next tile's first chunk: issued here (not during the last chunk)
+        // so the staging registers are not live across the epilogue
+        nhave = walk.next(nR, nC);
+        if (nhave) pre.load(a, nR, nC, 0);
 
         // epilogue: 32x32 block layout, lane holds column lane%32 and rows
         // (q/4)*8 + (lane/32)*4 + q%4 of each block.
-        const bool interior = bi < bj && (bj + 1) * TILE <= a.n;
+        const int i0 = R * TM + wm * WR, j0 = C * TN + wn * WC;
+        const bool interior = C * TN >= R * TM + TM && (C + 1) * TN <= a.n;
+        if (MODE == MODE_HIST) {
 #pragma unroll
-        for (int bm = 0; bm < 2; bm++)
+            for (int bm = 0; bm < BMW; bm++)
 #pragma unroll
-            for (int bn = 0; bn < 2; bn++)
+                for (int bn = 0; bn < BNW; bn++)
 #pragma unroll
-                for (int q = 0; q < 16; q++) {
-                    const float v = acc[bm][bn][q];
-                    const int i = bi * TILE + wm * 64 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
-                    const int j = bj * TILE + wn * 64 + bn * 32 + (lane & 31);
-                    if (MODE == MODE_HIST) {
+                    for (int q = 0; q < 16; q++) {
+                        const float v = acc[bm][bn][q];
+                        const int i = i0 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
+                        const int j = j0 + bn * 32 + (lane & 31);
                         if (interior || ((j < a.n) && (i < j))) {
                             int b = (int)((v + 1.0f) * (NBIN * 0.5f));
                             b = b < 0 ? 0 : (b >= NBIN ? NBIN - 1 : b);
                             atomicAdd(&s_hist[b], 1u);
                         }
                     }
-                }
-        if (MODE == MODE_EMIT) {
+        } else {
             // pass 1: count edges and rechecks with one compare per value and
             // scalar popcounts of the ballots; pass 2 writes them after one
-            // LDS reservation per wave and tile.
+            // reservation per wave and tile.
             unsigned ne = 0, nc = 0;
 #pragma unroll
-            for (int bm = 0; bm < 2; bm++)
+            for (int bm = 0; bm < BMW; bm++)
 #pragma unroll
-                for (int bn = 0; bn < 2; bn++)
+                for (int bn = 0; bn < BNW; bn++)
 #pragma unroll
                     for (int q = 0; q < 16; q++) {
                         const float v = acc[bm][bn][q];
-                        const int i = bi * TILE + wm * 64 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
-                        const int j = bj * TILE + wn * 64 + bn * 32 + (lane & 31);
+                        const int i = i0 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
+                        const int j = j0 + bn * 32 + (lane & 31);
                         bool hit = v >= a.lower;
                         if (!interior) hit = hit && (j < a.n) && (i < j);
                         const unsigned long long mh = __ballot(hit);
@@ -450,20 +463,24 @@ __global__ __launch_bounds__(THREADS, PG_GRAM_WG_PER_CU) void gram_kernel(GramAr
                 }
                 be = __shfl(be, 0);
                 bc = __shfl(bc, 0);
+                // opaque copies: pass 2 recomputes its ballots instead of the
+                // compiler holding all of pass 1's live (register pressure)
+                float lo = a.lower, up = a.upper;
+                asm volatile("" : "+s"(lo), "+s"(up));
 #pragma unroll
-                for (int bm = 0; bm < 2; bm++)
+                for (int bm = 0; bm < BMW; bm++)
 #pragma unroll
-                    for (int bn = 0; bn < 2; bn++)
+                    for (int bn = 0; bn < BNW; bn++)
 #pragma unroll
                         for (int q = 0; q < 16; q++) {
                             const float v = acc[bm][bn][q];
-                            const int i = bi * TILE + wm * 64 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
-                            const int j = bj * TILE + wn * 64 + bn * 32 + (lane & 31);
-                            bool hit = v >= a.lower;
+                            const int i = i0 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
+                            const int j = j0 + bn * 32 + (lane & 31);
+                            bool hit = v >= lo;
                             if (!interior) hit = hit && (j < a.n) && (i < j);
                             const unsigned long long mh = __ballot(hit);
                             if (mh == 0ull) continue;
-                            const bool e = hit && v > a.upper;
+                            const bool e = hit && v > up;
                             const unsigned long long me = __ballot(e);
                             const unsigned long long mc = mh & ~me;
                             if (e) {
@@ -484,8 +501,8 @@ __global__ __launch_bounds__(THREADS, PG_GRAM_WG_PER_CU) void gram_kernel(GramAr
                         }
             }
         }
-        bi = nbi;
-        bj = nbj;
+        R = nR;
+        C = nC;
         have = nhave;
     }
     __syncthreads();
@@ -1005,8 +1022,9 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     }
 
     // ---- z-score and MFMA panels
-    const int nb = (int)pg_cdiv(std::max(n, 1), TILE);
-    const int npad = nb * TILE;
+    const int nr = (int)pg_cdiv(std::max(n, 1), TM);  // row tiles
+    const int npad = nr * TM;
+    const int nc = npad / TN;                           // column tiles
     const int ldp = (int)pg_cdiv(t, KC) * KC;
     const long long nt = (long long)t;
     double *mu, *sd, *z64;
@@ -1040,7 +1058,18 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
 
     // ---- Gram launch geometry: PG_GRAM_WG_PER_CU work-groups per CU, a
     // multiple of the XCD count (see TileWalk)
-    const long long ntiles = (long long)nb * (nb + 1) / 2;
+    // supertiles (SR, SC) holding some needed tile (C >= RT*R), row-major
+    std::vector<int2> stl;
+    {
+        const int nsr = (int)pg_cdiv(nr, SUPER), nsc = (int)pg_cdiv(nc, SUPER);
+        for (int sr = 0; sr < nsr; sr++)
+            for (int sc = std::max(0, RT * sr * SUPER / SUPER); sc < nsc; sc++)
+                if (sc * SUPER + SUPER - 1 >= RT * sr * SUPER) stl.push_back(make_int2(sr, sc));
+    }
+    int2 *dstl;
+    PG_TRY(pg_bufT(ctx, "corr.stl", stl.size() + 1, &dstl));
+    PG_HIP(ctx, hipMemcpyAsync(dstl, stl.data(), stl.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+    const long long nst = (long long)stl.size();
     long long nwg = (long long)ctx->num_cu * PG_GRAM_WG_PER_CU;
     nwg = std::max<long long>(NXCD, nwg / NXCD * NXCD);
     const double band = error_band(prec, t);
@@ -1053,9 +1082,12 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     ga.ldp = ldp;
     ga.kchunks = ldp / KC;
     ga.n = n;
-    ga.nb = nb;
-    ga.ntiles = ntiles;
-    const size_t stage = prec == PREC_BF16X3 ? 4 * TILE * LDB * 2 : 2 * TILE * LDF * 4;
+    ga.nr = nr;
+    ga.nc = nc;
+    ga.st_list = dstl;
+    ga.nst = nst;
+    ga.sstep = 0;
+    const size_t stage = prec == PREC_BF16X3 ? (size_t)stage_bytes<PREC_BF16X3>() : (size_t)stage_bytes<PREC_F32>();
 
     bool ran_hist = false, ran_emit = false;
     auto launch_gram = [&](int mode) -> pg_status {
@@ -1081,7 +1113,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         PG_HIP(ctx, hipEventRecord(ctx->ev[e0 + 1], st));
         if (mode == MODE_HIST) ran_hist = true;
         else ran_emit = true;
-        if (ga.sample_stride == 0) inf.gram_passes++;
+        if (ga.sstep <= 1) inf.gram_passes++;
         return PG_OK;
     };
 
@@ -1119,7 +1151,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             ga.band_r = band_r;
             ga.band_cap = band_cap;
             ga.band_cnt = counters + 0;
-            ga.sample_stride = 0;
+            ga.sstep = 0;
             if (n > 1) PG_TRY(launch_gram(MODE_EMIT));
             PG_HIP(ctx, hipMemcpyAsync(hseg.data(), seg_cnt, nwg * 8, hipMemcpyDeviceToHost, st));
             PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
@@ -1142,9 +1174,9 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         PG_TRY(pg_bufT(ctx, "corr.hist", (size_t)nwg * NBIN, &hist));
         PG_TRY(pg_bufT(ctx, "corr.htot", NBIN, &htot));
         ga.hist = hist;
-        ga.sample_stride = stride;
+        ga.sstep = stride;
         PG_TRY(launch_gram(MODE_HIST));
-        ga.sample_stride = 0;
+        ga.sstep = 0;
         hist_reduce<<<NBIN / 256, 256, 0, st>>>(hist, (int)nwg, htot);
         PG_KERNEL_CHECK(ctx);
         PG_HIP(ctx, hipMemcpyAsync(h.data(), htot, NBIN * 8, hipMemcpyDeviceToHost, st));
@@ -1208,7 +1240,8 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         const double w = 2.0 / NBIN;
         bool done = false;
         // (1) sampled histogram -> conservative lower bound L on the k-th r~
-        const long long stride = std::max<long long>(1, std::min<long long>(50, ntiles / 512));
+        // about 2% of the supertiles, when there are enough to sample
+        const long long stride = nst >= 8 ? std::max<long long>(2, std::min<long long>(50, nst / 16)) : 1;
         PG_TRY(hist_pass(stride));
         unsigned long long msamp = 0;
         for (auto c : h) msamp += c;

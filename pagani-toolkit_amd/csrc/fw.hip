@@ -153,6 +153,57 @@ __global__ void fw_from_csr(const int *__restrict__ row, const int *__restrict__
     d[(long long)i * n + i] = 0.0f;
 }
 
+// One block per row u: e_u and the FP64 row sum for Lp (CUBFW_Lp.cpp:70-113).
+__global__ __launch_bounds__(256) void fw_eff_rows(const float *__restrict__ d, int n, float *__restrict__ eff,
+                                                  double *__restrict__ rowsum, int *__restrict__ iso)
+{
+    __shared__ float rf[256];
+    __shared__ double rd[256];
+    const int u = blockIdx.x, tid = threadIdx.x;
+    float sf = 0.0f;
+    double sd = 0.0;
+    for (int v = tid; v < n; v += 256) {
+        const float x = d[(long long)u * n + v];
+        if (v == u || !(x >= 0.1f)) continue;  // the reference skips d < 0.1
+        const float q = 1.0f / x;
+        sf += q;
+        sd += (double)q;
+    }
+    rf[tid] = sf;
+    rd[tid] = sd;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) {
+            rf[tid] += rf[tid + o];
+            rd[tid] += rd[tid + o];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        float e = n > 1 ? rf[0] / (float)(n - 1) : 0.0f;
+        if (e < 1.0f / (float)n) {
+            e = 0.0f;
+            atomicAdd(iso, 1);
+        }
+        if (eff) eff[u] = e;
+        rowsum[u] = rd[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void fw_lp_sum(const double *__restrict__ rowsum, int n, double *__restrict__ out)
+{
+    __shared__ double r[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += rowsum[i];
+    r[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) r[threadIdx.x] += r[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = r[0];
+}
+
 template <int B>
 pg_status fw_run(pg_ctx *ctx, float *dm, int np)
 {
@@ -228,5 +279,41 @@ extern "C" pg_status pg_fw_init(pg_ctx *ctx, const int32_t *row, const int32_t *
     PG_KERNEL_CHECK(ctx);
     PG_TRY(pg_out_finish(ctx, d, dd, (size_t)nn));
     PG_HIP(ctx, hipStreamSynchronize(st));
+    return PG_OK;
+}
+
+extern "C" pg_status pg_fw_efficiency(pg_ctx *ctx, const float *d, int32_t n, float *eff, double *lp,
+                                      int32_t *isolated)
+{
+    if (!ctx) return PG_EINVAL;
+    if (!d || n < 1) return pg_fail(ctx, PG_EINVAL, "bad arguments");
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const float *dd;
+    PG_TRY(pg_in(ctx, d, (size_t)n * n, "fw.in", &dd));
+    float *de;
+    double *rs, *tot;
+    int *iso;
+    PG_TRY(pg_out(ctx, eff, (size_t)n, "out.eff", &de));
+    PG_TRY(pg_bufT(ctx, "fw.rowsum", (size_t)n, &rs));
+    PG_TRY(pg_bufT(ctx, "fw.lpsum", 1, &tot));
+    PG_TRY(pg_bufT(ctx, "fw.iso", 1, &iso));
+    PG_HIP(ctx, hipMemsetAsync(iso, 0, 4, st));
+    fw_eff_rows<<<(unsigned)n, 256, 0, st>>>(dd, n, de, rs, iso);
+    fw_lp_sum<<<1, 256, 0, st>>>(rs, n, tot);
+    PG_KERNEL_CHECK(ctx);
+    PG_TRY(pg_out_finish(ctx, eff, de, (size_t)n));
+    double s = 0.0;
+    int ni = 0;
+    PG_HIP(ctx, hipMemcpyAsync(&s, tot, 8, hipMemcpyDeviceToHost, st));
+    PG_HIP(ctx, hipMemcpyAsync(&ni, iso, 4, hipMemcpyDeviceToHost, st));
+    PG_HIP(ctx, hipStreamSynchronize(st));
+    if (lp) {
+        double x = s;
+        x /= n;
+        x /= (n - 1);
+        *lp = 1.0 / x;
+    }
+    if (isolated) *isolated = ni;
     return PG_OK;
 }

@@ -1,0 +1,63 @@
+"""File formats (Python mirror) and the tools' argument handling, no GPU."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from pagani_amd import formats
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "pagani-toolkit_amd", "bin")
+
+
+def test_nifti_header_words(tmp_path):
+    p = tmp_path / "a.nii"
+    data = np.arange(24, dtype=np.float32).reshape(3, 8)
+    formats.write_nifti(str(p), data, formats.DT_FLOAT32)
+    raw = p.read_bytes()
+    h = np.frombuffer(raw[:352], np.int16)
+    assert (h[21], h[22], h[23], h[24], h[35], h[36]) == (8, 1, 1, 3, 16, 32)
+    np.testing.assert_array_equal(np.frombuffer(raw[352:], np.float32), data.ravel())
+
+
+def test_csr_and_nm_round_trip(tmp_path):
+    row = np.array([0, 2, 3, 5], np.int32)
+    col = np.array([1, 2, 0, 0, 1], np.int32)
+    val = np.array([0.5, 0.25, 0.5, 0.25, 0.75], np.float32)
+    p = tmp_path / "g.csr"
+    formats.write_csr(str(p), row, col, val)
+    raw = p.read_bytes()
+    assert struct.unpack_from("<i", raw, 0)[0] == 4
+    assert len(raw) == 4 + 16 + 4 + 20 + 4 + 20
+    r, c, v = formats.read_csr(str(p))
+    np.testing.assert_array_equal(r, row)
+    np.testing.assert_array_equal(c, col)
+    np.testing.assert_array_equal(v, val)
+    formats.write_csr(str(p), row, col)
+    assert formats.read_csr(str(p))[2] is None
+    formats.write_nm(str(tmp_path / "x.nm"), [1.5, 2.5])
+    np.testing.assert_array_equal(formats.read_nm(str(tmp_path / "x.nm")), [1.5, 2.5])
+
+
+def test_csr_name_matches_reference_printf():
+    # sprintf_s(sparsity, "_spa%.3f%%_cor%.3f", s_thresh, r_thresh) with floats
+    s, r = np.float32(1.0), np.float32(0.2139584882412609)
+    assert formats.csr_name("sub01", 58000, float(s), float(r)) == "sub01_58000_spa1.000%_cor0.214.csr"
+    assert formats.csr_name("sub01", 5, 2.5, 0.4, True) == "sub01_5_spa2.500%_cor0.400_weighted.csr"
+
+
+@pytest.mark.parametrize("tool,args", [
+    ("CUCorMat", []), ("CUBC", []), ("Degree", []), ("Cp", ["x"]), ("CUBFW_Lp", ["x", "0"]),
+    ("CUmodularity", ["x"]),
+    ("CUCorMat", ["/nonexistent", "0.5", "y", "n", "r", "0.3"]),   # averaging not offered
+    ("CUCorMat", ["/nonexistent", "0.5", "n", "n", "q", "0.3"]),   # bad threshold type
+    ("Cp", ["/nonexistent", "3", "gk"]),                             # random networks not offered
+])
+def test_tools_reject_bad_arguments(tool, args):
+    exe = os.path.join(BIN, tool)
+    if not os.path.exists(exe):
+        pytest.skip("tools not built")
+    r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1, (r.returncode, r.stdout, r.stderr)

@@ -1,0 +1,122 @@
+"""GPU: the command-line tools end to end on synthetic NIfTI volumes.
+
+Files are compared with the FP64 oracle: edge sets and degrees bit-exact,
+BC / clustering / efficiency within the tolerances of the library tests.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import say
+from oracle import oracle as O
+from pagani_amd import formats, synth
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "pagani-toolkit_amd", "bin")
+
+
+def run(tool, *args, cwd=None):
+    say(f"run {tool} {' '.join(map(str, args))}")
+    r = subprocess.run([os.path.join(BIN, tool), *map(str, args)], capture_output=True, text=True,
+                       timeout=600, cwd=cwd)
+    assert r.returncode == 0, (tool, r.returncode, r.stdout[-2000:], r.stderr[-2000:])
+    return r.stdout
+
+
+@pytest.fixture(scope="module")
+def volume(tmp_path_factory):
+    d = tmp_path_factory.mktemp("bold")
+    v, t = 1400, 120
+    mask = np.ones(v, np.uint8)
+    mask[::17] = 0
+    formats.write_nifti(str(d / "mask.nii"), mask, formats.DT_UINT8)
+    subs = {}
+    for name, seed, dt in [("sub01", 21, formats.DT_FLOAT32), ("sub02", 22, formats.DT_FLOAT64)]:
+        x = synth.bold(seed, v, t, 8)
+        formats.write_nifti(str(d / f"{name}.nii"), x if dt == formats.DT_FLOAT32 else x.astype(np.float64), dt)
+        subs[name] = np.ascontiguousarray(x[:, mask >= 1])
+    return d, subs, int((mask >= 1).sum())
+
+
+def test_cucormat_rmode_files(volume):
+    d, subs, n = volume
+    run("CUCorMat", d, 0.5, "n", "n", "r", 0.35, 0.3)
+    for name, x in subs.items():
+        z64, _ = O.zscore(x)
+        for thr in (0.3, 0.35):
+            orow, ocol, _ = O.corr_csr_r(z64, thr)
+            m1 = n * (n - 1) // 2
+            s = np.float32(100.0 * orow[-1] / m1 / 2)
+            path = d / formats.csr_name(f"{name}", n, float(s), float(np.float32(thr)))
+            row, col, val = formats.read_csr(str(path))
+            assert val is None
+            np.testing.assert_array_equal(row, orow)
+            np.testing.assert_array_equal(col, ocol)
+
+
+def test_cucormat_documented_form_sparsity_weighted(volume):
+    d, subs, n = volume
+    run("CUCorMat", "--weighted", d, d / "mask.nii", 0.5, "n", "n", "s", 2.0)
+    out = d / "weighted"
+    for name, x in subs.items():
+        z64, _ = O.zscore(x)
+        orow, ocol, oval, tstar = O.corr_csr_s(z64, 2.0, True)
+        path = out / formats.csr_name(name, n, 2.0, float(np.float32(tstar)), True)
+        row, col, val = formats.read_csr(str(path))
+        np.testing.assert_array_equal(row, orow)
+        np.testing.assert_array_equal(col, ocol)
+        np.testing.assert_array_equal(val.view(np.uint32), oval.view(np.uint32))
+
+
+def _csr_files(d, weighted=False):
+    return sorted(f for f in os.listdir(d) if f.endswith(".csr"))
+
+
+def test_metric_tools_on_binary_networks(volume):
+    d, subs, n = volume
+    run("Degree", d)
+    run("CUBC", d)
+    run("Cp", d, 0, "gn")
+    for f in _csr_files(d):
+        base = str(d / f)[:-4]
+        row, col, _ = formats.read_csr(str(d / f))
+        _, dref = O.degree(row)
+        np.testing.assert_array_equal(formats.read_nm(base + "_deg.nm"), dref)
+        np.testing.assert_allclose(formats.read_nm(base + "_bc.nm"), O.bc(row, col).astype(np.float32), rtol=1e-6)
+        cp, mean = O.cp_binary(row, col)
+        np.testing.assert_array_equal(formats.read_nm(base + "_cp.nm"), cp)
+        assert abs(float(open(base + "_cp.txt").read()) - mean) < 1e-5
+    iso = open(d / "isolated_v_mark.txt").read().split()
+    assert len(iso) == len(_csr_files(d))
+
+
+def test_lp_and_modularity_tools_on_weighted_networks(volume):
+    d, subs, n = volume
+    w = d / "weighted"
+    run("CUBFW_Lp", w, 0, "gn")
+    run("CUmodularity", w, 0)
+    run("Degree", "--weighted", w)
+    run("Cp", "--weighted", w, 0, 2, "n")
+    for f in _csr_files(w):
+        base = str(w / f)[:-4]
+        row, col, val = formats.read_csr(str(w / f))
+        # efficiency from the oracle's blocked closure with the tool's tile edge
+        dist = O.fw_blocked(O.fw_init(row, col, val, 0), 128)
+        inv = np.where((dist >= 0.1) & ~np.eye(n, dtype=bool), 1.0 / dist.astype(np.float64), 0.0)
+        eff = (inv.sum(1) / (n - 1)).astype(np.float32)
+        eff[eff < np.float32(1.0 / n)] = 0
+        np.testing.assert_allclose(formats.read_nm(base + "_eff.nm"), eff, rtol=1e-5, atol=1e-7)
+        lp = 1.0 / (inv.sum() / n / (n - 1))
+        assert abs(float(open(base + "_Lp.txt").read()) - lp) <= 1e-5 * lp
+        np.testing.assert_array_equal(formats.read_nm(base + "_deg.nm"), O.strength(row, val))
+        cp, _ = O.cp_weighted(row, col, val, 2, 0)
+        np.testing.assert_allclose(formats.read_nm(base + "_cp.nm"), cp.astype(np.float32), rtol=1e-6)
+        lab, q, *_ = O.modularity(row, col, val, 1)
+        got = formats.read_nm(base + "_modu.nm").astype(np.int32)
+        assert (got == lab).mean() > 0.99
+        txt = open(base + "_modu.txt").read()
+        qfile = float(txt.split("Q=")[1].split()[0])
+        assert abs(qfile - q) <= 1e-5

@@ -25,10 +25,10 @@
 namespace {
 
 #ifndef PG_GRAM_TM
-#define PG_GRAM_TM 256
+#define PG_GRAM_TM 128
 #endif
 #ifndef PG_GRAM_KC
-#define PG_GRAM_KC 32
+#define PG_GRAM_KC 64
 #endif
 constexpr int TM = PG_GRAM_TM;  // Gram tile rows (128 or 256)
 constexpr int TN = 128;         // Gram tile columns

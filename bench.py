@@ -49,15 +49,17 @@ def cpu_baseline(x, n, t, thr_r, budget_s=15.0):
     t0 = time.perf_counter()
     z64, _ = O.zscore(x)
     tz = time.perf_counter() - t0
-    # calibrate on a few rows, then size the sample to the budget
-    rows = 8
-    t0 = time.perf_counter()
-    O.corr_count_rows(z64, thr_r, 0, rows)
-    dt = max(time.perf_counter() - t0, 1e-3)
-    rows = int(min(n // 2, max(rows, rows * budget_s / dt)))
-    t0 = time.perf_counter()
-    O.corr_count_rows(z64, thr_r, 0, rows)
-    dt = time.perf_counter() - t0
+    # consecutive row blocks, each sized from the previous block's rate to
+    # about 2 s, until the budget is spent
+    rows, dt, chunk = 0, 0.0, 16
+    while dt < budget_s and rows < n // 2:
+        chunk = min(chunk, n // 2 - rows)
+        t0 = time.perf_counter()
+        O.corr_count_rows(z64, thr_r, rows, rows + chunk)
+        step = max(time.perf_counter() - t0, 1e-4)
+        dt += step
+        rows += chunk
+        chunk = int(max(16, min(chunk * 2.0 / step, chunk * 8)))
     pairs = rows * (n - 1) - rows * (rows - 1) // 2
     pair_rate = pairs / dt
     full_pairs = n * (n - 1) // 2
@@ -70,7 +72,6 @@ def cpu_baseline(x, n, t, thr_r, budget_s=15.0):
         "sample": f"FP64 r + threshold for voxel rows 0..{rows} of N={n}, T={t} "
                   f"({pairs} of {full_pairs} pairs, {dt:.1f} s); z-score of all voxels {tz:.2f} s",
         "est_full_subject_s": est_s,
-        "np_threads": int(np.__config__ is not None),
     }
 
 
@@ -202,7 +203,6 @@ def main(argv=None) -> int:
     if not a.no_cpu_baseline:
         log("cpu baseline (bounded sample)")
         cb = cpu_baseline(x, n, t, float(last.t_star))
-        cb.pop("np_threads", None)
         out["cpu_baseline"] = cb
     ctx.close()
     print(json.dumps(out), flush=True)

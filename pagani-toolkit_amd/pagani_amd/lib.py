@@ -88,6 +88,7 @@ _SIGS = {
     "pg_bfs": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, C.c_int64, _VP]),
     "pg_fw": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32]),
     "pg_fw_init": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_int32, _VP]),
+    "pg_fw_efficiency": (C.c_int, [_VP, _VP, C.c_int32, _VP, _VP, _VP]),
     "pg_modularity": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_int32, _VP, _VP, _VP, _VP, _VP]),
 }
 
@@ -250,6 +251,15 @@ class Context:
         out = np.empty((n, n), np.float32) if out is None else out
         self._check(self._lib.pg_fw_init(self._h, _ptr(row), _ptr(col), _ptr(val), n, _len(col), kind, _ptr(out)))
         return out
+
+    def fw_efficiency(self, d, eff=None):
+        """(eff[n] float32, Lp, isolated count) from a closed distance matrix."""
+        n = int(d.shape[0])
+        eff = np.empty(n, np.float32) if eff is None else eff
+        lp, iso = C.c_double(0), C.c_int32(0)
+        self._check(self._lib.pg_fw_efficiency(self._h, _ptr(d), n, _ptr(eff), C.cast(C.pointer(lp), _VP),
+                                                C.cast(C.pointer(iso), _VP)))
+        return eff, lp.value, iso.value
 
     def modularity(self, row, col, val=None, kmode: int = MOD_STRENGTH):
         n = _len(row) - 1

@@ -165,6 +165,17 @@ def test_fw_bit_exact(ctx, weighted_graph, block, n):
     got = ctx.fw(d.copy(), block)
     want = O.fw_blocked(d, block)
     np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    # nodal efficiency / Lp (W CUBFW_Lp.cpp:70-113): skip d < 0.1, e < 1/n -> 0
+    eff, lp, iso = ctx.fw_efficiency(got)
+    with np.errstate(divide="ignore"):
+        q = (np.float32(1.0) / want).astype(np.float64)  # 1.0f / d, summed in FP64 for Lp
+    inv = np.where((want >= 0.1) & ~np.eye(n, dtype=bool), q, 0.0)
+    e_ref = (inv.sum(1) / (n - 1)).astype(np.float32)
+    small = e_ref < np.float32(1.0 / n)
+    e_ref[small] = 0
+    np.testing.assert_allclose(eff, e_ref, rtol=1e-5, atol=1e-7)
+    assert iso == int(small.sum())
+    assert abs(lp - 1.0 / (inv.sum() / n / (n - 1))) <= 1e-9 * lp
 
 
 def test_fw_unit_weights_equal_bfs(ctx):

@@ -32,6 +32,10 @@ for _p in (ROOT, os.path.join(ROOT, "pagani-toolkit_amd")):
 BF16_PEAK = 2516.6e12   # MI355X dense BF16 MFMA FLOP/s
 FP32_PEAK = 157.3e12    # MI355X FP32 matrix FLOP/s
 HBM_PEAK = 8.0e12       # bytes/s
+# Floyd-Warshall updates/s at the FP32 VALU peak without packed math: 256 CUs x
+# 64 lanes x 2.4 GHz = 39.3e12 ops/s, 1.5 ops per update (v_add_f32 +
+# half a v_min3_f32)
+FW_UPDATES_PEAK = 39.3e12 / 1.5
 
 METRIC = "per-subject wall time N=58k; corr TFLOP/s+MFMA%; BC GTEPS; BF16x3 vs FP32"
 
@@ -75,6 +79,159 @@ def cpu_baseline(x, n, t, thr_r, budget_s=15.0):
     }
 
 
+def _timed(ctx, fn):
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    r = fn()
+    ctx.synchronize()
+    return r, time.perf_counter() - t0
+
+
+def _device_csr(ctx, n, nnz, weighted=False):
+    import torch
+    import pagani_amd
+    row = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+    col = torch.empty(max(nnz, 1), dtype=torch.int32, device="cuda")
+    val = torch.empty(max(nnz, 1), dtype=torch.float32, device="cuda") if weighted else None
+    ctx._check(ctx._lib.pg_corr_csr(ctx._h, pagani_amd.lib._ptr(row), pagani_amd.lib._ptr(col),
+                                    pagani_amd.lib._ptr(val)))
+    return row, col, val
+
+
+def run_metrics(a):
+    """Graph metrics on the N=58,000 1%-density graph: exact BC from every
+    source, degree, binary clustering; Floyd-Warshall on the weighted graph of
+    the first 8,192 voxels (w = 1 - r for r above the 1% threshold)."""
+    import torch
+    import pagani_amd
+    from pagani_amd import synth
+    w = synth.WORKLOADS["corr_n58000_t240"]
+    n, t = w["n"], w["t"]
+    x = synth.bold(w["seed"], n, t, w["k"])
+    xd = torch.from_numpy(x).cuda().contiguous()
+    ctx = pagani_amd.Context(0)
+    info = ctx.corr_build(xd, n, t, 1.0, sparsity=True)
+    nnz = info.nnz
+    row, col, _ = _device_csr(ctx, n, nnz)
+    out = {"metric": "graph metrics on the N=58000 1% graph", "n_voxels": n, "nnz": nnz, "n_gpus": 1}
+    deg = torch.empty(n, dtype=torch.int32, device="cuda")
+    dref = torch.empty(n, dtype=torch.float32, device="cuda")
+    _, dt = _timed(ctx, lambda: ctx.degree(row, deg, dref))
+    out["degree_ms"] = dt * 1e3
+    tri = torch.empty(n, dtype=torch.int64, device="cuda")
+    _, dt = _timed(ctx, lambda: ctx.triangles(row, col, tri))
+    out["triangles_ms"] = dt * 1e3
+    cp = torch.empty(n, dtype=torch.float32, device="cuda")
+    (_, mean), dt = _timed(ctx, lambda: ctx.cp_binary(row, col, cp))
+    out["cp_binary_ms"] = dt * 1e3
+    out["cp_mean"] = mean
+    log(f"degree {out['degree_ms']:.3f} ms, triangles {out['triangles_ms']:.1f} ms, cp {out['cp_binary_ms']:.1f} ms")
+    bc = torch.empty(n, dtype=torch.float64, device="cuda")
+    _, dt = _timed(ctx, lambda: ctx.bc(row, col, None, out=bc))
+    te = float(n) * nnz
+    out["bc_all_sources"] = {"sources": n, "seconds": dt, "gteps": te / dt / 1e9,
+                             "hbm_model_frac": n * (16.0 * nnz + 40.0 * n) / dt / HBM_PEAK}
+    log(f"bc all {n} sources: {dt:.2f} s = {te / dt / 1e9:.1f} GTEPS")
+
+    # Floyd-Warshall on the first 8192 voxels: weighted graph r > t*, d = 1 - r
+    nf = 8192
+    xs = xd[:, :nf].contiguous()
+    thr = float(info.t_star)
+    winfo = ctx.corr_build(xs, nf, t, thr, weighted=True)
+    wr, wc, wv = _device_csr(ctx, nf, winfo.nnz, True)
+    d = torch.empty((nf, nf), dtype=torch.float32, device="cuda")
+    fw = {"n": nf, "nnz": winfo.nnz, "weights": "1 - r"}
+    for block in (64, 128):
+        ctx.fw_init(wr, wc, wv, 1, out=d)
+        _, dt = _timed(ctx, lambda: ctx.fw(d, block))
+        ups = float(nf) ** 3 / dt
+        fw[f"block{block}"] = {"ms": dt * 1e3, "updates_per_s": ups, "valu_frac": ups / FW_UPDATES_PEAK}
+        log(f"fw n={nf} block={block}: {dt * 1e3:.1f} ms")
+    eff = torch.empty(nf, dtype=torch.float32, device="cuda")
+    (_, lp, iso), dt = _timed(ctx, lambda: ctx.fw_efficiency(d, eff))
+    fw["efficiency_ms"] = dt * 1e3
+    fw["lp"] = lp
+    out["fw"] = fw
+    ctx.close()
+    return out
+
+
+def run_cohort(a):
+    """A cohort of subjects with N ~ U[20000, 70000], T ~ U[150, 300], 1%
+    density each: correlation -> CSR -> degree + clustering + BC from 4096
+    seeded sources.  Subject s uses voxels [0, N_s) and time points [0, T_s)
+    of one N=70000, T=300 draw of the K=90 latent model (a prefix of a draw is
+    a draw of the same model)."""
+    import numpy as np
+    import torch
+    import pagani_amd
+    from pagani_amd import synth
+    rng = np.random.default_rng(7)
+    ns = rng.integers(20000, 70001, a.subjects)
+    ts = rng.integers(150, 301, a.subjects)
+    x = torch.from_numpy(synth.bold(11, 70000, 300, 90)).cuda()
+    ctx = pagani_amd.Context(0)
+    lat, parts = [], []
+    t_all = time.perf_counter()
+    for s in range(a.subjects):
+        n, t = int(ns[s]), int(ts[s])
+        xs = x[:t, :n].contiguous()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = ctx.corr_build(xs, n, t, 1.0, sparsity=True)
+        t1 = time.perf_counter()
+        row, col, _ = _device_csr(ctx, n, info.nnz)
+        deg = torch.empty(n, dtype=torch.int32, device="cuda")
+        ctx.degree(row, deg, None)
+        cp = torch.empty(n, dtype=torch.float32, device="cuda")
+        ctx.cp_binary(row, col, cp)
+        src = torch.from_numpy(synth.sample_sources(n, 4096, 4242)).cuda()
+        bc = torch.empty(n, dtype=torch.float64, device="cuda")
+        ctx.bc(row, col, src, out=bc)
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        lat.append(t2 - t0)
+        parts.append((t1 - t0, t2 - t1))
+        log(f"subject {s}: N={n} T={t} nnz={info.nnz} corr {1e3 * (t1 - t0):.1f} ms, metrics {1e3 * (t2 - t1):.1f} ms")
+    total = time.perf_counter() - t_all
+    ctx.close()
+    lat_ms = np.array(lat) * 1e3
+    return {"metric": "cohort per-subject latency and throughput", "subjects": a.subjects, "n_gpus": 1,
+            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+            "subjects_per_s": a.subjects / total,
+            "corr_ms_mean": float(np.mean([p[0] for p in parts]) * 1e3),
+            "metrics_ms_mean": float(np.mean([p[1] for p in parts]) * 1e3),
+            "launch": "per-subject host loop (no graph capture)"}
+
+
+def run_weighted(a):
+    """N=120000, T=300, K=200: weighted network r > 0.3, Onnela clustering and
+    leading-eigenvector modularity."""
+    import torch
+    import pagani_amd
+    from pagani_amd import synth
+    w = synth.WORKLOADS["weighted_n120000_t300"]
+    n, t = w["n"], w["t"]
+    xd = torch.from_numpy(synth.bold(w["seed"], n, t, w["k"])).cuda()
+    ctx = pagani_amd.Context(0)
+    info, dt = _timed(ctx, lambda: ctx.corr_build(xd, n, t, 0.3, weighted=True))
+    out = {"metric": "weighted N=120000 r>0.3: construction, clustering, modularity", "n_gpus": 1,
+           "n_voxels": n, "t": t, "nnz": info.nnz, "corr_ms": dt * 1e3}
+    log(f"weighted corr: nnz={info.nnz} {dt * 1e3:.1f} ms")
+    row, col, val = _device_csr(ctx, n, info.nnz, True)
+    cpw = torch.empty(n, dtype=torch.float64, device="cuda")
+    (_, mean), dt = _timed(ctx, lambda: ctx.cp_weighted(row, col, val, 2, 0, out=cpw))
+    out["cp_onnela_ms"] = dt * 1e3
+    out["cp_onnela_mean"] = mean
+    log(f"onnela {dt * 1e3:.1f} ms")
+    (lab, q, lam, eig, rounds), dt = _timed(ctx, lambda: ctx.modularity(row, col, val))
+    out["modularity"] = {"seconds": dt, "q": q, "lambda0": lam, "rounds": rounds,
+                         "communities": int(lab.max())}
+    log(f"modularity {dt:.2f} s, Q={q:.4f}, {int(lab.max())} communities")
+    ctx.close()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +241,13 @@ def main(argv=None) -> int:
     ap.add_argument("--bc-sources", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-side", action="store_true", help="skip FP32 and BC side measurements")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only copy 1 GiB device to device 5 times (scale for HBM byte counters)")
+    ap.add_argument("--config", choices=["corr", "metrics", "cohort", "weighted"], default="corr",
+                    help="corr: the headline line (default); metrics: BC from all sources, clustering "
+                         "and N=8192 Floyd-Warshall on the 1%% graph; cohort: 64 subjects of uneven N, T; "
+                         "weighted: N=120000 r>0.3 weighted clustering + modularity")
+    ap.add_argument("--subjects", type=int, default=64, help="cohort size (--config cohort)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the construction path "
@@ -95,8 +259,25 @@ def main(argv=None) -> int:
 
     import numpy as np
     import torch
+    if a.calib_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 5
+        print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30,
+                          "copy_ms": dt * 1e3, "copy_GBps_rw": 2.0 * (1 << 30) / dt / 1e9}), flush=True)
+        return 0
     import pagani_amd
     from pagani_amd import synth
+
+    if a.config != "corr":
+        out = {"metrics": run_metrics, "cohort": run_cohort, "weighted": run_weighted}[a.config](a)
+        print(json.dumps(out), flush=True)
+        return 0
 
     w = synth.WORKLOADS["corr_n58000_t240"]
     n, t, k_comm, seed = w["n"], w["t"], w["k"], w["seed"]
@@ -154,7 +335,7 @@ def main(argv=None) -> int:
         "band_pairs": last.band_pairs,
         "band_flips": last.band_flips,
         "t_star": last.t_star,
-        "roofline": {"bound": "mfma", "kernel": "gram_kernel<BF16X3,EMIT>",
+        "roofline": {"bound": "mfma", "kernel": "gram2_kernel<EMIT> (BF16x3)",
                      "achieved": 3.0 * useful / (emit_ms * 1e-3) / 1e12, "peak": BF16_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": mfma_frac, "traffic": None},
     }

@@ -56,6 +56,8 @@ struct GramArgs {
     int kchunks;           // ldp / KC
     int n;                 // voxels
     int nr, nc;            // row tiles (npad / TM) and column tiles (npad / TN)
+    int rt;                // TM / TN of the kernel that walks the tiles
+    int npad;              // padded voxel count (panel rows)
     const int2 *st_list;   // supertiles (SR, SC), host-built
     long long nst;         // supertile count
     long long sstep;       // > 0: visit every sstep-th supertile only (histogram sample)
@@ -79,6 +81,29 @@ __device__ __forceinline__ uint16_t f2bf(float f)
     return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Geometry of the LDS-DMA staged BF16x3 Gram (gram2_kernel) and its panels.
+namespace g2 {
+constexpr int TM = 256, TN = 128, RT = TM / TN;
+constexpr int WR = 128, WC = 64, BMW = WR / 32, BNW = WC / 32;
+constexpr int KC = 16;
+constexpr int NSTAGE = 3;
+constexpr int A_BYTES = TM * KC * 2;              // one array of the row slice
+constexpr int B_BYTES = TN * KC * 2;              // one array of the column slice
+constexpr int STAGE = 2 * A_BYTES + 2 * B_BYTES;  // 24 KB
+constexpr int DMA_WAVE = STAGE / 1024 / 4;        // 1 KB wave loads per wave per stage (6)
+static_assert(DMA_WAVE * 4 * 1024 == STAGE, "stage must split into 1 KB wave loads");
+}  // namespace g2
+
+__host__ __device__ __forceinline__ int g2_swz(int i) { return ((i >> 2) ^ (i >> 3)) & 1; }
+
+// element offset of (voxel i, time l) in the chunk-major panel
+__host__ __device__ __forceinline__ long long g2_index(int i, int l, int npad)
+{
+    const int c = l >> 4, h = (l >> 3) & 1;
+    return ((long long)c * npad + i) * 16 + ((h ^ g2_swz(i)) << 3) + (l & 7);
+}
+
 
 // ---------------------------------------------------------------- z-score
 // U CorMat.cu:34-53: mean and sqrt(sum (x-mu)^2) in FP64 with each product
@@ -112,7 +137,7 @@ __global__ void zscore_stats(const float *__restrict__ x, int layout, int n, int
 __global__ void zscore_write(const float *__restrict__ x, int layout, int n, int t,
                              const double *__restrict__ mu, const double *__restrict__ sd,
                              double *__restrict__ z64, float *__restrict__ p32,
-                             uint16_t *__restrict__ phi, uint16_t *__restrict__ plo, int ldp)
+                             uint16_t *__restrict__ phi, uint16_t *__restrict__ plo, int ldp, int npad)
 {
 #pragma clang fp contract(off)
     __shared__ float tile[64][65];
@@ -135,10 +160,64 @@ __global__ void zscore_write(const float *__restrict__ x, int layout, int n, int
         double z = s > 0.0 ? ((double)tile[li][ii] - mu[i]) / s : 0.0;
         z64[(long long)i * t + l] = z;
         float zf = (float)z;
-        long long q = (long long)i * ldp + l;
-        if (p32) p32[q] = zf;
+        if (p32) p32[(long long)i * ldp + l] = zf;
         if (phi) {
+            const long long q = g2_index(i, l, npad);
             uint16_t h = f2bf(zf);
+            phi[q] = h;
+            plo[q] = f2bf(zf - bf2f(h));
+        }
+    }
+}
+
+// Fused form for T up to ZS_TMAX: a block stages the whole T x 64 tile of its
+// voxels in LDS (coalesced), 64 threads form the sequential FP64 sums from
+// LDS, then all threads write z64 and the panels voxel-major.
+constexpr int ZS_TMAX = 360;
+
+__global__ __launch_bounds__(256) void zscore_fused(const float *__restrict__ x, int layout, int n, int t,
+                                                   double *__restrict__ z64, float *__restrict__ p32,
+                                                   uint16_t *__restrict__ phi, uint16_t *__restrict__ plo, int ldp,
+                                                   int npad)
+{
+#pragma clang fp contract(off)
+    extern __shared__ float zt[];  // [t][65]
+    __shared__ double smu[64], ssd[64];
+    const int i0 = blockIdx.x * 64, tid = threadIdx.x;
+    for (int e = tid; e < 64 * t; e += 256) {
+        int l, ii;
+        if (layout == 0) { l = e >> 6; ii = e & 63; }
+        else { ii = e / t; l = e % t; }
+        const int i = i0 + ii;
+        zt[l * 65 + ii] = i < n ? (layout == 0 ? x[(long long)l * n + i] : x[(long long)i * t + l]) : 0.0f;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        double s1 = 0.0;
+        for (int l = 0; l < t; l++) s1 += (double)zt[l * 65 + tid];
+        s1 /= t;
+        double s2 = 0.0;
+        for (int l = 0; l < t; l++) {
+            double d = (double)zt[l * 65 + tid] - s1;
+            double p = d * d;
+            s2 += p;
+        }
+        smu[tid] = s1;
+        ssd[tid] = sqrt(s2);
+    }
+    __syncthreads();
+    for (int e = tid; e < 64 * t; e += 256) {
+        const int ii = e / t, l = e % t;
+        const int i = i0 + ii;
+        if (i >= n) continue;
+        const double sdv = ssd[ii];
+        const double z = sdv > 0.0 ? ((double)zt[l * 65 + ii] - smu[ii]) / sdv : 0.0;
+        z64[(long long)i * t + l] = z;
+        const float zf = (float)z;
+        if (p32) p32[(long long)i * ldp + l] = zf;
+        if (phi) {
+            const long long q = g2_index(i, l, npad);
+            const uint16_t h = f2bf(zf);
             phi[q] = h;
             plo[q] = f2bf(zf - bf2f(h));
         }
@@ -168,19 +247,21 @@ struct TileWalk {
     long long nst;      // its length
     long long sstep;    // visit list entries 0, sstep, 2 sstep, ... (sampling)
     int nr, nc;         // row and column tiles
+    int rt;             // tile aspect (rows / columns)
     int L, xcd;
     long long k, kend;  // position in this XCD's tile sequence
 
     // This XCD's sequence is the concatenation of its supertiles
     // (xcd + 8 m) * sstep, SUPER*SUPER slots each; work-group slot l takes
     // positions l, l + L, ... so the XCD's work-groups move together.
-    __device__ void init(const GramArgs &a)
+    __device__ __forceinline__ void init(const GramArgs &a)
     {
         st = a.st_list;
         nst = a.nst;
         sstep = a.sstep > 0 ? a.sstep : 1;
         nr = a.nr;
         nc = a.nc;
+        rt = a.rt;
         const int g = blockIdx.x, G = gridDim.x;
         xcd = g % NXCD;
         L = G / NXCD;
@@ -189,7 +270,7 @@ struct TileWalk {
         const long long mine = mx >= xcd ? (mx - xcd) / NXCD + 1 : 0;
         kend = mine * SUPER * SUPER;
     }
-    __device__ bool next(int &R, int &C)
+    __device__ __forceinline__ bool next(int &R, int &C)
     {
         while (k < kend) {
             const long long kk = k;
@@ -198,7 +279,7 @@ struct TileWalk {
             const int q = (int)(kk % (SUPER * SUPER));
             const int2 sc = st[s];
             const int r = sc.x * SUPER + q / SUPER, c = sc.y * SUPER + q % SUPER;
-            if (r < nr && c < nc && c >= RT * r) {
+            if (r < nr && c < nc && c >= rt * r) {
                 R = r;
                 C = c;
                 return true;
@@ -514,14 +595,266 @@ __global__ __launch_bounds__(THREADS, PG_GRAM_WG_PER_CU) void gram_kernel(GramAr
     }
 }
 
-// Sum the per-work-group histograms.
-__global__ void hist_reduce(const uint32_t *__restrict__ h, int nwg, unsigned long long *__restrict__ out)
+// --------------------------------------------- BF16x3 Gram, LDS-DMA staged
+//
+// Panel layout (bf16, hi and lo arrays): chunk-major.  Time is cut in chunks
+// of 16; chunk c of voxel i is 32 bytes at ((c * npad + i) * 16) elements,
+// its two 16-byte halves (k 0-7, k 8-15) stored in the order h ^ swz(i).  A
+// 256-row or 128-row panel slice of one chunk is then one contiguous block
+// that the work-group copies global -> LDS with 16-byte-per-lane direct loads
+// (no VGPR staging), and the swizzle makes the MFMA operand reads
+// (ds_read_b128: lane l reads row l % 32, half l / 32) bank-conflict free.
+//
+// Tile 256 x 128; four waves in 2 x 2, each a 128 x 64 block of 32 x 32 MFMA
+// tiles (8 accumulators).  A three-slot ring of 16-deep K stages: chunk c + 2
+// is in flight while chunk c is multiplied; the ring runs on across tile
+// boundaries.
+#define G2_LDS(p) ((__attribute__((address_space(3))) void *)(p))
+
+// This wave's share of the copies of chunk kc of tile (R, C) into LDS stage
+// slot `dst`.  A stage is [A-hi 8 KB][A-lo 8 KB][B-hi 4 KB][B-lo 4 KB], copied
+// in 1 KB wave loads; wave w takes A pieces 2w, 2w+1 of both arrays and B
+// piece w of both arrays.
+__device__ __forceinline__ void g2_dma(const GramArgs &a, char *dst, int R, int C, int kc)
 {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= NBIN) return;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const char *hi = (const char *)a.p0, *lo = (const char *)a.p1;
+    const long long arow = ((long long)kc * a.npad + (long long)R * g2::TM) * 32 + 2 * w * 1024;
+    const long long brow = ((long long)kc * a.npad + (long long)C * g2::TN) * 32 + w * 1024;
+    const int la = lane * 16;
+    char *da = dst + 2 * w * 1024, *db = dst + 2 * g2::A_BYTES + w * 1024;
+    __builtin_amdgcn_global_load_lds((const void *)(hi + arow + la), G2_LDS(da), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void *)(hi + arow + 1024 + la), G2_LDS(da + 1024), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void *)(lo + arow + la), G2_LDS(da + g2::A_BYTES), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void *)(lo + arow + 1024 + la), G2_LDS(da + g2::A_BYTES + 1024), 16,
+                                     0, 0);
+    __builtin_amdgcn_global_load_lds((const void *)(hi + brow + la), G2_LDS(db), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void *)(lo + brow + la), G2_LDS(db + g2::B_BYTES), 16, 0, 0);
+}
+
+// s_waitcnt vmcnt(n) (gfx9 encoding, other counters left alone)
+#define G2_WAIT_VM(n) __builtin_amdgcn_s_waitcnt((0x3F70) | ((n) & 15) | ((((n) >> 4) & 3) << 14))
+
+// Epilogue of one wave's 128 x 64 block: pass 1 counts per lane (packed: low
+// 16 bits edges v > upper, high 16 bits rechecks lower <= v <= upper; each at
+// most 8192 per wave); one reservation per wave; pass 2 has each lane write
+// its own hits in order from its exclusive prefix.  BOUNDED: every pair is
+// inside the matrix and above the diagonal.
+template <bool BOUNDED>
+__device__ __forceinline__ void g2_emit(const GramArgs &a, const f32x16 (&acc)[g2::BMW][g2::BNW], int i0, int j0,
+                                        int lane, int2 *myseg, unsigned long long *s_seg)
+{
+    const int ib0 = i0 + (lane >> 5) * 4, jb0 = j0 + (lane & 31);
+    unsigned cnt = 0;
+#pragma unroll
+    for (int bm = 0; bm < g2::BMW; bm++)
+#pragma unroll
+        for (int bn = 0; bn < g2::BNW; bn++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const float v = acc[bm][bn][q];
+                bool hit = v >= a.lower;
+                if (!BOUNDED) {
+                    const int i = ib0 + bm * 32 + (q >> 2) * 8 + (q & 3), j = jb0 + bn * 32;
+                    hit = hit & (j < a.n) & (i < j);
+                }
+                cnt += hit ? (v > a.upper ? 1u : 0x10000u) : 0u;
+            }
+    unsigned x = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    const unsigned tot = __builtin_amdgcn_readfirstlane(__shfl(x, 63));
+    if (tot == 0) return;
+    unsigned long long be = 0, bc = 0;
+    if (lane == 0) {
+        if (tot & 0xFFFFu) be = atomicAdd(s_seg, (unsigned long long)(tot & 0xFFFFu));
+        if (tot >> 16) bc = atomicAdd(a.band_cnt, (unsigned long long)(tot >> 16));
+    }
+    const unsigned ex = x - cnt;
+    be = __shfl(be, 0) + (ex & 0xFFFFu);
+    bc = __shfl(bc, 0) + (ex >> 16);
+    // opaque copies: pass 2 recomputes its predicates instead of the compiler
+    // keeping pass 1's 128 of them live
+    float lo = a.lower, up = a.upper;
+    int ib = ib0, jb = jb0, nn = a.n;
+    asm volatile("" : "+v"(lo), "+v"(up), "+v"(ib), "+v"(jb), "+v"(nn));
+    float *segr = a.seg_r ? a.seg_r + (long long)blockIdx.x * a.seg_cap : nullptr;
+#pragma unroll
+    for (int bm = 0; bm < g2::BMW; bm++)
+#pragma unroll
+        for (int bn = 0; bn < g2::BNW; bn++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const float v = acc[bm][bn][q];
+                const int i = ib + bm * 32 + (q >> 2) * 8 + (q & 3), j = jb + bn * 32;
+                bool hit = v >= lo;
+                if (!BOUNDED) hit = hit & (j < nn) & (i < j);
+                if (hit) {
+                    if (v > up) {
+                        if ((long long)be < a.seg_cap) {
+                            myseg[be] = make_int2(i, j);
+                            if (segr) segr[be] = v;
+                        }
+                        be++;
+                    } else {
+                        if ((long long)bc < a.band_cap) {
+                            a.band[bc] = make_int2(i, j);
+                            a.band_r[bc] = v;
+                        }
+                        bc++;
+                    }
+                }
+            }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
+{
+    constexpr int TM = g2::TM, TN = g2::TN, WR = g2::WR, WC = g2::WC, BMW = g2::BMW, BNW = g2::BNW;
+    constexpr int NSTAGE = g2::NSTAGE, STAGE = g2::STAGE, A_BYTES = g2::A_BYTES, B_BYTES = g2::B_BYTES;
+    constexpr int DMA_WAVE = g2::DMA_WAVE;
+    extern __shared__ __align__(16) char smem[];
+    uint32_t *s_hist = (uint32_t *)(smem + NSTAGE * STAGE);
+    __shared__ unsigned long long s_seg;
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    if (tid == 0) s_seg = 0;
+    if (MODE == MODE_HIST)
+        for (int b = tid; b < NBIN; b += THREADS) s_hist[b] = 0;
+
+    // per-lane operand offsets inside a stage (row l % 32, half l / 32)
+    const int lr = lane & 31, lh = lane >> 5;
+    const int opoff = lr * 32 + ((lh ^ g2_swz(lr)) << 4);
+    const int aoff = (wm * WR) * 32 + opoff;
+    const int boff = 2 * A_BYTES + (wn * WC) * 32 + opoff;
+
+    TileWalk walk;
+    walk.init(a);
+    int2 *myseg = a.seg + (long long)blockIdx.x * a.seg_cap;
+    const int K = a.kchunks;  // >= 2
+    int R = 0, C = 0, R1 = 0, C1 = 0;
+    bool have = walk.next(R, C);
+    bool have1 = have && walk.next(R1, C1);
+    if (have) {
+        g2_dma(a, smem, R, C, 0);
+        g2_dma(a, smem + STAGE, R, C, 1);
+    }
+    int slot = 0;
+
+    while (have) {
+        f32x16 acc[BMW][BNW];
+#pragma unroll
+        for (int bm = 0; bm < BMW; bm++)
+#pragma unroll
+            for (int bn = 0; bn < BNW; bn++)
+#pragma unroll
+                for (int q = 0; q < 16; q++) acc[bm][bn][q] = 0.0f;
+
+        for (int kc = 0; kc < K; kc++) {
+            // chunk kc has landed once at most the next chunk's loads are
+            // outstanding (vector memory operations complete in order)
+            if (kc + 1 < K || have1) G2_WAIT_VM(DMA_WAVE);
+            else G2_WAIT_VM(0);
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");  // no LDS access moves across the barrier
+            // the slot two ahead was last read one step ago by every wave
+            {
+                int s2 = slot + 2;
+                s2 -= s2 >= NSTAGE ? NSTAGE : 0;
+                if (kc + 2 < K) g2_dma(a, smem + s2 * STAGE, R, C, kc + 2);
+                else if (have1) g2_dma(a, smem + s2 * STAGE, R1, C1, kc + 2 - K);
+            }
+            const char *sp = smem + slot * STAGE;
+            bf16x8 ah[BMW], al[BMW], bh[BNW], bl[BNW];
+#pragma unroll
+            for (int b = 0; b < BMW; b++) {
+                ah[b] = *(const bf16x8 *)(sp + aoff + b * 32 * 32);
+                al[b] = *(const bf16x8 *)(sp + A_BYTES + aoff + b * 32 * 32);
+            }
+#pragma unroll
+            for (int b = 0; b < BNW; b++) {
+                bh[b] = *(const bf16x8 *)(sp + boff + b * 32 * 32);
+                bl[b] = *(const bf16x8 *)(sp + B_BYTES + boff + b * 32 * 32);
+            }
+            // same product order as the register-staged kernel: lo.hi, hi.lo, hi.hi
+#pragma unroll
+            for (int bm = 0; bm < BMW; bm++)
+#pragma unroll
+                for (int bn = 0; bn < BNW; bn++)
+                    acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[bm], bh[bn], acc[bm][bn], 0, 0, 0);
+#pragma unroll
+            for (int bm = 0; bm < BMW; bm++)
+#pragma unroll
+                for (int bn = 0; bn < BNW; bn++)
+                    acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[bm], bl[bn], acc[bm][bn], 0, 0, 0);
+#pragma unroll
+            for (int bm = 0; bm < BMW; bm++)
+#pragma unroll
+                for (int bn = 0; bn < BNW; bn++)
+                    acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[bm], bh[bn], acc[bm][bn], 0, 0, 0);
+            slot = slot + 1 == NSTAGE ? 0 : slot + 1;
+        }
+
+        // epilogue: lane holds column lane%32 and rows
+        // (q/4)*8 + (lane/32)*4 + q%4 of each 32 x 32 block
+        const int i0 = R * TM + wm * WR, j0 = C * TN + wn * WC;
+        const bool interior = C * TN >= R * TM + TM && (C + 1) * TN <= a.n;
+        if (MODE == MODE_HIST) {
+#pragma unroll
+            for (int bm = 0; bm < BMW; bm++)
+#pragma unroll
+                for (int bn = 0; bn < BNW; bn++)
+#pragma unroll
+                    for (int q = 0; q < 16; q++) {
+                        const float v = acc[bm][bn][q];
+                        const int i = i0 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
+                        const int j = j0 + bn * 32 + (lane & 31);
+                        if (interior || ((j < a.n) && (i < j))) {
+                            int b = (int)((v + 1.0f) * (NBIN * 0.5f));
+                            b = b < 0 ? 0 : (b >= NBIN ? NBIN - 1 : b);
+                            atomicAdd(&s_hist[b], 1u);
+                        }
+                    }
+        } else if (interior) {
+            g2_emit<true>(a, acc, i0, j0, lane, myseg, &s_seg);
+        } else {
+            g2_emit<false>(a, acc, i0, j0, lane, myseg, &s_seg);
+        }
+        R = R1;
+        C = C1;
+        have = have1;
+        if (have) have1 = walk.next(R1, C1);
+    }
+    G2_WAIT_VM(0);
+    __syncthreads();
+    if (MODE == MODE_HIST) {
+        uint32_t *out = a.hist + (long long)blockIdx.x * NBIN;
+        for (int b = tid; b < NBIN; b += THREADS) out[b] = s_hist[b];
+    } else if (tid == 0) {
+        a.seg_cnt[blockIdx.x] = s_seg;
+    }
+}
+
+// Sum the per-work-group histograms: 32 bins per block, 8 row groups each.
+__global__ __launch_bounds__(256) void hist_reduce(const uint32_t *__restrict__ h, int nwg,
+                                                   unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long part[8][32];
+    const int bl = threadIdx.x & 31, rg = threadIdx.x >> 5;
+    const int b = blockIdx.x * 32 + bl;
     unsigned long long s = 0;
-    for (int g = 0; g < nwg; g++) s += h[(long long)g * NBIN + b];
-    out[b] = s;
+    for (int g = rg; g < nwg; g += 8) s += h[(long long)g * NBIN + b];
+    part[rg][bl] = s;
+    __syncthreads();
+    if (rg == 0) {
+        for (int q = 1; q < 8; q++) s += part[q][bl];
+        out[b] = s;
+    }
 }
 
 // ---------------------------------------------------------- FP64 recheck
@@ -605,39 +938,67 @@ __device__ __forceinline__ int2 src_edge(const EdgeSrc &s, int g, long long k)
 // every segment's output range, a scatter of packed 32-bit entries
 // (row-in-bucket << 25 | column), then one work-group per bucket derives the
 // row offsets and sorts each row by LDS-bitmap rank.  No global atomics.
-constexpr int BSH = 7;
+constexpr int BSH = 6;
 constexpr int BROWS = 1 << BSH;
 constexpr unsigned COLMASK = (1u << 25) - 1u;
 
-__global__ __launch_bounds__(256) void bucket_count(EdgeSrc s, int nbk, unsigned *__restrict__ cnt)
+constexpr int SEG_THREADS = 1024;  // one work-group per segment: 16 waves in flight
+
+__global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, unsigned *__restrict__ cnt)
 {
     extern __shared__ unsigned sc[];
-    for (int b = threadIdx.x; b < nbk; b += 256) sc[b] = 0u;
+    for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) sc[b] = 0u;
     __syncthreads();
     const int g = blockIdx.x;
     const long long c = src_count(s, g);
-    for (long long k = threadIdx.x; k < c; k += 256) {
+    for (long long k = threadIdx.x; k < c; k += SEG_THREADS) {
         if (!src_keep(s, g, k)) continue;
         const int2 e = src_edge(s, g, k);
         atomicAdd(&sc[e.x >> BSH], 1u);
         atomicAdd(&sc[e.y >> BSH], 1u);
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < nbk; b += 256) cnt[(long long)g * nbk + b] = sc[b];
+    for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) cnt[(long long)b * gridDim.x + g] = sc[b];
 }
 
-// off[g][b] = bstart[b] + sum_{g' < g} cnt[g'][b];  bstart = exclusive scan of
-// the bucket totals (bstart[nbk] = total entries).
-__global__ __launch_bounds__(1024) void bucket_scan(const unsigned *__restrict__ cnt, int nsrc, int nbk,
-                                                    long long *__restrict__ off, long long *__restrict__ bstart)
+// Per bucket b (one block each): tot[b] and the within-bucket exclusive scan
+// over segments, off[b][g] = sum_{g' < g} cnt[b][g'] (counts bucket-major).
+__global__ __launch_bounds__(256) void bucket_scan_local(const unsigned *__restrict__ cnt, int nsrc,
+                                                         long long *__restrict__ off, long long *__restrict__ tot)
+{
+    __shared__ long long part[256];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const unsigned *c = cnt + (long long)b * nsrc;
+    const int per = (nsrc + 255) / 256;
+    const int g0 = tid * per, g1 = min(nsrc, g0 + per);
+    long long s = 0;
+    for (int g = g0; g < g1; g++) s += c[g];
+    part[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        long long v = tid >= o ? part[tid - o] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    long long run = part[tid] - s;
+    for (int g = g0; g < g1; g++) {
+        off[(long long)b * nsrc + g] = run;
+        run += c[g];
+    }
+    if (tid == 255) tot[b] = part[255];
+}
+
+// bstart = exclusive scan of the bucket totals; then off[b][g] += bstart[b].
+__global__ __launch_bounds__(1024) void bucket_scan_global(const long long *__restrict__ tot, int nbk,
+                                                           long long *__restrict__ bstart)
 {
     __shared__ long long part[1024];
     const int tid = threadIdx.x;
     const int per = (nbk + 1023) / 1024;
     const int b0 = tid * per, b1 = min(nbk, b0 + per);
     long long s = 0;
-    for (int b = b0; b < b1; b++)
-        for (int g = 0; g < nsrc; g++) s += cnt[(long long)g * nbk + b];
+    for (int b = b0; b < b1; b++) s += tot[b];
     part[tid] = s;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
@@ -649,29 +1010,27 @@ __global__ __launch_bounds__(1024) void bucket_scan(const unsigned *__restrict__
     long long run = part[tid] - s;
     for (int b = b0; b < b1; b++) {
         bstart[b] = run;
-        for (int g = 0; g < nsrc; g++) {
-            off[(long long)g * nbk + b] = run;
-            run += cnt[(long long)g * nbk + b];
-        }
+        run += tot[b];
     }
     if (tid == 1023) bstart[nbk] = part[1023];
 }
 
-__global__ __launch_bounds__(256) void bucket_scatter(EdgeSrc s, int nbk, const long long *__restrict__ off,
-                                                      unsigned *__restrict__ ent, float *__restrict__ entv,
-                                                      const double *__restrict__ z, int t)
+__global__ __launch_bounds__(SEG_THREADS) void bucket_scatter(EdgeSrc s, int nbk, const long long *__restrict__ off,
+                                                              const long long *__restrict__ bstart,
+                                                              unsigned *__restrict__ ent, float *__restrict__ entv,
+                                                              const double *__restrict__ z, int t)
 {
     extern __shared__ unsigned sc[];  // per-bucket cursors
-    for (int b = threadIdx.x; b < nbk; b += 256) sc[b] = 0u;
+    for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) sc[b] = 0u;
     __syncthreads();
-    const int g = blockIdx.x;
-    const long long *o = off + (long long)g * nbk;
+    const int g = blockIdx.x, nsrc = gridDim.x;
     const long long c = src_count(s, g);
-    for (long long k = threadIdx.x; k < c; k += 256) {
+    for (long long k = threadIdx.x; k < c; k += SEG_THREADS) {
         if (!src_keep(s, g, k)) continue;
         const int2 e = src_edge(s, g, k);
-        const long long pf = o[e.x >> BSH] + atomicAdd(&sc[e.x >> BSH], 1u);
-        const long long pr = o[e.y >> BSH] + atomicAdd(&sc[e.y >> BSH], 1u);
+        const int bf = e.x >> BSH, br = e.y >> BSH;
+        const long long pf = bstart[bf] + off[(long long)bf * nsrc + g] + atomicAdd(&sc[bf], 1u);
+        const long long pr = bstart[br] + off[(long long)br * nsrc + g] + atomicAdd(&sc[br], 1u);
         ent[pf] = ((unsigned)(e.x & (BROWS - 1)) << 25) | (unsigned)e.y;
         ent[pr] = ((unsigned)(e.y & (BROWS - 1)) << 25) | (unsigned)e.x;
         if (entv) {
@@ -682,20 +1041,102 @@ __global__ __launch_bounds__(256) void bucket_scatter(EdgeSrc s, int nbk, const 
     }
 }
 
+// Bitonic sort of one row held lane-major in a wave (element e = lane*P + r),
+// M = 64*P keys, ascending; padding keys sort last.  Key type: 32-bit column
+// or 64-bit (column << 32 | value bits) to carry the weight along.
+template <int P, class K>
+__device__ __forceinline__ void wave_bitonic(K (&x)[P], int lane)
+{
+    constexpr int M = 64 * P;
+#pragma unroll
+    for (int k = 2; k <= M; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j < P) {
+#pragma unroll
+                for (int r = 0; r < P; r++) {
+                    if (r & j) continue;
+                    const int r2 = r | j;
+                    const bool asc = k >= P ? (((lane * P) & k) == 0) : ((r & k) == 0);
+                    const K a = x[r], b = x[r2];
+                    const K lo = a < b ? a : b, hi = a < b ? b : a;
+                    x[r] = asc ? lo : hi;
+                    x[r2] = asc ? hi : lo;
+                }
+            } else {
+                const int lj = j / P;
+                const bool lower = (lane & lj) == 0;
+                const bool asc = k == M ? true : ((lane & (k / P)) == 0);
+                const bool keep_min = lower == asc;
+#pragma unroll
+                for (int r = 0; r < P; r++) {
+                    const K y = __shfl_xor(x[r], lj);
+                    x[r] = keep_min ? (x[r] < y ? x[r] : y) : (x[r] < y ? y : x[r]);
+                }
+            }
+        }
+    }
+}
+
+// Sort the d <= 64*P columns (and values) of the row starting at a.
+template <int P>
+__device__ __forceinline__ void sort_row_regs(const int *__restrict__ ctmp, const float *__restrict__ vtmp,
+                                              long long a, int d, int lane, int *__restrict__ col,
+                                              float *__restrict__ val)
+{
+    if (vtmp) {
+        unsigned long long x[P];
+#pragma unroll
+        for (int r = 0; r < P; r++) {
+            const int e = lane * P + r;
+            x[r] = e < d ? ((unsigned long long)(unsigned)ctmp[a + e] << 32) | __float_as_uint(vtmp[a + e]) : ~0ull;
+        }
+        wave_bitonic<P>(x, lane);
+#pragma unroll
+        for (int r = 0; r < P; r++) {
+            const int e = lane * P + r;
+            if (e < d) {
+                col[a + e] = (int)(x[r] >> 32);
+                val[a + e] = __uint_as_float((unsigned)x[r]);
+            }
+        }
+    } else {
+        int x[P];
+#pragma unroll
+        for (int r = 0; r < P; r++) {
+            const int e = lane * P + r;
+            x[r] = e < d ? ctmp[a + e] : INT_MAX;
+        }
+        wave_bitonic<P>(x, lane);
+#pragma unroll
+        for (int r = 0; r < P; r++) {
+            const int e = lane * P + r;
+            if (e < d) col[a + e] = x[r];
+        }
+    }
+}
+
+constexpr int ROWS_THREADS = 1024;
+constexpr int SORT_MAX = 64 * 16;  // longest row sorted in registers
+
 // One work-group per bucket: row counts -> row offsets, scatter into the
-// rows, then each wave sorts whole rows (nbm LDS bitmaps over [0, n)).
-__global__ __launch_bounds__(256) void bucket_rows(const long long *__restrict__ bstart, const unsigned *__restrict__ ent,
-                                                   const float *__restrict__ entv, int n, int nbk, int nbm,
-                                                   int *__restrict__ row, int *__restrict__ ctmp, float *__restrict__ vtmp,
-                                                   int *__restrict__ col, float *__restrict__ val)
+// rows, then each wave sorts whole rows: rows up to SORT_MAX entries by a
+// register bitonic sort, longer ones (waves wv < nbm) by LDS-bitmap rank.
+__global__ __launch_bounds__(ROWS_THREADS) void bucket_rows(const long long *__restrict__ bstart,
+                                                            const unsigned *__restrict__ ent,
+                                                            const float *__restrict__ entv, int n, int nbk, int nbm,
+                                                            int *__restrict__ row, int *__restrict__ ctmp,
+                                                            float *__restrict__ vtmp, int *__restrict__ col,
+                                                            float *__restrict__ val)
 {
     extern __shared__ uint32_t sm[];
     __shared__ unsigned rc[BROWS], rb[BROWS + 1];
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int NWAVE = ROWS_THREADS / 64;
     const long long e0 = bstart[b], e1 = bstart[b + 1];
     if (tid < BROWS) rc[tid] = 0u;
     __syncthreads();
-    for (long long k = e0 + tid; k < e1; k += 256) atomicAdd(&rc[ent[k] >> 25], 1u);
+    for (long long k = e0 + tid; k < e1; k += ROWS_THREADS) atomicAdd(&rc[ent[k] >> 25], 1u);
     __syncthreads();
     if (tid == 0) {
         unsigned run = 0;
@@ -711,7 +1152,7 @@ __global__ __launch_bounds__(256) void bucket_rows(const long long *__restrict__
     if (b == nbk - 1 && tid == 0) row[n] = (int)e1;
     if (tid < BROWS) rc[tid] = 0u;
     __syncthreads();
-    for (long long k = e0 + tid; k < e1; k += 256) {
+    for (long long k = e0 + tid; k < e1; k += ROWS_THREADS) {
         const unsigned x = ent[k];
         const unsigned lr = x >> 25;
         const long long p = e0 + rb[lr] + atomicAdd(&rc[lr], 1u);
@@ -719,23 +1160,30 @@ __global__ __launch_bounds__(256) void bucket_rows(const long long *__restrict__
         if (vtmp) vtmp[p] = entv[k];
     }
     __syncthreads();
-    // sort rows: wave wv < nbm takes rows wv, wv + nbm, ...
-    if (wv >= nbm) return;
+    // rows that fit in registers: wave wv takes rows wv, wv + NWAVE, ...
+    for (int lr = wv; lr < BROWS && r0 + lr < n; lr += NWAVE) {
+        const long long a = e0 + rb[lr];
+        const int d = (int)(rb[lr + 1] - rb[lr]);
+        if (d == 0 || d > SORT_MAX) continue;
+        if (d <= 64) sort_row_regs<1>(ctmp, vtmp, a, d, lane, col, val);
+        else if (d <= 128) sort_row_regs<2>(ctmp, vtmp, a, d, lane, col, val);
+        else if (d <= 256) sort_row_regs<4>(ctmp, vtmp, a, d, lane, col, val);
+        else if (d <= 512) sort_row_regs<8>(ctmp, vtmp, a, d, lane, col, val);
+        else sort_row_regs<16>(ctmp, vtmp, a, d, lane, col, val);
+    }
+    // longer rows: waves wv < nbm, one LDS bitmap over [0, n) each
+    if (wv >= nbm || rb[BROWS] <= (unsigned)SORT_MAX) return;
     const int nw = (n + 31) >> 5;
     uint32_t *bits = sm + (long long)wv * 2 * nw;
     uint32_t *pre = bits + nw;
+    bool clear = false;
     for (int lr = wv; lr < BROWS && r0 + lr < n; lr += nbm) {
         const long long a = e0 + rb[lr], z1 = e0 + rb[lr + 1];
-        const long long d = z1 - a;
-        if (d == 0) continue;
-        if (d == 1) {
-            if (lane == 0) {
-                col[a] = ctmp[a];
-                if (val) val[a] = vtmp[a];
-            }
-            continue;
+        if (z1 - a <= SORT_MAX) continue;
+        if (!clear) {
+            for (int q = lane; q < nw; q += 64) bits[q] = 0u;
+            clear = true;
         }
-        for (int q = lane; q < nw; q += 64) bits[q] = 0u;
         __builtin_amdgcn_wave_barrier();
         for (long long e = a + lane; e < z1; e += 64) {
             const int c = ctmp[e];
@@ -766,6 +1214,8 @@ __global__ __launch_bounds__(256) void bucket_rows(const long long *__restrict__
             if (val) val[a + pos] = vtmp[e];
         }
         __builtin_amdgcn_wave_barrier();
+        for (long long e = a + lane; e < z1; e += 64) bits[ctmp[e] >> 5] = 0u;  // leave the bitmap clear
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -774,38 +1224,54 @@ __global__ __launch_bounds__(256) void bucket_rows(const long long *__restrict__
 constexpr int NBIN2 = 16384;   // fine histogram over the emitted set
 
 // Histogram of r~ over the kept segment entries, bins over [lo, lo + NBIN2/scale).
-__global__ __launch_bounds__(256) void emitted_hist(const float *__restrict__ seg_r, long long seg_cap,
-                                                    const unsigned long long *__restrict__ seg_cnt, int nseg,
-                                                    float lo, float scale, uint32_t *__restrict__ out)
+// Segments start at multiples of 4 floats (seg_cap % 4 == 0): float4 loads.
+__global__ __launch_bounds__(1024) void emitted_hist(const float *__restrict__ seg_r, long long seg_cap,
+                                                     const unsigned long long *__restrict__ seg_cnt, int nseg,
+                                                     float lo, float scale, uint32_t *__restrict__ out)
 {
     __shared__ uint32_t h[NBIN2];
-    for (int b = threadIdx.x; b < NBIN2; b += 256) h[b] = 0;
+    for (int b = threadIdx.x; b < NBIN2; b += 1024) h[b] = 0;
     __syncthreads();
+    auto bin = [&](float r) {
+        int b = (int)((r - lo) * scale);
+        b = b < 0 ? 0 : (b >= NBIN2 ? NBIN2 - 1 : b);
+        atomicAdd(&h[b], 1u);
+    };
     for (int g = blockIdx.x; g < nseg; g += gridDim.x) {
         const long long c = min((long long)seg_cnt[g], seg_cap);
         const float *r = seg_r + (long long)g * seg_cap;
-        for (long long k = threadIdx.x; k < c; k += 256) {
-            int b = (int)((r[k] - lo) * scale);
-            b = b < 0 ? 0 : (b >= NBIN2 ? NBIN2 - 1 : b);
-            atomicAdd(&h[b], 1u);
+        const long long c4 = c >> 2;
+        for (long long k = threadIdx.x; k < c4; k += 1024) {
+            const float4 v = ((const float4 *)r)[k];
+            bin(v.x);
+            bin(v.y);
+            bin(v.z);
+            bin(v.w);
         }
+        for (long long k = (c4 << 2) + threadIdx.x; k < c; k += 1024) bin(r[k]);
     }
     __syncthreads();
     uint32_t *o = out + (long long)blockIdx.x * NBIN2;
-    for (int b = threadIdx.x; b < NBIN2; b += 256) o[b] = h[b];
+    for (int b = threadIdx.x; b < NBIN2; b += 1024) o[b] = h[b];
 }
 
+// out[b] += sum of rows [y*rpb, (y+1)*rpb) of column b (out zeroed before)
 __global__ void hist_sum(const uint32_t *__restrict__ h, int nrow, int nbin, unsigned long long *__restrict__ out)
 {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbin) return;
+    const int rpb = (nrow + gridDim.y - 1) / gridDim.y;
+    const int g0 = blockIdx.y * rpb, g1 = min(nrow, g0 + rpb);
     unsigned long long s = 0;
-    for (int g = 0; g < nrow; g++) s += h[(long long)g * nbin + b];
-    out[b] = s;
+    for (int g = g0; g < g1; g++) s += h[(long long)g * nbin + b];
+    if (s) atomicAdd(&out[b], s);
 }
 
 // Split the emitted set: r~ > hi counts as a certain edge (kept by the CSR
-// filter), lo <= r~ <= hi goes to the FP64 candidate list.
+// filter), lo <= r~ <= hi goes to the FP64 candidate list.  Block (x, g)
+// scans entries [x*WIN_CHUNK, (x+1)*WIN_CHUNK) of segment g: per-thread
+// counts, one block scan, one global reservation per block.
+constexpr int WIN_CHUNK = 8192;
 __global__ __launch_bounds__(256) void window_kernel(const int2 *__restrict__ seg, const float *__restrict__ seg_r,
                                                      long long seg_cap, const unsigned long long *__restrict__ seg_cnt,
                                                      int nseg, float lo, float hi, int2 *__restrict__ cand,
@@ -813,28 +1279,69 @@ __global__ __launch_bounds__(256) void window_kernel(const int2 *__restrict__ se
                                                      unsigned long long *__restrict__ cand_cnt,
                                                      unsigned long long *__restrict__ definite)
 {
-    __shared__ unsigned long long s_def;
-    if (threadIdx.x == 0) s_def = 0;
+    constexpr int PER = WIN_CHUNK / 256 / 4;  // float4 loads per thread
+    __shared__ unsigned s_scan[256];
+    __shared__ unsigned long long s_base;
+    const int g = blockIdx.y, tid = threadIdx.x;
+    const long long c = min((long long)seg_cnt[g], seg_cap);
+    const long long k0 = (long long)blockIdx.x * WIN_CHUNK;
+    if (k0 >= c) return;
+    const long long base = (long long)g * seg_cap;
+    float4 v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const long long k = k0 + ((long long)u * 256 + tid) * 4;
+        if (k + 3 < c) {
+            v[u] = *(const float4 *)(seg_r + base + k);
+        } else {
+            v[u].x = k < c ? seg_r[base + k] : -2.0f;
+            v[u].y = k + 1 < c ? seg_r[base + k + 1] : -2.0f;
+            v[u].z = k + 2 < c ? seg_r[base + k + 2] : -2.0f;
+            v[u].w = -2.0f;
+        }
+    }
+    unsigned nd = 0, nc = 0;
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const float e[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            nd += e[q] > hi;
+            nc += e[q] >= lo && e[q] <= hi;
+        }
+    }
+    // block exclusive scan of nc
+    s_scan[tid] = nc;
     __syncthreads();
-    unsigned long long mine = 0;
-    for (int g = blockIdx.x; g < nseg; g += gridDim.x) {
-        const long long c = min((long long)seg_cnt[g], seg_cap);
-        const long long base = (long long)g * seg_cap;
-        for (long long k = threadIdx.x; k < c; k += 256) {
-            const float r = seg_r[base + k];
-            if (r > hi) mine++;
-            else if (r >= lo) {
-                unsigned long long p = atomicAdd(cand_cnt, 1ull);
+    for (int o = 1; o < 256; o <<= 1) {
+        const unsigned y = tid >= o ? s_scan[tid - o] : 0u;
+        __syncthreads();
+        s_scan[tid] += y;
+        __syncthreads();
+    }
+    const unsigned tot = s_scan[255];
+    unsigned long long ndb = nd;
+    for (int o = 32; o > 0; o >>= 1) ndb += __shfl_xor(ndb, o);
+    if ((tid & 63) == 0 && ndb) atomicAdd(definite, ndb);
+    if (tot == 0) return;
+    if (tid == 0) s_base = atomicAdd(cand_cnt, (unsigned long long)tot);
+    __syncthreads();
+    unsigned long long p = s_base + s_scan[tid] - nc;
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const float e[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (e[q] >= lo && e[q] <= hi) {
+                const long long k = k0 + ((long long)u * 256 + tid) * 4 + q;
                 if ((long long)p < cand_cap) {
                     cand[p] = seg[base + k];
-                    cand_r[p] = r;
+                    cand_r[p] = e[q];
                 }
+                p++;
             }
         }
     }
-    atomicAdd(&s_def, mine);
-    __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(definite, s_def);
 }
 
 // Keys ordered like the reference's selection: r64 descending, then row-major.
@@ -852,9 +1359,34 @@ template <class KeyF, class WantF>
 __device__ unsigned long long radix_select(long long c, long long m, KeyF key, WantF want,
                                            unsigned *hist, unsigned long long *sh, long long *rank_left)
 {
-    unsigned long long prefix = 0, mask = 0;
+    // bytes shared by every wanted key are fixed up front: candidates sit in
+    // a narrow r window, so their top bytes agree and a digit histogram over
+    // them would serialise on one LDS bin
+    unsigned long long kmin = ~0ull, kmax = 0ull;
+    for (long long q = threadIdx.x; q < c; q += blockDim.x) {
+        if (!want(q)) continue;
+        const unsigned long long kq = key(q);
+        kmin = kq < kmin ? kq : kmin;
+        kmax = kq > kmax ? kq : kmax;
+    }
+    if (threadIdx.x == 0) {
+        sh[0] = ~0ull;
+        sh[1] = 0ull;
+    }
+    __syncthreads();
+    atomicMin(&sh[0], kmin);
+    atomicMax(&sh[1], kmax);
+    __syncthreads();
+    kmin = sh[0];
+    kmax = sh[1];
+    __syncthreads();
+    int top = 56;
+    while (top >= 0 && ((kmin ^ kmax) >> top) == 0) top -= 8;
+    unsigned long long mask = top >= 0 ? ~0ull << (top + 8) : ~0ull;
+    if (top == 56) mask = 0;
+    unsigned long long prefix = kmin & mask;
     long long mr = m;
-    for (int shift = 56; shift >= 0; shift -= 8) {
+    for (int shift = top; shift >= 0; shift -= 8) {
         for (int d = threadIdx.x; d < 256; d += blockDim.x) hist[d] = 0;
         __syncthreads();
         for (long long q = threadIdx.x; q < c; q += blockDim.x) {
@@ -1022,10 +1554,16 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     }
 
     // ---- z-score and MFMA panels
-    const int nr = (int)pg_cdiv(std::max(n, 1), TM);  // row tiles
-    const int npad = nr * TM;
-    const int nc = npad / TN;                           // column tiles
-    const int ldp = (int)pg_cdiv(t, KC) * KC;
+    // FP32: register-staged kernel, TM x TN tiles, voxel-major panel [npad][ldp].
+    // BF16x3: gram2_kernel, 256 x 128 tiles, chunk-major panels of 16-deep
+    // time chunks (at least two chunks).
+    const bool g2path = prec == PREC_BF16X3;
+    const int tm = g2path ? g2::TM : TM, tn = g2path ? g2::TN : TN, rt = tm / tn;
+    const int nr = (int)pg_cdiv(std::max(n, 1), tm);  // row tiles
+    const int npad = nr * tm;
+    const int nc = npad / tn;                           // column tiles
+    const int ldp = g2path ? (int)std::max<long long>(2 * g2::KC, pg_cdiv(t, g2::KC) * g2::KC)
+                           : (int)pg_cdiv(t, KC) * KC;
     const long long nt = (long long)t;
     double *mu, *sd, *z64;
     PG_TRY(pg_bufT(ctx, "corr.mu", n + 1, &mu));
@@ -1045,13 +1583,21 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     PG_TRY(pg_in(ctx, bold, (size_t)n * nt, "corr.bold", &dbold));
 
     PG_HIP(ctx, hipEventRecord(ctx->ev[0], st));
-    if (n > 0) {
+    if (n > 0 && t <= ZS_TMAX) {
+        const size_t shz = (size_t)t * 65 * 4;
+        PG_HIP(ctx, hipFuncSetAttribute((const void *)zscore_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shz));
+        zscore_fused<<<(unsigned)pg_cdiv(n, 64), 256, shz, st>>>(
+            dbold, opts->layout, n, t, z64, prec == PREC_F32 ? (float *)p0 : nullptr,
+            prec == PREC_BF16X3 ? (uint16_t *)p0 : nullptr, prec == PREC_BF16X3 ? (uint16_t *)p1 : nullptr, ldp,
+            npad);
+        PG_KERNEL_CHECK(ctx);
+    } else if (n > 0) {
         zscore_stats<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(dbold, opts->layout, n, t, mu, sd);
         dim3 g((unsigned)pg_cdiv(n, 64), (unsigned)pg_cdiv(t, 64));
         zscore_write<<<g, 256, 0, st>>>(dbold, opts->layout, n, t, mu, sd, z64,
                                          prec == PREC_F32 ? (float *)p0 : nullptr,
                                          prec == PREC_BF16X3 ? (uint16_t *)p0 : nullptr,
-                                         prec == PREC_BF16X3 ? (uint16_t *)p1 : nullptr, ldp);
+                                         prec == PREC_BF16X3 ? (uint16_t *)p1 : nullptr, ldp, npad);
         PG_KERNEL_CHECK(ctx);
     }
     PG_HIP(ctx, hipEventRecord(ctx->ev[1], st));
@@ -1063,8 +1609,8 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     {
         const int nsr = (int)pg_cdiv(nr, SUPER), nsc = (int)pg_cdiv(nc, SUPER);
         for (int sr = 0; sr < nsr; sr++)
-            for (int sc = std::max(0, RT * sr * SUPER / SUPER); sc < nsc; sc++)
-                if (sc * SUPER + SUPER - 1 >= RT * sr * SUPER) stl.push_back(make_int2(sr, sc));
+            for (int sc = rt * sr; sc < nsc; sc++)
+                if (sc * SUPER + SUPER - 1 >= rt * sr * SUPER) stl.push_back(make_int2(sr, sc));
     }
     int2 *dstl;
     PG_TRY(pg_bufT(ctx, "corr.stl", stl.size() + 1, &dstl));
@@ -1080,31 +1626,34 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     ga.p0 = p0;
     ga.p1 = p1;
     ga.ldp = ldp;
-    ga.kchunks = ldp / KC;
+    ga.kchunks = g2path ? ldp / g2::KC : ldp / KC;
     ga.n = n;
     ga.nr = nr;
     ga.nc = nc;
+    ga.rt = rt;
+    ga.npad = npad;
     ga.st_list = dstl;
     ga.nst = nst;
     ga.sstep = 0;
-    const size_t stage = prec == PREC_BF16X3 ? (size_t)stage_bytes<PREC_BF16X3>() : (size_t)stage_bytes<PREC_F32>();
+    const size_t stage = g2path ? (size_t)g2::NSTAGE * g2::STAGE : (size_t)stage_bytes<PREC_F32>();
+    // the histogram form of gram2_kernel holds one work-group per CU (LDS)
+    const long long nwg_hist = g2path ? std::max<long long>(NXCD, ctx->num_cu / NXCD * NXCD) : nwg;
 
     bool ran_hist = false, ran_emit = false;
     auto launch_gram = [&](int mode) -> pg_status {
         size_t shm = stage + (mode == MODE_HIST ? NBIN * 4 : 0);
         const void *fn;
-        if (prec == PREC_BF16X3)
-            fn = mode == MODE_HIST ? (const void *)gram_kernel<PREC_BF16X3, MODE_HIST>
-                                   : (const void *)gram_kernel<PREC_BF16X3, MODE_EMIT>;
+        if (g2path)
+            fn = mode == MODE_HIST ? (const void *)gram2_kernel<MODE_HIST> : (const void *)gram2_kernel<MODE_EMIT>;
         else
             fn = mode == MODE_HIST ? (const void *)gram_kernel<PREC_F32, MODE_HIST>
                                    : (const void *)gram_kernel<PREC_F32, MODE_EMIT>;
         PG_HIP(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         const int e0 = mode == MODE_HIST ? 6 : 8;
         PG_HIP(ctx, hipEventRecord(ctx->ev[e0], st));
-        if (prec == PREC_BF16X3) {
-            if (mode == MODE_HIST) gram_kernel<PREC_BF16X3, MODE_HIST><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
-            else gram_kernel<PREC_BF16X3, MODE_EMIT><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
+        if (g2path) {
+            if (mode == MODE_HIST) gram2_kernel<MODE_HIST><<<(unsigned)nwg_hist, THREADS, shm, st>>>(ga);
+            else gram2_kernel<MODE_EMIT><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
         } else {
             if (mode == MODE_HIST) gram_kernel<PREC_F32, MODE_HIST><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
             else gram_kernel<PREC_F32, MODE_EMIT><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
@@ -1133,7 +1682,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     float *band_r = nullptr, *seg_r = nullptr;
     long long seg_cap = 0, band_cap = 0;
     auto emit_pass = [&](long long seg_cap0, long long band_cap0, bool store_r) -> pg_status {
-        seg_cap = seg_cap0;
+        seg_cap = (seg_cap0 + 63) / 64 * 64;  // segments 256-byte aligned (vector loads)
         band_cap = band_cap0;
         for (int attempt = 0; attempt < 3; attempt++) {
             PG_TRY(pg_bufT(ctx, "corr.seg", (size_t)(nwg * seg_cap), &seg));
@@ -1160,7 +1709,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             for (auto c : hseg) mx = std::max<long long>(mx, (long long)c);
             if (mx <= seg_cap && (long long)hcnt[0] <= band_cap) return PG_OK;
             if (attempt == 2) break;
-            seg_cap = std::max(seg_cap, mx + mx / 8 + 4096);
+            seg_cap = (std::max(seg_cap, mx + mx / 8 + 4096) + 63) / 64 * 64;
             band_cap = std::max(band_cap, (long long)hcnt[0] + (long long)hcnt[0] / 8 + 4096);
         }
         return pg_fail(ctx, PG_ENOMEM, "edge buffers overflowed repeatedly");
@@ -1171,13 +1720,13 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     auto hist_pass = [&](long long stride) -> pg_status {
         uint32_t *hist;
         unsigned long long *htot;
-        PG_TRY(pg_bufT(ctx, "corr.hist", (size_t)nwg * NBIN, &hist));
+        PG_TRY(pg_bufT(ctx, "corr.hist", (size_t)nwg_hist * NBIN, &hist));
         PG_TRY(pg_bufT(ctx, "corr.htot", NBIN, &htot));
         ga.hist = hist;
         ga.sstep = stride;
         PG_TRY(launch_gram(MODE_HIST));
         ga.sstep = 0;
-        hist_reduce<<<NBIN / 256, 256, 0, st>>>(hist, (int)nwg, htot);
+        hist_reduce<<<NBIN / 32, 256, 0, st>>>(hist, (int)nwg_hist, htot);
         PG_KERNEL_CHECK(ctx);
         PG_HIP(ctx, hipMemcpyAsync(h.data(), htot, NBIN * 8, hipMemcpyDeviceToHost, st));
         PG_HIP(ctx, hipStreamSynchronize(st));
@@ -1270,13 +1819,14 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
                 // (3) fine histogram of the emitted r~ -> bracket the k-th
                 const double top = 1.0 + 1e-6;
                 const float scale = (float)(NBIN2 / (top - (double)lower));
-                const int hb = (int)std::min<long long>(64, nwg);
+                const int hb = (int)std::min<long long>(256, nwg);
                 uint32_t *h2;
                 unsigned long long *h2t;
                 PG_TRY(pg_bufT(ctx, "corr.h2", (size_t)hb * NBIN2, &h2));
                 PG_TRY(pg_bufT(ctx, "corr.h2t", NBIN2, &h2t));
-                emitted_hist<<<hb, 256, 0, st>>>(seg_r, seg_cap, seg_cnt, (int)nwg, lower, scale, h2);
-                hist_sum<<<NBIN2 / 256, 256, 0, st>>>(h2, hb, NBIN2, h2t);
+                PG_HIP(ctx, hipMemsetAsync(h2t, 0, NBIN2 * sizeof(unsigned long long), st));
+                emitted_hist<<<hb, 1024, 0, st>>>(seg_r, seg_cap, seg_cnt, (int)nwg, lower, scale, h2);
+                hist_sum<<<dim3(NBIN2 / 256, 16), 256, 0, st>>>(h2, hb, NBIN2, h2t);
                 PG_KERNEL_CHECK(ctx);
                 std::vector<unsigned long long> hh(NBIN2);
                 PG_HIP(ctx, hipMemcpyAsync(hh.data(), h2t, NBIN2 * 8, hipMemcpyDeviceToHost, st));
@@ -1307,7 +1857,10 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
                     PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)ccap, &extra));
                     PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
                     PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
-                    window_kernel<<<(unsigned)std::min<long long>(nwg, 1024), 256, 0, st>>>(
+                    long long mxc = 0;
+                    for (auto c : hseg) mxc = std::max<long long>(mxc, (long long)c);
+                    const unsigned wx = (unsigned)std::max<long long>(1, pg_cdiv(std::min(mxc, seg_cap), WIN_CHUNK));
+                    window_kernel<<<dim3(wx, (unsigned)nwg), 256, 0, st>>>(
                         seg, seg_r, seg_cap, seg_cnt, (int)nwg, L2, U2, cand, cand_r, ccap, counters + 0,
                         counters + 3);
                     recheck_kernel<<<(unsigned)pg_cdiv(ccap, 256), 256, 0, st>>>(cand, counters + 0, ccap, z64, t, cand64);
@@ -1390,6 +1943,8 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     PG_TRY(pg_bufT(ctx, "corr.bcnt", (size_t)nsrc * nbk, &bcnt));
     PG_TRY(pg_bufT(ctx, "corr.boff", (size_t)nsrc * nbk, &boff));
     PG_TRY(pg_bufT(ctx, "corr.bstart", (size_t)nbk + 1, &bstart));
+    long long *btot;
+    PG_TRY(pg_bufT(ctx, "corr.btot", (size_t)nbk + 1, &btot));
     PG_TRY(pg_bufT(ctx, "corr.ent", nnzs, &ent));
     PG_TRY(pg_bufT(ctx, "corr.row", (size_t)n + 2, &row));
     PG_TRY(pg_bufT(ctx, "corr.ctmp", nnzs, &ctmp));
@@ -1401,14 +1956,16 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     }
     if (n > 0) {
         const size_t shb = (size_t)nbk * 4;
-        bucket_count<<<(unsigned)nsrc, 256, shb, st>>>(es, nbk, bcnt);
-        bucket_scan<<<1, 1024, 0, st>>>(bcnt, nsrc, nbk, boff, bstart);
-        bucket_scatter<<<(unsigned)nsrc, 256, shb, st>>>(es, nbk, boff, ent, entv, z64, t);
+        bucket_count<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(es, nbk, bcnt);
+        bucket_scan_local<<<(unsigned)nbk, 256, 0, st>>>(bcnt, nsrc, boff, btot);
+        bucket_scan_global<<<1, 1024, 0, st>>>(btot, nbk, bstart);
+        bucket_scatter<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(es, nbk, boff, bstart, ent, entv, z64, t);
         int nbm = 4;
         while (nbm > 1 && (size_t)nbm * 8 * nw + 2048 > ctx->lds_per_block) nbm--;
         const size_t shr = (size_t)nbm * 8 * nw;
         PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shr));
-        bucket_rows<<<(unsigned)nbk, 256, shr, st>>>(bstart, ent, entv, n, nbk, nbm, row, ctmp, vtmp, col, val);
+        bucket_rows<<<(unsigned)nbk, ROWS_THREADS, shr, st>>>(bstart, ent, entv, n, nbk, nbm, row, ctmp, vtmp, col,
+                                                              val);
         PG_KERNEL_CHECK(ctx);
     } else {
         PG_HIP(ctx, hipMemsetAsync(row, 0, 4, st));

@@ -35,11 +35,13 @@ def _assert_same(row, col, val, orow, ocol, oval):
         np.testing.assert_array_equal(val.view(np.uint32), oval.view(np.uint32))
 
 
-def test_zscore_matches_oracle_bitwise(ctx):
-    x = _data(300, 173, 7, 11)
-    say("zscore n=300 t=173")
-    info, *_ = _build(ctx, x, 0.5)
-    z = np.frombuffer(hip_d2h(ctx.corr_z64_view(), 300 * 173 * 8), np.float64).reshape(300, 173)
+@pytest.mark.parametrize("t,layout", [(173, 0), (173, 1), (420, 0), (420, 1)])
+def test_zscore_matches_oracle_bitwise(ctx, t, layout):
+    # T <= 360 takes the LDS-staged z-score kernel, longer series the two-pass one
+    x = _data(300, t, 7, 11)
+    say(f"zscore n=300 t={t} layout={layout}")
+    info, *_ = _build(ctx, x, 0.5, layout=layout)
+    z = np.frombuffer(hip_d2h(ctx.corr_z64_view(), 300 * t * 8), np.float64).reshape(300, t)
     z64, _ = O.zscore(x)
     np.testing.assert_array_equal(z.view(np.uint64), z64.view(np.uint64))
 

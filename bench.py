@@ -211,11 +211,11 @@ def run_weighted(a):
     import pagani_amd
     from pagani_amd import synth
     w = synth.WORKLOADS["weighted_n120000_t300"]
-    n, t = w["n"], w["t"]
+    n, t = (a.voxels or w["n"]), w["t"]
     xd = torch.from_numpy(synth.bold(w["seed"], n, t, w["k"])).cuda()
     ctx = pagani_amd.Context(0)
     info, dt = _timed(ctx, lambda: ctx.corr_build(xd, n, t, 0.3, weighted=True))
-    out = {"metric": "weighted N=120000 r>0.3: construction, clustering, modularity", "n_gpus": 1,
+    out = {"metric": f"weighted N={n} r>0.3: construction, clustering, modularity", "n_gpus": 1,
            "n_voxels": n, "t": t, "nnz": info.nnz, "corr_ms": dt * 1e3}
     log(f"weighted corr: nnz={info.nnz} {dt * 1e3:.1f} ms")
     row, col, val = _device_csr(ctx, n, info.nnz, True)
@@ -248,6 +248,7 @@ def main(argv=None) -> int:
                          "and N=8192 Floyd-Warshall on the 1%% graph; cohort: 64 subjects of uneven N, T; "
                          "weighted: N=120000 r>0.3 weighted clustering + modularity")
     ap.add_argument("--subjects", type=int, default=64, help="cohort size (--config cohort)")
+    ap.add_argument("--voxels", type=int, default=0, help="voxel count override (--config weighted)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the construction path "

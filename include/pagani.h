@@ -154,7 +154,7 @@ pg_status pg_bfs(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
 /* Blocked Floyd-Warshall closure of a dense row-major n x n FP32 distance
  * matrix, in place (+inf = no edge).  Replaces cuAPSP (W/Lp/CUBFW_Lp/CUBFW_Lp/
  * CUBFW.cpp:19) and the tile kernels of CUBFW_ph1/2/3.cu.  block: tile edge
- * 64 or 128 (0 = default 128); results are bit-identical to the blocked algorithm of
+ * 64 or 128 (0 = default 64); results are bit-identical to the blocked algorithm of
  * W BFW.cpp:40-108 run with the same tile edge. */
 pg_status pg_fw(pg_ctx *ctx, float *d, int32_t n, int32_t block);
 

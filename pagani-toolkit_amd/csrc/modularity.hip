@@ -5,17 +5,21 @@
 // twins used for groups of up to 25000 nodes (Modularity_CPU.cpp:164-298).
 // The power iteration on B^(g) + beta I is one CSR SpMV (FP32 weights, FP64
 // accumulation) plus the rank-one degree term and the diagonal correction; all
-// norms and dot products are FP64 on the device.  Groups up to SMALL_N nodes
-// run the whole iteration inside one work-group (no host round trips); larger
-// groups use one kernel per step with device-side reductions.
+// norms and dot products are FP64 on the device.  Every round whose members
+// are known (a wave) iterates together: three launches per iteration cover all
+// of the wave's groups, each group stops on the device at the reference's
+// test, and the host checks the wave once per BATCH iterations.
 //
-// The bisection driver (group membership, sub-CSR, label bookkeeping) is host
-// code like the reference's Partition_GPU; groups are processed in the
-// reference's Round order so labels match it.
+// The bisection bookkeeping (group membership, sub-CSR, label order) is host
+// code like the reference's Partition_GPU and runs in the reference's round
+// order, so labels match it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -23,257 +27,211 @@
 
 namespace {
 
-constexpr int SMALL_N = 16384;
 constexpr int MAX_ITER = 10000;      // modularity_GPU.cu:25
 constexpr double EPS = 1e-6;         // modularity_GPU.cu:28
-constexpr int TPB = 1024;
+constexpr int TPB = 256;             // threads per block
+constexpr int NPB = 256;             // nodes per block
+constexpr int BATCH = 64;            // power iterations queued per host check
 
-struct Group {
-    int ng;
-    const int *gid;     // global ids (for the starting vector)
-    const int *srow;    // [ng+1]
-    const int *scol;    // local ids
-    const double *sval;
-    const double *kg;
-    const double *sumbg;
+// One wave of independent groups (the rounds created but not yet processed),
+// concatenated: group g owns nodes [no[g], no[g] + ng[g]) of the wave and
+// blocks [fb[g], fb[g] + nb[g]); a block never spans two groups, so every
+// per-group sum is a fixed-order sum of its blocks' partials.
+struct Wave {
+    int nnodes, nblk;
+    const int *bgrp;       // [nblk] group of each block
+    const int *no, *ng, *fb, *nb;   // [G]
+    const int *gid;        // [nnodes] global id (start vector)
+    const int *srow;       // [nnodes + 1] into scol/sval (wave-global)
+    const int *scol;       // wave-global node index
+    const float *sval;
+    const double *k, *sumbg;
+    double m2;
+    // per group state
+    int ngroups;
+    int *done, *stop, *iters, *it0;
+    double *beta, *vnorm, *lam;
+    // vectors and per-block partials
+    double *v, *v0, *pdot, *pss, *pe1, *pe2;
 };
 
-__device__ double block_sum1024(double v, double *red)
+__device__ double block_sum(double x, double *red)
 {
     const int tid = threadIdx.x;
-    red[tid] = v;
+    red[tid] = x;
     __syncthreads();
-    for (int off = TPB / 2; off > 0; off >>= 1) {
-        if (tid < off) red[tid] += red[tid + off];
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
         __syncthreads();
     }
-    double r = red[0];
+    const double r = red[0];
     __syncthreads();
     return r;
 }
 
-// First index of the largest |v| (ties to the lowest index, as the
-// reference's strict '>' scan).
-__device__ int block_argmax_abs(const double *v, int ng, double *redv, int *redi)
+// fixed-order sum of the group's block partials (every block gets the same value)
+__device__ double group_sum(const double *part, int f, int m, double *red)
 {
-    const int tid = threadIdx.x;
-    double best = -1.0;
-    int bi = 0x7fffffff;
-    for (int i = tid; i < ng; i += TPB) {
-        double a = fabs(v[i]);
-        if (a > best) { best = a; bi = i; }
-    }
-    redv[tid] = best;
-    redi[tid] = bi;
-    __syncthreads();
-    for (int off = TPB / 2; off > 0; off >>= 1) {
-        if (tid < off) {
-            double a = redv[tid + off];
-            int ia = redi[tid + off];
-            if (a > redv[tid] || (a == redv[tid] && ia < redi[tid])) { redv[tid] = a; redi[tid] = ia; }
-        }
-        __syncthreads();
-    }
-    int r = redi[0];
-    __syncthreads();
-    return r;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < m; q += TPB) s += part[f + q];
+    return block_sum(s, red);
 }
 
-// Whole power iteration (both the plain and the shifted pass) of one group.
-__global__ __launch_bounds__(TPB) void lead_small(Group g, double m2, double *__restrict__ v,
-                                                  double *__restrict__ v0, double *__restrict__ out)
+// One power iteration is three phases: (A) the stop test on the previous
+// iteration's partials, v0 = v and the partial of v0 . k; (B) v = B^(g) v0 +
+// beta v0 (CSR SpMV, FP32 weights, FP64 accumulation, one wave per row) and
+// the partial of |v|^2; (C) v /= |v| and the partials of |v - v0|^2 and
+// |v + v0|^2.  Work blocks b = blockIdx.x, blockIdx.x + gridDim.x, ... cover
+// the wave; every skip below is uniform in a block.
+struct PhaseSmem {
+    double red[TPB];
+    double rowv[NPB];
+    int skip;
+};
+
+__device__ __forceinline__ void phase_a(const Wave &w, int it, PhaseSmem &sm)
 {
 #pragma clang fp contract(off)
-    __shared__ double red[TPB];
-    __shared__ int redi[TPB];
-    const int tid = threadIdx.x, ng = g.ng;
-    double lam_total = 0.0;
-    int iters = 0;
-    for (int pass = 0; pass < 2; pass++) {
-        const double beta = pass == 0 ? 0.0 : -lam_total;
-        for (int i = tid; i < ng; i += TPB) v[i] = (double)g.gid[i];
-        __syncthreads();
-        double err1 = 1.0, err2 = 1.0, vnorm = 0.0;
-        int it = 0;
-        while (err1 > EPS && err2 > EPS && it < MAX_ITER) {
-            double pk = 0.0;
-            for (int i = tid; i < ng; i += TPB) {
-                double x = v[i];
-                v0[i] = x;
-                pk += x * g.kg[i];
+    for (int b = blockIdx.x; b < w.nblk; b += gridDim.x) {
+        const int g = w.bgrp[b];
+        const int li = it - w.it0[g];
+        if (w.done[g] || li < 0) continue;  // done[] is not written during this phase
+        if (li > 0) {
+            const double e1 = sqrt(group_sum(w.pe1, w.fb[g], w.nb[g], sm.red));
+            const double e2 = sqrt(group_sum(w.pe2, w.fb[g], w.nb[g], sm.red));
+            if (!(e1 > EPS && e2 > EPS) || li >= MAX_ITER) {
+                if (b == w.fb[g] && threadIdx.x == 0) {
+                    w.iters[g] = li;
+                    w.stop[g] = 1;
+                }
+                continue;
             }
-            const double vk = block_sum1024(pk, red);  // includes a barrier after v0 writes
-            double py = 0.0;
-            for (int i = tid; i < ng; i += TPB) {
-                double acc = 0.0;
-                for (int e = g.srow[i]; e < g.srow[i + 1]; e++) acc += v0[g.scol[e]] * g.sval[e];
-                acc -= vk / m2 * g.kg[i] + (g.sumbg[i] - beta) * v0[i];
-                v[i] = acc;
-                py += acc * acc;
-            }
-            vnorm = sqrt(block_sum1024(py, red));
-            double p1 = 0.0, p2 = 0.0;
-            for (int i = tid; i < ng; i += TPB) {
-                double x = v[i] / vnorm;
-                v[i] = x;
-                double a = x - v0[i], b = x + v0[i];
-                p1 += a * a;
-                p2 += b * b;
-            }
-            err1 = sqrt(block_sum1024(p1, red));
-            err2 = sqrt(block_sum1024(p2, red));
-            it++;
         }
-        iters += it;
-        const int mi = block_argmax_abs(v, ng, red, redi);
-        const double lam = (v[mi] * v0[mi] > 0) ? vnorm : -vnorm;
-        if (pass == 0) {
-            lam_total = lam;
-            // shift rule: groups above 25000 nodes use lambda < -0.01
-            // (modularity_GPU.cu:369), smaller ones lambda < 0 (Modularity_CPU.cpp:183)
-            bool shift = ng > 25000 ? (lam < -0.01) : (lam < 0.0);
-            if (!shift) break;
-        } else {
-            lam_total += lam;
+        const int i0 = w.no[g] + (b - w.fb[g]) * NPB, i1 = min(w.no[g] + w.ng[g], i0 + NPB);
+        double s = 0.0;
+        for (int i = i0 + threadIdx.x; i < i1; i += TPB) {
+            const double x = w.v[i];
+            w.v0[i] = x;
+            s += x * w.k[i];
+        }
+        s = block_sum(s, sm.red);
+        if (threadIdx.x == 0) w.pdot[b] = s;
+    }
+}
+
+__device__ __forceinline__ void phase_b(const Wave &w, int it, PhaseSmem &sm)
+{
+#pragma clang fp contract(off)
+    // the stop decisions of phase A become done flags; groups stopped in
+    // this iteration are skipped through stop[] (set before this phase)
+    for (int g = blockIdx.x * TPB + threadIdx.x; g < w.ngroups; g += gridDim.x * TPB)
+        if (w.stop[g]) w.done[g] = 1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int b = blockIdx.x; b < w.nblk; b += gridDim.x) {
+        const int g = w.bgrp[b];
+        if (threadIdx.x == 0) sm.skip = w.done[g] || w.stop[g] || it < w.it0[g];
+        __syncthreads();
+        const int skip = sm.skip;
+        __syncthreads();
+        if (skip) continue;
+        const double vk = group_sum(w.pdot, w.fb[g], w.nb[g], sm.red);
+        const double beta = w.beta[g];
+        const int i0 = w.no[g] + (b - w.fb[g]) * NPB, i1 = min(w.no[g] + w.ng[g], i0 + NPB);
+        for (int i = i0 + wv; i < i1; i += TPB / 64) {
+            double acc = 0.0;
+            for (int e = w.srow[i] + lane; e < w.srow[i + 1]; e += 64) acc += w.v0[w.scol[e]] * (double)w.sval[e];
+            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+            if (lane == 0) sm.rowv[i - i0] = acc;
+        }
+        __syncthreads();
+        double s = 0.0;
+        for (int i = i0 + threadIdx.x; i < i1; i += TPB) {
+            double acc = sm.rowv[i - i0];
+            acc -= vk / w.m2 * w.k[i] + (w.sumbg[i] - beta) * w.v0[i];
+            w.v[i] = acc;
+            s += acc * acc;
+        }
+        s = block_sum(s, sm.red);
+        if (threadIdx.x == 0) w.pss[b] = s;
+    }
+}
+
+__device__ __forceinline__ void phase_c(const Wave &w, int it, PhaseSmem &sm)
+{
+#pragma clang fp contract(off)
+    for (int b = blockIdx.x; b < w.nblk; b += gridDim.x) {
+        const int g = w.bgrp[b];
+        if (w.done[g] || w.stop[g] || it < w.it0[g]) continue;
+        const double vn = sqrt(group_sum(w.pss, w.fb[g], w.nb[g], sm.red));
+        const int i0 = w.no[g] + (b - w.fb[g]) * NPB, i1 = min(w.no[g] + w.ng[g], i0 + NPB);
+        double p1 = 0.0, p2 = 0.0;
+        for (int i = i0 + threadIdx.x; i < i1; i += TPB) {
+            const double x = w.v[i] / vn;
+            w.v[i] = x;
+            const double a = x - w.v0[i], c = x + w.v0[i];
+            p1 += a * a;
+            p2 += c * c;
+        }
+        p1 = block_sum(p1, sm.red);
+        p2 = block_sum(p2, sm.red);
+        if (threadIdx.x == 0) {
+            w.pe1[b] = p1;
+            w.pe2[b] = p2;
+            if (b == w.fb[g]) w.vnorm[g] = vn;
+        }
+    }
+}
+
+// One phase of one iteration.  (A cooperative single-launch form with grid
+// barriers between the phases measured slower: 80 us against 62 us per
+// iteration on a 1.1M-edge wave.)
+template <int P>
+__global__ __launch_bounds__(TPB) void mod_phase(Wave w, int it)
+{
+    __shared__ PhaseSmem sm;
+    if (P == 0) phase_a(w, it, sm);
+    else if (P == 1) phase_b(w, it, sm);
+    else phase_c(w, it, sm);
+}
+
+// v = global ids of the nodes of groups in list[0..m) (start of a pass)
+__global__ void mod_start(Wave w, const int *list, int m)
+{
+    const int g = list[blockIdx.y];
+    for (int i = w.no[g] + blockIdx.x * blockDim.x + threadIdx.x; i < w.no[g] + w.ng[g]; i += gridDim.x * blockDim.x)
+        w.v[i] = (double)w.gid[i];
+}
+
+// lam[g] = +-|v| by the sign of v[mi] v0[mi] at the first largest |v[mi]|
+// (the reference's strict '>' scan), for the groups in list[0..m)
+__global__ __launch_bounds__(TPB) void mod_lambda(Wave w, const int *list)
+{
+    __shared__ double rv[TPB];
+    __shared__ int ri[TPB];
+    const int g = list[blockIdx.x], tid = threadIdx.x;
+    const int a = w.no[g], z = a + w.ng[g];
+    double best = -1.0;
+    int bi = 0x7fffffff;
+    for (int i = a + tid; i < z; i += TPB) {
+        const double x = fabs(w.v[i]);
+        if (x > best) { best = x; bi = i; }
+    }
+    rv[tid] = best;
+    ri[tid] = bi;
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (tid < o) {
+            const double x = rv[tid + o];
+            const int ix = ri[tid + o];
+            if (x > rv[tid] || (x == rv[tid] && ix < ri[tid])) { rv[tid] = x; ri[tid] = ix; }
         }
         __syncthreads();
     }
     if (tid == 0) {
-        out[0] = lam_total;
-        out[1] = (double)iters;
+        const int mi = ri[0];
+        const double vn = w.vnorm[g];
+        w.lam[g] = (mi < z && w.v[mi] * w.v0[mi] > 0) ? vn : -vn;
     }
-}
-
-// ---- large groups: one kernel per step, FP64 reductions through partials
-
-__global__ void big_start(Group g, double *__restrict__ v)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < g.ng) v[i] = (double)g.gid[i];
-}
-
-// v0 = v, partial sums of v0 . k
-__global__ __launch_bounds__(TPB) void big_copy_dot(Group g, const double *__restrict__ v, double *__restrict__ v0,
-                                                    double *__restrict__ part)
-{
-#pragma clang fp contract(off)
-    __shared__ double red[TPB];
-    double s = 0.0;
-    for (int i = blockIdx.x * TPB + threadIdx.x; i < g.ng; i += gridDim.x * TPB) {
-        double x = v[i];
-        v0[i] = x;
-        s += x * g.kg[i];
-    }
-    s = block_sum1024(s, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-// scalars[slot] = sqrt?(sum of part[0..np))
-__global__ __launch_bounds__(TPB) void big_reduce(const double *__restrict__ part, int np, double *__restrict__ scal,
-                                                  int slot, int take_sqrt)
-{
-    __shared__ double red[TPB];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += TPB) s += part[i];
-    s = block_sum1024(s, red);
-    if (threadIdx.x == 0) scal[slot] = take_sqrt ? sqrt(s) : s;
-}
-
-__global__ __launch_bounds__(TPB) void big_spmv(Group g, double m2, double beta, const double *__restrict__ v0,
-                                                double *__restrict__ v, const double *__restrict__ scal,
-                                                double *__restrict__ part)
-{
-#pragma clang fp contract(off)
-    __shared__ double red[TPB];
-    const double vk = scal[0];
-    double s = 0.0;
-    for (int i = blockIdx.x * TPB + threadIdx.x; i < g.ng; i += gridDim.x * TPB) {
-        double acc = 0.0;
-        for (int e = g.srow[i]; e < g.srow[i + 1]; e++) acc += v0[g.scol[e]] * g.sval[e];
-        acc -= vk / m2 * g.kg[i] + (g.sumbg[i] - beta) * v0[i];
-        v[i] = acc;
-        s += acc * acc;
-    }
-    s = block_sum1024(s, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-__global__ __launch_bounds__(TPB) void big_normalize(int ng, double *__restrict__ v, const double *__restrict__ v0,
-                                                     const double *__restrict__ scal, double *__restrict__ part1,
-                                                     double *__restrict__ part2)
-{
-#pragma clang fp contract(off)
-    __shared__ double red[TPB];
-    const double vn = scal[1];
-    double p1 = 0.0, p2 = 0.0;
-    for (int i = blockIdx.x * TPB + threadIdx.x; i < ng; i += gridDim.x * TPB) {
-        double x = v[i] / vn;
-        v[i] = x;
-        double a = x - v0[i], b = x + v0[i];
-        p1 += a * a;
-        p2 += b * b;
-    }
-    p1 = block_sum1024(p1, red);
-    p2 = block_sum1024(p2, red);
-    if (threadIdx.x == 0) {
-        part1[blockIdx.x] = p1;
-        part2[blockIdx.x] = p2;
-    }
-}
-
-__global__ __launch_bounds__(TPB) void big_lambda(int ng, const double *__restrict__ v, const double *__restrict__ v0,
-                                                  const double *__restrict__ scal, double *__restrict__ out)
-{
-    __shared__ double red[TPB];
-    __shared__ int redi[TPB];
-    int mi = block_argmax_abs(v, ng, red, redi);
-    if (threadIdx.x == 0) out[0] = (v[mi] * v0[mi] > 0) ? scal[1] : -scal[1];
-}
-
-struct Work {
-    double *v, *v0, *part, *part2, *scal, *out;
-    int *gid, *srow, *scol;
-    double *sval, *kg, *sumbg;
-};
-
-pg_status lead_big(pg_ctx *ctx, const Group &g, double m2, Work &W, double *lam_out, int *iters)
-{
-    hipStream_t st = ctx->stream;
-    const int nblk = (int)std::min<long long>(pg_cdiv(g.ng, TPB), 1024);
-    double lam_total = 0.0;
-    for (int pass = 0; pass < 2; pass++) {
-        const double beta = pass == 0 ? 0.0 : -lam_total;
-        big_start<<<(unsigned)pg_cdiv(g.ng, 256), 256, 0, st>>>(g, W.v);
-        double err[2] = {1.0, 1.0};
-        int it = 0;
-        while (err[0] > EPS && err[1] > EPS && it < MAX_ITER) {
-            big_copy_dot<<<nblk, TPB, 0, st>>>(g, W.v, W.v0, W.part);
-            big_reduce<<<1, TPB, 0, st>>>(W.part, nblk, W.scal, 0, 0);
-            big_spmv<<<nblk, TPB, 0, st>>>(g, m2, beta, W.v0, W.v, W.scal, W.part);
-            big_reduce<<<1, TPB, 0, st>>>(W.part, nblk, W.scal, 1, 1);
-            big_normalize<<<nblk, TPB, 0, st>>>(g.ng, W.v, W.v0, W.scal, W.part, W.part2);
-            big_reduce<<<1, TPB, 0, st>>>(W.part, nblk, W.scal, 2, 1);
-            big_reduce<<<1, TPB, 0, st>>>(W.part2, nblk, W.scal, 3, 1);
-            PG_KERNEL_CHECK(ctx);
-            PG_HIP(ctx, hipMemcpyAsync(err, W.scal + 2, 16, hipMemcpyDeviceToHost, st));
-            PG_HIP(ctx, hipStreamSynchronize(st));
-            it++;
-        }
-        *iters += it;
-        big_lambda<<<1, TPB, 0, st>>>(g.ng, W.v, W.v0, W.scal, W.out);
-        double lam = 0.0;
-        PG_HIP(ctx, hipMemcpyAsync(&lam, W.out, 8, hipMemcpyDeviceToHost, st));
-        PG_HIP(ctx, hipStreamSynchronize(st));
-        if (pass == 0) {
-            lam_total = lam;
-            bool shift = g.ng > 25000 ? (lam < -0.01) : (lam < 0.0);
-            if (!shift) break;
-        } else {
-            lam_total += lam;
-        }
-    }
-    *lam_out = lam_total;
-    return PG_OK;
 }
 
 }  // namespace
@@ -311,111 +269,274 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
         k[i] = ki;
         m2 += ki;
     }
-    std::vector<int> res((size_t)n, 0), adj((size_t)2 * n + 2, 0), loc((size_t)n, -1);
-    std::vector<int> gid, srow, scol;
-    std::vector<double> sval, kg, sumbg, hv;
-    int round = 0, maxr = 0, index = 1, nrounds = 0;
+    std::vector<int> res((size_t)n, 0), adj((size_t)2 * n + 2, 0);
+    int maxr = 0, index = 1, nrounds = 0;
     if (lambda0) *lambda0 = 0.0;
     if (n == 0 || m2 <= 0.0) {
         for (int i = 0; i < n; i++) res[i] = 0;
         adj[0] = 1;
         maxr = -1;
     }
-    Work W;
-    PG_TRY(pg_bufT(ctx, "mod.v", (size_t)n + 1, &W.v));
-    PG_TRY(pg_bufT(ctx, "mod.v0", (size_t)n + 1, &W.v0));
-    PG_TRY(pg_bufT(ctx, "mod.part", 1024, &W.part));
-    PG_TRY(pg_bufT(ctx, "mod.part2", 1024, &W.part2));
-    PG_TRY(pg_bufT(ctx, "mod.scal", 8, &W.scal));
-    PG_TRY(pg_bufT(ctx, "mod.out", 4, &W.out));
-    PG_TRY(pg_bufT(ctx, "mod.gid", (size_t)n + 1, &W.gid));
-    PG_TRY(pg_bufT(ctx, "mod.srow", (size_t)n + 2, &W.srow));
-    PG_TRY(pg_bufT(ctx, "mod.scol", (size_t)nnz + 1, &W.scol));
-    PG_TRY(pg_bufT(ctx, "mod.sval", (size_t)nnz + 1, &W.sval));
-    PG_TRY(pg_bufT(ctx, "mod.kg", (size_t)n + 1, &W.kg));
-    PG_TRY(pg_bufT(ctx, "mod.sumbg", (size_t)n + 1, &W.sumbg));
-    while (round <= maxr) {
-        gid.clear();
-        double Kg = 0.0;
+
+    // Rounds are processed in waves: every round created by the previous wave
+    // has fixed membership, so the whole wave's power iterations run together;
+    // the bookkeeping (splits, new round numbers, labels) then runs in round
+    // order exactly as the sequential driver (modularity_GPU.cu:405-525).
+    int lo = 0, nwave = 0;
+    std::vector<int> rloc((size_t)n);
+    // PAGANI_MOD_TRACE=1: one line per wave on stderr (groups, nodes, edges,
+    // iterations, seconds)
+    const char *trace_env = std::getenv("PAGANI_MOD_TRACE");
+    const bool trace = trace_env && trace_env[0] == '1';
+    while (lo <= maxr) {
+        const int hi = maxr + 1;
+        const int G = hi - lo;
+        const auto wave_t0 = std::chrono::steady_clock::now();
+        // members of each round, in increasing global id
+        std::vector<int> gcount((size_t)G, 0);
         for (int i = 0; i < n; i++)
-            if (res[i] == round) {
-                loc[i] = (int)gid.size();
-                gid.push_back(i);
-                Kg += k[i];
+            if (res[i] >= lo && res[i] < hi) gcount[res[i] - lo]++;
+        std::vector<int> no((size_t)G), ng((size_t)G), fb((size_t)G), nb((size_t)G);
+        int nnodes = 0, nblk = 0;
+        for (int g = 0; g < G; g++) {
+            no[g] = nnodes;
+            ng[g] = gcount[g];
+            fb[g] = nblk;
+            nb[g] = (int)std::max<long long>(1, pg_cdiv(ng[g], NPB));
+            nnodes += ng[g];
+            nblk += nb[g];
+        }
+        std::vector<int> gid((size_t)nnodes), fill(no), bgrp((size_t)nblk);
+        for (int g = 0; g < G; g++)
+            for (int q = 0; q < nb[g]; q++) bgrp[fb[g] + q] = g;
+        for (int i = 0; i < n; i++)
+            if (res[i] >= lo && res[i] < hi) {
+                const int g = res[i] - lo;
+                rloc[i] = fill[g];
+                gid[fill[g]++] = i;
             }
-        const int ng = (int)gid.size();
-        if (ng > 0) {
-            nrounds++;
-            srow.assign((size_t)ng + 1, 0);
-            scol.clear();
-            sval.clear();
-            kg.resize(ng);
-            sumbg.resize(ng);
-            for (int a = 0; a < ng; a++) {
-                const int i = gid[a];
-                double kin = 0.0;
-                for (int e = R[i]; e < R[i + 1]; e++) {
-                    const int j = Cc[e];
-                    if (res[j] != round) continue;
-                    const double w = val ? (double)V[e] : 1.0;
-                    scol.push_back(loc[j]);
-                    sval.push_back(w);
-                    kin += (kmode == PG_MOD_BINARY_K && val) ? 1.0 : w;
-                }
-                srow[a + 1] = (int)scol.size();
-                kg[a] = k[i];
-                sumbg[a] = kin - k[i] * Kg / m2;
+        // sub-CSRs: edges inside the same round, wave-global node indices
+        std::vector<int> srow((size_t)nnodes + 1, 0), scol;
+        std::vector<float> sval;
+        std::vector<double> kw((size_t)nnodes), sumbg((size_t)nnodes), Kg((size_t)G, 0.0);
+        for (int a = 0; a < nnodes; a++) Kg[res[gid[a]] - lo] += k[gid[a]];
+        scol.reserve((size_t)nnz);
+        sval.reserve((size_t)nnz);
+        for (int a = 0; a < nnodes; a++) {
+            const int i = gid[a], r = res[i];
+            double kin = 0.0;
+            for (int e = R[i]; e < R[i + 1]; e++) {
+                const int j = Cc[e];
+                if (res[j] != r) continue;
+                const float wv = val ? V[e] : 1.0f;
+                scol.push_back(rloc[j]);
+                sval.push_back(wv);
+                kin += (kmode == PG_MOD_BINARY_K && val) ? 1.0 : (double)wv;
             }
-            const size_t gn = scol.size();
-            PG_HIP(ctx, hipMemcpyAsync(W.gid, gid.data(), (size_t)ng * 4, hipMemcpyHostToDevice, st));
-            PG_HIP(ctx, hipMemcpyAsync(W.srow, srow.data(), ((size_t)ng + 1) * 4, hipMemcpyHostToDevice, st));
-            if (gn) {
-                PG_HIP(ctx, hipMemcpyAsync(W.scol, scol.data(), gn * 4, hipMemcpyHostToDevice, st));
-                PG_HIP(ctx, hipMemcpyAsync(W.sval, sval.data(), gn * 8, hipMemcpyHostToDevice, st));
+            srow[a + 1] = (int)scol.size();
+            kw[a] = k[i];
+            sumbg[a] = kin - k[i] * Kg[r - lo] / m2;
+        }
+        const size_t gn = scol.size();
+
+        Wave w;
+        int *d_bgrp, *d_no, *d_ng, *d_fb, *d_nb, *d_gid, *d_srow, *d_scol, *d_done, *d_stop, *d_iters, *d_it0, *d_list;
+        float *d_sval;
+        double *d_k, *d_sumbg, *d_beta, *d_vnorm, *d_lam, *d_v, *d_v0, *d_pdot, *d_pss, *d_pe1, *d_pe2;
+        PG_TRY(pg_bufT(ctx, "mod.bgrp", (size_t)nblk, &d_bgrp));
+        PG_TRY(pg_bufT(ctx, "mod.no", (size_t)G, &d_no));
+        PG_TRY(pg_bufT(ctx, "mod.ng", (size_t)G, &d_ng));
+        PG_TRY(pg_bufT(ctx, "mod.fb", (size_t)G, &d_fb));
+        PG_TRY(pg_bufT(ctx, "mod.nb", (size_t)G, &d_nb));
+        PG_TRY(pg_bufT(ctx, "mod.gid", (size_t)nnodes + 1, &d_gid));
+        PG_TRY(pg_bufT(ctx, "mod.srow", (size_t)nnodes + 1, &d_srow));
+        PG_TRY(pg_bufT(ctx, "mod.scol", gn + 1, &d_scol));
+        PG_TRY(pg_bufT(ctx, "mod.sval", gn + 1, &d_sval));
+        PG_TRY(pg_bufT(ctx, "mod.k", (size_t)nnodes + 1, &d_k));
+        PG_TRY(pg_bufT(ctx, "mod.sumbg", (size_t)nnodes + 1, &d_sumbg));
+        PG_TRY(pg_bufT(ctx, "mod.done", (size_t)G, &d_done));
+        PG_TRY(pg_bufT(ctx, "mod.stop", (size_t)G, &d_stop));
+        PG_TRY(pg_bufT(ctx, "mod.iters", (size_t)G, &d_iters));
+        PG_TRY(pg_bufT(ctx, "mod.it0", (size_t)G, &d_it0));
+        PG_TRY(pg_bufT(ctx, "mod.list", (size_t)G, &d_list));
+        PG_TRY(pg_bufT(ctx, "mod.beta", (size_t)G, &d_beta));
+        PG_TRY(pg_bufT(ctx, "mod.vnorm", (size_t)G, &d_vnorm));
+        PG_TRY(pg_bufT(ctx, "mod.lam", (size_t)G, &d_lam));
+        PG_TRY(pg_bufT(ctx, "mod.v", (size_t)nnodes + 1, &d_v));
+        PG_TRY(pg_bufT(ctx, "mod.v0", (size_t)nnodes + 1, &d_v0));
+        PG_TRY(pg_bufT(ctx, "mod.pdot", (size_t)nblk, &d_pdot));
+        PG_TRY(pg_bufT(ctx, "mod.pss", (size_t)nblk, &d_pss));
+        PG_TRY(pg_bufT(ctx, "mod.pe1", (size_t)nblk, &d_pe1));
+        PG_TRY(pg_bufT(ctx, "mod.pe2", (size_t)nblk, &d_pe2));
+        auto h2d = [&](void *dst, const void *src, size_t bytes) -> pg_status {
+            if (bytes) PG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+            return PG_OK;
+        };
+        std::vector<int> zeros((size_t)G, 0);
+        std::vector<double> dzeros((size_t)G, 0.0);
+        PG_TRY(h2d(d_bgrp, bgrp.data(), (size_t)nblk * 4));
+        PG_TRY(h2d(d_no, no.data(), (size_t)G * 4));
+        PG_TRY(h2d(d_ng, ng.data(), (size_t)G * 4));
+        PG_TRY(h2d(d_fb, fb.data(), (size_t)G * 4));
+        PG_TRY(h2d(d_nb, nb.data(), (size_t)G * 4));
+        PG_TRY(h2d(d_gid, gid.data(), (size_t)nnodes * 4));
+        PG_TRY(h2d(d_srow, srow.data(), ((size_t)nnodes + 1) * 4));
+        PG_TRY(h2d(d_scol, scol.data(), gn * 4));
+        PG_TRY(h2d(d_sval, sval.data(), gn * 4));
+        PG_TRY(h2d(d_k, kw.data(), (size_t)nnodes * 8));
+        PG_TRY(h2d(d_sumbg, sumbg.data(), (size_t)nnodes * 8));
+        PG_TRY(h2d(d_done, zeros.data(), (size_t)G * 4));
+        PG_TRY(h2d(d_stop, zeros.data(), (size_t)G * 4));
+        PG_TRY(h2d(d_it0, zeros.data(), (size_t)G * 4));
+        PG_TRY(h2d(d_beta, dzeros.data(), (size_t)G * 8));
+        w.nnodes = nnodes;
+        w.nblk = nblk;
+        w.bgrp = d_bgrp;
+        w.no = d_no;
+        w.ng = d_ng;
+        w.fb = d_fb;
+        w.nb = d_nb;
+        w.gid = d_gid;
+        w.srow = d_srow;
+        w.scol = d_scol;
+        w.sval = d_sval;
+        w.k = d_k;
+        w.sumbg = d_sumbg;
+        w.m2 = m2;
+        w.ngroups = G;
+        w.done = d_done;
+        w.stop = d_stop;
+        w.iters = d_iters;
+        w.it0 = d_it0;
+        w.beta = d_beta;
+        w.vnorm = d_vnorm;
+        w.lam = d_lam;
+        w.v = d_v;
+        w.v0 = d_v0;
+        w.pdot = d_pdot;
+        w.pss = d_pss;
+        w.pe1 = d_pe1;
+        w.pe2 = d_pe2;
+
+        // pass state per group: 0 first pass, 1 shifted pass, 2 finished
+        std::vector<int> pass((size_t)G, 0), list, hit0((size_t)G, 0);
+        std::vector<double> lam_total((size_t)G, 0.0), hbeta((size_t)G, 0.0);
+        list.resize(G);
+        for (int g = 0; g < G; g++) list[g] = g;
+        PG_TRY(h2d(d_list, list.data(), (size_t)G * 4));
+        mod_start<<<dim3(4, (unsigned)G), 256, 0, st>>>(w, d_list, G);
+        PG_KERNEL_CHECK(ctx);
+        int it = 0, running = G;
+        std::vector<int> hdone((size_t)G);
+        std::vector<double> hlam((size_t)G);
+        while (running > 0) {
+            for (int q = 0; q < BATCH; q++) {
+                mod_phase<0><<<(unsigned)nblk, TPB, 0, st>>>(w, it + q);
+                mod_phase<1><<<(unsigned)nblk, TPB, 0, st>>>(w, it + q);
+                mod_phase<2><<<(unsigned)nblk, TPB, 0, st>>>(w, it + q);
             }
-            PG_HIP(ctx, hipMemcpyAsync(W.kg, kg.data(), (size_t)ng * 8, hipMemcpyHostToDevice, st));
-            PG_HIP(ctx, hipMemcpyAsync(W.sumbg, sumbg.data(), (size_t)ng * 8, hipMemcpyHostToDevice, st));
-            Group g{ng, W.gid, W.srow, W.scol, W.sval, W.kg, W.sumbg};
-            double lam = 0.0;
-            int its = 0;
-            if (ng <= SMALL_N) {
-                lead_small<<<1, TPB, 0, st>>>(g, m2, W.v, W.v0, W.out);
-                PG_KERNEL_CHECK(ctx);
-                double o[2];
-                PG_HIP(ctx, hipMemcpyAsync(o, W.out, 16, hipMemcpyDeviceToHost, st));
-                PG_HIP(ctx, hipStreamSynchronize(st));
-                lam = o[0];
-                its = (int)o[1];
-            } else {
-                PG_TRY(lead_big(ctx, g, m2, W, &lam, &its));
-            }
-            hv.resize(ng);
-            PG_HIP(ctx, hipMemcpyAsync(hv.data(), W.v, (size_t)ng * 8, hipMemcpyDeviceToHost, st));
+            it += BATCH;
+            PG_KERNEL_CHECK(ctx);
+            PG_HIP(ctx, hipMemcpyAsync(hdone.data(), d_done, (size_t)G * 4, hipMemcpyDeviceToHost, st));
             PG_HIP(ctx, hipStreamSynchronize(st));
+            list.clear();
+            for (int g = 0; g < G; g++)
+                if (pass[g] < 2 && hdone[g]) list.push_back(g);
+            if (list.empty()) continue;
+            const int m = (int)list.size();
+            PG_TRY(h2d(d_list, list.data(), (size_t)m * 4));
+            mod_lambda<<<(unsigned)m, TPB, 0, st>>>(w, d_list);
+            PG_KERNEL_CHECK(ctx);
+            PG_HIP(ctx, hipMemcpyAsync(hlam.data(), d_lam, (size_t)G * 8, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipStreamSynchronize(st));
+            std::vector<int> restart;
+            for (int g : list) {
+                const double lam = hlam[g];
+                if (pass[g] == 0) {
+                    lam_total[g] = lam;
+                    // shift rule: groups above 25000 nodes use lambda < -0.01
+                    // (modularity_GPU.cu:369), smaller ones lambda < 0
+                    // (Modularity_CPU.cpp:183)
+                    const bool shift = ng[g] > 25000 ? (lam < -0.01) : (lam < 0.0);
+                    if (shift) {
+                        pass[g] = 1;
+                        restart.push_back(g);
+                        continue;
+                    }
+                } else {
+                    lam_total[g] += lam;
+                }
+                pass[g] = 2;
+                running--;
+            }
+            if (!restart.empty()) {
+                // second pass on B + beta I, beta = -lambda, from the start vector
+                const int mr = (int)restart.size();
+                PG_TRY(h2d(d_list, restart.data(), (size_t)mr * 4));
+                mod_start<<<dim3(4, (unsigned)mr), 256, 0, st>>>(w, d_list, mr);
+                PG_KERNEL_CHECK(ctx);
+                // no iteration ran since hdone was read: it is the device state
+                for (int g : restart) {
+                    hbeta[g] = -lam_total[g];
+                    hit0[g] = it;
+                    hdone[g] = 0;
+                }
+                PG_TRY(h2d(d_beta, hbeta.data(), (size_t)G * 8));
+                PG_TRY(h2d(d_it0, hit0.data(), (size_t)G * 4));
+                PG_TRY(h2d(d_done, hdone.data(), (size_t)G * 4));
+                PG_TRY(h2d(d_stop, hdone.data(), (size_t)G * 4));  // stop == done between launches
+                PG_HIP(ctx, hipStreamSynchronize(st));
+            }
+        }
+        std::vector<double> hv((size_t)nnodes);
+        PG_HIP(ctx, hipMemcpyAsync(hv.data(), d_v, (size_t)nnodes * 8, hipMemcpyDeviceToHost, st));
+        PG_HIP(ctx, hipStreamSynchronize(st));
+        if (trace) {
+            std::vector<int> hits((size_t)G);
+            PG_HIP(ctx, hipMemcpy(hits.data(), d_iters, (size_t)G * 4, hipMemcpyDeviceToHost));
+            int mx = 0, big = 0;
+            for (int g = 0; g < G; g++) {
+                mx = std::max(mx, hits[g]);
+                big = std::max(big, ng[g]);
+            }
+            const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - wave_t0).count();
+            std::fprintf(stderr, "pg_modularity wave %d: rounds %d..%d, %d nodes (largest group %d), %zu edges, "
+                         "%d launched iterations, max %d in one pass, %.3f s\n",
+                         nwave, lo, hi - 1, nnodes, big, gn, it, mx, secs);
+        }
+        nwave++;
+
+        // bookkeeping in round order
+        for (int g = 0; g < G; g++) {
+            const int round = lo + g;
+            if (ng[g] == 0) continue;
+            nrounds++;
+            const double lam = lam_total[g];
+            const double *gv = hv.data() + no[g];
             if (round == 0) {
                 if (lambda0) *lambda0 = lam;
                 if (eig0) {
                     std::vector<double> e0((size_t)n, 0.0);
-                    for (int a = 0; a < ng; a++) e0[gid[a]] = hv[a];
+                    for (int a = 0; a < ng[g]; a++) e0[gid[no[g] + a]] = gv[a];
                     if (pg_is_device_ptr(eig0)) PG_HIP(ctx, hipMemcpy(eig0, e0.data(), (size_t)n * 8, hipMemcpyHostToDevice));
                     else std::memcpy(eig0, e0.data(), (size_t)n * 8);
                 }
             }
             long long sp = 0, sn = 0;
-            for (int a = 0; a < ng; a++) {
-                sp += hv[a] > 0;
-                sn += hv[a] <= 0;
+            for (int a = 0; a < ng[g]; a++) {
+                sp += gv[a] > 0;
+                sn += gv[a] <= 0;
             }
             const bool split = lam > 0.01 && sp > 1 && sn > 1;  // LAMBDA, MIN_GROUP
             if (split) {
-                for (int a = 0; a < ng; a++) res[gid[a]] = maxr + 1 + (hv[a] * ((double)(sp - sn) + 0.5) <= 0);
+                for (int a = 0; a < ng[g]; a++)
+                    res[gid[no[g] + a]] = maxr + 1 + (gv[a] * ((double)(sp - sn) + 0.5) <= 0);
                 maxr += 2;
             } else {
                 adj[round] = index++;
             }
-            for (int a = 0; a < ng; a++) loc[gid[a]] = -1;
         }
-        round++;
+        lo = hi;
     }
     std::vector<int> lab((size_t)n);
     for (int i = 0; i < n; i++) lab[i] = adj[res[i]];

@@ -103,8 +103,8 @@ def test_lp_and_modularity_tools_on_weighted_networks(volume):
     for f in _csr_files(w):
         base = str(w / f)[:-4]
         row, col, val = formats.read_csr(str(w / f))
-        # efficiency from the oracle's blocked closure with the tool's tile edge
-        dist = O.fw_blocked(O.fw_init(row, col, val, 0), 128)
+        # efficiency from the oracle's blocked closure with the tool's tile edge (64)
+        dist = O.fw_blocked(O.fw_init(row, col, val, 0), 64)
         inv = np.where((dist >= 0.1) & ~np.eye(n, dtype=bool), 1.0 / dist.astype(np.float64), 0.0)
         eff = (inv.sum(1) / (n - 1)).astype(np.float32)
         eff[eff < np.float32(1.0 / n)] = 0

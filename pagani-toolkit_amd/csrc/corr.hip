@@ -63,6 +63,7 @@ struct GramArgs {
     long long sstep;       // > 0: visit every sstep-th supertile only (histogram sample)
     float upper;           // r~ > upper: edge without recheck
     float lower;           // lower <= r~ <= upper: FP64 recheck
+    const float *lower_dev;  // if set, lower is read from device memory
     int2 *seg;             // [gridDim.x][seg_cap] private edge segments
     long long seg_cap;
     unsigned long long *seg_cnt;   // [gridDim.x]
@@ -462,6 +463,7 @@ __global__ __launch_bounds__(THREADS, PG_GRAM_WG_PER_CU) void gram_kernel(GramAr
     if (tid == 0) s_seg = 0;
     if (MODE == MODE_HIST)
         for (int b = tid; b < NBIN; b += THREADS) s_hist[b] = 0;
+    const float lower0 = a.lower_dev ? *a.lower_dev : a.lower;
 
     TileWalk walk;
     walk.init(a);
@@ -527,7 +529,7 @@ __global__ __launch_bounds__(THREADS, PG_GRAM_WG_PER_CU) void gram_kernel(GramAr
                         const float v = acc[bm][bn][q];
                         const int i = i0 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
                         const int j = j0 + bn * 32 + (lane & 31);
-                        bool hit = v >= a.lower;
+                        bool hit = v >= lower0;
                         if (!interior) hit = hit && (j < a.n) && (i < j);
                         const unsigned long long mh = __ballot(hit);
                         if (mh) {
@@ -546,7 +548,7 @@ __global__ __launch_bounds__(THREADS, PG_GRAM_WG_PER_CU) void gram_kernel(GramAr
                 bc = __shfl(bc, 0);
                 // opaque copies: pass 2 recomputes its ballots instead of the
                 // compiler holding all of pass 1's live (register pressure)
-                float lo = a.lower, up = a.upper;
+                float lo = lower0, up = a.upper;
                 asm volatile("" : "+s"(lo), "+s"(up));
 #pragma unroll
                 for (int bm = 0; bm < BMW; bm++)
@@ -641,9 +643,9 @@ __device__ __forceinline__ void g2_dma(const GramArgs &a, char *dst, int R, int 
 // most 8192 per wave); one reservation per wave; pass 2 has each lane write
 // its own hits in order from its exclusive prefix.  BOUNDED: every pair is
 // inside the matrix and above the diagonal.
-template <bool BOUNDED>
-__device__ __forceinline__ void g2_emit(const GramArgs &a, const f32x16 (&acc)[g2::BMW][g2::BNW], int i0, int j0,
-                                        int lane, int2 *myseg, unsigned long long *s_seg)
+template <bool BOUNDED, bool SPLIT>
+__device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f32x16 (&acc)[g2::BMW][g2::BNW], int i0,
+                                        int j0, int lane, int2 *myseg, unsigned long long *s_seg)
 {
     const int ib0 = i0 + (lane >> 5) * 4, jb0 = j0 + (lane & 31);
     unsigned cnt = 0;
@@ -654,12 +656,13 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, const f32x16 (&acc)[g
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const float v = acc[bm][bn][q];
-                bool hit = v >= a.lower;
+                bool hit = v >= lower;
                 if (!BOUNDED) {
                     const int i = ib0 + bm * 32 + (q >> 2) * 8 + (q & 3), j = jb0 + bn * 32;
                     hit = hit & (j < a.n) & (i < j);
                 }
-                cnt += hit ? (v > a.upper ? 1u : 0x10000u) : 0u;
+                if (SPLIT) cnt += hit ? (v > a.upper ? 1u : 0x10000u) : 0u;
+                else cnt += hit ? 1u : 0u;
             }
     unsigned x = cnt;
 #pragma unroll
@@ -679,7 +682,7 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, const f32x16 (&acc)[g
     bc = __shfl(bc, 0) + (ex >> 16);
     // opaque copies: pass 2 recomputes its predicates instead of the compiler
     // keeping pass 1's 128 of them live
-    float lo = a.lower, up = a.upper;
+    float lo = lower, up = a.upper;
     int ib = ib0, jb = jb0, nn = a.n;
     asm volatile("" : "+v"(lo), "+v"(up), "+v"(ib), "+v"(jb), "+v"(nn));
     float *segr = a.seg_r ? a.seg_r + (long long)blockIdx.x * a.seg_cap : nullptr;
@@ -694,7 +697,7 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, const f32x16 (&acc)[g
                 bool hit = v >= lo;
                 if (!BOUNDED) hit = hit & (j < nn) & (i < j);
                 if (hit) {
-                    if (v > up) {
+                    if (!SPLIT || v > up) {
                         if ((long long)be < a.seg_cap) {
                             myseg[be] = make_int2(i, j);
                             if (segr) segr[be] = v;
@@ -726,6 +729,9 @@ __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
     if (tid == 0) s_seg = 0;
     if (MODE == MODE_HIST)
         for (int b = tid; b < NBIN; b += THREADS) s_hist[b] = 0;
+    const float lower = a.lower_dev ? *a.lower_dev : a.lower;
+    // upper below -1: every hit is an edge (no recheck band)
+    const bool split = a.upper >= -1.0f;
 
     // per-lane operand offsets inside a stage (row l % 32, half l / 32)
     const int lr = lane & 31, lh = lane >> 5;
@@ -820,10 +826,12 @@ __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
                             atomicAdd(&s_hist[b], 1u);
                         }
                     }
-        } else if (interior) {
-            g2_emit<true>(a, acc, i0, j0, lane, myseg, &s_seg);
+        } else if (split) {
+            if (interior) g2_emit<true, true>(a, lower, acc, i0, j0, lane, myseg, &s_seg);
+            else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, &s_seg);
         } else {
-            g2_emit<false>(a, acc, i0, j0, lane, myseg, &s_seg);
+            if (interior) g2_emit<true, false>(a, lower, acc, i0, j0, lane, myseg, &s_seg);
+            else g2_emit<false, false>(a, lower, acc, i0, j0, lane, myseg, &s_seg);
         }
         R = R1;
         C = C1;
@@ -916,16 +924,24 @@ struct EdgeSrc {
     const unsigned long long *extra_cnt;
     const float *seg_r;   // optional filter: keep segment entries with r~ > keep_above
     float keep_above;
+    const float *keep_above_dev;  // if set, the bound is read from device memory
+    long long extra_cap;
 };
 
-__device__ __forceinline__ bool src_keep(const EdgeSrc &s, int g, long long k)
+__device__ __forceinline__ float src_bound(const EdgeSrc &s)
 {
-    return !(s.seg_r && g < s.nseg && s.seg_r[(long long)g * s.seg_cap + k] <= s.keep_above);
+    return s.keep_above_dev ? *s.keep_above_dev : s.keep_above;
 }
 
+__device__ __forceinline__ bool src_keep(const EdgeSrc &s, int g, long long k, float ka)
+{
+    return !(s.seg_r && g < s.nseg && s.seg_r[(long long)g * s.seg_cap + k] <= ka);
+}
+
+// entries of source g, clamped to what its buffer holds
 __device__ __forceinline__ long long src_count(const EdgeSrc &s, int g)
 {
-    return g < s.nseg ? (long long)s.seg_cnt[g] : (long long)*s.extra_cnt;
+    return g < s.nseg ? min((long long)s.seg_cnt[g], s.seg_cap) : min((long long)*s.extra_cnt, s.extra_cap);
 }
 __device__ __forceinline__ int2 src_edge(const EdgeSrc &s, int g, long long k)
 {
@@ -951,8 +967,9 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, 
     __syncthreads();
     const int g = blockIdx.x;
     const long long c = src_count(s, g);
+    const float ka = src_bound(s);
     for (long long k = threadIdx.x; k < c; k += SEG_THREADS) {
-        if (!src_keep(s, g, k)) continue;
+        if (!src_keep(s, g, k, ka)) continue;
         const int2 e = src_edge(s, g, k);
         atomicAdd(&sc[e.x >> BSH], 1u);
         atomicAdd(&sc[e.y >> BSH], 1u);
@@ -1015,18 +1032,22 @@ __global__ __launch_bounds__(1024) void bucket_scan_global(const long long *__re
     if (tid == 1023) bstart[nbk] = part[1023];
 }
 
+// Entries beyond `cap` (a selection that did not hold; the caller checks
+// row[n] and rebuilds) are dropped rather than written.
 __global__ __launch_bounds__(SEG_THREADS) void bucket_scatter(EdgeSrc s, int nbk, const long long *__restrict__ off,
                                                               const long long *__restrict__ bstart,
                                                               unsigned *__restrict__ ent, float *__restrict__ entv,
-                                                              const double *__restrict__ z, int t)
+                                                              const double *__restrict__ z, int t, long long cap)
 {
     extern __shared__ unsigned sc[];  // per-bucket cursors
     for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) sc[b] = 0u;
     __syncthreads();
     const int g = blockIdx.x, nsrc = gridDim.x;
     const long long c = src_count(s, g);
+    const float ka = src_bound(s);
+    if (bstart[nbk] > cap) return;
     for (long long k = threadIdx.x; k < c; k += SEG_THREADS) {
-        if (!src_keep(s, g, k)) continue;
+        if (!src_keep(s, g, k, ka)) continue;
         const int2 e = src_edge(s, g, k);
         const int bf = e.x >> BSH, br = e.y >> BSH;
         const long long pf = bstart[bf] + off[(long long)bf * nsrc + g] + atomicAdd(&sc[bf], 1u);
@@ -1127,12 +1148,16 @@ __global__ __launch_bounds__(ROWS_THREADS) void bucket_rows(const long long *__r
                                                             const float *__restrict__ entv, int n, int nbk, int nbm,
                                                             int *__restrict__ row, int *__restrict__ ctmp,
                                                             float *__restrict__ vtmp, int *__restrict__ col,
-                                                            float *__restrict__ val)
+                                                            float *__restrict__ val, long long cap)
 {
     extern __shared__ uint32_t sm[];
     __shared__ unsigned rc[BROWS], rb[BROWS + 1];
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     constexpr int NWAVE = ROWS_THREADS / 64;
+    if (bstart[nbk] > cap) {  // nothing was scattered; row[n] reports the total
+        if (b == nbk - 1 && tid == 0) row[n] = (int)min(bstart[nbk], (long long)INT_MAX);
+        return;
+    }
     const long long e0 = bstart[b], e1 = bstart[b + 1];
     if (tid < BROWS) rc[tid] = 0u;
     __syncthreads();
@@ -1219,17 +1244,120 @@ __global__ __launch_bounds__(ROWS_THREADS) void bucket_rows(const long long *__r
     }
 }
 
+// float nearest to x on the far side (outward rounding of window bounds)
+__host__ __device__ inline float round_up_f(double x)
+{
+    float f = (float)x;
+    if ((double)f < x) f = nextafterf(f, INFINITY);
+    return f;
+}
+__host__ __device__ inline float round_down_f(double x)
+{
+    float f = (float)x;
+    if ((double)f > x) f = nextafterf(f, -INFINITY);
+    return f;
+}
+
 // ------------------------------------------------------ sparsity selection
 
 constexpr int NBIN2 = 16384;   // fine histogram over the emitted set
+
+// Device-side decisions of the single-pass sparsity selection, so the pass
+// runs without host round trips.
+struct SelCtl {
+    float lower;   // emit bound L
+    float L2, U2;  // FP64 candidate window on r~
+    int ok;        // cleared by any step whose assumption fails
+};
+
+// Largest bin b of h[0..nb) with sum_{b' >= b} h[b'] >= target (-1 if none),
+// by one block: per-thread bin runs, a suffix scan over threads, one scan of
+// the crossing run.
+__device__ int top_bin_reaching(const unsigned long long *h, int nb, double target, unsigned long long *total)
+{
+    __shared__ unsigned long long sfx[1024];
+    __shared__ int s_bin;
+    const int tid = threadIdx.x, per = nb / 1024;
+    unsigned long long own = 0;
+    for (int q = 0; q < per; q++) own += h[tid * per + q];
+    sfx[tid] = own;
+    if (tid == 0) s_bin = -1;
+    __syncthreads();
+    // inclusive suffix sums: sfx[t] = sum over threads t' >= t
+    for (int o = 1; o < 1024; o <<= 1) {
+        const unsigned long long y = tid + o < 1024 ? sfx[tid + o] : 0ull;
+        __syncthreads();
+        sfx[tid] += y;
+        __syncthreads();
+    }
+    const unsigned long long above = sfx[tid] - own;  // threads after this one
+    if ((double)(above + own) >= target && (double)above < target) {
+        unsigned long long cum = above;
+        for (int q = per - 1; q >= 0; q--) {
+            cum += h[tid * per + q];
+            if ((double)cum >= target) {
+                s_bin = tid * per + q;
+                break;
+            }
+        }
+    }
+    if (target <= 0.0 && tid == 1023) s_bin = nb - 1;
+    __syncthreads();
+    if (total) *total = sfx[0];
+    const int r = s_bin;
+    __syncthreads();  // shared state is reused by the next call
+    return r;
+}
+
+// L from the sampled histogram: the bin where the top fraction f of the
+// sampled pairs is reached, minus one bin edge (bins of width 2/NBIN on [-1, 1]).
+__global__ __launch_bounds__(1024) void sample_bound(const unsigned long long *__restrict__ htot, double f,
+                                                     SelCtl *__restrict__ ctl)
+{
+    unsigned long long tot = 0;
+    // the target needs the total: a first call (target 0) returns it
+    top_bin_reaching(htot, NBIN, 0.0, &tot);
+    const double target = f * (double)tot;
+    const int bl = top_bin_reaching(htot, NBIN, target, nullptr);
+    if (threadIdx.x == 0) {
+        const bool ok = bl > 0 && tot > 0;
+        ctl->lower = ok ? round_down_f(bl * (2.0 / NBIN) - 1.0 - 1e-6) : 2.0f;  // 2: emit nothing
+        ctl->ok = ok ? 1 : 0;
+    }
+}
+
+constexpr double FINE_TOP = 1.0 + 1e-6;  // upper end of the fine histogram
+
+// Candidate window from the fine histogram of the emitted r~: the bin holding
+// the k-th largest, widened by twice the error band on each side.
+__global__ __launch_bounds__(1024) void window_bounds(const unsigned long long *__restrict__ h2, long long kwant,
+                                                      double band, SelCtl *__restrict__ ctl)
+{
+    unsigned long long tot = 0;
+    const int b2 = top_bin_reaching(h2, NBIN2, (double)kwant, &tot);
+    if (threadIdx.x == 0) {
+        const float lower = ctl->lower;
+        const float scale = (float)(NBIN2 / (FINE_TOP - (double)lower));
+        const double w2 = 1.0 / (double)scale;
+        const double lo2 = (double)lower + b2 * w2 - 1e-6;
+        const double hi2 = (double)lower + (b2 + 1) * w2 + 1e-6;
+        const float L2 = round_down_f(lo2 - 2.0 * band);
+        const float U2 = round_up_f(hi2 + 2.0 * band);
+        ctl->L2 = L2;
+        ctl->U2 = U2;
+        if (!(b2 >= 0 && (long long)tot >= kwant && (double)L2 >= (double)lower)) ctl->ok = 0;
+    }
+}
 
 // Histogram of r~ over the kept segment entries, bins over [lo, lo + NBIN2/scale).
 // Segments start at multiples of 4 floats (seg_cap % 4 == 0): float4 loads.
 __global__ __launch_bounds__(1024) void emitted_hist(const float *__restrict__ seg_r, long long seg_cap,
                                                      const unsigned long long *__restrict__ seg_cnt, int nseg,
-                                                     float lo, float scale, uint32_t *__restrict__ out)
+                                                     const SelCtl *__restrict__ ctl, uint32_t *__restrict__ out)
 {
     __shared__ uint32_t h[NBIN2];
+    const float lo = ctl->lower;
+    const float scale = (float)(NBIN2 / (FINE_TOP - (double)lo));
     for (int b = threadIdx.x; b < NBIN2; b += 1024) h[b] = 0;
     __syncthreads();
     auto bin = [&](float r) {
@@ -1274,15 +1402,17 @@ __global__ void hist_sum(const uint32_t *__restrict__ h, int nrow, int nbin, uns
 constexpr int WIN_CHUNK = 8192;
 __global__ __launch_bounds__(256) void window_kernel(const int2 *__restrict__ seg, const float *__restrict__ seg_r,
                                                      long long seg_cap, const unsigned long long *__restrict__ seg_cnt,
-                                                     int nseg, float lo, float hi, int2 *__restrict__ cand,
-                                                     float *__restrict__ cand_r, long long cand_cap,
-                                                     unsigned long long *__restrict__ cand_cnt,
+                                                     int nseg, const SelCtl *__restrict__ ctl,
+                                                     int2 *__restrict__ cand, float *__restrict__ cand_r,
+                                                     long long cand_cap, unsigned long long *__restrict__ cand_cnt,
                                                      unsigned long long *__restrict__ definite)
 {
     constexpr int PER = WIN_CHUNK / 256 / 4;  // float4 loads per thread
     __shared__ unsigned s_scan[256];
     __shared__ unsigned long long s_base;
     const int g = blockIdx.y, tid = threadIdx.x;
+    if (!ctl->ok) return;
+    const float lo = ctl->L2, hi = ctl->U2;
     const long long c = min((long long)seg_cnt[g], seg_cap);
     const long long k0 = (long long)blockIdx.x * WIN_CHUNK;
     if (k0 >= c) return;
@@ -1493,20 +1623,32 @@ double error_band(int prec, int t)
     return 1.5 * (z32 + acc + f64);
 }
 
-float round_up_f(double x)
-{
-    float f = (float)x;
-    if ((double)f < x) f = std::nextafter(f, INFINITY);
-    return f;
-}
-float round_down_f(double x)
-{
-    float f = (float)x;
-    if ((double)f > x) f = std::nextafter(f, -INFINITY);
-    return f;
-}
+
 
 }  // namespace
+
+// Pairs i < j < n inside the tiles of every stride-th supertile of the list:
+// what a histogram pass with that sampling stride counts.
+long long sampled_pairs(const std::vector<int2> &stl, long long stride, int nr, int nc, int rt, int tm, int tn,
+                        int n)
+{
+    long long tot = 0;
+    for (long long q = 0; q < (long long)stl.size(); q += stride) {
+        const int2 sc = stl[(size_t)q];
+        for (int a = 0; a < SUPER; a++)
+            for (int b = 0; b < SUPER; b++) {
+                const int r = sc.x * SUPER + a, c = sc.y * SUPER + b;
+                if (r >= nr || c >= nc || c < rt * r) continue;
+                const long long c0 = (long long)c * tn, c1 = std::min<long long>(n, c0 + tn);
+                const long long i1 = std::min<long long>(n, (long long)r * tm + tm);
+                for (long long i = (long long)r * tm; i < i1; i++) {
+                    const long long lo = std::max(i + 1, c0);
+                    if (c1 > lo) tot += c1 - lo;
+                }
+            }
+    }
+    return tot;
+}
 
 extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, int32_t t,
                                    const pg_corr_opts *opts, pg_corr_info *info)
@@ -1717,9 +1859,35 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
 
     // Histogram pass over all tiles (stride 0) or a strided sample of tiles.
     std::vector<unsigned long long> h(NBIN);
-    auto hist_pass = [&](long long stride) -> pg_status {
+    // One emitting Gram pass without a host check (the caller checks seg_cnt).
+    auto emit_launch = [&](long long seg_cap0, long long band_cap0, bool store_r) -> pg_status {
+        seg_cap = (seg_cap0 + 63) / 64 * 64;
+        band_cap = band_cap0;
+        PG_TRY(pg_bufT(ctx, "corr.seg", (size_t)(nwg * seg_cap), &seg));
+        if (store_r) PG_TRY(pg_bufT(ctx, "corr.segr", (size_t)(nwg * seg_cap), &seg_r));
+        else seg_r = nullptr;
+        PG_TRY(pg_bufT(ctx, "corr.band", (size_t)band_cap, &bandl));
+        PG_TRY(pg_bufT(ctx, "corr.bandr", (size_t)band_cap, &band_r));
+        PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
+        PG_HIP(ctx, hipMemsetAsync(seg_cnt, 0, nwg * sizeof(unsigned long long), st));
+        ga.seg = seg;
+        ga.seg_r = seg_r;
+        ga.seg_cap = seg_cap;
+        ga.seg_cnt = seg_cnt;
+        ga.band = bandl;
+        ga.band_r = band_r;
+        ga.band_cap = band_cap;
+        ga.band_cnt = counters + 0;
+        ga.sstep = 0;
+        if (n > 1) PG_TRY(launch_gram(MODE_EMIT));
+        return PG_OK;
+    };
+
+    // Histogram Gram pass over all tiles (stride 0) or a strided sample of
+    // tiles, reduced into htot on the device.
+    unsigned long long *htot = nullptr;
+    auto hist_launch = [&](long long stride) -> pg_status {
         uint32_t *hist;
-        unsigned long long *htot;
         PG_TRY(pg_bufT(ctx, "corr.hist", (size_t)nwg_hist * NBIN, &hist));
         PG_TRY(pg_bufT(ctx, "corr.htot", NBIN, &htot));
         ga.hist = hist;
@@ -1728,8 +1896,71 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         ga.sstep = 0;
         hist_reduce<<<NBIN / 32, 256, 0, st>>>(hist, (int)nwg_hist, htot);
         PG_KERNEL_CHECK(ctx);
+        return PG_OK;
+    };
+    auto hist_pass = [&](long long stride) -> pg_status {
+        PG_TRY(hist_launch(stride));
         PG_HIP(ctx, hipMemcpyAsync(h.data(), htot, NBIN * 8, hipMemcpyDeviceToHost, st));
         PG_HIP(ctx, hipStreamSynchronize(st));
+        return PG_OK;
+    };
+
+    // ---- CSR build (bucketed counting sort), launched without a host check
+    const int nbk = (int)pg_cdiv(std::max(n, 1), BROWS);
+    const int nsrc = (int)nwg + 1;  // segments + the accepted/selected list
+    int *row = nullptr, *col = nullptr;
+    float *val = nullptr;
+    auto launch_csr = [&](const EdgeSrc &src, long long nnz) -> pg_status {
+        if (nnz > INT32_MAX) return pg_fail(ctx, PG_ENOTSUP, "nnz %lld exceeds int32 CSR", (long long)nnz);
+        const size_t nnzs = (size_t)std::max<long long>(nnz, 1);
+        unsigned *bcnt, *ent;
+        long long *boff, *bstart, *btot;
+        int *ctmp;
+        float *entv = nullptr, *vtmp = nullptr;
+        PG_TRY(pg_bufT(ctx, "corr.bcnt", (size_t)nsrc * nbk, &bcnt));
+        PG_TRY(pg_bufT(ctx, "corr.boff", (size_t)nsrc * nbk, &boff));
+        PG_TRY(pg_bufT(ctx, "corr.bstart", (size_t)nbk + 1, &bstart));
+        PG_TRY(pg_bufT(ctx, "corr.btot", (size_t)nbk + 1, &btot));
+        PG_TRY(pg_bufT(ctx, "corr.ent", nnzs, &ent));
+        PG_TRY(pg_bufT(ctx, "corr.row", (size_t)n + 2, &row));
+        PG_TRY(pg_bufT(ctx, "corr.ctmp", nnzs, &ctmp));
+        PG_TRY(pg_bufT(ctx, "corr.col", nnzs, &col));
+        val = nullptr;
+        if (weighted) {
+            PG_TRY(pg_bufT(ctx, "corr.entv", nnzs, &entv));
+            PG_TRY(pg_bufT(ctx, "corr.vtmp", nnzs, &vtmp));
+            PG_TRY(pg_bufT(ctx, "corr.val", nnzs, &val));
+        }
+        if (n > 0) {
+            const size_t shb = (size_t)nbk * 4;
+            bucket_count<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(src, nbk, bcnt);
+            bucket_scan_local<<<(unsigned)nbk, 256, 0, st>>>(bcnt, nsrc, boff, btot);
+            bucket_scan_global<<<1, 1024, 0, st>>>(btot, nbk, bstart);
+            bucket_scatter<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(src, nbk, boff, bstart, ent, entv, z64, t,
+                                                                     (long long)nnzs);
+            int nbm = 4;
+            while (nbm > 1 && (size_t)nbm * 8 * nw + 2048 > ctx->lds_per_block) nbm--;
+            const size_t shr = (size_t)nbm * 8 * nw;
+            PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)shr));
+            bucket_rows<<<(unsigned)nbk, ROWS_THREADS, shr, st>>>(bstart, ent, entv, n, nbk, nbm, row, ctmp, vtmp,
+                                                                  col, val, (long long)nnzs);
+            PG_KERNEL_CHECK(ctx);
+        } else {
+            PG_HIP(ctx, hipMemsetAsync(row, 0, 4, st));
+        }
+        return PG_OK;
+    };
+    // launch, wait, and check row[n]
+    auto finish_csr = [&](const EdgeSrc &src, long long nnz) -> pg_status {
+        PG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+        PG_TRY(launch_csr(src, nnz));
+        PG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+        PG_HIP(ctx, hipStreamSynchronize(st));
+        int rn = 0;
+        PG_HIP(ctx, hipMemcpy(&rn, row + n, 4, hipMemcpyDeviceToHost));
+        if ((long long)rn != nnz)
+            return pg_fail(ctx, PG_EFORMAT, "CSR build mismatch: row[n]=%d nnz=%lld", rn, (long long)nnz);
         return PG_OK;
     };
 
@@ -1738,6 +1969,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     EdgeSrc es;
     std::memset(&es, 0, sizeof es);
     float ms_hist = 0.0f;
+    ga.lower_dev = nullptr;
 
     if (!smode) {
         // ---------------------------------------------------- r threshold
@@ -1769,6 +2001,10 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         es.seg_cap = seg_cap;
         es.seg_cnt = seg_cnt;
         es.nseg = (int)nwg;
+        es.extra = extra;
+        es.extra_cnt = extra_cnt;
+        es.extra_cap = std::max<long long>(nband, 1);
+        PG_TRY(finish_csr(es, inf.nnz));
     } else if (kwant == 0 || n < 2) {
         // ---------------------------------------------------- nothing asked
         PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
@@ -1781,109 +2017,98 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         es.seg_cap = 0;
         es.seg_cnt = seg_cnt;
         es.nseg = (int)nwg;
+        es.extra = extra;
+        es.extra_cnt = extra_cnt;
+        es.extra_cap = 1;
         inf.nnz = 0;
         inf.r_thresh = 1.0f;
         inf.t_star = 1.0;
+        PG_TRY(finish_csr(es, 0));
     } else {
         // ---------------------------------------------------- sparsity
         const double w = 2.0 / NBIN;
         bool done = false;
-        // (1) sampled histogram -> conservative lower bound L on the k-th r~
         // about 2% of the supertiles, when there are enough to sample
         const long long stride = nst >= 8 ? std::max<long long>(2, std::min<long long>(50, nst / 16)) : 1;
-        PG_TRY(hist_pass(stride));
-        unsigned long long msamp = 0;
-        for (auto c : h) msamp += c;
-        const double p = (double)kwant / (double)m1;
-        const double meff = std::max(1.0, (double)msamp / 64.0);  // pairs inside a tile are correlated
-        const double f = std::min(1.0, 1.15 * p + 6.0 * std::sqrt(p * (1.0 - p) / meff) + 1.0 / meff);
-        int bl = -1;
-        {
-            unsigned long long cum = 0;
-            for (int b = NBIN - 1; b >= 0; b--) {
-                cum += h[b];
-                if ((double)cum >= f * (double)msamp) { bl = b; break; }
-            }
-        }
-        PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-        if (stride > 1 && bl > 0 && msamp > 0) {
-            const float lower = round_down_f(bl * w - 1.0 - 1e-6);
-            // (2) one emitting pass: every pair with r~ >= lower, with its r~
-            ga.lower = lower;
+        if (stride > 1) {
+            // Single pass, no host round trip until the end:
+            // (1) sampled histogram -> a conservative lower bound L on the
+            //     k-th largest r~ (device: sample_bound);
+            // (2) one emitting pass keeps every pair with r~ >= L and its r~;
+            // (3) fine histogram of the kept r~ -> the bin of the k-th,
+            //     widened by 2 bands each side (device: window_bounds);
+            // (4) candidates in the window, FP64 recheck, exact selection;
+            // (5) CSR build; then one check of every step's assumption.
+            const long long msamp = sampled_pairs(stl, stride, nr, nc, rt, tm, tn, n);
+            const double p = (double)kwant / (double)m1;
+            const double meff = std::max(1.0, (double)msamp / 64.0);  // pairs inside a tile are correlated
+            const double f = std::min(1.0, 1.15 * p + 6.0 * std::sqrt(p * (1.0 - p) / meff) + 1.0 / meff);
+            SelCtl *ctl;
+            PG_TRY(pg_bufT(ctx, "corr.selctl", 1, &ctl));
+            PG_TRY(hist_launch(stride));
+            sample_bound<<<1, 1024, 0, st>>>(htot, f, ctl);
+            PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+            ga.lower_dev = &ctl->lower;
             ga.upper = -4.0f;  // every hit is a segment entry
             const double expect = std::min(1.0, f * 1.2 + 2.0 * w) * (double)m1;
-            PG_TRY(emit_pass(pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096, 1 << 10, true));
-            long long emitted = 0;
-            for (auto c : hseg) emitted += (long long)c;
-            if (emitted >= kwant) {
-                // (3) fine histogram of the emitted r~ -> bracket the k-th
-                const double top = 1.0 + 1e-6;
-                const float scale = (float)(NBIN2 / (top - (double)lower));
-                const int hb = (int)std::min<long long>(256, nwg);
-                uint32_t *h2;
-                unsigned long long *h2t;
-                PG_TRY(pg_bufT(ctx, "corr.h2", (size_t)hb * NBIN2, &h2));
-                PG_TRY(pg_bufT(ctx, "corr.h2t", NBIN2, &h2t));
-                PG_HIP(ctx, hipMemsetAsync(h2t, 0, NBIN2 * sizeof(unsigned long long), st));
-                emitted_hist<<<hb, 1024, 0, st>>>(seg_r, seg_cap, seg_cnt, (int)nwg, lower, scale, h2);
-                hist_sum<<<dim3(NBIN2 / 256, 16), 256, 0, st>>>(h2, hb, NBIN2, h2t);
-                PG_KERNEL_CHECK(ctx);
-                std::vector<unsigned long long> hh(NBIN2);
-                PG_HIP(ctx, hipMemcpyAsync(hh.data(), h2t, NBIN2 * 8, hipMemcpyDeviceToHost, st));
-                PG_HIP(ctx, hipStreamSynchronize(st));
-                int b2 = -1;
-                unsigned long long cum = 0;
-                for (int b = NBIN2 - 1; b >= 0; b--) {
-                    cum += hh[b];
-                    if ((long long)cum >= kwant) { b2 = b; break; }
-                }
-                const double w2 = 1.0 / (double)scale;
-                const double lo2 = (double)lower + b2 * w2 - 1e-6;
-                const double hi2 = (double)lower + (b2 + 1) * w2 + 1e-6;
-                const float L2 = round_down_f(lo2 - 2.0 * band);
-                const float U2 = round_up_f(hi2 + 2.0 * band);
-                // the window must lie inside the emitted range
-                if (b2 >= 0 && (double)L2 >= (double)lower) {
-                    long long ccap = 1024;
-                    const int blo = std::max(0, (int)(((double)L2 - lower) * scale) - 2);
-                    const int bhi = std::min(NBIN2 - 1, (int)(((double)U2 - lower) * scale) + 2);
-                    for (int b = blo; b <= bhi; b++) ccap += (long long)hh[b];
-                    int2 *cand;
-                    float *cand_r;
-                    double *cand64;
-                    PG_TRY(pg_bufT(ctx, "corr.band", (size_t)ccap, &cand));
-                    PG_TRY(pg_bufT(ctx, "corr.bandr", (size_t)ccap, &cand_r));
-                    PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)ccap, &cand64));
-                    PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)ccap, &extra));
-                    PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
-                    PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
-                    long long mxc = 0;
-                    for (auto c : hseg) mxc = std::max<long long>(mxc, (long long)c);
-                    const unsigned wx = (unsigned)std::max<long long>(1, pg_cdiv(std::min(mxc, seg_cap), WIN_CHUNK));
-                    window_kernel<<<dim3(wx, (unsigned)nwg), 256, 0, st>>>(
-                        seg, seg_r, seg_cap, seg_cnt, (int)nwg, L2, U2, cand, cand_r, ccap, counters + 0,
-                        counters + 3);
-                    recheck_kernel<<<(unsigned)pg_cdiv(ccap, 256), 256, 0, st>>>(cand, counters + 0, ccap, z64, t, cand64);
-                    select_kernel<<<1, 1024, 0, st>>>(cand, cand64, cand_r, counters + 0, ccap, kwant, counters + 3, 1,
-                                                      extra, extra_cnt, tstar_d, flips, (int *)(counters + 4));
-                    PG_KERNEL_CHECK(ctx);
-                    PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
-                    PG_HIP(ctx, hipMemcpyAsync(&inf.t_star, tstar_d, 8, hipMemcpyDeviceToHost, st));
-                    PG_HIP(ctx, hipStreamSynchronize(st));
-                    if (((int *)hcnt)[8] == 0 &&
-                        (long long)hcnt[3] + (long long)hcnt[1] == kwant) {
-                        inf.band_pairs = (long long)hcnt[0];
-                        inf.band_flips = (long long)hcnt[2];
-                        es.seg = seg;
-                        es.seg_cap = seg_cap;
-                        es.seg_cnt = seg_cnt;
-                        es.nseg = (int)nwg;
-                        es.seg_r = seg_r;
-                        es.keep_above = U2;
-                        done = true;
-                    }
-                }
+            PG_TRY(emit_launch(pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096, 1 << 10, true));
+            ga.lower_dev = nullptr;
+            const int hb = (int)std::min<long long>(256, nwg);
+            uint32_t *h2;
+            unsigned long long *h2t;
+            PG_TRY(pg_bufT(ctx, "corr.h2", (size_t)hb * NBIN2, &h2));
+            PG_TRY(pg_bufT(ctx, "corr.h2t", NBIN2, &h2t));
+            PG_HIP(ctx, hipMemsetAsync(h2t, 0, NBIN2 * sizeof(unsigned long long), st));
+            emitted_hist<<<hb, 1024, 0, st>>>(seg_r, seg_cap, seg_cnt, (int)nwg, ctl, h2);
+            hist_sum<<<dim3(NBIN2 / 256, 16), 256, 0, st>>>(h2, hb, NBIN2, h2t);
+            window_bounds<<<1, 1024, 0, st>>>(h2t, kwant, band, ctl);
+            PG_KERNEL_CHECK(ctx);
+            PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+            const long long ccap = std::min<long long>(1 << 22, std::max<long long>(1 << 16, kwant / 4));
+            int2 *cand;
+            float *cand_r;
+            double *cand64;
+            PG_TRY(pg_bufT(ctx, "corr.cand", (size_t)ccap, &cand));
+            PG_TRY(pg_bufT(ctx, "corr.candr", (size_t)ccap, &cand_r));
+            PG_TRY(pg_bufT(ctx, "corr.cand64", (size_t)ccap, &cand64));
+            PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)ccap, &extra));
+            PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
+            const unsigned wx = (unsigned)std::max<long long>(1, pg_cdiv(seg_cap, WIN_CHUNK));
+            window_kernel<<<dim3(wx, (unsigned)nwg), 256, 0, st>>>(seg, seg_r, seg_cap, seg_cnt, (int)nwg, ctl, cand,
+                                                                    cand_r, ccap, counters + 0, counters + 3);
+            recheck_kernel<<<(unsigned)pg_cdiv(ccap, 256), 256, 0, st>>>(cand, counters + 0, ccap, z64, t, cand64);
+            select_kernel<<<1, 1024, 0, st>>>(cand, cand64, cand_r, counters + 0, ccap, kwant, counters + 3, 1, extra,
+                                              extra_cnt, tstar_d, flips, (int *)(counters + 4));
+            PG_KERNEL_CHECK(ctx);
+            es.seg = seg;
+            es.seg_cap = seg_cap;
+            es.seg_cnt = seg_cnt;
+            es.nseg = (int)nwg;
+            es.seg_r = seg_r;
+            es.keep_above_dev = &ctl->U2;
+            es.extra = extra;
+            es.extra_cnt = extra_cnt;
+            es.extra_cap = ccap;
+            PG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+            PG_TRY(launch_csr(es, 2 * kwant));
+            PG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+            SelCtl hc;
+            int rn = -1;
+            PG_HIP(ctx, hipMemcpyAsync(hseg.data(), seg_cnt, nwg * 8, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipMemcpyAsync(&hc, ctl, sizeof hc, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipMemcpyAsync(&inf.t_star, tstar_d, 8, hipMemcpyDeviceToHost, st));
+            if (n > 0) PG_HIP(ctx, hipMemcpyAsync(&rn, row + n, 4, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipStreamSynchronize(st));
+            long long mx = 0;
+            for (auto c : hseg) mx = std::max<long long>(mx, (long long)c);
+            done = hc.ok == 1 && mx <= seg_cap && (long long)hcnt[0] <= ccap && ((int *)hcnt)[8] == 0 &&
+                   (long long)hcnt[3] + (long long)hcnt[1] == kwant && (long long)rn == 2 * kwant;
+            if (done) {
+                inf.band_pairs = (long long)hcnt[0];
+                inf.band_flips = (long long)hcnt[2];
             }
+            std::memset(&es, 0, sizeof es);
         }
         if (!done) {
             // (fallback) exact bracket from a histogram of every pair, then an
@@ -1900,6 +2125,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             const double hi_edge = (bstar + 1) * w - 1.0 + 1e-6;
             ga.upper = round_up_f(hi_edge + 2.0 * band);
             ga.lower = round_down_f(lo_edge - 2.0 * band);
+            PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
             PG_TRY(emit_pass(pg_cdiv(kwant, nwg) * 3 / 2 + 4096, 1 << 22, false));
             PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
             const long long nband = (long long)hcnt[0];
@@ -1923,59 +2149,14 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             es.seg_cap = seg_cap;
             es.seg_cnt = seg_cnt;
             es.nseg = (int)nwg;
+            es.extra = extra;
+            es.extra_cnt = extra_cnt;
+            es.extra_cap = std::max<long long>(nband, 1);
+            PG_TRY(finish_csr(es, 2 * kwant));
         }
         inf.nnz = 2 * kwant;
         inf.r_thresh = (float)inf.t_star;
     }
-    es.extra = extra;
-    es.extra_cnt = extra_cnt;
-    PG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
-    if (inf.nnz > INT32_MAX) return pg_fail(ctx, PG_ENOTSUP, "nnz %lld exceeds int32 CSR", (long long)inf.nnz);
-
-    // ---- CSR build (bucketed counting sort)
-    const int nbk = (int)pg_cdiv(std::max(n, 1), BROWS);
-    const int nsrc = (int)nwg + 1;  // segments + the accepted/selected list
-    const size_t nnzs = (size_t)std::max<long long>(inf.nnz, 1);
-    unsigned *bcnt, *ent;
-    long long *boff, *bstart;
-    int *row, *ctmp, *col;
-    float *entv = nullptr, *vtmp = nullptr, *val = nullptr;
-    PG_TRY(pg_bufT(ctx, "corr.bcnt", (size_t)nsrc * nbk, &bcnt));
-    PG_TRY(pg_bufT(ctx, "corr.boff", (size_t)nsrc * nbk, &boff));
-    PG_TRY(pg_bufT(ctx, "corr.bstart", (size_t)nbk + 1, &bstart));
-    long long *btot;
-    PG_TRY(pg_bufT(ctx, "corr.btot", (size_t)nbk + 1, &btot));
-    PG_TRY(pg_bufT(ctx, "corr.ent", nnzs, &ent));
-    PG_TRY(pg_bufT(ctx, "corr.row", (size_t)n + 2, &row));
-    PG_TRY(pg_bufT(ctx, "corr.ctmp", nnzs, &ctmp));
-    PG_TRY(pg_bufT(ctx, "corr.col", nnzs, &col));
-    if (weighted) {
-        PG_TRY(pg_bufT(ctx, "corr.entv", nnzs, &entv));
-        PG_TRY(pg_bufT(ctx, "corr.vtmp", nnzs, &vtmp));
-        PG_TRY(pg_bufT(ctx, "corr.val", nnzs, &val));
-    }
-    if (n > 0) {
-        const size_t shb = (size_t)nbk * 4;
-        bucket_count<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(es, nbk, bcnt);
-        bucket_scan_local<<<(unsigned)nbk, 256, 0, st>>>(bcnt, nsrc, boff, btot);
-        bucket_scan_global<<<1, 1024, 0, st>>>(btot, nbk, bstart);
-        bucket_scatter<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(es, nbk, boff, bstart, ent, entv, z64, t);
-        int nbm = 4;
-        while (nbm > 1 && (size_t)nbm * 8 * nw + 2048 > ctx->lds_per_block) nbm--;
-        const size_t shr = (size_t)nbm * 8 * nw;
-        PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shr));
-        bucket_rows<<<(unsigned)nbk, ROWS_THREADS, shr, st>>>(bstart, ent, entv, n, nbk, nbm, row, ctmp, vtmp, col,
-                                                              val);
-        PG_KERNEL_CHECK(ctx);
-    } else {
-        PG_HIP(ctx, hipMemsetAsync(row, 0, 4, st));
-    }
-    PG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
-    PG_HIP(ctx, hipStreamSynchronize(st));
-    int rn = 0;
-    PG_HIP(ctx, hipMemcpy(&rn, row + n, 4, hipMemcpyDeviceToHost));
-    if ((long long)rn != inf.nnz)
-        return pg_fail(ctx, PG_EFORMAT, "CSR build mismatch: row[n]=%d nnz=%lld", rn, (long long)inf.nnz);
 
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);

@@ -875,9 +875,21 @@ __device__ __forceinline__ double r64_pair(const double *__restrict__ z, int t, 
     const double *a = z + (long long)i * t;
     const double *b = z + (long long)j * t;
     double s = 0.0;
-    for (int l = 0; l < t; l++) {
-        double p = a[l] * b[l];
-        s += p;
+    if ((t & 1) == 0) {  // rows 16-byte aligned: paired loads, same order of sums
+        const double2 *a2 = (const double2 *)a, *b2 = (const double2 *)b;
+#pragma unroll 8
+        for (int l = 0; l < t / 2; l++) {
+            const double2 x = a2[l], y = b2[l];
+            const double p0 = x.x * y.x;
+            s += p0;
+            const double p1 = x.y * y.y;
+            s += p1;
+        }
+    } else {
+        for (int l = 0; l < t; l++) {
+            double p = a[l] * b[l];
+            s += p;
+        }
     }
     return s > 1.0 ? 1.0 : (s < -1.0 ? -1.0 : s);
 }
@@ -1405,17 +1417,21 @@ __global__ __launch_bounds__(256) void window_kernel(const int2 *__restrict__ se
                                                      int nseg, const SelCtl *__restrict__ ctl,
                                                      int2 *__restrict__ cand, float *__restrict__ cand_r,
                                                      long long cand_cap, unsigned long long *__restrict__ cand_cnt,
-                                                     unsigned long long *__restrict__ definite)
+                                                     unsigned long long *__restrict__ dpart)
 {
     constexpr int PER = WIN_CHUNK / 256 / 4;  // float4 loads per thread
     __shared__ unsigned s_scan[256];
-    __shared__ unsigned long long s_base;
+    __shared__ unsigned long long s_base, s_def;
     const int g = blockIdx.y, tid = threadIdx.x;
-    if (!ctl->ok) return;
-    const float lo = ctl->L2, hi = ctl->U2;
+    unsigned long long *my_def = dpart + (long long)blockIdx.y * gridDim.x + blockIdx.x;
     const long long c = min((long long)seg_cnt[g], seg_cap);
     const long long k0 = (long long)blockIdx.x * WIN_CHUNK;
-    if (k0 >= c) return;
+    if (!ctl->ok || k0 >= c) {
+        if (tid == 0) *my_def = 0;
+        return;
+    }
+    const float lo = ctl->L2, hi = ctl->U2;
+    if (tid == 0) s_def = 0;
     const long long base = (long long)g * seg_cap;
     float4 v[PER];
 #pragma unroll
@@ -1452,7 +1468,9 @@ __global__ __launch_bounds__(256) void window_kernel(const int2 *__restrict__ se
     const unsigned tot = s_scan[255];
     unsigned long long ndb = nd;
     for (int o = 32; o > 0; o >>= 1) ndb += __shfl_xor(ndb, o);
-    if ((tid & 63) == 0 && ndb) atomicAdd(definite, ndb);
+    if ((tid & 63) == 0 && ndb) atomicAdd(&s_def, ndb);
+    __syncthreads();
+    if (tid == 0) *my_def = s_def;  // summed by the selection, no global hot spot
     if (tot == 0) return;
     if (tid == 0) s_base = atomicAdd(cand_cnt, (unsigned long long)tot);
     __syncthreads();
@@ -1489,15 +1507,33 @@ template <class KeyF, class WantF>
 __device__ unsigned long long radix_select(long long c, long long m, KeyF key, WantF want,
                                            unsigned *hist, unsigned long long *sh, long long *rank_left)
 {
+    // one block; every pass reads RB keys per thread before using them, so
+    // the (L2-resident) loads overlap instead of serialising on latency
+    constexpr int RB = 8;
+    const long long stride = blockDim.x;
+    auto batch = [&](long long q0, unsigned long long (&kk)[RB], bool (&ww)[RB]) {
+#pragma unroll
+        for (int u = 0; u < RB; u++) {
+            const long long q = q0 + u * stride;
+            ww[u] = q < c && want(q);
+        }
+#pragma unroll
+        for (int u = 0; u < RB; u++) kk[u] = ww[u] ? key(q0 + u * stride) : 0ull;
+    };
     // bytes shared by every wanted key are fixed up front: candidates sit in
     // a narrow r window, so their top bytes agree and a digit histogram over
     // them would serialise on one LDS bin
     unsigned long long kmin = ~0ull, kmax = 0ull;
-    for (long long q = threadIdx.x; q < c; q += blockDim.x) {
-        if (!want(q)) continue;
-        const unsigned long long kq = key(q);
-        kmin = kq < kmin ? kq : kmin;
-        kmax = kq > kmax ? kq : kmax;
+    for (long long q0 = threadIdx.x; q0 < c; q0 += RB * stride) {
+        unsigned long long kk[RB];
+        bool ww[RB];
+        batch(q0, kk, ww);
+#pragma unroll
+        for (int u = 0; u < RB; u++)
+            if (ww[u]) {
+                kmin = kk[u] < kmin ? kk[u] : kmin;
+                kmax = kk[u] > kmax ? kk[u] : kmax;
+            }
     }
     if (threadIdx.x == 0) {
         sh[0] = ~0ull;
@@ -1519,10 +1555,13 @@ __device__ unsigned long long radix_select(long long c, long long m, KeyF key, W
     for (int shift = top; shift >= 0; shift -= 8) {
         for (int d = threadIdx.x; d < 256; d += blockDim.x) hist[d] = 0;
         __syncthreads();
-        for (long long q = threadIdx.x; q < c; q += blockDim.x) {
-            if (!want(q)) continue;
-            const unsigned long long kq = key(q);
-            if ((kq & mask) == prefix) atomicAdd(&hist[(kq >> shift) & 255u], 1u);
+        for (long long q0 = threadIdx.x; q0 < c; q0 += RB * stride) {
+            unsigned long long kk[RB];
+            bool ww[RB];
+            batch(q0, kk, ww);
+#pragma unroll
+            for (int u = 0; u < RB; u++)
+                if (ww[u] && (kk[u] & mask) == prefix) atomicAdd(&hist[(kk[u] >> shift) & 255u], 1u);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1585,16 +1624,42 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
     __syncthreads();
     const double ts = tstar[0];
     unsigned long long fl = 0;
-    for (long long q = threadIdx.x; q < c; q += blockDim.x) {
-        const unsigned long long kq = key_r_desc(r64[q]);
-        const unsigned long long k2q = ((unsigned long long)(unsigned)cand[q].x << 32) | (unsigned)cand[q].y;
-        if (kq < k1 || (kq == k1 && k2q <= k2)) {
-            unsigned long long p = atomicAdd(extra_cnt, 1ull);
-            extra[p] = cand[q];
+    const int lane = threadIdx.x & 63;
+    for (long long q0 = 0; q0 < c; q0 += blockDim.x) {  // uniform trip count: ballots below
+        const long long q = q0 + threadIdx.x;
+        bool sel = false;
+        int2 e = make_int2(0, 0);
+        if (q < c) {
+            const double rq = r64[q];
+            e = cand[q];
+            const unsigned long long kq = key_r_desc(rq);
+            const unsigned long long k2q = ((unsigned long long)(unsigned)e.x << 32) | (unsigned)e.y;
+            sel = kq < k1 || (kq == k1 && k2q <= k2);
+            fl += ((double)rfast[q] > ts) != (rq > ts);
         }
-        fl += ((double)rfast[q] > ts) != (r64[q] > ts);
+        const unsigned long long m = __ballot(sel);
+        unsigned long long base = 0;
+        if (lane == 0 && m) base = atomicAdd(extra_cnt, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0);
+        if (sel) extra[base + lane_rank(m)] = e;
     }
     atomicAdd(flips, fl);
+}
+
+// out = sum of part[0..np) (one block)
+__global__ __launch_bounds__(1024) void definite_sum(const unsigned long long *__restrict__ part, int np,
+                                                     unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long r[1024];
+    unsigned long long s = 0;
+    for (int i = threadIdx.x; i < np; i += 1024) s += part[i];
+    r[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) r[threadIdx.x] += r[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = r[0];
 }
 
 // ------------------------------------------------------------------ host
@@ -2074,11 +2139,16 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)ccap, &extra));
             PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
             const unsigned wx = (unsigned)std::max<long long>(1, pg_cdiv(seg_cap, WIN_CHUNK));
+            unsigned long long *dpart;
+            PG_TRY(pg_bufT(ctx, "corr.dpart", (size_t)wx * nwg, &dpart));
             window_kernel<<<dim3(wx, (unsigned)nwg), 256, 0, st>>>(seg, seg_r, seg_cap, seg_cnt, (int)nwg, ctl, cand,
-                                                                    cand_r, ccap, counters + 0, counters + 3);
+                                                                    cand_r, ccap, counters + 0, dpart);
             recheck_kernel<<<(unsigned)pg_cdiv(ccap, 256), 256, 0, st>>>(cand, counters + 0, ccap, z64, t, cand64);
-            select_kernel<<<1, 1024, 0, st>>>(cand, cand64, cand_r, counters + 0, ccap, kwant, counters + 3, 1, extra,
-                                              extra_cnt, tstar_d, flips, (int *)(counters + 4));
+            select_kernel<<<1, 1024, 0, st>>>(cand, cand64, cand_r, counters + 0, ccap, kwant, dpart,
+                                              (int)(wx * nwg), extra, extra_cnt, tstar_d, flips,
+                                              (int *)(counters + 4));
+            // definite count for the final check: written by the selection
+            definite_sum<<<1, 1024, 0, st>>>(dpart, (int)(wx * nwg), counters + 3);
             PG_KERNEL_CHECK(ctx);
             es.seg = seg;
             es.seg_cap = seg_cap;

@@ -36,6 +36,11 @@ HBM_PEAK = 8.0e12       # bytes/s
 # 64 lanes x 2.4 GHz = 39.3e12 ops/s, 1.5 ops per update (v_add_f32 +
 # half a v_min3_f32)
 FW_UPDATES_PEAK = 39.3e12 / 1.5
+# HBM bytes per launch of the emitting BF16x3 Gram kernel at this workload,
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch), each
+# scaled by what the counter reads for a 1 GiB device copy (FETCH_SIZE x2,
+# WRITE_SIZE x1); profiles/gram2_emit_n58000_t240_counters.md
+GRAM_EMIT_TRAFFIC = 1943480.69 * 1024 * 2 + 277847.88 * 1024
 
 METRIC = "per-subject wall time N=58k; corr TFLOP/s+MFMA%; BC GTEPS; BF16x3 vs FP32"
 
@@ -56,8 +61,8 @@ def cpu_baseline(x, n, t, thr_r, budget_s=15.0):
     # consecutive row blocks, each sized from the previous block's rate to
     # about 2 s, until the budget is spent
     rows, dt, chunk = 0, 0.0, 16
-    while dt < budget_s and rows < n // 2:
-        chunk = min(chunk, n // 2 - rows)
+    while dt < budget_s and rows < n:
+        chunk = min(chunk, n - rows)
         t0 = time.perf_counter()
         O.corr_count_rows(z64, thr_r, rows, rows + chunk)
         step = max(time.perf_counter() - t0, 1e-4)
@@ -338,7 +343,9 @@ def main(argv=None) -> int:
         "t_star": last.t_star,
         "roofline": {"bound": "mfma", "kernel": "gram2_kernel<EMIT> (BF16x3)",
                      "achieved": 3.0 * useful / (emit_ms * 1e-3) / 1e12, "peak": BF16_PEAK / 1e12,
-                     "unit": "TFLOP/s", "frac": mfma_frac, "traffic": None},
+                     "unit": "TFLOP/s", "frac": mfma_frac, "traffic": GRAM_EMIT_TRAFFIC,
+                     "traffic_unit": "bytes per launch",
+                     "algorithmic_flop": 3.0 * useful, "duration_ms": emit_ms},
     }
 
     if not a.no_side:

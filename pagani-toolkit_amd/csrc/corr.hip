@@ -171,10 +171,12 @@ __global__ void zscore_write(const float *__restrict__ x, int layout, int n, int
     }
 }
 
-// Fused form for T up to ZS_TMAX: a block stages the whole T x 64 tile of its
-// voxels in LDS (coalesced), 64 threads form the sequential FP64 sums from
-// LDS, then all threads write z64 and the panels voxel-major.
+// Fused form for T up to ZS_TMAX: a block stages the whole T x ZV tile of its
+// voxels in LDS (coalesced), ZV threads form the sequential FP64 sums from
+// LDS, then all threads write z64 and the panels.
 constexpr int ZS_TMAX = 360;
+
+constexpr int ZV = 32;  // voxels per block of zscore_fused (LDS [t][ZV+1] floats)
 
 __global__ __launch_bounds__(256) void zscore_fused(const float *__restrict__ x, int layout, int n, int t,
                                                    double *__restrict__ z64, float *__restrict__ p32,
@@ -182,24 +184,25 @@ __global__ __launch_bounds__(256) void zscore_fused(const float *__restrict__ x,
                                                    int npad)
 {
 #pragma clang fp contract(off)
-    extern __shared__ float zt[];  // [t][65]
-    __shared__ double smu[64], ssd[64];
-    const int i0 = blockIdx.x * 64, tid = threadIdx.x;
-    for (int e = tid; e < 64 * t; e += 256) {
+    extern __shared__ float zt[];  // [t][ZV + 1]
+    __shared__ double smu[ZV], ssd[ZV];
+    constexpr int LD = ZV + 1;
+    const int i0 = blockIdx.x * ZV, tid = threadIdx.x;
+    for (int e = tid; e < ZV * t; e += 256) {
         int l, ii;
-        if (layout == 0) { l = e >> 6; ii = e & 63; }
+        if (layout == 0) { l = e / ZV; ii = e % ZV; }
         else { ii = e / t; l = e % t; }
         const int i = i0 + ii;
-        zt[l * 65 + ii] = i < n ? (layout == 0 ? x[(long long)l * n + i] : x[(long long)i * t + l]) : 0.0f;
+        zt[l * LD + ii] = i < n ? (layout == 0 ? x[(long long)l * n + i] : x[(long long)i * t + l]) : 0.0f;
     }
     __syncthreads();
-    if (tid < 64) {
+    if (tid < ZV) {
         double s1 = 0.0;
-        for (int l = 0; l < t; l++) s1 += (double)zt[l * 65 + tid];
+        for (int l = 0; l < t; l++) s1 += (double)zt[l * LD + tid];
         s1 /= t;
         double s2 = 0.0;
         for (int l = 0; l < t; l++) {
-            double d = (double)zt[l * 65 + tid] - s1;
+            double d = (double)zt[l * LD + tid] - s1;
             double p = d * d;
             s2 += p;
         }
@@ -207,12 +210,12 @@ __global__ __launch_bounds__(256) void zscore_fused(const float *__restrict__ x,
         ssd[tid] = sqrt(s2);
     }
     __syncthreads();
-    for (int e = tid; e < 64 * t; e += 256) {
+    for (int e = tid; e < ZV * t; e += 256) {
         const int ii = e / t, l = e % t;
         const int i = i0 + ii;
         if (i >= n) continue;
         const double sdv = ssd[ii];
-        const double z = sdv > 0.0 ? ((double)zt[l * 65 + ii] - smu[ii]) / sdv : 0.0;
+        const double z = sdv > 0.0 ? ((double)zt[l * LD + ii] - smu[ii]) / sdv : 0.0;
         z64[(long long)i * t + l] = z;
         const float zf = (float)z;
         if (p32) p32[(long long)i * ldp + l] = zf;
@@ -1564,15 +1567,33 @@ __device__ unsigned long long radix_select(long long c, long long m, KeyF key, W
                 if (ww[u] && (kk[u] & mask) == prefix) atomicAdd(&hist[(kk[u] >> shift) & 255u], 1u);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            long long cum = 0;
-            int dsel = 255;
-            for (int d = 0; d < 256; d++) {
-                if (cum + (long long)hist[d] >= mr) { dsel = d; break; }
-                cum += hist[d];
+        // digit holding the mr-th key: the first wave scans the 256 bins,
+        // four per lane, with one wave prefix sum
+        if (threadIdx.x < 64) {
+            const int l = threadIdx.x;
+            const long long h0 = hist[4 * l], h1 = hist[4 * l + 1], h2 = hist[4 * l + 2], h3 = hist[4 * l + 3];
+            const long long own = h0 + h1 + h2 + h3;
+            long long incl = own;
+            for (int o = 1; o < 64; o <<= 1) {
+                const long long y = __shfl_up(incl, o);
+                if (l >= o) incl += y;
             }
-            sh[0] = (unsigned long long)dsel;
-            sh[1] = (unsigned long long)(mr - cum);
+            const long long before = incl - own;
+            if (before < mr && incl >= mr) {
+                long long cum = before;
+                int d = 4 * l;
+                const long long hh[4] = {h0, h1, h2, h3};
+                for (int q = 0; q < 4; q++) {
+                    if (cum + hh[q] >= mr) { d = 4 * l + q; break; }
+                    cum += hh[q];
+                }
+                sh[0] = (unsigned long long)d;
+                sh[1] = (unsigned long long)(mr - cum);
+            }
+            if (l == 63 && incl < mr) {  // fewer keys than mr: last digit, as a serial scan gives
+                sh[0] = 255ull;
+                sh[1] = (unsigned long long)(mr - incl);
+            }
         }
         __syncthreads();
         prefix |= sh[0] << shift;
@@ -1791,9 +1812,9 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
 
     PG_HIP(ctx, hipEventRecord(ctx->ev[0], st));
     if (n > 0 && t <= ZS_TMAX) {
-        const size_t shz = (size_t)t * 65 * 4;
+        const size_t shz = (size_t)t * (ZV + 1) * 4;
         PG_HIP(ctx, hipFuncSetAttribute((const void *)zscore_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shz));
-        zscore_fused<<<(unsigned)pg_cdiv(n, 64), 256, shz, st>>>(
+        zscore_fused<<<(unsigned)pg_cdiv(n, ZV), 256, shz, st>>>(
             dbold, opts->layout, n, t, z64, prec == PREC_F32 ? (float *)p0 : nullptr,
             prec == PREC_BF16X3 ? (uint16_t *)p0 : nullptr, prec == PREC_BF16X3 ? (uint16_t *)p1 : nullptr, ldp,
             npad);

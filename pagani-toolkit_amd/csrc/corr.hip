@@ -641,6 +641,40 @@ __device__ __forceinline__ void g2_dma(const GramArgs &a, char *dst, int R, int 
 // s_waitcnt vmcnt(n) (gfx9 encoding, other counters left alone)
 #define G2_WAIT_VM(n) __builtin_amdgcn_s_waitcnt((0x3F70) | ((n) & 15) | ((((n) >> 4) & 3) << 14))
 
+// Pass 2 of the epilogue: each lane writes its hits in order from its slots
+// be (edges) and bc (rechecks); CHECK guards every write against capacity.
+template <bool BOUNDED, bool SPLIT, bool CHECK>
+__device__ __forceinline__ void g2_write(const GramArgs &a, const f32x16 (&acc)[g2::BMW][g2::BNW], float lo, float up,
+                                         int ib, int jb, int nn, int2 *myseg, float *segr, unsigned long long be,
+                                         unsigned long long bc)
+{
+#pragma unroll
+    for (int bm = 0; bm < g2::BMW; bm++)
+#pragma unroll
+        for (int bn = 0; bn < g2::BNW; bn++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const float v = acc[bm][bn][q];
+                const int i = ib + bm * 32 + (q >> 2) * 8 + (q & 3), j = jb + bn * 32;
+                bool hit = v >= lo;
+                if (!BOUNDED) hit = hit & (j < nn) & (i < j);
+                if (!hit) continue;
+                if (SPLIT && !(v > up)) {
+                    if (!CHECK || (long long)bc < a.band_cap) {
+                        a.band[bc] = make_int2(i, j);
+                        a.band_r[bc] = v;
+                    }
+                    bc++;
+                } else {
+                    if (!CHECK || (long long)be < a.seg_cap) {
+                        myseg[be] = make_int2(i, j);
+                        if (!SPLIT) segr[be] = v;
+                    }
+                    be++;
+                }
+            }
+}
+
 // Epilogue of one wave's 128 x 64 block: pass 1 counts per lane (packed: low
 // 16 bits edges v > upper, high 16 bits rechecks lower <= v <= upper; each at
 // most 8192 per wave); one reservation per wave; pass 2 has each lane write
@@ -681,40 +715,23 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
         if (tot >> 16) bc = atomicAdd(a.band_cnt, (unsigned long long)(tot >> 16));
     }
     const unsigned ex = x - cnt;
-    be = __shfl(be, 0) + (ex & 0xFFFFu);
-    bc = __shfl(bc, 0) + (ex >> 16);
+    be = __shfl(be, 0);
+    bc = __shfl(bc, 0);
+    // every write of this wave and tile fits its buffer: no per-value checks
+    const bool fits = be + (tot & 0xFFFFu) <= (unsigned long long)a.seg_cap &&
+                      bc + (tot >> 16) <= (unsigned long long)a.band_cap;
+    be += ex & 0xFFFFu;
+    bc += ex >> 16;
     // opaque copies: pass 2 recomputes its predicates instead of the compiler
     // keeping pass 1's 128 of them live
     float lo = lower, up = a.upper;
     int ib = ib0, jb = jb0, nn = a.n;
     asm volatile("" : "+v"(lo), "+v"(up), "+v"(ib), "+v"(jb), "+v"(nn));
-    float *segr = a.seg_r ? a.seg_r + (long long)blockIdx.x * a.seg_cap : nullptr;
-#pragma unroll
-    for (int bm = 0; bm < g2::BMW; bm++)
-#pragma unroll
-        for (int bn = 0; bn < g2::BNW; bn++)
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const float v = acc[bm][bn][q];
-                const int i = ib + bm * 32 + (q >> 2) * 8 + (q & 3), j = jb + bn * 32;
-                bool hit = v >= lo;
-                if (!BOUNDED) hit = hit & (j < nn) & (i < j);
-                if (hit) {
-                    if (!SPLIT || v > up) {
-                        if ((long long)be < a.seg_cap) {
-                            myseg[be] = make_int2(i, j);
-                            if (segr) segr[be] = v;
-                        }
-                        be++;
-                    } else {
-                        if ((long long)bc < a.band_cap) {
-                            a.band[bc] = make_int2(i, j);
-                            a.band_r[bc] = v;
-                        }
-                        bc++;
-                    }
-                }
-            }
+    // r~ of every segment entry is kept exactly when there is no band (the
+    // sparsity single pass)
+    float *segr = SPLIT ? nullptr : a.seg_r + (long long)blockIdx.x * a.seg_cap;
+    if (fits) g2_write<BOUNDED, SPLIT, false>(a, acc, lo, up, ib, jb, nn, myseg, segr, be, bc);
+    else g2_write<BOUNDED, SPLIT, true>(a, acc, lo, up, ib, jb, nn, myseg, segr, be, bc);
 }
 
 template <int MODE>

@@ -6,8 +6,8 @@
 // round).  Per k-block round:
 //   phase 1: the diagonal tile, sequential k in LDS (one work-group);
 //   phase 2: row and column panel tiles, sequential k in LDS;
-//   phase 3: every other tile, min over k of (ik + kj), 8x8 (B=128) or 4x4
-//            (B=64) register micro-tiles over LDS panels.
+//   phase 3: every other tile, min over k of (ik + kj), 4x4 register
+//            micro-tiles over LDS panels.
 // Min-plus is not a dense contraction, so this runs on VALU (v_add_f32 +
 // v_min3_f32), not MFMA.  Each candidate path length is one FP32 add of the
 // same operands as the blocked CPU algorithm (W/Lp/CUBFW_Lp/CUBFW_Lp/BFW.cpp:
@@ -70,10 +70,17 @@ __global__ __launch_bounds__(256) void fw_seq_tiles(float *__restrict__ d, int n
 }
 
 // Phase 3: tile (r, c) = min(tile, min_k ik[i][k] + kj[k][j]) for r, c != kb.
+// 4 x 4 register micro-tile per thread, (B/4)^2 threads: 256 for B = 64, 1024
+// for B = 128 (whose 134 KB of LDS panels allow one work-group per CU).
 template <int B>
-__global__ __launch_bounds__(256) void fw_phase3(float *__restrict__ d, int np, int kb)
+constexpr int fw3_threads() { return (B / 4) * (B / 4); }
+
+template <int B>
+__global__ __launch_bounds__(fw3_threads<B>()) void fw_phase3(float *__restrict__ d, int np, int kb)
 {
-    constexpr int M = B / 16;  // micro-tile edge per thread (16 x 16 threads)
+    constexpr int M = 4;
+    constexpr int TS = B / M;           // threads per tile edge
+    constexpr int NT = fw3_threads<B>();
     __shared__ float sa[B][B + 1];  // ik panel, [i][k]
     __shared__ float sb[B][B + 4];  // kj panel, [k][j]
     const int nbk = np / B;
@@ -81,9 +88,9 @@ __global__ __launch_bounds__(256) void fw_phase3(float *__restrict__ d, int np, 
     r = r < kb ? r : r + 1;
     c = c < kb ? c : c + 1;
     if (r >= nbk || c >= nbk) return;
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int tid = threadIdx.x, tx = tid % TS, ty = tid / TS;
     const long long r0 = (long long)r * B, c0 = (long long)c * B, k0 = (long long)kb * B;
-    for (int e = tid; e < B * B; e += 256) {
+    for (int e = tid; e < B * B; e += NT) {
         int i = e / B, j = e % B;
         sa[i][j] = d[(r0 + i) * np + k0 + j];
         sb[i][j] = d[(k0 + i) * np + c0 + j];
@@ -214,7 +221,7 @@ pg_status fw_run(pg_ctx *ctx, float *dm, int np)
         if (nbk > 1) {
             fw_seq_tiles<B><<<2 * (nbk - 1), 256, 0, st>>>(dm, np, kb, 2);
             dim3 g(nbk - 1, nbk - 1);
-            fw_phase3<B><<<g, 256, 0, st>>>(dm, np, kb);
+            fw_phase3<B><<<g, fw3_threads<B>(), 0, st>>>(dm, np, kb);
         }
     }
     PG_KERNEL_CHECK(ctx);

@@ -6,7 +6,7 @@
 // The power iteration on B^(g) + beta I is one CSR SpMV (FP32 weights, FP64
 // accumulation) plus the rank-one degree term and the diagonal correction; all
 // norms and dot products are FP64 on the device.  Every round whose members
-// are known (a wave) iterates together: three launches per iteration cover all
+// are known (a wave) iterates together: two launches per iteration cover all
 // of the wave's groups, each group stops on the device at the reference's
 // test, and the host checks the wave once per BATCH iterations.
 //
@@ -30,7 +30,8 @@ namespace {
 constexpr int MAX_ITER = 10000;      // modularity_GPU.cu:25
 constexpr double EPS = 1e-6;         // modularity_GPU.cu:28
 constexpr int TPB = 256;             // threads per block
-constexpr int NPB = 256;             // nodes per block
+constexpr int NPB_MAX = 1024;        // nodes per work block, upper limit
+constexpr int GSUM_MAX = 512;        // target for block partials per group
 constexpr int BATCH = 64;            // power iterations queued per host check
 
 // One wave of independent groups (the rounds created but not yet processed),
@@ -39,6 +40,7 @@ constexpr int BATCH = 64;            // power iterations queued per host check
 // per-group sum is a fixed-order sum of its blocks' partials.
 struct Wave {
     int nnodes, nblk;
+    int npb;               // nodes per work block (64 .. NPB_MAX)
     const int *bgrp;       // [nblk] group of each block
     const int *no, *ng, *fb, *nb;   // [G]
     const int *gid;        // [nnodes] global id (start vector)
@@ -49,10 +51,12 @@ struct Wave {
     double m2;
     // per group state
     int ngroups;
-    int *done, *stop, *iters, *it0;
+    int *stop, *iters, *it0;
+    const int *restart;    // [G] 1: (re)start the group's pass from the start vector
     double *beta, *vnorm, *lam;
-    // vectors and per-block partials
-    double *v, *v0, *pdot, *pss, *pe1, *pe2;
+    // vectors: iteration li of a group reads buf[li & 1] and leaves its
+    // normalised result in buf[(li + 1) & 1]; per-block partials
+    double *buf[2], *vres, *pdot, *pss, *pe1, *pe2;
 };
 
 __device__ double block_sum(double x, double *red)
@@ -77,25 +81,33 @@ __device__ double group_sum(const double *part, int f, int m, double *red)
     return block_sum(s, red);
 }
 
-// One power iteration is three phases: (A) the stop test on the previous
-// iteration's partials, v0 = v and the partial of v0 . k; (B) v = B^(g) v0 +
-// beta v0 (CSR SpMV, FP32 weights, FP64 accumulation, one wave per row) and
-// the partial of |v|^2; (C) v /= |v| and the partials of |v - v0|^2 and
-// |v + v0|^2.  Work blocks b = blockIdx.x, blockIdx.x + gridDim.x, ... cover
-// the wave; every skip below is uniform in a block.
+// One power iteration is two launches.  (S) the stop test on the previous
+// iteration's partials, then y = B^(g) x + beta x (CSR SpMV, FP32 weights,
+// FP64 accumulation, one wave per row) into the other buffer, and the partial
+// of |y|^2; x . k comes from the partials of the previous launch.  (N) y /= |y|
+// in place, the partials of |y - x|^2, |y + x|^2 and y . k.  Work blocks b =
+// blockIdx.x, blockIdx.x + gridDim.x, ... cover the wave; every skip below is
+// uniform in a block.
 struct PhaseSmem {
     double red[TPB];
-    double rowv[NPB];
+    double rowv[NPB_MAX];
     int skip;
 };
 
-__device__ __forceinline__ void phase_a(const Wave &w, int it, PhaseSmem &sm)
+__device__ __forceinline__ void phase_s(const Wave &w, int it, PhaseSmem &sm)
 {
 #pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int b = blockIdx.x; b < w.nblk; b += gridDim.x) {
         const int g = w.bgrp[b];
-        const int li = it - w.it0[g];
-        if (w.done[g] || li < 0) continue;  // done[] is not written during this phase
+        const int li = it - w.it0[g];  // iterations completed in this pass
+        // stop[g] may be set by this launch's first block of g; every block of
+        // g reaches the same decision below, so either reading skips
+        if (threadIdx.x == 0) sm.skip = w.stop[g] || li < 0;
+        __syncthreads();
+        const int skip = sm.skip;
+        __syncthreads();
+        if (skip) continue;
         if (li > 0) {
             const double e1 = sqrt(group_sum(w.pe1, w.fb[g], w.nb[g], sm.red));
             const double e2 = sqrt(group_sum(w.pe2, w.fb[g], w.nb[g], sm.red));
@@ -107,39 +119,14 @@ __device__ __forceinline__ void phase_a(const Wave &w, int it, PhaseSmem &sm)
                 continue;
             }
         }
-        const int i0 = w.no[g] + (b - w.fb[g]) * NPB, i1 = min(w.no[g] + w.ng[g], i0 + NPB);
-        double s = 0.0;
-        for (int i = i0 + threadIdx.x; i < i1; i += TPB) {
-            const double x = w.v[i];
-            w.v0[i] = x;
-            s += x * w.k[i];
-        }
-        s = block_sum(s, sm.red);
-        if (threadIdx.x == 0) w.pdot[b] = s;
-    }
-}
-
-__device__ __forceinline__ void phase_b(const Wave &w, int it, PhaseSmem &sm)
-{
-#pragma clang fp contract(off)
-    // the stop decisions of phase A become done flags; groups stopped in
-    // this iteration are skipped through stop[] (set before this phase)
-    for (int g = blockIdx.x * TPB + threadIdx.x; g < w.ngroups; g += gridDim.x * TPB)
-        if (w.stop[g]) w.done[g] = 1;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int b = blockIdx.x; b < w.nblk; b += gridDim.x) {
-        const int g = w.bgrp[b];
-        if (threadIdx.x == 0) sm.skip = w.done[g] || w.stop[g] || it < w.it0[g];
-        __syncthreads();
-        const int skip = sm.skip;
-        __syncthreads();
-        if (skip) continue;
+        const double *x = w.buf[li & 1];
+        double *y = w.buf[(li + 1) & 1];
         const double vk = group_sum(w.pdot, w.fb[g], w.nb[g], sm.red);
         const double beta = w.beta[g];
-        const int i0 = w.no[g] + (b - w.fb[g]) * NPB, i1 = min(w.no[g] + w.ng[g], i0 + NPB);
+        const int i0 = w.no[g] + (b - w.fb[g]) * w.npb, i1 = min(w.no[g] + w.ng[g], i0 + w.npb);
         for (int i = i0 + wv; i < i1; i += TPB / 64) {
             double acc = 0.0;
-            for (int e = w.srow[i] + lane; e < w.srow[i + 1]; e += 64) acc += w.v0[w.scol[e]] * (double)w.sval[e];
+            for (int e = w.srow[i] + lane; e < w.srow[i + 1]; e += 64) acc += x[w.scol[e]] * (double)w.sval[e];
             for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
             if (lane == 0) sm.rowv[i - i0] = acc;
         }
@@ -147,8 +134,8 @@ __device__ __forceinline__ void phase_b(const Wave &w, int it, PhaseSmem &sm)
         double s = 0.0;
         for (int i = i0 + threadIdx.x; i < i1; i += TPB) {
             double acc = sm.rowv[i - i0];
-            acc -= vk / w.m2 * w.k[i] + (w.sumbg[i] - beta) * w.v0[i];
-            w.v[i] = acc;
+            acc -= vk / w.m2 * w.k[i] + (w.sumbg[i] - beta) * x[i];
+            y[i] = acc;
             s += acc * acc;
         }
         s = block_sum(s, sm.red);
@@ -156,27 +143,33 @@ __device__ __forceinline__ void phase_b(const Wave &w, int it, PhaseSmem &sm)
     }
 }
 
-__device__ __forceinline__ void phase_c(const Wave &w, int it, PhaseSmem &sm)
+__device__ __forceinline__ void phase_n(const Wave &w, int it, PhaseSmem &sm)
 {
 #pragma clang fp contract(off)
     for (int b = blockIdx.x; b < w.nblk; b += gridDim.x) {
         const int g = w.bgrp[b];
-        if (w.done[g] || w.stop[g] || it < w.it0[g]) continue;
+        const int li = it - w.it0[g];
+        if (w.stop[g] || li < 0) continue;  // stop[] is not written during this launch
+        const double *x = w.buf[li & 1];
+        double *y = w.buf[(li + 1) & 1];
         const double vn = sqrt(group_sum(w.pss, w.fb[g], w.nb[g], sm.red));
-        const int i0 = w.no[g] + (b - w.fb[g]) * NPB, i1 = min(w.no[g] + w.ng[g], i0 + NPB);
-        double p1 = 0.0, p2 = 0.0;
+        const int i0 = w.no[g] + (b - w.fb[g]) * w.npb, i1 = min(w.no[g] + w.ng[g], i0 + w.npb);
+        double p1 = 0.0, p2 = 0.0, pk = 0.0;
         for (int i = i0 + threadIdx.x; i < i1; i += TPB) {
-            const double x = w.v[i] / vn;
-            w.v[i] = x;
-            const double a = x - w.v0[i], c = x + w.v0[i];
+            const double v = y[i] / vn;
+            y[i] = v;
+            const double a = v - x[i], c = v + x[i];
             p1 += a * a;
             p2 += c * c;
+            pk += v * w.k[i];
         }
         p1 = block_sum(p1, sm.red);
         p2 = block_sum(p2, sm.red);
+        pk = block_sum(pk, sm.red);
         if (threadIdx.x == 0) {
             w.pe1[b] = p1;
             w.pe2[b] = p2;
+            w.pdot[b] = pk;
             if (b == w.fb[g]) w.vnorm[g] = vn;
         }
     }
@@ -184,36 +177,52 @@ __device__ __forceinline__ void phase_c(const Wave &w, int it, PhaseSmem &sm)
 
 // One phase of one iteration.  (A cooperative single-launch form with grid
 // barriers between the phases measured slower: 80 us against 62 us per
-// iteration on a 1.1M-edge wave.)
+// iteration of the three-launch form on a 1.1M-edge wave.)
 template <int P>
 __global__ __launch_bounds__(TPB) void mod_phase(Wave w, int it)
 {
     __shared__ PhaseSmem sm;
-    if (P == 0) phase_a(w, it, sm);
-    else if (P == 1) phase_b(w, it, sm);
-    else phase_c(w, it, sm);
+    if (P == 0) phase_s(w, it, sm);
+    else phase_n(w, it, sm);
 }
 
-// v = global ids of the nodes of groups in list[0..m) (start of a pass)
-__global__ void mod_start(Wave w, const int *list, int m)
+// Groups flagged in restart[]: buf[0] = global ids (the start vector) and the
+// partials of x . k for the first iteration.
+__global__ __launch_bounds__(TPB) void mod_start(Wave w)
 {
-    const int g = list[blockIdx.y];
-    for (int i = w.no[g] + blockIdx.x * blockDim.x + threadIdx.x; i < w.no[g] + w.ng[g]; i += gridDim.x * blockDim.x)
-        w.v[i] = (double)w.gid[i];
+#pragma clang fp contract(off)
+    __shared__ double red[TPB];
+    for (int b = blockIdx.x; b < w.nblk; b += gridDim.x) {
+        const int g = w.bgrp[b];
+        if (!w.restart[g]) continue;
+        const int i0 = w.no[g] + (b - w.fb[g]) * w.npb, i1 = min(w.no[g] + w.ng[g], i0 + w.npb);
+        double pk = 0.0;
+        for (int i = i0 + threadIdx.x; i < i1; i += TPB) {
+            const double x = (double)w.gid[i];
+            w.buf[0][i] = x;
+            pk += x * w.k[i];
+        }
+        pk = block_sum(pk, red);
+        if (threadIdx.x == 0) w.pdot[b] = pk;
+    }
 }
 
-// lam[g] = +-|v| by the sign of v[mi] v0[mi] at the first largest |v[mi]|
-// (the reference's strict '>' scan), for the groups in list[0..m)
+// For the groups in list[]: lam[g] = +-|v| by the sign of v[mi] v0[mi] at the
+// first largest |v[mi]| (the reference's strict '>' scan), v = the final
+// vector, v0 = the one before; vres[] = v.
 __global__ __launch_bounds__(TPB) void mod_lambda(Wave w, const int *list)
 {
     __shared__ double rv[TPB];
     __shared__ int ri[TPB];
     const int g = list[blockIdx.x], tid = threadIdx.x;
     const int a = w.no[g], z = a + w.ng[g];
+    const int li = w.iters[g];
+    const double *v = w.buf[li & 1], *v0 = w.buf[(li + 1) & 1];
     double best = -1.0;
     int bi = 0x7fffffff;
     for (int i = a + tid; i < z; i += TPB) {
-        const double x = fabs(w.v[i]);
+        const double x = fabs(v[i]);
+        w.vres[i] = v[i];
         if (x > best) { best = x; bi = i; }
     }
     rv[tid] = best;
@@ -230,7 +239,7 @@ __global__ __launch_bounds__(TPB) void mod_lambda(Wave w, const int *list)
     if (tid == 0) {
         const int mi = ri[0];
         const double vn = w.vnorm[g];
-        w.lam[g] = (mi < z && w.v[mi] * w.v0[mi] > 0) ? vn : -vn;
+        w.lam[g] = (mi < z && v[mi] * v0[mi] > 0) ? vn : -vn;
     }
 }
 
@@ -298,11 +307,17 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
             if (res[i] >= lo && res[i] < hi) gcount[res[i] - lo]++;
         std::vector<int> no((size_t)G), ng((size_t)G), fb((size_t)G), nb((size_t)G);
         int nnodes = 0, nblk = 0;
+        // small work blocks for parallelism, large enough that no group has
+        // more than GSUM_MAX block partials to sum
+        int gmax = 1;
+        for (int g = 0; g < G; g++) gmax = std::max(gmax, gcount[g]);
+        int npb = 64;
+        while (npb < NPB_MAX && pg_cdiv(gmax, npb) > GSUM_MAX) npb *= 2;
         for (int g = 0; g < G; g++) {
             no[g] = nnodes;
             ng[g] = gcount[g];
             fb[g] = nblk;
-            nb[g] = (int)std::max<long long>(1, pg_cdiv(ng[g], NPB));
+            nb[g] = (int)std::max<long long>(1, pg_cdiv(ng[g], npb));
             nnodes += ng[g];
             nblk += nb[g];
         }
@@ -340,9 +355,10 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
         const size_t gn = scol.size();
 
         Wave w;
-        int *d_bgrp, *d_no, *d_ng, *d_fb, *d_nb, *d_gid, *d_srow, *d_scol, *d_done, *d_stop, *d_iters, *d_it0, *d_list;
+        int *d_bgrp, *d_no, *d_ng, *d_fb, *d_nb, *d_gid, *d_srow, *d_scol, *d_restart, *d_stop, *d_iters, *d_it0,
+            *d_list;
         float *d_sval;
-        double *d_k, *d_sumbg, *d_beta, *d_vnorm, *d_lam, *d_v, *d_v0, *d_pdot, *d_pss, *d_pe1, *d_pe2;
+        double *d_k, *d_sumbg, *d_beta, *d_vnorm, *d_lam, *d_b0, *d_b1, *d_vres, *d_pdot, *d_pss, *d_pe1, *d_pe2;
         PG_TRY(pg_bufT(ctx, "mod.bgrp", (size_t)nblk, &d_bgrp));
         PG_TRY(pg_bufT(ctx, "mod.no", (size_t)G, &d_no));
         PG_TRY(pg_bufT(ctx, "mod.ng", (size_t)G, &d_ng));
@@ -354,7 +370,7 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
         PG_TRY(pg_bufT(ctx, "mod.sval", gn + 1, &d_sval));
         PG_TRY(pg_bufT(ctx, "mod.k", (size_t)nnodes + 1, &d_k));
         PG_TRY(pg_bufT(ctx, "mod.sumbg", (size_t)nnodes + 1, &d_sumbg));
-        PG_TRY(pg_bufT(ctx, "mod.done", (size_t)G, &d_done));
+        PG_TRY(pg_bufT(ctx, "mod.restart", (size_t)G, &d_restart));
         PG_TRY(pg_bufT(ctx, "mod.stop", (size_t)G, &d_stop));
         PG_TRY(pg_bufT(ctx, "mod.iters", (size_t)G, &d_iters));
         PG_TRY(pg_bufT(ctx, "mod.it0", (size_t)G, &d_it0));
@@ -362,8 +378,9 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
         PG_TRY(pg_bufT(ctx, "mod.beta", (size_t)G, &d_beta));
         PG_TRY(pg_bufT(ctx, "mod.vnorm", (size_t)G, &d_vnorm));
         PG_TRY(pg_bufT(ctx, "mod.lam", (size_t)G, &d_lam));
-        PG_TRY(pg_bufT(ctx, "mod.v", (size_t)nnodes + 1, &d_v));
-        PG_TRY(pg_bufT(ctx, "mod.v0", (size_t)nnodes + 1, &d_v0));
+        PG_TRY(pg_bufT(ctx, "mod.b0", (size_t)nnodes + 1, &d_b0));
+        PG_TRY(pg_bufT(ctx, "mod.b1", (size_t)nnodes + 1, &d_b1));
+        PG_TRY(pg_bufT(ctx, "mod.vres", (size_t)nnodes + 1, &d_vres));
         PG_TRY(pg_bufT(ctx, "mod.pdot", (size_t)nblk, &d_pdot));
         PG_TRY(pg_bufT(ctx, "mod.pss", (size_t)nblk, &d_pss));
         PG_TRY(pg_bufT(ctx, "mod.pe1", (size_t)nblk, &d_pe1));
@@ -385,12 +402,14 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
         PG_TRY(h2d(d_sval, sval.data(), gn * 4));
         PG_TRY(h2d(d_k, kw.data(), (size_t)nnodes * 8));
         PG_TRY(h2d(d_sumbg, sumbg.data(), (size_t)nnodes * 8));
-        PG_TRY(h2d(d_done, zeros.data(), (size_t)G * 4));
+        std::vector<int> ones((size_t)G, 1);
+        PG_TRY(h2d(d_restart, ones.data(), (size_t)G * 4));
         PG_TRY(h2d(d_stop, zeros.data(), (size_t)G * 4));
         PG_TRY(h2d(d_it0, zeros.data(), (size_t)G * 4));
         PG_TRY(h2d(d_beta, dzeros.data(), (size_t)G * 8));
         w.nnodes = nnodes;
         w.nblk = nblk;
+        w.npb = npb;
         w.bgrp = d_bgrp;
         w.no = d_no;
         w.ng = d_ng;
@@ -404,15 +423,16 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
         w.sumbg = d_sumbg;
         w.m2 = m2;
         w.ngroups = G;
-        w.done = d_done;
         w.stop = d_stop;
+        w.restart = d_restart;
         w.iters = d_iters;
         w.it0 = d_it0;
         w.beta = d_beta;
         w.vnorm = d_vnorm;
         w.lam = d_lam;
-        w.v = d_v;
-        w.v0 = d_v0;
+        w.buf[0] = d_b0;
+        w.buf[1] = d_b1;
+        w.vres = d_vres;
         w.pdot = d_pdot;
         w.pss = d_pss;
         w.pe1 = d_pe1;
@@ -421,10 +441,7 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
         // pass state per group: 0 first pass, 1 shifted pass, 2 finished
         std::vector<int> pass((size_t)G, 0), list, hit0((size_t)G, 0);
         std::vector<double> lam_total((size_t)G, 0.0), hbeta((size_t)G, 0.0);
-        list.resize(G);
-        for (int g = 0; g < G; g++) list[g] = g;
-        PG_TRY(h2d(d_list, list.data(), (size_t)G * 4));
-        mod_start<<<dim3(4, (unsigned)G), 256, 0, st>>>(w, d_list, G);
+        mod_start<<<(unsigned)nblk, TPB, 0, st>>>(w);
         PG_KERNEL_CHECK(ctx);
         int it = 0, running = G;
         std::vector<int> hdone((size_t)G);
@@ -433,11 +450,10 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
             for (int q = 0; q < BATCH; q++) {
                 mod_phase<0><<<(unsigned)nblk, TPB, 0, st>>>(w, it + q);
                 mod_phase<1><<<(unsigned)nblk, TPB, 0, st>>>(w, it + q);
-                mod_phase<2><<<(unsigned)nblk, TPB, 0, st>>>(w, it + q);
             }
             it += BATCH;
             PG_KERNEL_CHECK(ctx);
-            PG_HIP(ctx, hipMemcpyAsync(hdone.data(), d_done, (size_t)G * 4, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipMemcpyAsync(hdone.data(), d_stop, (size_t)G * 4, hipMemcpyDeviceToHost, st));
             PG_HIP(ctx, hipStreamSynchronize(st));
             list.clear();
             for (int g = 0; g < G; g++)
@@ -471,25 +487,26 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
             }
             if (!restart.empty()) {
                 // second pass on B + beta I, beta = -lambda, from the start vector
-                const int mr = (int)restart.size();
-                PG_TRY(h2d(d_list, restart.data(), (size_t)mr * 4));
-                mod_start<<<dim3(4, (unsigned)mr), 256, 0, st>>>(w, d_list, mr);
-                PG_KERNEL_CHECK(ctx);
-                // no iteration ran since hdone was read: it is the device state
+                // no iteration ran since hdone (stop flags) was read: it is
+                // the device state
+                std::vector<int> rflag((size_t)G, 0);
                 for (int g : restart) {
                     hbeta[g] = -lam_total[g];
                     hit0[g] = it;
                     hdone[g] = 0;
+                    rflag[g] = 1;
                 }
+                PG_TRY(h2d(d_restart, rflag.data(), (size_t)G * 4));
+                mod_start<<<(unsigned)nblk, TPB, 0, st>>>(w);
+                PG_KERNEL_CHECK(ctx);
                 PG_TRY(h2d(d_beta, hbeta.data(), (size_t)G * 8));
                 PG_TRY(h2d(d_it0, hit0.data(), (size_t)G * 4));
-                PG_TRY(h2d(d_done, hdone.data(), (size_t)G * 4));
-                PG_TRY(h2d(d_stop, hdone.data(), (size_t)G * 4));  // stop == done between launches
+                PG_TRY(h2d(d_stop, hdone.data(), (size_t)G * 4));
                 PG_HIP(ctx, hipStreamSynchronize(st));
             }
         }
         std::vector<double> hv((size_t)nnodes);
-        PG_HIP(ctx, hipMemcpyAsync(hv.data(), d_v, (size_t)nnodes * 8, hipMemcpyDeviceToHost, st));
+        PG_HIP(ctx, hipMemcpyAsync(hv.data(), d_vres, (size_t)nnodes * 8, hipMemcpyDeviceToHost, st));
         PG_HIP(ctx, hipStreamSynchronize(st));
         if (trace) {
             std::vector<int> hits((size_t)G);

@@ -137,6 +137,12 @@ def run_metrics(a):
     out["bc_all_sources"] = {"sources": n, "seconds": dt, "gteps": te / dt / 1e9,
                              "hbm_model_frac": n * (16.0 * nnz + 40.0 * n) / dt / HBM_PEAK}
     log(f"bc all {n} sources: {dt:.2f} s = {te / dt / 1e9:.1f} GTEPS")
+    eff = torch.empty(n, dtype=torch.float32, device="cuda")
+    ctx.bfs_efficiency(row, col, eff)  # first call sizes the context's buffers
+    (_, lp, iso, unc), dt = _timed(ctx, lambda: ctx.bfs_efficiency(row, col, eff))
+    out["bfs_efficiency_all_sources"] = {"seconds": dt, "gteps": te / dt / 1e9, "lp": lp, "isolated": iso,
+                                         "unconnected": unc}
+    log(f"bfs efficiency all {n} sources: {dt * 1e3:.1f} ms = {te / dt / 1e9:.1f} GTEPS, Lp {lp:.6g}")
 
     # Floyd-Warshall on the first 8192 voxels: weighted graph r > t*, d = 1 - r
     nf = 8192

@@ -151,6 +151,17 @@ pg_status pg_bc(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
 pg_status pg_bfs(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
                  int64_t nnz, const int32_t *sources, int64_t nsrc, int32_t *level);
 
+/* Binary nodal efficiency and characteristic path length from all-pairs BFS
+ * hop counts (U/BFS_MulCPU/BFS_MulCPU/APSP_BFS.cpp:26-192, replaces APSP_BFS):
+ * e_i = (FP32 sum over j != i, in increasing j, of 1.0f / d_ij) / (n - 1),
+ * unreachable pairs skipped; Lp = 1 / (S / n / (n - 1)) / 2 with S the FP64
+ * sum of 1.0f / d_ij over reachable i < j.  The graph must be undirected
+ * (symmetric CSR).  isolated counts e_i == 0; unconnected is 1 when some pair
+ * is unreachable.  eff, lp, isolated, unconnected may be NULL. */
+pg_status pg_bfs_efficiency(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
+                            int64_t nnz, float *eff, double *lp, int32_t *isolated,
+                            int32_t *unconnected);
+
 /* Blocked Floyd-Warshall closure of a dense row-major n x n FP32 distance
  * matrix, in place (+inf = no edge).  Replaces cuAPSP (W/Lp/CUBFW_Lp/CUBFW_Lp/
  * CUBFW.cpp:19) and the tile kernels of CUBFW_ph1/2/3.cu.  block: tile edge

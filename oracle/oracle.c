@@ -26,6 +26,10 @@
  *   orc_bc              networkx betweenness x2 fixtures, closed forms on paths
  *                       and stars, the compiled reference Betweenness_CPU
  *   orc_bfs             networkx shortest_path_length fixtures
+ *   orc_bfs_efficiency  networkx global_efficiency (= 1/Lp) and FP64 nodal sums
+ *                       from networkx hop counts; the FP32 nodal sums against a
+ *                       numpy float32 sequential sum; closed forms on complete
+ *                       graphs, stars and paths
  *   orc_fw_blocked      the compiled reference BFW_C (bit-exact), networkx
  *                       floyd_warshall_numpy, unit weights == BFS hops
  *   orc_modularity      disjoint cliques split exactly, Q against networkx
@@ -432,6 +436,45 @@ void orc_bfs(const int32_t *row, const int32_t *col, int64_t n, int32_t src, int
         }
     }
     free(q);
+}
+
+/* U BFS_MulCPU/BFS_MulCPU/APSP_BFS.cpp:100-133.  The reference fills a dense
+ * float hop matrix (0 where unreachable and on the diagonal) and then walks
+ * i < j in row-major order: a pair with d < 1 marks the network unconnected,
+ * otherwise r = 1.0f / d is added to the FP64 Lp sum and to the FP32 sums of
+ * both ends; after row i, e_i /= (n - 1) and a zero e_i counts as isolated.
+ * Lp = 1 / (sum / n / (n - 1)) / 2.  Row i's hops come from one BFS from i
+ * (d(i -> j), the value the matrix holds at [i][j]), so no n x n buffer. */
+double orc_bfs_efficiency(const int32_t *row, const int32_t *col, int64_t n, float *eff,
+                          int32_t *isolated, int32_t *unconnected)
+{
+    int32_t *lev = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    double lp = 0.0;
+    int32_t iso = 0, unc = 0;
+    for (int64_t v = 0; v < n; v++) eff[v] = 0.0f;
+    for (int64_t i = 0; i < n; i++) {
+        orc_bfs(row, col, n, (int32_t)i, lev);
+        for (int64_t j = i + 1; j < n; j++) {
+            const float d = lev[j] < 0 ? 0.0f : (float)lev[j];
+            if (d < 1) {
+                unc = 1;
+            } else {
+                const float r = 1.0f / d;
+                lp += r;
+                eff[i] += r;
+                eff[j] += r;
+            }
+        }
+        eff[i] /= (float)(n - 1);
+        if (eff[i] == 0) iso++;
+    }
+    free(lev);
+    lp /= (double)n;
+    lp /= (double)(n - 1);
+    lp = 1 / lp / 2;
+    if (isolated) *isolated = iso;
+    if (unconnected) *unconnected = unc;
+    return lp;
 }
 
 /* ----------------------------------------------------------- Floyd-Warshall */

@@ -69,6 +69,12 @@ void orc_bc(const int32_t *row, const int32_t *col, int64_t n,
 /* BFS hop levels from one source (-1 unreachable) (U APSP_BFS.cpp:150-192). */
 void orc_bfs(const int32_t *row, const int32_t *col, int64_t n, int32_t src, int32_t *level);
 
+/* Binary nodal efficiency and Lp from BFS hop counts (U APSP_BFS.cpp:100-133):
+ * the reference's i < j loop order, FP32 nodal sums, FP64 Lp sum; returns Lp.
+ * isolated counts e_i == 0; unconnected is 1 when some pair is unreachable. */
+double orc_bfs_efficiency(const int32_t *row, const int32_t *col, int64_t n, float *eff,
+                          int32_t *isolated, int32_t *unconnected);
+
 /* Blocked Floyd-Warshall, row-major n x n FP32 in place, block size b
  * (W BFW.cpp:26-108 on a padded matrix, W FormBlock.cpp:45-75). */
 void orc_fw_blocked(float *d, int64_t n, int64_t b);

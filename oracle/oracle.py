@@ -55,6 +55,9 @@ def lib():
         L.orc_cp_weighted.restype = C.c_double
         L.orc_bc.argtypes = [i32p, i32p, C.c_int64, C.c_void_p, C.c_int64, f64p]
         L.orc_bfs.argtypes = [i32p, i32p, C.c_int64, C.c_int32, i32p]
+        L.orc_bfs_efficiency.argtypes = [i32p, i32p, C.c_int64, f32p, C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32)]
+        L.orc_bfs_efficiency.restype = C.c_double
         L.orc_fw_blocked.argtypes = [f32p, C.c_int64, C.c_int64]
         L.orc_fw_init.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, C.c_int, f32p]
         L.orc_modularity.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, C.c_int, i32p,
@@ -191,6 +194,16 @@ def bfs(row, col, src):
     lev = np.empty(n, np.int32)
     lib().orc_bfs(np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32), n, src, lev)
     return lev
+
+
+def bfs_efficiency(row, col):
+    """(eff float32[n], Lp, isolated, unconnected) of a binary network."""
+    n = row.size - 1
+    eff = np.empty(n, np.float32)
+    iso, unc = C.c_int32(0), C.c_int32(0)
+    lp = lib().orc_bfs_efficiency(np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32), n,
+                                  eff, C.byref(iso), C.byref(unc))
+    return eff, lp, iso.value, unc.value
 
 
 def fw_init(row, col, val, kind):

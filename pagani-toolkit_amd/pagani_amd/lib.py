@@ -86,6 +86,7 @@ _SIGS = {
     "pg_cp_weighted": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_int32, C.c_int32, _VP, _VP]),
     "pg_bc": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, C.c_int64, _VP]),
     "pg_bfs": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, C.c_int64, _VP]),
+    "pg_bfs_efficiency": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP]),
     "pg_fw": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32]),
     "pg_fw_init": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_int32, _VP]),
     "pg_fw_efficiency": (C.c_int, [_VP, _VP, C.c_int32, _VP, _VP, _VP]),
@@ -239,6 +240,16 @@ class Context:
         out = np.empty((ns, n), np.int32) if out is None else out
         self._check(self._lib.pg_bfs(self._h, _ptr(row), _ptr(col), n, _len(col), _ptr(sources), ns, _ptr(out)))
         return out
+
+    def bfs_efficiency(self, row, col, eff=None):
+        """(eff[n] float32, Lp, isolated, unconnected) of a binary network."""
+        n = _len(row) - 1
+        eff = np.empty(n, np.float32) if eff is None else eff
+        lp, iso, unc = C.c_double(0), C.c_int32(0), C.c_int32(0)
+        self._check(self._lib.pg_bfs_efficiency(self._h, _ptr(row), _ptr(col), n, _len(col), _ptr(eff),
+                                                 C.cast(C.pointer(lp), _VP), C.cast(C.pointer(iso), _VP),
+                                                 C.cast(C.pointer(unc), _VP)))
+        return eff, lp.value, iso.value, unc.value
 
     def fw(self, d, block: int = 0):
         """Closure in place; d is an n x n float32 array/tensor."""

@@ -44,6 +44,13 @@ def graph_fixture(name, g, weighted):
         for v, dd in nx.single_source_shortest_path_length(g, int(s)).items():
             lev[k, v] = dd
     out["bfs_sources"], out["bfs_levels"] = src, lev
+    # binary nodal efficiency in FP64 from networkx hop counts, and the global
+    # efficiency (mean of 1/d over ordered pairs, 0 for unreachable) = 1 / Lp
+    eff = np.zeros(n)
+    for i, dist in nx.all_pairs_shortest_path_length(g):
+        eff[i] = sum(1.0 / dd for j, dd in dist.items() if j != i) / (n - 1)
+    out["nodal_eff"] = eff
+    out["global_eff"] = np.float64(nx.global_efficiency(g))
     if weighted:
         wcl = nx.clustering(g, weight="weight")
         out["clustering_onnela"] = np.array([wcl[i] for i in range(n)])

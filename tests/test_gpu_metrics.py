@@ -178,6 +178,47 @@ def test_fw_bit_exact(ctx, weighted_graph, block, n):
     assert abs(lp - 1.0 / (inv.sum() / n / (n - 1))) <= 1e-9 * lp
 
 
+def _check_eff(got, want):
+    e, lp, iso, unc = got
+    oe, olp, oiso, ounc = want
+    np.testing.assert_array_equal(e.view(np.uint32), oe.view(np.uint32))
+    # FP64 Lp sums in another grouping (per-source shares, then over sources)
+    assert abs(lp - olp) <= 1e-12 * abs(olp)
+    assert (iso, unc) == (oiso, ounc)
+
+
+@pytest.mark.parametrize("name", list(_graphs()))
+def test_bfs_efficiency_small(ctx, name):
+    row, col = _graphs()[name][:2]
+    say(f"bfs efficiency {name}")
+    _check_eff(ctx.bfs_efficiency(row, col), O.bfs_efficiency(row, col))
+
+
+def test_bfs_efficiency_deep_path(ctx):
+    row, col = path_graph(300)[:2]  # depth 299: nine level planes
+    say("bfs efficiency path 300")
+    _check_eff(ctx.bfs_efficiency(row, col), O.bfs_efficiency(row, col))
+
+
+@pytest.mark.parametrize("batch", [0, 4096])
+def test_bfs_efficiency_plain(ctx, plain_graph, batch, monkeypatch):
+    row, col = plain_graph
+    if batch:
+        monkeypatch.setenv("PAGANI_EFF_BATCH", str(batch))  # two batches of sources
+    say(f"bfs efficiency plain batch={batch}")
+    _check_eff(ctx.bfs_efficiency(row, col), O.bfs_efficiency(row, col))
+
+
+def test_bfs_efficiency_rejects_bad_csr(ctx):
+    row, col = path_graph(10)[:2]
+    bad = col.copy()
+    bad[3] = 10
+    say("bfs efficiency bad csr")
+    from pagani_amd.lib import PgError
+    with pytest.raises(PgError, match="malformed CSR"):
+        ctx.bfs_efficiency(row, bad)
+
+
 def test_fw_unit_weights_equal_bfs(ctx):
     row, col = random_graph(200, 0.03, 5)[:2]
     d = ctx.fw_init(row, col, None, 2)

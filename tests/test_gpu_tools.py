@@ -93,6 +93,17 @@ def test_metric_tools_on_binary_networks(volume):
     assert len(iso) == len(_csr_files(d))
 
 
+def test_bfs_lp_tool_on_binary_networks(volume):
+    d, subs, n = volume
+    run("CUBFS_Lp", d, 0, "b")
+    for f in _csr_files(d):
+        base = str(d / f)[:-4]
+        row, col, _ = formats.read_csr(str(d / f))
+        eff, lp, _, _ = O.bfs_efficiency(row, col)
+        np.testing.assert_array_equal(formats.read_nm(base + "_eff.nm").view(np.uint32), eff.view(np.uint32))
+        assert abs(float(open(base + "_Lp.txt").read()) - lp) <= 5e-6 * lp  # six significant digits
+
+
 def test_lp_and_modularity_tools_on_weighted_networks(volume):
     d, subs, n = volume
     w = d / "weighted"

@@ -221,6 +221,30 @@ def test_fw_unit_weights_equal_bfs():
         np.testing.assert_array_equal(d[s], lev)
 
 
+def test_bfs_efficiency_closed_forms():
+    n = 37
+    row, col = complete_graph(n)[:2]
+    eff, lp, iso, unc = O.bfs_efficiency(row, col)
+    assert np.all(eff == 1.0) and lp == 1.0 and iso == 0 and unc == 0
+    row, col = star_graph(n)[:2]  # centre 0, n - 1 leaves
+    eff, lp, iso, unc = O.bfs_efficiency(row, col)
+    assert eff[0] == 1.0
+    assert np.all(eff[1:] == np.float32((1 + (n - 2) * 0.5) / (n - 1)))
+    want = (n * (n - 1) / 2) / ((n - 1) + (n - 1) * (n - 2) / 2 * 0.5)
+    assert abs(lp - want) <= 1e-15 * want
+    row, col = path_graph(n)[:2]
+    eff, lp, iso, unc = O.bfs_efficiency(row, col)
+    exact = sum((n - k) / k for k in range(1, n))
+    assert abs(lp - (n * (n - 1) / 2) / exact) <= 1e-6 * lp
+    # two components and an isolated vertex
+    r1, c1 = complete_graph(5)[:2]
+    row = np.concatenate([r1, r1[1:] + r1[-1], [2 * r1[-1]]]).astype(np.int32)
+    col = np.concatenate([c1, c1 + 5]).astype(np.int32)
+    eff, lp, iso, unc = O.bfs_efficiency(row, col)
+    assert unc == 1 and iso == 1 and eff[10] == 0
+    assert np.all(eff[:10] == np.float32(np.float32(4) / np.float32(10)))
+
+
 def test_modularity_cliques_split_exactly():
     row, col = cliques([10, 10, 10, 10])[:2]
     lab, q, lam, eig, nr = O.modularity(row, col)

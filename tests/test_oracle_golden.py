@@ -33,6 +33,27 @@ def test_graph_metrics_match_networkx(path):
     assert abs(q - float(g["partition_q"])) < 1e-12
 
 
+@pytest.mark.parametrize("path", GRAPHS, ids=[os.path.basename(p) for p in GRAPHS])
+def test_bfs_efficiency_matches_networkx(path):
+    g = load(path)
+    row, col = g["row"], g["col"]
+    n = row.size - 1
+    eff, lp, iso, unc = O.bfs_efficiency(row, col)
+    # FP32 sums of float reciprocals against FP64 networkx values
+    np.testing.assert_allclose(eff, g["nodal_eff"], rtol=1e-6, atol=1e-12)
+    assert abs(1.0 / lp - float(g["global_eff"])) <= 1e-6 * float(g["global_eff"])
+    assert iso == int((g["nodal_eff"] == 0).sum())
+    assert unc == int(np.any(g["bfs_levels"] < 0))
+    # the FP32 nodal sum is sequential over j != i in increasing j:
+    # numpy float32 add.accumulate of float32 reciprocals of the hop counts
+    for i in range(0, n, max(1, n // 25)):
+        lev = O.bfs(row, col, i)
+        m = (lev > 0)
+        r = np.float32(1.0) / lev[m].astype(np.float32)
+        s = np.add.accumulate(r, dtype=np.float32)[-1] if r.size else np.float32(0)
+        assert (s / np.float32(n - 1)).view(np.uint32) == eff[i].view(np.uint32), i
+
+
 def test_weighted_clustering_and_fw_match_networkx():
     g = load(os.path.join(GOLD, "graph_gnp150_weighted.npz"))
     row, col, val = g["row"], g["col"], g["val"]

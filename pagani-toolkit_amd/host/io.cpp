@@ -64,7 +64,7 @@ bool make_dir(const std::string &dir)
     return mkdir(dir.c_str(), 0755) == 0;
 }
 
-bool read_mask(const std::string &path, std::vector<float> &mask, int dims[3], std::string &err)
+bool read_mask(const std::string &path, std::vector<float> &mask, int dims[3], std::string &err, short *hdr)
 {
     std::ifstream f(path, std::ios::binary);
     if (!f.good()) {
@@ -76,6 +76,7 @@ bool read_mask(const std::string &path, std::vector<float> &mask, int dims[3], s
         err = "short header in " + path;
         return false;
     }
+    if (hdr) std::memcpy(hdr, h, HDR);
     dims[0] = h[21];
     dims[1] = h[22];
     dims[2] = h[23];

@@ -31,8 +31,10 @@ std::string strip_ext(const std::string &path);
 bool make_dir(const std::string &dir);
 
 // NIfTI mask: values converted to float (uint8 or float32 storage).  Returns
-// false with a message for other datatypes (U CorMat.cpp:150-167).
-bool read_mask(const std::string &path, std::vector<float> &mask, int dims[3], std::string &err);
+// false with a message for other datatypes (U CorMat.cpp:150-167).  hdr, when
+// given, receives the 352-byte header as 176 shorts.
+bool read_mask(const std::string &path, std::vector<float> &mask, int dims[3], std::string &err,
+               short *hdr = nullptr);
 // 4-D series gathered through the mask (mask >= thr), time-major T x N as the
 // reference's BOLD[l*N+i]; float64 input is narrowed to float32.
 bool read_bold(const std::string &path, const std::vector<float> &mask, float thr, int &n, int &t,

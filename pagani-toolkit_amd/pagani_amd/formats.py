@@ -89,3 +89,31 @@ def write_nm(path: str, x) -> None:
 def csr_name(subject_base: str, n: int, s_thresh: float, r_thresh: float, weighted: bool = False) -> str:
     """U CorMat.cpp:284,333-335: <base>_<N>_spa%.3f%%_cor%.3f.csr."""
     return f"{subject_base}_{n}_spa{s_thresh:.3f}%_cor{r_thresh:.3f}" + ("_weighted.csr" if weighted else ".csr")
+
+
+def nm_to_nii(nm_path: str, mask_path: str, out_path: str, thr: float = 0.5) -> bool:
+    """One .nm vector back to a NIfTI volume through the mask, as the
+    ConvertNII tool does (U/ConvertNII/ConvertNII/ConvertNII.cpp:107-170): the
+    mask's 352-byte header with datatype 16 and bitpix 32, then float32 values
+    at voxels with (double) mask >= thr, 0 elsewhere.  Returns False (and
+    writes nothing) when the vector length differs from the mask's count."""
+    raw = open(mask_path, "rb").read()
+    hdr = np.frombuffer(raw, np.int16, HDR // 2).copy()
+    total = int(hdr[21]) * int(hdr[22]) * int(hdr[23])
+    if hdr[35] == DT_UINT8:
+        mask = np.frombuffer(raw, np.uint8, total, HDR).astype(np.float32)
+    elif hdr[35] == DT_FLOAT32:
+        mask = np.frombuffer(raw, np.float32, total, HDR)
+    else:
+        raise ValueError("mask data-type error, only unsigned char and float can be handled")
+    sel = mask.astype(np.float64) >= float(thr)
+    x = read_nm(nm_path)
+    if x.size != int(sel.sum()):
+        return False
+    vol = np.zeros(total, np.float32)
+    vol[sel] = x
+    hdr[35], hdr[36] = DT_FLOAT32, 32
+    with open(out_path, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(vol.tobytes())
+    return True

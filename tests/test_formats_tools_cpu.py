@@ -50,7 +50,8 @@ def test_csr_name_matches_reference_printf():
 
 @pytest.mark.parametrize("tool,args", [
     ("CUCorMat", []), ("CUBC", []), ("Degree", []), ("Cp", ["x"]), ("CUBFW_Lp", ["x", "0"]),
-    ("CUmodularity", ["x"]),
+    ("CUmodularity", ["x"]), ("CUBFS_Lp", ["x", "0"]), ("CUBFS_Lp", ["/nonexistent", "2", "b"]),
+    ("ConvertNII", ["x"]), ("ConvertNII", ["x", "y", "0.5", "z"]),
     ("CUCorMat", ["/nonexistent", "0.5", "y", "n", "r", "0.3"]),   # averaging not offered
     ("CUCorMat", ["/nonexistent", "0.5", "n", "n", "q", "0.3"]),   # bad threshold type
     ("Cp", ["/nonexistent", "3", "gk"]),                             # random networks not offered
@@ -61,3 +62,55 @@ def test_tools_reject_bad_arguments(tool, args):
         pytest.skip("tools not built")
     r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1, (r.returncode, r.stdout, r.stderr)
+
+
+def _mask_volume(path, values, dt, dims=(5, 4, 3)):
+    formats.write_nifti(str(path), values, dt)
+    raw = bytearray(path.read_bytes())
+    h = np.frombuffer(bytes(raw[:352]), np.int16).copy()
+    h[21], h[22], h[23] = dims
+    h[40] = 1234  # an unrelated header word the conversion must carry over
+    raw[:352] = h.tobytes()
+    path.write_bytes(bytes(raw))
+
+
+@pytest.mark.parametrize("dt,thr", [(formats.DT_UINT8, None), (formats.DT_FLOAT32, "0.7")])
+def test_convertnii_tool(tmp_path, dt, thr):
+    exe = os.path.join(BIN, "ConvertNII")
+    if not os.path.exists(exe):
+        pytest.skip("tools not built")
+    rng = np.random.default_rng(5)
+    if dt == formats.DT_UINT8:
+        m = (rng.random(60) < 0.6).astype(np.uint8)
+        sel = m >= 1
+    else:
+        # 0.7f is below the double 0.7: the reference's double cut drops it
+        m = rng.choice(np.array([0.0, 0.5, 0.7, 0.9, 1.0], np.float32), 60)
+        sel = m.astype(np.float64) >= 0.7
+        assert (m == np.float32(0.7)).any()
+    d = tmp_path / "res"
+    d.mkdir()
+    _mask_volume(tmp_path / "mask.nii", m, dt)
+    x = rng.standard_normal(int(sel.sum())).astype(np.float32)
+    formats.write_nm(str(d / "sub01_deg.nm"), x)
+    formats.write_nm(str(d / "sub02_deg.nm"), x[:-1])  # wrong length: skipped
+    formats.write_nm(str(d / "notes.nmx"), x)          # not a .nm file
+    args = [str(d), str(tmp_path / "mask.nii")] + ([thr] if thr else [])
+    r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    assert "doesn't match" in r.stdout
+    assert not (d / "sub02_deg.nii").exists() and not (d / "notes.nii").exists()
+    out = (d / "sub01_deg.nii").read_bytes()
+    mh = np.frombuffer((tmp_path / "mask.nii").read_bytes()[:352], np.int16)
+    oh = np.frombuffer(out[:352], np.int16)
+    diff = np.nonzero(mh != oh)[0].tolist()
+    assert set(diff) <= {35, 36} and oh[35] == 16 and oh[36] == 32 and oh[40] == 1234
+    vol = np.frombuffer(out[352:], np.float32)
+    assert vol.size == 60
+    np.testing.assert_array_equal(vol[sel], x)
+    assert np.all(vol[~sel] == 0)
+    # the Python mirror writes the same bytes
+    assert formats.nm_to_nii(str(d / "sub01_deg.nm"), str(tmp_path / "mask.nii"), str(tmp_path / "py.nii"),
+                             float(thr) if thr else 0.5)
+    assert (tmp_path / "py.nii").read_bytes() == out
+    assert not formats.nm_to_nii(str(d / "sub02_deg.nm"), str(tmp_path / "mask.nii"), str(tmp_path / "q.nii"))

@@ -143,6 +143,20 @@ def run_metrics(a):
     out["bfs_efficiency_all_sources"] = {"seconds": dt, "gteps": te / dt / 1e9, "lp": lp, "isolated": iso,
                                          "unconnected": unc}
     log(f"bfs efficiency all {n} sources: {dt * 1e3:.1f} ms = {te / dt / 1e9:.1f} GTEPS, Lp {lp:.6g}")
+    ctx.eigenvector_centrality(row, col)
+    (_, _, lam, it), dt = _timed(ctx, lambda: ctx.eigenvector_centrality(row, col))
+    # per iteration: the column indices (4 B per stored entry) and one FP64
+    # gather per entry from an L2-resident vector; 3 FP64 vectors of n
+    spmv_bytes = 4.0 * nnz + 8.0 * 3 * n
+    out["eigenvector_centrality"] = {"seconds": dt, "iterations": it, "eigenvalue": lam,
+                                     "us_per_iteration": dt * 1e6 / max(it, 1),
+                                     "hbm_frac": spmv_bytes * it / dt / HBM_PEAK}
+    log(f"eigenvector centrality: {it} iterations, {dt * 1e3:.1f} ms, eigenvalue {lam:.6g}")
+    lab = (torch.arange(n, dtype=torch.int32, device="cuda") % 64) + 1
+    ctx.participation(row, col, lab)
+    (_, _, k), dt = _timed(ctx, lambda: ctx.participation(row, col, lab))
+    out["participation_ms"] = dt * 1e3
+    log(f"participation coefficient, 64 modules: {dt * 1e3:.2f} ms")
 
     # Floyd-Warshall on the first 8192 voxels: weighted graph r > t*, d = 1 - r
     nf = 8192

@@ -197,6 +197,28 @@ pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_t *col,
                         int32_t *labels, double *q, double *lambda0, double *eig0,
                         int32_t *rounds);
 
+/* Eigenvector centrality by the power iteration of CUEC (U/CUEC/CUEC/
+ * CUEC_unweighted.cu:61-196; weighted: W/CUEC/weighted-net.cu:55-203): start
+ * vector 0..n-1; per iteration x = u / |u|, u = A x; stop when |u - u_prev| or
+ * |u + u_prev| <= 1e-6 (u_prev before normalising) or after 10000 iterations.
+ * A is weighted by val (FP32 weights, FP64 arithmetic), binary when val is
+ * NULL.  vhat = x of the last iteration (what the unweighted tool writes),
+ * avec = A x of the last iteration (what the weighted tool writes), lambda =
+ * |u| of the last iteration (the printed eigenvalue).  Any output may be NULL. */
+pg_status pg_eigenvector_centrality(pg_ctx *ctx, const int32_t *row, const int32_t *col,
+                                    const float *val, int32_t n, int64_t nnz, double *vhat,
+                                    double *avec, double *lambda, int32_t *iters);
+
+/* Participation coefficient (W/PC_CPU/PC_CPU/main.cpp:96-160): with s_i the
+ * FP32 strength (CSR order) and s_ij the FP32 weight from i into module j,
+ * pc_i = 1 - sum over j = 1..K of (s_ij / s_i)^2 (0 when s_i == 0), pc_norm =
+ * pc / ((K-1)/K) (pc when K == 1); K = largest label.  labels (n, >= 1) are the
+ * module of each vertex, e.g. from pg_modularity; val NULL means unit weights.
+ * pc_norm and nmod may be NULL. */
+pg_status pg_participation(pg_ctx *ctx, const int32_t *row, const int32_t *col, const float *val,
+                           int32_t n, int64_t nnz, const int32_t *labels, float *pc, float *pc_norm,
+                           int32_t *nmod);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,11 @@
  *                       floyd_warshall_numpy, unit weights == BFS hops
  *   orc_modularity      disjoint cliques split exactly, Q against networkx
  *                       modularity, leading vector against numpy eigh
+ *   orc_ec              numpy eigh leading pair and networkx
+ *                       eigenvector_centrality_numpy fixtures, uniform vector on
+ *                       regular graphs, 10000 iterations on bipartite graphs
+ *   orc_pc              FP64 brute force per vertex, closed forms (one module:
+ *                       0; equal weight over K modules: 1 - 1/K, normalised 1)
  */
 #include "oracle.h"
 
@@ -711,4 +716,84 @@ double orc_modularity_q(const int32_t *row, const int32_t *col, const float *val
     free(k); free(kc);
     if (m2 <= 0.0) return 0.0;
     return (in - s2 / m2) / m2;
+}
+
+/* ------------------------------------------------- eigenvector centrality */
+
+/* U CUEC/CUEC/CUEC_unweighted.cu:61-196 (W CUEC/weighted-net.cu:55-203 with
+ * the stored weights): u = 0..n-1; while err1 > 1e-6 and err2 > 1e-6 and
+ * fewer than 10000 iterations: u0 = u, |u| (nrm2), u *= 1/|u| (scal), u = A u
+ * (csrmv), err1 = |u - u0|, err2 = |u + u0|.  Sums are sequential here. */
+double orc_ec(const int32_t *row, const int32_t *col, const float *val, int64_t n,
+              double *vhat, double *avec, int32_t *iters)
+{
+    double *u = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double *u0 = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double *h = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    for (int64_t i = 0; i < n; i++) u[i] = (double)i;
+    double err1 = 1, err2 = 1, vnorm = 0;
+    int32_t it = 0;
+    while (err1 > 1e-6 && err2 > 1e-6 && it < 10000) {
+        double ss = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            u0[i] = u[i];
+            ss += u[i] * u[i];
+        }
+        vnorm = sqrt(ss);
+        const double t = 1 / vnorm;
+        for (int64_t i = 0; i < n; i++) h[i] = u[i] * t;
+        double e1 = 0.0, e2 = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            double y = 0.0;
+            for (int32_t e = row[i]; e < row[i + 1]; e++) {
+                const double w = val ? (double)val[e] : 1.0;
+                y += w * h[col[e]];
+            }
+            u[i] = y;
+            const double d1 = y - u0[i], d2 = y + u0[i];
+            e1 += d1 * d1;
+            e2 += d2 * d2;
+        }
+        err1 = sqrt(e1);
+        err2 = sqrt(e2);
+        it++;
+    }
+    if (vhat) memcpy(vhat, h, sizeof(double) * (size_t)n);
+    if (avec) memcpy(avec, u, sizeof(double) * (size_t)n);
+    if (iters) *iters = it;
+    free(u); free(u0); free(h);
+    return vnorm;
+}
+
+/* --------------------------------------------- participation coefficient */
+
+/* W PC_CPU/PC_CPU/main.cpp:96-160, in FP32 as there. */
+int32_t orc_pc(const int32_t *row, const int32_t *col, const float *val, int64_t n,
+               const int32_t *labels, float *pc, float *pc_norm)
+{
+    int32_t k = 0;
+    for (int64_t i = 0; i < n; i++)
+        if (labels[i] > k) k = labels[i];
+    float *wm = (float *)calloc((size_t)(k > 0 ? k : 1), sizeof(float));
+    for (int64_t i = 0; i < n; i++) {
+        float deg = 0.0f;
+        for (int32_t e = row[i]; e < row[i + 1]; e++) deg += val ? val[e] : 1.0f;
+        if (deg == 0) {
+            pc[i] = 0.0f;
+            continue;
+        }
+        memset(wm, 0, sizeof(float) * (size_t)k);
+        for (int32_t e = row[i]; e < row[i + 1]; e++) wm[labels[col[e]] - 1] += val ? val[e] : 1.0f;
+        float sum = 0.0f;
+        for (int32_t j = 0; j < k; j++) {
+            const float t = wm[j] / deg;
+            const float t2 = t * t;
+            sum += t2;
+        }
+        pc[i] = 1.0f - sum;
+    }
+    const float mx = (float)(k - 1) / (float)k;
+    for (int64_t i = 0; i < n; i++) pc_norm[i] = k != 1 ? pc[i] / mx : pc[i];
+    free(wm);
+    return k;
 }

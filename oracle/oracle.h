@@ -97,6 +97,21 @@ double orc_modularity(const int32_t *row, const int32_t *col, const float *val,
 double orc_modularity_q(const int32_t *row, const int32_t *col, const float *val,
                         int64_t n, int kmode, const int32_t *labels);
 
+/* Eigenvector centrality by the reference's power iteration (U CUEC/CUEC/
+ * CUEC_unweighted.cu:61-196, W CUEC/weighted-net.cu:55-203): start vector
+ * 0..n-1, A weighted by val (1 when NULL).  vhat = normalised vector of the
+ * last iteration (the unweighted tool's output), avec = A vhat (the weighted
+ * tool's); returns that iteration's norm (the printed eigenvalue). */
+double orc_ec(const int32_t *row, const int32_t *col, const float *val, int64_t n,
+              double *vhat, double *avec, int32_t *iters);
+
+/* Participation coefficient (W PC_CPU/PC_CPU/main.cpp:96-160): FP32 strength
+ * and per-module sums in CSR order, pc = 1 - sum_j (s_ij / s_i)^2 over modules
+ * 1..K (K = max label), pc_norm = pc / ((K-1)/K) (pc when K == 1).  val NULL
+ * means unit weights.  Returns K. */
+int32_t orc_pc(const int32_t *row, const int32_t *col, const float *val, int64_t n,
+               const int32_t *labels, float *pc, float *pc_norm);
+
 #ifdef __cplusplus
 }
 #endif

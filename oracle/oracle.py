@@ -65,6 +65,10 @@ def lib():
         L.orc_modularity.restype = C.c_double
         L.orc_modularity_q.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, C.c_int, i32p]
         L.orc_modularity_q.restype = C.c_double
+        L.orc_ec.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, f64p, f64p, C.POINTER(C.c_int32)]
+        L.orc_ec.restype = C.c_double
+        L.orc_pc.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, i32p, f32p, f32p]
+        L.orc_pc.restype = C.c_int32
         _LIB = L
     return _LIB
 
@@ -237,3 +241,25 @@ def modularity_q(row, col, labels, val=None, kmode=0):
     v = None if val is None else np.ascontiguousarray(val, np.float32)
     return lib().orc_modularity_q(np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32),
                                   _ptr(v), n, kmode, np.ascontiguousarray(labels, np.int32))
+
+
+
+def ec(row, col, val=None):
+    """(vhat, avec, eigenvalue, iterations) of the reference's power iteration."""
+    n = row.size - 1
+    vh, av = np.empty(n, np.float64), np.empty(n, np.float64)
+    it = C.c_int32(0)
+    v = None if val is None else np.ascontiguousarray(val, np.float32)
+    lam = lib().orc_ec(np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32), _ptr(v), n,
+                       vh, av, C.byref(it))
+    return vh, av, lam, it.value
+
+
+def pc(row, col, labels, val=None):
+    """(pc, pc_normalized) float32 and the module count K."""
+    n = row.size - 1
+    p, q = np.empty(n, np.float32), np.empty(n, np.float32)
+    v = None if val is None else np.ascontiguousarray(val, np.float32)
+    k = lib().orc_pc(np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32), _ptr(v), n,
+                     np.ascontiguousarray(labels, np.int32), p, q)
+    return p, q, k

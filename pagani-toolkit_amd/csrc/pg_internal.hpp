@@ -107,3 +107,7 @@ pg_status pg_check_csr_args(pg_ctx *ctx, const int32_t *row, const int32_t *col,
                             int64_t nnz);
 
 inline int64_t pg_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Device check of a staged CSR (device pointers): row non-decreasing from 0 to
+// nnz, every column in [0, n).  PG_EFORMAT otherwise.  Synchronises the stream.
+pg_status pg_validate_csr(pg_ctx *ctx, const int32_t *drow, const int32_t *dcol, int32_t n, int64_t nnz);

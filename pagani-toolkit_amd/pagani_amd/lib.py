@@ -87,6 +87,8 @@ _SIGS = {
     "pg_bc": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, C.c_int64, _VP]),
     "pg_bfs": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, C.c_int64, _VP]),
     "pg_bfs_efficiency": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP]),
+    "pg_eigenvector_centrality": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP]),
+    "pg_participation": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP]),
     "pg_fw": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32]),
     "pg_fw_init": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_int32, _VP]),
     "pg_fw_efficiency": (C.c_int, [_VP, _VP, C.c_int32, _VP, _VP, _VP]),
@@ -250,6 +252,25 @@ class Context:
                                                  C.cast(C.pointer(lp), _VP), C.cast(C.pointer(iso), _VP),
                                                  C.cast(C.pointer(unc), _VP)))
         return eff, lp.value, iso.value, unc.value
+
+    def eigenvector_centrality(self, row, col, val=None):
+        """(vhat, avec, eigenvalue, iterations) of the reference's power iteration."""
+        n = _len(row) - 1
+        vh, av = np.empty(n, np.float64), np.empty(n, np.float64)
+        lam, it = C.c_double(0), C.c_int32(0)
+        self._check(self._lib.pg_eigenvector_centrality(self._h, _ptr(row), _ptr(col), _ptr(val), n, _len(col),
+                                                         _ptr(vh), _ptr(av), C.cast(C.pointer(lam), _VP),
+                                                         C.cast(C.pointer(it), _VP)))
+        return vh, av, lam.value, it.value
+
+    def participation(self, row, col, labels, val=None):
+        """(pc, pc_normalized) float32 and the module count K."""
+        n = _len(row) - 1
+        p, q = np.empty(n, np.float32), np.empty(n, np.float32)
+        k = C.c_int32(0)
+        self._check(self._lib.pg_participation(self._h, _ptr(row), _ptr(col), _ptr(val), n, _len(col),
+                                                _ptr(labels), _ptr(p), _ptr(q), C.cast(C.pointer(k), _VP)))
+        return p, q, k.value
 
     def fw(self, d, block: int = 0):
         """Closure in place; d is an n x n float32 array/tensor."""

@@ -51,6 +51,14 @@ def graph_fixture(name, g, weighted):
         eff[i] = sum(1.0 / dd for j, dd in dist.items() if j != i) / (n - 1)
     out["nodal_eff"] = eff
     out["global_eff"] = np.float64(nx.global_efficiency(g))
+    # leading eigenvector, unit 2-norm, positive (binary adjacency; the
+    # weighted graph also gets the weighted one); connected graphs only
+    if nx.is_connected(g):
+        ec = nx.eigenvector_centrality_numpy(g)
+        out["ec"] = np.array([ec[i] for i in range(n)])
+    if weighted and nx.is_connected(g):
+        ecw = nx.eigenvector_centrality_numpy(g, weight="weight")
+        out["ec_weighted"] = np.array([ecw[i] for i in range(n)])
     if weighted:
         wcl = nx.clustering(g, weight="weight")
         out["clustering_onnela"] = np.array([wcl[i] for i in range(n)])

@@ -252,3 +252,52 @@ def test_modularity_weighted(ctx, weighted_graph, kmode):
     assert abs(q - oq) <= 1e-6
     agree = (lab == olab).mean()
     assert agree > 0.99, agree
+
+
+def _check_ec(got, want):
+    vh, av, lam, it = got
+    ovh, oav, olam, oit = want
+    assert it == oit
+    assert abs(lam - olam) <= 1e-12 * abs(olam)
+    # FP64, sums in another order: the converged vectors agree far below the
+    # 1e-6 stop tolerance
+    np.testing.assert_allclose(vh, ovh, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(av, oav, rtol=1e-9, atol=1e-12 * abs(olam))
+
+
+@pytest.mark.parametrize("name", list(_graphs()))
+def test_ec_small(ctx, name):
+    row, col = _graphs()[name][:2]
+    say(f"ec {name}")
+    _check_ec(ctx.eigenvector_centrality(row, col), O.ec(row, col))
+
+
+def test_ec_plain_and_weighted(ctx, plain_graph, weighted_graph):
+    row, col = plain_graph
+    say("ec plain")
+    _check_ec(ctx.eigenvector_centrality(row, col), O.ec(row, col))
+    row, col, val = weighted_graph
+    say("ec weighted")
+    _check_ec(ctx.eigenvector_centrality(row, col, val), O.ec(row, col, val))
+
+
+def test_pc_bit_exact(ctx, plain_graph, weighted_graph):
+    row, col, val = weighted_graph
+    lab = O.modularity(row, col, val, 1)[0]
+    say("pc weighted")
+    p, q, k = ctx.participation(row, col, lab, val)
+    op, oq, ok = O.pc(row, col, lab, val)
+    assert k == ok
+    np.testing.assert_array_equal(p.view(np.uint32), op.view(np.uint32))
+    np.testing.assert_array_equal(q.view(np.uint32), oq.view(np.uint32))
+    row, col = plain_graph
+    lab = (np.arange(row.size - 1) % 37 + 1).astype(np.int32)
+    say("pc plain, 37 modules")
+    p, q, k = ctx.participation(row, col, lab)
+    op, oq, ok = O.pc(row, col, lab)
+    np.testing.assert_array_equal(p.view(np.uint32), op.view(np.uint32))
+    np.testing.assert_array_equal(q.view(np.uint32), oq.view(np.uint32))
+    lab[5] = 0
+    from pagani_amd.lib import PgError
+    with pytest.raises(PgError, match="labels"):
+        ctx.participation(row, col, lab)

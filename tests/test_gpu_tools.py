@@ -131,3 +131,30 @@ def test_lp_and_modularity_tools_on_weighted_networks(volume):
         txt = open(base + "_modu.txt").read()
         qfile = float(txt.split("Q=")[1].split()[0])
         assert abs(qfile - q) <= 1e-5
+
+
+def test_ec_and_pc_tools(volume):
+    d, subs, n = volume
+    w = d / "weighted"
+    run("CUmodularity", w, 0)      # module labels for PC_CPU
+    run("PC_CPU", w)
+    run("CUEC", "--weighted", w)
+    run("CUEC", d)
+    for f in _csr_files(w):
+        base = str(w / f)[:-4]
+        row, col, val = formats.read_csr(str(w / f))
+        lab = formats.read_nm(base + "_modu.nm").astype(np.int32)
+        p, q, _ = O.pc(row, col, lab, val)
+        np.testing.assert_array_equal(formats.read_nm(base + "_pc.nm").view(np.uint32), p.view(np.uint32))
+        np.testing.assert_array_equal(formats.read_nm(base + "_pc.nm_Normalized.nm").view(np.uint32),
+                                      q.view(np.uint32))
+        vh, av, lam, it = O.ec(row, col, val)
+        np.testing.assert_allclose(formats.read_nm(base + "_ec.nm"), av.astype(np.float32), rtol=1e-6)
+        assert abs(float(open(base + "_evalue.txt").read()) - lam) <= 1e-5 * lam
+    for f in _csr_files(d):
+        base = str(d / f)[:-4]
+        row, col, _ = formats.read_csr(str(d / f))
+        vh, av, lam, it = O.ec(row, col)
+        np.testing.assert_allclose(formats.read_nm(base + "_ec.nm"), vh.astype(np.float32), rtol=1e-6,
+                                   atol=1e-9)
+        assert abs(float(open(base + "_evalue.txt").read()) - lam) <= 1e-5 * lam

@@ -245,6 +245,71 @@ def test_bfs_efficiency_closed_forms():
     assert np.all(eff[:10] == np.float32(np.float32(4) / np.float32(10)))
 
 
+def _dense(row, col, val=None):
+    n = row.size - 1
+    a = np.zeros((n, n))
+    for i in range(n):
+        a[i, col[row[i]:row[i + 1]]] = 1.0 if val is None else val[row[i]:row[i + 1]]
+    return a
+
+
+def test_ec_leading_pair_matches_eigh():
+    for seed, wt in [(3, False), (4, True)]:
+        row, col, val = random_graph(160, 0.07, seed)
+        v = val if wt else None
+        vh, av, lam, it = O.ec(row, col, v)
+        w, vec = np.linalg.eigh(_dense(row, col, v))
+        lead = vec[:, -1] * np.sign(vec[:, -1].sum())
+        assert abs(lam - w[-1]) <= 1e-9 * w[-1]
+        np.testing.assert_allclose(vh, lead, atol=1e-6)
+        np.testing.assert_allclose(av, lam * vh, atol=1e-5 * lam)
+
+
+def test_ec_closed_forms():
+    # odd cycle (2-regular, not bipartite): eigenvalue 2, uniform vector
+    n = 31
+    row = np.array([2 * i for i in range(n + 1)], np.int32)
+    col = np.array([c for i in range(n) for c in sorted(((i - 1) % n, (i + 1) % n))], np.int32)
+    vh, av, lam, it = O.ec(row, col)
+    assert abs(lam - 2.0) < 1e-6 and it < 10000
+    np.testing.assert_allclose(vh, 1 / np.sqrt(n), atol=1e-6)
+    row, col = complete_graph(12)[:2]
+    vh, av, lam, it = O.ec(row, col)
+    assert abs(lam - 11.0) < 1e-9
+    # a star is bipartite: the iteration oscillates and runs to the cap
+    row, col = star_graph(9)[:2]
+    assert O.ec(row, col)[3] == 10000
+
+
+def test_pc_closed_forms_and_fp64():
+    # each clique its own module: every edge stays inside, pc = 0
+    lab = np.repeat(np.arange(1, 4), [6, 9, 5]).astype(np.int32)
+    row, col = cliques([6, 9, 5], bridge=False)[:2]
+    p, q, k = O.pc(row, col, lab)
+    assert k == 3 and np.all(p == 0) and np.all(q == 0)
+    # with bridges 0-6 and 6-15: vertex 0 has 5 + 1 edges, vertex 6 has 1 + 8 + 1
+    row, col = cliques([6, 9, 5])[:2]
+    p, q, k = O.pc(row, col, lab)
+    assert abs(p[0] - (1 - (5 / 6) ** 2 - (1 / 6) ** 2)) < 1e-7
+    assert abs(p[6] - (1 - (8 / 10) ** 2 - 2 * (1 / 10) ** 2)) < 1e-7
+    # star centre whose 12 leaves are spread evenly over 4 modules
+    row, col = star_graph(13)[:2]
+    lab = np.array([1] + [1 + (i % 4) for i in range(12)], np.int32)
+    p, q, k = O.pc(row, col, lab)
+    assert k == 4 and abs(p[0] - 0.75) < 1e-7 and abs(q[0] - 1.0) < 1e-6
+    assert np.all(p[1:] == 0)  # a leaf has one neighbour
+    # FP64 brute force on a weighted graph
+    row, col, val = random_graph(150, 0.08, 9)
+    lab = (np.arange(150) % 7 + 1).astype(np.int32)
+    p, q, k = O.pc(row, col, lab, val)
+    a = _dense(row, col, val)
+    s = a.sum(1)
+    share = np.stack([a[:, lab == j].sum(1) for j in range(1, 8)], 1) / np.where(s > 0, s, 1)[:, None]
+    want = np.where(s > 0, 1 - (share ** 2).sum(1), 0)
+    np.testing.assert_allclose(p, want, atol=2e-6)
+    np.testing.assert_allclose(q, want / (6 / 7), atol=3e-6)
+
+
 def test_modularity_cliques_split_exactly():
     row, col = cliques([10, 10, 10, 10])[:2]
     lab, q, lam, eig, nr = O.modularity(row, col)

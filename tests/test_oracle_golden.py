@@ -54,6 +54,22 @@ def test_bfs_efficiency_matches_networkx(path):
         assert (s / np.float32(n - 1)).view(np.uint32) == eff[i].view(np.uint32), i
 
 
+@pytest.mark.parametrize("path", GRAPHS, ids=[os.path.basename(p) for p in GRAPHS])
+def test_ec_matches_networkx(path):
+    g = load(path)
+    if "ec" not in g:
+        pytest.skip("disconnected graph: no networkx reference vector")
+    row, col = g["row"], g["col"]
+    vh, av, lam, it = O.ec(row, col)
+    assert it < 10000
+    # the stop rule |A x_k - A x_(k-1)| <= 1e-6 leaves the vector within ~1e-6 / gap
+    np.testing.assert_allclose(vh, g["ec"], atol=2e-6)
+    np.testing.assert_allclose(av / lam, g["ec"], atol=2e-6)
+    if "ec_weighted" in g:
+        vh, av, lam, it = O.ec(row, col, g["val"])
+        np.testing.assert_allclose(vh, g["ec_weighted"], atol=2e-6)
+
+
 def test_weighted_clustering_and_fw_match_networkx():
     g = load(os.path.join(GOLD, "graph_gnp150_weighted.npz"))
     row, col, val = g["row"], g["col"], g["val"]

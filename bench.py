@@ -257,6 +257,45 @@ def run_weighted(a):
     return out
 
 
+FP64_PEAK = 78.6e12  # MI355X dense FP64 MFMA FLOP/s
+
+
+def run_average(a):
+    """Subject averaging of the dense matrix at N=58,000, T=240: per subject the
+    FP64 MFMA Gram with the exact FP32 rounding and the packed accumulate (Fisher
+    z), then the finish and both threshold kinds on the averaged matrix."""
+    import torch
+    import pagani_amd
+    from pagani_amd import synth
+    w = synth.WORKLOADS["corr_n58000_t240"]
+    n, t = w["n"], w["t"]
+    ns = max(2, min(a.subjects, 4))
+    ctx = pagani_amd.Context(0)
+    ctx.cormat_begin(n, True)
+    per = []
+    for s in range(ns):
+        x = torch.from_numpy(synth.bold(w["seed"] + s, n, t, w["k"])).cuda().contiguous()
+        torch.cuda.synchronize()
+        _, dt = _timed(ctx, lambda: ctx.cormat_add(x, t))
+        per.append(dt)
+        log(f"subject {s}: FP64 Gram + accumulate {dt * 1e3:.1f} ms")
+    flop = float(n) * (n - 1) / 2 * t * 2
+    _, dfin = _timed(ctx, ctx.cormat_finish)
+    _, dr = _timed(ctx, lambda: ctx.cormat_threshold(0.3))
+    info_r = ctx.cormat_threshold(0.3)
+    _, ds = _timed(ctx, lambda: ctx.cormat_threshold(1.0, sparsity=True))
+    info_s = ctx.cormat_threshold(1.0, sparsity=True)
+    best = min(per[1:]) if len(per) > 1 else per[0]
+    out = {"metric": "subject averaging of the dense matrix (Fisher z), N=58000, T=240", "n_voxels": n,
+           "subjects": ns, "ms_per_subject": [p * 1e3 for p in per], "gram64_tflops": flop / best / 1e12,
+           "gram64_fp64_mfma_frac": flop / best / FP64_PEAK, "finish_ms": dfin * 1e3,
+           "threshold_r_ms": dr * 1e3, "threshold_r_nnz": info_r.nnz, "threshold_s_ms": ds * 1e3,
+           "threshold_s_r": info_s.r_thresh, "n_gpus": 1}
+    log(f"finish {dfin * 1e3:.1f} ms, r-threshold {dr * 1e3:.1f} ms, 1% sparsity {ds * 1e3:.1f} ms")
+    ctx.close()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,7 +307,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-side", action="store_true", help="skip FP32 and BC side measurements")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only copy 1 GiB device to device 5 times (scale for HBM byte counters)")
-    ap.add_argument("--config", choices=["corr", "metrics", "cohort", "weighted"], default="corr",
+    ap.add_argument("--config", choices=["corr", "metrics", "cohort", "weighted", "average"], default="corr",
                     help="corr: the headline line (default); metrics: BC from all sources, clustering "
                          "and N=8192 Floyd-Warshall on the 1%% graph; cohort: 64 subjects of uneven N, T; "
                          "weighted: N=120000 r>0.3 weighted clustering + modularity")
@@ -301,7 +340,8 @@ def main(argv=None) -> int:
     from pagani_amd import synth
 
     if a.config != "corr":
-        out = {"metrics": run_metrics, "cohort": run_cohort, "weighted": run_weighted}[a.config](a)
+        out = {"metrics": run_metrics, "cohort": run_cohort, "weighted": run_weighted,
+               "average": run_average}[a.config](a)
         print(json.dumps(out), flush=True)
         return 0
 

@@ -107,6 +107,27 @@ pg_status pg_corr_csr_view(pg_ctx *ctx, const int32_t **row, const int32_t **col
 /* FP64 z-scored series of the last build, voxel-major [n][t] (device pointer). */
 pg_status pg_corr_z64_view(pg_ctx *ctx, const double **z64);
 
+/* Dense correlation matrix of one or more subjects: subject averaging and the
+ * .cormat output (U CorMat.cpp:236-262, 380-620, 775-838; W CorMat.cpp:386-600).
+ * The matrix is the packed upper triangle, row-major over i < j, M1 = n(n-1)/2
+ * floats, held by the context.  pg_cormat_begin zeroes it; pg_cormat_add adds
+ * one subject's (float) r per pair (r = the FP64 Pearson r, clamped to [-1,1]),
+ * or its Fisher z when fisher != 0 (U CorMat.cpp:35-42); every subject has the
+ * same n, t may differ.  pg_cormat_finish divides by the subject count and,
+ * for Fisher, maps back (U CorMat.cpp:507-517); a single subject without
+ * Fisher is its own r matrix.  pg_cormat_read copies values [offset,
+ * offset+count) out (host or device).  pg_cormat_threshold applies a threshold
+ * to the finished matrix as the reference does for averaged networks
+ * (FormFullGraph: a > thr, no 1e-6 offset; FormFullGraph_s: the k largest, ties
+ * at the k-th value in row-major order) and leaves the CSR for pg_corr_csr /
+ * pg_corr_csr_view (values = matrix entries when weighted). */
+pg_status pg_cormat_begin(pg_ctx *ctx, int32_t n, int32_t fisher);
+pg_status pg_cormat_add(pg_ctx *ctx, const float *bold, int32_t t, int32_t layout);
+pg_status pg_cormat_finish(pg_ctx *ctx);
+pg_status pg_cormat_read(pg_ctx *ctx, int64_t offset, int64_t count, float *dst);
+pg_status pg_cormat_threshold(pg_ctx *ctx, int32_t thr_kind, float thr, int32_t weighted,
+                              pg_corr_info *info);
+
 /* ----------------------------------------------------------------- metrics */
 /* CSR arguments: row[n+1] (row[0] = 0, row[n] = nnz), col[nnz] ascending within
  * each row, symmetric, no self loops; val[nnz] FP32 weights where used. */

@@ -39,6 +39,12 @@
  *                       regular graphs, 10000 iterations on bipartite graphs
  *   orc_pc              FP64 brute force per vertex, closed forms (one module:
  *                       0; equal weight over K modules: 1 - 1/K, normalised 1)
+ *   orc_cormat_*        numpy corrcoef cast to float32 and summed in float32
+ *                       (1 ulp: corrcoef rounds differently), numpy arctanh /
+ *                       tanh for the Fisher pair; averaging one subject is the
+ *                       identity on (float)r
+ *   orc_dense_csr_*     brute force with numpy (mask a > thr; argsort for the
+ *                       k-th largest and the row-major tie rule)
  */
 #include "oracle.h"
 
@@ -796,4 +802,115 @@ int32_t orc_pc(const int32_t *row, const int32_t *col, const float *val, int64_t
     for (int64_t i = 0; i < n; i++) pc_norm[i] = k != 1 ? pc[i] / mx : pc[i];
     free(wm);
     return k;
+}
+
+/* ------------------------------------------------------ subject averaging */
+
+/* U CorMat.cpp:35-42 in FP32, the logarithm in FP64. */
+static float fisher_f(float r)
+{
+    if (r == 1) r = (float)((double)r - 1e-6);
+    const float q = (1 + r) / (1 - r);
+    return (float)(0.5 * log((double)q));
+}
+
+/* U CorMat.cpp:44-49 in FP32, the exponential in FP64. */
+static float inv_fisher_f(float z)
+{
+    float r = (float)exp((double)(2 * z));
+    r = (r - 1) / (r + 1);
+    return r;
+}
+
+/* post_block (U CorMat.cpp:806-838): every pair i < j in row-major order adds
+ * its value to the packed accumulator.  The reference drops |r| > 1 from FP32
+ * rounding; r here is FP64 and clamped, as in orc_r. */
+void orc_cormat_add(const double *z64, int64_t n, int t, int fisher, float *acc)
+{
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int64_t i = 0; i < n; i++) {
+        int64_t idx = i * n - i * (i + 1) / 2;
+        for (int64_t j = i + 1; j < n; j++, idx++) {
+            const float r = (float)orc_r(z64, t, i, j);
+            acc[idx] += fisher ? fisher_f(r) : r;
+        }
+    }
+}
+
+/* U CorMat.cpp:507-517. */
+void orc_cormat_finish(float *acc, int64_t m1, int nsub, int fisher)
+{
+    for (int64_t q = 0; q < m1; q++) {
+        acc[q] /= (float)nsub;
+        if (fisher) acc[q] = inv_fisher_f(acc[q]);
+    }
+}
+
+static void dense_csr(const float *a, int64_t n, const unsigned char *keep_upper, int32_t *row, int32_t *col,
+                      float *val)
+{
+    int64_t pos = 0;
+    row[0] = 0;
+    for (int64_t i = 0; i < n; i++) {
+        for (int64_t j = 0; j < n; j++) {
+            if (j == i) continue;
+            const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+            const int64_t idx = lo * n - lo * (lo + 1) / 2 + (hi - lo - 1);
+            if (keep_upper[idx]) {
+                col[pos] = (int32_t)j;
+                if (val) val[pos] = a[idx];
+                pos++;
+            }
+        }
+        row[i + 1] = (int32_t)pos;
+    }
+}
+
+/* FormFullGraph (U CorMat.cpp:647-664) + FormCSRGraph. */
+int64_t orc_dense_csr_r(const float *a, int64_t n, float thr, int32_t *row, int32_t *col, float *val)
+{
+    const int64_t m1 = n * (n - 1) / 2;
+    unsigned char *keep = (unsigned char *)malloc((size_t)(m1 > 0 ? m1 : 1));
+    int64_t cnt = 0;
+    for (int64_t q = 0; q < m1; q++) {
+        keep[q] = a[q] > thr;
+        cnt += keep[q];
+    }
+    if (row) dense_csr(a, n, keep, row, col, val);
+    free(keep);
+    return 2 * cnt;
+}
+
+static int cmp_desc_f(const void *x, const void *y)
+{
+    const float a = *(const float *)x, b = *(const float *)y;
+    return a < b ? 1 : (a > b ? -1 : 0);
+}
+
+/* FormFullGraph_s (U CorMat.cpp:693-773): t = k-th largest; every a > t and
+ * the earliest ties a == t (a stack that drops the newest tie whenever the
+ * count passes k leaves the earliest ones). */
+float orc_dense_csr_s(const float *a, int64_t n, int64_t k, int32_t *row, int32_t *col, float *val)
+{
+    const int64_t m1 = n * (n - 1) / 2;
+    if (k < 1 || k > m1) return NAN;
+    float *s = (float *)malloc(sizeof(float) * (size_t)m1);
+    memcpy(s, a, sizeof(float) * (size_t)m1);
+    qsort(s, (size_t)m1, sizeof(float), cmp_desc_f);
+    const float t = s[k - 1];
+    free(s);
+    unsigned char *keep = (unsigned char *)malloc((size_t)m1);
+    int64_t gt = 0;
+    for (int64_t q = 0; q < m1; q++) gt += a[q] > t;
+    int64_t ties = k - gt;
+    for (int64_t q = 0; q < m1; q++) {
+        keep[q] = a[q] > t;
+        if (a[q] == t && ties > 0) {
+            keep[q] = 1;
+            ties--;
+        }
+    }
+    if (row) dense_csr(a, n, keep, row, col, val);
+    free(keep);
+    return t;
 }

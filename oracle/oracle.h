@@ -112,6 +112,23 @@ double orc_ec(const int32_t *row, const int32_t *col, const float *val, int64_t 
 int32_t orc_pc(const int32_t *row, const int32_t *col, const float *val, int64_t n,
                const int32_t *labels, float *pc, float *pc_norm);
 
+/* Subject averaging of the dense correlation matrix (U CorMat.cpp:380-540 with
+ * post_block :775-838 and fisher_trans / inv_fisher_trans :35-49).  acc is the
+ * packed upper triangle, row-major over i < j, M1 = n(n-1)/2 floats.
+ * orc_cormat_add adds one subject: (float)clamp(r64) per pair, or its Fisher z
+ * (FP32 (1+r)/(1-r), FP64 log, rounded to FP32; r == 1 first lowered by 1e-6).
+ * orc_cormat_finish divides by nsub in FP32 and, for Fisher, maps back with
+ * (e - 1)/(e + 1), e = exp(2 z) rounded to FP32. */
+void orc_cormat_add(const double *z64, int64_t n, int t, int fisher, float *acc);
+void orc_cormat_finish(float *acc, int64_t m1, int nsub, int fisher);
+
+/* Thresholds on a packed FP32 matrix (U CorMat.cpp:647-664, 693-773, 839-857;
+ * W CorMat.cpp:853-885 for the weights): r-mode keeps a > thr; returns the
+ * directed edge count, CSR in row/col/val when row != NULL.  s-mode keeps the k
+ * largest (ties at the k-th value in row-major order) and returns that value. */
+int64_t orc_dense_csr_r(const float *a, int64_t n, float thr, int32_t *row, int32_t *col, float *val);
+float orc_dense_csr_s(const float *a, int64_t n, int64_t k, int32_t *row, int32_t *col, float *val);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,12 @@ def lib():
         L.orc_ec.restype = C.c_double
         L.orc_pc.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, i32p, f32p, f32p]
         L.orc_pc.restype = C.c_int32
+        L.orc_cormat_add.argtypes = [f64p, C.c_int64, C.c_int, C.c_int, f32p]
+        L.orc_cormat_finish.argtypes = [f32p, C.c_int64, C.c_int, C.c_int]
+        L.orc_dense_csr_r.argtypes = [f32p, C.c_int64, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_dense_csr_r.restype = C.c_int64
+        L.orc_dense_csr_s.argtypes = [f32p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_dense_csr_s.restype = C.c_float
         _LIB = L
     return _LIB
 
@@ -263,3 +269,35 @@ def pc(row, col, labels, val=None):
     k = lib().orc_pc(np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32), _ptr(v), n,
                      np.ascontiguousarray(labels, np.int32), p, q)
     return p, q, k
+
+
+def cormat_average(subjects, fisher=False):
+    """Packed upper-triangle float32 average over subjects (list of [T, N]
+    time-major float32 series with the same N)."""
+    n = subjects[0].shape[1]
+    acc = np.zeros(n * (n - 1) // 2, np.float32)
+    for x in subjects:
+        z64, _ = zscore(x)
+        lib().orc_cormat_add(np.ascontiguousarray(z64), n, x.shape[0], int(fisher), acc)
+    lib().orc_cormat_finish(acc, acc.size, len(subjects), int(fisher))
+    return acc
+
+
+def dense_csr_r(a, n, thr, weighted=False):
+    """(row, col, val-or-None) of a > thr on a packed float32 matrix."""
+    a = np.ascontiguousarray(a, np.float32)
+    nnz = lib().orc_dense_csr_r(a, n, thr, None, None, None)
+    row, col = np.empty(n + 1, np.int32), np.empty(max(nnz, 1), np.int32)
+    val = np.empty(max(nnz, 1), np.float32) if weighted else None
+    lib().orc_dense_csr_r(a, n, thr, _ptr(row), _ptr(col), _ptr(val))
+    return row, col[:nnz], (val[:nnz] if weighted else None)
+
+
+def dense_csr_s(a, n, s_percent, weighted=False):
+    """(row, col, val-or-None, t) keeping the k largest, k from the sparsity rule."""
+    a = np.ascontiguousarray(a, np.float32)
+    k = sparsity_k(n, s_percent)
+    row, col = np.empty(n + 1, np.int32), np.empty(max(2 * k, 1), np.int32)
+    val = np.empty(max(2 * k, 1), np.float32) if weighted else None
+    t = lib().orc_dense_csr_s(a, n, k, _ptr(row), _ptr(col), _ptr(val))
+    return row, col[:2 * k], (val[:2 * k] if weighted else None), t

@@ -2291,6 +2291,28 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     return PG_OK;
 }
 
+pg_status pg_zscore64(pg_ctx *ctx, const float *dbold, int layout, int n, int t, double *z64)
+{
+    if (n <= 0 || t <= 0) return PG_OK;
+    hipStream_t st = ctx->stream;
+    if (t <= ZS_TMAX) {
+        const size_t shz = (size_t)t * (ZV + 1) * 4;
+        PG_HIP(ctx, hipFuncSetAttribute((const void *)zscore_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)shz));
+        zscore_fused<<<(unsigned)pg_cdiv(n, ZV), 256, shz, st>>>(dbold, layout, n, t, z64, nullptr, nullptr,
+                                                                  nullptr, 0, 0);
+    } else {
+        double *mu, *sd;
+        PG_TRY(pg_bufT(ctx, "zs.mu", (size_t)n + 1, &mu));
+        PG_TRY(pg_bufT(ctx, "zs.sd", (size_t)n + 1, &sd));
+        zscore_stats<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(dbold, layout, n, t, mu, sd);
+        dim3 g((unsigned)pg_cdiv(n, 64), (unsigned)pg_cdiv(t, 64));
+        zscore_write<<<g, 256, 0, st>>>(dbold, layout, n, t, mu, sd, z64, nullptr, nullptr, nullptr, 0, 0);
+    }
+    PG_KERNEL_CHECK(ctx);
+    return PG_OK;
+}
+
 extern "C" pg_status pg_corr_csr(pg_ctx *ctx, int32_t *row, int32_t *col, float *val)
 {
     if (!ctx) return PG_EINVAL;

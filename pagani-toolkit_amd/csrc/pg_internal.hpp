@@ -33,6 +33,12 @@ struct pg_ctx {
     int32_t *c_row = nullptr, *c_col = nullptr;
     float *c_val = nullptr;
     double *c_z64 = nullptr;
+
+    // dense packed correlation matrix (pg_cormat_*): n voxels, nsub subjects
+    // added, fisher mode, finished flag; the buffer is the named "cm.acc"
+    int32_t cm_n = -1, cm_nsub = 0, cm_fisher = 0;
+    bool cm_finished = false;
+    float *cm_acc = nullptr;
 };
 
 pg_status pg_fail(pg_ctx *ctx, pg_status st, const char *fmt, ...);
@@ -107,6 +113,10 @@ pg_status pg_check_csr_args(pg_ctx *ctx, const int32_t *row, const int32_t *col,
                             int64_t nnz);
 
 inline int64_t pg_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// FP64 z-scores of a device time series into z64[i * t + l] (voxel-major), by
+// the same kernels and arithmetic as pg_corr_build.
+pg_status pg_zscore64(pg_ctx *ctx, const float *dbold, int layout, int n, int t, double *z64);
 
 // Device check of a staged CSR (device pointers): row non-decreasing from 0 to
 // nnz, every column in [0, n).  PG_EFORMAT otherwise.  Synchronises the stream.

@@ -89,6 +89,11 @@ _SIGS = {
     "pg_bfs_efficiency": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP]),
     "pg_eigenvector_centrality": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP]),
     "pg_participation": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP]),
+    "pg_cormat_begin": (C.c_int, [_VP, C.c_int32, C.c_int32]),
+    "pg_cormat_add": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32]),
+    "pg_cormat_finish": (C.c_int, [_VP]),
+    "pg_cormat_read": (C.c_int, [_VP, C.c_int64, C.c_int64, _VP]),
+    "pg_cormat_threshold": (C.c_int, [_VP, C.c_int32, C.c_float, C.c_int32, _VP]),
     "pg_fw": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32]),
     "pg_fw_init": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_int32, _VP]),
     "pg_fw_efficiency": (C.c_int, [_VP, _VP, C.c_int32, _VP, _VP, _VP]),
@@ -192,6 +197,29 @@ class Context:
         z = _VP()
         self._check(self._lib.pg_corr_z64_view(self._h, C.byref(z)))
         return z.value
+
+    # ------------------------------------------------- dense matrix, averaging
+    def cormat_begin(self, n: int, fisher: bool = False):
+        self._check(self._lib.pg_cormat_begin(self._h, n, int(fisher)))
+
+    def cormat_add(self, bold, t: int, layout: int = LAYOUT_TIME_MAJOR):
+        self._check(self._lib.pg_cormat_add(self._h, _ptr(bold), t, layout))
+
+    def cormat_finish(self):
+        self._check(self._lib.pg_cormat_finish(self._h))
+
+    def cormat_read(self, n: int, offset: int = 0, count=None, out=None):
+        m1 = n * (n - 1) // 2
+        count = m1 - offset if count is None else count
+        out = np.empty(count, np.float32) if out is None else out
+        self._check(self._lib.pg_cormat_read(self._h, offset, count, _ptr(out)))
+        return out
+
+    def cormat_threshold(self, thr: float, *, sparsity: bool = False, weighted: bool = False) -> CorrInfo:
+        info = _Info()
+        self._check(self._lib.pg_cormat_threshold(self._h, THR_SPARSITY if sparsity else THR_R, thr,
+                                                   int(weighted), C.byref(info)))
+        return CorrInfo(**{f: getattr(info, f) for f, _ in _Info._fields_})
 
     # ------------------------------------------------------------ metrics
     def degree(self, row, deg=None, deg_ref=None):

@@ -158,3 +158,48 @@ def test_ec_and_pc_tools(volume):
         np.testing.assert_allclose(formats.read_nm(base + "_ec.nm"), vh.astype(np.float32), rtol=1e-6,
                                    atol=1e-9)
         assert abs(float(open(base + "_evalue.txt").read()) - lam) <= 1e-5 * lam
+
+
+def _read_cormat(path, n):
+    raw = open(path, "rb").read()
+    m1 = n * (n - 1) // 2
+    assert np.frombuffer(raw[:4], np.int32)[0] == np.int32(np.int64(m1) & 0xffffffff)
+    a = np.frombuffer(raw[4:], np.float32)
+    assert a.size == m1
+    return a
+
+
+def test_cucormat_average_and_cormat(volume, tmp_path):
+    d, subs, n = volume
+    src = tmp_path / "grp"
+    src.mkdir()
+    for f in ["mask.nii", "sub01.nii", "sub02.nii"]:
+        (src / f).write_bytes((d / f).read_bytes())
+    run("CUCorMat", src, 0.5, "bf", "y", "r", 0.3)
+    xs = [subs["sub01"], subs["sub02"]]
+    # per-subject dense matrices: (float) of the FP64 r, bit-exact
+    for name, x in subs.items():
+        a = _read_cormat(src / f"{name}_{n}.cormat", n)
+        np.testing.assert_array_equal(a.view(np.uint32), O.cormat_average([x]).view(np.uint32))
+    # the Fisher average: within one FP32 ulp of the oracle
+    base = src / f"grp_2_{n}"
+    avg = _read_cormat(str(base) + ".cormat", n)
+    want = O.cormat_average(xs, True)
+    assert np.max(np.abs(avg.astype(np.float64) - want)) <= 2 * np.spacing(np.float32(1))
+    # its network: exactly the reference's threshold rule on the written matrix
+    orow, ocol, _ = O.dense_csr_r(avg, n, 0.3)
+    m1 = n * (n - 1) // 2
+    name = f"grp_2_{n}_spa{np.float32(100.0 * orow[-1] / m1 / 2):.3f}%_cor{np.float32(0.3):.3f}.csr"
+    row, col, val = formats.read_csr(str(src / name))
+    assert val is None
+    np.testing.assert_array_equal(row, orow)
+    np.testing.assert_array_equal(col, ocol)
+    # weighted sparsity form of the plain average: weights are the matrix values
+    run("CUCorMat", "--weighted", src, "0.5", "yn", "n", "s", 1.5)
+    avg = O.cormat_average(xs)
+    orow, ocol, oval, t = O.dense_csr_s(avg, n, 1.5, True)
+    name = f"grp_2_{n}_spa{1.5:.3f}%_cor{t:.3f}.csr"
+    row, col, val = formats.read_csr(str(src / name))
+    np.testing.assert_array_equal(row, orow)
+    np.testing.assert_array_equal(col, ocol)
+    np.testing.assert_array_equal(val.view(np.uint32), oval.view(np.uint32))

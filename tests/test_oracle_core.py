@@ -330,3 +330,59 @@ def test_modularity_leading_vector_matches_eigh():
     assert abs(lam - w[-1]) < 1e-5
     vv = v[:, -1] * np.sign(v[:, -1] @ eig)
     np.testing.assert_allclose(eig, vv, atol=1e-4)
+
+
+def _dense_csr_numpy(keep_upper, n):
+    iu = np.triu_indices(n, 1)
+    m = np.zeros((n, n), bool)
+    m[iu[0][keep_upper], iu[1][keep_upper]] = True
+    m |= m.T
+    row = np.concatenate([[0], np.cumsum(m.sum(1))]).astype(np.int32)
+    col = np.nonzero(m)[1].astype(np.int32)
+    return row, col
+
+
+def test_cormat_average_matches_numpy():
+    from pagani_amd import synth
+    n = 90
+    subs = [synth.bold(40 + s, n, t, 5) for s, t in enumerate((60, 75, 50))]
+    iu = np.triu_indices(n, 1)
+    ccs = [np.corrcoef(x.T.astype(np.float64))[iu] for x in subs]
+    # one subject: (float) r
+    a1 = O.cormat_average(subs[:1])
+    np.testing.assert_allclose(a1, ccs[0].astype(np.float32), rtol=0, atol=6e-8)
+    # three subjects: float32 running sum of float32 r, then / 3 in float32
+    acc = np.zeros(iu[0].size, np.float32)
+    for cc in ccs:
+        acc += cc.astype(np.float32)
+    acc /= np.float32(3)
+    np.testing.assert_allclose(O.cormat_average(subs), acc, rtol=0, atol=2e-7)
+    # Fisher: tanh(mean(arctanh r))
+    want = np.tanh(np.mean([np.arctanh(cc) for cc in ccs], 0))
+    np.testing.assert_allclose(O.cormat_average(subs, True), want, rtol=0, atol=2e-6)
+
+
+def test_dense_thresholds_brute_force():
+    rng = np.random.default_rng(3)
+    n = 70
+    m1 = n * (n - 1) // 2
+    a = np.round(rng.uniform(-1, 1, m1), 2).astype(np.float32)  # many exact ties
+    row, col, val = O.dense_csr_r(a, n, 0.5, True)
+    r2, c2 = _dense_csr_numpy(a > np.float32(0.5), n)
+    np.testing.assert_array_equal(row, r2)
+    np.testing.assert_array_equal(col, c2)
+    for s in (1.0, 7.3, 20.0):
+        k = O.sparsity_k(n, s)
+        row, col, val, t = O.dense_csr_s(a, n, s, True)
+        order = np.argsort(-a, kind="stable")  # descending, ties in row-major order
+        keep = np.zeros(m1, bool)
+        keep[order[:k]] = True
+        assert t == a[order[k - 1]]
+        r2, c2 = _dense_csr_numpy(keep, n)
+        np.testing.assert_array_equal(row, r2)
+        np.testing.assert_array_equal(col, c2)
+        # weights are the matrix values of the kept pairs
+        full = np.zeros((n, n), np.float32)
+        full[np.triu_indices(n, 1)] = a
+        full += full.T
+        np.testing.assert_array_equal(val, full[np.repeat(np.arange(n), np.diff(row)), col])

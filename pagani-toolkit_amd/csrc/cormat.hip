@@ -12,8 +12,11 @@
 //    rounded to FP32 exactly as the FP64 oracle rounds it: the MFMA sum and the
 //    oracle's sequential sum differ by at most delta = 2.5 t 2^-53 (both are
 //    FP64 dot products of unit-norm rows), so when r - delta and r + delta
-//    round to the same float that float is the oracle's; otherwise (about one
-//    pair in 10^5) the lane recomputes the sequential sum.
+//    round to the same float that float is the oracle's; otherwise (mostly
+//    pairs with r near 0, where FP32 steps are tiny) the pair goes to a list
+//    that a second kernel recomputes in the oracle's sequential order, one
+//    thread per pair (a lane stalling its wave for a 240-step dependent sum
+//    would idle the matrix cores).
 //  * The epilogue adds (float) r, or its Fisher z, to the packed accumulator
 //    straight from the MFMA result registers (16 consecutive columns of a row
 //    per 16 lanes).
@@ -30,6 +33,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -67,15 +71,11 @@ __device__ float inv_fisher_f(float z)
     return r;
 }
 
-// FP32 value of the clamped FP64 r of the pair, as the oracle forms it
-__device__ float exact_r(double r, double delta, const double *__restrict__ zi, const double *__restrict__ zj,
-                         int t)
+// The oracle's FP32 value of the pair, (float) of its clamped sequential FP64
+// sum, recomputed in that order.
+__device__ float seq_r(const double *__restrict__ zi, const double *__restrict__ zj, int t)
 {
 #pragma clang fp contract(off)
-    const double lo = fmax(-1.0, fmin(1.0, r - delta));
-    const double hi = fmax(-1.0, fmin(1.0, r + delta));
-    const float flo = (float)lo;
-    if (flo == (float)hi) return flo;
     double s = 0.0;
     for (int l = 0; l < t; l++) {
         const double p = zi[l] * zj[l];
@@ -86,62 +86,137 @@ __device__ float exact_r(double r, double delta, const double *__restrict__ zi, 
     return (float)s;
 }
 
-// Tile (blockIdx.y, blockIdx.x) of rows x columns; waves 2 x 2, each 64 x 64
-// as 4 x 4 MFMA blocks of 16 x 16.  MFMA f64 16x16x4 operands: lane l holds
-// A[l % 16][l / 16] and B[l / 16][l % 16]; result element v of lane l is
-// D[l / 16 + 4 v][l % 16].
-__global__ __launch_bounds__(256) void gram64_accum(const double *__restrict__ z, int n, int t, double delta,
-                                                    int fisher, float *__restrict__ acc)
+// (float) of the clamped MFMA r when every value within delta of it rounds to
+// the same float (then it is the oracle's); *amb set otherwise.
+__device__ __forceinline__ float round_r(double r, double delta, bool *amb)
 {
-    __shared__ double As[GT][GLD];
-    __shared__ double Bs[GT][GLD];
+    const double lo = fmax(-1.0, fmin(1.0, r - delta));
+    const double hi = fmax(-1.0, fmin(1.0, r + delta));
+    const float flo = (float)lo;
+    *amb = flo != (float)hi;
+    return flo;
+}
+
+// Tile (blockIdx.y, blockIdx.x) of rows x columns; 8 waves as 4 x 2, each
+// 32 x 64 as 2 x 4 MFMA blocks of 16 x 16 (two waves per SIMD hide each
+// other's prologue, barrier and epilogue).  MFMA f64 16x16x4 operands: lane l
+// holds A[l % 16][l / 16] and B[l / 16][l % 16]; result element v of lane l is
+// D[l / 16 + 4 v][l % 16].  Software pipeline: the next 16 time points of both
+// panels are loaded into registers while the MFMAs run on the current LDS
+// buffer, then stored into the other buffer; one barrier per step.
+constexpr int GTHREADS = 512;
+constexpr int GLDS = 2 * 2 * GT * GLD;      // doubles: 2 buffers x (A, B)
+constexpr int GNLD = GT * GKC / GTHREADS;   // doubles per thread per panel and step
+
+__global__ __launch_bounds__(GTHREADS) void gram64_accum(const double *__restrict__ z, int n, int t,
+                                                         double delta, int fisher, float *__restrict__ acc,
+                                                         int2 *__restrict__ amb, unsigned *__restrict__ namb,
+                                                         unsigned cap)
+{
+    extern __shared__ double sm[];  // [buf][A|B][GT][GLD]
     const int bi = blockIdx.y, bj = blockIdx.x;
     if (bj < bi) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int wr = wv >> 1, wc = wv & 1;
+    const int wr = wv >> 1, wc = wv & 1;  // rows [32 wr, +32), columns [64 wc, +64)
     const int i0 = bi * GT, j0 = bj * GT;
-    v4d c[4][4];
+    // the accumulator values this lane updates, loaded before the Gram so
+    // their latency hides behind it
+    float old[2][4][4];
 #pragma unroll
-    for (int m = 0; m < 4; m++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) c[m][q] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < t; k0 += GKC) {
-        for (int e = tid; e < GT * GKC; e += 256) {
-            const int r = e / GKC, k = e % GKC;
-            const int gi = i0 + r, gj = j0 + r, gk = k0 + k;
-            As[r][k] = (gi < n && gk < t) ? z[(long long)gi * t + gk] : 0.0;
-            Bs[r][k] = (gj < n && gk < t) ? z[(long long)gj * t + gk] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < GKC / 4; ks++) {
-            double a[4], b[4];
-#pragma unroll
-            for (int m = 0; m < 4; m++) a[m] = As[wr * 64 + m * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
-#pragma unroll
-            for (int q = 0; q < 4; q++) b[q] = Bs[wc * 64 + q * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
-#pragma unroll
-            for (int m = 0; m < 4; m++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) c[m][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[q], c[m][q], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int m = 0; m < 4; m++)
+    for (int m = 0; m < 2; m++)
 #pragma unroll
         for (int q = 0; q < 4; q++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int i = i0 + wr * 64 + m * 16 + (lane >> 4) + 4 * v;
+                const int i = i0 + wr * 32 + m * 16 + (lane >> 4) + 4 * v;
+                const int j = j0 + wc * 64 + q * 16 + (lane & 15);
+                old[m][q][v] = (i < j && j < n) ? acc[pidx(i, j, n)] : 0.0f;
+            }
+    v4d c[2][4];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) c[m][q] = v4d{0.0, 0.0, 0.0, 0.0};
+    double ra[GNLD], rb[GNLD];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int x = 0; x < GNLD; x++) {
+            const int e = tid + GTHREADS * x, r = e / GKC, k = e % GKC;
+            const int gi = i0 + r, gj = j0 + r, gk = k0 + k;
+            ra[x] = (gi < n && gk < t) ? z[(long long)gi * t + gk] : 0.0;
+            rb[x] = (gj < n && gk < t) ? z[(long long)gj * t + gk] : 0.0;
+        }
+    };
+    auto sstore = [&](int buf) {
+        double *A = sm + (size_t)(2 * buf) * GT * GLD, *B = A + GT * GLD;
+#pragma unroll
+        for (int x = 0; x < GNLD; x++) {
+            const int e = tid + GTHREADS * x, r = e / GKC, k = e % GKC;
+            A[r * GLD + k] = ra[x];
+            B[r * GLD + k] = rb[x];
+        }
+    };
+    const int nsteps = (t + GKC - 1) / GKC;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int s = 0; s < nsteps; s++) {
+        const int cur = s & 1;
+        if (s + 1 < nsteps) gload((s + 1) * GKC);
+        const double *A = sm + (size_t)(2 * cur) * GT * GLD, *B = A + GT * GLD;
+#pragma unroll
+        for (int ks = 0; ks < GKC / 4; ks++) {
+            double a[2], b[4];
+#pragma unroll
+            for (int m = 0; m < 2; m++) a[m] = A[(wr * 32 + m * 16 + (lane & 15)) * GLD + ks * 4 + (lane >> 4)];
+#pragma unroll
+            for (int q = 0; q < 4; q++) b[q] = B[(wc * 64 + q * 16 + (lane & 15)) * GLD + ks * 4 + (lane >> 4)];
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) c[m][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[q], c[m][q], 0, 0, 0);
+        }
+        if (s + 1 < nsteps) sstore(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = i0 + wr * 32 + m * 16 + (lane >> 4) + 4 * v;
                 const int j = j0 + wc * 64 + q * 16 + (lane & 15);
                 if (i < j && j < n) {
-                    const float f = exact_r(c[m][q][v], delta, z + (long long)i * t, z + (long long)j * t, t);
+                    bool a;
+                    float f = round_r(c[m][q][v], delta, &a);
+                    if (a) {
+                        // left for gram64_recheck; inline when the list is full
+                        const unsigned slot = atomicAdd(namb, 1u);
+                        if (slot < cap) {
+                            amb[slot] = make_int2(i, j);
+                            continue;
+                        }
+                        f = seq_r(z + (long long)i * t, z + (long long)j * t, t);
+                    }
                     const float x = fisher ? fisher_f(f) : f;
-                    float *p = acc + pidx(i, j, n);
-                    *p = *p + x;
+                    acc[pidx(i, j, n)] = old[m][q][v] + x;
                 }
             }
+}
+
+// The pairs whose rounding the Gram left open: one thread each, the oracle's
+// sequential sum, then the same accumulate.
+__global__ void gram64_recheck(const double *__restrict__ z, int n, int t, int fisher, float *__restrict__ acc,
+                               const int2 *__restrict__ amb, const unsigned *__restrict__ namb, unsigned cap)
+{
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= min(*namb, cap)) return;
+    const int2 p = amb[k];
+    const float f = seq_r(z + (long long)p.x * t, z + (long long)p.y * t, t);
+    const float x = fisher ? fisher_f(f) : f;
+    float *q = acc + pidx(p.x, p.y, n);
+    *q = *q + x;
 }
 
 __global__ void cm_zero(long long m, float *__restrict__ a)
@@ -449,7 +524,22 @@ extern "C" pg_status pg_cormat_add(pg_ctx *ctx, const float *bold, int32_t t, in
     PG_TRY(pg_zscore64(ctx, dbold, layout, n, t, z64));
     const int nt = (int)pg_cdiv(n, GT);
     const double delta = 2.5 * (double)t * std::ldexp(1.0, -53);
-    gram64_accum<<<dim3((unsigned)nt, (unsigned)nt), 256, 0, st>>>(z64, n, t, delta, ctx->cm_fisher, ctx->cm_acc);
+    const size_t shm = (size_t)GLDS * sizeof(double);
+    PG_HIP(ctx, hipFuncSetAttribute((const void *)gram64_accum, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    // ambiguous-rounding list: about one pair in 10^4 is expected
+    const long long m1 = (long long)n * (n - 1) / 2;
+    unsigned cap = (unsigned)std::min<long long>(std::max<long long>(m1 / 256, 1 << 16), 1 << 26);
+    if (const char *e = std::getenv("PAGANI_CM_AMBCAP")) cap = (unsigned)std::max(0, std::atoi(e));
+    int2 *amb;
+    unsigned *namb;
+    PG_TRY(pg_bufT(ctx, "cm.amb", std::max(cap, 1u), &amb));
+    PG_TRY(pg_bufT(ctx, "cm.namb", 1, &namb));
+    PG_HIP(ctx, hipMemsetAsync(namb, 0, 4, st));
+    gram64_accum<<<dim3((unsigned)nt, (unsigned)nt), GTHREADS, shm, st>>>(z64, n, t, delta, ctx->cm_fisher,
+                                                                      ctx->cm_acc, amb, namb, cap);
+    if (cap > 0)
+        gram64_recheck<<<(unsigned)pg_cdiv(cap, 256), 256, 0, st>>>(z64, n, t, ctx->cm_fisher, ctx->cm_acc, amb, namb,
+                                                               cap);
     PG_KERNEL_CHECK(ctx);
     PG_HIP(ctx, hipStreamSynchronize(st));
     ctx->cm_nsub++;

@@ -56,6 +56,15 @@ def test_average_plain_bit_exact(ctx, subjects):
     np.testing.assert_array_equal(got.view(np.uint32), O.cormat_average(subs).view(np.uint32))
 
 
+def test_average_inline_recheck_path(ctx, subjects, monkeypatch):
+    # no room in the ambiguous-pair list: every such pair is recomputed inline
+    n, subs = subjects
+    monkeypatch.setenv("PAGANI_CM_AMBCAP", "0")
+    say("cormat average, inline rechecks")
+    got = _gpu_average(ctx, n, subs[:2], False)
+    np.testing.assert_array_equal(got.view(np.uint32), O.cormat_average(subs[:2]).view(np.uint32))
+
+
 def test_average_fisher_within_one_ulp(ctx, subjects):
     n, subs = subjects
     say("cormat average fisher")

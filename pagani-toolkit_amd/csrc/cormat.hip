@@ -17,9 +17,10 @@
 //    that a second kernel recomputes in the oracle's sequential order, one
 //    thread per pair (a lane stalling its wave for a 240-step dependent sum
 //    would idle the matrix cores).
-//  * The epilogue adds (float) r, or its Fisher z, to the packed accumulator
+//  * The epilogue writes every pair's FP32 r into a per-subject packed buffer
 //    straight from the MFMA result registers (16 consecutive columns of a row
-//    per 16 lanes).
+//    per 16 lanes); a streaming kernel then adds r, or its Fisher z, to the
+//    packed accumulator (the first subject without Fisher is written in place).
 //
 // After the last subject: divide by the subject count, inverse Fisher.  Then
 // thresholds as the reference applies them to the averaged matrix: r-mode
@@ -108,10 +109,9 @@ constexpr int GTHREADS = 512;
 constexpr int GLDS = 2 * 2 * GT * GLD;      // doubles: 2 buffers x (A, B)
 constexpr int GNLD = GT * GKC / GTHREADS;   // doubles per thread per panel and step
 
-__global__ __launch_bounds__(GTHREADS) void gram64_accum(const double *__restrict__ z, int n, int t,
-                                                         double delta, int fisher, float *__restrict__ acc,
-                                                         int2 *__restrict__ amb, unsigned *__restrict__ namb,
-                                                         unsigned cap)
+__global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(4))) void gram64_r(const double *__restrict__ z, int n, int t, int tp,
+                                                     double delta, float *__restrict__ out, int2 *__restrict__ amb,
+                                                     unsigned *__restrict__ namb, unsigned cap)
 {
     extern __shared__ double sm[];  // [buf][A|B][GT][GLD]
     const int bi = blockIdx.y, bj = blockIdx.x;
@@ -119,19 +119,6 @@ __global__ __launch_bounds__(GTHREADS) void gram64_accum(const double *__restric
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv >> 1, wc = wv & 1;  // rows [32 wr, +32), columns [64 wc, +64)
     const int i0 = bi * GT, j0 = bj * GT;
-    // the accumulator values this lane updates, loaded before the Gram so
-    // their latency hides behind it
-    float old[2][4][4];
-#pragma unroll
-    for (int m = 0; m < 2; m++)
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int i = i0 + wr * 32 + m * 16 + (lane >> 4) + 4 * v;
-                const int j = j0 + wc * 64 + q * 16 + (lane & 15);
-                old[m][q][v] = (i < j && j < n) ? acc[pidx(i, j, n)] : 0.0f;
-            }
     v4d c[2][4];
 #pragma unroll
     for (int m = 0; m < 2; m++)
@@ -142,9 +129,8 @@ __global__ __launch_bounds__(GTHREADS) void gram64_accum(const double *__restric
 #pragma unroll
         for (int x = 0; x < GNLD; x++) {
             const int e = tid + GTHREADS * x, r = e / GKC, k = e % GKC;
-            const int gi = i0 + r, gj = j0 + r, gk = k0 + k;
-            ra[x] = (gi < n && gk < t) ? z[(long long)gi * t + gk] : 0.0;
-            rb[x] = (gj < n && gk < t) ? z[(long long)gj * t + gk] : 0.0;
+            ra[x] = z[(long long)(i0 + r) * tp + k0 + k];
+            rb[x] = z[(long long)(j0 + r) * tp + k0 + k];
         }
     };
     auto sstore = [&](int buf) {
@@ -156,7 +142,7 @@ __global__ __launch_bounds__(GTHREADS) void gram64_accum(const double *__restric
             B[r * GLD + k] = rb[x];
         }
     };
-    const int nsteps = (t + GKC - 1) / GKC;
+    const int nsteps = tp / GKC;
     gload(0);
     sstore(0);
     __syncthreads();
@@ -179,6 +165,8 @@ __global__ __launch_bounds__(GTHREADS) void gram64_accum(const double *__restric
         if (s + 1 < nsteps) sstore(cur ^ 1);
         __syncthreads();
     }
+    // every pair's FP32 r, straight from the MFMA registers (16 consecutive
+    // columns of a row per 16 lanes)
 #pragma unroll
     for (int m = 0; m < 2; m++)
 #pragma unroll
@@ -197,26 +185,43 @@ __global__ __launch_bounds__(GTHREADS) void gram64_accum(const double *__restric
                             amb[slot] = make_int2(i, j);
                             continue;
                         }
-                        f = seq_r(z + (long long)i * t, z + (long long)j * t, t);
+                        f = seq_r(z + (long long)i * tp, z + (long long)j * tp, t);
                     }
-                    const float x = fisher ? fisher_f(f) : f;
-                    acc[pidx(i, j, n)] = old[m][q][v] + x;
+                    out[pidx(i, j, n)] = f;
                 }
             }
 }
 
 // The pairs whose rounding the Gram left open: one thread each, the oracle's
-// sequential sum, then the same accumulate.
-__global__ void gram64_recheck(const double *__restrict__ z, int n, int t, int fisher, float *__restrict__ acc,
+// sequential sum.
+__global__ void gram64_recheck(const double *__restrict__ z, int n, int t, int tp, float *__restrict__ out,
                                const int2 *__restrict__ amb, const unsigned *__restrict__ namb, unsigned cap)
 {
     const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= min(*namb, cap)) return;
     const int2 p = amb[k];
-    const float f = seq_r(z + (long long)p.x * t, z + (long long)p.y * t, t);
-    const float x = fisher ? fisher_f(f) : f;
-    float *q = acc + pidx(p.x, p.y, n);
-    *q = *q + x;
+    out[pidx(p.x, p.y, n)] = seq_r(z + (long long)p.x * tp, z + (long long)p.y * tp, t);
+}
+
+// post_block (U CorMat.cpp:806-838): acc += r, or its Fisher z
+__global__ void cm_accumulate(long long m, const float *__restrict__ r, int fisher, float *__restrict__ acc)
+{
+#pragma clang fp contract(off)
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < m;
+         q += (long long)gridDim.x * blockDim.x) {
+        const float x = r[q];
+        acc[q] = acc[q] + (fisher ? fisher_f(x) : x);
+    }
+}
+
+// z rows padded with zeros to [npad][tp]: exact zeros leave every FP64 sum
+// unchanged and let the Gram load without bounds tests
+__global__ void cm_pad(const double *__restrict__ z, int n, int t, int npad, int tp, double *__restrict__ zp)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)npad * tp) return;
+    const int i = (int)(q / tp), l = (int)(q % tp);
+    zp[q] = (i < n && l < t) ? z[(long long)i * t + l] : 0.0;
 }
 
 __global__ void cm_zero(long long m, float *__restrict__ a)
@@ -523,9 +528,13 @@ extern "C" pg_status pg_cormat_add(pg_ctx *ctx, const float *bold, int32_t t, in
     PG_TRY(pg_bufT(ctx, "cm.z64", (size_t)n * t + 1, &z64));
     PG_TRY(pg_zscore64(ctx, dbold, layout, n, t, z64));
     const int nt = (int)pg_cdiv(n, GT);
+    const int npad = nt * GT, tp = (int)pg_cdiv(t, GKC) * GKC;
+    double *zp;
+    PG_TRY(pg_bufT(ctx, "cm.zp", (size_t)npad * tp, &zp));
+    cm_pad<<<(unsigned)pg_cdiv((long long)npad * tp, 256), 256, 0, st>>>(z64, n, t, npad, tp, zp);
     const double delta = 2.5 * (double)t * std::ldexp(1.0, -53);
     const size_t shm = (size_t)GLDS * sizeof(double);
-    PG_HIP(ctx, hipFuncSetAttribute((const void *)gram64_accum, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    PG_HIP(ctx, hipFuncSetAttribute((const void *)gram64_r, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     // ambiguous-rounding list: about one pair in 10^4 is expected
     const long long m1 = (long long)n * (n - 1) / 2;
     unsigned cap = (unsigned)std::min<long long>(std::max<long long>(m1 / 256, 1 << 16), 1 << 26);
@@ -535,11 +544,17 @@ extern "C" pg_status pg_cormat_add(pg_ctx *ctx, const float *bold, int32_t t, in
     PG_TRY(pg_bufT(ctx, "cm.amb", std::max(cap, 1u), &amb));
     PG_TRY(pg_bufT(ctx, "cm.namb", 1, &namb));
     PG_HIP(ctx, hipMemsetAsync(namb, 0, 4, st));
-    gram64_accum<<<dim3((unsigned)nt, (unsigned)nt), GTHREADS, shm, st>>>(z64, n, t, delta, ctx->cm_fisher,
-                                                                      ctx->cm_acc, amb, namb, cap);
+    // the first subject without Fisher is its own sum (0 + r = r): written in
+    // place; otherwise into the subject buffer, then accumulated
+    const bool direct = ctx->cm_nsub == 0 && !ctx->cm_fisher;
+    float *r = ctx->cm_acc;
+    if (!direct) PG_TRY(pg_bufT(ctx, "cm.r", (size_t)m1, &r));
+    gram64_r<<<dim3((unsigned)nt, (unsigned)nt), GTHREADS, shm, st>>>(zp, n, t, tp, delta, r, amb, namb, cap);
     if (cap > 0)
-        gram64_recheck<<<(unsigned)pg_cdiv(cap, 256), 256, 0, st>>>(z64, n, t, ctx->cm_fisher, ctx->cm_acc, amb, namb,
-                                                               cap);
+        gram64_recheck<<<(unsigned)pg_cdiv(cap, 256), 256, 0, st>>>(zp, n, t, tp, r, amb, namb, cap);
+    if (!direct)
+        cm_accumulate<<<(unsigned)std::min<long long>(pg_cdiv(m1, 256), 65536), 256, 0, st>>>(
+            m1, r, ctx->cm_fisher, ctx->cm_acc);
     PG_KERNEL_CHECK(ctx);
     PG_HIP(ctx, hipStreamSynchronize(st));
     ctx->cm_nsub++;

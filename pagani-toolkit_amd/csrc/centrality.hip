@@ -10,6 +10,11 @@
 //    |u - u0|^2, |u + u0|^2) in a fixed order, so all blocks take the same stop
 //    decision on the device and launches past the stop return at once.  The
 //    host looks once per batch of launches.  FP64 throughout, FP32 weights.
+//    The SpMV is column-segmented: each block stages 12,288-entry segments of
+//    the scaled vector in LDS and every wave gathers from there, one row at a
+//    time, over the row's entries in that segment (rows are column-sorted, so
+//    each is a contiguous range found once per call); an FP64 gather straight
+//    from L2 would move a whole cache line per 8-byte element.
 //  * Participation coefficient (W/PC_CPU/PC_CPU/main.cpp:96-160): one wave per
 //    vertex, FP32 module sums accumulated in LDS in CSR order by one lane
 //    (entries staged 64 at a time), then the FP32 sum of squared shares over
@@ -27,8 +32,8 @@ namespace {
 constexpr int EC_MAX_ITER = 10000;  // CUEC_unweighted.cu:25
 constexpr double EC_EPS = 1e-6;     // CUEC_unweighted.cu:28
 constexpr int EC_TPB = 1024;
-constexpr int EC_GMAX = 512;        // row blocks (partials per reduction)
 constexpr int EC_BATCH = 32;        // launches per host check
+constexpr int EC_SEG = 12288;       // vector entries per LDS segment (96 KB)
 
 struct EcState {
     int stopped, iters;
@@ -69,16 +74,38 @@ __global__ __launch_bounds__(EC_TPB) void ec_init(int n, int R, double *__restri
     }
 }
 
-// Iteration k: block b owns rows [b R, (b+1) R); one wave per row.
-__global__ __launch_bounds__(EC_TPB) void ec_iter(const int *__restrict__ row, const int *__restrict__ col,
-                                                  const float *__restrict__ val, int n, int R, int G, int k,
-                                                  const double *__restrict__ uin, double *__restrict__ uout,
-                                                  double *__restrict__ vhat, const double *__restrict__ pin,
-                                                  double *__restrict__ pout, EcState *__restrict__ st)
+// segoff[i (nseg + 1) + s] = first entry of row i with column >= s * EC_SEG
+__global__ void ec_segments(const int *__restrict__ row, const int *__restrict__ col, int n, int nseg,
+                            int *__restrict__ segoff)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int a = row[i], b = row[i + 1];
+    for (int s = 0; s <= nseg; s++) {
+        const long long x = (long long)s * EC_SEG;
+        int lo = a, hi = b;  // first e in [a, b) with col[e] >= x
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (col[mid] < x) lo = mid + 1;
+            else hi = mid;
+        }
+        segoff[(long long)i * (nseg + 1) + s] = lo;
+    }
+}
+
+// Iteration k: block b owns rows [b R, (b+1) R); per segment 16 lanes per row.
+// Dynamic LDS: [EC_SEG] scaled vector segment, then [R] row partials.
+__global__ __launch_bounds__(EC_TPB) void ec_iter(const int *__restrict__ col, const float *__restrict__ val,
+                                                  int n, int R, int G, int k, int nseg,
+                                                  const int *__restrict__ segoff, const double *__restrict__ uin,
+                                                  double *__restrict__ uout, double *__restrict__ vhat,
+                                                  const double *__restrict__ pin, double *__restrict__ pout,
+                                                  EcState *__restrict__ st)
 {
 #pragma clang fp contract(off)
+    extern __shared__ double xs[];
+    double *yp = xs + EC_SEG;
     __shared__ double red[EC_TPB / 64];
-    __shared__ double acc[3][EC_TPB / 64];
     if (st->stopped) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
@@ -106,43 +133,59 @@ __global__ __launch_bounds__(EC_TPB) void ec_iter(const int *__restrict__ row, c
     const double t = 1 / vnorm;
     if (blockIdx.x == 0 && tid == 0) st->lam = vnorm;
     const int r0 = blockIdx.x * R, r1 = min(n, r0 + R);
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;  // lane 0 of each wave, rows in order
-    for (int i = r0 + wv; i < r1; i += EC_TPB / 64) {
-        const int a = row[i], b = row[i + 1];
-        double y = 0.0;
-        if (val) {
-            for (int e = a + lane; e < b; e += 64) y = fma((double)val[e], uin[col[e]] * t, y);
-        } else {
-            for (int e = a + lane; e < b; e += 64) y += uin[col[e]] * t;
-        }
+    for (int q = tid; q < r1 - r0; q += EC_TPB) yp[q] = 0.0;
+    for (int s = 0; s < nseg; s++) {
+        const int x0 = s * EC_SEG, xn = min(EC_SEG, n - x0);
+        __syncthreads();  // the previous segment's readers are done
+        for (int x = tid; x < xn; x += EC_TPB) xs[x] = uin[x0 + x] * t;
+        __syncthreads();
+        // four rows per wave, 16 lanes each, two accumulators per lane: the
+        // rows' index loads are in flight together
+        const int grp = lane >> 4, gl = lane & 15;
+        for (int base = r0 + wv * 4; base < r1; base += EC_TPB / 16) {
+            const int i = base + grp;
+            int a = 0, b = 0;
+            if (i < r1) {
+                const long long so = (long long)i * (nseg + 1) + s;
+                a = segoff[so];
+                b = segoff[so + 1];
+            }
+            double y0 = 0.0, y1 = 0.0;
+            if (val) {
+                for (int e = a + gl; e < b; e += 32) {
+                    y0 = fma((double)val[e], xs[col[e] - x0], y0);
+                    if (e + 16 < b) y1 = fma((double)val[e + 16], xs[col[e + 16] - x0], y1);
+                }
+            } else {
+                for (int e = a + gl; e < b; e += 32) {
+                    y0 += xs[col[e] - x0];
+                    if (e + 16 < b) y1 += xs[col[e + 16] - x0];
+                }
+            }
+            double y = y0 + y1;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o);
-        if (lane == 0) {
-            const double ui = uin[i];
-            vhat[i] = ui * t;
-            uout[i] = y;
-            const double d1 = y - ui, d2 = y + ui;
-            s0 += d1 * d1;
-            s1 += d2 * d2;
-            s2 += y * y;
+            for (int o = 8; o > 0; o >>= 1) y += __shfl_xor(y, o);
+            if (i < r1 && gl == 0) yp[i - r0] += y;
         }
-    }
-    if (lane == 0) {
-        acc[0][wv] = s0;
-        acc[1][wv] = s1;
-        acc[2][wv] = s2;
     }
     __syncthreads();
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int i = r0 + tid; i < r1; i += EC_TPB) {
+        const double y = yp[i - r0], ui = uin[i];
+        vhat[i] = ui * t;
+        uout[i] = y;
+        const double d1 = y - ui, d2 = y + ui;
+        s0 += d1 * d1;
+        s1 += d2 * d2;
+        s2 += y * y;
+    }
+    s0 = ec_block_sum(s0, red);
+    s1 = ec_block_sum(s1, red);
+    s2 = ec_block_sum(s2, red);
     if (tid == 0) {
-        double p0 = 0.0, p1 = 0.0, p2 = 0.0;
-        for (int q = 0; q < EC_TPB / 64; q++) {
-            p0 += acc[0][q];
-            p1 += acc[1][q];
-            p2 += acc[2][q];
-        }
-        pout[3 * blockIdx.x + 0] = p0;
-        pout[3 * blockIdx.x + 1] = p1;
-        pout[3 * blockIdx.x + 2] = p2;
+        pout[3 * blockIdx.x + 0] = s0;
+        pout[3 * blockIdx.x + 1] = s1;
+        pout[3 * blockIdx.x + 2] = s2;
     }
 }
 
@@ -217,10 +260,18 @@ extern "C" pg_status pg_eigenvector_centrality(pg_ctx *ctx, const int32_t *row, 
     PG_TRY(pg_in(ctx, row, (size_t)n + 1, "in.row", &drow));
     PG_TRY(pg_in(ctx, col, (size_t)nnz, "in.col", &dcol));
     if (val) PG_TRY(pg_in(ctx, val, (size_t)nnz, "in.val", &dval));
-    PG_TRY(pg_validate_csr(ctx, drow, dcol, n, nnz));
+    PG_TRY(pg_validate_csr(ctx, drow, dcol, n, nnz, true));
     hipStream_t st = ctx->stream;
-    const int R = (int)std::max<int64_t>(16, pg_cdiv(n, EC_GMAX));
-    const int G = (int)pg_cdiv(n, R);
+    // one block per CU (96 KB segment + row partials in LDS)
+    const int G = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->num_cu, pg_cdiv(n, 64)));
+    const int R = (int)pg_cdiv(n, G);
+    const int nseg = (int)pg_cdiv(n, EC_SEG);
+    const size_t lds = ((size_t)EC_SEG + (size_t)R) * sizeof(double);
+    if (lds > 160 * 1024) return pg_fail(ctx, PG_ENOTSUP, "n = %d needs %zu bytes of LDS per block", n, lds);
+    PG_HIP(ctx, hipFuncSetAttribute((const void *)ec_iter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int *segoff;
+    PG_TRY(pg_bufT(ctx, "ec.segoff", (size_t)n * (nseg + 1), &segoff));
+    ec_segments<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(drow, dcol, n, nseg, segoff);
     double *u[2], *part[2], *dvh, *dav;
     EcState *dst;
     PG_TRY(pg_bufT(ctx, "ec.u0", (size_t)n, &u[0]));
@@ -237,8 +288,8 @@ extern "C" pg_status pg_eigenvector_centrality(pg_ctx *ctx, const int32_t *row, 
     EcState h{};
     for (int k0 = 0; k0 <= EC_MAX_ITER; k0 += EC_BATCH) {
         for (int k = k0; k < k0 + EC_BATCH && k <= EC_MAX_ITER; k++)
-            ec_iter<<<(unsigned)G, EC_TPB, 0, st>>>(drow, dcol, dval, n, R, G, k, u[k & 1], u[(k + 1) & 1], dvh,
-                                                    part[(k + 1) & 1], part[k & 1], dst);
+            ec_iter<<<(unsigned)G, EC_TPB, lds, st>>>(dcol, dval, n, R, G, k, nseg, segoff, u[k & 1],
+                                                      u[(k + 1) & 1], dvh, part[(k + 1) & 1], part[k & 1], dst);
         PG_KERNEL_CHECK(ctx);
         PG_HIP(ctx, hipMemcpyAsync(&h, dst, sizeof h, hipMemcpyDeviceToHost, st));
         PG_HIP(ctx, hipStreamSynchronize(st));
@@ -272,7 +323,7 @@ extern "C" pg_status pg_participation(pg_ctx *ctx, const int32_t *row, const int
     PG_TRY(pg_in(ctx, col, (size_t)nnz, "in.col", &dcol));
     if (val) PG_TRY(pg_in(ctx, val, (size_t)nnz, "in.val", &dval));
     PG_TRY(pg_in(ctx, labels, (size_t)n, "in.labels", &dlab));
-    PG_TRY(pg_validate_csr(ctx, drow, dcol, n, nnz));
+    PG_TRY(pg_validate_csr(ctx, drow, dcol, n, nnz, false));
     hipStream_t st = ctx->stream;
     int *flags, hf[2] = {0, 0};
     PG_TRY(pg_bufT(ctx, "pc.flags", 2, &flags));

@@ -230,7 +230,7 @@ extern "C" pg_status pg_bfs_efficiency(pg_ctx *ctx, const int32_t *row, const in
     PG_TRY(pg_out(ctx, eff, (size_t)n, "out.eff", &deff));
     if (!deff) PG_TRY(pg_bufT(ctx, "eff.scratch", (size_t)n, &deff));
     PG_HIP(ctx, hipMemsetAsync(flags, 0, 8, st));
-    PG_TRY(pg_validate_csr(ctx, drow, dcol, n, nnz));
+    PG_TRY(pg_validate_csr(ctx, drow, dcol, n, nnz, false));
 
     Planes pl = {};
     int nplanes = 0;
@@ -292,30 +292,39 @@ extern "C" pg_status pg_bfs_efficiency(pg_ctx *ctx, const int32_t *row, const in
 }
 
 namespace {
-// bad = 1 unless row is non-decreasing from 0 to nnz and every col is in [0, n)
+// bad = 1 unless row is non-decreasing from 0 to nnz and every col is in
+// [0, n); sorted: also strictly ascending columns inside each row
 __global__ void csr_validate(const int *__restrict__ row, const int *__restrict__ col, int n, long long nnz,
-                             int *__restrict__ bad)
+                             int sorted, int *__restrict__ bad)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     bool ok = true;
-    if (t < n) ok = row[t] <= row[t + 1];
+    if (t < n) {
+        const int a = row[t], b = row[t + 1];
+        ok = a <= b;
+        if (ok && sorted)
+            for (int e = a + 1; e < b; e++) ok = ok && col[e - 1] < col[e];
+    }
     if (t < nnz) ok = ok && col[t] >= 0 && col[t] < n;
     if (t == 0) ok = ok && row[0] == 0 && (long long)row[n] == nnz;
     if (!ok) atomicOr(bad, 1);
 }
 }  // namespace
 
-pg_status pg_validate_csr(pg_ctx *ctx, const int32_t *drow, const int32_t *dcol, int32_t n, int64_t nnz)
+pg_status pg_validate_csr(pg_ctx *ctx, const int32_t *drow, const int32_t *dcol, int32_t n, int64_t nnz,
+                          bool sorted)
 {
     hipStream_t st = ctx->stream;
     int *bad, hbad = 0;
     PG_TRY(pg_bufT(ctx, "csr.bad", 1, &bad));
     PG_HIP(ctx, hipMemsetAsync(bad, 0, 4, st));
     const long long m = std::max<long long>(std::max<long long>(n, nnz), 1);
-    csr_validate<<<(unsigned)pg_cdiv(m, 256), 256, 0, st>>>(drow, dcol, n, nnz, bad);
+    csr_validate<<<(unsigned)pg_cdiv(m, 256), 256, 0, st>>>(drow, dcol, n, nnz, sorted ? 1 : 0, bad);
     PG_KERNEL_CHECK(ctx);
     PG_HIP(ctx, hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
     PG_HIP(ctx, hipStreamSynchronize(st));
-    if (hbad) return pg_fail(ctx, PG_EFORMAT, "malformed CSR (row not 0..nnz non-decreasing or col out of range)");
+    if (hbad)
+        return pg_fail(ctx, PG_EFORMAT, "malformed CSR (row not 0..nnz non-decreasing, col out of range%s)",
+                       sorted ? " or columns not ascending in a row" : "");
     return PG_OK;
 }

@@ -119,5 +119,7 @@ inline int64_t pg_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 pg_status pg_zscore64(pg_ctx *ctx, const float *dbold, int layout, int n, int t, double *z64);
 
 // Device check of a staged CSR (device pointers): row non-decreasing from 0 to
-// nnz, every column in [0, n).  PG_EFORMAT otherwise.  Synchronises the stream.
-pg_status pg_validate_csr(pg_ctx *ctx, const int32_t *drow, const int32_t *dcol, int32_t n, int64_t nnz);
+// nnz, every column in [0, n), and with `sorted` strictly ascending columns in
+// each row.  PG_EFORMAT otherwise.  Synchronises the stream.
+pg_status pg_validate_csr(pg_ctx *ctx, const int32_t *drow, const int32_t *dcol, int32_t n, int64_t nnz,
+                          bool sorted);

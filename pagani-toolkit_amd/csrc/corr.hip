@@ -1651,11 +1651,26 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
     const unsigned long long k1 = radix_select(
         c, need, [&](long long q) { return key_r_desc(r64[q]); }, [](long long) { return true; }, hist, sh,
         &ties_left);
-    long long dummy = 0;
-    const unsigned long long k2 = radix_select(
-        c, ties_left,
-        [&](long long q) { return ((unsigned long long)(unsigned)cand[q].x << 32) | (unsigned)cand[q].y; },
-        [&](long long q) { return key_r_desc(r64[q]) == k1; }, hist, sh, &dummy);
+    // ties at t*: ordered by (i, j) only when some of them are left out (with
+    // FP64 r an exact tie is rare, and then all of the few are wanted)
+    unsigned long long k2 = ~0ull;
+    {
+        unsigned long long nt = 0;
+        for (long long q = threadIdx.x; q < c; q += blockDim.x) nt += key_r_desc(r64[q]) == k1;
+        if (threadIdx.x == 0) sh[0] = 0;
+        __syncthreads();
+        atomicAdd(&sh[0], nt);
+        __syncthreads();
+        const long long ntie = (long long)sh[0];
+        __syncthreads();
+        if (ntie != ties_left) {
+            long long dummy = 0;
+            k2 = radix_select(
+                c, ties_left,
+                [&](long long q) { return ((unsigned long long)(unsigned)cand[q].x << 32) | (unsigned)cand[q].y; },
+                [&](long long q) { return key_r_desc(r64[q]) == k1; }, hist, sh, &dummy);
+        }
+    }
     // t* is the r64 carrying key k1
     for (long long q = threadIdx.x; q < c; q += blockDim.x)
         if (key_r_desc(r64[q]) == k1) tstar[0] = r64[q];

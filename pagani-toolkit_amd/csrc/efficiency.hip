@@ -102,16 +102,48 @@ __global__ __launch_bounds__(256) void eff_level(const int *__restrict__ row, co
     if (__ballot(nb != 0) != 0 && lane == 0) atomicAdd(&prog[d], 1u);
 }
 
-// One wave per 64 sources (word k of the batch), one lane per source.
-template <int NP>
-__device__ __forceinline__ int level_of(const Planes &pl, size_t at, int lane)
+// Level 0 -> 1 from the column indices alone: the sources at distance 1 from
+// w are its neighbours inside the batch.  One wave per (w, 64-word chunk); the
+// chunk's 4096 sources are a contiguous column range of w's sorted row, found
+// by binary search, and each such neighbour sets its bit in the lane owning
+// its word -- no neighbour visited words are read.
+__global__ __launch_bounds__(256) void eff_level0(const int *__restrict__ row, const int *__restrict__ col, int n,
+                                                  int W, int base, const u64 *__restrict__ visA,
+                                                  u64 *__restrict__ visB, Planes planes,
+                                                  unsigned *__restrict__ prog)
 {
-    int lv = 0;
-#pragma unroll
-    for (int b = 0; b < NP; b++) lv |= (int)((pl.p[b][at] >> lane) & 1ull) << b;
-    return lv;
+    const int lane = threadIdx.x & 63;
+    const int chunks = W >> 6;
+    const long long wid = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= (long long)n * chunks) return;
+    const int w = __builtin_amdgcn_readfirstlane((int)(wid / chunks));
+    const int c = __builtin_amdgcn_readfirstlane((int)(wid % chunks));
+    const int k = (c << 6) + lane;
+    const long long v0 = (long long)base + 4096ll * c, v1 = v0 + 4096;
+    int lo = row[w], hi = row[w + 1];
+    while (lo < hi) {  // first entry with col >= v0
+        const int mid = (lo + hi) >> 1;
+        if (col[mid] < v0) lo = mid + 1;
+        else hi = mid;
+    }
+    u64 acc = 0;
+    for (int e = lo; e < row[w + 1]; e++) {
+        const long long v = col[e];
+        if (v >= v1) break;
+        const int rel = (int)(v - v0);
+        if ((rel >> 6) == lane) acc |= 1ull << (rel & 63);
+    }
+    const u64 vis = visA[(size_t)w * W + k];
+    const u64 nb = acc & ~vis;
+    visB[(size_t)w * W + k] = vis | nb;
+    if (nb) planes.p[0][(size_t)w * W + k] |= nb;
+    if (__ballot(nb != 0) != 0 && lane == 0) atomicAdd(&prog[0], 1u);
 }
 
+// One wave per 64 sources (word k of the batch), one lane per source.  The
+// plane words of 64 consecutive j are fetched by one vector load per plane
+// (lane l holds j0 + l), then j walks them in order through readlane, so the
+// FP32 sums keep the reference's order while the loads are batched.
 template <int NP>
 __global__ __launch_bounds__(64) void eff_sum(int n, int W, int base, int nsrc, Planes planes,
                                               float *__restrict__ eff, double *__restrict__ lpp,
@@ -123,28 +155,31 @@ __global__ __launch_bounds__(64) void eff_sum(int n, int W, int base, int nsrc, 
     float acc = 0.0f;
     double dacc = 0.0;
     int reached = 0;
-    int j = 0;
-    for (; j + 4 <= n; j += 4) {
-        int lv[4];
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        const int jl = j0 + lane;
+        uint32_t wlo[NP], whi[NP];
 #pragma unroll
-        for (int q = 0; q < 4; q++) lv[q] = level_of<NP>(planes, (size_t)(j + q) * W + k, lane);
+        for (int b = 0; b < NP; b++) {
+            const u64 x = jl < n ? planes.p[b][(size_t)jl * W + k] : 0ull;
+            wlo[b] = (uint32_t)x;
+            whi[b] = (uint32_t)(x >> 32);
+        }
+        const int cnt = min(64, n - j0);
+        for (int jj = 0; jj < cnt; jj++) {
+            int lv = 0;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            if (lv[q] != 0) {
-                const float r = 1.0f / (float)lv[q];
+            for (int b = 0; b < NP; b++) {
+                const uint32_t lo = __builtin_amdgcn_readlane(wlo[b], jj);
+                const uint32_t hi = __builtin_amdgcn_readlane(whi[b], jj);
+                const uint32_t word = lane < 32 ? lo : hi;
+                lv |= (int)((word >> (lane & 31)) & 1u) << b;
+            }
+            if (lv != 0) {
+                const float r = 1.0f / (float)lv;
                 acc += r;
                 reached++;
-                if (j + q > s) dacc += (double)r;
+                if (j0 + jj > s) dacc += (double)r;
             }
-        }
-    }
-    for (; j < n; j++) {
-        const int lv = level_of<NP>(planes, (size_t)j * W + k, lane);
-        if (lv != 0) {
-            const float r = 1.0f / (float)lv;
-            acc += r;
-            reached++;
-            if (j > s) dacc += (double)r;
         }
     }
     if (64 * k + lane >= nsrc) return;
@@ -230,7 +265,7 @@ extern "C" pg_status pg_bfs_efficiency(pg_ctx *ctx, const int32_t *row, const in
     PG_TRY(pg_out(ctx, eff, (size_t)n, "out.eff", &deff));
     if (!deff) PG_TRY(pg_bufT(ctx, "eff.scratch", (size_t)n, &deff));
     PG_HIP(ctx, hipMemsetAsync(flags, 0, 8, st));
-    PG_TRY(pg_validate_csr(ctx, drow, dcol, n, nnz, false));
+    PG_TRY(pg_validate_csr(ctx, drow, dcol, n, nnz, true));
 
     Planes pl = {};
     int nplanes = 0;
@@ -259,7 +294,10 @@ extern "C" pg_status pg_bfs_efficiency(pg_ctx *ctx, const int32_t *row, const in
             if (bit_width((unsigned)top) > MAXPLANES) return pg_fail(ctx, PG_ENOTSUP, "graph too deep");
             PG_TRY(grow(bit_width((unsigned)top)));
             for (int q = 0; q < LEVEL_GROUP; q++) {
-                eff_level<<<blocks, 256, 0, st>>>(drow, dcol, n, W, d + q, va, vb, pl, nplanes, prog);
+                if (d + q == 0)
+                    eff_level0<<<blocks, 256, 0, st>>>(drow, dcol, n, W, base, va, vb, pl, prog);
+                else
+                    eff_level<<<blocks, 256, 0, st>>>(drow, dcol, n, W, d + q, va, vb, pl, nplanes, prog);
                 std::swap(va, vb);
             }
             PG_KERNEL_CHECK(ctx);

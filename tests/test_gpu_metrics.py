@@ -217,6 +217,14 @@ def test_bfs_efficiency_rejects_bad_csr(ctx):
     from pagani_amd.lib import PgError
     with pytest.raises(PgError, match="malformed CSR"):
         ctx.bfs_efficiency(row, bad)
+    # rows must list their columns in ascending order
+    row, col = star_graph(6)[:2]
+    swapped = col.copy()
+    swapped[[0, 1]] = swapped[[1, 0]]
+    with pytest.raises(PgError, match="ascending"):
+        ctx.bfs_efficiency(row, swapped)
+    with pytest.raises(PgError, match="ascending"):
+        ctx.eigenvector_centrality(row, swapped)
 
 
 def test_fw_unit_weights_equal_bfs(ctx):

@@ -75,8 +75,18 @@ def _csr_files(d, weighted=False):
     return sorted(f for f in os.listdir(d) if f.endswith(".csr"))
 
 
+def _ensure_networks(d):
+    """The per-subject networks the metric tools read (the first two tests
+    write them; a selection that skips those tests gets them here)."""
+    if not _csr_files(d):
+        run("CUCorMat", d, 0.5, "n", "n", "r", 0.35, 0.3)
+    if not (d / "weighted").exists() or not _csr_files(d / "weighted"):
+        run("CUCorMat", "--weighted", d, d / "mask.nii", 0.5, "n", "n", "s", 2.0)
+
+
 def test_metric_tools_on_binary_networks(volume):
     d, subs, n = volume
+    _ensure_networks(d)
     run("Degree", d)
     run("CUBC", d)
     run("Cp", d, 0, "gn")
@@ -95,6 +105,7 @@ def test_metric_tools_on_binary_networks(volume):
 
 def test_bfs_lp_tool_on_binary_networks(volume):
     d, subs, n = volume
+    _ensure_networks(d)
     run("CUBFS_Lp", d, 0, "b")
     for f in _csr_files(d):
         base = str(d / f)[:-4]
@@ -106,6 +117,7 @@ def test_bfs_lp_tool_on_binary_networks(volume):
 
 def test_lp_and_modularity_tools_on_weighted_networks(volume):
     d, subs, n = volume
+    _ensure_networks(d)
     w = d / "weighted"
     run("CUBFW_Lp", w, 0, "gn")
     run("CUmodularity", w, 0)
@@ -135,6 +147,7 @@ def test_lp_and_modularity_tools_on_weighted_networks(volume):
 
 def test_ec_and_pc_tools(volume):
     d, subs, n = volume
+    _ensure_networks(d)
     w = d / "weighted"
     run("CUmodularity", w, 0)      # module labels for PC_CPU
     run("PC_CPU", w)

@@ -104,3 +104,14 @@ def test_state_errors(ctx):
         ctx.cormat_threshold(0.5)
     with pytest.raises(PgError, match="no subject"):
         ctx.cormat_finish()
+
+
+def test_smallest_matrix(ctx):
+    # n = 2, t = 2: one pair, r = +-1 exactly; 100% sparsity keeps it
+    x = np.array([[0.0, 1.0], [2.0, 3.0]], np.float32)
+    say("cormat n=2 t=2")
+    got = _gpu_average(ctx, 2, [x], False)
+    np.testing.assert_array_equal(got.view(np.uint32), O.cormat_average([x]).view(np.uint32))
+    info = ctx.cormat_threshold(100.0, sparsity=True)
+    row, col, _ = ctx.corr_csr(2, info.nnz)
+    assert info.nnz == 2 and list(row) == [0, 1, 2] and list(col) == [1, 0]

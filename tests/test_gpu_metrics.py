@@ -309,3 +309,31 @@ def test_pc_bit_exact(ctx, plain_graph, weighted_graph):
     from pagani_amd.lib import PgError
     with pytest.raises(PgError, match="labels"):
         ctx.participation(row, col, lab)
+
+
+def _same(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+
+
+@pytest.mark.parametrize("edges", [[], [(0, 1)]])
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_edge_cases_tiny_graphs(ctx, n, edges):
+    from helpers import csr_from_edges
+    edges = [e for e in edges if max(e) < n]
+    row, col = csr_from_edges(n, edges)[:2]
+    say(f"tiny graph n={n} edges={edges}")
+    e, lp, iso, unc = ctx.bfs_efficiency(row, col)
+    oe, olp, oiso, ounc = O.bfs_efficiency(row, col)
+    assert _same(e, oe) and _same([lp], [olp]) and (iso, unc) == (oiso, ounc)
+    # power iteration: no edges leave u = 0 after one step (x = 0/0 on the next)
+    vh, av, lam, it = ctx.eigenvector_centrality(row, col)
+    ovh, oav, olam, oit = O.ec(row, col)
+    assert it == oit and _same(vh, ovh) and _same(av, oav) and _same([lam], [olam])
+    # one module: the normalised coefficient is the coefficient
+    lab = np.ones(n, np.int32)
+    p, q, k = ctx.participation(row, col, lab)
+    op, oq, ok = O.pc(row, col, lab)
+    assert k == ok == 1
+    np.testing.assert_array_equal(p.view(np.uint32), op.view(np.uint32))
+    np.testing.assert_array_equal(q.view(np.uint32), p.view(np.uint32))

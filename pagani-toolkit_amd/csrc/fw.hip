@@ -4,8 +4,9 @@
 // kernels of CUBFW_ph1.cu / CUBFW_ph2.cu / CUBFW_ph3.cu.  The whole padded
 // matrix stays resident in HBM (the reference streams every tile over PCIe each
 // round).  Per k-block round:
-//   phase 1: the diagonal tile, sequential k in LDS (one work-group);
-//   phase 2: row and column panel tiles, sequential k in LDS;
+//   phase 1: the diagonal tile, sequential k over register micro-tiles (one
+//            work-group);
+//   phase 2: row and column panel tiles, the same;
 //   phase 3: every other tile, min over k of (ik + kj), 4x4 register
 //            micro-tiles over LDS panels.
 // Min-plus is not a dense contraction, so this runs on VALU (v_add_f32 +
@@ -26,13 +27,19 @@ __device__ __forceinline__ float relax(float cur, float cand) { return cand < cu
 
 // Phase 1/2 tile update with sequential k.  role 0: diagonal tile (ik = kj =
 // self); role 1: row panel (ik = diag, kj = self); role 2: column panel
-// (ik = self, kj = diag).
+// (ik = self, kj = diag).  The tile lives in registers, an M x M micro-tile per
+// thread (M = B/16, 256 threads); at step k the threads holding row k and
+// column k of the tile publish them to a double-buffered LDS row / column
+// (one barrier per step), everyone relaxes its micro-tile against them.  Row
+// and column k do not change during step k (d_kk = 0), as in the sequential
+// reference loop, so every element sees the same operands there.
 template <int B>
 __global__ __launch_bounds__(256) void fw_seq_tiles(float *__restrict__ d, int np, int kb, int phase)
 {
-    __shared__ float self[B][B + 1];
+    constexpr int M = B / 16;
     __shared__ float diag[B][B + 1];
-    const int tid = threadIdx.x;
+    __shared__ float rowk[2][B], colk[2][B];
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     int tr, tc, role;
     const int nbk = np / B;
     if (phase == 1) {
@@ -46,27 +53,43 @@ __global__ __launch_bounds__(256) void fw_seq_tiles(float *__restrict__ d, int n
         else { t -= h; tr = t < kb ? t : t + 1; tc = kb; role = 2; }
     }
     const long long r0 = (long long)tr * B, c0 = (long long)tc * B, k0 = (long long)kb * B;
-    for (int e = tid; e < B * B; e += 256) {
-        int i = e / B, j = e % B;
-        self[i][j] = d[(r0 + i) * np + c0 + j];
-        if (role != 0) diag[i][j] = d[(k0 + i) * np + k0 + j];
-    }
-    __syncthreads();
-    for (int k = 0; k < B; k++) {
+    if (role != 0)
         for (int e = tid; e < B * B; e += 256) {
-            int i = e / B, j = e % B;
-            float ik, kj;
-            if (role == 0) { ik = self[i][k]; kj = self[k][j]; }
-            else if (role == 1) { ik = diag[i][k]; kj = self[k][j]; }
-            else { ik = self[i][k]; kj = diag[k][j]; }
-            self[i][j] = relax(self[i][j], ik + kj);
+            const int i = e / B, j = e % B;
+            diag[i][j] = d[(k0 + i) * np + k0 + j];
         }
-        __syncthreads();
+    float acc[M][M];
+#pragma unroll
+    for (int a = 0; a < M; a++)
+#pragma unroll
+        for (int b = 0; b < M; b++) acc[a][b] = d[(r0 + ty * M + a) * np + c0 + tx * M + b];
+    __syncthreads();
+    for (int q = 0; q < 16; q++) {  // k = q M + kk
+#pragma unroll
+        for (int kk = 0; kk < M; kk++) {
+            const int k = q * M + kk, buf = k & 1;
+            if (role != 2 && ty == q)
+#pragma unroll
+                for (int b = 0; b < M; b++) rowk[buf][tx * M + b] = acc[kk][b];
+            if (role != 1 && tx == q)
+#pragma unroll
+                for (int a = 0; a < M; a++) colk[buf][ty * M + a] = acc[a][kk];
+            __syncthreads();
+            float ik[M], kj[M];
+#pragma unroll
+            for (int a = 0; a < M; a++) ik[a] = role == 1 ? diag[ty * M + a][k] : colk[buf][ty * M + a];
+#pragma unroll
+            for (int b = 0; b < M; b++) kj[b] = role == 2 ? diag[k][tx * M + b] : rowk[buf][tx * M + b];
+#pragma unroll
+            for (int a = 0; a < M; a++)
+#pragma unroll
+                for (int b = 0; b < M; b++) acc[a][b] = relax(acc[a][b], ik[a] + kj[b]);
+        }
     }
-    for (int e = tid; e < B * B; e += 256) {
-        int i = e / B, j = e % B;
-        d[(r0 + i) * np + c0 + j] = self[i][j];
-    }
+#pragma unroll
+    for (int a = 0; a < M; a++)
+#pragma unroll
+        for (int b = 0; b < M; b++) d[(r0 + ty * M + a) * np + c0 + tx * M + b] = acc[a][b];
 }
 
 // Phase 3: tile (r, c) = min(tile, min_k ik[i][k] + kj[k][j]) for r, c != kb.

@@ -240,6 +240,23 @@ pg_status pg_participation(pg_ctx *ctx, const int32_t *row, const int32_t *col, 
                            int32_t n, int64_t nnz, const int32_t *labels, float *pc, float *pc_norm,
                            int32_t *nmod);
 
+/* Degree-preserving random rewiring by double edge swaps, the random
+ * reference network of Cp (gamma), CUBFS_Lp / CUBFW_Lp (lambda) and the
+ * Maslov tool: replaces Maslov (U/BFS_MulCPU/BFS_MulCPU/Maslov.cpp:64) and
+ * Maslov_weighted_1 (W/Maslov/Maslov/Maslov.cpp:106).  The input is a
+ * symmetric CSR without self-loops, columns strictly ascending in each row
+ * (PG_EFORMAT otherwise).  attempts swap attempts are made (0: 2 * nnz/2, the
+ * reference's count); an attempt draws two edges, gives up when they share a
+ * vertex, picks 1-3,2-4 or 1-4,3-2 by a fair coin and gives up when either new
+ * edge exists.  The row offsets are unchanged (every vertex keeps its degree),
+ * so only out_col (nnz) is written, rows sorted by column; with val and out_val
+ * both non-NULL each edge keeps its weight in out_val (nnz).  The reference
+ * seeds with the wall clock; here the result is a function of seed alone.
+ * swaps (may be NULL) receives the number of swaps made. */
+pg_status pg_rewire(pg_ctx *ctx, const int32_t *row, const int32_t *col, const float *val, int32_t n,
+                    int64_t nnz, uint64_t seed, int64_t attempts, int32_t *out_col, float *out_val,
+                    int64_t *swaps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,16 @@
  *                       identity on (float)r
  *   orc_dense_csr_*     brute force with numpy (mask a > thr; argsort for the
  *                       k-th largest and the row-major tie rule)
+ *   orc_rewire          invariants (degrees, symmetry, no self-loops or
+ *                       multi-edges, each edge's weight in both directions,
+ *                       weight multiset); closed cases: graphs where no swap
+ *                       is possible (stars, complete graphs) stay unchanged,
+ *                       two disjoint edges reach all three perfect matchings;
+ *                       clustering of a rewired ring lattice against networkx
+ *                       double_edge_swap with the same number of swaps.  The
+ *                       reference Maslov.cpp is not built: its index
+ *                       expression assumes RAND_MAX = 0x7fff and overflows int
+ *                       with glibc's rand().
  */
 #include "oracle.h"
 
@@ -913,4 +923,158 @@ float orc_dense_csr_s(const float *a, int64_t n, int64_t k, int32_t *row, int32_
     if (row) dense_csr(a, n, keep, row, col, val);
     free(keep);
     return t;
+}
+
+/* -------------------------------------------------------------- rewiring */
+
+/* splitmix64 (Steele, Lea, Flood 2014), the random stream standing in for the
+ * reference's srand(time(0)) / rand(). */
+static uint64_t sm64_next(uint64_t *s)
+{
+    uint64_t z = (*s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+/* Set of undirected edges: linear probing, backward-shift deletion. */
+typedef struct {
+    uint64_t *t;
+    uint64_t mask;
+} edge_set;
+#define ES_EMPTY (~0ull)
+
+static uint64_t es_key(int32_t a, int32_t b)
+{
+    const uint32_t lo = (uint32_t)(a < b ? a : b), hi = (uint32_t)(a < b ? b : a);
+    return ((uint64_t)lo << 32) | hi;
+}
+static uint64_t es_home(const edge_set *s, uint64_t k)
+{
+    uint64_t z = k * 0x9e3779b97f4a7c15ull;
+    return (z ^ (z >> 29)) & s->mask;
+}
+static uint64_t es_slot(const edge_set *s, uint64_t k)  /* slot of k, or its first empty */
+{
+    uint64_t i = es_home(s, k);
+    while (s->t[i] != ES_EMPTY && s->t[i] != k) i = (i + 1) & s->mask;
+    return i;
+}
+static int es_has(const edge_set *s, int32_t a, int32_t b)
+{
+    const uint64_t k = es_key(a, b);
+    return s->t[es_slot(s, k)] == k;
+}
+static void es_add(edge_set *s, int32_t a, int32_t b)
+{
+    const uint64_t k = es_key(a, b);
+    s->t[es_slot(s, k)] = k;
+}
+static void es_del(edge_set *s, int32_t a, int32_t b)
+{
+    uint64_t i = es_slot(s, es_key(a, b)), j = i;
+    for (;;) {
+        j = (j + 1) & s->mask;
+        if (s->t[j] == ES_EMPTY) break;
+        const uint64_t h = es_home(s, s->t[j]);
+        /* t[j] may move back to i unless its home lies cyclically in (i, j] */
+        const int stays = i <= j ? (h > i && h <= j) : (h > i || h <= j);
+        if (!stays) {
+            s->t[i] = s->t[j];
+            i = j;
+        }
+    }
+    s->t[i] = ES_EMPTY;
+}
+
+typedef struct {
+    int32_t c;
+    float v;
+} col_val;
+static int cmp_col_val(const void *a, const void *b)
+{
+    const int32_t x = ((const col_val *)a)->c, y = ((const col_val *)b)->c;
+    return (x > y) - (x < y);
+}
+
+/* Maslov / Rewire (U BFS_MulCPU/Maslov.cpp:8-110) and Maslov_weighted_1 (W
+ * Maslov/Maslov/Maslov.cpp:50-196): the edges (upper triangle, row-major)
+ * into E; per attempt i = rnd % M, j = rnd % M, give up when the two edges
+ * share a vertex, else rnd & 1 picks 1-3,2-4 (E[i].C = V3, E[j].R = V2) or
+ * 1-4,3-2 (E[i].C = V4, E[j].C = V2), given up when either new edge exists.
+ * Weights stay with their E slot.  The output keeps the row offsets; columns
+ * (and weights) are sorted per row.  attempts 0 means 2M.  Returns the swap
+ * count. */
+int64_t orc_rewire(const int32_t *row, const int32_t *col, const float *val, int64_t n, uint64_t seed,
+                   int64_t attempts, int32_t *out_col, float *out_val)
+{
+    const int64_t nnz = row[n];
+    int64_t m = 0;
+    for (int64_t u = 0; u < n; u++)
+        for (int32_t e = row[u]; e < row[u + 1]; e++) m += col[e] > u;
+    int32_t *er = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m + 1));
+    int32_t *ec = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m + 1));
+    float *ev = (float *)malloc(sizeof(float) * (size_t)(m + 1));
+    edge_set s;
+    uint64_t cap = 1024;
+    while (cap < 4 * (uint64_t)m) cap <<= 1;
+    s.mask = cap - 1;
+    s.t = (uint64_t *)malloc(sizeof(uint64_t) * cap);
+    for (uint64_t q = 0; q < cap; q++) s.t[q] = ES_EMPTY;
+    int64_t k = 0;
+    for (int64_t u = 0; u < n; u++)
+        for (int32_t e = row[u]; e < row[u + 1]; e++)
+            if (col[e] > u) {
+                er[k] = (int32_t)u;
+                ec[k] = col[e];
+                ev[k] = val ? val[e] : 1.0f;
+                es_add(&s, (int32_t)u, col[e]);
+                k++;
+            }
+    const int64_t tries = attempts ? attempts : 2 * m;
+    uint64_t st = seed;
+    int64_t swaps = 0;
+    for (int64_t q = 0; m > 0 && q < tries; q++) {
+        const int64_t i = (int64_t)(sm64_next(&st) % (uint64_t)m);
+        const int64_t j = (int64_t)(sm64_next(&st) % (uint64_t)m);
+        const int32_t v1 = er[i], v2 = ec[i], v3 = er[j], v4 = ec[j];
+        if (v1 == v3 || v1 == v4 || v2 == v3 || v2 == v4) continue;
+        if (sm64_next(&st) & 1) {
+            if (es_has(&s, v1, v3) || es_has(&s, v2, v4)) continue;
+            es_del(&s, v1, v2);
+            es_del(&s, v3, v4);
+            es_add(&s, v1, v3);
+            es_add(&s, v2, v4);
+            ec[i] = v3;
+            er[j] = v2;
+        } else {
+            if (es_has(&s, v1, v4) || es_has(&s, v2, v3)) continue;
+            es_del(&s, v1, v2);
+            es_del(&s, v3, v4);
+            es_add(&s, v1, v4);
+            es_add(&s, v2, v3);
+            ec[i] = v4;
+            ec[j] = v2;
+        }
+        swaps++;
+    }
+    col_val *buf = (col_val *)malloc(sizeof(col_val) * (size_t)(nnz + 1));
+    int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    for (int64_t u = 0; u < n; u++) cur[u] = row[u];
+    for (int64_t e = 0; e < m; e++) {
+        buf[cur[er[e]]++] = (col_val){ec[e], ev[e]};
+        buf[cur[ec[e]]++] = (col_val){er[e], ev[e]};
+    }
+    for (int64_t u = 0; u < n; u++) qsort(buf + row[u], (size_t)(row[u + 1] - row[u]), sizeof(col_val), cmp_col_val);
+    for (int64_t e = 0; e < nnz; e++) {
+        out_col[e] = buf[e].c;
+        if (out_val) out_val[e] = buf[e].v;
+    }
+    free(buf);
+    free(cur);
+    free(s.t);
+    free(er);
+    free(ec);
+    free(ev);
+    return swaps;
 }

@@ -129,6 +129,15 @@ void orc_cormat_finish(float *acc, int64_t m1, int nsub, int fisher);
 int64_t orc_dense_csr_r(const float *a, int64_t n, float thr, int32_t *row, int32_t *col, float *val);
 float orc_dense_csr_s(const float *a, int64_t n, int64_t k, int32_t *row, int32_t *col, float *val);
 
+/* Degree-preserving rewiring by double edge swaps (U BFS_MulCPU/Maslov.cpp:
+ * 8-110, W Maslov/Maslov/Maslov.cpp:50-196), sequential as the reference, with
+ * a splitmix64 stream seeded by seed for its rand().  attempts 0 means 2M
+ * (M = nnz/2).  out_col (nnz) gets the rewired columns under the same row
+ * offsets, sorted per row; out_val (may be NULL) each edge's weight.  Returns
+ * the swaps made. */
+int64_t orc_rewire(const int32_t *row, const int32_t *col, const float *val, int64_t n, uint64_t seed,
+                   int64_t attempts, int32_t *out_col, float *out_val);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,8 @@ def lib():
         L.orc_ec.restype = C.c_double
         L.orc_pc.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, i32p, f32p, f32p]
         L.orc_pc.restype = C.c_int32
+        L.orc_rewire.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, C.c_uint64, C.c_int64, i32p, C.c_void_p]
+        L.orc_rewire.restype = C.c_int64
         L.orc_cormat_add.argtypes = [f64p, C.c_int64, C.c_int, C.c_int, f32p]
         L.orc_cormat_finish.argtypes = [f32p, C.c_int64, C.c_int, C.c_int]
         L.orc_dense_csr_r.argtypes = [f32p, C.c_int64, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -269,6 +271,17 @@ def pc(row, col, labels, val=None):
     k = lib().orc_pc(np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32), _ptr(v), n,
                      np.ascontiguousarray(labels, np.int32), p, q)
     return p, q, k
+
+
+def rewire(row, col, val=None, seed=0, attempts=0):
+    """Sequential degree-preserving rewiring: (col, val or None, swaps)."""
+    n = row.size - 1
+    oc = np.empty(col.size, np.int32)
+    v = None if val is None else np.ascontiguousarray(val, np.float32)
+    ov = None if val is None else np.empty(col.size, np.float32)
+    sw = lib().orc_rewire(np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32), _ptr(v), n,
+                          seed & (2**64 - 1), attempts, oc, _ptr(ov))
+    return oc, ov, sw
 
 
 def cormat_average(subjects, fisher=False):

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "pg_internal.hpp"
+#include "pg_rowsort.hpp"
 
 namespace {
 
@@ -1097,78 +1098,6 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_scatter(EdgeSrc s, int nbk
 // Bitonic sort of one row held lane-major in a wave (element e = lane*P + r),
 // M = 64*P keys, ascending; padding keys sort last.  Key type: 32-bit column
 // or 64-bit (column << 32 | value bits) to carry the weight along.
-template <int P, class K>
-__device__ __forceinline__ void wave_bitonic(K (&x)[P], int lane)
-{
-    constexpr int M = 64 * P;
-#pragma unroll
-    for (int k = 2; k <= M; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j < P) {
-#pragma unroll
-                for (int r = 0; r < P; r++) {
-                    if (r & j) continue;
-                    const int r2 = r | j;
-                    const bool asc = k >= P ? (((lane * P) & k) == 0) : ((r & k) == 0);
-                    const K a = x[r], b = x[r2];
-                    const K lo = a < b ? a : b, hi = a < b ? b : a;
-                    x[r] = asc ? lo : hi;
-                    x[r2] = asc ? hi : lo;
-                }
-            } else {
-                const int lj = j / P;
-                const bool lower = (lane & lj) == 0;
-                const bool asc = k == M ? true : ((lane & (k / P)) == 0);
-                const bool keep_min = lower == asc;
-#pragma unroll
-                for (int r = 0; r < P; r++) {
-                    const K y = __shfl_xor(x[r], lj);
-                    x[r] = keep_min ? (x[r] < y ? x[r] : y) : (x[r] < y ? y : x[r]);
-                }
-            }
-        }
-    }
-}
-
-// Sort the d <= 64*P columns (and values) of the row starting at a.
-template <int P>
-__device__ __forceinline__ void sort_row_regs(const int *__restrict__ ctmp, const float *__restrict__ vtmp,
-                                              long long a, int d, int lane, int *__restrict__ col,
-                                              float *__restrict__ val)
-{
-    if (vtmp) {
-        unsigned long long x[P];
-#pragma unroll
-        for (int r = 0; r < P; r++) {
-            const int e = lane * P + r;
-            x[r] = e < d ? ((unsigned long long)(unsigned)ctmp[a + e] << 32) | __float_as_uint(vtmp[a + e]) : ~0ull;
-        }
-        wave_bitonic<P>(x, lane);
-#pragma unroll
-        for (int r = 0; r < P; r++) {
-            const int e = lane * P + r;
-            if (e < d) {
-                col[a + e] = (int)(x[r] >> 32);
-                val[a + e] = __uint_as_float((unsigned)x[r]);
-            }
-        }
-    } else {
-        int x[P];
-#pragma unroll
-        for (int r = 0; r < P; r++) {
-            const int e = lane * P + r;
-            x[r] = e < d ? ctmp[a + e] : INT_MAX;
-        }
-        wave_bitonic<P>(x, lane);
-#pragma unroll
-        for (int r = 0; r < P; r++) {
-            const int e = lane * P + r;
-            if (e < d) col[a + e] = x[r];
-        }
-    }
-}
-
 constexpr int ROWS_THREADS = 1024;
 constexpr int SORT_MAX = 64 * 16;  // longest row sorted in registers
 
@@ -1222,11 +1151,11 @@ __global__ __launch_bounds__(ROWS_THREADS) void bucket_rows(const long long *__r
         const long long a = e0 + rb[lr];
         const int d = (int)(rb[lr + 1] - rb[lr]);
         if (d == 0 || d > SORT_MAX) continue;
-        if (d <= 64) sort_row_regs<1>(ctmp, vtmp, a, d, lane, col, val);
-        else if (d <= 128) sort_row_regs<2>(ctmp, vtmp, a, d, lane, col, val);
-        else if (d <= 256) sort_row_regs<4>(ctmp, vtmp, a, d, lane, col, val);
-        else if (d <= 512) sort_row_regs<8>(ctmp, vtmp, a, d, lane, col, val);
-        else sort_row_regs<16>(ctmp, vtmp, a, d, lane, col, val);
+        if (d <= 64) pgdev::sort_row_regs<1>(ctmp, vtmp, a, d, lane, col, val);
+        else if (d <= 128) pgdev::sort_row_regs<2>(ctmp, vtmp, a, d, lane, col, val);
+        else if (d <= 256) pgdev::sort_row_regs<4>(ctmp, vtmp, a, d, lane, col, val);
+        else if (d <= 512) pgdev::sort_row_regs<8>(ctmp, vtmp, a, d, lane, col, val);
+        else pgdev::sort_row_regs<16>(ctmp, vtmp, a, d, lane, col, val);
     }
     // longer rows: waves wv < nbm, one LDS bitmap over [0, n) each
     if (wv >= nbm || rb[BROWS] <= (unsigned)SORT_MAX) return;

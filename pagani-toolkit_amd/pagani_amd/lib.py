@@ -89,6 +89,7 @@ _SIGS = {
     "pg_bfs_efficiency": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP]),
     "pg_eigenvector_centrality": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP]),
     "pg_participation": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP]),
+    "pg_rewire": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_uint64, C.c_int64, _VP, _VP, _VP]),
     "pg_cormat_begin": (C.c_int, [_VP, C.c_int32, C.c_int32]),
     "pg_cormat_add": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32]),
     "pg_cormat_finish": (C.c_int, [_VP]),
@@ -107,6 +108,14 @@ def library():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} not built; run python pagani-toolkit_amd/build.py")
+        # PyTorch ships its own HIP runtime with the same SONAME
+        # (libamdhip64.so.7) but asks for it by the file name libamdhip64.so.
+        # Loaded first, it also serves libpagani; loaded after libpagani has
+        # pulled in ROCm's copy, it becomes a second runtime that finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(lib, name)
@@ -299,6 +308,19 @@ class Context:
         self._check(self._lib.pg_participation(self._h, _ptr(row), _ptr(col), _ptr(val), n, _len(col),
                                                 _ptr(labels), _ptr(p), _ptr(q), C.cast(C.pointer(k), _VP)))
         return p, q, k.value
+
+    def rewire(self, row, col, val=None, seed: int = 0, attempts: int = 0, out_col=None, out_val=None):
+        """Degree-preserving random rewiring: (col, val or None, swaps made).
+
+        The row offsets are unchanged; out_col / out_val may be device tensors."""
+        nnz = _len(col)
+        oc = np.empty(nnz, np.int32) if out_col is None else out_col
+        ov = None if val is None else (np.empty(nnz, np.float32) if out_val is None else out_val)
+        sw = C.c_int64(0)
+        self._check(self._lib.pg_rewire(self._h, _ptr(row), _ptr(col), _ptr(val), _len(row) - 1, nnz,
+                                        seed & (2**64 - 1), attempts, _ptr(oc), _ptr(ov),
+                                        C.cast(C.pointer(sw), _VP)))
+        return oc, ov, sw.value
 
     def fw(self, d, block: int = 0):
         """Closure in place; d is an n x n float32 array/tensor."""

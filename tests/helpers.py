@@ -53,3 +53,30 @@ def cliques(sizes, bridge=True):
         for a, b in zip(starts[:-1], starts[1:]):
             edges.append((a, b))
     return csr_from_edges(base, edges)
+
+
+def ring_lattice(n, k):
+    """Each vertex joined to its k/2 nearest neighbours on either side."""
+    return csr_from_edges(n, [(i, (i + d) % n) for i in range(n) for d in range(1, k // 2 + 1)])
+
+
+def check_rewired(row, col_in, val_in, col, val):
+    """Invariants of a degree-preserving rewiring under the same row offsets:
+    columns in range, strictly ascending per row, no self-loops, a symmetric
+    edge set, each edge's weight equal in both directions and the multiset of
+    edge weights unchanged."""
+    n = row.size - 1
+    assert col.shape == col_in.shape
+    r = np.repeat(np.arange(n, dtype=np.int64), np.diff(row))
+    c = col.astype(np.int64)
+    assert ((c >= 0) & (c < n)).all()
+    assert (c != r).all(), "self-loop"
+    same = r[1:] == r[:-1]
+    assert (np.diff(c)[same] > 0).all(), "row not strictly ascending"
+    fwd, bwd = r * n + c, c * n + r
+    o1, o2 = np.argsort(fwd), np.argsort(bwd)
+    assert np.array_equal(fwd[o1], bwd[o2]), "edge set not symmetric"
+    if val is not None:
+        assert np.array_equal(val[o1].view(np.uint32), val[o2].view(np.uint32)), "weights not symmetric"
+        up_in = col_in.astype(np.int64) > r
+        assert np.array_equal(np.sort(val[c > r]), np.sort(val_in[up_in])), "weights changed"

@@ -2,7 +2,8 @@
 import numpy as np
 import pytest
 
-from helpers import cliques, complete_graph, path_graph, random_graph, star_graph
+from helpers import (check_rewired, cliques, complete_graph, csr_from_edges, path_graph, random_graph,
+                     ring_lattice, star_graph)
 from oracle import oracle as O
 
 
@@ -386,3 +387,54 @@ def test_dense_thresholds_brute_force():
         full[np.triu_indices(n, 1)] = a
         full += full.T
         np.testing.assert_array_equal(val, full[np.repeat(np.arange(n), np.diff(row)), col])
+
+
+# ------------------------------------------------------------------ rewiring
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_rewire_invariants(weighted):
+    row, col, val = random_graph(300, 0.05, 3, weighted=weighted)
+    v = val if weighted else None
+    oc, ov, sw = O.rewire(row, col, v, seed=7)
+    check_rewired(row, col, v, oc, ov)
+    m = col.size // 2
+    assert 0.5 * m < sw <= 2 * m  # sparse: most of the 2M attempts succeed
+    assert not np.array_equal(oc, col)
+    oc2, ov2, sw2 = O.rewire(row, col, v, seed=7)
+    assert sw2 == sw and np.array_equal(oc2, oc)
+
+
+def test_rewire_no_swap_possible():
+    # every pair of star edges shares the centre; in a complete graph every new
+    # edge exists already
+    for row, col, _ in (star_graph(9), complete_graph(7)):
+        oc, _, sw = O.rewire(row, col, None, seed=1)
+        assert sw == 0
+        np.testing.assert_array_equal(oc, col)
+
+
+def test_rewire_two_edges_reach_every_matching():
+    # {0-1, 2-3}: each swap moves to one of the two other perfect matchings,
+    # the coin choosing which; all three must turn up
+    row, col, _ = csr_from_edges(4, [(0, 1), (2, 3)])
+    seen = {}
+    for s in range(300):
+        oc, _, _ = O.rewire(row, col, None, seed=s)
+        seen[tuple(oc.tolist())] = seen.get(tuple(oc.tolist()), 0) + 1
+    assert len(seen) == 3 and min(seen.values()) > 50, seen
+
+
+def test_rewire_clustering_matches_networkx():
+    nx = pytest.importorskip("networkx")
+    n, k = 200, 10
+    row, col, _ = ring_lattice(n, k)
+    assert abs(O.cp_binary(row, col)[1] - 2 / 3) < 1e-7  # 3(k-2) / (4(k-1)), stored float
+    ours, theirs = [], []
+    for s in range(8):
+        oc, _, sw = O.rewire(row, col, None, seed=s)
+        ours.append(O.cp_binary(row, oc)[1])
+        g = nx.watts_strogatz_graph(n, k, 0.0)
+        nx.double_edge_swap(g, nswap=sw, max_tries=100 * sw, seed=s)
+        theirs.append(nx.average_clustering(g))
+    assert np.mean(ours) < 0.15
+    assert abs(np.mean(ours) - np.mean(theirs)) < 0.015, (np.mean(ours), np.mean(theirs))

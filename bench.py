@@ -105,7 +105,8 @@ def _device_csr(ctx, n, nnz, weighted=False):
 
 def run_metrics(a):
     """Graph metrics on the N=58,000 1%-density graph: exact BC from every
-    source, degree, binary clustering; Floyd-Warshall on the weighted graph of
+    source, degree, binary clustering, efficiency, eigenvector centrality,
+    participation, one degree-preserving rewiring; Floyd-Warshall on the weighted graph of
     the first 8,192 voxels (w = 1 - r for r above the 1% threshold)."""
     import torch
     import pagani_amd
@@ -157,6 +158,13 @@ def run_metrics(a):
     (_, _, k), dt = _timed(ctx, lambda: ctx.participation(row, col, lab))
     out["participation_ms"] = dt * 1e3
     log(f"participation coefficient, 64 modules: {dt * 1e3:.2f} ms")
+    # one random reference network: 2M degree-preserving swap attempts
+    rc = torch.empty_like(col)
+    ctx.rewire(row, col, seed=1, out_col=rc)
+    (_, _, sw), dt = _timed(ctx, lambda: ctx.rewire(row, col, seed=2, out_col=rc))
+    m = nnz // 2
+    out["rewire"] = {"seconds": dt, "attempts": 2 * m, "swaps": sw, "attempts_per_s": 2 * m / dt}
+    log(f"rewire: {2 * m} attempts, {sw} swaps in {dt * 1e3:.1f} ms")
 
     # Floyd-Warshall on the first 8192 voxels: weighted graph r > t*, d = 1 - r
     nf = 8192

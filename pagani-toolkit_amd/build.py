@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-TOOLS = ["CUCorMat", "CUBC", "Cp", "Degree", "CUBFW_Lp", "CUBFS_Lp", "CUmodularity", "CUEC", "PC_CPU", "ConvertNII"]
+TOOLS = ["CUCorMat", "CUBC", "Cp", "Degree", "CUBFW_Lp", "CUBFS_Lp", "CUmodularity", "CUEC", "PC_CPU", "ConvertNII", "Maslov"]
 
 
 def run(cmd):

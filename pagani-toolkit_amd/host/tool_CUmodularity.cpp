@@ -6,7 +6,11 @@
 //   CUmodularity [--strength] dir_for_csr num_of_random_networks
 // Default degrees are binary with weighted adjacency, as the reference tool;
 // --strength uses strengths and the total weight instead.
-// num_of_random_networks must be 0 (no Maslov rewiring in this build).
+// With num_of_random_networks > 0 each network's degree-preserving rewirings
+// (pg_rewire, weights carried with their edges; network l seeded with
+// PAGANI_SEED + l, wall clock when unset) are partitioned too, as the
+// reference does (modularity_GPU.cu:665-684); their module counts and Q are
+// printed and appended to _modu.txt (the reference only prints them).
 #include <iomanip>
 #include <iostream>
 #include <set>
@@ -27,10 +31,13 @@ int main(int argc, char **argv)
         return 1;
     }
     const std::string dir = pgtool::dir_arg(argv[a0]);
-    if (std::atoi(argv[a0 + 1]) != 0) {
-        std::cerr << "random-network comparison (num_of_random_networks > 0) is not available\n";
+    const int nrand = std::atoi(argv[a0 + 1]);
+    if (nrand < 0) {
+        std::cerr << "num_of_random_networks must be >= 0\n";
         return 1;
     }
+    const uint64_t seed = pgtool::random_seed();
+    if (nrand > 0) std::cout << "random network seed " << seed << std::endl;
     const auto files = pgio::list_dir(dir, ".csr");
     std::cout << files.size() << " files to be processed." << std::endl;
     pgtool::Ctx ctx;
@@ -46,23 +53,39 @@ int main(int argc, char **argv)
             return 2;
         }
         std::vector<int32_t> lab(g.n);
-        double q = 0.0;
         int32_t rounds = 0;
+        // modules of a network into lab; returns Q
+        auto partition = [&](const pgio::Csr &h, int32_t *rnd) {
+            double q = 0.0;
+            ctx.check(pg_modularity(ctx.c, h.row.data(), h.col.data(), weighted ? h.val.data() : nullptr, h.n,
+                                    (int64_t)h.col.size(), kmode, lab.data(), &q, nullptr, nullptr, rnd),
+                      "pg_modularity");
+            return q;
+        };
         pgtool::Timer tm;
-        ctx.check(pg_modularity(ctx.c, g.row.data(), g.col.data(), weighted ? g.val.data() : nullptr, g.n,
-                                (int64_t)g.col.size(), kmode, lab.data(), &q, nullptr, nullptr, &rounds),
-                  "pg_modularity");
+        const double q = partition(g, &rounds);
         const double secs = tm.seconds();
         const std::set<int32_t> mods(lab.begin(), lab.end());
         std::cout << "Number of Modules: " << mods.size() << ",\tQ=" << q << "\nElapsed time:   " << secs << "s"
                   << std::endl;
-        {
-            std::ofstream fo(pgio::strip_ext(a) + "_modu.txt");
-            fo << "Number of Modules: " << mods.size() << ",\tQ=" << q << "\nBisection rounds: " << rounds
-               << "\nElapsed time:   " << secs << "s" << std::endl;
-        }
         std::vector<float> out(g.n);
         for (int i = 0; i < g.n; i++) out[i] = (float)lab[i];
+        std::ofstream fo(pgio::strip_ext(a) + "_modu.txt");
+        fo << "Number of Modules: " << mods.size() << ",\tQ=" << q << "\nBisection rounds: " << rounds
+           << "\nElapsed time:   " << secs << "s" << std::endl;
+        if (nrand > 0) {
+            std::cout << "Modular analysis for random networks..." << std::endl;
+            pgtool::Timer tr;
+            for (int l = 0; l < nrand; l++) {
+                const double rq = partition(pgtool::rewired(ctx, g, seed + (uint64_t)l), nullptr);
+                const std::set<int32_t> rm(lab.begin(), lab.end());
+                std::cout << "\nNumber of Modules: " << rm.size() << ",\tQ=" << rq << std::endl;
+                fo << "Random network " << l + 1 << ": Number of Modules: " << rm.size() << ",\tQ=" << rq
+                   << std::endl;
+            }
+            pgio::time_log(std::string("Modularity\tRandomGPU\t(Maslov+kernel) time\t") +
+                           std::to_string(tr.seconds()) + "s");
+        }
         const std::string x = pgio::strip_ext(a) + "_modu.nm";
         std::cout << "Save partition results as " << x << std::endl;
         if (!pgio::write_nm(x, out.data(), g.n)) return 2;

@@ -4,10 +4,12 @@
 //   Cp dir_for_csr num_of_random_networks {g|n|k|gn|gk|nk|gnk}
 //   Cp --weighted dir_for_csr num_of_random_networks [1|2] {g|n|...}
 //     1 = Barrat, 2 = Onnela (default, weights divided by the largest weight)
-// Outputs: <file>_cp.nm (nodal, 'n'), <file>_cp.txt (network mean, 'g').
-// Random (Maslov-rewired) reference networks are not generated: the rewiring
-// is seeded by wall-clock time in the reference and is outside this build, so
-// num_of_random_networks must be 0 and 'k' (gamma) is rejected.
+// Outputs: <file>_cp.nm (nodal, 'n'), <file>_cp.txt (the network's mean Cp,
+// then one line per random network, 'g'), <file>_gamma.txt (mean Cp over the
+// mean of the random networks' mean Cp, 'k').  Random networks are degree-
+// preserving rewirings (pg_rewire) of the network, weights travelling with
+// their edge (U Cp_main.cpp:150-195, W Cp_main.cpp:172-213); network l is
+// seeded with PAGANI_SEED + l (wall clock when unset).
 #include <iomanip>
 #include <iostream>
 
@@ -42,10 +44,14 @@ int main(int argc, char **argv)
     const std::string sel = argv[q];
     const bool global = sel.find('g') != std::string::npos;
     const bool nodal = sel.find('n') != std::string::npos;
-    if (sel.find('k') != std::string::npos || nrand != 0) {
-        std::cerr << "random-network comparison (gamma, num_of_random_networks > 0) is not available\n";
+    const bool gamma = sel.find('k') != std::string::npos;
+    if (nrand < 0) {
+        std::cerr << "num_of_random_networks must be >= 0\n";
         return 1;
     }
+    if (gamma && nrand == 0) std::cerr << "gamma needs random networks (num_of_random_networks > 0): writing nan\n";
+    const uint64_t seed = pgtool::random_seed();
+    if (nrand > 0) std::cout << "random network seed " << seed << std::endl;
     const auto files = pgio::list_dir(dir, ".csr");
     std::cout << files.size() << " files to be processed." << std::endl;
     pgtool::Ctx ctx;
@@ -60,18 +66,24 @@ int main(int argc, char **argv)
             return 2;
         }
         std::vector<float> out(g.n);
-        double mean = 0.0;
+        // mean Cp of a network; the per-node values into out
+        auto mean_cp = [&](const pgio::Csr &h) {
+            double mean = 0.0;
+            if (weighted) {
+                std::vector<double> cp(h.n);
+                ctx.check(pg_cp_weighted(ctx.c, h.row.data(), h.col.data(), h.val.data(), h.n,
+                                         (int64_t)h.col.size(), kind, 0, cp.data(), &mean),
+                          "pg_cp_weighted");
+                for (int i = 0; i < h.n; i++) out[i] = (float)cp[i];
+            } else {
+                ctx.check(pg_cp_binary(ctx.c, h.row.data(), h.col.data(), h.n, (int64_t)h.col.size(), out.data(),
+                                       &mean),
+                          "pg_cp_binary");
+            }
+            return mean;
+        };
         pgtool::Timer tm;
-        if (weighted) {
-            std::vector<double> cp(g.n);
-            ctx.check(pg_cp_weighted(ctx.c, g.row.data(), g.col.data(), g.val.data(), g.n, (int64_t)g.col.size(),
-                                     kind, 0, cp.data(), &mean),
-                      "pg_cp_weighted");
-            for (int i = 0; i < g.n; i++) out[i] = (float)cp[i];
-        } else {
-            ctx.check(pg_cp_binary(ctx.c, g.row.data(), g.col.data(), g.n, (int64_t)g.col.size(), out.data(), &mean),
-                      "pg_cp_binary");
-        }
+        const double mean = mean_cp(g);
         const double secs = tm.seconds();
         std::cout << "average Cp:\t" << std::setprecision(6) << mean << std::endl;
         std::cout << "Elapsed time: " << secs << " s." << std::endl;
@@ -81,9 +93,29 @@ int main(int argc, char **argv)
             std::cout << "Save Clustering Coefficient for each node as " << x << std::endl;
             if (!pgio::write_nm(x, out.data(), g.n)) return 2;
         }
+        std::vector<double> means{mean};
+        if (nrand > 0) {
+            std::cout << "Calculating Cp for random networks..." << std::endl;
+            pgtool::Timer tr;
+            for (int l = 0; l < nrand; l++) means.push_back(mean_cp(pgtool::rewired(ctx, g, seed + (uint64_t)l)));
+            const double rs = tr.seconds();
+            std::cout << "Elapsed time: " << rs << " s." << std::endl;
+            pgio::time_log(std::string("Cp\tRandomGPU\t(Maslov+kernel) time\t") + std::to_string(rs) + "s");
+        }
+        double rmean = 0.0;
+        for (int l = 1; l <= nrand; l++) rmean += means[l];
+        rmean /= nrand;  // nan when there are no random networks, as the reference
         if (global) {
-            std::ofstream fo(pgio::strip_ext(a) + "_cp.txt");
-            fo << std::setprecision(6) << mean << std::endl;
+            const std::string x = pgio::strip_ext(a) + "_cp.txt";
+            std::cout << "Save average Cp for the brain network and random networks as " << x << std::endl;
+            std::ofstream fo(x);
+            for (double v : means) fo << std::setprecision(6) << v << std::endl;
+        }
+        if (gamma) {
+            const std::string x = pgio::strip_ext(a) + "_gamma.txt";
+            std::cout << "Save corresponding gamma for the brain network and random networks as " << x << std::endl;
+            std::ofstream fo(x);
+            fo << std::setprecision(6) << mean / rmean << std::endl;
         }
     }
     std::cout << "total elapsed time: " << total.seconds() << " s." << std::endl;

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <fstream>
 #include <string>
 #include <vector>
@@ -55,6 +56,32 @@ inline bool csr_has_values(const std::string &path)
     f.seekg(4 + 4LL * rlen);
     f.read(reinterpret_cast<char *>(&clen), 4);
     return size > 8 + 4LL * rlen + 4LL * clen;
+}
+
+// Seed of the random reference networks: PAGANI_SEED when set, else the wall
+// clock (the reference calls srand(time(0)) inside every rewiring, so its
+// networks made within one second repeat; here network l uses seed + l).
+inline uint64_t random_seed()
+{
+    const char *e = std::getenv("PAGANI_SEED");
+    if (e && *e) return std::strtoull(e, nullptr, 10);
+    return (uint64_t)std::time(nullptr);
+}
+
+// Random reference network of g (Maslov / Maslov_weighted_1 of the reference):
+// degree-preserving rewiring with 2M attempts; weights travel with their edge.
+inline pgio::Csr rewired(const Ctx &ctx, const pgio::Csr &g, uint64_t seed)
+{
+    pgio::Csr r;
+    r.n = g.n;
+    r.row = g.row;
+    r.col.resize(g.col.size());
+    if (!g.val.empty()) r.val.resize(g.val.size());
+    ctx.check(pg_rewire(ctx.c, g.row.data(), g.col.data(), g.val.empty() ? nullptr : g.val.data(), g.n,
+                        (int64_t)g.col.size(), seed, 0, r.col.data(), r.val.empty() ? nullptr : r.val.data(),
+                        nullptr),
+              "pg_rewire");
+    return r;
 }
 
 inline std::string dir_arg(const char *a)

@@ -50,14 +50,16 @@ def test_csr_name_matches_reference_printf():
 
 @pytest.mark.parametrize("tool,args", [
     ("CUCorMat", []), ("CUBC", []), ("Degree", []), ("Cp", ["x"]), ("CUBFW_Lp", ["x", "0"]),
-    ("CUmodularity", ["x"]), ("CUBFS_Lp", ["x", "0"]), ("CUBFS_Lp", ["/nonexistent", "2", "b"]),
+    ("CUmodularity", ["x"]), ("CUBFS_Lp", ["x", "0"]), ("CUBFS_Lp", ["/nonexistent", "-2", "b"]),
     ("ConvertNII", ["x"]), ("ConvertNII", ["x", "y", "0.5", "z"]), ("CUEC", []), ("PC_CPU", []),
     ("PC_CPU", ["/nonexistent"]),
     ("CUCorMat", ["/nonexistent", "0.5", "y", "n", "r", "0.3"]),   # averaging needs f or n
     ("CUCorMat", ["/nonexistent", "0.5", "q", "n", "r", "0.3"]),   # bad averaging mode
     ("CUCorMat", ["/nonexistent", "0.5", "n", "q", "r", "0.3"]),   # bad cormat flag
     ("CUCorMat", ["/nonexistent", "0.5", "n", "n", "q", "0.3"]),   # bad threshold type
-    ("Cp", ["/nonexistent", "3", "gk"]),                             # random networks not offered
+    ("Cp", ["/nonexistent", "-3", "gk"]),                            # negative random-network count
+    ("CUBFW_Lp", ["/nonexistent", "-1", "gl"]), ("CUmodularity", ["/nonexistent", "-1"]),
+    ("Maslov", []), ("Maslov", ["net.txt"]), ("Maslov", ["a.csr", "b.csr"]),
 ])
 def test_tools_reject_bad_arguments(tool, args):
     exe = os.path.join(BIN, tool)

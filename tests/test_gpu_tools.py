@@ -18,10 +18,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "pagani-toolkit_amd", "bin")
 
 
-def run(tool, *args, cwd=None):
+def run(tool, *args, cwd=None, env=None):
     say(f"run {tool} {' '.join(map(str, args))}")
     r = subprocess.run([os.path.join(BIN, tool), *map(str, args)], capture_output=True, text=True,
-                       timeout=600, cwd=cwd)
+                       timeout=600, cwd=cwd, env=None if env is None else {**os.environ, **env})
     assert r.returncode == 0, (tool, r.returncode, r.stdout[-2000:], r.stderr[-2000:])
     return r.stdout
 
@@ -216,3 +216,68 @@ def test_cucormat_average_and_cormat(volume, tmp_path):
     np.testing.assert_array_equal(row, orow)
     np.testing.assert_array_equal(col, ocol)
     np.testing.assert_array_equal(val.view(np.uint32), oval.view(np.uint32))
+
+
+def _lines(path):
+    return [float(x) for x in open(path).read().split()]
+
+
+def test_random_network_outputs(volume, tmp_path, ctx):
+    # Random networks: network l is the library's rewiring with seed
+    # PAGANI_SEED + l, so every line the tools write can be recomputed here.
+    d, subs, n = volume
+    _ensure_networks(d)
+    env = {"PAGANI_SEED": "11"}
+    b = tmp_path / "bin"
+    b.mkdir()
+    f = _csr_files(d)[0]
+    (b / f).write_bytes((d / f).read_bytes())
+    base = str(b / f)[:-4]
+    row, col, _ = formats.read_csr(str(b / f))
+    rnd = [ctx.rewire(row, col, seed=11 + l)[0] for l in range(3)]
+    for oc in rnd:
+        assert O.cp_binary(row, oc)[1] < O.cp_binary(row, col)[1]
+    run("Cp", b, 3, "gk", env=env)
+    cps = _lines(base + "_cp.txt")
+    want = [O.cp_binary(row, col)[1]] + [O.cp_binary(row, oc)[1] for oc in rnd]
+    np.testing.assert_allclose(cps, want, rtol=5e-6)
+    assert abs(_lines(base + "_gamma.txt")[0] / (want[0] / np.mean(want[1:])) - 1) < 5e-6
+    run("CUBFS_Lp", b, 2, "bl", env=env)
+    lps = _lines(base + "_Lp.txt")
+    want = [O.bfs_efficiency(row, col)[1]] + [O.bfs_efficiency(row, oc)[1] for oc in rnd[:2]]
+    np.testing.assert_allclose(lps, want, rtol=5e-6)
+    assert abs(_lines(base + "_lambda.txt")[0] / (want[0] / np.mean(want[1:])) - 1) < 5e-6
+    run("Maslov", b / f, env=env)
+    r2, c2, v2 = formats.read_csr(base + "_Maslov.csr")
+    assert v2 is None
+    np.testing.assert_array_equal(r2, row)
+    np.testing.assert_array_equal(c2, rnd[0])
+
+    w = tmp_path / "w"
+    w.mkdir()
+    f = _csr_files(d / "weighted")[0]
+    (w / f).write_bytes((d / "weighted" / f).read_bytes())
+    base = str(w / f)[:-4]
+    row, col, val = formats.read_csr(str(w / f))
+    oc, ov, _ = ctx.rewire(row, col, val, seed=11)
+    run("CUBFW_Lp", w, 1, "gl", env=env)
+    lps = _lines(base + "_Lp.txt")
+    want = []
+    for c, v in ((col, val), (oc, ov)):
+        dist = O.fw_blocked(O.fw_init(row, c, v, 0), 64)
+        inv = np.where((dist >= 0.1) & ~np.eye(n, dtype=bool), 1.0 / dist.astype(np.float64), 0.0)
+        want.append(1.0 / (inv.sum() / n / (n - 1)))
+    np.testing.assert_allclose(lps, want, rtol=1e-5)
+    assert abs(_lines(base + "_lambda.txt")[0] / (want[0] / want[1]) - 1) < 2e-5
+    run("Cp", "--weighted", w, 1, 2, "gk", env=env)
+    want = [O.cp_weighted(row, col, val, 2, 0)[1], O.cp_weighted(row, oc, ov, 2, 0)[1]]
+    np.testing.assert_allclose(_lines(base + "_cp.txt"), want, rtol=1e-5)
+    run("CUmodularity", w, 1, env=env)
+    txt = open(base + "_modu.txt").read()
+    q = float(txt.split("Random network 1:")[1].split("Q=")[1].split()[0])
+    assert 0.0 < q < float(txt.split("Q=")[1].split()[0])  # a rewired network is less modular
+    run("Maslov", w / f, env=env)
+    r2, c2, v2 = formats.read_csr(base + "_Maslov.csr")
+    np.testing.assert_array_equal(r2, row)
+    np.testing.assert_array_equal(c2, oc)
+    np.testing.assert_array_equal(v2.view(np.uint32), ov.view(np.uint32))

@@ -432,9 +432,12 @@ extern "C" pg_status pg_rewire(pg_ctx *ctx, const int32_t *row, const int32_t *c
     PG_HIP(ctx, hipStreamSynchronize(st));
     if (fl & 4) return pg_fail(ctx, PG_EFORMAT, "network is not symmetric (an entry or its weight has no mirror)");
 
-    // rounds of up to m/2 disjoint attempts (the reference makes 2M attempts)
+    // equal rounds of at most m/2 disjoint attempts (the reference makes 2M
+    // attempts); each round costs a hash rebuild, so no short tail round
     const long long total = attempts == 0 ? 2LL * m : attempts;
-    const long long per_round = m / 2;
+    const long long max_round = m / 2;
+    const long long rounds = max_round > 0 ? (total + max_round - 1) / max_round : 0;
+    const long long per_round = rounds > 0 ? (total + rounds - 1) / rounds : 0;
     unsigned long long *dswaps;
     PG_TRY(pg_bufT(ctx, "rw.swaps", 1, &dswaps));
     PG_HIP(ctx, hipMemsetAsync(dswaps, 0, 8, st));

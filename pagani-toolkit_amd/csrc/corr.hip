@@ -992,6 +992,7 @@ constexpr int BROWS = 1 << BSH;
 constexpr unsigned COLMASK = (1u << 25) - 1u;
 
 constexpr int SEG_THREADS = 1024;  // one work-group per segment: 16 waves in flight
+constexpr int SEG_UNROLL = 4;      // segment entries per thread in flight
 
 __global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, unsigned *__restrict__ cnt)
 {
@@ -1001,11 +1002,23 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, 
     const int g = blockIdx.x;
     const long long c = src_count(s, g);
     const float ka = src_bound(s);
-    for (long long k = threadIdx.x; k < c; k += SEG_THREADS) {
-        if (!src_keep(s, g, k, ka)) continue;
-        const int2 e = src_edge(s, g, k);
-        atomicAdd(&sc[e.x >> BSH], 1u);
-        atomicAdd(&sc[e.y >> BSH], 1u);
+    // SEG_UNROLL entries per thread per pass: every load is issued before the
+    // first LDS atomic that needs one
+    for (long long k0 = threadIdx.x; k0 < c; k0 += (long long)SEG_THREADS * SEG_UNROLL) {
+        int2 e[SEG_UNROLL];
+        bool keep[SEG_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SEG_UNROLL; u++) {
+            const long long k = k0 + (long long)u * SEG_THREADS;
+            keep[u] = k < c && src_keep(s, g, k, ka);
+            e[u] = k < c ? src_edge(s, g, k) : make_int2(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < SEG_UNROLL; u++)
+            if (keep[u]) {
+                atomicAdd(&sc[e[u].x >> BSH], 1u);
+                atomicAdd(&sc[e[u].y >> BSH], 1u);
+            }
     }
     __syncthreads();
     for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) cnt[(long long)b * gridDim.x + g] = sc[b];
@@ -1072,25 +1085,40 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_scatter(EdgeSrc s, int nbk
                                                               unsigned *__restrict__ ent, float *__restrict__ entv,
                                                               const double *__restrict__ z, int t, long long cap)
 {
-    extern __shared__ unsigned sc[];  // per-bucket cursors
-    for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) sc[b] = 0u;
-    __syncthreads();
+    // per-bucket: this segment's output base (bstart + off) and a cursor
+    extern __shared__ long long sbase[];
+    unsigned *sc = (unsigned *)(sbase + nbk);
     const int g = blockIdx.x, nsrc = gridDim.x;
+    for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) {
+        sbase[b] = bstart[b] + off[(long long)b * nsrc + g];
+        sc[b] = 0u;
+    }
+    __syncthreads();
     const long long c = src_count(s, g);
     const float ka = src_bound(s);
     if (bstart[nbk] > cap) return;
-    for (long long k = threadIdx.x; k < c; k += SEG_THREADS) {
-        if (!src_keep(s, g, k, ka)) continue;
-        const int2 e = src_edge(s, g, k);
-        const int bf = e.x >> BSH, br = e.y >> BSH;
-        const long long pf = bstart[bf] + off[(long long)bf * nsrc + g] + atomicAdd(&sc[bf], 1u);
-        const long long pr = bstart[br] + off[(long long)br * nsrc + g] + atomicAdd(&sc[br], 1u);
-        ent[pf] = ((unsigned)(e.x & (BROWS - 1)) << 25) | (unsigned)e.y;
-        ent[pr] = ((unsigned)(e.y & (BROWS - 1)) << 25) | (unsigned)e.x;
-        if (entv) {
-            const float r = (float)r64_pair(z, t, e.x, e.y);
-            entv[pf] = r;
-            entv[pr] = r;
+    for (long long k0 = threadIdx.x; k0 < c; k0 += (long long)SEG_THREADS * SEG_UNROLL) {
+        int2 e[SEG_UNROLL];
+        bool keep[SEG_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SEG_UNROLL; u++) {
+            const long long k = k0 + (long long)u * SEG_THREADS;
+            keep[u] = k < c && src_keep(s, g, k, ka);
+            e[u] = k < c ? src_edge(s, g, k) : make_int2(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < SEG_UNROLL; u++) {
+            if (!keep[u]) continue;
+            const int bf = e[u].x >> BSH, br = e[u].y >> BSH;
+            const long long pf = sbase[bf] + atomicAdd(&sc[bf], 1u);
+            const long long pr = sbase[br] + atomicAdd(&sc[br], 1u);
+            ent[pf] = ((unsigned)(e[u].x & (BROWS - 1)) << 25) | (unsigned)e[u].y;
+            ent[pr] = ((unsigned)(e[u].y & (BROWS - 1)) << 25) | (unsigned)e[u].x;
+            if (entv) {
+                const float r = (float)r64_pair(z, t, e[u].x, e[u].y);
+                entv[pf] = r;
+                entv[pr] = r;
+            }
         }
     }
 }
@@ -1099,6 +1127,7 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_scatter(EdgeSrc s, int nbk
 // M = 64*P keys, ascending; padding keys sort last.  Key type: 32-bit column
 // or 64-bit (column << 32 | value bits) to carry the weight along.
 constexpr int ROWS_THREADS = 1024;
+constexpr int ROWS_UNROLL = 8;  // bucket entries per thread in flight
 constexpr int SORT_MAX = 64 * 16;  // longest row sorted in registers
 
 // One work-group per bucket: row counts -> row offsets, scatter into the
@@ -1122,7 +1151,17 @@ __global__ __launch_bounds__(ROWS_THREADS) void bucket_rows(const long long *__r
     const long long e0 = bstart[b], e1 = bstart[b + 1];
     if (tid < BROWS) rc[tid] = 0u;
     __syncthreads();
-    for (long long k = e0 + tid; k < e1; k += ROWS_THREADS) atomicAdd(&rc[ent[k] >> 25], 1u);
+    for (long long k0 = e0 + tid; k0 < e1; k0 += (long long)ROWS_THREADS * ROWS_UNROLL) {
+        unsigned x[ROWS_UNROLL];
+#pragma unroll
+        for (int u = 0; u < ROWS_UNROLL; u++) {
+            const long long k = k0 + (long long)u * ROWS_THREADS;
+            x[u] = k < e1 ? ent[k] : ~0u;
+        }
+#pragma unroll
+        for (int u = 0; u < ROWS_UNROLL; u++)
+            if (x[u] != ~0u) atomicAdd(&rc[x[u] >> 25], 1u);
+    }
     __syncthreads();
     if (tid == 0) {
         unsigned run = 0;
@@ -1138,12 +1177,23 @@ __global__ __launch_bounds__(ROWS_THREADS) void bucket_rows(const long long *__r
     if (b == nbk - 1 && tid == 0) row[n] = (int)e1;
     if (tid < BROWS) rc[tid] = 0u;
     __syncthreads();
-    for (long long k = e0 + tid; k < e1; k += ROWS_THREADS) {
-        const unsigned x = ent[k];
-        const unsigned lr = x >> 25;
-        const long long p = e0 + rb[lr] + atomicAdd(&rc[lr], 1u);
-        ctmp[p] = (int)(x & COLMASK);
-        if (vtmp) vtmp[p] = entv[k];
+    for (long long k0 = e0 + tid; k0 < e1; k0 += (long long)ROWS_THREADS * ROWS_UNROLL) {
+        unsigned x[ROWS_UNROLL];
+        float v[ROWS_UNROLL];
+#pragma unroll
+        for (int u = 0; u < ROWS_UNROLL; u++) {
+            const long long k = k0 + (long long)u * ROWS_THREADS;
+            x[u] = k < e1 ? ent[k] : ~0u;
+            v[u] = (vtmp && k < e1) ? entv[k] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < ROWS_UNROLL; u++) {
+            if (x[u] == ~0u) continue;
+            const unsigned lr = x[u] >> 25;
+            const long long p = e0 + rb[lr] + atomicAdd(&rc[lr], 1u);
+            ctmp[p] = (int)(x[u] & COLMASK);
+            if (vtmp) vtmp[p] = v[u];
+        }
     }
     __syncthreads();
     // rows that fit in registers: wave wv takes rows wv, wv + NWAVE, ...
@@ -1983,7 +2033,11 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             bucket_count<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(src, nbk, bcnt);
             bucket_scan_local<<<(unsigned)nbk, 256, 0, st>>>(bcnt, nsrc, boff, btot);
             bucket_scan_global<<<1, 1024, 0, st>>>(btot, nbk, bstart);
-            bucket_scatter<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(src, nbk, boff, bstart, ent, entv, z64, t,
+            const size_t shs = (size_t)nbk * (8 + 4);  // bases + cursors
+            if (shs > 64 * 1024)
+                PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_scatter,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
+            bucket_scatter<<<(unsigned)nsrc, SEG_THREADS, shs, st>>>(src, nbk, boff, bstart, ent, entv, z64, t,
                                                                      (long long)nnzs);
             int nbm = 4;
             while (nbm > 1 && (size_t)nbm * 8 * nw + 2048 > ctx->lds_per_block) nbm--;

@@ -1133,13 +1133,22 @@ constexpr int SORT_MAX = 64 * 16;  // longest row sorted in registers
 // One work-group per bucket: row counts -> row offsets, scatter into the
 // rows, then each wave sorts whole rows: rows up to SORT_MAX entries by a
 // register bitonic sort, longer ones (waves wv < nbm) by LDS-bitmap rank.
-__global__ __launch_bounds__(ROWS_THREADS) void bucket_rows(const long long *__restrict__ bstart,
-                                                            const unsigned *__restrict__ ent,
-                                                            const float *__restrict__ entv, int n, int nbk, int nbm,
-                                                            int *__restrict__ row, int *__restrict__ ctmp,
-                                                            float *__restrict__ vtmp, int *__restrict__ col,
-                                                            float *__restrict__ val, long long cap)
+// W: weights travel with the columns (64-bit sort keys).  Without them the
+// kernel fits two work-groups per CU, so one's phases overlap the other's.
+template <bool W>
+__global__ __launch_bounds__(ROWS_THREADS, W ? 4 : 8) void bucket_rows(const long long *__restrict__ bstart,
+                                                                       const unsigned *__restrict__ ent,
+                                                                       const float *__restrict__ entv, int n, int nbk,
+                                                                       int nbm, int *__restrict__ row,
+                                                                       int *__restrict__ ctmp, float *__restrict__ vtmp,
+                                                                       int *__restrict__ col, float *__restrict__ val,
+                                                                       long long cap)
 {
+    if (!W) {
+        entv = nullptr;
+        vtmp = nullptr;
+        val = nullptr;
+    }
     extern __shared__ uint32_t sm[];
     __shared__ unsigned rc[BROWS], rb[BROWS + 1];
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -2042,10 +2051,15 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             int nbm = 4;
             while (nbm > 1 && (size_t)nbm * 8 * nw + 2048 > ctx->lds_per_block) nbm--;
             const size_t shr = (size_t)nbm * 8 * nw;
-            PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                            (int)shr));
-            bucket_rows<<<(unsigned)nbk, ROWS_THREADS, shr, st>>>(bstart, ent, entv, n, nbk, nbm, row, ctmp, vtmp,
-                                                                  col, val, (long long)nnzs);
+            const void *rows_fn = weighted ? (const void *)bucket_rows<true> : (const void *)bucket_rows<false>;
+            PG_HIP(ctx, hipFuncSetAttribute(rows_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shr));
+            if (weighted)
+                bucket_rows<true><<<(unsigned)nbk, ROWS_THREADS, shr, st>>>(bstart, ent, entv, n, nbk, nbm, row, ctmp,
+                                                                            vtmp, col, val, (long long)nnzs);
+            else
+                bucket_rows<false><<<(unsigned)nbk, ROWS_THREADS, shr, st>>>(bstart, ent, nullptr, n, nbk, nbm, row,
+                                                                             ctmp, nullptr, col, nullptr,
+                                                                             (long long)nnzs);
             PG_KERNEL_CHECK(ctx);
         } else {
             PG_HIP(ctx, hipMemsetAsync(row, 0, 4, st));

@@ -8,6 +8,36 @@
 
 namespace pgdev {
 
+// x from lane (lane ^ lj) of the wave.  Distances inside a 16-lane row are DPP
+// moves on the VALU (quad_perm for 1 and 2, row_ror:8 for 8, row_shl:4 /
+// row_shr:4 picked by lane bit 2 for 4); 16 and 32 go through ds_bpermute.
+__device__ __forceinline__ unsigned xor_lanes32(unsigned x, int lj, int lane)
+{
+    switch (lj) {
+    case 1: return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    case 2: return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    case 8: return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, false);  // row_ror:8
+    case 4: {
+        const unsigned up = (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x104, 0xF, 0xF, false);  // row_shl:4
+        const unsigned dn = (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x114, 0xF, 0xF, false);  // row_shr:4
+        return (lane & 4) ? dn : up;
+    }
+    default: return (unsigned)__shfl_xor((int)x, lj);
+    }
+}
+
+template <class K>
+__device__ __forceinline__ K xor_lanes(K x, int lj, int lane)
+{
+    if constexpr (sizeof(K) == 8) {
+        const unsigned lo = xor_lanes32((unsigned)x, lj, lane);
+        const unsigned hi = xor_lanes32((unsigned)((unsigned long long)x >> 32), lj, lane);
+        return (K)(((unsigned long long)hi << 32) | lo);
+    } else {
+        return (K)xor_lanes32((unsigned)x, lj, lane);
+    }
+}
+
 // Bitonic sort of 64*P keys held P per lane (lane l holds keys l*P .. l*P+P-1),
 // ascending in that order: register compare-exchanges for partner distances
 // below P, cross-lane shuffles above.
@@ -37,7 +67,7 @@ __device__ __forceinline__ void wave_bitonic(K (&x)[P], int lane)
                 const bool keep_min = lower == asc;
 #pragma unroll
                 for (int r = 0; r < P; r++) {
-                    const K y = __shfl_xor(x[r], lj);
+                    const K y = xor_lanes(x[r], lj, lane);
                     x[r] = keep_min ? (x[r] < y ? x[r] : y) : (x[r] < y ? y : x[r]);
                 }
             }
@@ -47,6 +77,8 @@ __device__ __forceinline__ void wave_bitonic(K (&x)[P], int lane)
 
 // Sort the d <= 64*P columns (and values) of the row starting at a: ctmp /
 // vtmp hold the unsorted entries, col / val receive them in column order.
+// Keys are loaded lane-interleaved (entry r*64 + lane into register r) and the
+// sorted keys written lane-major (lane*P + r), the network's order.
 template <int P>
 __device__ __forceinline__ void sort_row_regs(const int *__restrict__ ctmp, const float *__restrict__ vtmp,
                                               long long a, int d, int lane, int *__restrict__ col,
@@ -56,7 +88,7 @@ __device__ __forceinline__ void sort_row_regs(const int *__restrict__ ctmp, cons
         unsigned long long x[P];
 #pragma unroll
         for (int r = 0; r < P; r++) {
-            const int e = lane * P + r;
+            const int e = r * 64 + lane;  // coalesced: the sort does not care where a key starts
             x[r] = e < d ? ((unsigned long long)(unsigned)ctmp[a + e] << 32) | __float_as_uint(vtmp[a + e]) : ~0ull;
         }
         wave_bitonic<P>(x, lane);
@@ -72,7 +104,7 @@ __device__ __forceinline__ void sort_row_regs(const int *__restrict__ ctmp, cons
         int x[P];
 #pragma unroll
         for (int r = 0; r < P; r++) {
-            const int e = lane * P + r;
+            const int e = r * 64 + lane;
             x[r] = e < d ? ctmp[a + e] : INT_MAX;
         }
         wave_bitonic<P>(x, lane);

@@ -1601,10 +1601,27 @@ __device__ unsigned long long radix_select(long long c, long long m, KeyF key, W
             }
         }
         __syncthreads();
-        prefix |= sh[0] << shift;
+        const unsigned long long dsel = sh[0];
+        const bool single = hist[dsel] == 1u;  // one key left under the new prefix
+        prefix |= dsel << shift;
         mask |= 255ull << shift;
         mr = (long long)sh[1];
         __syncthreads();
+        if (single && shift > 0) {
+            // its remaining bytes in one scan instead of a pass per byte
+            for (long long q0 = threadIdx.x; q0 < c; q0 += RB * stride) {
+                unsigned long long kk[RB];
+                bool ww[RB];
+                batch(q0, kk, ww);
+#pragma unroll
+                for (int u = 0; u < RB; u++)
+                    if (ww[u] && (kk[u] & mask) == prefix) sh[0] = kk[u];
+            }
+            __syncthreads();
+            prefix = sh[0];
+            __syncthreads();
+            break;
+        }
     }
     *rank_left = mr;
     return prefix;
@@ -1643,8 +1660,13 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
     // FP64 r an exact tie is rare, and then all of the few are wanted)
     unsigned long long k2 = ~0ull;
     {
+        // ties at k1, and t* (every key k1 carries the same r64)
         unsigned long long nt = 0;
-        for (long long q = threadIdx.x; q < c; q += blockDim.x) nt += key_r_desc(r64[q]) == k1;
+        for (long long q = threadIdx.x; q < c; q += blockDim.x)
+            if (key_r_desc(r64[q]) == k1) {
+                nt++;
+                tstar[0] = r64[q];
+            }
         if (threadIdx.x == 0) sh[0] = 0;
         __syncthreads();
         atomicAdd(&sh[0], nt);
@@ -1659,9 +1681,6 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
                 [&](long long q) { return key_r_desc(r64[q]) == k1; }, hist, sh, &dummy);
         }
     }
-    // t* is the r64 carrying key k1
-    for (long long q = threadIdx.x; q < c; q += blockDim.x)
-        if (key_r_desc(r64[q]) == k1) tstar[0] = r64[q];
     __syncthreads();
     const double ts = tstar[0];
     unsigned long long fl = 0;

@@ -10,11 +10,16 @@
 //    |u - u0|^2, |u + u0|^2) in a fixed order, so all blocks take the same stop
 //    decision on the device and launches past the stop return at once.  The
 //    host looks once per batch of launches.  FP64 throughout, FP32 weights.
-//    The SpMV is column-segmented: each block stages 12,288-entry segments of
-//    the scaled vector in LDS and every wave gathers from there, one row at a
-//    time, over the row's entries in that segment (rows are column-sorted, so
-//    each is a contiguous range found once per call); an FP64 gather straight
-//    from L2 would move a whole cache line per 8-byte element.
+//    The SpMV is column-segmented and sliced: each block stages 12,288-entry
+//    segments of the scaled vector in LDS (an FP64 gather straight from L2
+//    would move a whole cache line per 8-byte element); the matrix is
+//    re-laid once per call as slices of 64 rows, one per wave, whose entries
+//    in each segment are stored in groups of 8 positions: group g holds, for
+//    each of the 64 rows in turn, its entries 8g .. 8g+7 as 16-bit
+//    segment-relative indices (one 16-byte load per lane, 1 KB per wave),
+//    padded to the slice's longest row with the index of a zero LDS slot.
+//    Each lane sums its own row in column order, y = y + w x, as the
+//    reference's loop.
 //  * Participation coefficient (W/PC_CPU/PC_CPU/main.cpp:96-160): one wave per
 //    vertex, FP32 module sums accumulated in LDS in CSR order by one lane
 //    (entries staged 64 at a time), then the FP32 sum of squared shares over
@@ -34,13 +39,16 @@ constexpr double EC_EPS = 1e-6;     // CUEC_unweighted.cu:28
 constexpr int EC_TPB = 1024;
 constexpr int EC_BATCH = 32;        // launches per host check
 constexpr int EC_SEG = 12288;       // vector entries per LDS segment (96 KB)
+constexpr int EC_RB = 256;          // rows per block of the sliced SpMV: 4 slices of 64
+constexpr int EC_PG = 8;            // positions per lane load (16 bytes of indices)
 
 struct EcState {
     int stopped, iters;
     double lam, err1, err2;
 };
 
-// fixed-order sum over the block (every thread gets the result)
+// fixed-order sum over a block of TPB threads (every thread gets the result)
+template <int TPB = EC_TPB>
 __device__ double ec_block_sum(double x, double *red)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -49,7 +57,7 @@ __device__ double ec_block_sum(double x, double *red)
     if (lane == 0) red[wv] = x;
     __syncthreads();
     double s = 0.0;
-    for (int q = 0; q < EC_TPB / 64; q++) s += red[q];
+    for (int q = 0; q < TPB / 64; q++) s += red[q];
     __syncthreads();
     return s;
 }
@@ -93,30 +101,99 @@ __global__ void ec_segments(const int *__restrict__ row, const int *__restrict__
     }
 }
 
-// Iteration k: block b owns rows [b R, (b+1) R); per segment 16 lanes per row.
-// Dynamic LDS: [EC_SEG] scaled vector segment, then [R] row partials.
-__global__ __launch_bounds__(EC_TPB) void ec_iter(const int *__restrict__ col, const float *__restrict__ val,
-                                                  int n, int R, int G, int k, int nseg,
-                                                  const int *__restrict__ segoff, const double *__restrict__ uin,
-                                                  double *__restrict__ uout, double *__restrict__ vhat,
-                                                  const double *__restrict__ pin, double *__restrict__ pout,
-                                                  EcState *__restrict__ st)
+// Slice q (rows 64q .. 64q+63), segment s: len = the longest row part.
+__global__ void ec_slice_len(const int *__restrict__ segoff, int n, int nslice, int nseg, int *__restrict__ len)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nslice * nseg) return;
+    const int q = x / nseg, s = x % nseg;
+    int m = 0;
+    for (int i = q * 64; i < min(n, q * 64 + 64); i++) {
+        const long long so = (long long)i * (nseg + 1) + s;
+        m = max(m, segoff[so + 1] - segoff[so]);
+    }
+    len[x] = (m + EC_PG - 1) / EC_PG * EC_PG;
+}
+
+// off[x] = 64 * (len[0] + ... + len[x - 1]), x = 0 .. m, by one work-group.
+__global__ __launch_bounds__(1024) void ec_len_scan(const int *__restrict__ len, int m, long long *__restrict__ off)
+{
+    __shared__ long long part[1024];
+    const int tid = threadIdx.x;
+    const int per = (m + 1023) / 1024;
+    const int a = min(m, tid * per), b = min(m, a + per);
+    long long s = 0;
+    for (int q = a; q < b; q++) s += len[q];
+    part[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const long long y = tid >= o ? part[tid - o] : 0;
+        __syncthreads();
+        part[tid] += y;
+        __syncthreads();
+    }
+    long long run = part[tid] - s;
+    for (int q = a; q < b; q++) {
+        off[q] = 64 * run;
+        run += len[q];
+    }
+    if (tid == 1023) off[m] = 64 * part[1023];
+}
+
+// Row slot i (0 .. 64 nslice), segment s: entry p at off + (p / 8) 512 +
+// (i % 64) 8 + p % 8, then the padding index EC_SEG (a zero in LDS) with
+// weight 0.
+__global__ void ec_fill(const int *__restrict__ col, const float *__restrict__ val, const int *__restrict__ segoff,
+                        const long long *__restrict__ off, int n, int nslice, int nseg,
+                        uint16_t *__restrict__ ecol, float *__restrict__ eval)
+{
+    const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= (long long)nslice * 64 * nseg) return;
+    const int s = (int)(x % nseg), i = (int)(x / nseg);
+    const int q = i >> 6, l = i & 63;
+    const long long base = off[(long long)q * nseg + s], len = (off[(long long)q * nseg + s + 1] - base) / 64;
+    int a = 0, cnt = 0;
+    if (i < n) {
+        const long long so = (long long)i * (nseg + 1) + s;
+        a = segoff[so];
+        cnt = segoff[so + 1] - a;
+    }
+    const int x0 = s * EC_SEG;
+    for (long long p = 0; p < len; p++) {
+        const long long d = base + (p / EC_PG) * (64 * EC_PG) + l * EC_PG + p % EC_PG;
+        if (p < cnt) {
+            ecol[d] = (uint16_t)(col[a + p] - x0);
+            if (eval) eval[d] = val[a + p];
+        } else {
+            ecol[d] = (uint16_t)EC_SEG;
+            if (eval) eval[d] = 0.0f;
+        }
+    }
+}
+
+// Iteration k: block b owns rows [256 b, 256 b + 256), lane = row.
+// Dynamic LDS: [EC_SEG + 1] scaled vector segment, the last entry 0.
+__global__ __launch_bounds__(EC_RB) void ec_iter(const uint16_t *__restrict__ ecol, const float *__restrict__ eval,
+                                                 const long long *__restrict__ off, int n, int nslice, int G, int k,
+                                                 int nseg,
+                                                 const double *__restrict__ uin, double *__restrict__ uout,
+                                                 double *__restrict__ vhat, const double *__restrict__ pin,
+                                                 double *__restrict__ pout, EcState *__restrict__ st)
 {
 #pragma clang fp contract(off)
     extern __shared__ double xs[];
-    double *yp = xs + EC_SEG;
-    __shared__ double red[EC_TPB / 64];
+    __shared__ double red[EC_RB / 64];
     if (st->stopped) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int q = tid; q < G; q += EC_TPB) {
+    for (int q = tid; q < G; q += EC_RB) {
         a0 += pin[3 * q + 0];
         a1 += pin[3 * q + 1];
         a2 += pin[3 * q + 2];
     }
-    a0 = ec_block_sum(a0, red);
-    a1 = ec_block_sum(a1, red);
-    a2 = ec_block_sum(a2, red);
+    a0 = ec_block_sum<EC_RB>(a0, red);
+    a1 = ec_block_sum<EC_RB>(a1, red);
+    a2 = ec_block_sum<EC_RB>(a2, red);
     if (k > 0) {
         const double e1 = sqrt(a0), e2 = sqrt(a1);
         if (!(e1 > EC_EPS && e2 > EC_EPS) || k >= EC_MAX_ITER) {
@@ -132,56 +209,73 @@ __global__ __launch_bounds__(EC_TPB) void ec_iter(const int *__restrict__ col, c
     const double vnorm = sqrt(a2);
     const double t = 1 / vnorm;
     if (blockIdx.x == 0 && tid == 0) st->lam = vnorm;
-    const int r0 = blockIdx.x * R, r1 = min(n, r0 + R);
-    for (int q = tid; q < r1 - r0; q += EC_TPB) yp[q] = 0.0;
+    if (tid == 0) xs[EC_SEG] = 0.0;  // the padding slot; staging never writes it
+    const int i = blockIdx.x * EC_RB + tid;
+    const int qi = blockIdx.x * (EC_RB / 64) + wv;  // this wave's slice (none past the last row)
+    const long long qs = (long long)qi * nseg;
+    double y = 0.0;
     for (int s = 0; s < nseg; s++) {
         const int x0 = s * EC_SEG, xn = min(EC_SEG, n - x0);
         __syncthreads();  // the previous segment's readers are done
-        for (int x = tid; x < xn; x += EC_TPB) xs[x] = uin[x0 + x] * t;
-        __syncthreads();
-        // four rows per wave, 16 lanes each, two accumulators per lane: the
-        // rows' index loads are in flight together
-        const int grp = lane >> 4, gl = lane & 15;
-        for (int base = r0 + wv * 4; base < r1; base += EC_TPB / 16) {
-            const int i = base + grp;
-            int a = 0, b = 0;
-            if (i < r1) {
-                const long long so = (long long)i * (nseg + 1) + s;
-                a = segoff[so];
-                b = segoff[so + 1];
-            }
-            double y0 = 0.0, y1 = 0.0;
-            if (val) {
-                for (int e = a + gl; e < b; e += 32) {
-                    y0 = fma((double)val[e], xs[col[e] - x0], y0);
-                    if (e + 16 < b) y1 = fma((double)val[e + 16], xs[col[e + 16] - x0], y1);
-                }
-            } else {
-                for (int e = a + gl; e < b; e += 32) {
-                    y0 += xs[col[e] - x0];
-                    if (e + 16 < b) y1 += xs[col[e + 16] - x0];
-                }
-            }
-            double y = y0 + y1;
+        {
+            // 16-byte loads, eight in flight per thread; an odd tail reads the
+            // buffer's padding word into a slot no column refers to
+            const double2 *src = (const double2 *)(uin + x0);
+            double2 *dst = (double2 *)xs;
+            const int n2 = (xn + 1) / 2;
+            for (int x = tid; x < n2; x += EC_RB * 8) {
+                double2 v[8];
 #pragma unroll
-            for (int o = 8; o > 0; o >>= 1) y += __shfl_xor(y, o);
-            if (i < r1 && gl == 0) yp[i - r0] += y;
+                for (int u = 0; u < 8; u++) {
+                    const int q = x + u * EC_RB;
+                    v[u] = q < n2 ? src[q] : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int q = x + u * EC_RB;
+                    if (q < n2) dst[q] = make_double2(v[u].x * t, v[u].y * t);
+                }
+            }
+        }
+        __syncthreads();
+        const long long a = qi < nslice ? off[qs + s] : 0;
+        const int ng = qi < nslice ? (int)((off[qs + s + 1] - a) / (64 * EC_PG)) : 0;
+        const uint4 *cp = (const uint4 *)(ecol + a) + lane;  // 8 indices per lane and group
+        if (eval) {
+            const float4 *vp = (const float4 *)(eval + a) + 2 * lane;
+#pragma unroll 4
+            for (int g = 0; g < ng; g++) {
+                const uint4 c = cp[(long long)g * 64];
+                const float4 w0 = vp[(long long)g * 128], w1 = vp[(long long)g * 128 + 1];
+                const unsigned cc[4] = {c.x, c.y, c.z, c.w};
+                const float ww[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+                for (int h = 0; h < EC_PG; h++)
+                    y = y + (double)ww[h] * xs[(cc[h >> 1] >> (16 * (h & 1))) & 0xffffu];
+            }
+        } else {
+#pragma unroll 4
+            for (int g = 0; g < ng; g++) {
+                const uint4 c = cp[(long long)g * 64];
+                const unsigned cc[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+                for (int h = 0; h < EC_PG; h++) y = y + xs[(cc[h >> 1] >> (16 * (h & 1))) & 0xffffu];
+            }
         }
     }
-    __syncthreads();
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int i = r0 + tid; i < r1; i += EC_TPB) {
-        const double y = yp[i - r0], ui = uin[i];
+    if (i < n) {
+        const double ui = uin[i];
         vhat[i] = ui * t;
         uout[i] = y;
         const double d1 = y - ui, d2 = y + ui;
-        s0 += d1 * d1;
-        s1 += d2 * d2;
-        s2 += y * y;
+        s0 = d1 * d1;
+        s1 = d2 * d2;
+        s2 = y * y;
     }
-    s0 = ec_block_sum(s0, red);
-    s1 = ec_block_sum(s1, red);
-    s2 = ec_block_sum(s2, red);
+    s0 = ec_block_sum<EC_RB>(s0, red);
+    s1 = ec_block_sum<EC_RB>(s1, red);
+    s2 = ec_block_sum<EC_RB>(s2, red);
     if (tid == 0) {
         pout[3 * blockIdx.x + 0] = s0;
         pout[3 * blockIdx.x + 1] = s1;
@@ -262,16 +356,32 @@ extern "C" pg_status pg_eigenvector_centrality(pg_ctx *ctx, const int32_t *row, 
     if (val) PG_TRY(pg_in(ctx, val, (size_t)nnz, "in.val", &dval));
     PG_TRY(pg_validate_csr(ctx, drow, dcol, n, nnz, true));
     hipStream_t st = ctx->stream;
-    // one block per CU (96 KB segment + row partials in LDS)
-    const int G = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->num_cu, pg_cdiv(n, 64)));
-    const int R = (int)pg_cdiv(n, G);
+    // 256 rows per block (4 slices of 64), the scaled vector staged in LDS
+    // segment by segment
+    const int G = (int)pg_cdiv(n, EC_RB);
+    const int R = EC_RB;
     const int nseg = (int)pg_cdiv(n, EC_SEG);
-    const size_t lds = ((size_t)EC_SEG + (size_t)R) * sizeof(double);
-    if (lds > 160 * 1024) return pg_fail(ctx, PG_ENOTSUP, "n = %d needs %zu bytes of LDS per block", n, lds);
+    const int nslice = (int)pg_cdiv(n, 64);
+    const size_t lds = ((size_t)EC_SEG + 1) * sizeof(double);
     PG_HIP(ctx, hipFuncSetAttribute((const void *)ec_iter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int *segoff;
+    int *segoff, *slen;
+    long long *soff;
     PG_TRY(pg_bufT(ctx, "ec.segoff", (size_t)n * (nseg + 1), &segoff));
+    PG_TRY(pg_bufT(ctx, "ec.slen", (size_t)nslice * nseg, &slen));
+    PG_TRY(pg_bufT(ctx, "ec.soff", (size_t)nslice * nseg + 1, &soff));
     ec_segments<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(drow, dcol, n, nseg, segoff);
+    ec_slice_len<<<(unsigned)pg_cdiv((long long)nslice * nseg, 256), 256, 0, st>>>(segoff, n, nslice, nseg, slen);
+    ec_len_scan<<<1, 1024, 0, st>>>(slen, nslice * nseg, soff);
+    PG_KERNEL_CHECK(ctx);
+    long long npad = 0;
+    PG_HIP(ctx, hipMemcpyAsync(&npad, soff + (long long)nslice * nseg, 8, hipMemcpyDeviceToHost, st));
+    PG_HIP(ctx, hipStreamSynchronize(st));
+    uint16_t *ecol;
+    float *eval = nullptr;
+    PG_TRY(pg_bufT(ctx, "ec.ecol", (size_t)std::max<long long>(npad, 1), &ecol));
+    if (dval) PG_TRY(pg_bufT(ctx, "ec.eval", (size_t)std::max<long long>(npad, 1), &eval));
+    ec_fill<<<(unsigned)pg_cdiv((long long)nslice * 64 * nseg, 256), 256, 0, st>>>(dcol, dval, segoff, soff, n, nslice,
+                                                                                  nseg, ecol, eval);
     double *u[2], *part[2], *dvh, *dav;
     EcState *dst;
     PG_TRY(pg_bufT(ctx, "ec.u0", (size_t)n, &u[0]));
@@ -288,8 +398,8 @@ extern "C" pg_status pg_eigenvector_centrality(pg_ctx *ctx, const int32_t *row, 
     EcState h{};
     for (int k0 = 0; k0 <= EC_MAX_ITER; k0 += EC_BATCH) {
         for (int k = k0; k < k0 + EC_BATCH && k <= EC_MAX_ITER; k++)
-            ec_iter<<<(unsigned)G, EC_TPB, lds, st>>>(dcol, dval, n, R, G, k, nseg, segoff, u[k & 1],
-                                                      u[(k + 1) & 1], dvh, part[(k + 1) & 1], part[k & 1], dst);
+            ec_iter<<<(unsigned)G, EC_RB, lds, st>>>(ecol, eval, soff, n, nslice, G, k, nseg, u[k & 1], u[(k + 1) & 1],
+                                                     dvh, part[(k + 1) & 1], part[k & 1], dst);
         PG_KERNEL_CHECK(ctx);
         PG_HIP(ctx, hipMemcpyAsync(&h, dst, sizeof h, hipMemcpyDeviceToHost, st));
         PG_HIP(ctx, hipStreamSynchronize(st));

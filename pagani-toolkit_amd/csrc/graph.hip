@@ -51,15 +51,19 @@ __global__ void strength_kernel(const int *__restrict__ row, const float *__rest
 
 // -------------------------------------------------------------- clustering
 
-__device__ __forceinline__ int first_greater(const int *__restrict__ col, int lo, int hi, int key)
+// A scan start for the columns of row [rv, vb) above key, at or before the
+// first of them: 64 evenly spaced samples (one load per lane) and a ballot,
+// in place of a binary search's chain of dependent loads.  The scan filters
+// col > key itself.  Called by a whole wave.
+__device__ __forceinline__ int tail_start(const int *__restrict__ col, int rv, int vb, int key, int lane)
 {
-    // smallest e in [lo, hi) with col[e] > key
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (col[mid] <= key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
+    const int dv = vb - rv;
+    if (dv <= 0) return vb;
+    const int ps = rv + (int)(((long long)lane * dv) >> 6);  // < vb
+    const unsigned long long m = __ballot(col[ps] > key);
+    if (m & 1ull) return rv;
+    const int f = m ? __builtin_ctzll(m) : 64;
+    return rv + (int)(((long long)(f - 1) * dv) >> 6);
 }
 
 template <class T>
@@ -102,12 +106,10 @@ __global__ __launch_bounds__(256) void triangles_kernel(const int *__restrict__ 
     for (int k = wv; k < d; k += 4) {
         const int v = col[a + k];
         const int vb = row[v + 1];
-        int start = 0;
-        if (lane == 0) start = first_greater(col, row[v], vb, v);
-        start = __shfl(start, 0);
+        const int start = tail_start(col, row[v], vb, v, lane);
         for (int e = start + lane; e < vb; e += 64) {
-            int w = col[e];
-            cnt += (bits[w >> 5] >> (w & 31)) & 1u;
+            const int w = col[e];
+            cnt += (w > v) & ((bits[w >> 5] >> (w & 31)) & 1u);
         }
     }
     long long s = block_sum<long long>(cnt, red);
@@ -199,13 +201,11 @@ __global__ __launch_bounds__(256) void cpw_kernel(const int *__restrict__ row, c
         const int v = col[a + k];
         const double wuv = (double)val[a + k];
         const int vb = row[v + 1];
-        int start = 0;
-        if (lane == 0) start = first_greater(col, row[v], vb, v);
-        start = __shfl(start, 0);
+        const int start = tail_start(col, row[v], vb, v, lane);
         for (int e = start + lane; e < vb; e += 64) {
             const int w = col[e];
             const uint32_t word = bits[w >> 5];
-            if ((word >> (w & 31)) & 1u) {
+            if (w > v && ((word >> (w & 31)) & 1u)) {
                 const double wvw = (double)val[e];
                 if (wvw != 0.0) {
                     const int pos = pre[w >> 5] + __popc(word & ((1u << (w & 31)) - 1u));

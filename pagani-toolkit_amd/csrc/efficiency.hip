@@ -105,34 +105,48 @@ __global__ __launch_bounds__(256) void eff_level(const int *__restrict__ row, co
 // Level 0 -> 1 from the column indices alone: the sources at distance 1 from
 // w are its neighbours inside the batch.  One wave per (w, 64-word chunk); the
 // chunk's 4096 sources are a contiguous column range of w's sorted row, found
-// by binary search, and each such neighbour sets its bit in the lane owning
-// its word -- no neighbour visited words are read.
+// from a sampled ballot and read 64 entries at a time, each such neighbour
+// setting its bit in the LDS word of the lane owning it -- no neighbour
+// visited words are read.
 __global__ __launch_bounds__(256) void eff_level0(const int *__restrict__ row, const int *__restrict__ col, int n,
                                                   int W, int base, const u64 *__restrict__ visA,
                                                   u64 *__restrict__ visB, Planes planes,
                                                   unsigned *__restrict__ prog)
 {
-    const int lane = threadIdx.x & 63;
+    __shared__ u64 sbits[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int chunks = W >> 6;
-    const long long wid = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long long wid = (long long)blockIdx.x * 4 + wv;
     if (wid >= (long long)n * chunks) return;
     const int w = __builtin_amdgcn_readfirstlane((int)(wid / chunks));
     const int c = __builtin_amdgcn_readfirstlane((int)(wid % chunks));
     const int k = (c << 6) + lane;
     const long long v0 = (long long)base + 4096ll * c, v1 = v0 + 4096;
-    int lo = row[w], hi = row[w + 1];
-    while (lo < hi) {  // first entry with col >= v0
-        const int mid = (lo + hi) >> 1;
-        if (col[mid] < v0) lo = mid + 1;
-        else hi = mid;
+    const int a = row[w], b = row[w + 1], dw = b - a;
+    // a start at or before the first column >= v0: 64 evenly spaced samples
+    // and a ballot (the loop below filters v >= v0 itself)
+    int start = b;
+    if (dw > 0) {
+        const int ps = a + (int)(((long long)lane * dw) >> 6);  // < b
+        const unsigned long long m = __ballot(col[ps] >= v0);
+        const int f = m ? __builtin_ctzll(m) : 64;
+        start = f == 0 ? a : a + (int)(((long long)(f - 1) * dw) >> 6);
     }
-    u64 acc = 0;
-    for (int e = lo; e < row[w + 1]; e++) {
-        const long long v = col[e];
-        if (v >= v1) break;
-        const int rel = (int)(v - v0);
-        if ((rel >> 6) == lane) acc |= 1ull << (rel & 63);
+    // the chunk's neighbours, 64 at a time, each OR-ed into the LDS word of
+    // the lane owning it
+    sbits[wv][lane] = 0ull;
+    __threadfence_block();
+    for (int e0 = start; e0 < b; e0 += 64) {
+        const int e = e0 + lane;
+        const long long v = e < b ? (long long)col[e] : v1;
+        if (v >= v0 && v < v1) {
+            const int rel = (int)(v - v0);
+            atomicOr(&sbits[wv][rel >> 6], 1ull << (rel & 63));
+        }
+        if (__ballot(e < b && v >= v1)) break;  // columns ascend: the chunk is done
     }
+    __threadfence_block();
+    const u64 acc = atomicOr(&sbits[wv][lane], 0ull);
     const u64 vis = visA[(size_t)w * W + k];
     const u64 nb = acc & ~vis;
     visB[(size_t)w * W + k] = vis | nb;

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "pg_internal.hpp"
+#include "pg_scan.hpp"
 
 namespace {
 
@@ -118,26 +119,7 @@ __global__ void ec_slice_len(const int *__restrict__ segoff, int n, int nslice, 
 // off[x] = 64 * (len[0] + ... + len[x - 1]), x = 0 .. m, by one work-group.
 __global__ __launch_bounds__(1024) void ec_len_scan(const int *__restrict__ len, int m, long long *__restrict__ off)
 {
-    __shared__ long long part[1024];
-    const int tid = threadIdx.x;
-    const int per = (m + 1023) / 1024;
-    const int a = min(m, tid * per), b = min(m, a + per);
-    long long s = 0;
-    for (int q = a; q < b; q++) s += len[q];
-    part[tid] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const long long y = tid >= o ? part[tid - o] : 0;
-        __syncthreads();
-        part[tid] += y;
-        __syncthreads();
-    }
-    long long run = part[tid] - s;
-    for (int q = a; q < b; q++) {
-        off[q] = 64 * run;
-        run += len[q];
-    }
-    if (tid == 1023) off[m] = 64 * part[1023];
+    pgdev::block_scan_1024(len, m, off, 64);
 }
 
 // Row slot i (0 .. 64 nslice), segment s: entry p at off + (p / 8) 512 +

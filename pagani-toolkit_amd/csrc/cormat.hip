@@ -39,6 +39,7 @@
 #include <vector>
 
 #include "pg_internal.hpp"
+#include "pg_scan.hpp"
 
 namespace {
 
@@ -335,29 +336,7 @@ __global__ __launch_bounds__(256) void cm_row_counts(const float *__restrict__ a
 // Exclusive scan of n ints into long long (one work-group), total in out[n].
 __global__ __launch_bounds__(1024) void cm_scan(const int *__restrict__ in, int n, long long *__restrict__ out)
 {
-    __shared__ long long part[1024];
-    const int tid = threadIdx.x;
-    const long long per = (n + 1023) / 1024;
-    const long long a = std::min<long long>(n, tid * per), b = std::min<long long>(n, a + per);
-    long long s = 0;
-    for (long long q = a; q < b; q++) s += in[q];
-    part[tid] = s;
-    __syncthreads();
-    if (tid == 0) {
-        long long run = 0;
-        for (int q = 0; q < 1024; q++) {
-            const long long x = part[q];
-            part[q] = run;
-            run += x;
-        }
-        out[n] = run;
-    }
-    __syncthreads();
-    long long run = part[tid];
-    for (long long q = a; q < b; q++) {
-        out[q] = run;
-        run += in[q];
-    }
+    pgdev::block_scan_1024(in, n, out, 1);
 }
 
 // Keep byte per pair in packed order; ties at the bound are kept while their

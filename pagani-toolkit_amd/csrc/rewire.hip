@@ -36,6 +36,7 @@
 
 #include "pg_internal.hpp"
 #include "pg_rowsort.hpp"
+#include "pg_scan.hpp"
 
 namespace {
 
@@ -128,26 +129,7 @@ __global__ void rw_upper_count(const int *__restrict__ row, const int *__restric
 // Exclusive scan of n ints into out[0..n], by one work-group.
 __global__ __launch_bounds__(1024) void rw_scan(const int *__restrict__ in, int n, long long *__restrict__ out)
 {
-    __shared__ long long part[1024];
-    const int tid = threadIdx.x;
-    const long long per = (n + 1023) / 1024;
-    const long long a = std::min<long long>(n, tid * per), b = std::min<long long>(n, a + per);
-    long long s = 0;
-    for (long long q = a; q < b; q++) s += in[q];
-    part[tid] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
-        const long long y = tid >= o ? part[tid - o] : 0;
-        __syncthreads();
-        part[tid] += y;
-        __syncthreads();
-    }
-    long long run = part[tid] - s;
-    for (long long q = a; q < b; q++) {
-        out[q] = run;
-        run += in[q];
-    }
-    if (tid == 1023) out[n] = part[1023];
+    pgdev::block_scan_1024(in, n, out, 1);
 }
 
 // One wave per row: the row's upper entries into E at base[u], in column order.

@@ -267,11 +267,11 @@ __global__ void rw_cursor_init(const int *__restrict__ row, int n, int *__restri
 }
 
 // Both directions of every edge into their rows (unordered inside a row).
-// Swaps preserve degrees, so each row receives exactly its length; the guard
-// keeps every write inside its row regardless.
+// Swaps preserve degrees, so each row receives exactly its length; a write
+// that would leave its row is dropped and flagged (bit 3), and the call fails.
 __global__ void rw_fill(const int *__restrict__ row, const int *__restrict__ eu, const int *__restrict__ ev,
                         const float *__restrict__ ew, int m, int *__restrict__ cur, int *__restrict__ ctmp,
-                        float *__restrict__ vtmp)
+                        float *__restrict__ vtmp, int *__restrict__ flags)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m) return;
@@ -286,6 +286,7 @@ __global__ void rw_fill(const int *__restrict__ row, const int *__restrict__ eu,
         ctmp[pv] = u;
         if (ew) vtmp[pv] = w;
     }
+    if (pu >= row[u + 1] || pv >= row[v + 1]) atomicOr(flags, 8);
 }
 
 // One wave per row: rows up to SHORT_ROW entries sorted by column in registers.
@@ -463,7 +464,7 @@ extern "C" pg_status pg_rewire(pg_ctx *ctx, const int32_t *row, const int32_t *c
     PG_TRY(pg_bufT(ctx, "rw.ctmp", (size_t)nnz, &ctmp));
     if (weighted) PG_TRY(pg_bufT(ctx, "rw.vtmp", (size_t)nnz, &vtmp));
     rw_cursor_init<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(drow, n, cur);
-    rw_fill<<<(unsigned)pg_cdiv(m, 256), 256, 0, st>>>(drow, eu, ev, ew, m, cur, ctmp, vtmp);
+    rw_fill<<<(unsigned)pg_cdiv(m, 256), 256, 0, st>>>(drow, eu, ev, ew, m, cur, ctmp, vtmp, flags);
     rw_sort_short<<<(unsigned)pg_cdiv(n, 4), 256, 0, st>>>(drow, n, ctmp, vtmp, dout, dvout);
     if (long_rows) rw_sort_long<<<(unsigned)n, 256, long_lds, st>>>(drow, n, ctmp, vtmp, dout, dvout);
     PG_KERNEL_CHECK(ctx);
@@ -471,7 +472,9 @@ extern "C" pg_status pg_rewire(pg_ctx *ctx, const int32_t *row, const int32_t *c
     if (weighted) PG_TRY(pg_out_finish(ctx, out_val, dvout, (size_t)nnz));
     unsigned long long ns = 0;
     PG_HIP(ctx, hipMemcpyAsync(&ns, dswaps, 8, hipMemcpyDeviceToHost, st));
+    PG_HIP(ctx, hipMemcpyAsync(&fl, flags, 4, hipMemcpyDeviceToHost, st));
     PG_HIP(ctx, hipStreamSynchronize(st));
+    if (fl & 8) return pg_fail(ctx, PG_EDEVICE, "internal error: the rewiring changed a vertex degree");
     if (swaps) *swaps = (int64_t)ns;
     return PG_OK;
 }

@@ -180,3 +180,17 @@ def test_rejects_bad_input(ctx):
     c4[row[0]], c4[row[0] + 1] = c4[row[0] + 1], c4[row[0]]
     with pytest.raises(PgError, match="ascending"):
         ctx.rewire(row, c4, seed=1)
+
+
+def test_full_size_network(ctx):
+    # the N=58,000, T=240, 1%-sparsity network (33.6M entries): every invariant
+    # at full size, and nearly all of the 2M attempts succeed at this density
+    from pagani_amd import synth
+    w = synth.WORKLOADS["corr_n58000_t240"]
+    x = synth.bold(w["seed"], w["n"], w["t"], w["k"])
+    info = ctx.corr_build(x, w["n"], w["t"], 1.0, sparsity=True)
+    row, col, _ = ctx.corr_csr(w["n"], info.nnz)
+    say(f"rewire full-size network n={w['n']} nnz={col.size}")
+    oc, _, sw = ctx.rewire(row, col, seed=12)
+    check_rewired(row, col, None, oc, None)
+    assert sw > 0.9 * col.size

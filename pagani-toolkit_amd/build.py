@@ -4,7 +4,7 @@
 
 Outputs:
     pagani-toolkit_amd/lib/libpagani.so
-    pagani-toolkit_amd/bin/{CUCorMat,CUBC,Cp,Degree,CUBFW_Lp,CUBFS_Lp,CUmodularity,CUEC,PC_CPU,ConvertNII}
+    pagani-toolkit_amd/bin/{CUCorMat,CUBC,Cp,Degree,CUBFW_Lp,CUBFS_Lp,CUmodularity,CUEC,PC_CPU,ConvertNII,Maslov,SmallWorldProperty}
 """
 from __future__ import annotations
 
@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-TOOLS = ["CUCorMat", "CUBC", "Cp", "Degree", "CUBFW_Lp", "CUBFS_Lp", "CUmodularity", "CUEC", "PC_CPU", "ConvertNII", "Maslov"]
+TOOLS = ["CUCorMat", "CUBC", "Cp", "Degree", "CUBFW_Lp", "CUBFS_Lp", "CUmodularity", "CUEC", "PC_CPU", "ConvertNII", "Maslov", "SmallWorldProperty"]
 
 
 def run(cmd):

@@ -60,6 +60,7 @@ def test_csr_name_matches_reference_printf():
     ("Cp", ["/nonexistent", "-3", "gk"]),                            # negative random-network count
     ("CUBFW_Lp", ["/nonexistent", "-1", "gl"]), ("CUmodularity", ["/nonexistent", "-1"]),
     ("Maslov", []), ("Maslov", ["net.txt"]), ("Maslov", ["a.csr", "b.csr"]),
+    ("SmallWorldProperty", []), ("SmallWorldProperty", ["a", "b"]),
 ])
 def test_tools_reject_bad_arguments(tool, args):
     exe = os.path.join(BIN, tool)
@@ -119,3 +120,22 @@ def test_convertnii_tool(tmp_path, dt, thr):
                              float(thr) if thr else 0.5)
     assert (tmp_path / "py.nii").read_bytes() == out
     assert not formats.nm_to_nii(str(d / "sub02_deg.nm"), str(tmp_path / "mask.nii"), str(tmp_path / "q.nii"))
+
+
+def test_small_world_property_tool(tmp_path):
+    exe = os.path.join(BIN, "SmallWorldProperty")
+    if not os.path.exists(exe):
+        pytest.skip("tools not built")
+    files = {"sub01_1317_spa1.000%_cor0.300_lambda.txt": "1.25", "sub01_1317_spa1.000%_cor0.300_gamma.txt": "2.5",
+             "sub02_1317_spa2.000%_cor0.250_lambda.txt": "1.1", "sub02_1317_spa2.000%_cor0.250_gamma.txt": "3.3",
+             "sub03_1317_spa5.000%_cor0.100_lambda.txt": "1.0"}  # no gamma: left alone
+    for name, v in files.items():
+        (tmp_path / name).write_text(v + "\n")
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    sw1 = float((tmp_path / "sub01_1317_spa1.000%_cor0.300_smallWorld.txt").read_text())
+    sw2 = float((tmp_path / "sub02_1317_spa2.000%_cor0.250_smallWorld.txt").read_text())
+    assert sw1 == 2.0
+    assert abs(sw2 - 3.3 / 1.1) < 5e-6 * 3.0  # six significant digits
+    assert not (tmp_path / "sub03_1317_spa5.000%_cor0.100_smallWorld.txt").exists()
+    assert "no matching gamma" in r.stdout

@@ -247,6 +247,9 @@ def test_random_network_outputs(volume, tmp_path, ctx):
     want = [O.bfs_efficiency(row, col)[1]] + [O.bfs_efficiency(row, oc)[1] for oc in rnd[:2]]
     np.testing.assert_allclose(lps, want, rtol=5e-6)
     assert abs(_lines(base + "_lambda.txt")[0] / (want[0] / np.mean(want[1:])) - 1) < 5e-6
+    run("SmallWorldProperty", b)  # sigma = gamma / lambda from the two files above
+    sigma = _lines(base + "_gamma.txt")[0] / _lines(base + "_lambda.txt")[0]
+    assert abs(_lines(base + "_smallWorld.txt")[0] / sigma - 1) < 5e-6
     run("Maslov", b / f, env=env)
     r2, c2, v2 = formats.read_csr(base + "_Maslov.csr")
     assert v2 is None

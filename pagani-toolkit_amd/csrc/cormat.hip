@@ -205,11 +205,25 @@ __global__ void gram64_recheck(const double *__restrict__ z, int n, int t, int t
 }
 
 // post_block (U CorMat.cpp:806-838): acc += r, or its Fisher z
+// 16-byte accesses (4 pairs per thread and step), the last m % 4 pairs one by
+// one; the buffers are device allocations, so 16-byte aligned.
 __global__ void cm_accumulate(long long m, const float *__restrict__ r, int fisher, float *__restrict__ acc)
 {
 #pragma clang fp contract(off)
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < m;
-         q += (long long)gridDim.x * blockDim.x) {
+    const long long m4 = m >> 2, stride = (long long)gridDim.x * blockDim.x;
+    const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const float4 *r4 = (const float4 *)r;
+    float4 *a4 = (float4 *)acc;
+    for (long long q = t0; q < m4; q += stride) {
+        const float4 x = r4[q];
+        float4 y = a4[q];
+        y.x = y.x + (fisher ? fisher_f(x.x) : x.x);
+        y.y = y.y + (fisher ? fisher_f(x.y) : x.y);
+        y.z = y.z + (fisher ? fisher_f(x.z) : x.z);
+        y.w = y.w + (fisher ? fisher_f(x.w) : x.w);
+        a4[q] = y;
+    }
+    for (long long q = 4 * m4 + t0; q < m; q += stride) {
         const float x = r[q];
         acc[q] = acc[q] + (fisher ? fisher_f(x) : x);
     }
@@ -236,8 +250,24 @@ __global__ void cm_finish(long long m, float *__restrict__ a, int nsub, int fish
 {
 #pragma clang fp contract(off)
     const float s = (float)nsub;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < m;
-         q += (long long)gridDim.x * blockDim.x) {
+    const long long m4 = m >> 2, stride = (long long)gridDim.x * blockDim.x;
+    const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    float4 *a4 = (float4 *)a;
+    for (long long q = t0; q < m4; q += stride) {  // 16-byte accesses, as cm_accumulate
+        float4 v = a4[q];
+        v.x = v.x / s;
+        v.y = v.y / s;
+        v.z = v.z / s;
+        v.w = v.w / s;
+        if (fisher) {
+            v.x = inv_fisher_f(v.x);
+            v.y = inv_fisher_f(v.y);
+            v.z = inv_fisher_f(v.z);
+            v.w = inv_fisher_f(v.w);
+        }
+        a4[q] = v;
+    }
+    for (long long q = 4 * m4 + t0; q < m; q += stride) {
         float x = a[q] / s;
         if (fisher) x = inv_fisher_f(x);
         a[q] = x;

@@ -431,10 +431,28 @@ __global__ __launch_bounds__(256) void cm_csr_group(const uint8_t *__restrict__ 
         const int j = j0 + tid;
         u64 mask = 0;
         if (j < jend) {
-            // rows r with i0 + r > j; their bytes are contiguous in packed row j
+            // rows r with i0 + r > j; their bytes are contiguous in packed row j:
+            // fetched as at most five aligned 16-byte words (only those that
+            // overlap [b, b + cnt), so no read passes b + cnt + 15), then
+            // unpacked by compile-time byte position
             const int r0 = j >= i0 ? j - i0 + 1 : 0;
             const long long b = pidx(j, i0 + r0, n);
-            for (int r = r0; r < nr; r++) mask |= (u64)(keep[b + (r - r0)] != 0) << r;
+            const long long b16 = b & ~15ll;
+            const int off = (int)(b - b16), cnt = nr - r0;
+            uint4 wd[5];
+#pragma unroll
+            for (int q = 0; q < 5; q++)
+                wd[q] = 16 * q < off + cnt ? *(const uint4 *)(keep + b16 + 16 * q) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                const unsigned w4[4] = {wd[q].x, wd[q].y, wd[q].z, wd[q].w};
+#pragma unroll
+                for (int p = 0; p < 16; p++) {
+                    const int r = 16 * q + p - off + r0;  // row of byte 16q + p
+                    const bool set = ((w4[p >> 2] >> (8 * (p & 3))) & 0xffu) != 0u;
+                    if (set && r >= r0 && r < nr) mask |= 1ull << r;
+                }
+            }
         }
         for (int r = 0; r < nr; r++) {
             const u64 bm = __ballot((mask >> r) & 1ull);

@@ -479,26 +479,32 @@ __global__ __launch_bounds__(256) void cm_csr_group(const uint8_t *__restrict__ 
         if (tid < nr) cl[i0 + tid] = run[tid];
         return;
     }
-    // upper parts, one row at a time: packed segment of row i, j = i+1 .. n-1
-    for (int r = 0; r < nr; r++) {
+    // upper parts: packed segment of row i, j = i+1 .. n-1, one row per wave
+    // (no block barriers), 64 bytes per step and four steps in flight, each
+    // kept column placed by its rank in the step's ballot
+    for (int r = wv; r < nr; r += 4) {
         const int i = i0 + r;
         const long long b = pidx(i, i + 1, n);
         const int len = n - 1 - i;
         long long pos = rowptr[i] + cl[i];
-        for (int q0 = 0; q0 < len; q0 += 256) {
-            const int q = q0 + tid;
-            const bool k = q < len && keep[b + q];
-            const u64 bm = __ballot(k);
-            if (lane == 0) wcnt[wv][0] = __popcll(bm);
-            __syncthreads();
-            if (k) {
-                long long at = pos + __popcll(bm & ((1ull << lane) - 1ull));
-                for (int w = 0; w < wv; w++) at += wcnt[w][0];
-                col[at] = i + 1 + q;
-                if (val) val[at] = a[b + q];
+        for (int q0 = 0; q0 < len; q0 += 4 * 64) {
+            bool k[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int q = q0 + 64 * u + lane;
+                k[u] = q < len && keep[b + q];
             }
-            pos += wcnt[0][0] + wcnt[1][0] + wcnt[2][0] + wcnt[3][0];
-            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const u64 bm = __ballot(k[u]);
+                if (k[u]) {
+                    const int q = q0 + 64 * u + lane;
+                    const long long at = pos + __popcll(bm & ((1ull << lane) - 1ull));
+                    col[at] = i + 1 + q;
+                    if (val) val[at] = a[b + q];
+                }
+                pos += __popcll(bm);
+            }
         }
     }
 }

@@ -376,38 +376,36 @@ __global__ __launch_bounds__(256) void cm_keep(const float *__restrict__ a, int 
                                                const long long *__restrict__ tbase, long long ntie,
                                                uint8_t *__restrict__ keep, int *__restrict__ cu)
 {
-    __shared__ int wsum[4];
-    __shared__ int kept[4];
-    const int i = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // one packed row per wave, 64 pairs per step and four steps in flight; tie
+    // ranks advance in column order by ballot, with no block barriers
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + wv;
+    if (i >= n) return;
     const long long base = pidx(i, i + 1, n);
     const int len = n - 1 - i;
     long long rank = ties ? tbase[i] : 0;
     int nk = 0;
-    for (int q0 = 0; q0 < len; q0 += 256) {
-        const int q = q0 + tid;
-        float x = 0.0f;
-        bool gt = false, tie = false;
-        if (q < len) {
-            x = a[base + q];
-            gt = x > bound;
-            tie = ties && x == bound;
+    for (int q0 = 0; q0 < len; q0 += 4 * 64) {
+        float x[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int q = q0 + 64 * u + lane;
+            x[u] = q < len ? a[base + q] : 0.0f;
         }
-        const u64 tm = __ballot(tie);
-        const int before = __popcll(tm & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[wv] = __popcll(tm);
-        __syncthreads();
-        long long r = rank + before;
-        for (int w = 0; w < wv; w++) r += wsum[w];
-        const bool k = gt || (tie && r < ntie);
-        if (q < len) keep[base + q] = k ? 1 : 0;
-        const u64 km = __ballot(k);
-        if (lane == 0) kept[wv] = __popcll(km);
-        __syncthreads();
-        rank += wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        nk += kept[0] + kept[1] + kept[2] + kept[3];
-        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int q = q0 + 64 * u + lane;
+            const bool in = q < len;
+            const bool gt = in && x[u] > bound, tie = in && ties && x[u] == bound;
+            const u64 tm = __ballot(tie);
+            const long long r = rank + __popcll(tm & ((1ull << lane) - 1ull));
+            const bool k = gt || (tie && r < ntie);
+            if (in) keep[base + q] = k ? 1 : 0;
+            nk += __popcll(__ballot(k));
+            rank += __popcll(tm);
+        }
     }
-    if (tid == 0) cu[i] = nk;
+    if (lane == 0) cu[i] = nk;
 }
 
 // One work-group per 64 CSR rows [i0, i0 + 64).  Count pass (rowptr == NULL):
@@ -694,7 +692,8 @@ extern "C" pg_status pg_cormat_threshold(pg_ctx *ctx, int32_t thr_kind, float th
     }
     uint8_t *keep;
     PG_TRY(pg_bufT(ctx, "cm.keep", (size_t)m1, &keep));
-    cm_keep<<<(unsigned)n, 256, 0, st>>>(a, n, bound, thr_kind == PG_THR_SPARSITY, tbase, ntie, keep, cu);
+    cm_keep<<<(unsigned)pg_cdiv(n, 4), 256, 0, st>>>(a, n, bound, thr_kind == PG_THR_SPARSITY, tbase, ntie, keep,
+                                                      cu);
     const int ng = (int)pg_cdiv(n, 64);
     cm_csr_group<<<(unsigned)ng, 256, 0, st>>>(keep, a, n, nullptr, cl, nullptr, nullptr);
     cm_degrees<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(n, cl, cu, deg);

@@ -343,6 +343,7 @@ __global__ __launch_bounds__(256) void cm_row_counts(const float *__restrict__ a
     const long long base = pidx(i, i + 1, n);
     const int len = n - 1 - i;
     int g = 0, e = 0;
+#pragma unroll 4
     for (int q = threadIdx.x; q < len; q += 256) {
         const float x = a[base + q];
         g += x > bound;

@@ -257,10 +257,14 @@ def run_weighted(a):
     out["cp_onnela_ms"] = dt * 1e3
     out["cp_onnela_mean"] = mean
     log(f"onnela {dt * 1e3:.1f} ms")
-    (lab, q, lam, eig, rounds), dt = _timed(ctx, lambda: ctx.modularity(row, col, val))
-    out["modularity"] = {"seconds": dt, "q": q, "lambda0": lam, "rounds": rounds,
-                         "communities": int(lab.max())}
-    log(f"modularity {dt:.2f} s, Q={q:.4f}, {int(lab.max())} communities")
+    stops = {"rayleigh": pagani_amd.MOD_STOP_RAYLEIGH, "vector": pagani_amd.MOD_STOP_VECTOR}
+    for name in a.mod_stop.split(","):
+        log(f"modularity ({name} stop) ...")
+        (lab, q, lam, eig, rounds), dt = _timed(
+            ctx, lambda: ctx.modularity(row, col, val, pagani_amd.MOD_STRENGTH, stop=stops[name], progress=True))
+        out[f"modularity_{name}"] = {"seconds": dt, "q": q, "lambda0": lam, "rounds": rounds,
+                                     "communities": int(lab.max())}
+        log(f"modularity ({name} stop) {dt:.2f} s, Q={q:.4f}, {int(lab.max())} communities, {rounds} rounds")
     ctx.close()
     return out
 
@@ -321,6 +325,8 @@ def main(argv=None) -> int:
                          "weighted: N=120000 r>0.3 weighted clustering + modularity")
     ap.add_argument("--subjects", type=int, default=64, help="cohort size (--config cohort)")
     ap.add_argument("--voxels", type=int, default=0, help="voxel count override (--config weighted)")
+    ap.add_argument("--mod-stop", default="rayleigh",
+                    help="modularity stopping rules to run, comma separated: rayleigh, vector (--config weighted)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the construction path "

@@ -85,6 +85,9 @@ typedef struct pg_corr_info {
     double ms_zscore, ms_gram, ms_select, ms_csr;  /* device time per stage */
     double ms_gram_hist;  /* histogram Gram pass kernel alone (sparsity mode) */
     double ms_gram_emit;  /* emitting Gram pass kernel alone (last launch) */
+    double band_max_err;  /* largest |r_fast - r_fp64| over the rechecked pairs; the
+                             build fails (PG_EDEVICE) if it exceeds band / 1.5, the
+                             bound before its safety factor */
 } pg_corr_info;
 
 /* Voxel time series -> thresholded CSR adjacency, on the device.
@@ -167,6 +170,12 @@ pg_status pg_cp_weighted(pg_ctx *ctx, const int32_t *row, const int32_t *col,
 pg_status pg_bc(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
                 int64_t nnz, const int32_t *sources, int64_t nsrc, double *bc);
 
+/* pg_bc with the reference's output type: the FP64 sums rounded to float,
+ * for callers of CUBC(float *BC, int *row, int *col, int N, int nnz)
+ * (U/BC/CUBC/CUBC.cpp:13), which fills N floats. */
+pg_status pg_bc_float(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
+                      int64_t nnz, const int32_t *sources, int64_t nsrc, float *bc);
+
 /* BFS hop levels for nsrc sources: level[s*n + v], -1 where unreachable
  * (the BFS of U/BFS_MulCPU/BFS_MulCPU/APSP_BFS.cpp:150-192). */
 pg_status pg_bfs(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
@@ -217,6 +226,29 @@ pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_t *col,
                         const float *val, int32_t n, int64_t nnz, int32_t kmode,
                         int32_t *labels, double *q, double *lambda0, double *eig0,
                         int32_t *rounds);
+
+typedef enum pg_mod_stop {
+    PG_MOD_STOP_VECTOR = 0,   /* the reference: |v - v0| or |v + v0| <= 1e-6 */
+    PG_MOD_STOP_RAYLEIGH = 1  /* relative change of the Rayleigh quotient <= tol */
+} pg_mod_stop;
+
+typedef struct pg_mod_opts {
+    int32_t kmode;        /* pg_mod_kmode */
+    int32_t stop;         /* pg_mod_stop */
+    double tol;           /* PG_MOD_STOP_RAYLEIGH: |rho_k - rho_(k-1)| <= tol |rho_k|
+                             (0 means 1e-8) */
+    int32_t max_iter;     /* power iterations per pass (0 means 10000, the reference's cap) */
+    int32_t progress;     /* 1: one line per wave of rounds on stderr */
+} pg_mod_opts;
+
+/* pg_modularity with a choice of stopping rule.  With PG_MOD_STOP_RAYLEIGH the
+ * quotient rho_k = x.(B + beta I)x / x.x of each iteration (FP64) is tracked
+ * and a group's eigenvalue is the final quotient instead of +-|(B + beta I)x|;
+ * the bisection (shift rule, split rule, label order) is the reference's. */
+pg_status pg_modularity_ex(pg_ctx *ctx, const int32_t *row, const int32_t *col,
+                           const float *val, int32_t n, int64_t nnz, const pg_mod_opts *opts,
+                           int32_t *labels, double *q, double *lambda0, double *eig0,
+                           int32_t *rounds);
 
 /* Eigenvector centrality by the power iteration of CUEC (U/CUEC/CUEC/
  * CUEC_unweighted.cu:61-196; weighted: W/CUEC/weighted-net.cu:55-203): start

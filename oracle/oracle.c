@@ -167,6 +167,30 @@ void orc_corr_csr_r(const double *z64, int64_t n, int t, float thr_eff,
     }
 }
 
+/* Rows rows[0..nrows) of the r-mode CSR above, each formed alone (for
+ * networks too large for the whole matrix): deg[q] entries of row rows[q] go
+ * to col/val[q * n ...], columns ascending. */
+void orc_corr_rows_r(const double *z64, int64_t n, int t, float thr_eff, const int32_t *rows, int64_t nrows,
+                     int32_t *deg, int32_t *col, float *val)
+{
+    const double th = (double)thr_eff;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t q = 0; q < nrows; q++) {
+        const int64_t i = rows[q];
+        int64_t pos = 0;
+        for (int64_t j = 0; j < n; j++) {
+            if (j == i) continue;
+            double r = (i < j) ? orc_r(z64, t, i, j) : orc_r(z64, t, j, i);
+            if (r > th) {
+                col[q * n + pos] = (int32_t)j;
+                if (val) val[q * n + pos] = (float)r;
+                pos++;
+            }
+        }
+        deg[q] = (int32_t)pos;
+    }
+}
+
 /* ------------------------------------------------------- sparsity select */
 
 typedef struct { double r; int64_t idx; } rpair;
@@ -338,50 +362,69 @@ static int find_w(const int32_t *row, const int32_t *col, const float *val,
  * W Cp/Cp/Cp_2.cpp:27-30 and W Cp_GPU/CUCP/Cp.cu:223-239):
  *   C_k = 2 sum_{i<j in N(k), w_ij != 0} (w^_ij w^_ki w^_kj)^(1/3) / deg_k / (deg_k - 1)
  * Both are 0 for deg_k < 2.  Returns the mean. */
-double orc_cp_weighted(const int32_t *row, const int32_t *col, const float *val,
-                       int64_t n, int kind, int wmax_mode, double *cp)
+static double cpw_wmax(const int32_t *row, const float *val, int64_t n, int wmax_mode)
 {
     double wmax = 0.0;
     int64_t nnz = row[n];
     int64_t lim = (wmax_mode == 1) ? (n < nnz ? n : nnz) : nnz;
     if (lim > 0) wmax = val[0];
     for (int64_t e = 1; e < lim; e++) if (val[e] > wmax) wmax = val[e];
+    return wmax;
+}
+
+static double cpw_vertex(const int32_t *row, const int32_t *col, const float *val, int kind, double wmax,
+                         int64_t k)
+{
+    int32_t b = row[k], e = row[k + 1];
+    int64_t deg = e - b;
+    if (deg < 2) return 0.0;
+    double sk = 0.0;
+    for (int32_t q = b; q < e; q++) sk += (double)val[q];
+    double acc = 0.0;
+    for (int32_t p = b; p < e; p++) {
+        int32_t i = col[p];
+        double wki = (double)val[p];
+        /* j in N(k) ∩ N(i), j > i */
+        int32_t a0 = p + 1, a1 = e, b0 = row[i], b1 = row[i + 1];
+        while (a0 < a1 && b0 < b1) {
+            int32_t x = col[a0], y = col[b0];
+            if (x < y) a0++;
+            else if (y < x) b0++;
+            else {
+                double wij = (double)val[b0], wkj = (double)val[a0];
+                if (wij != 0.0) {
+                    if (kind == 1) acc += wki + wkj;
+                    else acc += pow((wij / wmax) * (wki / wmax) * (wkj / wmax), 1.0 / 3.0);
+                }
+                a0++; b0++;
+            }
+        }
+    }
+    if (kind == 1) return acc / (double)(deg - 1) / sk;
+    return 2.0 * acc / (double)deg / (double)(deg - 1);
+}
+
+double orc_cp_weighted(const int32_t *row, const int32_t *col, const float *val,
+                       int64_t n, int kind, int wmax_mode, double *cp)
+{
+    const double wmax = cpw_wmax(row, val, n, wmax_mode);
     double mean = 0.0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : mean)
     for (int64_t k = 0; k < n; k++) {
-        int32_t b = row[k], e = row[k + 1];
-        int64_t deg = e - b;
-        if (deg < 2) { cp[k] = 0.0; continue; }
-        double sk = 0.0;
-        for (int32_t q = b; q < e; q++) sk += (double)val[q];
-        double acc = 0.0;
-        for (int32_t p = b; p < e; p++) {
-            int32_t i = col[p];
-            double wki = (double)val[p];
-            /* j in N(k) ∩ N(i), j > i */
-            int32_t a0 = p + 1, a1 = e, b0 = row[i], b1 = row[i + 1];
-            while (a0 < a1 && b0 < b1) {
-                int32_t x = col[a0], y = col[b0];
-                if (x < y) a0++;
-                else if (y < x) b0++;
-                else {
-                    double wij = (double)val[b0], wkj = (double)val[a0];
-                    if (wij != 0.0) {
-                        if (kind == 1) acc += wki + wkj;
-                        else acc += pow((wij / wmax) * (wki / wmax) * (wkj / wmax), 1.0 / 3.0);
-                    }
-                    a0++; b0++;
-                }
-            }
-        }
-        double c;
-        if (kind == 1) c = acc / (double)(deg - 1) / sk;
-        else c = 2.0 * acc / (double)deg / (double)(deg - 1);
-        cp[k] = c;
-        mean += c;
+        cp[k] = cpw_vertex(row, col, val, kind, wmax, k);
+        mean += cp[k];
     }
     (void)find_w;
     return n > 0 ? mean / (double)n : 0.0;
+}
+
+/* orc_cp_weighted at the vertices verts[0..nv) only (same normalisation). */
+void orc_cp_weighted_at(const int32_t *row, const int32_t *col, const float *val, int64_t n, int kind,
+                        int wmax_mode, const int32_t *verts, int64_t nv, double *cp)
+{
+    const double wmax = cpw_wmax(row, val, n, wmax_mode);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t q = 0; q < nv; q++) cp[q] = cpw_vertex(row, col, val, kind, wmax, verts[q]);
 }
 
 /* -------------------------------------------------------------------- BC */
@@ -569,15 +612,23 @@ typedef struct {
 
 /* W Newman_Modularity_GPU/CUmodularity/Modularity_CPU.cpp:224-298 and
  * modularity_GPU.cu:225-325: power iteration on B^(g) + beta I from
- * v_i = global index, until ||v - v0|| or ||v + v0|| <= 1e-6 or 10000 steps;
- * returns +-||(B + beta I) v0|| with the sign of v0 . v at the largest |v|. */
-static double lead_vector(const mgroup *G, double m2, double beta, double *v, double *v0, int *iters)
+ * v_i = global index, until ||v - v0|| or ||v + v0|| <= 1e-6 or max_iter
+ * (10000) steps; returns +-||(B + beta I) v0|| with the sign of v0 . v at the
+ * largest |v|.  stop = 1 (the Rayleigh rule) instead tracks
+ * rho_k = v0 . (B + beta I) v0 / v0 . v0, stops once two consecutive values
+ * satisfy |rho_k - rho_(k-1)| <= tol |rho_k| (or at max_iter) and returns the
+ * last rho. */
+static double lead_vector(const mgroup *G, double m2, double beta, double *v, double *v0, int *iters,
+                          int stop, double tol, int max_iter)
 {
     int64_t ng = G->ng;
     for (int64_t i = 0; i < ng; i++) v[i] = (double)G->g[i];
-    double err1 = 1.0, err2 = 1.0, vnorm = 0.0;
+    double err1 = 1.0, err2 = 1.0, vnorm = 0.0, rho = 0.0, rho_prev = 0.0;
     int it = 0;
-    while (err1 > 1e-6 && err2 > 1e-6 && it < 10000) {
+    for (;;) {
+        if (it >= max_iter) break;
+        if (stop == 0 && !(err1 > 1e-6 && err2 > 1e-6)) break;
+        if (stop == 1 && it >= 2 && fabs(rho - rho_prev) <= tol * fabs(rho)) break;
         for (int64_t i = 0; i < ng; i++) v0[i] = v[i];
         double vk = 0.0;
         for (int64_t i = 0; i < ng; i++) vk += v0[i] * G->kg[i];
@@ -587,8 +638,14 @@ static double lead_vector(const mgroup *G, double m2, double beta, double *v, do
             acc -= vk / m2 * G->kg[i] + (G->sumbg[i] - beta) * v0[i];
             v[i] = acc;
         }
-        double s = 0.0;
-        for (int64_t i = 0; i < ng; i++) s += v[i] * v[i];
+        double s = 0.0, sxy = 0.0, sxx = 0.0;
+        for (int64_t i = 0; i < ng; i++) {
+            s += v[i] * v[i];
+            sxy += v0[i] * v[i];
+            sxx += v0[i] * v0[i];
+        }
+        rho_prev = rho;
+        rho = sxy / sxx;
         vnorm = sqrt(s);
         for (int64_t i = 0; i < ng; i++) v[i] = v[i] / vnorm;
         double e1 = 0.0, e2 = 0.0;
@@ -602,6 +659,7 @@ static double lead_vector(const mgroup *G, double m2, double beta, double *v, do
     int64_t mi = 0;
     for (int64_t i = 0; i < ng; i++) if (fabs(v[i]) > fabs(v[mi])) mi = i;
     if (iters) *iters += it;
+    if (stop == 1) return rho;
     return (v[mi] * v0[mi] > 0) ? vnorm : -vnorm;
 }
 
@@ -628,6 +686,13 @@ static void node_k(const int32_t *row, const float *val, int64_t n, int kmode, d
 double orc_modularity(const int32_t *row, const int32_t *col, const float *val,
                       int64_t n, int kmode, int32_t *labels, double *lambda_out,
                       double *eig_out, int32_t *nrounds_out)
+{
+    return orc_modularity_ex(row, col, val, n, kmode, 0, 1e-8, 10000, labels, lambda_out, eig_out, nrounds_out);
+}
+
+double orc_modularity_ex(const int32_t *row, const int32_t *col, const float *val, int64_t n, int kmode,
+                         int stop, double tol, int max_iter, int32_t *labels, double *lambda_out,
+                         double *eig_out, int32_t *nrounds_out)
 {
     double *k = (double *)malloc(sizeof(double) * (size_t)(n + 1));
     double m2 = 0.0;
@@ -679,9 +744,9 @@ double orc_modularity(const int32_t *row, const int32_t *col, const float *val,
                 G.sumbg[a] = kin - k[i] * Kg / m2;
             }
             int its = 0;
-            double lam = lead_vector(&G, m2, 0.0, v, v0, &its);
+            double lam = lead_vector(&G, m2, 0.0, v, v0, &its, stop, tol, max_iter);
             int shift = (ng > 25000) ? (lam < -0.01) : (lam < 0.0);
-            if (shift) lam += lead_vector(&G, m2, -lam, v, v0, &its);
+            if (shift) lam += lead_vector(&G, m2, -lam, v, v0, &its, stop, tol, max_iter);
             if (round == 0) {
                 if (lambda_out) *lambda_out = lam;
                 if (eig_out) for (int64_t a = 0; a < ng; a++) eig_out[G.g[a]] = v[a];

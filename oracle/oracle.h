@@ -37,6 +37,8 @@ int64_t orc_corr_count_r(const double *z64, int64_t n, int t, float thr_eff);
 /* r-mode: undirected edge count over rows [i0, i1) only (pairs i < j); a
  * bounded sample of the all-pairs work for CPU timing. */
 int64_t orc_corr_count_rows(const double *z64, int64_t n, int t, float thr_eff, int64_t i0, int64_t i1);
+void orc_corr_rows_r(const double *z64, int64_t n, int t, float thr_eff, const int32_t *rows, int64_t nrows,
+                     int32_t *deg, int32_t *col, float *val);
 
 /* r-mode: CSR.  row[n+1]; col/val sized by orc_corr_count_r()*2. val may be NULL. */
 void orc_corr_csr_r(const double *z64, int64_t n, int t, float thr_eff,
@@ -60,6 +62,8 @@ double orc_cp_binary(const int32_t *row, const int32_t *col, int64_t n, float *c
  * wmax_mode 0 = max over all weights, 1 = max over val[0..n-1] (reference CPU). */
 double orc_cp_weighted(const int32_t *row, const int32_t *col, const float *val,
                        int64_t n, int kind, int wmax_mode, double *cp);
+void orc_cp_weighted_at(const int32_t *row, const int32_t *col, const float *val, int64_t n, int kind,
+                        int wmax_mode, const int32_t *verts, int64_t nv, double *cp);
 
 /* Brandes betweenness over the given sources (all if sources == NULL),
  * uint64 path counts, FP64 dependencies, ordered-pair sum (U BC_CPU.cpp:6-82). */
@@ -92,6 +96,9 @@ void orc_fw_init(const int32_t *row, const int32_t *col, const float *val, int64
 double orc_modularity(const int32_t *row, const int32_t *col, const float *val,
                       int64_t n, int kmode, int32_t *labels, double *lambda_out,
                       double *eig_out, int32_t *nrounds_out);
+double orc_modularity_ex(const int32_t *row, const int32_t *col, const float *val, int64_t n, int kmode,
+                         int stop, double tol, int max_iter, int32_t *labels, double *lambda_out,
+                         double *eig_out, int32_t *nrounds_out);
 
 /* Modularity Q of a given labelling (same k/m convention as kmode). */
 double orc_modularity_q(const int32_t *row, const int32_t *col, const float *val,

@@ -43,6 +43,9 @@ def lib():
         L.orc_corr_count_r.restype = C.c_int64
         L.orc_corr_count_rows.argtypes = [f64p, C.c_int64, C.c_int, C.c_float, C.c_int64, C.c_int64]
         L.orc_corr_count_rows.restype = C.c_int64
+        L.orc_corr_rows_r.argtypes = [f64p, C.c_int64, C.c_int, C.c_float, i32p, C.c_int64, i32p, i32p,
+                                      C.c_void_p]
+        L.orc_cp_weighted_at.argtypes = [i32p, i32p, f32p, C.c_int64, C.c_int, C.c_int, i32p, C.c_int64, f64p]
         L.orc_corr_csr_r.argtypes = [f64p, C.c_int64, C.c_int, C.c_float, i32p, i32p, C.c_void_p]
         L.orc_corr_csr_s.argtypes = [f64p, C.c_int64, C.c_int, C.c_int64, i32p, i32p, C.c_void_p]
         L.orc_corr_csr_s.restype = C.c_double
@@ -63,6 +66,9 @@ def lib():
         L.orc_modularity.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, C.c_int, i32p,
                                      C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_int32)]
         L.orc_modularity.restype = C.c_double
+        L.orc_modularity_ex.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_int,
+                                        i32p, C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_int32)]
+        L.orc_modularity_ex.restype = C.c_double
         L.orc_modularity_q.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, C.c_int, i32p]
         L.orc_modularity_q.restype = C.c_double
         L.orc_ec.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, f64p, f64p, C.POINTER(C.c_int32)]
@@ -142,6 +148,20 @@ def corr_csr_r(z64: np.ndarray, thr: float, weighted: bool = False):
     return row, col[:nnz], (val[:nnz] if weighted else None)
 
 
+def corr_rows_r(z64: np.ndarray, thr: float, rows, weighted: bool = False):
+    """The rows `rows` of the r-threshold CSR, each formed alone: a list of
+    (cols, vals-or-None) per row."""
+    z64 = np.ascontiguousarray(z64)
+    n, t = z64.shape
+    rows = np.ascontiguousarray(rows, np.int32)
+    th = threshold_eff(thr, weighted)
+    deg = np.empty(rows.size, np.int32)
+    col = np.empty((rows.size, n), np.int32)
+    val = np.empty((rows.size, n), np.float32) if weighted else None
+    lib().orc_corr_rows_r(z64, n, t, th, rows, rows.size, deg, col, _ptr(val))
+    return [(col[q, :deg[q]].copy(), None if val is None else val[q, :deg[q]].copy()) for q in range(rows.size)]
+
+
 def corr_csr_s(z64: np.ndarray, s_percent: float, weighted: bool = False):
     """(row, col, val-or-None, t_star) for the sparsity rule."""
     z64 = np.ascontiguousarray(z64)
@@ -192,6 +212,16 @@ def cp_weighted(row, col, val, kind=2, wmax_mode=0):
     return cp, mean
 
 
+def cp_weighted_at(row, col, val, verts, kind=2, wmax_mode=0):
+    """cp_weighted at the vertices verts only."""
+    n = row.size - 1
+    verts = np.ascontiguousarray(verts, np.int32)
+    cp = np.empty(verts.size, np.float64)
+    lib().orc_cp_weighted_at(np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32),
+                             np.ascontiguousarray(val, np.float32), n, kind, wmax_mode, verts, verts.size, cp)
+    return cp
+
+
 def bc(row, col, sources=None):
     n = row.size - 1
     out = np.empty(n, np.float64)
@@ -232,15 +262,18 @@ def fw_blocked(d: np.ndarray, b: int) -> np.ndarray:
     return d
 
 
-def modularity(row, col, val=None, kmode=0):
+def modularity(row, col, val=None, kmode=0, stop=0, tol=1e-8, max_iter=10000):
+    """(labels, Q, lambda0, round-0 vector, rounds); stop 0 = the reference's
+    vector rule, 1 = the Rayleigh-quotient rule with relative tolerance tol."""
     n = row.size - 1
     lab = np.empty(n, np.int32)
     lam = C.c_double(0.0)
     nr = C.c_int32(0)
     eig = np.zeros(n, np.float64)
     v = None if val is None else np.ascontiguousarray(val, np.float32)
-    q = lib().orc_modularity(np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32),
-                             _ptr(v), n, kmode, lab, C.byref(lam), _ptr(eig), C.byref(nr))
+    q = lib().orc_modularity_ex(np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32),
+                                _ptr(v), n, kmode, stop, tol, max_iter, lab, C.byref(lam), _ptr(eig),
+                                C.byref(nr))
     return lab, q, lam.value, eig, nr.value
 
 

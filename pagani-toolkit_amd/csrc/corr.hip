@@ -915,15 +915,27 @@ __device__ __forceinline__ double r64_pair(const double *__restrict__ z, int t, 
     return s > 1.0 ? 1.0 : (s < -1.0 ? -1.0 : s);
 }
 
+// FP64 r of every listed pair; with rfast, also the largest |r~ - r64| over
+// them into *errmax (bit pattern of a non-negative double, whose unsigned
+// order is its numeric order), one atomic per wave.
 __global__ void recheck_kernel(const int2 *__restrict__ pairs, const unsigned long long *__restrict__ cnt,
                                long long cap, const double *__restrict__ z, int t,
-                               double *__restrict__ r_out)
+                               double *__restrict__ r_out, const float *__restrict__ rfast,
+                               unsigned long long *__restrict__ errmax)
 {
     long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long c = (long long)min((unsigned long long)cap, *cnt);
-    if (k >= c) return;
-    int2 p = pairs[k];
-    r_out[k] = r64_pair(z, t, p.x, p.y);
+    double err = 0.0;
+    if (k < c) {
+        int2 p = pairs[k];
+        const double r = r64_pair(z, t, p.x, p.y);
+        r_out[k] = r;
+        if (rfast) err = fabs((double)rfast[k] - r);
+    }
+    if (!rfast) return;
+    for (int o = 32; o > 0; o >>= 1) err = fmax(err, __shfl_xor(err, o));
+    if ((threadIdx.x & 63) == 0 && err > 0.0)
+        atomicMax(errmax, (unsigned long long)__double_as_longlong(err));
 }
 
 // r-mode: keep rechecked pairs with r64 > thr; count decisions that differed
@@ -1732,6 +1744,13 @@ __global__ __launch_bounds__(1024) void definite_sum(const unsigned long long *_
 //  FP32 inside MFMA (at most one rounding per term plus per instruction):
 //  gamma_{3.2t + 16}.  The FP64 side (t products, t adds, 2^-53) is added too.
 //  A factor 1.5 covers the rest.
+double host_err(unsigned long long bits)
+{
+    double e;
+    std::memcpy(&e, &bits, 8);
+    return e;
+}
+
 double error_band(int prec, int t)
 {
     const double u = std::ldexp(1.0, -24);
@@ -2121,7 +2140,8 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)std::max<long long>(nband, 1), &band64));
         PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)std::max<long long>(nband, 1), &extra));
         if (nband > 0) {
-            recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(bandl, counters + 0, band_cap, z64, t, band64);
+            recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(bandl, counters + 0, band_cap, z64, t, band64,
+                                                                          band_r, counters + 5);
             accept_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(
                 bandl, band_r, band64, counters + 0, band_cap, (double)thr_eff, extra, extra_cnt, flips);
         }
@@ -2129,6 +2149,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
         PG_HIP(ctx, hipStreamSynchronize(st));
         inf.band_flips = (long long)hcnt[2];
+        inf.band_max_err = host_err(hcnt[5]);
         inf.nnz = 2 * (definite + (long long)hcnt[1]);
         inf.s_thresh = (float)(m1 > 0 ? 100.0 * (double)inf.nnz / (double)m1 / 2.0 : 0.0);
         es.seg = seg;
@@ -2212,7 +2233,8 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             PG_TRY(pg_bufT(ctx, "corr.dpart", (size_t)wx * nwg, &dpart));
             window_kernel<<<dim3(wx, (unsigned)nwg), 256, 0, st>>>(seg, seg_r, seg_cap, seg_cnt, (int)nwg, ctl, cand,
                                                                     cand_r, ccap, counters + 0, dpart);
-            recheck_kernel<<<(unsigned)pg_cdiv(ccap, 256), 256, 0, st>>>(cand, counters + 0, ccap, z64, t, cand64);
+            recheck_kernel<<<(unsigned)pg_cdiv(ccap, 256), 256, 0, st>>>(cand, counters + 0, ccap, z64, t, cand64, cand_r,
+                                                                         counters + 5);
             select_kernel<<<1, 1024, 0, st>>>(cand, cand64, cand_r, counters + 0, ccap, kwant, dpart,
                                               (int)(wx * nwg), extra, extra_cnt, tstar_d, flips,
                                               (int *)(counters + 4));
@@ -2246,6 +2268,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             if (done) {
                 inf.band_pairs = (long long)hcnt[0];
                 inf.band_flips = (long long)hcnt[2];
+                inf.band_max_err = host_err(hcnt[5]);
             }
             std::memset(&es, 0, sizeof es);
         }
@@ -2272,7 +2295,8 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)std::max<long long>(nband, 1), &band64));
             PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)std::max<long long>(nband, 1), &extra));
             if (nband > 0)
-                recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(bandl, counters + 0, band_cap, z64, t, band64);
+                recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(bandl, counters + 0, band_cap, z64, t,
+                                                                              band64, band_r, counters + 5);
             select_kernel<<<1, 1024, 0, st>>>(bandl, band64, band_r, counters + 0, band_cap, kwant, seg_cnt, (int)nwg,
                                               extra, extra_cnt, tstar_d, flips, (int *)(counters + 4));
             PG_KERNEL_CHECK(ctx);
@@ -2284,6 +2308,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
                                (long long)hcnt[0]);
             inf.band_pairs = nband;
             inf.band_flips = (long long)hcnt[2];
+            inf.band_max_err = host_err(hcnt[5]);
             es.seg = seg;
             es.seg_cap = seg_cap;
             es.seg_cnt = seg_cnt;
@@ -2296,6 +2321,15 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         inf.nnz = 2 * kwant;
         inf.r_thresh = (float)inf.t_star;
     }
+
+    // The band is error_band()'s proven bound times 1.5; a rechecked pair whose
+    // fast r~ is further from its FP64 r than the bound itself means the
+    // rounding model behind the band does not hold on this device, and pairs
+    // outside the band may then be misclassified.
+    if (inf.band_max_err > band / 1.5)
+        return pg_fail(ctx, PG_EDEVICE,
+                       "fast-path error %.3g exceeds the proven bound %.3g (band %.3g): edge set not guaranteed",
+                       inf.band_max_err, band / 1.5, band);
 
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);

@@ -344,21 +344,27 @@ extern "C" pg_status pg_bfs_efficiency(pg_ctx *ctx, const int32_t *row, const in
 }
 
 namespace {
-// bad = 1 unless row is non-decreasing from 0 to nnz and every col is in
-// [0, n); sorted: also strictly ascending columns inside each row
-__global__ void csr_validate(const int *__restrict__ row, const int *__restrict__ col, int n, long long nnz,
-                             int sorted, int *__restrict__ bad)
+// bad = 1 unless row is non-decreasing from row[0] = 0 to row[n] = nnz and
+// every col is in [0, n); sorted: also strictly ascending columns inside each
+// row.  One wave per row; a row whose bounds fall outside [0, nnz] is flagged
+// before any of its columns is read, so the check itself stays in bounds.
+// col == nullptr checks the row offsets only.
+__global__ __launch_bounds__(256) void csr_validate(const int *__restrict__ row, const int *__restrict__ col, int n,
+                                                    long long nnz, int sorted, int *__restrict__ bad)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
     bool ok = true;
-    if (t < n) {
-        const int a = row[t], b = row[t + 1];
-        ok = a <= b;
-        if (ok && sorted)
-            for (int e = a + 1; e < b; e++) ok = ok && col[e - 1] < col[e];
+    if (w < n) {
+        const long long a = row[w], b = row[w + 1];
+        ok = a >= 0 && a <= b && b <= nnz;
+        if (ok && col)
+            for (long long e = a + lane; e < b; e += 64) {
+                const int c = col[e];
+                ok = ok && c >= 0 && c < n && !(sorted && e > a && col[e - 1] >= c);
+            }
     }
-    if (t < nnz) ok = ok && col[t] >= 0 && col[t] < n;
-    if (t == 0) ok = ok && row[0] == 0 && (long long)row[n] == nnz;
+    if (w == 0 && lane == 0) ok = ok && row[0] == 0 && (long long)row[n] == nnz;
     if (!ok) atomicOr(bad, 1);
 }
 }  // namespace
@@ -370,8 +376,8 @@ pg_status pg_validate_csr(pg_ctx *ctx, const int32_t *drow, const int32_t *dcol,
     int *bad, hbad = 0;
     PG_TRY(pg_bufT(ctx, "csr.bad", 1, &bad));
     PG_HIP(ctx, hipMemsetAsync(bad, 0, 4, st));
-    const long long m = std::max<long long>(std::max<long long>(n, nnz), 1);
-    csr_validate<<<(unsigned)pg_cdiv(m, 256), 256, 0, st>>>(drow, dcol, n, nnz, sorted ? 1 : 0, bad);
+    const long long waves = std::max<long long>(n, 1);
+    csr_validate<<<(unsigned)pg_cdiv(waves, 4), 256, 0, st>>>(drow, dcol, n, nnz, sorted ? 1 : 0, bad);
     PG_KERNEL_CHECK(ctx);
     PG_HIP(ctx, hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
     PG_HIP(ctx, hipStreamSynchronize(st));

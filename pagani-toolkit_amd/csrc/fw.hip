@@ -298,6 +298,7 @@ extern "C" pg_status pg_fw_init(pg_ctx *ctx, const int32_t *row, const int32_t *
     PG_TRY(pg_in(ctx, row, (size_t)n + 1, "in.row", &drow));
     PG_TRY(pg_in(ctx, col, (size_t)nnz, "in.col", &dcol));
     if (val) PG_TRY(pg_in(ctx, val, (size_t)nnz, "in.val", &dval));
+    PG_TRY(pg_validate_csr(ctx, drow, dcol, n, nnz, false));
     const long long nn = (long long)n * n;
     float *dd;
     PG_TRY(pg_out(ctx, d, (size_t)nn, "out.fw", &dd));

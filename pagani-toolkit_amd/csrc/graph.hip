@@ -482,10 +482,12 @@ extern "C" pg_status pg_strength(pg_ctx *ctx, const int32_t *row, const float *v
     int nnz = 0;
     if (pg_is_device_ptr(row)) PG_HIP(ctx, hipMemcpy(&nnz, row + n, 4, hipMemcpyDeviceToHost));
     else nnz = row[n];
+    if (nnz < 0) return pg_fail(ctx, PG_EFORMAT, "row[n]=%d", nnz);
     const int *drow;
     const float *dval;
     float *ds;
     PG_TRY(pg_in(ctx, row, (size_t)n + 1, "in.row", &drow));
+    PG_TRY(pg_validate_csr(ctx, drow, nullptr, n, nnz, false));  // offsets only
     PG_TRY(pg_in(ctx, val, (size_t)nnz, "in.val", &dval));
     PG_TRY(pg_out(ctx, s, (size_t)n, "out.s", &ds));
     if (n > 0) strength_kernel<<<(unsigned)pg_cdiv(n, 256), 256, 0, ctx->stream>>>(drow, dval, n, ds);
@@ -496,14 +498,17 @@ extern "C" pg_status pg_strength(pg_ctx *ctx, const int32_t *row, const float *v
 }
 
 namespace {
+// Device views of a caller's CSR, checked on the device before any kernel
+// indexes with it (PG_EFORMAT for offsets outside [0, nnz], columns outside
+// [0, n), and with `sorted` columns not strictly ascending in a row).
 pg_status stage_csr(pg_ctx *ctx, const int32_t *row, const int32_t *col, const float *val, int32_t n,
-                    int64_t nnz, const int **drow, const int **dcol, const float **dval)
+                    int64_t nnz, bool sorted, const int **drow, const int **dcol, const float **dval)
 {
     PG_TRY(pg_check_csr_args(ctx, row, col, n, nnz));
     PG_TRY(pg_in(ctx, row, (size_t)n + 1, "in.row", drow));
     PG_TRY(pg_in(ctx, col, (size_t)nnz, "in.col", dcol));
     if (dval) PG_TRY(pg_in(ctx, val, (size_t)nnz, "in.val", dval));
-    return PG_OK;
+    return pg_validate_csr(ctx, *drow, *dcol, n, nnz, sorted);
 }
 }  // namespace
 
@@ -514,7 +519,7 @@ extern "C" pg_status pg_triangles(pg_ctx *ctx, const int32_t *row, const int32_t
     if (!tri) return pg_fail(ctx, PG_EINVAL, "tri is NULL");
     PG_HIP(ctx, hipSetDevice(ctx->device));
     const int *drow, *dcol;
-    PG_TRY(stage_csr(ctx, row, col, nullptr, n, nnz, &drow, &dcol, nullptr));
+    PG_TRY(stage_csr(ctx, row, col, nullptr, n, nnz, true, &drow, &dcol, nullptr));
     const int nw = (n + 31) / 32;
     if ((size_t)nw * 4 + 4096 > ctx->lds_per_block) return pg_fail(ctx, PG_ENOTSUP, "n too large for LDS bitmap");
     long long *dtri;
@@ -533,7 +538,7 @@ extern "C" pg_status pg_cp_binary(pg_ctx *ctx, const int32_t *row, const int32_t
     if (!cp) return pg_fail(ctx, PG_EINVAL, "cp is NULL");
     PG_HIP(ctx, hipSetDevice(ctx->device));
     const int *drow, *dcol;
-    PG_TRY(stage_csr(ctx, row, col, nullptr, n, nnz, &drow, &dcol, nullptr));
+    PG_TRY(stage_csr(ctx, row, col, nullptr, n, nnz, true, &drow, &dcol, nullptr));
     const int nw = (n + 31) / 32;
     if ((size_t)nw * 4 + 4096 > ctx->lds_per_block) return pg_fail(ctx, PG_ENOTSUP, "n too large for LDS bitmap");
     long long *dtri;
@@ -565,7 +570,7 @@ extern "C" pg_status pg_cp_weighted(pg_ctx *ctx, const int32_t *row, const int32
     PG_HIP(ctx, hipSetDevice(ctx->device));
     const int *drow, *dcol;
     const float *dval = nullptr;
-    PG_TRY(stage_csr(ctx, row, col, val, n, nnz, &drow, &dcol, &dval));
+    PG_TRY(stage_csr(ctx, row, col, val, n, nnz, true, &drow, &dcol, &dval));
     const int nw = (n + 31) / 32;
     if ((size_t)nw * 8 + 4096 > ctx->lds_per_block) return pg_fail(ctx, PG_ENOTSUP, "n too large for LDS bitmap");
     double *dcp, *dmean;
@@ -592,7 +597,7 @@ extern "C" pg_status pg_bc(pg_ctx *ctx, const int32_t *row, const int32_t *col, 
     if (!bc) return pg_fail(ctx, PG_EINVAL, "bc is NULL");
     PG_HIP(ctx, hipSetDevice(ctx->device));
     const int *drow, *dcol;
-    PG_TRY(stage_csr(ctx, row, col, nullptr, n, nnz, &drow, &dcol, nullptr));
+    PG_TRY(stage_csr(ctx, row, col, nullptr, n, nnz, false, &drow, &dcol, nullptr));
     std::vector<int32_t> hsrc;
     if (sources) {
         if (nsrc < 0) return pg_fail(ctx, PG_EINVAL, "nsrc < 0");
@@ -632,6 +637,32 @@ extern "C" pg_status pg_bc(pg_ctx *ctx, const int32_t *row, const int32_t *col, 
     return PG_OK;
 }
 
+namespace {
+__global__ void to_float(const double *__restrict__ x, int n, float *__restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (float)x[i];
+}
+}  // namespace
+
+extern "C" pg_status pg_bc_float(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n, int64_t nnz,
+                                 const int32_t *sources, int64_t nsrc, float *bc)
+{
+    if (!ctx) return PG_EINVAL;
+    if (!bc || n < 0) return pg_fail(ctx, PG_EINVAL, "bad arguments");
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    double *acc;
+    PG_TRY(pg_bufT(ctx, "bcf.acc", (size_t)n + 1, &acc));
+    PG_TRY(pg_bc(ctx, row, col, n, nnz, sources, nsrc, acc));
+    float *dbc;
+    PG_TRY(pg_out(ctx, bc, (size_t)n, "out.bcf", &dbc));
+    if (n > 0) to_float<<<(unsigned)pg_cdiv(n, 256), 256, 0, ctx->stream>>>(acc, n, dbc);
+    PG_KERNEL_CHECK(ctx);
+    PG_TRY(pg_out_finish(ctx, bc, dbc, (size_t)n));
+    PG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PG_OK;
+}
+
 extern "C" pg_status pg_bfs(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n, int64_t nnz,
                             const int32_t *sources, int64_t nsrc, int32_t *level)
 {
@@ -639,7 +670,7 @@ extern "C" pg_status pg_bfs(pg_ctx *ctx, const int32_t *row, const int32_t *col,
     if (!level || !sources || nsrc < 0) return pg_fail(ctx, PG_EINVAL, "bad arguments");
     PG_HIP(ctx, hipSetDevice(ctx->device));
     const int *drow, *dcol;
-    PG_TRY(stage_csr(ctx, row, col, nullptr, n, nnz, &drow, &dcol, nullptr));
+    PG_TRY(stage_csr(ctx, row, col, nullptr, n, nnz, false, &drow, &dcol, nullptr));
     std::vector<int32_t> hsrc((size_t)nsrc);
     if (pg_is_device_ptr(sources)) PG_HIP(ctx, hipMemcpy(hsrc.data(), sources, nsrc * 4, hipMemcpyDeviceToHost));
     else std::memcpy(hsrc.data(), sources, nsrc * 4);

@@ -29,6 +29,7 @@ namespace {
 
 constexpr int MAX_ITER = 10000;      // modularity_GPU.cu:25
 constexpr double EPS = 1e-6;         // modularity_GPU.cu:28
+constexpr double RAYLEIGH_TOL = 1e-8;  // default relative tolerance of the quotient stop
 constexpr int TPB = 256;             // threads per block
 constexpr int NPB_MAX = 1024;        // nodes per work block, upper limit
 constexpr int GSUM_MAX = 512;        // target for block partials per group
@@ -49,6 +50,9 @@ struct Wave {
     const float *sval;
     const double *k, *sumbg;
     double m2;
+    int stopmode;          // pg_mod_stop
+    double tol;            // Rayleigh-quotient tolerance
+    int max_iter;          // iterations per pass
     // per group state
     int ngroups;
     int *stop, *iters, *it0;
@@ -57,6 +61,9 @@ struct Wave {
     // vectors: iteration li of a group reads buf[li & 1] and leaves its
     // normalised result in buf[(li + 1) & 1]; per-block partials
     double *buf[2], *vres, *pdot, *pss, *pe1, *pe2;
+    // Rayleigh quotient: per-block partials of x.y and x.x (phase S), and the
+    // quotient of iteration li of group g in rho[2 g + (li & 1)] (phase N)
+    double *pxy, *pxx, *rho;
 };
 
 __device__ double block_sum(double x, double *red)
@@ -109,9 +116,21 @@ __device__ __forceinline__ void phase_s(const Wave &w, int it, PhaseSmem &sm)
         __syncthreads();
         if (skip) continue;
         if (li > 0) {
-            const double e1 = sqrt(group_sum(w.pe1, w.fb[g], w.nb[g], sm.red));
-            const double e2 = sqrt(group_sum(w.pe2, w.fb[g], w.nb[g], sm.red));
-            if (!(e1 > EPS && e2 > EPS) || li >= MAX_ITER) {
+            bool done;
+            if (w.stopmode == PG_MOD_STOP_VECTOR) {
+                const double e1 = sqrt(group_sum(w.pe1, w.fb[g], w.nb[g], sm.red));
+                const double e2 = sqrt(group_sum(w.pe2, w.fb[g], w.nb[g], sm.red));
+                done = !(e1 > EPS && e2 > EPS);
+            } else {
+                // quotients of the last two completed iterations (written by
+                // earlier launches)
+                done = false;
+                if (li >= 2) {
+                    const double ra = w.rho[2 * g + ((li - 1) & 1)], rb = w.rho[2 * g + ((li - 2) & 1)];
+                    done = fabs(ra - rb) <= w.tol * fabs(ra);
+                }
+            }
+            if (done || li >= w.max_iter) {
                 if (b == w.fb[g] && threadIdx.x == 0) {
                     w.iters[g] = li;
                     w.stop[g] = 1;
@@ -131,15 +150,25 @@ __device__ __forceinline__ void phase_s(const Wave &w, int it, PhaseSmem &sm)
             if (lane == 0) sm.rowv[i - i0] = acc;
         }
         __syncthreads();
-        double s = 0.0;
+        double s = 0.0, sxy = 0.0, sxx = 0.0;
         for (int i = i0 + threadIdx.x; i < i1; i += TPB) {
             double acc = sm.rowv[i - i0];
             acc -= vk / w.m2 * w.k[i] + (w.sumbg[i] - beta) * x[i];
             y[i] = acc;
             s += acc * acc;
+            sxy += x[i] * acc;
+            sxx += x[i] * x[i];
         }
         s = block_sum(s, sm.red);
         if (threadIdx.x == 0) w.pss[b] = s;
+        if (w.stopmode == PG_MOD_STOP_RAYLEIGH) {
+            sxy = block_sum(sxy, sm.red);
+            sxx = block_sum(sxx, sm.red);
+            if (threadIdx.x == 0) {
+                w.pxy[b] = sxy;
+                w.pxx[b] = sxx;
+            }
+        }
     }
 }
 
@@ -171,6 +200,12 @@ __device__ __forceinline__ void phase_n(const Wave &w, int it, PhaseSmem &sm)
             w.pe2[b] = p2;
             w.pdot[b] = pk;
             if (b == w.fb[g]) w.vnorm[g] = vn;
+        }
+        if (w.stopmode == PG_MOD_STOP_RAYLEIGH && b == w.fb[g]) {
+            // this iteration's quotient from phase S's partials (fixed order)
+            const double xy = group_sum(w.pxy, w.fb[g], w.nb[g], sm.red);
+            const double xx = group_sum(w.pxx, w.fb[g], w.nb[g], sm.red);
+            if (threadIdx.x == 0) w.rho[2 * g + (li & 1)] = xy / xx;
         }
     }
 }
@@ -209,7 +244,7 @@ __global__ __launch_bounds__(TPB) void mod_start(Wave w)
 
 // For the groups in list[]: lam[g] = +-|v| by the sign of v[mi] v0[mi] at the
 // first largest |v[mi]| (the reference's strict '>' scan), v = the final
-// vector, v0 = the one before; vres[] = v.
+// vector, v0 = the one before (Rayleigh stop: the last quotient); vres[] = v.
 __global__ __launch_bounds__(TPB) void mod_lambda(Wave w, const int *list)
 {
     __shared__ double rv[TPB];
@@ -239,7 +274,8 @@ __global__ __launch_bounds__(TPB) void mod_lambda(Wave w, const int *list)
     if (tid == 0) {
         const int mi = ri[0];
         const double vn = w.vnorm[g];
-        w.lam[g] = (mi < z && v[mi] * v0[mi] > 0) ? vn : -vn;
+        if (w.stopmode == PG_MOD_STOP_RAYLEIGH) w.lam[g] = w.rho[2 * g + ((li - 1) & 1)];
+        else w.lam[g] = (mi < z && v[mi] * v0[mi] > 0) ? vn : -vn;
     }
 }
 
@@ -249,9 +285,26 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
                                    int32_t n, int64_t nnz, int32_t kmode, int32_t *labels, double *q,
                                    double *lambda0, double *eig0, int32_t *rounds)
 {
+    pg_mod_opts o;
+    std::memset(&o, 0, sizeof o);
+    o.kmode = kmode;
+    o.stop = PG_MOD_STOP_VECTOR;
+    return pg_modularity_ex(ctx, row, col, val, n, nnz, &o, labels, q, lambda0, eig0, rounds);
+}
+
+extern "C" pg_status pg_modularity_ex(pg_ctx *ctx, const int32_t *row, const int32_t *col, const float *val,
+                                      int32_t n, int64_t nnz, const pg_mod_opts *opts, int32_t *labels,
+                                      double *q, double *lambda0, double *eig0, int32_t *rounds)
+{
     if (!ctx) return PG_EINVAL;
-    if (!labels) return pg_fail(ctx, PG_EINVAL, "labels is NULL");
+    if (!labels || !opts) return pg_fail(ctx, PG_EINVAL, "labels/opts is NULL");
+    const int kmode = opts->kmode;
     if (kmode != PG_MOD_STRENGTH && kmode != PG_MOD_BINARY_K) return pg_fail(ctx, PG_EINVAL, "bad kmode");
+    if (opts->stop != PG_MOD_STOP_VECTOR && opts->stop != PG_MOD_STOP_RAYLEIGH)
+        return pg_fail(ctx, PG_EINVAL, "bad stop rule %d", opts->stop);
+    if (opts->tol < 0.0 || opts->max_iter < 0) return pg_fail(ctx, PG_EINVAL, "bad tol / max_iter");
+    const double tol = opts->tol > 0.0 ? opts->tol : RAYLEIGH_TOL;
+    const int max_iter = opts->max_iter > 0 ? opts->max_iter : MAX_ITER;
     PG_TRY(pg_check_csr_args(ctx, row, col, n, nnz));
     PG_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
@@ -267,7 +320,13 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
     PG_TRY(fetch(R.data(), row, ((size_t)n + 1) * 4));
     PG_TRY(fetch(Cc.data(), col, (size_t)nnz * 4));
     if (val) PG_TRY(fetch(V.data(), val, (size_t)nnz * 4));
-    if (R[n] != nnz) return pg_fail(ctx, PG_EFORMAT, "row[n]=%d != nnz=%lld", R[n], (long long)nnz);
+    // the driver indexes host and device arrays with these offsets and columns
+    if (R[0] != 0 || R[n] != nnz) return pg_fail(ctx, PG_EFORMAT, "row[0]=%d row[n]=%d, nnz=%lld", R[0], R[n], (long long)nnz);
+    for (int i = 0; i < n; i++)
+        if (R[i] > R[i + 1]) return pg_fail(ctx, PG_EFORMAT, "row offsets decrease at %d", i);
+    for (int64_t e = 0; e < nnz; e++)
+        if (Cc[(size_t)e] < 0 || Cc[(size_t)e] >= n)
+            return pg_fail(ctx, PG_EFORMAT, "column %d at %lld outside [0, %d)", Cc[(size_t)e], (long long)e, n);
 
     std::vector<double> k((size_t)n);
     double m2 = 0.0;
@@ -296,7 +355,7 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
     // PAGANI_MOD_TRACE=1: one line per wave on stderr (groups, nodes, edges,
     // iterations, seconds)
     const char *trace_env = std::getenv("PAGANI_MOD_TRACE");
-    const bool trace = trace_env && trace_env[0] == '1';
+    const bool trace = opts->progress || (trace_env && trace_env[0] == '1');
     while (lo <= maxr) {
         const int hi = maxr + 1;
         const int G = hi - lo;
@@ -358,7 +417,8 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
         int *d_bgrp, *d_no, *d_ng, *d_fb, *d_nb, *d_gid, *d_srow, *d_scol, *d_restart, *d_stop, *d_iters, *d_it0,
             *d_list;
         float *d_sval;
-        double *d_k, *d_sumbg, *d_beta, *d_vnorm, *d_lam, *d_b0, *d_b1, *d_vres, *d_pdot, *d_pss, *d_pe1, *d_pe2;
+        double *d_k, *d_sumbg, *d_beta, *d_vnorm, *d_lam, *d_b0, *d_b1, *d_vres, *d_pdot, *d_pss, *d_pe1, *d_pe2,
+            *d_pxy, *d_pxx, *d_rho;
         PG_TRY(pg_bufT(ctx, "mod.bgrp", (size_t)nblk, &d_bgrp));
         PG_TRY(pg_bufT(ctx, "mod.no", (size_t)G, &d_no));
         PG_TRY(pg_bufT(ctx, "mod.ng", (size_t)G, &d_ng));
@@ -385,6 +445,9 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
         PG_TRY(pg_bufT(ctx, "mod.pss", (size_t)nblk, &d_pss));
         PG_TRY(pg_bufT(ctx, "mod.pe1", (size_t)nblk, &d_pe1));
         PG_TRY(pg_bufT(ctx, "mod.pe2", (size_t)nblk, &d_pe2));
+        PG_TRY(pg_bufT(ctx, "mod.pxy", (size_t)nblk, &d_pxy));
+        PG_TRY(pg_bufT(ctx, "mod.pxx", (size_t)nblk, &d_pxx));
+        PG_TRY(pg_bufT(ctx, "mod.rho", (size_t)G * 2, &d_rho));
         auto h2d = [&](void *dst, const void *src, size_t bytes) -> pg_status {
             if (bytes) PG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
             return PG_OK;
@@ -422,6 +485,9 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
         w.k = d_k;
         w.sumbg = d_sumbg;
         w.m2 = m2;
+        w.stopmode = opts->stop;
+        w.tol = tol;
+        w.max_iter = max_iter;
         w.ngroups = G;
         w.stop = d_stop;
         w.restart = d_restart;
@@ -437,6 +503,9 @@ extern "C" pg_status pg_modularity(pg_ctx *ctx, const int32_t *row, const int32_
         w.pss = d_pss;
         w.pe1 = d_pe1;
         w.pe2 = d_pe2;
+        w.pxy = d_pxy;
+        w.pxx = d_pxx;
+        w.rho = d_rho;
 
         // pass state per group: 0 first pass, 1 shifted pass, 2 finished
         std::vector<int> pass((size_t)G, 0), list, hit0((size_t)G, 0);

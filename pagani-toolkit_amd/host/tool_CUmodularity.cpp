@@ -3,9 +3,12 @@
 // modularity_GPU.cu:527-720).  Output: <file>_modu.nm (float module labels
 // 1..K) and <file>_modu.txt (module count and Q).
 //
-//   CUmodularity [--strength] dir_for_csr num_of_random_networks
+//   CUmodularity [--strength] [--rayleigh[=tol]] dir_for_csr num_of_random_networks
 // Default degrees are binary with weighted adjacency, as the reference tool;
-// --strength uses strengths and the total weight instead.
+// --strength uses strengths and the total weight instead.  --rayleigh stops
+// each power iteration when the Rayleigh quotient changes by at most tol
+// (default 1e-8) relative, instead of the reference's vector test.  Networks
+// of 20000 nodes or more print one progress line per wave of rounds.
 // With num_of_random_networks > 0 each network's degree-preserving rewirings
 // (pg_rewire, weights carried with their edges; network l seeded with
 // PAGANI_SEED + l, wall clock when unset) are partitioned too, as the
@@ -20,13 +23,23 @@
 int main(int argc, char **argv)
 {
     int a0 = 1;
-    int kmode = PG_MOD_BINARY_K;
-    if (a0 < argc && !std::strcmp(argv[a0], "--strength")) {
-        kmode = PG_MOD_STRENGTH;
-        a0++;
+    pg_mod_opts mo;
+    std::memset(&mo, 0, sizeof mo);
+    mo.kmode = PG_MOD_BINARY_K;
+    mo.stop = PG_MOD_STOP_VECTOR;
+    for (; a0 < argc && argv[a0][0] == '-' && argv[a0][1] == '-'; a0++) {
+        if (!std::strcmp(argv[a0], "--strength")) {
+            mo.kmode = PG_MOD_STRENGTH;
+        } else if (!std::strncmp(argv[a0], "--rayleigh", 10) && (argv[a0][10] == 0 || argv[a0][10] == '=')) {
+            mo.stop = PG_MOD_STOP_RAYLEIGH;
+            if (argv[a0][10] == '=') mo.tol = std::atof(argv[a0] + 11);
+        } else {
+            std::cerr << "unknown option " << argv[a0] << "\n";
+            return 1;
+        }
     }
     if (argc - a0 != 2) {
-        std::cerr << "Input format: CUmodularity [--strength] dir_for_csr num_of_random_networks \n"
+        std::cerr << "Input format: CUmodularity [--strength] [--rayleigh[=tol]] dir_for_csr num_of_random_networks \n"
                      "For example: CUmodularity /data 0" << std::endl;
         return 1;
     }
@@ -57,9 +70,11 @@ int main(int argc, char **argv)
         // modules of a network into lab; returns Q
         auto partition = [&](const pgio::Csr &h, int32_t *rnd) {
             double q = 0.0;
-            ctx.check(pg_modularity(ctx.c, h.row.data(), h.col.data(), weighted ? h.val.data() : nullptr, h.n,
-                                    (int64_t)h.col.size(), kmode, lab.data(), &q, nullptr, nullptr, rnd),
-                      "pg_modularity");
+            pg_mod_opts o = mo;
+            o.progress = h.n >= 20000;
+            ctx.check(pg_modularity_ex(ctx.c, h.row.data(), h.col.data(), weighted ? h.val.data() : nullptr, h.n,
+                                       (int64_t)h.col.size(), &o, lab.data(), &q, nullptr, nullptr, rnd),
+                      "pg_modularity_ex");
             return q;
         };
         pgtool::Timer tm;

@@ -20,6 +20,8 @@ from .lib import (  # noqa: F401
     CP_ONNELA,
     MOD_STRENGTH,
     MOD_BINARY_K,
+    MOD_STOP_VECTOR,
+    MOD_STOP_RAYLEIGH,
     library,
 )
 from . import synth  # noqa: F401

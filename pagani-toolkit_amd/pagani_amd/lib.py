@@ -19,6 +19,7 @@ THR_R, THR_SPARSITY = 0, 1
 LAYOUT_TIME_MAJOR, LAYOUT_VOXEL_MAJOR = 0, 1
 CP_BARRAT, CP_ONNELA = 1, 2
 MOD_STRENGTH, MOD_BINARY_K = 0, 1
+MOD_STOP_VECTOR, MOD_STOP_RAYLEIGH = 0, 1
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "out of device memory", 3: "device error",
           4: "malformed input", 5: "threshold error", 6: "not supported", 7: "bad call order"}
@@ -35,13 +36,19 @@ class _Opts(C.Structure):
                 ("prec", C.c_int32), ("layout", C.c_int32)]
 
 
+class _ModOpts(C.Structure):
+    _fields_ = [("kmode", C.c_int32), ("stop", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32),
+                ("progress", C.c_int32)]
+
+
 class _Info(C.Structure):
     _fields_ = [("n", C.c_int32), ("t", C.c_int32), ("nnz", C.c_int64),
                 ("r_thresh", C.c_float), ("s_thresh", C.c_float), ("t_star", C.c_double),
                 ("band", C.c_double), ("band_pairs", C.c_int64), ("band_flips", C.c_int64),
                 ("gram_passes", C.c_int64), ("ms_zscore", C.c_double), ("ms_gram", C.c_double),
                 ("ms_select", C.c_double), ("ms_csr", C.c_double),
-                ("ms_gram_hist", C.c_double), ("ms_gram_emit", C.c_double)]
+                ("ms_gram_hist", C.c_double), ("ms_gram_emit", C.c_double),
+                ("band_max_err", C.c_double)]
 
 
 @dataclass
@@ -62,6 +69,7 @@ class CorrInfo:
     ms_csr: float
     ms_gram_hist: float
     ms_gram_emit: float
+    band_max_err: float
 
 
 _LIB = None
@@ -85,6 +93,7 @@ _SIGS = {
     "pg_cp_binary": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP]),
     "pg_cp_weighted": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_int32, C.c_int32, _VP, _VP]),
     "pg_bc": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, C.c_int64, _VP]),
+    "pg_bc_float": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, C.c_int64, _VP]),
     "pg_bfs": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, C.c_int64, _VP]),
     "pg_bfs_efficiency": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP]),
     "pg_eigenvector_centrality": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP]),
@@ -99,6 +108,8 @@ _SIGS = {
     "pg_fw_init": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_int32, _VP]),
     "pg_fw_efficiency": (C.c_int, [_VP, _VP, C.c_int32, _VP, _VP, _VP]),
     "pg_modularity": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.c_int32, _VP, _VP, _VP, _VP, _VP]),
+    "pg_modularity_ex": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int32, C.c_int64, C.POINTER(_ModOpts), _VP, _VP, _VP,
+                                   _VP, _VP]),
 }
 
 
@@ -273,6 +284,15 @@ class Context:
         self._check(self._lib.pg_bc(self._h, _ptr(row), _ptr(col), n, _len(col), _ptr(sources), ns, _ptr(out)))
         return out
 
+    def bc_float(self, row, col, sources=None, out=None):
+        """pg_bc rounded to float32, the output type of CUBC."""
+        n = _len(row) - 1
+        out = np.empty(n, np.float32) if out is None else out
+        ns = 0 if sources is None else _len(sources)
+        self._check(self._lib.pg_bc_float(self._h, _ptr(row), _ptr(col), n, _len(col), _ptr(sources), ns,
+                                          _ptr(out)))
+        return out
+
     def bfs(self, row, col, sources, out=None):
         n = _len(row) - 1
         ns = _len(sources)
@@ -343,13 +363,17 @@ class Context:
                                                 C.cast(C.pointer(iso), _VP)))
         return eff, lp.value, iso.value
 
-    def modularity(self, row, col, val=None, kmode: int = MOD_STRENGTH):
+    def modularity(self, row, col, val=None, kmode: int = MOD_STRENGTH, stop: int = MOD_STOP_VECTOR,
+                   tol: float = 0.0, max_iter: int = 0, progress: bool = False):
+        """(labels, Q, lambda0, round-0 vector, rounds): Partition_GPU with the
+        reference's stopping rule (MOD_STOP_VECTOR) or the Rayleigh-quotient rule."""
         n = _len(row) - 1
         labels = np.empty(n, np.int32)
         eig = np.empty(n, np.float64)
         q, lam, nr = C.c_double(0), C.c_double(0), C.c_int32(0)
-        self._check(self._lib.pg_modularity(self._h, _ptr(row), _ptr(col), _ptr(val), n, _len(col), kmode,
-                                            _ptr(labels), C.cast(C.pointer(q), _VP),
-                                            C.cast(C.pointer(lam), _VP), _ptr(eig),
-                                            C.cast(C.pointer(nr), _VP)))
+        o = _ModOpts(kmode, stop, tol, max_iter, int(progress))
+        self._check(self._lib.pg_modularity_ex(self._h, _ptr(row), _ptr(col), _ptr(val), n, _len(col), C.byref(o),
+                                               _ptr(labels), C.cast(C.pointer(q), _VP),
+                                               C.cast(C.pointer(lam), _VP), _ptr(eig),
+                                               C.cast(C.pointer(nr), _VP)))
         return labels, q.value, lam.value, eig, nr.value

@@ -83,6 +83,30 @@ def test_rmode_threshold_brute_force():
         np.testing.assert_array_equal(np.diff(row), dense.sum(1))
 
 
+def test_rows_alone_equal_whole_csr():
+    # the per-row form (used where the whole matrix is too large) gives the
+    # same rows, values and order as the whole CSR, which is pinned above
+    x = _x(260, 80, 5, 9)
+    z64, _ = O.zscore(x)
+    rows = np.array([0, 1, 77, 130, 259], np.int32)
+    for thr, weighted in [(0.2, False), (0.2, True), (-0.05, True)]:
+        row, col, val = O.corr_csr_r(z64, thr, weighted)
+        got = O.corr_rows_r(z64, thr, rows, weighted)
+        for q, i in enumerate(rows):
+            np.testing.assert_array_equal(got[q][0], col[row[i]:row[i + 1]])
+            if weighted:
+                np.testing.assert_array_equal(got[q][1].view(np.uint32), val[row[i]:row[i + 1]].view(np.uint32))
+
+
+def test_weighted_cp_at_vertices_equals_whole():
+    row, col, val = random_graph(120, 0.2, 5, weighted=True)
+    verts = np.array([3, 0, 119, 57, 57], np.int32)
+    for kind in (1, 2):
+        for wm in (0, 1):
+            full, _ = O.cp_weighted(row, col, val, kind, wm)
+            np.testing.assert_array_equal(O.cp_weighted_at(row, col, val, verts, kind, wm), full[verts])
+
+
 def test_weighted_threshold_is_float_rounded():
     # W CorMat.cpp:310 passes r_thresh - 1e-6 into a float parameter
     for thr in [0.2, 0.25, 0.3, 0.504]:
@@ -331,6 +355,45 @@ def test_modularity_leading_vector_matches_eigh():
     assert abs(lam - w[-1]) < 1e-5
     vv = v[:, -1] * np.sign(v[:, -1] @ eig)
     np.testing.assert_allclose(eig, vv, atol=1e-4)
+
+
+def _bipartites(spec):
+    """Disjoint complete bipartite graphs K_{a,b}."""
+    edges, o = [], 0
+    for a, b in spec:
+        edges += [(o + i, o + a + j) for i in range(a) for j in range(b)]
+        o += a + b
+    return csr_from_edges(o, edges)
+
+
+@pytest.mark.parametrize("graph", ["cliques", "gnp_weighted", "bipartite"])
+def test_modularity_rayleigh_stop_matches_eigh(graph):
+    # the quotient rule: lambda0 is the final quotient, which converges to the
+    # largest eigenvalue of B; the complete bipartite graph's B has a dominant
+    # negative eigenvalue, so its round 0 takes the shifted second pass
+    if graph == "cliques":
+        row, col, val = cliques([12, 9, 15])
+        val = None
+    elif graph == "gnp_weighted":
+        row, col, val = random_graph(90, 0.12, 4, weighted=True)
+    else:
+        row, col, val = _bipartites([(8, 8), (3, 3)])  # eigenvalues -8 ... 3.616
+        val = None
+    n = row.size - 1
+    a = _dense(row, col, val)
+    k = a.sum(1)
+    bmat = a - np.outer(k, k) / k.sum()
+    w = np.linalg.eigvalsh(bmat)
+    lab, q, lam, eig, nr = O.modularity(row, col, val, stop=1, tol=1e-12)
+    assert abs(lam - w[-1]) <= 1e-6 * abs(w[-1])
+    # the returned vector is an eigenvector: small residual of B v - lambda v
+    assert np.linalg.norm(bmat @ eig - lam * eig) <= 1e-3 * abs(lam)
+    if graph == "bipartite":
+        assert w[0] < -abs(w[-1])  # the shifted pass was needed
+    if graph == "cliques":
+        lab0, q0, *_ = O.modularity(row, col)
+        np.testing.assert_array_equal(lab, lab0)
+        assert q == q0
 
 
 def _dense_csr_numpy(keep_upper, n):

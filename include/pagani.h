@@ -99,6 +99,21 @@ typedef struct pg_corr_info {
 pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, int32_t t,
                         const pg_corr_opts *opts, pg_corr_info *info);
 
+/* Several thresholds from one Gram pass (U CorMat.cpp:300-346 computes r
+ * once and applies every threshold of its command line).  pg_corr_emit runs
+ * the z-score and ONE emitting Gram pass that keeps, with its fast r~, every
+ * pair the loosest threshold opts->thr can keep (r-mode: the smallest r;
+ * sparsity: the largest percent).  pg_corr_select then forms the CSR of one
+ * threshold of the same kind at least as strict -- the FP64 recheck of the
+ * kept pairs near it and the CSR build, no Gram pass -- with the same edge set
+ * as pg_corr_build, and leaves it for pg_corr_csr / pg_corr_csr_view.  The
+ * kept pairs stay until the next pg_corr_emit or pg_corr_build; a looser
+ * threshold than the emitted one is PG_ETHRESH.  pg_corr_emit's info holds
+ * the z-score and Gram figures and, in band_pairs, the number of kept pairs. */
+pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int32_t t,
+                       const pg_corr_opts *opts, pg_corr_info *info);
+pg_status pg_corr_select(pg_ctx *ctx, float thr, pg_corr_info *info);
+
 /* Copy the last CSR out: row[n+1], col[nnz], val[nnz] (val may be NULL; it is
  * only written for weighted builds).  Host or device destinations. */
 pg_status pg_corr_csr(pg_ctx *ctx, int32_t *row, int32_t *col, float *val);
@@ -152,6 +167,19 @@ pg_status pg_triangles(pg_ctx *ctx, const int32_t *row, const int32_t *col, int3
  * (replaces Cp, U/Cp/Cp/Cp.cpp:21-115).  mean: FP64 mean of cp (may be NULL). */
 pg_status pg_cp_binary(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
                        int64_t nnz, float *cp, double *mean);
+
+/* pg_cp_binary without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, fixed-memory serving loops).  row, col and cp must be device
+ * memory (PG_EINVAL otherwise: staging host buffers would allocate); mean may
+ * be host or device memory or NULL.  n <= PG_CP_STATIC_MAX_N (PG_ENOTSUP
+ * above it).  The per-vertex triangle counts live in one fixed array per GPU
+ * of 8 * PG_CP_STATIC_MAX_N bytes (3.9 MB), held while the library is loaded
+ * and shared by every call on that GPU: calls that overlap on one GPU (two
+ * contexts, two threads) race. */
+#define PG_CP_STATIC_MAX_N 491520
+pg_status pg_cp_binary_static(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
+                              int64_t nnz, float *cp, double *mean);
 
 typedef enum pg_cp_kind { PG_CP_BARRAT = 1, PG_CP_ONNELA = 2 } pg_cp_kind;
 
@@ -238,7 +266,9 @@ typedef struct pg_mod_opts {
     double tol;           /* PG_MOD_STOP_RAYLEIGH: |rho_k - rho_(k-1)| <= tol |rho_k|
                              (0 means 1e-8) */
     int32_t max_iter;     /* power iterations per pass (0 means 10000, the reference's cap) */
-    int32_t progress;     /* 1: one line per wave of rounds on stderr */
+    int32_t progress;     /* 1: one line per bisection level on stderr */
+    int32_t max_depth;    /* 0: bisect until no group splits; d > 0: groups d
+                             bisections deep are final (labels in round order) */
 } pg_mod_opts;
 
 /* pg_modularity with a choice of stopping rule.  With PG_MOD_STOP_RAYLEIGH the

@@ -632,6 +632,8 @@ static double lead_vector(const mgroup *G, double m2, double beta, double *v, do
         for (int64_t i = 0; i < ng; i++) v0[i] = v[i];
         double vk = 0.0;
         for (int64_t i = 0; i < ng; i++) vk += v0[i] * G->kg[i];
+        /* rows in parallel, each row's sum in CSR order (deterministic) */
+#pragma omp parallel for schedule(dynamic, 256)
         for (int64_t i = 0; i < ng; i++) {
             double acc = 0.0;
             for (int32_t e = G->srow[i]; e < G->srow[i + 1]; e++) acc += v0[G->scol[e]] * G->sval[e];
@@ -687,11 +689,15 @@ double orc_modularity(const int32_t *row, const int32_t *col, const float *val,
                       int64_t n, int kmode, int32_t *labels, double *lambda_out,
                       double *eig_out, int32_t *nrounds_out)
 {
-    return orc_modularity_ex(row, col, val, n, kmode, 0, 1e-8, 10000, labels, lambda_out, eig_out, nrounds_out);
+    return orc_modularity_ex(row, col, val, n, kmode, 0, 1e-8, 10000, 0, labels, lambda_out, eig_out,
+                             nrounds_out);
 }
 
+/* max_depth > 0: a group max_depth bisections below the whole network is not
+ * split again (it keeps one label, given in round order like any final
+ * group). */
 double orc_modularity_ex(const int32_t *row, const int32_t *col, const float *val, int64_t n, int kmode,
-                         int stop, double tol, int max_iter, int32_t *labels, double *lambda_out,
+                         int stop, double tol, int max_iter, int max_depth, int32_t *labels, double *lambda_out,
                          double *eig_out, int32_t *nrounds_out)
 {
     double *k = (double *)malloc(sizeof(double) * (size_t)(n + 1));
@@ -700,6 +706,7 @@ double orc_modularity_ex(const int32_t *row, const int32_t *col, const float *va
     int32_t *res = (int32_t *)calloc((size_t)(n + 1), sizeof(int32_t));
     int32_t *adj = (int32_t *)malloc(sizeof(int32_t) * (size_t)(2 * n + 2));
     int32_t *loc = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    int32_t *depth = (int32_t *)calloc((size_t)(2 * n + 4), sizeof(int32_t));
     int64_t nnz = row[n];
     mgroup G;
     G.g = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
@@ -722,7 +729,10 @@ double orc_modularity_ex(const int32_t *row, const int32_t *col, const float *va
         int64_t ng = 0;
         double Kg = 0.0;
         for (int64_t i = 0; i < n; i++) if (res[i] == round) { loc[i] = (int32_t)ng; G.g[ng++] = (int32_t)i; Kg += k[i]; }
-        if (ng > 0) {
+        if (ng > 0 && max_depth > 0 && depth[round] >= max_depth) {
+            adj[round] = index++;
+            for (int64_t a = 0; a < ng; a++) loc[G.g[a]] = -1;
+        } else if (ng > 0) {
             rounds++;
             G.ng = ng;
             int64_t pos = 0;
@@ -757,6 +767,7 @@ double orc_modularity_ex(const int32_t *row, const int32_t *col, const float *va
             if (split) {
                 for (int64_t a = 0; a < ng; a++)
                     res[G.g[a]] = maxr + 1 + (v[a] * ((double)(sp - sn) + 0.5) <= 0);
+                depth[maxr + 1] = depth[maxr + 2] = depth[round] + 1;
                 maxr += 2;
             } else {
                 adj[round] = index++;
@@ -769,7 +780,7 @@ double orc_modularity_ex(const int32_t *row, const int32_t *col, const float *va
     if (nrounds_out) *nrounds_out = rounds;
 done:;
     double q = orc_modularity_q(row, col, val, n, kmode, labels);
-    free(k); free(res); free(adj); free(loc);
+    free(k); free(res); free(adj); free(loc); free(depth);
     free(G.g); free(G.srow); free(G.scol); free(G.sval); free(G.kg); free(G.sumbg);
     free(v); free(v0);
     return q;

@@ -97,7 +97,7 @@ double orc_modularity(const int32_t *row, const int32_t *col, const float *val,
                       int64_t n, int kmode, int32_t *labels, double *lambda_out,
                       double *eig_out, int32_t *nrounds_out);
 double orc_modularity_ex(const int32_t *row, const int32_t *col, const float *val, int64_t n, int kmode,
-                         int stop, double tol, int max_iter, int32_t *labels, double *lambda_out,
+                         int stop, double tol, int max_iter, int max_depth, int32_t *labels, double *lambda_out,
                          double *eig_out, int32_t *nrounds_out);
 
 /* Modularity Q of a given labelling (same k/m convention as kmode). */

@@ -67,7 +67,7 @@ def lib():
                                      C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_int32)]
         L.orc_modularity.restype = C.c_double
         L.orc_modularity_ex.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_int,
-                                        i32p, C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_int32)]
+                                        C.c_int, i32p, C.POINTER(C.c_double), C.c_void_p, C.POINTER(C.c_int32)]
         L.orc_modularity_ex.restype = C.c_double
         L.orc_modularity_q.argtypes = [i32p, i32p, C.c_void_p, C.c_int64, C.c_int, i32p]
         L.orc_modularity_q.restype = C.c_double
@@ -262,9 +262,10 @@ def fw_blocked(d: np.ndarray, b: int) -> np.ndarray:
     return d
 
 
-def modularity(row, col, val=None, kmode=0, stop=0, tol=1e-8, max_iter=10000):
+def modularity(row, col, val=None, kmode=0, stop=0, tol=1e-8, max_iter=10000, max_depth=0):
     """(labels, Q, lambda0, round-0 vector, rounds); stop 0 = the reference's
-    vector rule, 1 = the Rayleigh-quotient rule with relative tolerance tol."""
+    vector rule, 1 = the Rayleigh-quotient rule with relative tolerance tol;
+    max_depth > 0 stops the bisection that many levels down."""
     n = row.size - 1
     lab = np.empty(n, np.int32)
     lam = C.c_double(0.0)
@@ -272,7 +273,7 @@ def modularity(row, col, val=None, kmode=0, stop=0, tol=1e-8, max_iter=10000):
     eig = np.zeros(n, np.float64)
     v = None if val is None else np.ascontiguousarray(val, np.float32)
     q = lib().orc_modularity_ex(np.ascontiguousarray(row, np.int32), np.ascontiguousarray(col, np.int32),
-                                _ptr(v), n, kmode, stop, tol, max_iter, lab, C.byref(lam), _ptr(eig),
+                                _ptr(v), n, kmode, stop, tol, max_iter, max_depth, lab, C.byref(lam), _ptr(eig),
                                 C.byref(nr))
     return lab, q, lam.value, eig, nr.value
 

@@ -1794,10 +1794,10 @@ long long sampled_pairs(const std::vector<int2> &stl, long long stride, int nr, 
     return tot;
 }
 
-extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, int32_t t,
-                                   const pg_corr_opts *opts, pg_corr_info *info)
+namespace {
+
+pg_status check_opts(pg_ctx *ctx, const float *bold, int32_t n, int32_t t, const pg_corr_opts *opts)
 {
-    if (!ctx) return PG_EINVAL;
     if (!bold || !opts || n < 0 || t < 2) return pg_fail(ctx, PG_EINVAL, "bad arguments (n=%d t=%d)", n, t);
     if (opts->prec != PG_PREC_FP32 && opts->prec != PG_PREC_BF16X3)
         return pg_fail(ctx, PG_EINVAL, "unknown precision %d", opts->prec);
@@ -1812,121 +1812,159 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     const int nw = (n + 31) / 32;
     if ((size_t)nw * 8 + 64 > ctx->lds_per_block)
         return pg_fail(ctx, PG_ENOTSUP, "n=%d voxels exceeds the row-sort LDS bitmap", n);
-    PG_HIP(ctx, hipSetDevice(ctx->device));
-    hipStream_t st = ctx->stream;
-    const int prec = opts->prec == PG_PREC_BF16X3 ? PREC_BF16X3 : PREC_F32;
-    const bool weighted = opts->weighted != 0;
+    return PG_OK;
+}
 
-    pg_corr_info inf;
-    std::memset(&inf, 0, sizeof inf);
-    inf.n = n;
-    inf.t = t;
-    const long long m1 = (long long)n * (n - 1) / 2;
+// k of the sparsity rule with the reference's float arithmetic
+// (U CorMat.cpp:311-313): Clength = (long long)(2 M1 s / 100) rounded up to
+// even, k = Clength / 2.
+long long sparsity_k(long long m1, float s)
+{
+    float prod = (float)(2 * m1) * s;
+    long long clen = (long long)((double)prod / 100.0);
+    clen += clen % 2;
+    return clen / 2;
+}
 
-    // threshold bookkeeping, as the reference names its outputs
-    // (U CorMat.cpp:306-313,333; W CorMat.cpp:310)
-    long long kwant = 0;
-    float thr_eff = opts->thr;
-    if (smode) {
-        float prod = (float)(2 * m1) * opts->thr;
-        long long clen = (long long)((double)prod / 100.0);
-        clen += clen % 2;
-        kwant = clen / 2;
-        inf.s_thresh = opts->thr;
-    } else {
-        thr_eff = weighted ? (float)((double)opts->thr - 1e-6) : opts->thr;
-        inf.r_thresh = opts->thr;
-        inf.t_star = (double)thr_eff;
-    }
+// r threshold as the edge test applies it (W CorMat.cpp:310: the weighted
+// tool passes thr - 1e-6 into a float)
+float thr_effective(float thr, bool weighted) { return weighted ? (float)((double)thr - 1e-6) : thr; }
 
-    // ---- z-score and MFMA panels
+// One construction's geometry, device buffers and launch helpers, shared by
+// pg_corr_build and the pg_corr_emit / pg_corr_select pair.
+struct CorrRun {
+    pg_ctx *ctx = nullptr;
+    hipStream_t st = nullptr;
+    int n = 0, t = 0, prec = PREC_BF16X3;
+    bool weighted = false, g2path = true;
+    int tm = 0, tn = 0, rt = 0, nr = 0, npad = 0, nc = 0, ldp = 0, nbk = 0, nsrc = 0;
+    long long m1 = 0, nst = 0, nwg = 0, nwg_hist = 0;
+    double band = 0.0;
+    double *z64 = nullptr;
+    void *p0 = nullptr, *p1 = nullptr;
+    std::vector<int2> stl;
+    GramArgs ga;
+    size_t stage = 0;
+    unsigned long long *counters = nullptr, *seg_cnt = nullptr, *htot = nullptr;
+    int2 *seg = nullptr, *bandl = nullptr;
+    float *seg_r = nullptr, *band_r = nullptr;
+    long long seg_cap = 0, band_cap = 0;
+    int *row = nullptr, *col = nullptr;
+    float *val = nullptr;
+    bool ran_hist = false, ran_emit = false;
+    long long gram_passes = 0;
+    std::vector<unsigned long long> hseg;
+    unsigned long long hcnt[16] = {};
+
+    // z-score and MFMA panels of the series (events ev[0], ev[1] around it),
+    // supertile list and Gram arguments.
     // FP32: register-staged kernel, TM x TN tiles, voxel-major panel [npad][ldp].
     // BF16x3: gram2_kernel, 256 x 128 tiles, chunk-major panels of 16-deep
     // time chunks (at least two chunks).
-    const bool g2path = prec == PREC_BF16X3;
-    const int tm = g2path ? g2::TM : TM, tn = g2path ? g2::TN : TN, rt = tm / tn;
-    const int nr = (int)pg_cdiv(std::max(n, 1), tm);  // row tiles
-    const int npad = nr * tm;
-    const int nc = npad / tn;                           // column tiles
-    const int ldp = g2path ? (int)std::max<long long>(2 * g2::KC, pg_cdiv(t, g2::KC) * g2::KC)
-                           : (int)pg_cdiv(t, KC) * KC;
-    const long long nt = (long long)t;
-    double *mu, *sd, *z64;
-    PG_TRY(pg_bufT(ctx, "corr.mu", n + 1, &mu));
-    PG_TRY(pg_bufT(ctx, "corr.sd", n + 1, &sd));
-    PG_TRY(pg_bufT(ctx, "corr.z64", (size_t)n * nt + 1, &z64));
-    void *p0 = nullptr, *p1 = nullptr;
-    if (prec == PREC_F32) {
-        PG_TRY(pg_buf(ctx, "corr.p32", (size_t)npad * ldp * 4, &p0));
-        PG_HIP(ctx, hipMemsetAsync(p0, 0, (size_t)npad * ldp * 4, st));
-    } else {
-        PG_TRY(pg_buf(ctx, "corr.phi", (size_t)npad * ldp * 2, &p0));
-        PG_TRY(pg_buf(ctx, "corr.plo", (size_t)npad * ldp * 2, &p1));
-        PG_HIP(ctx, hipMemsetAsync(p0, 0, (size_t)npad * ldp * 2, st));
-        PG_HIP(ctx, hipMemsetAsync(p1, 0, (size_t)npad * ldp * 2, st));
-    }
-    const float *dbold = nullptr;
-    PG_TRY(pg_in(ctx, bold, (size_t)n * nt, "corr.bold", &dbold));
-
-    PG_HIP(ctx, hipEventRecord(ctx->ev[0], st));
-    if (n > 0 && t <= ZS_TMAX) {
-        const size_t shz = (size_t)t * (ZV + 1) * 4;
-        PG_HIP(ctx, hipFuncSetAttribute((const void *)zscore_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shz));
-        zscore_fused<<<(unsigned)pg_cdiv(n, ZV), 256, shz, st>>>(
-            dbold, opts->layout, n, t, z64, prec == PREC_F32 ? (float *)p0 : nullptr,
-            prec == PREC_BF16X3 ? (uint16_t *)p0 : nullptr, prec == PREC_BF16X3 ? (uint16_t *)p1 : nullptr, ldp,
-            npad);
-        PG_KERNEL_CHECK(ctx);
-    } else if (n > 0) {
-        zscore_stats<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(dbold, opts->layout, n, t, mu, sd);
-        dim3 g((unsigned)pg_cdiv(n, 64), (unsigned)pg_cdiv(t, 64));
-        zscore_write<<<g, 256, 0, st>>>(dbold, opts->layout, n, t, mu, sd, z64,
-                                         prec == PREC_F32 ? (float *)p0 : nullptr,
-                                         prec == PREC_BF16X3 ? (uint16_t *)p0 : nullptr,
-                                         prec == PREC_BF16X3 ? (uint16_t *)p1 : nullptr, ldp, npad);
-        PG_KERNEL_CHECK(ctx);
-    }
-    PG_HIP(ctx, hipEventRecord(ctx->ev[1], st));
-
-    // ---- Gram launch geometry: PG_GRAM_WG_PER_CU work-groups per CU, a
-    // multiple of the XCD count (see TileWalk)
-    // supertiles (SR, SC) holding some needed tile (C >= RT*R), row-major
-    std::vector<int2> stl;
+    pg_status setup(pg_ctx *c, const float *bold, int n_, int t_, const pg_corr_opts *opts)
     {
-        const int nsr = (int)pg_cdiv(nr, SUPER), nsc = (int)pg_cdiv(nc, SUPER);
-        for (int sr = 0; sr < nsr; sr++)
-            for (int sc = rt * sr; sc < nsc; sc++)
-                if (sc * SUPER + SUPER - 1 >= rt * sr * SUPER) stl.push_back(make_int2(sr, sc));
+        ctx = c;
+        st = c->stream;
+        n = n_;
+        t = t_;
+        prec = opts->prec == PG_PREC_BF16X3 ? PREC_BF16X3 : PREC_F32;
+        weighted = opts->weighted != 0;
+        m1 = (long long)n * (n - 1) / 2;
+        g2path = prec == PREC_BF16X3;
+        tm = g2path ? g2::TM : TM;
+        tn = g2path ? g2::TN : TN;
+        rt = tm / tn;
+        nr = (int)pg_cdiv(std::max(n, 1), tm);  // row tiles
+        npad = nr * tm;
+        nc = npad / tn;  // column tiles
+        ldp = g2path ? (int)std::max<long long>(2 * g2::KC, pg_cdiv(t, g2::KC) * g2::KC) : (int)pg_cdiv(t, KC) * KC;
+        const long long nt = (long long)t;
+        double *mu, *sd;
+        PG_TRY(pg_bufT(ctx, "corr.mu", n + 1, &mu));
+        PG_TRY(pg_bufT(ctx, "corr.sd", n + 1, &sd));
+        PG_TRY(pg_bufT(ctx, "corr.z64", (size_t)n * nt + 1, &z64));
+        if (prec == PREC_F32) {
+            PG_TRY(pg_buf(ctx, "corr.p32", (size_t)npad * ldp * 4, &p0));
+            PG_HIP(ctx, hipMemsetAsync(p0, 0, (size_t)npad * ldp * 4, st));
+        } else {
+            PG_TRY(pg_buf(ctx, "corr.phi", (size_t)npad * ldp * 2, &p0));
+            PG_TRY(pg_buf(ctx, "corr.plo", (size_t)npad * ldp * 2, &p1));
+            PG_HIP(ctx, hipMemsetAsync(p0, 0, (size_t)npad * ldp * 2, st));
+            PG_HIP(ctx, hipMemsetAsync(p1, 0, (size_t)npad * ldp * 2, st));
+        }
+        const float *dbold = nullptr;
+        PG_TRY(pg_in(ctx, bold, (size_t)n * nt, "corr.bold", &dbold));
+
+        PG_HIP(ctx, hipEventRecord(ctx->ev[0], st));
+        if (n > 0 && t <= ZS_TMAX) {
+            const size_t shz = (size_t)t * (ZV + 1) * 4;
+            PG_HIP(ctx, hipFuncSetAttribute((const void *)zscore_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)shz));
+            zscore_fused<<<(unsigned)pg_cdiv(n, ZV), 256, shz, st>>>(
+                dbold, opts->layout, n, t, z64, prec == PREC_F32 ? (float *)p0 : nullptr,
+                prec == PREC_BF16X3 ? (uint16_t *)p0 : nullptr, prec == PREC_BF16X3 ? (uint16_t *)p1 : nullptr, ldp,
+                npad);
+            PG_KERNEL_CHECK(ctx);
+        } else if (n > 0) {
+            zscore_stats<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(dbold, opts->layout, n, t, mu, sd);
+            dim3 g((unsigned)pg_cdiv(n, 64), (unsigned)pg_cdiv(t, 64));
+            zscore_write<<<g, 256, 0, st>>>(dbold, opts->layout, n, t, mu, sd, z64,
+                                             prec == PREC_F32 ? (float *)p0 : nullptr,
+                                             prec == PREC_BF16X3 ? (uint16_t *)p0 : nullptr,
+                                             prec == PREC_BF16X3 ? (uint16_t *)p1 : nullptr, ldp, npad);
+            PG_KERNEL_CHECK(ctx);
+        }
+        PG_HIP(ctx, hipEventRecord(ctx->ev[1], st));
+
+        // Gram launch geometry: PG_GRAM_WG_PER_CU work-groups per CU, a
+        // multiple of the XCD count (see TileWalk); supertiles (SR, SC)
+        // holding some needed tile (C >= RT*R), row-major
+        stl.clear();
+        {
+            const int nsr = (int)pg_cdiv(nr, SUPER), nsc = (int)pg_cdiv(nc, SUPER);
+            for (int sr = 0; sr < nsr; sr++)
+                for (int sc = rt * sr; sc < nsc; sc++)
+                    if (sc * SUPER + SUPER - 1 >= rt * sr * SUPER) stl.push_back(make_int2(sr, sc));
+        }
+        int2 *dstl;
+        PG_TRY(pg_bufT(ctx, "corr.stl", stl.size() + 1, &dstl));
+        PG_HIP(ctx, hipMemcpyAsync(dstl, stl.data(), stl.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+        nst = (long long)stl.size();
+        nwg = (long long)ctx->num_cu * PG_GRAM_WG_PER_CU;
+        nwg = std::max<long long>(NXCD, nwg / NXCD * NXCD);
+        band = error_band(prec, t);
+
+        std::memset(&ga, 0, sizeof ga);
+        ga.p0 = p0;
+        ga.p1 = p1;
+        ga.ldp = ldp;
+        ga.kchunks = g2path ? ldp / g2::KC : ldp / KC;
+        ga.n = n;
+        ga.nr = nr;
+        ga.nc = nc;
+        ga.rt = rt;
+        ga.npad = npad;
+        ga.st_list = dstl;
+        ga.nst = nst;
+        ga.sstep = 0;
+        ga.lower_dev = nullptr;
+        stage = g2path ? (size_t)g2::NSTAGE * g2::STAGE : (size_t)stage_bytes<PREC_F32>();
+        // the histogram form of gram2_kernel holds one work-group per CU (LDS)
+        nwg_hist = g2path ? std::max<long long>(NXCD, ctx->num_cu / NXCD * NXCD) : nwg;
+
+        // counters: [0] band/candidate count, [1] accepted/selected count,
+        // [2] flips, [3] definite (sparsity single pass), [4] error flag,
+        // [5] largest |r~ - r64| of the rechecked pairs (double bits)
+        PG_TRY(pg_bufT(ctx, "corr.counters", 16, &counters));
+        PG_TRY(pg_bufT(ctx, "corr.segcnt", (size_t)nwg, &seg_cnt));
+        hseg.assign((size_t)nwg, 0ull);
+        nbk = (int)pg_cdiv(std::max(n, 1), BROWS);
+        nsrc = (int)nwg + 1;  // segments + the accepted/selected list
+        return PG_OK;
     }
-    int2 *dstl;
-    PG_TRY(pg_bufT(ctx, "corr.stl", stl.size() + 1, &dstl));
-    PG_HIP(ctx, hipMemcpyAsync(dstl, stl.data(), stl.size() * sizeof(int2), hipMemcpyHostToDevice, st));
-    const long long nst = (long long)stl.size();
-    long long nwg = (long long)ctx->num_cu * PG_GRAM_WG_PER_CU;
-    nwg = std::max<long long>(NXCD, nwg / NXCD * NXCD);
-    const double band = error_band(prec, t);
-    inf.band = band;
 
-    GramArgs ga;
-    std::memset(&ga, 0, sizeof ga);
-    ga.p0 = p0;
-    ga.p1 = p1;
-    ga.ldp = ldp;
-    ga.kchunks = g2path ? ldp / g2::KC : ldp / KC;
-    ga.n = n;
-    ga.nr = nr;
-    ga.nc = nc;
-    ga.rt = rt;
-    ga.npad = npad;
-    ga.st_list = dstl;
-    ga.nst = nst;
-    ga.sstep = 0;
-    const size_t stage = g2path ? (size_t)g2::NSTAGE * g2::STAGE : (size_t)stage_bytes<PREC_F32>();
-    // the histogram form of gram2_kernel holds one work-group per CU (LDS)
-    const long long nwg_hist = g2path ? std::max<long long>(NXCD, ctx->num_cu / NXCD * NXCD) : nwg;
-
-    bool ran_hist = false, ran_emit = false;
-    auto launch_gram = [&](int mode) -> pg_status {
+    pg_status launch_gram(int mode)
+    {
         size_t shm = stage + (mode == MODE_HIST ? NBIN * 4 : 0);
         const void *fn;
         if (g2path)
@@ -1948,64 +1986,15 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         PG_HIP(ctx, hipEventRecord(ctx->ev[e0 + 1], st));
         if (mode == MODE_HIST) ran_hist = true;
         else ran_emit = true;
-        if (ga.sstep <= 1) inf.gram_passes++;
+        if (ga.sstep <= 1) gram_passes++;
         return PG_OK;
-    };
+    }
 
-    // counters: [0] band/candidate count, [1] accepted/selected count,
-    // [2] flips, [3] definite (sparsity single pass), [4] error flag
-    unsigned long long *counters = nullptr, *seg_cnt = nullptr;
-    PG_TRY(pg_bufT(ctx, "corr.counters", 16, &counters));
-    PG_TRY(pg_bufT(ctx, "corr.segcnt", (size_t)nwg, &seg_cnt));
-    double *tstar_d = nullptr;
-    PG_TRY(pg_bufT(ctx, "corr.tstar", 2, &tstar_d));
-    std::vector<unsigned long long> hseg(nwg);
-    unsigned long long hcnt[16];
-
-    // One emitting Gram pass, re-run with larger buffers on overflow.
-    // store_r keeps r~ of every segment entry.
-    int2 *seg = nullptr, *bandl = nullptr;
-    float *band_r = nullptr, *seg_r = nullptr;
-    long long seg_cap = 0, band_cap = 0;
-    auto emit_pass = [&](long long seg_cap0, long long band_cap0, bool store_r) -> pg_status {
+    // One emitting Gram pass without a host check (the caller checks
+    // seg_cnt); store_r keeps r~ of every segment entry.
+    pg_status emit_launch(long long seg_cap0, long long band_cap0, bool store_r)
+    {
         seg_cap = (seg_cap0 + 63) / 64 * 64;  // segments 256-byte aligned (vector loads)
-        band_cap = band_cap0;
-        for (int attempt = 0; attempt < 3; attempt++) {
-            PG_TRY(pg_bufT(ctx, "corr.seg", (size_t)(nwg * seg_cap), &seg));
-            if (store_r) PG_TRY(pg_bufT(ctx, "corr.segr", (size_t)(nwg * seg_cap), &seg_r));
-            else seg_r = nullptr;
-            PG_TRY(pg_bufT(ctx, "corr.band", (size_t)band_cap, &bandl));
-            PG_TRY(pg_bufT(ctx, "corr.bandr", (size_t)band_cap, &band_r));
-            PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
-            PG_HIP(ctx, hipMemsetAsync(seg_cnt, 0, nwg * sizeof(unsigned long long), st));
-            ga.seg = seg;
-            ga.seg_r = seg_r;
-            ga.seg_cap = seg_cap;
-            ga.seg_cnt = seg_cnt;
-            ga.band = bandl;
-            ga.band_r = band_r;
-            ga.band_cap = band_cap;
-            ga.band_cnt = counters + 0;
-            ga.sstep = 0;
-            if (n > 1) PG_TRY(launch_gram(MODE_EMIT));
-            PG_HIP(ctx, hipMemcpyAsync(hseg.data(), seg_cnt, nwg * 8, hipMemcpyDeviceToHost, st));
-            PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
-            PG_HIP(ctx, hipStreamSynchronize(st));
-            long long mx = 0;
-            for (auto c : hseg) mx = std::max<long long>(mx, (long long)c);
-            if (mx <= seg_cap && (long long)hcnt[0] <= band_cap) return PG_OK;
-            if (attempt == 2) break;
-            seg_cap = (std::max(seg_cap, mx + mx / 8 + 4096) + 63) / 64 * 64;
-            band_cap = std::max(band_cap, (long long)hcnt[0] + (long long)hcnt[0] / 8 + 4096);
-        }
-        return pg_fail(ctx, PG_ENOMEM, "edge buffers overflowed repeatedly");
-    };
-
-    // Histogram pass over all tiles (stride 0) or a strided sample of tiles.
-    std::vector<unsigned long long> h(NBIN);
-    // One emitting Gram pass without a host check (the caller checks seg_cnt).
-    auto emit_launch = [&](long long seg_cap0, long long band_cap0, bool store_r) -> pg_status {
-        seg_cap = (seg_cap0 + 63) / 64 * 64;
         band_cap = band_cap0;
         PG_TRY(pg_bufT(ctx, "corr.seg", (size_t)(nwg * seg_cap), &seg));
         if (store_r) PG_TRY(pg_bufT(ctx, "corr.segr", (size_t)(nwg * seg_cap), &seg_r));
@@ -2025,12 +2014,38 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         ga.sstep = 0;
         if (n > 1) PG_TRY(launch_gram(MODE_EMIT));
         return PG_OK;
-    };
+    }
+
+    // The same with a host check of the segment and band counts (hseg, hcnt),
+    // re-run with larger buffers on overflow.
+    pg_status emit_pass(long long seg_cap0, long long band_cap0, bool store_r)
+    {
+        long long sc = seg_cap0, bcap = band_cap0;
+        for (int attempt = 0; attempt < 3; attempt++) {
+            PG_TRY(emit_launch(sc, bcap, store_r));
+            PG_HIP(ctx, hipMemcpyAsync(hseg.data(), seg_cnt, nwg * 8, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipStreamSynchronize(st));
+            const long long mx = max_segment();
+            if (mx <= seg_cap && (long long)hcnt[0] <= band_cap) return PG_OK;
+            if (attempt == 2) break;
+            sc = std::max(seg_cap, mx + mx / 8 + 4096);
+            bcap = std::max(band_cap, (long long)hcnt[0] + (long long)hcnt[0] / 8 + 4096);
+        }
+        return pg_fail(ctx, PG_ENOMEM, "edge buffers overflowed repeatedly");
+    }
+
+    long long max_segment() const
+    {
+        long long mx = 0;
+        for (auto c : hseg) mx = std::max<long long>(mx, (long long)c);
+        return mx;
+    }
 
     // Histogram Gram pass over all tiles (stride 0) or a strided sample of
     // tiles, reduced into htot on the device.
-    unsigned long long *htot = nullptr;
-    auto hist_launch = [&](long long stride) -> pg_status {
+    pg_status hist_launch(long long stride)
+    {
         uint32_t *hist;
         PG_TRY(pg_bufT(ctx, "corr.hist", (size_t)nwg_hist * NBIN, &hist));
         PG_TRY(pg_bufT(ctx, "corr.htot", NBIN, &htot));
@@ -2041,22 +2056,23 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         hist_reduce<<<NBIN / 32, 256, 0, st>>>(hist, (int)nwg_hist, htot);
         PG_KERNEL_CHECK(ctx);
         return PG_OK;
-    };
-    auto hist_pass = [&](long long stride) -> pg_status {
+    }
+
+    pg_status hist_pass(long long stride, std::vector<unsigned long long> &h)
+    {
         PG_TRY(hist_launch(stride));
+        h.resize(NBIN);
         PG_HIP(ctx, hipMemcpyAsync(h.data(), htot, NBIN * 8, hipMemcpyDeviceToHost, st));
         PG_HIP(ctx, hipStreamSynchronize(st));
         return PG_OK;
-    };
+    }
 
-    // ---- CSR build (bucketed counting sort), launched without a host check
-    const int nbk = (int)pg_cdiv(std::max(n, 1), BROWS);
-    const int nsrc = (int)nwg + 1;  // segments + the accepted/selected list
-    int *row = nullptr, *col = nullptr;
-    float *val = nullptr;
-    auto launch_csr = [&](const EdgeSrc &src, long long nnz) -> pg_status {
+    // CSR build (bucketed counting sort), launched without a host check
+    pg_status launch_csr(const EdgeSrc &src, long long nnz)
+    {
         if (nnz > INT32_MAX) return pg_fail(ctx, PG_ENOTSUP, "nnz %lld exceeds int32 CSR", (long long)nnz);
         const size_t nnzs = (size_t)std::max<long long>(nnz, 1);
+        const int nw = (n + 31) / 32;
         unsigned *bcnt, *ent;
         long long *boff, *bstart, *btot;
         int *ctmp;
@@ -2103,9 +2119,11 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             PG_HIP(ctx, hipMemsetAsync(row, 0, 4, st));
         }
         return PG_OK;
-    };
+    }
+
     // launch, wait, and check row[n]
-    auto finish_csr = [&](const EdgeSrc &src, long long nnz) -> pg_status {
+    pg_status finish_csr(const EdgeSrc &src, long long nnz)
+    {
         PG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
         PG_TRY(launch_csr(src, nnz));
         PG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
@@ -2115,70 +2133,213 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         if ((long long)rn != nnz)
             return pg_fail(ctx, PG_EFORMAT, "CSR build mismatch: row[n]=%d nnz=%lld", rn, (long long)nnz);
         return PG_OK;
-    };
+    }
 
-    int2 *extra = nullptr;
-    unsigned long long *extra_cnt = counters + 1, *flips = counters + 2;
-    EdgeSrc es;
-    std::memset(&es, 0, sizeof es);
-    float ms_hist = 0.0f;
-    ga.lower_dev = nullptr;
-
-    if (!smode) {
-        // ---------------------------------------------------- r threshold
-        ga.upper = round_up_f((double)thr_eff + band);
-        ga.lower = round_down_f((double)thr_eff - band);
-        PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-        const long long seg0 = pg_cdiv(std::max<long long>((long long)(m1 * 0.03), 1 << 20), nwg) + 4096;
-        PG_TRY(emit_pass(seg0, 1 << 22, false));
-        PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
-        long long definite = 0;
-        for (auto c : hseg) definite += (long long)c;
-        const long long nband = (long long)hcnt[0];
-        inf.band_pairs = nband;
-        double *band64;
-        PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)std::max<long long>(nband, 1), &band64));
-        PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)std::max<long long>(nband, 1), &extra));
-        if (nband > 0) {
-            recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(bandl, counters + 0, band_cap, z64, t, band64,
-                                                                          band_r, counters + 5);
-            accept_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(
-                bandl, band_r, band64, counters + 0, band_cap, (double)thr_eff, extra, extra_cnt, flips);
-        }
-        PG_KERNEL_CHECK(ctx);
-        PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
-        PG_HIP(ctx, hipStreamSynchronize(st));
-        inf.band_flips = (long long)hcnt[2];
-        inf.band_max_err = host_err(hcnt[5]);
-        inf.nnz = 2 * (definite + (long long)hcnt[1]);
-        inf.s_thresh = (float)(m1 > 0 ? 100.0 * (double)inf.nnz / (double)m1 / 2.0 : 0.0);
+    EdgeSrc edges(int2 *extra, unsigned long long *extra_cnt, long long extra_cap) const
+    {
+        EdgeSrc es;
+        std::memset(&es, 0, sizeof es);
         es.seg = seg;
         es.seg_cap = seg_cap;
         es.seg_cnt = seg_cnt;
         es.nseg = (int)nwg;
         es.extra = extra;
         es.extra_cnt = extra_cnt;
-        es.extra_cap = std::max<long long>(nband, 1);
-        PG_TRY(finish_csr(es, inf.nnz));
+        es.extra_cap = extra_cap;
+        return es;
+    }
+
+    // Stage times from the events, the result kept in the context.
+    void finish(pg_corr_info &inf)
+    {
+        float ms = 0.0f, ms_hist = 0.0f;
+        (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+        inf.ms_zscore = ms;
+        (void)hipEventElapsedTime(&ms_hist, ctx->ev[1], ctx->ev[2]);
+        (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+        inf.ms_gram = ms + ms_hist;
+        (void)hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]);
+        inf.ms_select = ms;
+        (void)hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
+        inf.ms_csr = ms;
+        if (ran_hist && hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess) inf.ms_gram_hist = ms;
+        if (ran_emit && hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]) == hipSuccess) inf.ms_gram_emit = ms;
+        inf.gram_passes = gram_passes;
+        ctx->cn = n;
+        ctx->ct = t;
+        ctx->cnnz = inf.nnz;
+        ctx->cweighted = weighted;
+        ctx->c_row = row;
+        ctx->c_col = col;
+        ctx->c_val = val;
+        ctx->c_z64 = z64;
+    }
+};
+
+// Fine histogram of the kept r~ (bins over [ctl->lower, 1 + 1e-6]) into *h2t.
+pg_status fine_hist(CorrRun &R, const SelCtl *ctl, unsigned long long **h2t_out)
+{
+    pg_ctx *ctx = R.ctx;
+    hipStream_t st = R.st;
+    const int hb = (int)std::min<long long>(256, R.nwg);
+    uint32_t *h2;
+    unsigned long long *h2t;
+    PG_TRY(pg_bufT(ctx, "corr.h2", (size_t)hb * NBIN2, &h2));
+    PG_TRY(pg_bufT(ctx, "corr.h2t", NBIN2, &h2t));
+    PG_HIP(ctx, hipMemsetAsync(h2t, 0, NBIN2 * sizeof(unsigned long long), st));
+    emitted_hist<<<hb, 1024, 0, st>>>(R.seg_r, R.seg_cap, R.seg_cnt, (int)R.nwg, ctl, h2);
+    hist_sum<<<dim3(NBIN2 / 256, 16), 256, 0, st>>>(h2, hb, NBIN2, h2t);
+    PG_KERNEL_CHECK(ctx);
+    *h2t_out = h2t;
+    return PG_OK;
+}
+
+struct Window {
+    int2 *cand = nullptr, *extra = nullptr;
+    float *cand_r = nullptr;
+    double *cand64 = nullptr;
+    unsigned long long *dpart = nullptr;
+    int nd = 0;  // entries of dpart
+};
+
+// Split the kept segment entries by ctl's window [L2, U2]: r~ > U2 are
+// counted as definite (per-block counts in dpart), the rest of the window
+// goes to the candidate list (count in counters[0], at most ccap written),
+// whose FP64 r is then computed (largest |r~ - r64| into counters[5]).
+pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Window *win)
+{
+    pg_ctx *ctx = R.ctx;
+    hipStream_t st = R.st;
+    PG_TRY(pg_bufT(ctx, "corr.cand", (size_t)ccap, &win->cand));
+    PG_TRY(pg_bufT(ctx, "corr.candr", (size_t)ccap, &win->cand_r));
+    PG_TRY(pg_bufT(ctx, "corr.cand64", (size_t)ccap, &win->cand64));
+    PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)ccap, &win->extra));
+    PG_HIP(ctx, hipMemsetAsync(R.counters, 0, 16 * sizeof(unsigned long long), st));
+    const unsigned wx = (unsigned)std::max<long long>(1, pg_cdiv(R.seg_cap, WIN_CHUNK));
+    win->nd = (int)(wx * R.nwg);
+    PG_TRY(pg_bufT(ctx, "corr.dpart", (size_t)win->nd, &win->dpart));
+    window_kernel<<<dim3(wx, (unsigned)R.nwg), 256, 0, st>>>(R.seg, R.seg_r, R.seg_cap, R.seg_cnt, (int)R.nwg, ctl,
+                                                              win->cand, win->cand_r, ccap, R.counters + 0,
+                                                              win->dpart);
+    recheck_kernel<<<(unsigned)pg_cdiv(ccap, 256), 256, 0, st>>>(win->cand, R.counters + 0, ccap, R.z64, R.t,
+                                                                  win->cand64, win->cand_r, R.counters + 5);
+    PG_KERNEL_CHECK(ctx);
+    return PG_OK;
+}
+
+// Sparsity: the exact selection of the k - definite best candidates
+// (select_kernel; t* into tstar_d, errors into counters[4]) and the definite
+// count into counters[3].
+pg_status window_select(CorrRun &R, const SelCtl *ctl, long long ccap, long long kwant, double *tstar_d,
+                        Window *win)
+{
+    PG_TRY(window_candidates(R, ctl, ccap, win));
+    hipStream_t st = R.st;
+    unsigned long long *c = R.counters;
+    select_kernel<<<1, 1024, 0, st>>>(win->cand, win->cand64, win->cand_r, c + 0, ccap, kwant, win->dpart, win->nd,
+                                      win->extra, c + 1, tstar_d, c + 2, (int *)(c + 4));
+    definite_sum<<<1, 1024, 0, st>>>(win->dpart, win->nd, c + 3);
+    PG_KERNEL_CHECK(R.ctx);
+    return PG_OK;
+}
+
+// The band is error_band()'s proven bound times 1.5; a rechecked pair whose
+// fast r~ is further from its FP64 r than the bound itself means the rounding
+// model behind the band does not hold on this device, and pairs outside the
+// band may then be misclassified.
+pg_status check_band(pg_ctx *ctx, const pg_corr_info &inf)
+{
+    if (inf.band_max_err > inf.band / 1.5)
+        return pg_fail(ctx, PG_EDEVICE,
+                       "fast-path error %.3g exceeds the proven bound %.3g (band %.3g): edge set not guaranteed",
+                       inf.band_max_err, inf.band / 1.5, inf.band);
+    return PG_OK;
+}
+
+}  // namespace
+
+extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, int32_t t,
+                                   const pg_corr_opts *opts, pg_corr_info *info)
+{
+    if (!ctx) return PG_EINVAL;
+    PG_TRY(check_opts(ctx, bold, n, t, opts));
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    pg_emit_invalidate(ctx);
+    const bool smode = opts->thr_kind == PG_THR_SPARSITY;
+    CorrRun R;
+    pg_corr_info inf;
+    std::memset(&inf, 0, sizeof inf);
+    inf.n = n;
+    inf.t = t;
+    const long long m1 = (long long)n * (n - 1) / 2;
+
+    // threshold bookkeeping, as the reference names its outputs
+    // (U CorMat.cpp:306-313,333; W CorMat.cpp:310)
+    long long kwant = 0;
+    float thr_eff = opts->thr;
+    if (smode) {
+        kwant = sparsity_k(m1, opts->thr);
+        inf.s_thresh = opts->thr;
+    } else {
+        thr_eff = thr_effective(opts->thr, opts->weighted != 0);
+        inf.r_thresh = opts->thr;
+        inf.t_star = (double)thr_eff;
+    }
+    PG_TRY(R.setup(ctx, bold, n, t, opts));
+    hipStream_t st = R.st;
+    const double band = R.band;
+    inf.band = band;
+    double *tstar_d = nullptr;
+    PG_TRY(pg_bufT(ctx, "corr.tstar", 2, &tstar_d));
+    unsigned long long *counters = R.counters;
+    unsigned long long *hcnt = R.hcnt;
+    const long long nwg = R.nwg, nst = R.nst;
+
+    int2 *extra = nullptr;
+    unsigned long long *extra_cnt = counters + 1, *flips = counters + 2;
+
+    if (!smode) {
+        // ---------------------------------------------------- r threshold
+        R.ga.upper = round_up_f((double)thr_eff + band);
+        R.ga.lower = round_down_f((double)thr_eff - band);
+        PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+        const long long seg0 = pg_cdiv(std::max<long long>((long long)(m1 * 0.03), 1 << 20), nwg) + 4096;
+        PG_TRY(R.emit_pass(seg0, 1 << 22, false));
+        PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+        long long definite = 0;
+        for (auto c : R.hseg) definite += (long long)c;
+        const long long nband = (long long)hcnt[0];
+        inf.band_pairs = nband;
+        double *band64;
+        PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)std::max<long long>(nband, 1), &band64));
+        PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)std::max<long long>(nband, 1), &extra));
+        if (nband > 0) {
+            recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(R.bandl, counters + 0, R.band_cap, R.z64, t,
+                                                                          band64, R.band_r, counters + 5);
+            accept_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(
+                R.bandl, R.band_r, band64, counters + 0, R.band_cap, (double)thr_eff, extra, extra_cnt, flips);
+        }
+        PG_KERNEL_CHECK(ctx);
+        PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof R.hcnt, hipMemcpyDeviceToHost, st));
+        PG_HIP(ctx, hipStreamSynchronize(st));
+        inf.band_flips = (long long)hcnt[2];
+        inf.band_max_err = host_err(hcnt[5]);
+        inf.nnz = 2 * (definite + (long long)hcnt[1]);
+        inf.s_thresh = (float)(m1 > 0 ? 100.0 * (double)inf.nnz / (double)m1 / 2.0 : 0.0);
+        PG_TRY(R.finish_csr(R.edges(extra, extra_cnt, std::max<long long>(nband, 1)), inf.nnz));
     } else if (kwant == 0 || n < 2) {
         // ---------------------------------------------------- nothing asked
         PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
         PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
-        PG_HIP(ctx, hipMemsetAsync(seg_cnt, 0, nwg * sizeof(unsigned long long), st));
+        PG_HIP(ctx, hipMemsetAsync(R.seg_cnt, 0, nwg * sizeof(unsigned long long), st));
         PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
         PG_TRY(pg_bufT(ctx, "corr.extra", 1, &extra));
-        PG_TRY(pg_bufT(ctx, "corr.seg", 1, &seg));
-        es.seg = seg;
-        es.seg_cap = 0;
-        es.seg_cnt = seg_cnt;
-        es.nseg = (int)nwg;
-        es.extra = extra;
-        es.extra_cnt = extra_cnt;
-        es.extra_cap = 1;
+        PG_TRY(pg_bufT(ctx, "corr.seg", 1, &R.seg));
+        R.seg_cap = 0;
         inf.nnz = 0;
         inf.r_thresh = 1.0f;
         inf.t_star = 1.0;
-        PG_TRY(finish_csr(es, 0));
+        PG_TRY(R.finish_csr(R.edges(extra, extra_cnt, 1), 0));
     } else {
         // ---------------------------------------------------- sparsity
         const double w = 2.0 / NBIN;
@@ -2194,113 +2355,85 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             //     widened by 2 bands each side (device: window_bounds);
             // (4) candidates in the window, FP64 recheck, exact selection;
             // (5) CSR build; then one check of every step's assumption.
-            const long long msamp = sampled_pairs(stl, stride, nr, nc, rt, tm, tn, n);
+            const long long msamp = sampled_pairs(R.stl, stride, R.nr, R.nc, R.rt, R.tm, R.tn, n);
             const double p = (double)kwant / (double)m1;
             const double meff = std::max(1.0, (double)msamp / 64.0);  // pairs inside a tile are correlated
             const double f = std::min(1.0, 1.15 * p + 6.0 * std::sqrt(p * (1.0 - p) / meff) + 1.0 / meff);
             SelCtl *ctl;
             PG_TRY(pg_bufT(ctx, "corr.selctl", 1, &ctl));
-            PG_TRY(hist_launch(stride));
-            sample_bound<<<1, 1024, 0, st>>>(htot, f, ctl);
+            PG_TRY(R.hist_launch(stride));
+            sample_bound<<<1, 1024, 0, st>>>(R.htot, f, ctl);
             PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-            ga.lower_dev = &ctl->lower;
-            ga.upper = -4.0f;  // every hit is a segment entry
+            R.ga.lower_dev = &ctl->lower;
+            R.ga.upper = -4.0f;  // every hit is a segment entry
             const double expect = std::min(1.0, f * 1.2 + 2.0 * w) * (double)m1;
-            PG_TRY(emit_launch(pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096, 1 << 10, true));
-            ga.lower_dev = nullptr;
-            const int hb = (int)std::min<long long>(256, nwg);
-            uint32_t *h2;
+            PG_TRY(R.emit_launch(pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096, 1 << 10, true));
+            R.ga.lower_dev = nullptr;
             unsigned long long *h2t;
-            PG_TRY(pg_bufT(ctx, "corr.h2", (size_t)hb * NBIN2, &h2));
-            PG_TRY(pg_bufT(ctx, "corr.h2t", NBIN2, &h2t));
-            PG_HIP(ctx, hipMemsetAsync(h2t, 0, NBIN2 * sizeof(unsigned long long), st));
-            emitted_hist<<<hb, 1024, 0, st>>>(seg_r, seg_cap, seg_cnt, (int)nwg, ctl, h2);
-            hist_sum<<<dim3(NBIN2 / 256, 16), 256, 0, st>>>(h2, hb, NBIN2, h2t);
+            PG_TRY(fine_hist(R, ctl, &h2t));
             window_bounds<<<1, 1024, 0, st>>>(h2t, kwant, band, ctl);
             PG_KERNEL_CHECK(ctx);
             PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
             const long long ccap = std::min<long long>(1 << 22, std::max<long long>(1 << 16, kwant / 4));
-            int2 *cand;
-            float *cand_r;
-            double *cand64;
-            PG_TRY(pg_bufT(ctx, "corr.cand", (size_t)ccap, &cand));
-            PG_TRY(pg_bufT(ctx, "corr.candr", (size_t)ccap, &cand_r));
-            PG_TRY(pg_bufT(ctx, "corr.cand64", (size_t)ccap, &cand64));
-            PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)ccap, &extra));
-            PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
-            const unsigned wx = (unsigned)std::max<long long>(1, pg_cdiv(seg_cap, WIN_CHUNK));
-            unsigned long long *dpart;
-            PG_TRY(pg_bufT(ctx, "corr.dpart", (size_t)wx * nwg, &dpart));
-            window_kernel<<<dim3(wx, (unsigned)nwg), 256, 0, st>>>(seg, seg_r, seg_cap, seg_cnt, (int)nwg, ctl, cand,
-                                                                    cand_r, ccap, counters + 0, dpart);
-            recheck_kernel<<<(unsigned)pg_cdiv(ccap, 256), 256, 0, st>>>(cand, counters + 0, ccap, z64, t, cand64, cand_r,
-                                                                         counters + 5);
-            select_kernel<<<1, 1024, 0, st>>>(cand, cand64, cand_r, counters + 0, ccap, kwant, dpart,
-                                              (int)(wx * nwg), extra, extra_cnt, tstar_d, flips,
-                                              (int *)(counters + 4));
-            // definite count for the final check: written by the selection
-            definite_sum<<<1, 1024, 0, st>>>(dpart, (int)(wx * nwg), counters + 3);
-            PG_KERNEL_CHECK(ctx);
-            es.seg = seg;
-            es.seg_cap = seg_cap;
-            es.seg_cnt = seg_cnt;
-            es.nseg = (int)nwg;
-            es.seg_r = seg_r;
+            Window win;
+            PG_TRY(window_select(R, ctl, ccap, kwant, tstar_d, &win));
+            extra = win.extra;
+            EdgeSrc es = R.edges(extra, extra_cnt, ccap);
+            es.seg_r = R.seg_r;
             es.keep_above_dev = &ctl->U2;
-            es.extra = extra;
-            es.extra_cnt = extra_cnt;
-            es.extra_cap = ccap;
             PG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
-            PG_TRY(launch_csr(es, 2 * kwant));
+            PG_TRY(R.launch_csr(es, 2 * kwant));
             PG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
             SelCtl hc;
             int rn = -1;
-            PG_HIP(ctx, hipMemcpyAsync(hseg.data(), seg_cnt, nwg * 8, hipMemcpyDeviceToHost, st));
-            PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipMemcpyAsync(R.hseg.data(), R.seg_cnt, nwg * 8, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof R.hcnt, hipMemcpyDeviceToHost, st));
             PG_HIP(ctx, hipMemcpyAsync(&hc, ctl, sizeof hc, hipMemcpyDeviceToHost, st));
             PG_HIP(ctx, hipMemcpyAsync(&inf.t_star, tstar_d, 8, hipMemcpyDeviceToHost, st));
-            if (n > 0) PG_HIP(ctx, hipMemcpyAsync(&rn, row + n, 4, hipMemcpyDeviceToHost, st));
+            if (n > 0) PG_HIP(ctx, hipMemcpyAsync(&rn, R.row + n, 4, hipMemcpyDeviceToHost, st));
             PG_HIP(ctx, hipStreamSynchronize(st));
-            long long mx = 0;
-            for (auto c : hseg) mx = std::max<long long>(mx, (long long)c);
-            done = hc.ok == 1 && mx <= seg_cap && (long long)hcnt[0] <= ccap && ((int *)hcnt)[8] == 0 &&
+            const long long mx = R.max_segment();
+            done = hc.ok == 1 && mx <= R.seg_cap && (long long)hcnt[0] <= ccap && ((int *)hcnt)[8] == 0 &&
                    (long long)hcnt[3] + (long long)hcnt[1] == kwant && (long long)rn == 2 * kwant;
             if (done) {
                 inf.band_pairs = (long long)hcnt[0];
                 inf.band_flips = (long long)hcnt[2];
                 inf.band_max_err = host_err(hcnt[5]);
             }
-            std::memset(&es, 0, sizeof es);
         }
         if (!done) {
             // (fallback) exact bracket from a histogram of every pair, then an
             // emitting pass with a definite/candidate split
-            PG_TRY(hist_pass(0));
+            std::vector<unsigned long long> h;
+            PG_TRY(R.hist_pass(0, h));
             long long above = 0;
             int bstar = -1;
             for (int b = NBIN - 1; b >= 0; b--) {
-                if (above + (long long)h[b] >= kwant) { bstar = b; break; }
+                if (above + (long long)h[b] >= kwant) {
+                    bstar = b;
+                    break;
+                }
                 above += (long long)h[b];
             }
             if (bstar < 0) return pg_fail(ctx, PG_ETHRESH, "histogram holds fewer than k=%lld pairs", kwant);
             const double lo_edge = bstar * w - 1.0 - 1e-6;
             const double hi_edge = (bstar + 1) * w - 1.0 + 1e-6;
-            ga.upper = round_up_f(hi_edge + 2.0 * band);
-            ga.lower = round_down_f(lo_edge - 2.0 * band);
+            R.ga.upper = round_up_f(hi_edge + 2.0 * band);
+            R.ga.lower = round_down_f(lo_edge - 2.0 * band);
             PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-            PG_TRY(emit_pass(pg_cdiv(kwant, nwg) * 3 / 2 + 4096, 1 << 22, false));
+            PG_TRY(R.emit_pass(pg_cdiv(kwant, nwg) * 3 / 2 + 4096, 1 << 22, false));
             PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
             const long long nband = (long long)hcnt[0];
             double *band64;
             PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)std::max<long long>(nband, 1), &band64));
             PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)std::max<long long>(nband, 1), &extra));
             if (nband > 0)
-                recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(bandl, counters + 0, band_cap, z64, t,
-                                                                              band64, band_r, counters + 5);
-            select_kernel<<<1, 1024, 0, st>>>(bandl, band64, band_r, counters + 0, band_cap, kwant, seg_cnt, (int)nwg,
-                                              extra, extra_cnt, tstar_d, flips, (int *)(counters + 4));
+                recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(R.bandl, counters + 0, R.band_cap, R.z64,
+                                                                              t, band64, R.band_r, counters + 5);
+            select_kernel<<<1, 1024, 0, st>>>(R.bandl, band64, R.band_r, counters + 0, R.band_cap, kwant, R.seg_cnt,
+                                              (int)nwg, extra, extra_cnt, tstar_d, flips, (int *)(counters + 4));
             PG_KERNEL_CHECK(ctx);
-            PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof hcnt, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof R.hcnt, hipMemcpyDeviceToHost, st));
             PG_HIP(ctx, hipMemcpyAsync(&inf.t_star, tstar_d, 8, hipMemcpyDeviceToHost, st));
             PG_HIP(ctx, hipStreamSynchronize(st));
             if (((int *)hcnt)[8] != 0)
@@ -2309,49 +2442,325 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             inf.band_pairs = nband;
             inf.band_flips = (long long)hcnt[2];
             inf.band_max_err = host_err(hcnt[5]);
-            es.seg = seg;
-            es.seg_cap = seg_cap;
-            es.seg_cnt = seg_cnt;
-            es.nseg = (int)nwg;
-            es.extra = extra;
-            es.extra_cnt = extra_cnt;
-            es.extra_cap = std::max<long long>(nband, 1);
-            PG_TRY(finish_csr(es, 2 * kwant));
+            PG_TRY(R.finish_csr(R.edges(extra, extra_cnt, std::max<long long>(nband, 1)), 2 * kwant));
         }
         inf.nnz = 2 * kwant;
         inf.r_thresh = (float)inf.t_star;
     }
 
-    // The band is error_band()'s proven bound times 1.5; a rechecked pair whose
-    // fast r~ is further from its FP64 r than the bound itself means the
-    // rounding model behind the band does not hold on this device, and pairs
-    // outside the band may then be misclassified.
-    if (inf.band_max_err > band / 1.5)
-        return pg_fail(ctx, PG_EDEVICE,
-                       "fast-path error %.3g exceeds the proven bound %.3g (band %.3g): edge set not guaranteed",
-                       inf.band_max_err, band / 1.5, band);
+    PG_TRY(check_band(ctx, inf));
+    R.finish(inf);
+    if (info) *info = inf;
+    return PG_OK;
+}
 
+// ------------------------------------------------ one Gram pass, many thresholds
+
+// State of the last pg_corr_emit: the kept pairs (segments with r~), the
+// emission bound and, in sparsity mode, the fine histogram of the kept r~.
+struct PgEmit {
+    bool valid = false;
+    CorrRun R;
+    int kind = PG_THR_R;
+    float lower = 2.0f;           // every pair with r~ >= lower is kept
+    long long kwant_max = 0;      // sparsity: the k the emission serves
+    SelCtl *ctl = nullptr;        // device control block (lower, window)
+    unsigned long long *h2t = nullptr;
+    pg_corr_info base;            // the emission's own figures
+};
+
+void pg_emit_invalidate(pg_ctx *ctx)
+{
+    if (ctx && ctx->emit) ctx->emit->valid = false;
+}
+
+void pg_emit_free(PgEmit *e) { delete e; }
+
+extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int32_t t, const pg_corr_opts *opts,
+                                  pg_corr_info *info)
+{
+    if (!ctx) return PG_EINVAL;
+    PG_TRY(check_opts(ctx, bold, n, t, opts));
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    if (!ctx->emit) ctx->emit = new PgEmit();
+    PgEmit &E = *ctx->emit;
+    E.valid = false;
+    ctx->cnnz = -1;  // no CSR until pg_corr_select
+    E.R = CorrRun();
+    CorrRun &R = E.R;
+    const bool smode = opts->thr_kind == PG_THR_SPARSITY;
+    E.kind = smode ? PG_THR_SPARSITY : PG_THR_R;
+    pg_corr_info inf;
+    std::memset(&inf, 0, sizeof inf);
+    inf.n = n;
+    inf.t = t;
+    PG_TRY(R.setup(ctx, bold, n, t, opts));
+    hipStream_t st = R.st;
+    const double band = R.band;
+    const long long m1 = R.m1, nwg = R.nwg;
+    inf.band = band;
+    PG_TRY(pg_bufT(ctx, "corr.selctl", 1, &E.ctl));
+    SelCtl hc;
+    std::memset(&hc, 0, sizeof hc);
+    if (!smode) {
+        // every pair the loosest threshold could keep, with its r~
+        const float thr_eff = thr_effective(opts->thr, opts->weighted != 0);
+        E.lower = round_down_f((double)thr_eff - band);
+        // segment sizes from a sampled histogram (about 2% of the supertiles)
+        // when there are enough supertiles to sample, else a fixed guess
+        long long seg0 = pg_cdiv(std::max<long long>((long long)(m1 * 0.03), 1 << 20), nwg) + 4096;
+        const long long stride = R.nst >= 8 ? std::max<long long>(2, std::min<long long>(50, R.nst / 16)) : 1;
+        if (stride > 1 && n > 1) {
+            std::vector<unsigned long long> h;
+            PG_TRY(R.hist_pass(stride, h));
+            const long long msamp = sampled_pairs(R.stl, stride, R.nr, R.nc, R.rt, R.tm, R.tn, n);
+            int b0 = (int)(((double)E.lower + 1.0) * (NBIN * 0.5));
+            b0 = std::max(0, std::min(NBIN - 1, b0));
+            unsigned long long above = 0;
+            for (int b = b0; b < NBIN; b++) above += h[b];
+            const double frac = msamp > 0 ? (double)above / (double)msamp : 1.0;
+            const double expect = std::min(1.0, frac * 1.25 + 0.002) * (double)m1;
+            seg0 = pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096;
+        }
+        R.ga.lower = E.lower;
+        R.ga.upper = -4.0f;
+        PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+        PG_TRY(R.emit_pass(seg0, 1 << 10, true));
+        PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+        hc.lower = E.lower;
+        hc.ok = 1;
+        PG_HIP(ctx, hipMemcpyAsync(E.ctl, &hc, sizeof hc, hipMemcpyHostToDevice, st));
+        inf.r_thresh = opts->thr;
+        inf.t_star = (double)thr_eff;
+    } else {
+        E.kwant_max = sparsity_k(m1, opts->thr);
+        inf.s_thresh = opts->thr;
+        const double w = 2.0 / NBIN;
+        bool ok = false;
+        const long long stride =
+            R.nst >= 8 ? std::max<long long>(2, std::min<long long>(50, R.nst / 16)) : 1;
+        if (E.kwant_max > 0 && n > 1 && stride > 1) {
+            // the single pass's sampled bound for the largest k
+            const long long msamp = sampled_pairs(R.stl, stride, R.nr, R.nc, R.rt, R.tm, R.tn, n);
+            const double p = (double)E.kwant_max / (double)m1;
+            const double meff = std::max(1.0, (double)msamp / 64.0);
+            const double f = std::min(1.0, 1.15 * p + 6.0 * std::sqrt(p * (1.0 - p) / meff) + 1.0 / meff);
+            PG_TRY(R.hist_launch(stride));
+            sample_bound<<<1, 1024, 0, st>>>(R.htot, f, E.ctl);
+            PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+            R.ga.lower_dev = &E.ctl->lower;
+            R.ga.upper = -4.0f;
+            const double expect = std::min(1.0, f * 1.2 + 2.0 * w) * (double)m1;
+            long long cap = pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096;
+            for (int attempt = 0; attempt < 2; attempt++) {
+                PG_TRY(R.emit_launch(cap, 1 << 10, true));
+                PG_HIP(ctx, hipMemcpyAsync(R.hseg.data(), R.seg_cnt, nwg * 8, hipMemcpyDeviceToHost, st));
+                PG_HIP(ctx, hipStreamSynchronize(st));
+                const long long mx = R.max_segment();
+                if (mx <= R.seg_cap) break;
+                cap = mx + mx / 8 + 4096;
+            }
+            R.ga.lower_dev = nullptr;
+            PG_TRY(fine_hist(R, E.ctl, &E.h2t));
+            window_bounds<<<1, 1024, 0, st>>>(E.h2t, E.kwant_max, band, E.ctl);
+            PG_KERNEL_CHECK(ctx);
+            PG_HIP(ctx, hipMemcpyAsync(&hc, E.ctl, sizeof hc, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipStreamSynchronize(st));
+            ok = hc.ok == 1 && R.max_segment() <= R.seg_cap;
+            E.lower = hc.lower;
+        }
+        if (E.kwant_max > 0 && n > 1 && !ok) {
+            // the bound from a histogram of every pair: the bin of the k-th
+            // largest r~, widened by two bands and one bin
+            std::vector<unsigned long long> h;
+            PG_TRY(R.hist_pass(0, h));
+            long long above = 0;
+            int bstar = -1;
+            for (int b = NBIN - 1; b >= 0; b--) {
+                if (above + (long long)h[b] >= E.kwant_max) {
+                    bstar = b;
+                    break;
+                }
+                above += (long long)h[b];
+            }
+            if (bstar < 0) return pg_fail(ctx, PG_ETHRESH, "histogram holds fewer than k=%lld pairs", E.kwant_max);
+            E.lower = round_down_f(bstar * w - 1.0 - 1e-6 - 2.0 * band - w);
+            hc.lower = E.lower;
+            hc.ok = 1;
+            PG_HIP(ctx, hipMemcpyAsync(E.ctl, &hc, sizeof hc, hipMemcpyHostToDevice, st));
+            PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+            R.ga.lower = E.lower;
+            R.ga.upper = -4.0f;
+            PG_TRY(R.emit_pass(pg_cdiv(above + (long long)h[bstar], nwg) * 3 / 2 + 4096, 1 << 10, true));
+            PG_TRY(fine_hist(R, E.ctl, &E.h2t));
+            window_bounds<<<1, 1024, 0, st>>>(E.h2t, E.kwant_max, band, E.ctl);
+            PG_KERNEL_CHECK(ctx);
+            PG_HIP(ctx, hipMemcpyAsync(&hc, E.ctl, sizeof hc, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipStreamSynchronize(st));
+            if (hc.ok != 1) return pg_fail(ctx, PG_ETHRESH, "selection window below the emitted pairs");
+        }
+        if (E.kwant_max == 0 || n < 2) {
+            // nothing to keep: empty segments, no Gram pass
+            PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+            R.seg_cap = 64;
+            PG_TRY(pg_bufT(ctx, "corr.seg", (size_t)(nwg * R.seg_cap), &R.seg));
+            PG_TRY(pg_bufT(ctx, "corr.segr", (size_t)(nwg * R.seg_cap), &R.seg_r));
+            PG_HIP(ctx, hipMemsetAsync(R.seg_cnt, 0, nwg * sizeof(unsigned long long), st));
+            hc.lower = 2.0f;
+            hc.ok = 1;
+            PG_HIP(ctx, hipMemcpyAsync(E.ctl, &hc, sizeof hc, hipMemcpyHostToDevice, st));
+        }
+        PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+    }
+    PG_HIP(ctx, hipStreamSynchronize(st));
+    long long kept = 0;
+    PG_HIP(ctx, hipMemcpy(R.hseg.data(), R.seg_cnt, nwg * 8, hipMemcpyDeviceToHost));
+    for (auto c : R.hseg) kept += (long long)c;
+    inf.band_pairs = kept;  // pairs kept for the selections
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
     inf.ms_zscore = ms;
+    float ms_hist = 0.0f;
     (void)hipEventElapsedTime(&ms_hist, ctx->ev[1], ctx->ev[2]);
     (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
     inf.ms_gram = ms + ms_hist;
-    (void)hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]);
-    inf.ms_select = ms;
-    (void)hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
-    inf.ms_csr = ms;
-    if (ran_hist && hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess) inf.ms_gram_hist = ms;
-    if (ran_emit && hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]) == hipSuccess) inf.ms_gram_emit = ms;
+    if (R.ran_hist && hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess) inf.ms_gram_hist = ms;
+    if (R.ran_emit && hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]) == hipSuccess) inf.ms_gram_emit = ms;
+    inf.gram_passes = R.gram_passes;
+    inf.nnz = -1;
+    E.base = inf;
+    E.valid = true;
+    if (info) *info = inf;
+    return PG_OK;
+}
 
+extern "C" pg_status pg_corr_select(pg_ctx *ctx, float thr, pg_corr_info *info)
+{
+    if (!ctx) return PG_EINVAL;
+    if (!ctx->emit || !ctx->emit->valid) return pg_fail(ctx, PG_ESTATE, "pg_corr_emit first");
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    PgEmit &E = *ctx->emit;
+    CorrRun &R = E.R;
+    hipStream_t st = R.st;
+    const double band = R.band;
+    const long long m1 = R.m1;
+    const int n = R.n;
+    pg_corr_info inf = E.base;
+    inf.band_pairs = inf.band_flips = 0;
+    inf.band_max_err = 0.0;
+    inf.ms_select = inf.ms_csr = 0.0;
+    unsigned long long *c = R.counters;
+    double *tstar_d = nullptr;
+    PG_TRY(pg_bufT(ctx, "corr.tstar", 2, &tstar_d));
+    SelCtl hc;
+    std::memset(&hc, 0, sizeof hc);
+    hc.lower = E.lower;
+    hc.ok = 1;
+    long long ccap = 1 << 16;
+    PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+    if (E.kind == PG_THR_R) {
+        if (!(thr >= -1.0f && thr <= 1.0f)) return pg_fail(ctx, PG_ETHRESH, "r threshold %g outside [-1, 1]", (double)thr);
+        const float thr_eff = thr_effective(thr, R.weighted);
+        hc.L2 = round_down_f((double)thr_eff - band);
+        hc.U2 = round_up_f((double)thr_eff + band);
+        if (hc.L2 < E.lower)
+            return pg_fail(ctx, PG_ETHRESH, "r threshold %g is looser than the emitted one", (double)thr);
+        long long nnz = 0;
+        Window win;
+        for (int attempt = 0; attempt < 2; attempt++) {
+            PG_HIP(ctx, hipMemcpyAsync(E.ctl, &hc, sizeof hc, hipMemcpyHostToDevice, st));
+            PG_TRY(window_candidates(R, E.ctl, ccap, &win));
+            accept_kernel<<<(unsigned)pg_cdiv(ccap, 256), 256, 0, st>>>(win.cand, win.cand_r, win.cand64, c + 0, ccap,
+                                                                        (double)thr_eff, win.extra, c + 1, c + 2);
+            definite_sum<<<1, 1024, 0, st>>>(win.dpart, win.nd, c + 3);
+            PG_KERNEL_CHECK(ctx);
+            PG_HIP(ctx, hipMemcpyAsync(R.hcnt, c, sizeof R.hcnt, hipMemcpyDeviceToHost, st));
+            PG_HIP(ctx, hipStreamSynchronize(st));
+            if ((long long)R.hcnt[0] <= ccap) break;
+            ccap = (long long)R.hcnt[0] + (long long)R.hcnt[0] / 8 + 4096;
+        }
+        if ((long long)R.hcnt[0] > ccap) return pg_fail(ctx, PG_ENOMEM, "candidate list overflowed");
+        inf.band_pairs = (long long)R.hcnt[0];
+        inf.band_flips = (long long)R.hcnt[2];
+        inf.band_max_err = host_err(R.hcnt[5]);
+        nnz = 2 * ((long long)R.hcnt[3] + (long long)R.hcnt[1]);
+        inf.nnz = nnz;
+        inf.r_thresh = thr;
+        inf.t_star = (double)thr_eff;
+        inf.s_thresh = (float)(m1 > 0 ? 100.0 * (double)nnz / (double)m1 / 2.0 : 0.0);
+        EdgeSrc es = R.edges(win.extra, c + 1, ccap);
+        es.seg_r = R.seg_r;
+        es.keep_above_dev = &E.ctl->U2;
+        PG_TRY(R.finish_csr(es, nnz));
+    } else {
+        if (!(thr > 0.0f && thr <= 100.0f)) return pg_fail(ctx, PG_ETHRESH, "sparsity %g%% outside (0, 100]", (double)thr);
+        const long long kwant = sparsity_k(m1, thr);
+        if (kwant > E.kwant_max)
+            return pg_fail(ctx, PG_ETHRESH, "sparsity %g%% keeps more pairs than the emitted one", (double)thr);
+        inf.s_thresh = thr;
+        if (kwant == 0 || n < 2) {
+            Window win;
+            PG_TRY(pg_bufT(ctx, "corr.extra", 1, &win.extra));
+            PG_HIP(ctx, hipMemsetAsync(c, 0, 16 * sizeof(unsigned long long), st));
+            EdgeSrc es = R.edges(win.extra, c + 1, 1);
+            es.seg_r = R.seg_r;
+            es.keep_above = 2.0f;  // no r~ passes: no segment entry is kept
+            PG_TRY(R.finish_csr(es, 0));
+            inf.nnz = 0;
+            inf.r_thresh = 1.0f;
+            inf.t_star = 1.0;
+        } else {
+            ccap = std::min<long long>(1 << 22, std::max<long long>(1 << 16, kwant / 4));
+            bool done = false;
+            for (int attempt = 0; attempt < 2 && !done; attempt++) {
+                PG_HIP(ctx, hipMemcpyAsync(E.ctl, &hc, sizeof hc, hipMemcpyHostToDevice, st));
+                window_bounds<<<1, 1024, 0, st>>>(E.h2t, kwant, band, E.ctl);
+                Window win;
+                PG_TRY(window_select(R, E.ctl, ccap, kwant, tstar_d, &win));
+                EdgeSrc es = R.edges(win.extra, c + 1, ccap);
+                es.seg_r = R.seg_r;
+                es.keep_above_dev = &E.ctl->U2;
+                PG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+                PG_TRY(R.launch_csr(es, 2 * kwant));
+                PG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+                SelCtl hw;
+                int rn = -1;
+                PG_HIP(ctx, hipMemcpyAsync(R.hcnt, c, sizeof R.hcnt, hipMemcpyDeviceToHost, st));
+                PG_HIP(ctx, hipMemcpyAsync(&hw, E.ctl, sizeof hw, hipMemcpyDeviceToHost, st));
+                PG_HIP(ctx, hipMemcpyAsync(&inf.t_star, tstar_d, 8, hipMemcpyDeviceToHost, st));
+                PG_HIP(ctx, hipMemcpyAsync(&rn, R.row + n, 4, hipMemcpyDeviceToHost, st));
+                PG_HIP(ctx, hipStreamSynchronize(st));
+                if (hw.ok != 1) return pg_fail(ctx, PG_ETHRESH, "selection window below the emitted pairs");
+                if ((long long)R.hcnt[0] > ccap) {
+                    ccap = (long long)R.hcnt[0] + (long long)R.hcnt[0] / 8 + 4096;
+                    continue;
+                }
+                if (((int *)R.hcnt)[8] != 0 || (long long)R.hcnt[3] + (long long)R.hcnt[1] != kwant ||
+                    (long long)rn != 2 * kwant)
+                    return pg_fail(ctx, PG_ETHRESH, "selection of k=%lld failed (definite %lld, selected %lld)", kwant,
+                                   (long long)R.hcnt[3], (long long)R.hcnt[1]);
+                done = true;
+            }
+            if (!done) return pg_fail(ctx, PG_ENOMEM, "candidate list overflowed");
+            inf.band_pairs = (long long)R.hcnt[0];
+            inf.band_flips = (long long)R.hcnt[2];
+            inf.band_max_err = host_err(R.hcnt[5]);
+            inf.nnz = 2 * kwant;
+            inf.r_thresh = (float)inf.t_star;
+        }
+    }
+    PG_TRY(check_band(ctx, inf));
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]) == hipSuccess) inf.ms_select = ms;
+    if (hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]) == hipSuccess) inf.ms_csr = ms;
     ctx->cn = n;
-    ctx->ct = t;
+    ctx->ct = R.t;
     ctx->cnnz = inf.nnz;
-    ctx->cweighted = weighted;
-    ctx->c_row = row;
-    ctx->c_col = col;
-    ctx->c_val = val;
-    ctx->c_z64 = z64;
+    ctx->cweighted = R.weighted;
+    ctx->c_row = R.row;
+    ctx->c_col = R.col;
+    ctx->c_val = R.val;
+    ctx->c_z64 = R.z64;
     if (info) *info = inf;
     return PG_OK;
 }

@@ -114,6 +114,11 @@ pg_status pg_create(pg_ctx **out, int device)
         return PG_EDEVICE;
     }
     for (auto &e : ctx->ev) (void)hipEventCreate(&e);
+    // the allocation-free entry points' scratch address on this device
+    if (pg_static_scratch(ctx, &ctx->static_scratch) != PG_OK) {
+        pg_destroy(ctx);
+        return PG_EDEVICE;
+    }
     *out = ctx;
     return PG_OK;
 }
@@ -123,6 +128,7 @@ void pg_destroy(pg_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    pg_emit_free(ctx->emit);
     for (auto &kv : ctx->bufs)
         if (kv.second.p) (void)hipFree(kv.second.p);
     for (auto &e : ctx->ev)

@@ -369,13 +369,16 @@ __global__ __launch_bounds__(256) void csr_validate(const int *__restrict__ row,
 }
 }  // namespace
 
-pg_status pg_validate_csr(pg_ctx *ctx, const int32_t *drow, const int32_t *dcol, int32_t n, int64_t nnz,
-                          bool sorted)
+namespace {
+__global__ void flag_clear(int *f) { *f = 0; }
+}  // namespace
+
+pg_status pg_validate_csr_flag(pg_ctx *ctx, const int32_t *drow, const int32_t *dcol, int32_t n, int64_t nnz,
+                               bool sorted, int *bad)
 {
     hipStream_t st = ctx->stream;
-    int *bad, hbad = 0;
-    PG_TRY(pg_bufT(ctx, "csr.bad", 1, &bad));
-    PG_HIP(ctx, hipMemsetAsync(bad, 0, 4, st));
+    int hbad = 0;
+    flag_clear<<<1, 1, 0, st>>>(bad);
     const long long waves = std::max<long long>(n, 1);
     csr_validate<<<(unsigned)pg_cdiv(waves, 4), 256, 0, st>>>(drow, dcol, n, nnz, sorted ? 1 : 0, bad);
     PG_KERNEL_CHECK(ctx);
@@ -385,4 +388,12 @@ pg_status pg_validate_csr(pg_ctx *ctx, const int32_t *drow, const int32_t *dcol,
         return pg_fail(ctx, PG_EFORMAT, "malformed CSR (row not 0..nnz non-decreasing, col out of range%s)",
                        sorted ? " or columns not ascending in a row" : "");
     return PG_OK;
+}
+
+pg_status pg_validate_csr(pg_ctx *ctx, const int32_t *drow, const int32_t *dcol, int32_t n, int64_t nnz,
+                          bool sorted)
+{
+    int *bad;
+    PG_TRY(pg_bufT(ctx, "csr.bad", 1, &bad));
+    return pg_validate_csr_flag(ctx, drow, dcol, n, nnz, sorted, bad);
 }

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "pg_internal.hpp"
@@ -230,45 +231,74 @@ __global__ __launch_bounds__(256) void cpw_kernel(const int *__restrict__ row, c
 
 constexpr int S = 64;  // sources per batch = lanes of a wave64
 constexpr int GATHER = 8;
+constexpr int MAX_BATCHES = 16;  // batches per launch (blockIdx.y)
 
-__global__ void bfs_init(int n, int nsrc, u64 *__restrict__ sigma, int *__restrict__ lev,
-                         u64 *__restrict__ vis, u64 *__restrict__ p0, u64 *__restrict__ p1)
+// Per-launch state of up to MAX_BATCHES batches of S sources: batch b's
+// arrays start at b * n * S (per-source rows), b * n (per-vertex masks),
+// b * S (sources, -1 for an absent lane), b * lcap (level counters) and
+// b * n (its share of the BC sums).
+struct BatchBufs {
+    int nb = 1;            // batches per launch
+    int lcap = 0;          // level counters per batch
+    u64 *sigma, *visA, *visB, *p0, *p1;
+    int *lev, *srcs;
+    double *coef, *bcpart;
+    unsigned *prog;        // [nb][lcap]: waves that found new vertices at level d + 1
+    unsigned *tot;         // [lcap]: the same summed over the batches
+};
+
+__global__ void bfs_init(int n, BatchBufs B)
 {
-    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (long long)n * S) return;
-    sigma[k] = 0;
-    lev[k] = -1;
-    if ((k & (S - 1)) == 0) {
-        long long v = k / S;
-        vis[v] = nsrc >= S ? 0ull : (~0ull << nsrc);  // absent lanes count as visited
-        p0[v] = 0;
-        p1[v] = 0;
+    const long long ns = (long long)n * S;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;  // ns is a multiple of 64: whole waves return
+    const int bb = blockIdx.y, lane = threadIdx.x & 63;
+    const long long o = (long long)bb * ns + k;
+    B.sigma[o] = 0;
+    B.lev[o] = -1;
+    // absent lanes count as visited
+    const u64 absent = __ballot(B.srcs[bb * S + lane] < 0);
+    if (lane == 0) {
+        const long long v = (long long)bb * n + k / S;
+        B.visA[v] = absent;
+        B.p0[v] = 0;
+        B.p1[v] = 0;
+    }
+    // the level counters restart with the group (lcap <= n + 2 <= n * S)
+    if (k < B.lcap) {
+        B.prog[(long long)bb * B.lcap + k] = 0u;
+        if (bb == 0) B.tot[k] = 0u;
     }
 }
 
-__global__ void bfs_seed(int nsrc, const int *__restrict__ srcs, u64 *__restrict__ sigma,
-                         int *__restrict__ lev, u64 *__restrict__ vis)
+__global__ void bfs_seed(int n, BatchBufs B)
 {
-    int s = threadIdx.x;
-    if (s >= nsrc) return;
-    int v = srcs[s];
-    sigma[(long long)v * S + s] = 1;
-    lev[(long long)v * S + s] = 0;
-    atomicOr(&vis[v], 1ull << s);
+    const int s = threadIdx.x, bb = blockIdx.y;
+    const int v = B.srcs[bb * S + s];
+    if (v < 0) return;
+    const long long o = (long long)bb * n * S + (long long)v * S + s;
+    B.sigma[o] = 1;
+    B.lev[o] = 0;
+    atomicOr(&B.visA[(long long)bb * n + v], 1ull << s);
 }
 
-// Level d -> d+1.  A vertex w not yet reached from source s collects
-// sigma[w][s] = sum of sigma[v][s] over neighbours v reached from s; in an
-// undirected graph those are exactly its level-d neighbours.
+// Level d -> d+1 of every batch.  A vertex w not yet reached from source s
+// collects sigma[w][s] = sum of sigma[v][s] over neighbours v reached from s;
+// in an undirected graph those are exactly its level-d neighbours.  A batch
+// whose level d is empty (no progress at d - 1) returns at once.
 __global__ __launch_bounds__(256) void bfs_forward(const int *__restrict__ row, const int *__restrict__ col, int n,
-                                                   int d, const u64 *__restrict__ visA, u64 *__restrict__ visB,
-                                                   u64 *__restrict__ sigma, int *__restrict__ lev,
-                                                   u64 *__restrict__ p0, u64 *__restrict__ p1,
-                                                   unsigned *__restrict__ progress)
+                                                   int d, int flip, BatchBufs B)
 {
+    const int bb = blockIdx.y;
+    unsigned *prog = B.prog + (long long)bb * B.lcap;
+    if (d > 0 && prog[d - 1] == 0u) return;
     const int lane = threadIdx.x & 63;
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= n) return;
+    const long long vb0 = (long long)bb * n;
+    const u64 *__restrict__ visA = (flip ? B.visB : B.visA) + vb0;
+    u64 *__restrict__ visB = (flip ? B.visA : B.visB) + vb0;
+    u64 *__restrict__ sigma = B.sigma + vb0 * S;
     const u64 vis = visA[w];
     const u64 need = ~vis;
     if (need == 0) {
@@ -316,14 +346,15 @@ __global__ __launch_bounds__(256) void bfs_forward(const int *__restrict__ row, 
     const u64 nbits = __ballot(fresh);
     if (fresh) {
         sigma[(long long)w * S + lane] = acc;
-        lev[(long long)w * S + lane] = d + 1;
+        B.lev[(vb0 + w) * S + lane] = d + 1;
     }
     if (lane == 0) {
         visB[w] = vis | nbits;
         if (nbits) {
-            if ((d + 1) & 1) p0[w] |= nbits;
-            if ((d + 1) & 2) p1[w] |= nbits;
-            atomicAdd(progress, 1u);
+            if ((d + 1) & 1) B.p0[vb0 + w] |= nbits;
+            if ((d + 1) & 2) B.p1[vb0 + w] |= nbits;
+            atomicAdd(&prog[d], 1u);
+            atomicAdd(&B.tot[d], 1u);
         }
     }
 }
@@ -336,19 +367,27 @@ __global__ void bc_coef_init(long long total, const u64 *__restrict__ sigma, dou
     coef[k] = s ? 1.0 / (double)s : 0.0;
 }
 
-// Level d: delta[v][s] = sigma[v][s] * sum over children w (level d+1) of
-// (1 + delta[w][s]) / sigma[w][s]; coef[v][s] = (1 + delta[v][s]) / sigma[v][s].
-// BC[v] += sum over sources s != v of delta[v][s] (fixed lane order).
+// Level d of every batch: delta[v][s] = sigma[v][s] * sum over children w
+// (level d+1) of (1 + delta[w][s]) / sigma[w][s]; coef[v][s] = (1 + delta[v][s]) /
+// sigma[v][s].  The batch's BC share bcpart[v] += sum over sources s != v of
+// delta[v][s] (fixed lane order).  A batch without level d + 1 returns.  vis:
+// the final visited masks (either ping-pong buffer holds them once the
+// batch's BFS has ended).
 __global__ __launch_bounds__(256) void bc_backward(const int *__restrict__ row, const int *__restrict__ col, int n,
-                                                   int d, const u64 *__restrict__ vis, const u64 *__restrict__ p0,
-                                                   const u64 *__restrict__ p1, const u64 *__restrict__ sigma,
-                                                   const int *__restrict__ lev, double *__restrict__ coef,
-                                                   const int *__restrict__ srcs, double *__restrict__ bc)
+                                                   int d, BatchBufs B)
 {
+    const int bb = blockIdx.y;
+    if (B.prog[(long long)bb * B.lcap + d] == 0u) return;
     const int lane = threadIdx.x & 63;
     const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (v >= n) return;
-    const bool own = lev[(long long)v * S + lane] == d;
+    const long long vb0 = (long long)bb * n;
+    const u64 *__restrict__ vis = B.visA + vb0;
+    const u64 *__restrict__ p0 = B.p0 + vb0;
+    const u64 *__restrict__ p1 = B.p1 + vb0;
+    const u64 *__restrict__ sigma = B.sigma + vb0 * S;
+    double *__restrict__ coef = B.coef + vb0 * S;
+    const bool own = B.lev[(vb0 + v) * S + lane] == d;
     const u64 ownm = __ballot(own);
     if (ownm == 0) return;
     const u64 x0 = ((d + 1) & 1) ? ~0ull : 0ull;
@@ -386,71 +425,101 @@ __global__ __launch_bounds__(256) void bc_backward(const int *__restrict__ row, 
         const double sg = (double)sigma[(long long)v * S + lane];
         const double dl = sg * acc;
         coef[(long long)v * S + lane] = (1.0 + dl) / sg;
-        if (srcs[lane] != v) contrib = dl;
+        if (B.srcs[bb * S + lane] != v) contrib = dl;
     }
     // fixed-order butterfly over the wave
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off);
-    if (lane == 0) bc[v] += contrib;
+    if (lane == 0) B.bcpart[vb0 + v] += contrib;
 }
 
-__global__ void bfs_export(int n, int nsrc, const int *__restrict__ lev, int *__restrict__ out)
+// bc[v] = sum over batch slots of their shares, in slot order
+__global__ void bc_reduce(const double *__restrict__ part, int nb, int n, double *__restrict__ bc)
 {
-    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (long long)n * nsrc) return;
-    long long s = k / n, v = k % n;
-    out[k] = lev[v * S + s];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += part[(long long)b * n + v];
+    bc[v] = s;
 }
 
-struct BatchBufs {
-    u64 *sigma, *visA, *visB, *p0, *p1;
-    int *lev, *srcs;
-    double *coef;
-    unsigned *progress;
-};
-
-pg_status batch_bufs(pg_ctx *ctx, int n, BatchBufs &B, bool need_coef)
+__global__ void fill_f64(double *__restrict__ p, long long count)
 {
-    const size_t ns = (size_t)n * S;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) p[k] = 0.0;
+}
+
+// Levels of batch bb's sources into out[(bb*S + s) * n + v] for its first
+// nsrc[bb] lanes.
+__global__ void bfs_export(int n, int nvalid, const int *__restrict__ lev, int *__restrict__ out)
+{
+    const int bb = blockIdx.y;
+    const int nb = min(S, nvalid - bb * S);
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nb <= 0 || k >= (long long)n * nb) return;
+    const long long s = k / n, v = k % n;
+    out[((long long)bb * S + s) * n + v] = lev[((long long)bb * n + v) * S + s];
+}
+
+pg_status batch_bufs(pg_ctx *ctx, int n, int nb, int lcap, BatchBufs &B, bool need_coef)
+{
+    const size_t ns = (size_t)n * S * nb, nv = (size_t)n * nb + 1;
+    B.nb = nb;
+    B.lcap = lcap;
     PG_TRY(pg_bufT(ctx, "bc.sigma", ns, &B.sigma));
     PG_TRY(pg_bufT(ctx, "bc.lev", ns, &B.lev));
-    PG_TRY(pg_bufT(ctx, "bc.visA", (size_t)n + 1, &B.visA));
-    PG_TRY(pg_bufT(ctx, "bc.visB", (size_t)n + 1, &B.visB));
-    PG_TRY(pg_bufT(ctx, "bc.p0", (size_t)n + 1, &B.p0));
-    PG_TRY(pg_bufT(ctx, "bc.p1", (size_t)n + 1, &B.p1));
-    PG_TRY(pg_bufT(ctx, "bc.srcs", S, &B.srcs));
-    PG_TRY(pg_bufT(ctx, "bc.progress", 4, &B.progress));
+    PG_TRY(pg_bufT(ctx, "bc.visA", nv, &B.visA));
+    PG_TRY(pg_bufT(ctx, "bc.visB", nv, &B.visB));
+    PG_TRY(pg_bufT(ctx, "bc.p0", nv, &B.p0));
+    PG_TRY(pg_bufT(ctx, "bc.p1", nv, &B.p1));
+    PG_TRY(pg_bufT(ctx, "bc.srcs", (size_t)S * nb, &B.srcs));
+    PG_TRY(pg_bufT(ctx, "bc.prog", (size_t)nb * lcap, &B.prog));
+    PG_TRY(pg_bufT(ctx, "bc.tot", (size_t)lcap, &B.tot));
+    PG_TRY(pg_bufT(ctx, "bc.part", nv, &B.bcpart));
     B.coef = nullptr;
     if (need_coef) PG_TRY(pg_bufT(ctx, "bc.coef", ns, &B.coef));
     return PG_OK;
 }
 
-// Forward sweep of one batch; returns the deepest level reached and leaves
-// the final visited masks in *vis_final.
-pg_status run_forward(pg_ctx *ctx, const int *row, const int *col, int n, int nsrc, BatchBufs &B,
-                      int *maxlev, u64 **vis_final)
+// Forward sweep of one group of batches (sources already in B.srcs), host
+// driven: a level is launched while some batch progressed at the previous
+// one.  Returns the number of levels launched after level 0; the final
+// visited masks are in B.visA (and B.visB).
+pg_status run_forward(pg_ctx *ctx, const int *row, const int *col, int n, BatchBufs &B, int *maxlev)
 {
     hipStream_t st = ctx->stream;
     const long long ns = (long long)n * S;
-    bfs_init<<<(unsigned)pg_cdiv(ns, 256), 256, 0, st>>>(n, nsrc, B.sigma, B.lev, B.visA, B.p0, B.p1);
-    bfs_seed<<<1, S, 0, st>>>(nsrc, B.srcs, B.sigma, B.lev, B.visA);
+    bfs_init<<<dim3((unsigned)pg_cdiv(ns, 256), (unsigned)B.nb), 256, 0, st>>>(n, B);
+    bfs_seed<<<dim3(1, (unsigned)B.nb), S, 0, st>>>(n, B);
     PG_KERNEL_CHECK(ctx);
-    u64 *va = B.visA, *vb = B.visB;
     int d = 0;
     for (;; d++) {
-        PG_HIP(ctx, hipMemsetAsync(B.progress, 0, 4, st));
-        bfs_forward<<<(unsigned)pg_cdiv(n, 4), 256, 0, st>>>(row, col, n, d, va, vb, B.sigma, B.lev, B.p0,
-                                                             B.p1, B.progress);
+        if (d + 1 >= B.lcap) return pg_fail(ctx, PG_ENOTSUP, "BFS deeper than %d levels", B.lcap - 1);
+        bfs_forward<<<dim3((unsigned)pg_cdiv(n, 4), (unsigned)B.nb), 256, 0, st>>>(row, col, n, d, d & 1, B);
         PG_KERNEL_CHECK(ctx);
         unsigned prog = 0;
-        PG_HIP(ctx, hipMemcpyAsync(&prog, B.progress, 4, hipMemcpyDeviceToHost, st));
+        PG_HIP(ctx, hipMemcpyAsync(&prog, B.tot + d, 4, hipMemcpyDeviceToHost, st));
         PG_HIP(ctx, hipStreamSynchronize(st));
-        std::swap(va, vb);
-        if (prog == 0 || d > n) break;
+        if (prog == 0) break;
     }
     *maxlev = d;  // levels 0..d hold vertices
-    *vis_final = va;
     return PG_OK;
+}
+
+// Batches per launch: from PAGANI_BC_BATCHES when set (1..16), else enough
+// 64-source batches for about 16384 work-groups of four vertex waves, capped
+// by the sources and by 1 GiB of per-source state.
+int bc_batches(pg_ctx *ctx, int n, long long nsrc)
+{
+    int nb = 0;
+    if (const char *e = std::getenv("PAGANI_BC_BATCHES")) nb = std::atoi(e);
+    if (nb <= 0) nb = pg_tune_get(ctx, "bc_batches", n, 0, 0);
+    if (nb <= 0) {
+        nb = (int)std::max<long long>(1, std::min<long long>(8, pg_cdiv(65536, std::max(n, 1))));
+        while (nb > 1 && (long long)nb * n * S * 20 > (1ll << 30)) nb--;
+    }
+    nb = std::max(1, std::min(MAX_BATCHES, nb));
+    return (int)std::max<long long>(1, std::min<long long>(nb, pg_cdiv(std::max<long long>(nsrc, 1), S)));
 }
 
 }  // namespace
@@ -560,6 +629,69 @@ extern "C" pg_status pg_cp_binary(pg_ctx *ctx, const int32_t *row, const int32_t
     return PG_OK;
 }
 
+// ------------------------------------------- allocation-free binary Cp
+
+namespace {
+// Fixed scratch of pg_cp_binary_static: triangle counts of up to
+// PG_CP_STATIC_MAX_N vertices, then the FP64 mean, then the CSR check flag.
+// One copy per GPU, held while the library is loaded; host code reaches it
+// only through the address pg_static_scratch looks up per device.
+constexpr size_t STATIC_TRI_BYTES = (size_t)PG_CP_STATIC_MAX_N * 8;
+constexpr size_t STATIC_SCRATCH_BYTES = STATIC_TRI_BYTES + 256;
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+pg_status pg_static_scratch(pg_ctx *ctx, void **ptr)
+{
+    if (ctx->device < 0 || ctx->device >= MAX_DEVICES)
+        return pg_fail(ctx, PG_ENOTSUP, "device %d beyond the scratch table", ctx->device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        PG_HIP(ctx, hipSetDevice(ctx->device));
+        void *p = nullptr;
+        PG_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[ctx->device] = p;
+    }
+    *ptr = g_scratch_ptr[ctx->device];
+    return PG_OK;
+}
+
+extern "C" pg_status pg_cp_binary_static(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
+                                         int64_t nnz, float *cp, double *mean)
+{
+    if (!ctx) return PG_EINVAL;
+    PG_TRY(pg_check_csr_args(ctx, row, col, n, nnz));
+    if (!cp) return pg_fail(ctx, PG_EINVAL, "cp is NULL");
+    if (n > PG_CP_STATIC_MAX_N)
+        return pg_fail(ctx, PG_ENOTSUP, "n=%d exceeds the fixed scratch of %d vertices", n, PG_CP_STATIC_MAX_N);
+    // staging a host buffer would need device memory
+    if (!pg_is_device_ptr(row) || (nnz > 0 && !pg_is_device_ptr(col)) || (n > 0 && !pg_is_device_ptr(cp)))
+        return pg_fail(ctx, PG_EINVAL, "row, col and cp must be device memory");
+    if (!ctx->static_scratch) return pg_fail(ctx, PG_ESTATE, "no scratch address for device %d", ctx->device);
+    PG_HIP(ctx, hipSetDevice(ctx->device));
+    unsigned char *base = (unsigned char *)ctx->static_scratch;
+    long long *dtri = (long long *)base;
+    double *dmean = (double *)(base + STATIC_TRI_BYTES);
+    int *bad = (int *)(base + STATIC_TRI_BYTES + 64);
+    PG_TRY(pg_validate_csr_flag(ctx, row, col, n, nnz, true, bad));
+    const int nw = (n + 31) / 32;
+    if ((size_t)nw * 4 + 4096 > ctx->lds_per_block) return pg_fail(ctx, PG_ENOTSUP, "n too large for LDS bitmap");
+    hipStream_t st = ctx->stream;
+    if (n > 0) {
+        triangles_kernel<<<(unsigned)n, 256, (size_t)nw * 4, st>>>(row, col, n, dtri);
+        cp_from_tri<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(row, dtri, n, cp);
+    }
+    mean_kernel<float><<<1, 256, 0, st>>>(cp, n, dmean);
+    PG_KERNEL_CHECK(ctx);
+    if (mean) PG_HIP(ctx, hipMemcpyAsync(mean, dmean, 8, pg_is_device_ptr(mean) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+    PG_HIP(ctx, hipStreamSynchronize(st));
+    return PG_OK;
+}
+
 extern "C" pg_status pg_cp_weighted(pg_ctx *ctx, const int32_t *row, const int32_t *col, const float *val,
                                     int32_t n, int64_t nnz, int32_t kind, int32_t wmax_mode, double *cp,
                                     double *mean)
@@ -614,24 +746,25 @@ extern "C" pg_status pg_bc(pg_ctx *ctx, const int32_t *row, const int32_t *col, 
     double *dbc;
     PG_TRY(pg_out(ctx, bc, (size_t)n, "out.bc", &dbc));
     hipStream_t st = ctx->stream;
-    if (n > 0) PG_HIP(ctx, hipMemsetAsync(dbc, 0, (size_t)n * 8, st));
+    if (n == 0) return PG_OK;
+    const int nb = bc_batches(ctx, n, nsrc);
     BatchBufs B;
-    PG_TRY(batch_bufs(ctx, n, B, true));
+    PG_TRY(batch_bufs(ctx, n, nb, n + 2, B, true));
+    fill_f64<<<(unsigned)pg_cdiv((long long)nb * n, 256), 256, 0, st>>>(B.bcpart, (long long)nb * n);
     const long long ns = (long long)n * S;
-    for (long long b0 = 0; b0 < nsrc; b0 += S) {
-        const int nb = (int)std::min<long long>(S, nsrc - b0);
-        int pad[S];
-        for (int s = 0; s < S; s++) pad[s] = s < nb ? hsrc[b0 + s] : -1;
-        PG_HIP(ctx, hipMemcpyAsync(B.srcs, pad, sizeof pad, hipMemcpyHostToDevice, st));
+    std::vector<int32_t> pad((size_t)nb * S);
+    for (long long g0 = 0; g0 < nsrc; g0 += (long long)nb * S) {
+        for (int s = 0; s < nb * S; s++) pad[s] = g0 + s < nsrc ? hsrc[g0 + s] : -1;
+        PG_HIP(ctx, hipMemcpyAsync(B.srcs, pad.data(), pad.size() * 4, hipMemcpyHostToDevice, st));
         int maxlev = 0;
-        u64 *vis = nullptr;
-        PG_TRY(run_forward(ctx, drow, dcol, n, nb, B, &maxlev, &vis));
-        bc_coef_init<<<(unsigned)pg_cdiv(ns, 256), 256, 0, st>>>(ns, B.sigma, B.coef);
+        PG_TRY(run_forward(ctx, drow, dcol, n, B, &maxlev));
+        bc_coef_init<<<(unsigned)pg_cdiv(ns * nb, 256), 256, 0, st>>>(ns * nb, B.sigma, B.coef);
         for (int d = maxlev - 1; d >= 1; d--)
-            bc_backward<<<(unsigned)pg_cdiv(n, 4), 256, 0, st>>>(drow, dcol, n, d, vis, B.p0, B.p1, B.sigma,
-                                                                 B.lev, B.coef, B.srcs, dbc);
+            bc_backward<<<dim3((unsigned)pg_cdiv(n, 4), (unsigned)nb), 256, 0, st>>>(drow, dcol, n, d, B);
         PG_KERNEL_CHECK(ctx);
     }
+    bc_reduce<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(B.bcpart, nb, n, dbc);
+    PG_KERNEL_CHECK(ctx);
     PG_TRY(pg_out_finish(ctx, bc, dbc, (size_t)n));
     PG_HIP(ctx, hipStreamSynchronize(st));
     return PG_OK;
@@ -678,19 +811,22 @@ extern "C" pg_status pg_bfs(pg_ctx *ctx, const int32_t *row, const int32_t *col,
         if (s < 0 || s >= n) return pg_fail(ctx, PG_EINVAL, "source %d out of range", s);
     int *dlev;
     PG_TRY(pg_out(ctx, level, (size_t)n * nsrc, "out.level", &dlev));
-    BatchBufs B;
-    PG_TRY(batch_bufs(ctx, n, B, false));
     hipStream_t st = ctx->stream;
-    for (long long b0 = 0; b0 < nsrc; b0 += S) {
-        const int nb = (int)std::min<long long>(S, nsrc - b0);
-        int pad[S];
-        for (int s = 0; s < S; s++) pad[s] = s < nb ? hsrc[b0 + s] : -1;
-        PG_HIP(ctx, hipMemcpyAsync(B.srcs, pad, sizeof pad, hipMemcpyHostToDevice, st));
-        int maxlev = 0;
-        u64 *vis = nullptr;
-        PG_TRY(run_forward(ctx, drow, dcol, n, nb, B, &maxlev, &vis));
-        bfs_export<<<(unsigned)pg_cdiv((long long)n * nb, 256), 256, 0, st>>>(n, nb, B.lev, dlev + b0 * n);
-        PG_KERNEL_CHECK(ctx);
+    if (n > 0 && nsrc > 0) {
+        const int nb = bc_batches(ctx, n, nsrc);
+        BatchBufs B;
+        PG_TRY(batch_bufs(ctx, n, nb, n + 2, B, false));
+        std::vector<int32_t> pad((size_t)nb * S);
+        for (long long g0 = 0; g0 < nsrc; g0 += (long long)nb * S) {
+            for (int s = 0; s < nb * S; s++) pad[s] = g0 + s < nsrc ? hsrc[g0 + s] : -1;
+            PG_HIP(ctx, hipMemcpyAsync(B.srcs, pad.data(), pad.size() * 4, hipMemcpyHostToDevice, st));
+            int maxlev = 0;
+            PG_TRY(run_forward(ctx, drow, dcol, n, B, &maxlev));
+            const int nvalid = (int)std::min<long long>((long long)nb * S, nsrc - g0);
+            bfs_export<<<dim3((unsigned)pg_cdiv((long long)n * S, 256), (unsigned)nb), 256, 0, st>>>(n, nvalid, B.lev,
+                                                                                                 dlev + g0 * n);
+            PG_KERNEL_CHECK(ctx);
+        }
     }
     PG_TRY(pg_out_finish(ctx, level, dlev, (size_t)n * nsrc));
     PG_HIP(ctx, hipStreamSynchronize(st));

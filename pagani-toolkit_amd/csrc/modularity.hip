@@ -302,7 +302,8 @@ extern "C" pg_status pg_modularity_ex(pg_ctx *ctx, const int32_t *row, const int
     if (kmode != PG_MOD_STRENGTH && kmode != PG_MOD_BINARY_K) return pg_fail(ctx, PG_EINVAL, "bad kmode");
     if (opts->stop != PG_MOD_STOP_VECTOR && opts->stop != PG_MOD_STOP_RAYLEIGH)
         return pg_fail(ctx, PG_EINVAL, "bad stop rule %d", opts->stop);
-    if (opts->tol < 0.0 || opts->max_iter < 0) return pg_fail(ctx, PG_EINVAL, "bad tol / max_iter");
+    if (opts->tol < 0.0 || opts->max_iter < 0 || opts->max_depth < 0)
+        return pg_fail(ctx, PG_EINVAL, "bad tol / max_iter / max_depth");
     const double tol = opts->tol > 0.0 ? opts->tol : RAYLEIGH_TOL;
     const int max_iter = opts->max_iter > 0 ? opts->max_iter : MAX_ITER;
     PG_TRY(pg_check_csr_args(ctx, row, col, n, nnz));
@@ -359,6 +360,15 @@ extern "C" pg_status pg_modularity_ex(pg_ctx *ctx, const int32_t *row, const int
     while (lo <= maxr) {
         const int hi = maxr + 1;
         const int G = hi - lo;
+        if (opts->max_depth > 0 && nwave >= opts->max_depth) {
+            // depth limit: this level's groups are final, labelled in round order
+            std::vector<char> used((size_t)G, 0);
+            for (int i = 0; i < n; i++)
+                if (res[i] >= lo && res[i] < hi) used[res[i] - lo] = 1;
+            for (int g = 0; g < G; g++)
+                if (used[g]) adj[lo + g] = index++;
+            break;
+        }
         const auto wave_t0 = std::chrono::steady_clock::now();
         // members of each round, in increasing global id
         std::vector<int> gcount((size_t)G, 0);

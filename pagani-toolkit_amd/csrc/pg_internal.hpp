@@ -17,6 +17,8 @@ struct PgBuf {
     size_t cap = 0;
 };
 
+struct PgEmit;  // kept pairs of pg_corr_emit (corr.hip)
+
 struct pg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -25,6 +27,8 @@ struct pg_ctx {
     std::string err;
     std::map<std::string, PgBuf> bufs;  // named grow-only device buffers
     hipEvent_t ev[12] = {};
+    void *static_scratch = nullptr;  // this device's fixed scratch array (pg_static_scratch)
+    PgEmit *emit = nullptr;          // pg_corr_emit state, owned
 
     // result of the last pg_corr_build
     int32_t cn = 0, ct = 0;
@@ -123,3 +127,24 @@ pg_status pg_zscore64(pg_ctx *ctx, const float *dbold, int layout, int n, int t,
 // each row.  PG_EFORMAT otherwise.  Synchronises the stream.
 pg_status pg_validate_csr(pg_ctx *ctx, const int32_t *drow, const int32_t *dcol, int32_t n, int64_t nnz,
                           bool sorted);
+// The same check with a caller-provided device flag (no allocation).
+pg_status pg_validate_csr_flag(pg_ctx *ctx, const int32_t *drow, const int32_t *dcol, int32_t n, int64_t nnz,
+                               bool sorted, int *bad);
+
+// Device address of this device's fixed scratch array of the allocation-free
+// entry points (graph.hip), looked up once per device; called by pg_create
+// with the device current.
+pg_status pg_static_scratch(pg_ctx *ctx, void **ptr);
+
+// pg_corr_emit state: dropped by any build that reuses its buffers, freed
+// with the context.
+void pg_emit_invalidate(pg_ctx *ctx);
+void pg_emit_free(PgEmit *e);
+
+// Autotuner cache (tune.cpp): the tuned value of a knob for this device's
+// architecture and the half-octave buckets of up to three sizes (0 or less
+// where unused), or 0 when none is stored; pg_tune_put stores one (and the
+// time it measured) in memory and in the cache file.
+std::string pg_arch(pg_ctx *ctx);
+int pg_tune_get(pg_ctx *ctx, const char *knob, long long a, long long b, long long c);
+void pg_tune_put(pg_ctx *ctx, const char *knob, long long a, long long b, long long c, int value, double ms);

@@ -166,6 +166,22 @@ int main(int argc, char *argv[])
             ctx.check(pg_cormat_finish(ctx.c), "pg_cormat_finish");
             if (!write_cormat(ctx, base + ".cormat", n)) return 2;
         }
+        // several thresholds: one Gram pass at the loosest, then one
+        // selection per threshold (U CorMat.cpp:300-346 computes r once)
+        const bool multi = thr.size() > 1;
+        if (multi) {
+            pg_corr_opts o;
+            o.thr_kind = rmode ? PG_THR_R : PG_THR_SPARSITY;
+            o.thr = rmode ? *std::min_element(thr.begin(), thr.end()) : *std::max_element(thr.begin(), thr.end());
+            o.weighted = weighted ? 1 : 0;
+            o.prec = prec;
+            o.layout = PG_BOLD_TIME_MAJOR;
+            pg_corr_info einfo;
+            pgtool::Timer tm;
+            ctx.check(pg_corr_emit(ctx.c, bold.data(), n, t, &o, &einfo), "pg_corr_emit");
+            std::cout << "GPU correlation (one Gram pass for " << thr.size() << " thresholds): "
+                      << tm.seconds() * 1000.0 << " ms, " << einfo.band_pairs << " pairs kept" << std::endl;
+        }
         for (float th : thr) {
             pg_corr_opts o;
             o.thr_kind = rmode ? PG_THR_R : PG_THR_SPARSITY;
@@ -175,7 +191,8 @@ int main(int argc, char *argv[])
             o.layout = PG_BOLD_TIME_MAJOR;
             pg_corr_info info;
             pgtool::Timer tm;
-            ctx.check(pg_corr_build(ctx.c, bold.data(), n, t, &o, &info), "pg_corr_build");
+            if (multi) ctx.check(pg_corr_select(ctx.c, th, &info), "pg_corr_select");
+            else ctx.check(pg_corr_build(ctx.c, bold.data(), n, t, &o, &info), "pg_corr_build");
             std::vector<int32_t> row(n + 1), col((size_t)std::max<int64_t>(info.nnz, 1));
             std::vector<float> val(weighted ? col.size() : 0);
             ctx.check(pg_corr_csr(ctx.c, row.data(), col.data(), weighted ? val.data() : nullptr), "pg_corr_csr");

@@ -38,7 +38,7 @@ class _Opts(C.Structure):
 
 class _ModOpts(C.Structure):
     _fields_ = [("kmode", C.c_int32), ("stop", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32),
-                ("progress", C.c_int32)]
+                ("progress", C.c_int32), ("max_depth", C.c_int32)]
 
 
 class _Info(C.Structure):
@@ -84,6 +84,8 @@ _SIGS = {
     "pg_abi_version": (C.c_int, []),
     "pg_synchronize": (C.c_int, [_VP]),
     "pg_corr_build": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.POINTER(_Opts), C.POINTER(_Info)]),
+    "pg_corr_emit": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.POINTER(_Opts), C.POINTER(_Info)]),
+    "pg_corr_select": (C.c_int, [_VP, C.c_float, C.POINTER(_Info)]),
     "pg_corr_csr": (C.c_int, [_VP, _VP, _VP, _VP]),
     "pg_corr_csr_view": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP)]),
     "pg_corr_z64_view": (C.c_int, [_VP, C.POINTER(_VP)]),
@@ -199,6 +201,21 @@ class Context:
         o = _Opts(THR_SPARSITY if sparsity else THR_R, thr, int(weighted), prec, layout)
         info = _Info()
         self._check(self._lib.pg_corr_build(self._h, _ptr(bold), n, t, C.byref(o), C.byref(info)))
+        return CorrInfo(**{f: getattr(info, f) for f, _ in _Info._fields_})
+
+    def corr_emit(self, bold, n: int, t: int, thr: float, *, sparsity: bool = False,
+                  weighted: bool = False, prec: int = PREC_BF16X3,
+                  layout: int = LAYOUT_TIME_MAJOR) -> CorrInfo:
+        """One Gram pass keeping every pair the loosest threshold `thr` can keep."""
+        o = _Opts(THR_SPARSITY if sparsity else THR_R, thr, int(weighted), prec, layout)
+        info = _Info()
+        self._check(self._lib.pg_corr_emit(self._h, _ptr(bold), n, t, C.byref(o), C.byref(info)))
+        return CorrInfo(**{f: getattr(info, f) for f, _ in _Info._fields_})
+
+    def corr_select(self, thr: float) -> CorrInfo:
+        """CSR of one threshold (as strict as the emitted one or stricter) from the kept pairs."""
+        info = _Info()
+        self._check(self._lib.pg_corr_select(self._h, thr, C.byref(info)))
         return CorrInfo(**{f: getattr(info, f) for f, _ in _Info._fields_})
 
     def corr_csr(self, n: int, nnz: int, weighted: bool = False):
@@ -364,14 +381,14 @@ class Context:
         return eff, lp.value, iso.value
 
     def modularity(self, row, col, val=None, kmode: int = MOD_STRENGTH, stop: int = MOD_STOP_VECTOR,
-                   tol: float = 0.0, max_iter: int = 0, progress: bool = False):
+                   tol: float = 0.0, max_iter: int = 0, progress: bool = False, max_depth: int = 0):
         """(labels, Q, lambda0, round-0 vector, rounds): Partition_GPU with the
         reference's stopping rule (MOD_STOP_VECTOR) or the Rayleigh-quotient rule."""
         n = _len(row) - 1
         labels = np.empty(n, np.int32)
         eig = np.empty(n, np.float64)
         q, lam, nr = C.c_double(0), C.c_double(0), C.c_int32(0)
-        o = _ModOpts(kmode, stop, tol, max_iter, int(progress))
+        o = _ModOpts(kmode, stop, tol, max_iter, int(progress), max_depth)
         self._check(self._lib.pg_modularity_ex(self._h, _ptr(row), _ptr(col), _ptr(val), n, _len(col), C.byref(o),
                                                _ptr(labels), C.cast(C.pointer(q), _VP),
                                                C.cast(C.pointer(lam), _VP), _ptr(eig),

@@ -178,3 +178,88 @@ def test_plain_workload_n5000(ctx, prec):
     np.testing.assert_array_equal(col, ocol)
     say(f"  nnz={info.nnz} band_pairs={info.band_pairs} flips={info.band_flips} "
         f"gram_ms={info.ms_gram:.3f} csr_ms={info.ms_csr:.3f}")
+
+
+_MULTI = {}
+
+
+def _multi_oracle(key, fn):
+    """Oracle results shared by the multi-threshold cases (one per threshold)."""
+    if key not in _MULTI:
+        _MULTI[key] = fn()
+    return _MULTI[key]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("weighted", [False, True])
+def test_multi_threshold_one_gram_pass(ctx, prec, weighted):
+    """pg_corr_emit + pg_corr_select: three r thresholds and three sparsities
+    from one emitting Gram pass each (N large enough for the sampled emission
+    bound), every CSR identical to the oracle's for that threshold alone."""
+    import pagani_amd
+    n, t = 6000, 100
+    x = _multi_oracle("x", lambda: _data(n, t, 12, 13))
+    z64 = _multi_oracle("z", lambda: O.zscore(x)[0])
+    say(f"multi r prec={prec} weighted={weighted}")
+    e = ctx.corr_emit(x, n, t, 0.25, weighted=weighted, prec=prec)
+    assert e.gram_passes == 1 and e.band_pairs > 0
+    for thr in (0.45, 0.25, 0.3):
+        info = ctx.corr_select(thr)
+        row, col, val = ctx.corr_csr(n, info.nnz, weighted)
+        orow, ocol, oval = _multi_oracle(("r", thr, weighted), lambda: O.corr_csr_r(z64, thr, weighted))
+        _assert_same(row, col, val, orow, ocol, oval)
+        assert info.r_thresh == np.float32(thr)
+        assert info.band_max_err <= info.band / 1.5
+    with pytest.raises(pagani_amd.PgError) as ex:
+        ctx.corr_select(0.2)  # looser than the emitted threshold
+    assert ex.value.status == 5
+    say(f"multi s prec={prec} weighted={weighted}")
+    e = ctx.corr_emit(x, n, t, 3.0, sparsity=True, weighted=weighted, prec=prec)
+    assert e.gram_passes == 1  # the sampled bound held: no histogram of every pair
+    for s_ in (3.0, 0.5, 1.0):
+        info = ctx.corr_select(s_)
+        row, col, val = ctx.corr_csr(n, info.nnz, weighted)
+        orow, ocol, oval, tstar = _multi_oracle(("s", s_, weighted), lambda: O.corr_csr_s(z64, s_, weighted))
+        _assert_same(row, col, val, orow, ocol, oval)
+        assert info.t_star == tstar
+        assert info.nnz == 2 * O.sparsity_k(n, s_)
+    with pytest.raises(pagani_amd.PgError):
+        ctx.corr_select(4.0)  # more pairs than the emission kept
+    # a build in between drops the kept pairs
+    ctx.corr_build(x, n, t, 0.5, prec=prec)
+    with pytest.raises(pagani_amd.PgError) as ex:
+        ctx.corr_select(1.0)
+    assert ex.value.status == 7
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_multi_threshold_small_and_tied(ctx, prec):
+    """Emission paths without the sampled bound (few supertiles) and a
+    sparsity whose cut splits a tie group."""
+    x = _data(400, 120, 6, 9)
+    for dst, src in [(50, 10), (51, 10), (300, 10), (201, 77), (399, 77)]:
+        x[:, dst] = x[:, src]
+    z64, _ = O.zscore(x)
+    s_tie = _sparsity_splitting_a_tie(z64, 400)
+    say(f"multi small prec={prec} s_tie={s_tie}")
+    ctx.corr_emit(x, 400, 120, max(20.0, s_tie), sparsity=True, prec=prec)
+    for s in (s_tie, 20.0, 1.0):
+        if s > max(20.0, s_tie):
+            continue
+        info = ctx.corr_select(s)
+        row, col, _ = ctx.corr_csr(400, info.nnz)
+        orow, ocol, _, tstar = O.corr_csr_s(z64, s)
+        np.testing.assert_array_equal(row, orow)
+        np.testing.assert_array_equal(col, ocol)
+        assert info.t_star == tstar
+    for n in (1, 2):
+        xs = _data(n, 40, 2, 4)
+        ctx.corr_emit(xs, n, 40, 10.0, sparsity=True, prec=prec)
+        info = ctx.corr_select(5.0)
+        assert info.nnz == 0
+        ctx.corr_emit(xs, n, 40, -0.5, prec=prec)
+        info = ctx.corr_select(0.0)
+        row, col, _ = ctx.corr_csr(n, info.nnz)
+        orow, ocol, _ = O.corr_csr_r(O.zscore(xs)[0], 0.0)
+        np.testing.assert_array_equal(row, orow)
+        np.testing.assert_array_equal(col, ocol)

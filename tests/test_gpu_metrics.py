@@ -278,6 +278,19 @@ def test_modularity_weighted(ctx, weighted_graph, kmode, stop):
     assert err["agree"] > 0.99, err
 
 
+@pytest.mark.parametrize("depth", [1, 2])
+def test_modularity_depth_limit(ctx, weighted_graph, depth):
+    import pagani_amd
+    row, col, val = weighted_graph
+    say(f"modularity depth {depth}")
+    got = ctx.modularity(row, col, val, 0, stop=pagani_amd.MOD_STOP_RAYLEIGH, max_depth=depth)
+    want = O.modularity(row, col, val, 0, stop=1, max_depth=depth)
+    err = mod_errors(got, want)
+    assert got[4] == want[4]  # rounds processed
+    assert err["lam"] <= 1e-7 and err["q"] <= 1e-6 and err["agree"] > 0.99, err
+    assert len(set(got[0].tolist())) <= 2 ** depth
+
+
 def _check_ec(got, want):
     vh, av, lam, it = got
     ovh, oav, olam, oit = want
@@ -353,3 +366,55 @@ def test_edge_cases_tiny_graphs(ctx, n, edges):
     assert k == ok == 1
     np.testing.assert_array_equal(p.view(np.uint32), op.view(np.uint32))
     np.testing.assert_array_equal(q.view(np.uint32), p.view(np.uint32))
+
+
+def _malformed():
+    """(name, row, col, val) cases every CSR entry point must reject with
+    PG_EFORMAT before a kernel indexes with them."""
+    row, col, val = random_graph(50, 0.2, 9, weighted=True)
+    bad_col = col.copy()
+    bad_col[7] = 50  # column == n
+    neg_col = col.copy()
+    neg_col[3] = -1
+    bad_rn = row.copy()
+    bad_rn[-1] = col.size - 1  # row[n] != nnz
+    bad_mono = row.copy()
+    bad_mono[10], bad_mono[11] = bad_mono[11], bad_mono[10]  # offsets decrease
+    unsorted = col.copy()
+    a = row[4]
+    unsorted[a], unsorted[a + 1] = unsorted[a + 1], unsorted[a]
+    return [("col_ge_n", row, bad_col, val, False), ("col_negative", row, neg_col, val, False),
+            ("row_n_ne_nnz", bad_rn, col, val, False), ("row_decreasing", bad_mono, col, val, False),
+            ("unsorted_row", row, unsorted, val, True)]
+
+
+@pytest.mark.parametrize("case", [c[0] for c in _malformed()])
+def test_malformed_csr_rejected(ctx, case):
+    import pagani_amd
+    name, row, col, val, sort_only = next(c for c in _malformed() if c[0] == case)
+    say(f"malformed csr {name}")
+    calls = {
+        "triangles": lambda: ctx.triangles(row, col),
+        "cp_binary": lambda: ctx.cp_binary(row, col),
+        "cp_weighted": lambda: ctx.cp_weighted(row, col, val),
+        "bc": lambda: ctx.bc(row, col, np.array([0, 1], np.int32)),
+        "bfs": lambda: ctx.bfs(row, col, np.array([0, 1], np.int32)),
+        "modularity": lambda: ctx.modularity(row, col, val),
+        "fw_init": lambda: ctx.fw_init(row, col, val, 0),
+        "bfs_efficiency": lambda: ctx.bfs_efficiency(row, col),
+    }
+    # unsorted columns are an error only where the kernels rely on the order
+    needs_sorted = {"triangles", "cp_binary", "cp_weighted", "bfs_efficiency"}
+    for fn, call in calls.items():
+        if sort_only and fn not in needs_sorted:
+            continue
+        with pytest.raises(pagani_amd.PgError) as e:
+            call()
+        assert e.value.status == 4, (fn, str(e.value))
+    if name == "row_decreasing":  # strength reads the offsets only
+        with pytest.raises(pagani_amd.PgError) as e:
+            ctx.strength(row, val)
+        assert e.value.status == 4
+    # the context stays usable
+    r, c = path_graph(5)[:2]
+    np.testing.assert_allclose(ctx.bc(r, c), O.bc(r, c))

@@ -14,9 +14,11 @@ C3  metrics on the C2 graph: triangles and binary Cp (bit-exact, all
 C5  weighted network N=120,000, T=300, K=200, w = r for r > 0.3: CSR
     invariants over all entries, 64 seeded rows (columns and weights) against
     rows the oracle forms alone, Onnela clustering at 256 seeded vertices, and
-    leading-eigenvector modularity with the Rayleigh-quotient stop: Q
-    recomputed by the oracle from the GPU labels, the round-0 eigenpair
-    residual |B v - lambda0 v| / |lambda0| from one FP64 host product.
+    leading-eigenvector modularity with the Rayleigh-quotient stop for the
+    first three bisection levels against the oracle to the same depth (labels,
+    Q, lambda0, the round-0 vector), Q recomputed by the oracle from the GPU
+    labels and the round-0 eigenpair residual |B v - lambda0 v| / |lambda0|
+    from one FP64 host product.
 """
 import numpy as np
 import pytest
@@ -235,21 +237,31 @@ def test_c5_onnela_clustering_sampled(ctx, c5):
 
 
 def test_c5_modularity_rayleigh(ctx, c5):
+    """The first three bisection levels (the full network, then its halves) with
+    the Rayleigh-quotient stop, against the FP64 oracle run to the same depth."""
     import pagani_amd
+    from test_gpu_metrics import mod_errors
     w, _, _, row, col, val = c5
     n = w["n"]
-    say("C5 modularity, Rayleigh-quotient stop")
-    lab, q, lam, eig, rounds = ctx.modularity(row, col, val, pagani_amd.MOD_STRENGTH,
-                                              stop=pagani_amd.MOD_STOP_RAYLEIGH, progress=True)
+    say("C5 modularity, Rayleigh-quotient stop, 3 levels")
+    got = ctx.modularity(row, col, val, pagani_amd.MOD_STRENGTH, stop=pagani_amd.MOD_STOP_RAYLEIGH,
+                         progress=True, max_depth=3)
+    lab, q, lam, eig, rounds = got
+    say("C5 modularity oracle, 3 levels")
+    want = O.modularity(row, col, val, 0, stop=1, max_depth=3)
+    err = mod_errors(got, want)
     oq = O.modularity_q(row, col, lab, val, 0)
-    assert abs(q - oq) <= 1e-9 * max(1.0, abs(oq))
     # round-0 eigenpair residual, B v = A v - k (k . v) / 2m in FP64 on the host
     r64 = np.repeat(np.arange(n), np.diff(row))
     k = np.bincount(r64, weights=val.astype(np.float64), minlength=n)
     av = np.bincount(r64, weights=val.astype(np.float64) * eig[col], minlength=n)
     bv = av - k * (k @ eig) / k.sum()
     res = np.linalg.norm(bv - lam * eig) / abs(lam) / np.linalg.norm(eig)
-    print(f"C5 modularity: Q={q:.6f} (oracle Q of the same labels {oq:.6f}), {len(set(lab.tolist()))} modules, "
-          f"{rounds} rounds, lambda0={lam:.6f}, eigenpair residual {res:.2e}")
+    print(f"C5 modularity 3 levels: Q={q:.6f} (oracle {want[1]:.6f}; oracle Q of the GPU labels {oq:.6f}), "
+          f"{len(set(lab.tolist()))} modules, {rounds} rounds, lambda0={lam:.6f} (rel err {err['lam']:.2e}), "
+          f"eigenvector rel err {err['eig']:.2e}, label agreement {err['agree']:.5f}, "
+          f"eigenpair residual {res:.2e}")
+    assert abs(q - oq) <= 1e-9 * max(1.0, abs(oq))
+    assert rounds == want[4]
+    assert err["lam"] <= 1e-7 and err["q"] <= 1e-6 and err["agree"] > 0.99, err
     assert res <= 1e-3
-    assert len(set(lab.tolist())) > 1

@@ -396,6 +396,22 @@ def test_modularity_rayleigh_stop_matches_eigh(graph):
         assert q == q0
 
 
+def test_modularity_depth_limit_coarsens():
+    # four cliques: one bisection level gives two groups of two cliques each;
+    # a depth-limited partition is always a union of the unlimited one's
+    # communities, and its labels run 1..K in round order
+    row, col, _ = cliques([20, 20, 20, 20])
+    full, q_full, *_ = O.modularity(row, col)
+    assert len(set(full.tolist())) == 4
+    for d, k in [(1, 2), (2, 4), (5, 4)]:
+        lab, q, *_ = O.modularity(row, col, max_depth=d)
+        assert sorted(set(lab.tolist())) == list(range(1, k + 1))
+        for c in set(full.tolist()):
+            assert len(set(lab[full == c].tolist())) == 1
+        assert q == O.modularity_q(row, col, lab)
+    assert O.modularity(row, col, max_depth=1)[1] < q_full
+
+
 def _dense_csr_numpy(keep_upper, n):
     iu = np.triu_indices(n, 1)
     m = np.zeros((n, n), bool)

@@ -146,6 +146,62 @@ pg_status pg_cormat_read(pg_ctx *ctx, int64_t offset, int64_t count, float *dst)
 pg_status pg_cormat_threshold(pg_ctx *ctx, int32_t thr_kind, float thr, int32_t weighted,
                               pg_corr_info *info);
 
+/* ------------------------------------------------------------------ cohort */
+
+/* Edges (undirected) the sparsity rule keeps for n voxels at `sparsity`
+ * percent, with the reference's float arithmetic (U CorMat.cpp:311-313);
+ * a subject's CSR has 2k column entries. */
+int64_t pg_sparsity_pairs(int32_t n, float sparsity);
+
+/* Bits of pg_subject.status: why a subject was rebuilt outside the graph. */
+#define PG_COHORT_WINDOW 1u       /* selection window below the kept pairs */
+#define PG_COHORT_SEGMENTS 2u     /* an edge segment overflowed */
+#define PG_COHORT_CANDIDATES 4u   /* the candidate list overflowed */
+#define PG_COHORT_SELECTION 8u    /* the exact selection did not hold */
+#define PG_COHORT_CSR 16u         /* row[n] != 2k */
+#define PG_COHORT_DEPTH 32u       /* a BFS deeper than the captured levels */
+#define PG_COHORT_OUTSIDE 64u     /* too small for the single pass (n, k or supertiles) or T > 360 */
+
+typedef struct pg_subject {
+    const float *bold;       /* t x n FP32 series, time-major (bold[l*n + i]), device memory */
+    int32_t n, t;
+    float sparsity;          /* percent in (0, 100] */
+    const int32_t *sources;  /* BC sources (device or host), nsrc of them; ignored when bc is NULL */
+    int64_t nsrc;
+    int32_t *row;            /* outputs, device memory: row[n + 1] */
+    int32_t *col;            /* col[2 * pg_sparsity_pairs(n, sparsity)] */
+    int32_t *deg;            /* deg[n], or NULL */
+    float *cp;               /* cp[n] (binary clustering), or NULL */
+    double *bc;              /* bc[n] (BC from the sources), or NULL */
+    uint32_t status;         /* out: 0 = built in the graph, else PG_COHORT_* bits of the
+                                checks that sent it through the ordinary entry points */
+    float ms;                /* out: device time of the subject inside the last graph launch */
+} pg_subject;
+
+typedef struct pg_cohort_info {
+    int32_t graph_nodes;     /* nodes of the captured graph */
+    int32_t reruns;          /* subjects rebuilt outside the graph */
+    double ms_build;         /* host time to plan, capture and instantiate */
+    double ms_graph;         /* device time of the last graph launch */
+    double ms_reruns;        /* host time of the rebuilds */
+} pg_cohort_info;
+
+/* A cohort of subjects of uneven n and t in ONE captured HIP graph: per
+ * subject the BF16x3 sparsity construction (the single pass of pg_corr_build:
+ * z-score, sampled histogram, emitting Gram pass, window, FP64 recheck, exact
+ * selection, CSR) and then degree, binary clustering and BC from its sources,
+ * all enqueued without host round trips on workspaces sized for the largest
+ * subject, captured once and launched `launches` times (>= 1; every launch
+ * recomputes the same results).  Each subject's device-side checks are read
+ * after the last launch; a subject whose single-pass assumptions did not hold
+ * (or whose BFS went deeper than the captured levels) is rebuilt with
+ * pg_corr_build, pg_degree, pg_cp_binary and pg_bc, so the outputs are always
+ * those of the ordinary entry points.  The reference runs subjects one after
+ * another in a host loop (U/CorMat/CUCorMat/CUCORMAT/CorMat.cpp:190-378,
+ * then one tool per metric); there is no single reference function. */
+pg_status pg_cohort_run(pg_ctx *ctx, pg_subject *subjects, int32_t nsub, int32_t launches,
+                        pg_cohort_info *info);
+
 /* ----------------------------------------------------------------- metrics */
 /* CSR arguments: row[n+1] (row[0] = 0, row[n] = nnz), col[nnz] ascending within
  * each row, symmetric, no self loops; val[nnz] FP32 weights where used. */
@@ -318,6 +374,40 @@ pg_status pg_participation(pg_ctx *ctx, const int32_t *row, const int32_t *col, 
 pg_status pg_rewire(pg_ctx *ctx, const int32_t *row, const int32_t *col, const float *val, int32_t n,
                     int64_t nnz, uint64_t seed, int64_t attempts, int32_t *out_col, float *out_val,
                     int64_t *swaps);
+
+/* ---------------------------------------------------------------- autotuner */
+
+typedef struct pg_tune_result {
+    int32_t value;        /* the chosen launch shape */
+    int32_t candidates;   /* shapes timed now (0: the cached value was used) */
+    double ms;            /* time of the chosen shape */
+    double ms_default;    /* time of the default shape (when timed) */
+} pg_tune_result;
+
+/* Autotuner.  The reference hard-codes its launch shapes (Batch_size =
+ * 1024*3, U/CorMat/CUCorMat/CUCORMAT/CorMat.cpp:238; the BC grid from free
+ * memory, U/BC/CUBC/CUBC.cpp:45-64; sizeBlock = 96*32,
+ * W/Lp/CUBFW_Lp/CUBFW_Lp/CUBFW_Lp.cpp:15).  Each call below times the
+ * candidate shapes on the caller's input through the ordinary entry point and
+ * keeps the fastest in a cache keyed by the device architecture and the
+ * half-octave size buckets (file: $PAGANI_TUNE_CACHE, else
+ * ~/.cache/pagani/tune.txt; PAGANI_TUNE=off ignores it, =refresh searches
+ * again); later calls on inputs of the same bucket use it.  A cached value is
+ * returned without a search.  With PAGANI_AUTOTUNE=1, pg_corr_build, pg_bc
+ * and pg_fw (block 0) search by themselves on first use.  Results never
+ * depend on the shape: every shape gives the same edge set, levels and
+ * distances, and BC to FP64 summation order.
+ *  - pg_autotune_corr: the supertile edge of the Gram walk (8, 4, 16 tiles),
+ *    timed on the emitting Gram pass of pg_corr_build(opts); key (n, t).
+ *  - pg_autotune_bc: 64-source BFS batches per launch (1..16) of pg_bc and
+ *    pg_bfs, timed on pg_bc over the given sources; key (n, nnz).
+ *  - pg_autotune_fw: the Floyd-Warshall tile (64, 128), timed on pg_fw over a
+ *    copy of d (d is not changed); key n. */
+pg_status pg_autotune_corr(pg_ctx *ctx, const float *bold, int32_t n, int32_t t,
+                           const pg_corr_opts *opts, pg_tune_result *res);
+pg_status pg_autotune_bc(pg_ctx *ctx, const int32_t *row, const int32_t *col, int32_t n,
+                         int64_t nnz, const int32_t *sources, int64_t nsrc, pg_tune_result *res);
+pg_status pg_autotune_fw(pg_ctx *ctx, const float *d, int32_t n, pg_tune_result *res);
 
 #ifdef __cplusplus
 }

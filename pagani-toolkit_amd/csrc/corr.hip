@@ -17,7 +17,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "pg_internal.hpp"
@@ -61,6 +63,7 @@ struct GramArgs {
     int npad;              // padded voxel count (panel rows)
     const int2 *st_list;   // supertiles (SR, SC), host-built
     long long nst;         // supertile count
+    int super;             // supertile edge in tiles
     long long sstep;       // > 0: visit every sstep-th supertile only (histogram sample)
     float upper;           // r~ > upper: edge without recheck
     float lower;           // lower <= r~ <= upper: FP64 recheck
@@ -241,7 +244,7 @@ __global__ __launch_bounds__(256) void zscore_fused(const float *__restrict__ x,
 // (listed by the host); the work-groups of one XCD walk the same supertile
 // together, so the panels it touches (8 row panels + 8 column panels) stay in
 // that XCD's L2 instead of each work-group streaming its own from HBM/MALL.
-constexpr int SUPER = 8;
+constexpr int SUPER = 8;  // default supertile edge (GramArgs::super; tunable 4, 8, 16)
 constexpr int NXCD = 8;
 #ifndef PG_GRAM_WG_PER_CU
 #define PG_GRAM_WG_PER_CU 2
@@ -253,6 +256,7 @@ struct TileWalk {
     long long sstep;    // visit list entries 0, sstep, 2 sstep, ... (sampling)
     int nr, nc;         // row and column tiles
     int rt;             // tile aspect (rows / columns)
+    int sup, sup2;      // supertile edge and area in tiles
     int L, xcd;
     long long k, kend;  // position in this XCD's tile sequence
 
@@ -267,23 +271,25 @@ struct TileWalk {
         nr = a.nr;
         nc = a.nc;
         rt = a.rt;
+        sup = a.super;
+        sup2 = sup * sup;
         const int g = blockIdx.x, G = gridDim.x;
         xcd = g % NXCD;
         L = G / NXCD;
         k = g / NXCD;
         const long long mx = nst > 0 ? (nst - 1) / sstep : -1;
         const long long mine = mx >= xcd ? (mx - xcd) / NXCD + 1 : 0;
-        kend = mine * SUPER * SUPER;
+        kend = mine * sup2;
     }
     __device__ __forceinline__ bool next(int &R, int &C)
     {
         while (k < kend) {
             const long long kk = k;
             k += L;
-            const long long s = (xcd + (kk / (SUPER * SUPER)) * NXCD) * sstep;
-            const int q = (int)(kk % (SUPER * SUPER));
+            const long long s = (xcd + (kk / sup2) * NXCD) * sstep;
+            const int q = (int)(kk % sup2);
             const int2 sc = st[s];
-            const int r = sc.x * SUPER + q / SUPER, c = sc.y * SUPER + q % SUPER;
+            const int r = sc.x * sup + q / sup, c = sc.y * sup + q % sup;
             if (r < nr && c < nc && c >= rt * r) {
                 R = r;
                 C = c;
@@ -1771,17 +1777,39 @@ double error_band(int prec, int t)
 
 }  // namespace
 
+// Supertiles (SR, SC) of edge `sup` tiles holding some needed tile
+// (C >= RT*R), row-major.
+void supertiles(int nr, int nc, int rt, int sup, std::vector<int2> &stl)
+{
+    stl.clear();
+    const int nsr = (int)pg_cdiv(nr, sup), nsc = (int)pg_cdiv(nc, sup);
+    for (int sr = 0; sr < nsr; sr++)
+        for (int sc = rt * sr; sc < nsc; sc++)
+            if (sc * sup + sup - 1 >= rt * sr * sup) stl.push_back(make_int2(sr, sc));
+}
+
+// Supertile edge of the Gram walk: PAGANI_GRAM_SUPER, else a value forced by
+// the autotuner's search, else the tuned value for (arch, n, t), else 8.
+int gram_super(pg_ctx *ctx, int n, int t)
+{
+    int v = 0;
+    if (const char *e = std::getenv("PAGANI_GRAM_SUPER")) v = std::atoi(e);
+    if (v <= 0) v = ctx->force_gram_super;
+    if (v <= 0) v = pg_tune_get(ctx, "gram_super", n, t, 0);
+    return (v == 4 || v == 8 || v == 16) ? v : SUPER;
+}
+
 // Pairs i < j < n inside the tiles of every stride-th supertile of the list:
 // what a histogram pass with that sampling stride counts.
 long long sampled_pairs(const std::vector<int2> &stl, long long stride, int nr, int nc, int rt, int tm, int tn,
-                        int n)
+                        int n, int sup)
 {
     long long tot = 0;
     for (long long q = 0; q < (long long)stl.size(); q += stride) {
         const int2 sc = stl[(size_t)q];
-        for (int a = 0; a < SUPER; a++)
-            for (int b = 0; b < SUPER; b++) {
-                const int r = sc.x * SUPER + a, c = sc.y * SUPER + b;
+        for (int a = 0; a < sup; a++)
+            for (int b = 0; b < sup; b++) {
+                const int r = sc.x * sup + a, c = sc.y * sup + b;
                 if (r >= nr || c >= nc || c < rt * r) continue;
                 const long long c0 = (long long)c * tn, c1 = std::min<long long>(n, c0 + tn);
                 const long long i1 = std::min<long long>(n, (long long)r * tm + tm);
@@ -1837,7 +1865,7 @@ struct CorrRun {
     hipStream_t st = nullptr;
     int n = 0, t = 0, prec = PREC_BF16X3;
     bool weighted = false, g2path = true;
-    int tm = 0, tn = 0, rt = 0, nr = 0, npad = 0, nc = 0, ldp = 0, nbk = 0, nsrc = 0;
+    int tm = 0, tn = 0, rt = 0, nr = 0, npad = 0, nc = 0, ldp = 0, nbk = 0, nsrc = 0, sup = SUPER;
     long long m1 = 0, nst = 0, nwg = 0, nwg_hist = 0;
     double band = 0.0;
     double *z64 = nullptr;
@@ -1919,13 +1947,8 @@ struct CorrRun {
         // Gram launch geometry: PG_GRAM_WG_PER_CU work-groups per CU, a
         // multiple of the XCD count (see TileWalk); supertiles (SR, SC)
         // holding some needed tile (C >= RT*R), row-major
-        stl.clear();
-        {
-            const int nsr = (int)pg_cdiv(nr, SUPER), nsc = (int)pg_cdiv(nc, SUPER);
-            for (int sr = 0; sr < nsr; sr++)
-                for (int sc = rt * sr; sc < nsc; sc++)
-                    if (sc * SUPER + SUPER - 1 >= rt * sr * SUPER) stl.push_back(make_int2(sr, sc));
-        }
+        sup = gram_super(ctx, n, t);
+        supertiles(nr, nc, rt, sup, stl);
         int2 *dstl;
         PG_TRY(pg_bufT(ctx, "corr.stl", stl.size() + 1, &dstl));
         PG_HIP(ctx, hipMemcpyAsync(dstl, stl.data(), stl.size() * sizeof(int2), hipMemcpyHostToDevice, st));
@@ -1946,6 +1969,7 @@ struct CorrRun {
         ga.npad = npad;
         ga.st_list = dstl;
         ga.nst = nst;
+        ga.super = sup;
         ga.sstep = 0;
         ga.lower_dev = nullptr;
         stage = g2path ? (size_t)g2::NSTAGE * g2::STAGE : (size_t)stage_bytes<PREC_F32>();
@@ -2265,6 +2289,10 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     PG_TRY(check_opts(ctx, bold, n, t, opts));
     PG_HIP(ctx, hipSetDevice(ctx->device));
     pg_emit_invalidate(ctx);
+    if (n > 1 && pg_autotune_on_first_use(ctx) && pg_tune_get(ctx, "gram_super", n, t, 0) == 0) {
+        pg_tune_result tr;
+        PG_TRY(pg_autotune_corr(ctx, bold, n, t, opts, &tr));
+    }
     const bool smode = opts->thr_kind == PG_THR_SPARSITY;
     CorrRun R;
     pg_corr_info inf;
@@ -2355,7 +2383,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             //     widened by 2 bands each side (device: window_bounds);
             // (4) candidates in the window, FP64 recheck, exact selection;
             // (5) CSR build; then one check of every step's assumption.
-            const long long msamp = sampled_pairs(R.stl, stride, R.nr, R.nc, R.rt, R.tm, R.tn, n);
+            const long long msamp = sampled_pairs(R.stl, stride, R.nr, R.nc, R.rt, R.tm, R.tn, n, R.sup);
             const double p = (double)kwant / (double)m1;
             const double meff = std::max(1.0, (double)msamp / 64.0);  // pairs inside a tile are correlated
             const double f = std::min(1.0, 1.15 * p + 6.0 * std::sqrt(p * (1.0 - p) / meff) + 1.0 / meff);
@@ -2513,7 +2541,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
         if (stride > 1 && n > 1) {
             std::vector<unsigned long long> h;
             PG_TRY(R.hist_pass(stride, h));
-            const long long msamp = sampled_pairs(R.stl, stride, R.nr, R.nc, R.rt, R.tm, R.tn, n);
+            const long long msamp = sampled_pairs(R.stl, stride, R.nr, R.nc, R.rt, R.tm, R.tn, n, R.sup);
             int b0 = (int)(((double)E.lower + 1.0) * (NBIN * 0.5));
             b0 = std::max(0, std::min(NBIN - 1, b0));
             unsigned long long above = 0;
@@ -2541,7 +2569,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
             R.nst >= 8 ? std::max<long long>(2, std::min<long long>(50, R.nst / 16)) : 1;
         if (E.kwant_max > 0 && n > 1 && stride > 1) {
             // the single pass's sampled bound for the largest k
-            const long long msamp = sampled_pairs(R.stl, stride, R.nr, R.nc, R.rt, R.tm, R.tn, n);
+            const long long msamp = sampled_pairs(R.stl, stride, R.nr, R.nc, R.rt, R.tm, R.tn, n, R.sup);
             const double p = (double)E.kwant_max / (double)m1;
             const double meff = std::max(1.0, (double)msamp / 64.0);
             const double f = std::min(1.0, 1.15 * p + 6.0 * std::sqrt(p * (1.0 - p) / meff) + 1.0 / meff);
@@ -2764,6 +2792,290 @@ extern "C" pg_status pg_corr_select(pg_ctx *ctx, float thr, pg_corr_info *info)
     if (info) *info = inf;
     return PG_OK;
 }
+
+// ------------------------------------------- captured cohort: construction
+
+namespace {
+
+__global__ void fill_u32(uint32_t *__restrict__ p, long long count)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) p[k] = 0u;
+}
+
+// Every assumption of the single sparsity pass that pg_corr_build checks on
+// the host after it, checked on the device; a failed one sets a bit of
+// *status and the subject is rebuilt outside the graph.
+__global__ __launch_bounds__(256) void cohort_corr_check(const SelCtl *__restrict__ ctl,
+                                                         const unsigned long long *__restrict__ counters,
+                                                         const unsigned long long *__restrict__ seg_cnt, int nseg,
+                                                         long long seg_cap, long long ccap, long long kwant,
+                                                         const int *__restrict__ row, int n,
+                                                         unsigned *__restrict__ status)
+{
+    bool over = false;
+    for (int g = threadIdx.x; g < nseg; g += 256) over |= (long long)seg_cnt[g] > seg_cap;
+    if (over) atomicOr(status, PG_COHORT_SEGMENTS);
+    if (threadIdx.x == 0) {
+        unsigned bad = 0;
+        if (ctl->ok != 1) bad |= PG_COHORT_WINDOW;
+        if ((long long)counters[0] > ccap) bad |= PG_COHORT_CANDIDATES;
+        if (((const int *)(counters + 4))[0] != 0 || (long long)counters[3] + (long long)counters[1] != kwant)
+            bad |= PG_COHORT_SELECTION;
+        if ((long long)row[n] != 2 * kwant) bad |= PG_COHORT_CSR;
+        if (bad) atomicOr(status, bad);
+    }
+}
+
+}  // namespace
+
+// Plan of the construction part of a captured cohort: per-subject geometry
+// and launch sizes (all fixed on the host, as capture requires) and one
+// workspace sized for the largest subject, reused by every subject in turn.
+struct PgCohortCorr {
+    struct Subj {
+        int n = 0, t = 0, ldp = 0, nr = 0, npad = 0, nc = 0, nbk = 0, nbm = 1, wx = 1;
+        long long m1 = 0, kwant = 0, nst = 0, stride = 1, stl_off = 0, seg_cap = 0, ccap = 0;
+        double f = 0.0, band = 0.0;
+        bool graph = false;  // built in the graph (else outside it)
+    };
+    pg_ctx *ctx = nullptr;
+    std::vector<Subj> subj;
+    long long nwg = 0, nwg_hist = 0;
+    int sup = SUPER;
+    double *z64 = nullptr, *cand64 = nullptr, *tstar = nullptr;
+    uint16_t *phi = nullptr, *plo = nullptr;
+    int2 *stl = nullptr, *seg = nullptr, *cand = nullptr, *extra = nullptr, *band = nullptr;
+    uint32_t *hist = nullptr, *h2 = nullptr;
+    unsigned long long *htot = nullptr, *seg_cnt = nullptr, *counters = nullptr, *h2t = nullptr, *dpart = nullptr;
+    float *seg_r = nullptr, *cand_r = nullptr, *band_r = nullptr;
+    SelCtl *ctl = nullptr;
+    unsigned *bcnt = nullptr, *ent = nullptr;
+    long long *boff = nullptr, *bstart = nullptr, *btot = nullptr;
+    int *ctmp = nullptr;
+};
+
+pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int32_t *t, const float *spars,
+                              PgCohortCorr **out)
+{
+    PgCohortCorr *P = new PgCohortCorr();
+    std::unique_ptr<PgCohortCorr> guard(P);
+    P->ctx = ctx;
+    P->subj.resize((size_t)nsub);
+    P->nwg = std::max<long long>(NXCD, (long long)ctx->num_cu * PG_GRAM_WG_PER_CU / NXCD * NXCD);
+    P->nwg_hist = std::max<long long>(NXCD, ctx->num_cu / NXCD * NXCD);
+    int max_n0 = 1, max_t0 = 2;
+    for (int s = 0; s < nsub; s++) {
+        max_n0 = std::max(max_n0, n[s]);
+        max_t0 = std::max(max_t0, t[s]);
+    }
+    P->sup = gram_super(ctx, max_n0, max_t0);
+    const long long nwg = P->nwg;
+    size_t max_zt = 1, max_panel = 1;
+    long long max_seg = 64, max_ccap = 1, max_dpart = 1, max_nnz = 1, max_bsrc = 1, max_nbk = 1;
+    int max_t = 2, max_n = 1;
+    std::vector<int2> stl_all;
+    for (int s = 0; s < nsub; s++) {
+        auto &S = P->subj[(size_t)s];
+        S.n = n[s];
+        S.t = t[s];
+        S.m1 = (long long)S.n * (S.n - 1) / 2;
+        S.kwant = sparsity_k(S.m1, spars[s]);
+        S.ldp = (int)std::max<long long>(2 * g2::KC, pg_cdiv(S.t, g2::KC) * g2::KC);
+        S.nr = (int)pg_cdiv(std::max(S.n, 1), g2::TM);
+        S.npad = S.nr * g2::TM;
+        S.nc = S.npad / g2::TN;
+        std::vector<int2> stl;
+        supertiles(S.nr, S.nc, g2::RT, P->sup, stl);
+        S.nst = (long long)stl.size();
+        S.stl_off = (long long)stl_all.size();
+        stl_all.insert(stl_all.end(), stl.begin(), stl.end());
+        S.stride = S.nst >= 8 ? std::max<long long>(2, std::min<long long>(50, S.nst / 16)) : 1;
+        // the single pass needs a sample of the supertiles and T within the
+        // LDS-staged z-score; anything else is built outside the graph
+        S.graph = S.kwant > 0 && S.n > 1 && S.stride > 1 && S.t <= ZS_TMAX;
+        if (!S.graph) continue;
+        const long long msamp = sampled_pairs(stl, S.stride, S.nr, S.nc, g2::RT, g2::TM, g2::TN, S.n, P->sup);
+        const double p = (double)S.kwant / (double)S.m1;
+        const double meff = std::max(1.0, (double)msamp / 64.0);
+        S.f = std::min(1.0, 1.15 * p + 6.0 * std::sqrt(p * (1.0 - p) / meff) + 1.0 / meff);
+        const double w = 2.0 / NBIN;
+        const double expect = std::min(1.0, S.f * 1.2 + 2.0 * w) * (double)S.m1;
+        S.seg_cap = (pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096 + 63) / 64 * 64;
+        // PAGANI_COHORT_SEG_SCALE scales the segment headroom (a value far
+        // below 1 makes segments overflow: the subject is then flagged and
+        // rebuilt outside the graph; writes past a segment are dropped)
+        if (const char *e = std::getenv("PAGANI_COHORT_SEG_SCALE"))
+            S.seg_cap = std::max<long long>(64, (long long)((double)S.seg_cap * std::atof(e)) / 64 * 64);
+        S.ccap = std::min<long long>(1 << 22, std::max<long long>(1 << 16, S.kwant / 4));
+        S.wx = (int)std::max<long long>(1, pg_cdiv(S.seg_cap, WIN_CHUNK));
+        S.band = error_band(PREC_BF16X3, S.t);
+        S.nbk = (int)pg_cdiv(S.n, BROWS);
+        const int nw = (S.n + 31) / 32;
+        S.nbm = 4;
+        while (S.nbm > 1 && (size_t)S.nbm * 8 * nw + 2048 > ctx->lds_per_block) S.nbm--;
+        max_zt = std::max(max_zt, (size_t)S.n * S.t);
+        max_panel = std::max(max_panel, (size_t)S.npad * S.ldp);
+        max_seg = std::max(max_seg, S.seg_cap);
+        max_ccap = std::max(max_ccap, S.ccap);
+        max_dpart = std::max(max_dpart, (long long)S.wx * nwg);
+        max_nnz = std::max(max_nnz, 2 * S.kwant);
+        max_nbk = std::max<long long>(max_nbk, S.nbk);
+        max_bsrc = std::max<long long>(max_bsrc, (nwg + 1) * S.nbk);
+        max_t = std::max(max_t, S.t);
+        max_n = std::max(max_n, S.n);
+    }
+    PG_TRY(pg_bufT(ctx, "coh.z64", max_zt, &P->z64));
+    PG_TRY(pg_bufT(ctx, "coh.phi", max_panel, &P->phi));
+    PG_TRY(pg_bufT(ctx, "coh.plo", max_panel, &P->plo));
+    PG_TRY(pg_bufT(ctx, "coh.stl", stl_all.size() + 1, &P->stl));
+    PG_HIP(ctx, hipMemcpy(P->stl, stl_all.data(), stl_all.size() * sizeof(int2), hipMemcpyHostToDevice));
+    PG_TRY(pg_bufT(ctx, "coh.hist", (size_t)P->nwg_hist * NBIN, &P->hist));
+    PG_TRY(pg_bufT(ctx, "coh.htot", NBIN, &P->htot));
+    PG_TRY(pg_bufT(ctx, "coh.ctl", 1, &P->ctl));
+    PG_TRY(pg_bufT(ctx, "coh.seg", (size_t)(nwg * max_seg), &P->seg));
+    PG_TRY(pg_bufT(ctx, "coh.segr", (size_t)(nwg * max_seg), &P->seg_r));
+    PG_TRY(pg_bufT(ctx, "coh.segcnt", (size_t)nwg, &P->seg_cnt));
+    PG_TRY(pg_bufT(ctx, "coh.counters", 16, &P->counters));
+    PG_TRY(pg_bufT(ctx, "coh.band", 1, &P->band));
+    PG_TRY(pg_bufT(ctx, "coh.bandr", 1, &P->band_r));
+    const int hb = (int)std::min<long long>(256, nwg);
+    PG_TRY(pg_bufT(ctx, "coh.h2", (size_t)hb * NBIN2, &P->h2));
+    PG_TRY(pg_bufT(ctx, "coh.h2t", NBIN2, &P->h2t));
+    PG_TRY(pg_bufT(ctx, "coh.cand", (size_t)max_ccap, &P->cand));
+    PG_TRY(pg_bufT(ctx, "coh.candr", (size_t)max_ccap, &P->cand_r));
+    PG_TRY(pg_bufT(ctx, "coh.cand64", (size_t)max_ccap, &P->cand64));
+    PG_TRY(pg_bufT(ctx, "coh.extra", (size_t)max_ccap, &P->extra));
+    PG_TRY(pg_bufT(ctx, "coh.dpart", (size_t)max_dpart, &P->dpart));
+    PG_TRY(pg_bufT(ctx, "coh.tstar", 2, &P->tstar));
+    PG_TRY(pg_bufT(ctx, "coh.bcnt", (size_t)max_bsrc, &P->bcnt));
+    PG_TRY(pg_bufT(ctx, "coh.boff", (size_t)max_bsrc, &P->boff));
+    PG_TRY(pg_bufT(ctx, "coh.bstart", (size_t)max_nbk + 1, &P->bstart));
+    PG_TRY(pg_bufT(ctx, "coh.btot", (size_t)max_nbk + 1, &P->btot));
+    PG_TRY(pg_bufT(ctx, "coh.ent", (size_t)max_nnz, &P->ent));
+    PG_TRY(pg_bufT(ctx, "coh.ctmp", (size_t)max_nnz, &P->ctmp));
+    // dynamic LDS limits for the largest subject, set before any capture
+    const size_t shz = (size_t)max_t * (ZV + 1) * 4;
+    PG_HIP(ctx, hipFuncSetAttribute((const void *)zscore_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shz));
+    const size_t stage = (size_t)g2::NSTAGE * g2::STAGE;
+    PG_HIP(ctx, hipFuncSetAttribute((const void *)gram2_kernel<MODE_HIST>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(stage + NBIN * 4)));
+    PG_HIP(ctx, hipFuncSetAttribute((const void *)gram2_kernel<MODE_EMIT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)stage));
+    const size_t shs = (size_t)max_nbk * 12;
+    if (shs > 64 * 1024)
+        PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)shs));
+    {
+        const int nw = (max_n + 31) / 32;
+        int nbm = 4;
+        while (nbm > 1 && (size_t)nbm * 8 * nw + 2048 > ctx->lds_per_block) nbm--;
+        PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)((size_t)nbm * 8 * nw)));
+    }
+    *out = guard.release();
+    return PG_OK;
+}
+
+bool pg_cohort_corr_in_graph(const PgCohortCorr *P, int s) { return P->subj[(size_t)s].graph; }
+
+// The subject's launches on the context's stream: no allocation, no host
+// synchronisation, no memset (fills are kernels), so the sequence can be
+// captured.  row/col: the subject's CSR outputs (device, n + 1 and 2k).
+pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int32_t *row, int32_t *col,
+                                 unsigned *status)
+{
+    pg_ctx *ctx = P->ctx;
+    hipStream_t st = ctx->stream;
+    const auto &S = P->subj[(size_t)s];
+    const long long nwg = P->nwg;
+    if (!S.graph) return PG_OK;
+    const long long panel = (long long)S.npad * S.ldp;  // bf16 elements per array
+    fill_u32<<<(unsigned)pg_cdiv(panel / 2, 256), 256, 0, st>>>((uint32_t *)P->phi, panel / 2);
+    fill_u32<<<(unsigned)pg_cdiv(panel / 2, 256), 256, 0, st>>>((uint32_t *)P->plo, panel / 2);
+    const size_t shz = (size_t)S.t * (ZV + 1) * 4;
+    zscore_fused<<<(unsigned)pg_cdiv(S.n, ZV), 256, shz, st>>>(dbold, PG_BOLD_TIME_MAJOR, S.n, S.t, P->z64, nullptr,
+                                                               P->phi, P->plo, S.ldp, S.npad);
+    GramArgs ga;
+    std::memset(&ga, 0, sizeof ga);
+    ga.p0 = P->phi;
+    ga.p1 = P->plo;
+    ga.ldp = S.ldp;
+    ga.kchunks = S.ldp / g2::KC;
+    ga.n = S.n;
+    ga.nr = S.nr;
+    ga.nc = S.nc;
+    ga.rt = g2::RT;
+    ga.npad = S.npad;
+    ga.st_list = P->stl + S.stl_off;
+    ga.nst = S.nst;
+    ga.super = P->sup;
+    ga.sstep = S.stride;
+    ga.hist = P->hist;
+    const size_t stage = (size_t)g2::NSTAGE * g2::STAGE;
+    gram2_kernel<MODE_HIST><<<(unsigned)P->nwg_hist, THREADS, stage + NBIN * 4, st>>>(ga);
+    hist_reduce<<<NBIN / 32, 256, 0, st>>>(P->hist, (int)P->nwg_hist, P->htot);
+    sample_bound<<<1, 1024, 0, st>>>(P->htot, S.f, P->ctl);
+    fill_u32<<<1, 32, 0, st>>>((uint32_t *)P->counters, 32);
+    ga.sstep = 0;
+    ga.lower_dev = &P->ctl->lower;
+    ga.upper = -4.0f;  // every hit is a segment entry, with its r~
+    ga.seg = P->seg;
+    ga.seg_r = P->seg_r;
+    ga.seg_cap = S.seg_cap;
+    ga.seg_cnt = P->seg_cnt;
+    ga.band = P->band;
+    ga.band_r = P->band_r;
+    ga.band_cap = 0;
+    ga.band_cnt = P->counters + 0;
+    gram2_kernel<MODE_EMIT><<<(unsigned)nwg, THREADS, stage, st>>>(ga);
+    const int hb = (int)std::min<long long>(256, nwg);
+    fill_u32<<<(unsigned)pg_cdiv(2 * NBIN2, 256), 256, 0, st>>>((uint32_t *)P->h2t, 2 * NBIN2);
+    emitted_hist<<<hb, 1024, 0, st>>>(P->seg_r, S.seg_cap, P->seg_cnt, (int)nwg, P->ctl, P->h2);
+    hist_sum<<<dim3(NBIN2 / 256, 16), 256, 0, st>>>(P->h2, hb, NBIN2, P->h2t);
+    window_bounds<<<1, 1024, 0, st>>>(P->h2t, S.kwant, S.band, P->ctl);
+    fill_u32<<<1, 32, 0, st>>>((uint32_t *)P->counters, 32);
+    unsigned long long *c = P->counters;
+    window_kernel<<<dim3((unsigned)S.wx, (unsigned)nwg), 256, 0, st>>>(P->seg, P->seg_r, S.seg_cap, P->seg_cnt,
+                                                                       (int)nwg, P->ctl, P->cand, P->cand_r, S.ccap,
+                                                                       c + 0, P->dpart);
+    recheck_kernel<<<(unsigned)pg_cdiv(S.ccap, 256), 256, 0, st>>>(P->cand, c + 0, S.ccap, P->z64, S.t, P->cand64,
+                                                                    P->cand_r, c + 5);
+    const int nd = (int)(S.wx * nwg);
+    select_kernel<<<1, 1024, 0, st>>>(P->cand, P->cand64, P->cand_r, c + 0, S.ccap, S.kwant, P->dpart, nd, P->extra,
+                                      c + 1, P->tstar, c + 2, (int *)(c + 4));
+    definite_sum<<<1, 1024, 0, st>>>(P->dpart, nd, c + 3);
+    // CSR straight into the subject's outputs
+    EdgeSrc es;
+    std::memset(&es, 0, sizeof es);
+    es.seg = P->seg;
+    es.seg_cap = S.seg_cap;
+    es.seg_cnt = P->seg_cnt;
+    es.nseg = (int)nwg;
+    es.extra = P->extra;
+    es.extra_cnt = c + 1;
+    es.extra_cap = S.ccap;
+    es.seg_r = P->seg_r;
+    es.keep_above_dev = &P->ctl->U2;
+    const int nsrcb = (int)nwg + 1;
+    const long long nnz = 2 * S.kwant;
+    bucket_count<<<(unsigned)nsrcb, SEG_THREADS, (size_t)S.nbk * 4, st>>>(es, S.nbk, P->bcnt);
+    bucket_scan_local<<<(unsigned)S.nbk, 256, 0, st>>>(P->bcnt, nsrcb, P->boff, P->btot);
+    bucket_scan_global<<<1, 1024, 0, st>>>(P->btot, S.nbk, P->bstart);
+    bucket_scatter<<<(unsigned)nsrcb, SEG_THREADS, (size_t)S.nbk * 12, st>>>(es, S.nbk, P->boff, P->bstart, P->ent,
+                                                                            nullptr, P->z64, S.t, nnz);
+    const int nw = (S.n + 31) / 32;
+    bucket_rows<false><<<(unsigned)S.nbk, ROWS_THREADS, (size_t)S.nbm * 8 * nw, st>>>(
+        P->bstart, P->ent, nullptr, S.n, S.nbk, S.nbm, row, P->ctmp, nullptr, col, nullptr, nnz);
+    cohort_corr_check<<<1, 256, 0, st>>>(P->ctl, c, P->seg_cnt, (int)nwg, S.seg_cap, S.ccap, S.kwant, row, S.n,
+                                         status);
+    PG_KERNEL_CHECK(ctx);
+    return PG_OK;
+}
+
+void pg_cohort_corr_free(PgCohortCorr *P) { delete P; }
+
+long long pg_sparsity_k(long long n, float s) { return sparsity_k(n * (n - 1) / 2, s); }
 
 pg_status pg_zscore64(pg_ctx *ctx, const float *dbold, int layout, int n, int t, double *z64)
 {

@@ -257,7 +257,17 @@ extern "C" pg_status pg_fw(pg_ctx *ctx, float *d, int32_t n, int32_t block)
 {
     if (!ctx) return PG_EINVAL;
     if (!d || n < 0) return pg_fail(ctx, PG_EINVAL, "bad arguments");
-    if (block == 0) block = 64;  // measured faster than 128 at N = 8192
+    if (block == 0 && n > 0 && pg_autotune_on_first_use(ctx) && pg_tune_get(ctx, "fw_block", n, 0, 0) == 0) {
+        pg_tune_result tr;
+        PG_TRY(pg_autotune_fw(ctx, d, n, &tr));
+    }
+    if (block == 0) {
+        // a value forced by the autotuner's search, else the tuned one for
+        // (arch, n), else 64 (measured faster than 128 at N = 8192)
+        block = ctx->force_fw_block;
+        if (block <= 0) block = pg_tune_get(ctx, "fw_block", n, 0, 0);
+        if (block != 64 && block != 128) block = 64;
+    }
     if (block != 64 && block != 128) return pg_fail(ctx, PG_EINVAL, "tile edge %d: use 64 or 128", block);
     if (n == 0) return PG_OK;
     PG_HIP(ctx, hipSetDevice(ctx->device));

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -506,14 +507,16 @@ pg_status run_forward(pg_ctx *ctx, const int *row, const int *col, int n, BatchB
     return PG_OK;
 }
 
-// Batches per launch: from PAGANI_BC_BATCHES when set (1..16), else enough
-// 64-source batches for about 16384 work-groups of four vertex waves, capped
+// Batches per launch: PAGANI_BC_BATCHES (1..16), else a value forced by the
+// autotuner's search, else the tuned value for (arch, n, nnz), else enough
+// 64-source batches for about 16384 work-groups of four vertex waves; capped
 // by the sources and by 1 GiB of per-source state.
-int bc_batches(pg_ctx *ctx, int n, long long nsrc)
+int bc_batches(pg_ctx *ctx, int n, long long nnz, long long nsrc)
 {
     int nb = 0;
     if (const char *e = std::getenv("PAGANI_BC_BATCHES")) nb = std::atoi(e);
-    if (nb <= 0) nb = pg_tune_get(ctx, "bc_batches", n, 0, 0);
+    if (nb <= 0) nb = ctx->force_bc_batches;
+    if (nb <= 0) nb = pg_tune_get(ctx, "bc_batches", n, nnz, 0);
     if (nb <= 0) {
         nb = (int)std::max<long long>(1, std::min<long long>(8, pg_cdiv(65536, std::max(n, 1))));
         while (nb > 1 && (long long)nb * n * S * 20 > (1ll << 30)) nb--;
@@ -629,6 +632,84 @@ extern "C" pg_status pg_cp_binary(pg_ctx *ctx, const int32_t *row, const int32_t
     return PG_OK;
 }
 
+int pg_bc_batches_for(pg_ctx *ctx, int n, long long nnz, long long nsrc) { return bc_batches(ctx, n, nnz, nsrc); }
+
+// ------------------------------------------------ captured cohort: metrics
+
+namespace {
+// status bit when level `cap` of some batch still found vertices: the BFS may
+// go deeper than the captured launches
+__global__ void bc_depth_check(const unsigned *__restrict__ tot, int cap, unsigned *__restrict__ status)
+{
+    if (threadIdx.x == 0 && tot[cap - 1] != 0u) atomicOr(status, PG_COHORT_DEPTH);
+}
+}  // namespace
+
+struct PgCohortMetrics {
+    pg_ctx *ctx = nullptr;
+    int levels = 0;    // forward launches per group (BFS levels 1..levels)
+    int nb = 1;        // batches of 64 sources per launch
+    BatchBufs B;
+    long long *tri = nullptr;
+};
+
+pg_status pg_cohort_metrics_plan(pg_ctx *ctx, int max_n, long long max_nsrc, int levels, PgCohortMetrics **out)
+{
+    PgCohortMetrics *M = new PgCohortMetrics();
+    std::unique_ptr<PgCohortMetrics> guard(M);
+    M->ctx = ctx;
+    M->levels = levels;
+    M->nb = bc_batches(ctx, max_n, 0, max_nsrc);
+    PG_TRY(batch_bufs(ctx, max_n, M->nb, levels, M->B, true));
+    PG_TRY(pg_bufT(ctx, "coh.tri", (size_t)max_n + 1, &M->tri));
+    *out = guard.release();
+    return PG_OK;
+}
+
+int pg_cohort_metrics_group(const PgCohortMetrics *M) { return M->nb * S; }
+
+// Degree, binary clustering and BC (sources: nsrc device entries padded with
+// -1 to a multiple of pg_cohort_metrics_group) of one subject's CSR, launched
+// without allocation, synchronisation or memset; every BFS gets `levels`
+// level launches, each returning at once once its batch has ended.
+pg_status pg_cohort_metrics_enqueue(PgCohortMetrics *M, const int32_t *row, const int32_t *col, int n,
+                                    const int32_t *srcs, long long nsrc, int32_t *deg, float *cp, double *bc,
+                                    unsigned *status)
+{
+    pg_ctx *ctx = M->ctx;
+    hipStream_t st = ctx->stream;
+    if (n <= 0) return PG_OK;
+    if (deg) degree_kernel<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(row, n, deg, nullptr);
+    if (cp) {
+        const int nw = (n + 31) / 32;
+        triangles_kernel<<<(unsigned)n, 256, (size_t)nw * 4, st>>>(row, col, n, M->tri);
+        cp_from_tri<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(row, M->tri, n, cp);
+    }
+    if (bc && nsrc <= 0) fill_f64<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(bc, n);
+    if (bc && nsrc > 0) {
+        BatchBufs B = M->B;
+        const int nb = M->nb, L = M->levels;
+        const long long ns = (long long)n * S;
+        fill_f64<<<(unsigned)pg_cdiv((long long)nb * n, 256), 256, 0, st>>>(B.bcpart, (long long)nb * n);
+        for (long long g0 = 0; g0 < nsrc; g0 += (long long)nb * S) {
+            B.srcs = const_cast<int32_t *>(srcs) + g0;
+            bfs_init<<<dim3((unsigned)pg_cdiv(ns, 256), (unsigned)nb), 256, 0, st>>>(n, B);
+            bfs_seed<<<dim3(1, (unsigned)nb), S, 0, st>>>(n, B);
+            for (int d = 0; d < L; d++)
+                bfs_forward<<<dim3((unsigned)pg_cdiv(n, 4), (unsigned)nb), 256, 0, st>>>(row, col, n, d, d & 1, B);
+            bc_depth_check<<<1, 64, 0, st>>>(B.tot, L, status);
+            bc_coef_init<<<(unsigned)pg_cdiv(ns * nb, 256), 256, 0, st>>>(ns * nb, B.sigma, B.coef);
+            for (int d = L - 1; d >= 1; d--)
+                bc_backward<<<dim3((unsigned)pg_cdiv(n, 4), (unsigned)nb), 256, 0, st>>>(row, col, n, d, B);
+        }
+        bc_reduce<<<(unsigned)pg_cdiv(n, 256), 256, 0, st>>>(B.bcpart, nb, n, bc);
+    }
+    PG_KERNEL_CHECK(ctx);
+    return PG_OK;
+}
+
+void pg_cohort_metrics_free(PgCohortMetrics *M) { delete M; }
+
 // ------------------------------------------- allocation-free binary Cp
 
 namespace {
@@ -743,11 +824,17 @@ extern "C" pg_status pg_bc(pg_ctx *ctx, const int32_t *row, const int32_t *col, 
     }
     for (auto s : hsrc)
         if (s < 0 || s >= n) return pg_fail(ctx, PG_EINVAL, "source %d out of range", s);
+    if (n > 0 && nsrc > 0 && pg_autotune_on_first_use(ctx) && pg_tune_get(ctx, "bc_batches", n, nnz, 0) == 0) {
+        // search on up to 1024 of the sources, then run with the result
+        pg_tune_result tr;
+        const long long m = std::min<long long>(nsrc, 1024);
+        PG_TRY(pg_autotune_bc(ctx, drow, dcol, n, nnz, hsrc.data(), m, &tr));
+    }
     double *dbc;
     PG_TRY(pg_out(ctx, bc, (size_t)n, "out.bc", &dbc));
     hipStream_t st = ctx->stream;
     if (n == 0) return PG_OK;
-    const int nb = bc_batches(ctx, n, nsrc);
+    const int nb = bc_batches(ctx, n, nnz, nsrc);
     BatchBufs B;
     PG_TRY(batch_bufs(ctx, n, nb, n + 2, B, true));
     fill_f64<<<(unsigned)pg_cdiv((long long)nb * n, 256), 256, 0, st>>>(B.bcpart, (long long)nb * n);
@@ -813,7 +900,7 @@ extern "C" pg_status pg_bfs(pg_ctx *ctx, const int32_t *row, const int32_t *col,
     PG_TRY(pg_out(ctx, level, (size_t)n * nsrc, "out.level", &dlev));
     hipStream_t st = ctx->stream;
     if (n > 0 && nsrc > 0) {
-        const int nb = bc_batches(ctx, n, nsrc);
+        const int nb = bc_batches(ctx, n, nnz, nsrc);
         BatchBufs B;
         PG_TRY(batch_bufs(ctx, n, nb, n + 2, B, false));
         std::vector<int32_t> pad((size_t)nb * S);

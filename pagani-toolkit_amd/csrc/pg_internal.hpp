@@ -29,6 +29,9 @@ struct pg_ctx {
     hipEvent_t ev[12] = {};
     void *static_scratch = nullptr;  // this device's fixed scratch array (pg_static_scratch)
     PgEmit *emit = nullptr;          // pg_corr_emit state, owned
+    // launch shapes forced by the autotuner while it times candidates (0: none)
+    int force_gram_super = 0, force_bc_batches = 0, force_fw_block = 0;
+    bool tuning = false;             // inside a search: no nested search
 
     // result of the last pg_corr_build
     int32_t cn = 0, ct = 0;
@@ -148,3 +151,28 @@ void pg_emit_free(PgEmit *e);
 std::string pg_arch(pg_ctx *ctx);
 int pg_tune_get(pg_ctx *ctx, const char *knob, long long a, long long b, long long c);
 void pg_tune_put(pg_ctx *ctx, const char *knob, long long a, long long b, long long c, int value, double ms);
+
+// Captured cohort (pg_cohort_run): the construction part (corr.hip) and the
+// metrics part (graph.hip) plan their workspaces for the largest subject, then
+// enqueue each subject's launches with no allocation, synchronisation or
+// memset, so the whole cohort can be captured into one graph.
+struct PgCohortCorr;
+struct PgCohortMetrics;
+pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int32_t *t, const float *spars,
+                              PgCohortCorr **out);
+bool pg_cohort_corr_in_graph(const PgCohortCorr *P, int s);
+pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int32_t *row, int32_t *col,
+                                 unsigned *status);
+void pg_cohort_corr_free(PgCohortCorr *P);
+pg_status pg_cohort_metrics_plan(pg_ctx *ctx, int max_n, long long max_nsrc, int levels, PgCohortMetrics **out);
+int pg_cohort_metrics_group(const PgCohortMetrics *M);
+pg_status pg_cohort_metrics_enqueue(PgCohortMetrics *M, const int32_t *row, const int32_t *col, int n,
+                                    const int32_t *srcs, long long nsrc, int32_t *deg, float *cp, double *bc,
+                                    unsigned *status);
+void pg_cohort_metrics_free(PgCohortMetrics *M);
+long long pg_sparsity_k(long long n, float s);
+
+// Autotuner hooks: PAGANI_AUTOTUNE=1 and not inside a search; the BC batch
+// count the heuristic (or a forced or tuned value) gives.
+bool pg_autotune_on_first_use(pg_ctx *ctx);
+int pg_bc_batches_for(pg_ctx *ctx, int n, long long nnz, long long nsrc);

@@ -26,6 +26,7 @@ struct Entry {
 };
 
 std::mutex g_mu;
+std::string g_path;  // the file the in-memory cache mirrors
 bool g_loaded = false;
 std::map<std::string, Entry> g_cache;
 
@@ -53,9 +54,12 @@ std::string key(const std::string &arch, const char *knob, int b0, int b1, int b
 
 void load_locked()
 {
-    if (g_loaded) return;
+    const std::string path = cache_path();
+    if (g_loaded && path == g_path) return;
+    g_cache.clear();
+    g_path = path;
     g_loaded = true;
-    FILE *f = std::fopen(cache_path().c_str(), "r");
+    FILE *f = std::fopen(path.c_str(), "r");
     if (!f) return;
     char arch[128], knob[64];
     int b0, b1, b2, v;

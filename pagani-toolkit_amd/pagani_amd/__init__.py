@@ -7,7 +7,10 @@ when it is missing -- there is no CPU fallback.
 """
 from .lib import (  # noqa: F401
     CorrInfo,
+    CohortInfo,
     Context,
+    Subject,
+    TuneResult,
     PgError,
     LIB_PATH,
     PREC_FP32,

@@ -20,6 +20,9 @@ LAYOUT_TIME_MAJOR, LAYOUT_VOXEL_MAJOR = 0, 1
 CP_BARRAT, CP_ONNELA = 1, 2
 MOD_STRENGTH, MOD_BINARY_K = 0, 1
 MOD_STOP_VECTOR, MOD_STOP_RAYLEIGH = 0, 1
+# pg_subject.status bits
+PG_COHORT_WINDOW, PG_COHORT_SEGMENTS, PG_COHORT_CANDIDATES, PG_COHORT_SELECTION = 1, 2, 4, 8
+PG_COHORT_CSR, PG_COHORT_DEPTH, PG_COHORT_OUTSIDE = 16, 32, 64
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "out of device memory", 3: "device error",
           4: "malformed input", 5: "threshold error", 6: "not supported", 7: "bad call order"}
@@ -39,6 +42,23 @@ class _Opts(C.Structure):
 class _ModOpts(C.Structure):
     _fields_ = [("kmode", C.c_int32), ("stop", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32),
                 ("progress", C.c_int32), ("max_depth", C.c_int32)]
+
+
+class Subject(C.Structure):
+    """pg_subject: one subject of pg_cohort_run (device pointers as integers)."""
+    _fields_ = [("bold", C.c_void_p), ("n", C.c_int32), ("t", C.c_int32), ("sparsity", C.c_float),
+                ("sources", C.c_void_p), ("nsrc", C.c_int64), ("row", C.c_void_p), ("col", C.c_void_p),
+                ("deg", C.c_void_p), ("cp", C.c_void_p), ("bc", C.c_void_p), ("status", C.c_uint32),
+                ("ms", C.c_float)]
+
+
+class CohortInfo(C.Structure):
+    _fields_ = [("graph_nodes", C.c_int32), ("reruns", C.c_int32), ("ms_build", C.c_double),
+                ("ms_graph", C.c_double), ("ms_reruns", C.c_double)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("value", C.c_int32), ("candidates", C.c_int32), ("ms", C.c_double), ("ms_default", C.c_double)]
 
 
 class _Info(C.Structure):
@@ -87,6 +107,11 @@ _SIGS = {
     "pg_corr_emit": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.POINTER(_Opts), C.POINTER(_Info)]),
     "pg_corr_select": (C.c_int, [_VP, C.c_float, C.POINTER(_Info)]),
     "pg_corr_csr": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "pg_sparsity_pairs": (C.c_int64, [C.c_int32, C.c_float]),
+    "pg_autotune_corr": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.POINTER(_Opts), C.POINTER(TuneResult)]),
+    "pg_autotune_bc": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int64, _VP, C.c_int64, C.POINTER(TuneResult)]),
+    "pg_autotune_fw": (C.c_int, [_VP, _VP, C.c_int32, C.POINTER(TuneResult)]),
+    "pg_cohort_run": (C.c_int, [_VP, C.POINTER(Subject), C.c_int32, C.c_int32, C.POINTER(CohortInfo)]),
     "pg_corr_csr_view": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP)]),
     "pg_corr_z64_view": (C.c_int, [_VP, C.POINTER(_VP)]),
     "pg_degree": (C.c_int, [_VP, _VP, C.c_int32, _VP, _VP]),
@@ -234,6 +259,43 @@ class Context:
         z = _VP()
         self._check(self._lib.pg_corr_z64_view(self._h, C.byref(z)))
         return z.value
+
+    # ---------------------------------------------------------- autotuner
+    def autotune_corr(self, bold, n: int, t: int, thr: float, *, sparsity: bool = False, weighted: bool = False,
+                      prec: int = PREC_BF16X3, layout: int = LAYOUT_TIME_MAJOR) -> TuneResult:
+        """Gram supertile edge for (arch, n, t): cached, or searched on this input."""
+        o = _Opts(THR_SPARSITY if sparsity else THR_R, thr, int(weighted), prec, layout)
+        r = TuneResult()
+        self._check(self._lib.pg_autotune_corr(self._h, _ptr(bold), n, t, C.byref(o), C.byref(r)))
+        return r
+
+    def autotune_bc(self, row, col, sources) -> TuneResult:
+        """64-source BFS batches per launch for (arch, n, nnz)."""
+        r = TuneResult()
+        self._check(self._lib.pg_autotune_bc(self._h, _ptr(row), _ptr(col), _len(row) - 1, _len(col), _ptr(sources),
+                                             _len(sources), C.byref(r)))
+        return r
+
+    def autotune_fw(self, d) -> TuneResult:
+        """Floyd-Warshall tile for (arch, n); d (n x n) is not changed."""
+        r = TuneResult()
+        self._check(self._lib.pg_autotune_fw(self._h, _ptr(d), int(d.shape[0]), C.byref(r)))
+        return r
+
+    # ------------------------------------------------------------- cohort
+    def sparsity_pairs(self, n: int, sparsity: float) -> int:
+        return int(self._lib.pg_sparsity_pairs(n, sparsity))
+
+    def cohort_run(self, subjects, launches: int = 1) -> CohortInfo:
+        """pg_cohort_run over a list of Subject structures (updated in place:
+        status and ms)."""
+        arr = (Subject * len(subjects))(*subjects)
+        info = CohortInfo()
+        self._check(self._lib.pg_cohort_run(self._h, arr, len(subjects), launches, C.byref(info)))
+        for k in range(len(subjects)):
+            subjects[k].status = arr[k].status
+            subjects[k].ms = arr[k].ms
+        return info
 
     # ------------------------------------------------- dense matrix, averaging
     def cormat_begin(self, n: int, fisher: bool = False):

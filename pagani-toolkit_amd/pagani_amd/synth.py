@@ -94,6 +94,17 @@ def sample_sources(n: int, count: int, seed: int = 4242) -> np.ndarray:
     return np.ascontiguousarray(perm[:count].astype(np.int32))
 
 
+TAG_COHORT = 6
+
+
+def cohort_subject(s: int):
+    """(seed, voxels, time points) of cohort subject s: seed 4000 + s,
+    N ~ U[20000, 70000] and T ~ U[150, 300] from that seed's own stream."""
+    seed = 4000 + s
+    u = uniforms(seed, TAG_COHORT, 2)
+    return seed, 20000 + int(u[0] * 50001), 150 + int(u[1] * 151)
+
+
 # Workload definitions (voxels, time points, communities, seed).
 WORKLOADS = {
     "plain_n5000_t200": dict(n=5000, t=200, k=10, seed=1),

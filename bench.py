@@ -84,6 +84,29 @@ def cpu_baseline(x, n, t, thr_r, budget_s=15.0):
     }
 
 
+def bc_cpu_baseline(row, col, n, budget_s=20.0):
+    """The oracle's Brandes BC (the reference's BC_CPU.cpp algorithm with
+    uint64 sigma and FP64 dependencies) on seeded source samples of the graph,
+    one thread and every thread, extrapolated linearly to all n sources (the
+    per-source cost is the same for every source of a connected graph)."""
+    from oracle import oracle as O
+    from pagani_amd import synth
+    out = {}
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    for label, threads, count in (("1_thread", 1, 8), ("all_threads", cores, 16 * cores)):
+        O.set_threads(threads)
+        src = synth.sample_sources(n, count, 4242)
+        t0 = time.perf_counter()
+        O.bc(row, col, src)
+        dt = time.perf_counter() - t0
+        out[label] = {"cores": threads, "sources": count, "seconds": dt,
+                      "gteps": count * float(row[-1]) / dt / 1e9, "all_sources_est_s": dt * n / count}
+        if dt > budget_s:
+            break
+    O.set_threads(0)
+    return out
+
+
 def _timed(ctx, fn):
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -174,6 +197,10 @@ def run_metrics(a):
     wr, wc, wv = _device_csr(ctx, nf, winfo.nnz, True)
     d = torch.empty((nf, nf), dtype=torch.float32, device="cuda")
     fw = {"n": nf, "nnz": winfo.nnz, "weights": "1 - r"}
+    ctx.fw_init(wr, wc, wv, 1, out=d)
+    trf = ctx.autotune_fw(d)  # cached, or searched on a copy of this matrix
+    fw["tuned_block"] = {"value": trf.value, "candidates_timed": trf.candidates}
+    log(f"fw tile {trf.value} ({trf.candidates} candidates timed)")
     for block in (64, 128):
         ctx.fw_init(wr, wc, wv, 1, out=d)
         _, dt = _timed(ctx, lambda: ctx.fw(d, block))
@@ -385,6 +412,12 @@ def main(argv=None) -> int:
     def step(prec):
         return ctx.corr_build(xd, n, t, sparsity, sparsity=True, prec=prec)
 
+    # launch shape of the Gram walk: cached, or searched on this input
+    tr = ctx.autotune_corr(xd, n, t, sparsity, sparsity=True)
+    log(f"gram supertile edge {tr.value} ({tr.candidates} candidates timed)")
+    tuned = {"gram_super": {"value": tr.value, "candidates_timed": tr.candidates,
+                            "emit_ms": tr.ms if tr.candidates else None,
+                            "default_emit_ms": tr.ms_default if tr.candidates else None}}
     for i in range(a.warmup):
         info = step(pagani_amd.PREC_BF16X3)
         log(f"warmup {i}: nnz={info.nnz} gram_emit={info.ms_gram_emit:.3f} ms")
@@ -433,6 +466,7 @@ def main(argv=None) -> int:
                      "unit": "TFLOP/s", "frac": mfma_frac, "traffic": GRAM_EMIT_TRAFFIC,
                      "traffic_unit": "bytes per launch",
                      "algorithmic_flop": 3.0 * useful, "duration_ms": emit_ms},
+        "tuned": tuned,
     }
 
     if not a.no_side:
@@ -463,6 +497,9 @@ def main(argv=None) -> int:
         bc = torch.empty(n, dtype=torch.float64, device="cuda")
         nnz = int(row_b[-1].item())
         srcd = torch.from_numpy(src).cuda()
+        trb = ctx.autotune_bc(row_b, col_b, srcd)  # cached, or searched on this graph
+        out["tuned"]["bc_batches"] = {"value": trb.value, "candidates_timed": trb.candidates,
+                                      "best_s": trb.ms / 1e3 if trb.candidates else None}
         ctx.bc(row_b, col_b, srcd, out=bc)  # warm
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -475,6 +512,9 @@ def main(argv=None) -> int:
                      "hbm_model_frac": bytes_model / bdt / HBM_PEAK,
                      "all_sources_est_s": bdt * n / a.bc_sources}
         log(f"bc: {a.bc_sources} sources in {bdt:.3f} s = {te / bdt / 1e9:.1f} GTEPS")
+        if not a.no_cpu_baseline:
+            log("bc cpu baseline (bounded source samples)")
+            out["bc"]["cpu_baseline"] = bc_cpu_baseline(row_b.cpu().numpy(), col_b.cpu().numpy(), n)
 
     if not a.no_cpu_baseline:
         log("cpu baseline (bounded sample)")

@@ -59,8 +59,13 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Threads of the OpenMP loops below (timing the CPU baseline at 1 thread). */
+void orc_set_threads(int nthreads) { omp_set_num_threads(nthreads > 0 ? nthreads : omp_get_num_procs()); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 
 /* ---------------------------------------------------------------- z-score */
 

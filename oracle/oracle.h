@@ -20,6 +20,8 @@ extern "C" {
 #endif
 
 /* z-score of every voxel series.  bold is time-major [T][N] (BOLD[l*N+i]). */
+void orc_set_threads(int nthreads);
+int orc_get_threads(void);
 void orc_zscore(const float *bold, int64_t n, int t, double *z64, float *z32);
 
 /* r(i,j) for i<j from FP64 z rows (sequential sum, no FMA), clamped to [-1,1]. */

@@ -32,6 +32,8 @@ def lib():
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
         L.orc_zscore.argtypes = [f32p, C.c_int64, C.c_int, f64p, f32p]
         L.orc_r.argtypes = [f64p, C.c_int, C.c_int64, C.c_int64]
         L.orc_r.restype = C.c_double
@@ -107,6 +109,15 @@ def _ptr(a):
 
 
 # ------------------------------------------------------------------ wrappers
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's OpenMP loops (0: all processors)."""
+    lib().orc_set_threads(n)
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
+
 
 def zscore(bold_tn: np.ndarray):
     bold_tn = np.ascontiguousarray(bold_tn, dtype=np.float32)

@@ -258,18 +258,20 @@ def run_cohort(a):
     del xs
     log("capture + warmup launch")
     info0 = ctx.cohort_run(subs, 1)
-    log(f"graph of {info0.graph_nodes} nodes built in {info0.ms_build:.0f} ms; launch {info0.ms_graph:.1f} ms")
+    log(f"{info0.graphs} graph(s) of {info0.graph_nodes} nodes built in {info0.ms_build:.0f} ms; "
+        f"launch {info0.ms_graph:.1f} ms")
     info = ctx.cohort_run(subs, a.launches)
     lat = np.array([x.ms for x in subs])
     sec = info.ms_graph / 1e3
     out = {"metric": "cohort per-subject latency and throughput (one captured graph)", "subjects": a.subjects,
-           "n_gpus": 1, "seeds": "4000+s", "graph_nodes": info.graph_nodes, "graph_launches": a.launches,
+           "n_gpus": 1, "seeds": "4000+s", "graphs": info.graphs, "graph_nodes": info.graph_nodes,
+           "graph_launches": a.launches,
            "capture_instantiate_ms": info.ms_build, "graph_ms": info.ms_graph,
            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
            "subjects_per_s": a.subjects / sec, "reruns": info.reruns,
            "statuses": sorted(set(int(x.status) for x in subs)),
            "useful_corr_tflops": flop / sec / 1e12, "bc_gteps": te / sec / 1e9,
-           "launch": "pg_cohort_run: one captured HIP graph for every subject"}
+           "launch": "pg_cohort_run: captured HIP graphs of consecutive subjects, launched back to back"}
     log(f"cohort: {a.subjects} subjects in {info.ms_graph:.1f} ms = {out['subjects_per_s']:.2f} subjects/s; "
         f"p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms; {info.reruns} reruns")
     ctx.close()

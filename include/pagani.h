@@ -179,19 +179,22 @@ typedef struct pg_subject {
 } pg_subject;
 
 typedef struct pg_cohort_info {
-    int32_t graph_nodes;     /* nodes of the captured graph */
+    int32_t graph_nodes;     /* nodes of the captured graphs */
+    int32_t graphs;          /* captured graphs (consecutive subjects, about 2048 nodes each at most) */
     int32_t reruns;          /* subjects rebuilt outside the graph */
     double ms_build;         /* host time to plan, capture and instantiate */
     double ms_graph;         /* device time of the last graph launch */
     double ms_reruns;        /* host time of the rebuilds */
 } pg_cohort_info;
 
-/* A cohort of subjects of uneven n and t in ONE captured HIP graph: per
- * subject the BF16x3 sparsity construction (the single pass of pg_corr_build:
+/* A cohort of subjects of uneven n and t in captured HIP graphs: per subject
+ * the BF16x3 sparsity construction (the single pass of pg_corr_build:
  * z-score, sampled histogram, emitting Gram pass, window, FP64 recheck, exact
  * selection, CSR) and then degree, binary clustering and BC from its sources,
  * all enqueued without host round trips on workspaces sized for the largest
- * subject, captured once and launched `launches` times (>= 1; every launch
+ * subject.  Consecutive subjects are captured into one graph until it holds
+ * about 2048 kernel nodes (PAGANI_COHORT_GRAPH_NODES), so a cohort is one
+ * graph or a few, launched back to back `launches` times (>= 1; every launch
  * recomputes the same results).  Each subject's device-side checks are read
  * after the last launch; a subject whose single-pass assumptions did not hold
  * (or whose BFS went deeper than the captured levels) is rebuilt with

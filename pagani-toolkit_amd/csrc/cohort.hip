@@ -22,7 +22,8 @@
 
 namespace {
 
-constexpr int BFS_LEVELS = 16;  // level launches per BFS inside the graph
+constexpr int BFS_LEVELS = 8;       // level launches per BFS inside the graph
+constexpr int GRAPH_NODES = 2048;   // kernel nodes per captured graph, at most about
 
 // ts[s] = the device wall clock when the stream reaches it
 __global__ void stamp(unsigned long long *ts, int s)
@@ -42,9 +43,29 @@ __global__ void fill_i32(int *p, long long count, int v)
     if (k < count) p[k] = v;
 }
 
+// Kernel nodes one subject adds to a graph: the construction (24 launches)
+// and its time stamp, degree, triangles and Cp, and per group of BFS batches
+// init, seed, `levels` forward launches, the depth check, the coefficient
+// init and `levels` - 1 backward launches.
+long long subject_nodes(bool in_graph, long long nsrc, int group, int levels)
+{
+    if (!in_graph) return 1;
+    long long k = 25 + 3;
+    if (nsrc > 0) k += 2 + pg_cdiv(nsrc, group) * (2 * levels + 3);
+    return k;
+}
+
 struct GraphGuard {
     hipGraph_t g = nullptr;
     hipGraphExec_t e = nullptr;
+    GraphGuard() = default;
+    GraphGuard(const GraphGuard &) = delete;
+    GraphGuard &operator=(const GraphGuard &) = delete;
+    GraphGuard(GraphGuard &&o) noexcept : g(o.g), e(o.e)
+    {
+        o.g = nullptr;
+        o.e = nullptr;
+    }
     ~GraphGuard()
     {
         if (e) (void)hipGraphExecDestroy(e);
@@ -135,31 +156,50 @@ extern "C" pg_status pg_cohort_run(pg_ctx *ctx, pg_subject *subj, int32_t nsub, 
     }
     PG_HIP(ctx, hipStreamSynchronize(st));
 
-    // capture: every subject's launches back to back, a time stamp between
-    GraphGuard G;
-    PG_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-    pg_status cap = PG_OK;
-    clear_u32<<<(unsigned)pg_cdiv(nsub + 1, 256), 256, 0, st>>>(dstatus, nsub + 1);
-    for (int s = 0; s < nsub && cap == PG_OK; s++) {
-        pg_subject &S = subj[s];
-        stamp<<<1, 64, 0, st>>>(dts, s);
-        if (!pg_cohort_corr_in_graph(pc, s)) continue;
-        cap = pg_cohort_corr_enqueue(pc, s, S.bold, S.row, S.col, dstatus + s);
-        if (cap == PG_OK)
-            cap = pg_cohort_metrics_enqueue(pm, S.row, S.col, S.n, dsrc + src_off[s], S.bc ? S.nsrc : 0, S.deg, S.cp,
-                                            S.bc, dstatus + s);
+    // capture: every subject's launches back to back, a time stamp between;
+    // consecutive subjects share a graph up to about GRAPH_NODES kernel nodes
+    // (PAGANI_COHORT_GRAPH_NODES), and the graphs are launched back to back
+    long long budget = GRAPH_NODES;
+    if (const char *e = std::getenv("PAGANI_COHORT_GRAPH_NODES")) budget = std::max(1, std::atoi(e));
+    std::vector<GraphGuard> graphs;
+    graphs.reserve((size_t)nsub + 1);
+    int s0 = 0;
+    while (s0 < nsub) {
+        graphs.emplace_back();
+        GraphGuard &G = graphs.back();
+        PG_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        pg_status cap = PG_OK;
+        if (s0 == 0) clear_u32<<<(unsigned)pg_cdiv(nsub + 1, 256), 256, 0, st>>>(dstatus, nsub + 1);
+        long long nodes = 0;
+        int s = s0;
+        for (; s < nsub && cap == PG_OK; s++) {
+            const long long est =
+                subject_nodes(pg_cohort_corr_in_graph(pc, s), subj[s].bc ? subj[s].nsrc : 0, grp, levels);
+            if (s > s0 && nodes + est > budget) break;  // the next graph takes it
+            nodes += est;
+            pg_subject &S = subj[s];
+            stamp<<<1, 64, 0, st>>>(dts, s);
+            if (!pg_cohort_corr_in_graph(pc, s)) continue;
+            cap = pg_cohort_corr_enqueue(pc, s, S.bold, S.row, S.col, dstatus + s);
+            if (cap == PG_OK)
+                cap = pg_cohort_metrics_enqueue(pm, S.row, S.col, S.n, dsrc + src_off[s], S.bc ? S.nsrc : 0, S.deg,
+                                                S.cp, S.bc, dstatus + s);
+        }
+        if (s == nsub) stamp<<<1, 64, 0, st>>>(dts, nsub);
+        const hipError_t ec = hipStreamEndCapture(st, &G.g);
+        if (cap != PG_OK) return cap;
+        PG_HIP(ctx, ec);
+        PG_HIP(ctx, hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0));
+        size_t gn = 0;
+        PG_HIP(ctx, hipGraphGetNodes(G.g, nullptr, &gn));
+        inf.graph_nodes += (int32_t)gn;
+        inf.graphs++;
+        s0 = s;
     }
-    stamp<<<1, 64, 0, st>>>(dts, nsub);
-    const hipError_t ec = hipStreamEndCapture(st, &G.g);
-    if (cap != PG_OK) return cap;
-    PG_HIP(ctx, ec);
-    PG_HIP(ctx, hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0));
-    size_t nodes = 0;
-    PG_HIP(ctx, hipGraphGetNodes(G.g, nullptr, &nodes));
-    inf.graph_nodes = (int32_t)nodes;
     inf.ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 
-    for (int l = 0; l < launches; l++) PG_HIP(ctx, hipGraphLaunch(G.e, st));
+    for (int l = 0; l < launches; l++)
+        for (auto &G : graphs) PG_HIP(ctx, hipGraphLaunch(G.e, st));
     PG_HIP(ctx, hipStreamSynchronize(st));
     std::vector<unsigned> hstatus((size_t)nsub + 1);
     std::vector<unsigned long long> hts((size_t)nsub + 1);

@@ -741,6 +741,41 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
     else g2_write<BOUNDED, SPLIT, true>(a, acc, lo, up, ib, jb, nn, myseg, segr, be, bc);
 }
 
+// Epilogue of one wave's block of tile (R, C): lane holds column lane%32 and
+// rows (q/4)*8 + (lane/32)*4 + q%4 of each 32 x 32 block.  Histogram mode bins
+// every pair i < j < n; emit mode writes the hits (g2_emit).
+template <int MODE>
+__device__ __forceinline__ void g2_tile_epilogue(const GramArgs &a, const f32x16 (&acc)[g2::BMW][g2::BNW], int R,
+                                                 int C, int wm, int wn, int lane, float lower, bool split,
+                                                 int2 *myseg, unsigned long long *s_seg, uint32_t *s_hist)
+{
+    const int i0 = R * g2::TM + wm * g2::WR, j0 = C * g2::TN + wn * g2::WC;
+    const bool interior = C * g2::TN >= R * g2::TM + g2::TM && (C + 1) * g2::TN <= a.n;
+    if (MODE == MODE_HIST) {
+#pragma unroll
+        for (int bm = 0; bm < g2::BMW; bm++)
+#pragma unroll
+            for (int bn = 0; bn < g2::BNW; bn++)
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const float v = acc[bm][bn][q];
+                    const int i = i0 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
+                    const int j = j0 + bn * 32 + (lane & 31);
+                    if (interior || ((j < a.n) && (i < j))) {
+                        int b = (int)((v + 1.0f) * (NBIN * 0.5f));
+                        b = b < 0 ? 0 : (b >= NBIN ? NBIN - 1 : b);
+                        atomicAdd(&s_hist[b], 1u);
+                    }
+                }
+    } else if (split) {
+        if (interior) g2_emit<true, true>(a, lower, acc, i0, j0, lane, myseg, s_seg);
+        else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, s_seg);
+    } else {
+        if (interior) g2_emit<true, false>(a, lower, acc, i0, j0, lane, myseg, s_seg);
+        else g2_emit<false, false>(a, lower, acc, i0, j0, lane, myseg, s_seg);
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
 {
@@ -833,33 +868,7 @@ __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
             slot = slot + 1 == NSTAGE ? 0 : slot + 1;
         }
 
-        // epilogue: lane holds column lane%32 and rows
-        // (q/4)*8 + (lane/32)*4 + q%4 of each 32 x 32 block
-        const int i0 = R * TM + wm * WR, j0 = C * TN + wn * WC;
-        const bool interior = C * TN >= R * TM + TM && (C + 1) * TN <= a.n;
-        if (MODE == MODE_HIST) {
-#pragma unroll
-            for (int bm = 0; bm < BMW; bm++)
-#pragma unroll
-                for (int bn = 0; bn < BNW; bn++)
-#pragma unroll
-                    for (int q = 0; q < 16; q++) {
-                        const float v = acc[bm][bn][q];
-                        const int i = i0 + bm * 32 + (q >> 2) * 8 + (lane >> 5) * 4 + (q & 3);
-                        const int j = j0 + bn * 32 + (lane & 31);
-                        if (interior || ((j < a.n) && (i < j))) {
-                            int b = (int)((v + 1.0f) * (NBIN * 0.5f));
-                            b = b < 0 ? 0 : (b >= NBIN ? NBIN - 1 : b);
-                            atomicAdd(&s_hist[b], 1u);
-                        }
-                    }
-        } else if (split) {
-            if (interior) g2_emit<true, true>(a, lower, acc, i0, j0, lane, myseg, &s_seg);
-            else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, &s_seg);
-        } else {
-            if (interior) g2_emit<true, false>(a, lower, acc, i0, j0, lane, myseg, &s_seg);
-            else g2_emit<false, false>(a, lower, acc, i0, j0, lane, myseg, &s_seg);
-        }
+        g2_tile_epilogue<MODE>(a, acc, R, C, wm, wn, lane, lower, split, myseg, &s_seg, s_hist);
         R = R1;
         C = C1;
         have = have1;

@@ -659,7 +659,11 @@ pg_status pg_cohort_metrics_plan(pg_ctx *ctx, int max_n, long long max_nsrc, int
     std::unique_ptr<PgCohortMetrics> guard(M);
     M->ctx = ctx;
     M->levels = levels;
-    M->nb = bc_batches(ctx, max_n, 0, max_nsrc);
+    // as many batches per launch as 2 GiB of per-source state allows (fewer
+    // graph nodes: each batch group costs 2 * levels + 3 launches)
+    M->nb = MAX_BATCHES;
+    while (M->nb > 1 && (long long)M->nb * max_n * S * 20 > (2ll << 30)) M->nb--;
+    M->nb = (int)std::max<long long>(1, std::min<long long>(M->nb, pg_cdiv(std::max<long long>(max_nsrc, 1), S)));
     PG_TRY(batch_bufs(ctx, max_n, M->nb, levels, M->B, true));
     PG_TRY(pg_bufT(ctx, "coh.tri", (size_t)max_n + 1, &M->tri));
     *out = guard.release();

@@ -53,7 +53,7 @@ class Subject(C.Structure):
 
 
 class CohortInfo(C.Structure):
-    _fields_ = [("graph_nodes", C.c_int32), ("reruns", C.c_int32), ("ms_build", C.c_double),
+    _fields_ = [("graph_nodes", C.c_int32), ("graphs", C.c_int32), ("reruns", C.c_int32), ("ms_build", C.c_double),
                 ("ms_graph", C.c_double), ("ms_reruns", C.c_double)]
 
 

@@ -60,18 +60,32 @@ def test_cohort_equals_entry_points(ctx, cohort_data, launches):
     subs, keep = _subjects(ctx, cohort_data)
     say(f"cohort {len(subs)} subjects, launches={launches}")
     info = ctx.cohort_run(subs, launches)
-    print(f"cohort: {info.graph_nodes} graph nodes, build {info.ms_build:.1f} ms, graph {info.ms_graph:.2f} ms, "
+    print(f"cohort: {info.graphs} graph(s), {info.graph_nodes} nodes, build {info.ms_build:.1f} ms, "
+          f"graph {info.ms_graph:.2f} ms, "
           f"{info.reruns} reruns; per subject ms {[round(s.ms, 3) for s in subs]}")
     assert [s.status for s in subs[:-1]] == [0] * (len(subs) - 1)
     assert subs[-1].status == pagani_amd.lib.PG_COHORT_OUTSIDE  # too few supertiles for the single pass
     assert info.reruns == 1
-    assert info.graph_nodes > 0 and info.ms_graph > 0
+    assert info.graph_nodes > 0 and info.ms_graph > 0 and info.graphs == 1
     _check_against_entry_points(ctx, cohort_data, keep)
     # and the first subject against the FP64 oracle
     n, t, x, _ = cohort_data[0]
     orow, ocol, _, _ = O.corr_csr_s(O.zscore(x)[0], 1.0)
     np.testing.assert_array_equal(keep[0][1].cpu().numpy(), orow)
     np.testing.assert_array_equal(keep[0][2].cpu().numpy(), ocol)
+
+
+def test_cohort_split_into_several_graphs(ctx, cohort_data, monkeypatch):
+    # a node budget below one subject's launches: one graph per subject,
+    # launched back to back, same results
+    monkeypatch.setenv("PAGANI_COHORT_GRAPH_NODES", "10")
+    data = cohort_data[:4]
+    subs, keep = _subjects(ctx, data)
+    say("cohort, one graph per subject")
+    info = ctx.cohort_run(subs, 2)
+    assert info.graphs == 4, info.graphs
+    assert [s.status for s in subs] == [0] * 4
+    _check_against_entry_points(ctx, data, keep)
 
 
 @pytest.mark.parametrize("knob,value,bit", [("PAGANI_COHORT_SEG_SCALE", "0.001", 2),

@@ -263,7 +263,7 @@ def run_cohort(a):
     info = ctx.cohort_run(subs, a.launches)
     lat = np.array([x.ms for x in subs])
     sec = info.ms_graph / 1e3
-    out = {"metric": "cohort per-subject latency and throughput (one captured graph)", "subjects": a.subjects,
+    out = {"metric": "cohort per-subject latency and throughput (captured HIP graphs)", "subjects": a.subjects,
            "n_gpus": 1, "seeds": "4000+s", "graphs": info.graphs, "graph_nodes": info.graph_nodes,
            "graph_launches": a.launches,
            "capture_instantiate_ms": info.ms_build, "graph_ms": info.ms_graph,

@@ -1,11 +1,12 @@
-// cohort.hip -- many subjects in one captured HIP graph (pg_cohort_run).
+// cohort.hip -- many subjects in captured HIP graphs (pg_cohort_run).
 //
 // The reference builds one subject's network after another in a host loop
 // (U/CorMat/CUCorMat/CUCORMAT/CorMat.cpp:190-378) and runs each metric tool
 // over the files afterwards.  Here the per-subject construction and metrics
 // are enqueued back to back on the context's stream for every subject, with
-// workspaces sized for the largest one, captured into a single graph and
-// launched as one unit: no host round trip between kernels or subjects.  The
+// workspaces sized for the largest one, captured into graphs of consecutive
+// subjects (about GRAPH_NODES kernel nodes each) that are launched back to
+// back: no host round trip between kernels or subjects.  The
 // decisions the ordinary path takes on the host (the single sparsity pass's
 // checks, the BFS depth) are device flags here, read once after the launch;
 // a flagged subject is rebuilt through the ordinary entry points.

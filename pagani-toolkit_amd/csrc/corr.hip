@@ -1014,7 +1014,10 @@ __device__ __forceinline__ int2 src_edge(const EdgeSrc &s, int g, long long k)
 // every segment's output range, a scatter of packed 32-bit entries
 // (row-in-bucket << 25 | column), then one work-group per bucket derives the
 // row offsets and sorts each row by LDS-bitmap rank.  No global atomics.
-constexpr int BSH = 6;
+#ifndef PG_CSR_BSH
+#define PG_CSR_BSH 6
+#endif
+constexpr int BSH = PG_CSR_BSH;  // log2 of the rows per bucket
 constexpr int BROWS = 1 << BSH;
 constexpr unsigned COLMASK = (1u << 25) - 1u;
 

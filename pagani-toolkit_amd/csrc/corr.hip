@@ -98,6 +98,9 @@ constexpr int B_BYTES = TN * KC * 2;              // one array of the column sli
 constexpr int STAGE = 2 * A_BYTES + 2 * B_BYTES;  // 24 KB
 constexpr int DMA_WAVE = STAGE / 1024 / 4;        // 1 KB wave loads per wave per stage (6)
 static_assert(DMA_WAVE * 4 * 1024 == STAGE, "stage must split into 1 KB wave loads");
+// hits of one wave's block staged in LDS before the global writes: a quarter
+// of the ring slot that is free during the epilogue, pairs then r~
+constexpr int STAGE_HITS = STAGE / 4 / 12;  // 512
 }  // namespace g2
 
 __host__ __device__ __forceinline__ int g2_swz(int i) { return ((i >> 2) ^ (i >> 3)) & 1; }
@@ -682,6 +685,29 @@ __device__ __forceinline__ void g2_write(const GramArgs &a, const f32x16 (&acc)[
             }
 }
 
+// Pass 2 into LDS: lane hits in order from its slots pe (edges) and pc
+// (rechecks, placed after every edge) of the staging area.
+template <bool BOUNDED, bool SPLIT>
+__device__ __forceinline__ void g2_stage(const f32x16 (&acc)[g2::BMW][g2::BNW], float lo, float up, int ib, int jb,
+                                         int nn, int2 *spair, float *sval, unsigned pe, unsigned pc)
+{
+#pragma unroll
+    for (int bm = 0; bm < g2::BMW; bm++)
+#pragma unroll
+        for (int bn = 0; bn < g2::BNW; bn++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const float v = acc[bm][bn][q];
+                const int i = ib + bm * 32 + (q >> 2) * 8 + (q & 3), j = jb + bn * 32;
+                bool hit = v >= lo;
+                if (!BOUNDED) hit = hit & (j < nn) & (i < j);
+                if (!hit) continue;
+                const unsigned p = (SPLIT && !(v > up)) ? pc++ : pe++;
+                spair[p] = make_int2(i, j);
+                sval[p] = v;
+            }
+}
+
 // Epilogue of one wave's 128 x 64 block: pass 1 counts per lane (packed: low
 // 16 bits edges v > upper, high 16 bits rechecks lower <= v <= upper; each at
 // most 8192 per wave); one reservation per wave; pass 2 has each lane write
@@ -689,7 +715,7 @@ __device__ __forceinline__ void g2_write(const GramArgs &a, const f32x16 (&acc)[
 // inside the matrix and above the diagonal.
 template <bool BOUNDED, bool SPLIT>
 __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f32x16 (&acc)[g2::BMW][g2::BNW], int i0,
-                                        int j0, int lane, int2 *myseg, unsigned long long *s_seg)
+                                        int j0, int lane, int2 *myseg, unsigned long long *s_seg, char *stage)
 {
     const int ib0 = i0 + (lane >> 5) * 4, jb0 = j0 + (lane & 31);
     unsigned cnt = 0;
@@ -737,6 +763,28 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
     // r~ of every segment entry is kept exactly when there is no band (the
     // sparsity single pass)
     float *segr = SPLIT ? nullptr : a.seg_r + (long long)blockIdx.x * a.seg_cap;
+    const unsigned te = tot & 0xFFFFu, tc = tot >> 16;
+    if (fits && te + tc <= (unsigned)g2::STAGE_HITS) {
+        // a few hits: stage them in LDS in the same order, then write each
+        // output run with whole-wave stores
+        int2 *spair = (int2 *)stage;
+        float *sval = (float *)(stage + g2::STAGE_HITS * 8);
+        g2_stage<BOUNDED, SPLIT>(acc, lo, up, ib, jb, nn, spair, sval, ex & 0xFFFFu, te + (ex >> 16));
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        const unsigned long long be0 = be - (ex & 0xFFFFu), bc0 = bc - (ex >> 16);
+        for (unsigned k = lane; k < te; k += 64) {
+            myseg[be0 + k] = spair[k];
+            if (!SPLIT) segr[be0 + k] = sval[k];
+        }
+        if (SPLIT)
+            for (unsigned k = lane; k < tc; k += 64) {
+                a.band[bc0 + k] = spair[te + k];
+                a.band_r[bc0 + k] = sval[te + k];
+            }
+        __builtin_amdgcn_wave_barrier();
+        return;
+    }
     if (fits) g2_write<BOUNDED, SPLIT, false>(a, acc, lo, up, ib, jb, nn, myseg, segr, be, bc);
     else g2_write<BOUNDED, SPLIT, true>(a, acc, lo, up, ib, jb, nn, myseg, segr, be, bc);
 }
@@ -747,7 +795,8 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
 template <int MODE>
 __device__ __forceinline__ void g2_tile_epilogue(const GramArgs &a, const f32x16 (&acc)[g2::BMW][g2::BNW], int R,
                                                  int C, int wm, int wn, int lane, float lower, bool split,
-                                                 int2 *myseg, unsigned long long *s_seg, uint32_t *s_hist)
+                                                 int2 *myseg, unsigned long long *s_seg, uint32_t *s_hist,
+                                                 char *stage)
 {
     const int i0 = R * g2::TM + wm * g2::WR, j0 = C * g2::TN + wn * g2::WC;
     const bool interior = C * g2::TN >= R * g2::TM + g2::TM && (C + 1) * g2::TN <= a.n;
@@ -768,11 +817,11 @@ __device__ __forceinline__ void g2_tile_epilogue(const GramArgs &a, const f32x16
                     }
                 }
     } else if (split) {
-        if (interior) g2_emit<true, true>(a, lower, acc, i0, j0, lane, myseg, s_seg);
-        else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, s_seg);
+        if (interior) g2_emit<true, true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage);
+        else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage);
     } else {
-        if (interior) g2_emit<true, false>(a, lower, acc, i0, j0, lane, myseg, s_seg);
-        else g2_emit<false, false>(a, lower, acc, i0, j0, lane, myseg, s_seg);
+        if (interior) g2_emit<true, false>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage);
+        else g2_emit<false, false>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage);
     }
 }
 
@@ -868,7 +917,17 @@ __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
             slot = slot + 1 == NSTAGE ? 0 : slot + 1;
         }
 
-        g2_tile_epilogue<MODE>(a, acc, R, C, wm, wn, lane, lower, split, myseg, &s_seg, s_hist);
+        // the slot of the tile's last chunk is free once every wave has
+        // multiplied it (the next tile's chunks 0 and 1 are in the other two)
+        char *stage = nullptr;
+        if (MODE == MODE_EMIT) {
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            int fs = slot + NSTAGE - 1;
+            fs -= fs >= NSTAGE ? NSTAGE : 0;
+            stage = smem + fs * STAGE + w * (STAGE / 4);
+        }
+        g2_tile_epilogue<MODE>(a, acc, R, C, wm, wn, lane, lower, split, myseg, &s_seg, s_hist, stage);
         R = R1;
         C = C1;
         have = have1;
@@ -1015,7 +1074,7 @@ __device__ __forceinline__ int2 src_edge(const EdgeSrc &s, int g, long long k)
 // (row-in-bucket << 25 | column), then one work-group per bucket derives the
 // row offsets and sorts each row by LDS-bitmap rank.  No global atomics.
 #ifndef PG_CSR_BSH
-#define PG_CSR_BSH 6
+#define PG_CSR_BSH 4
 #endif
 constexpr int BSH = PG_CSR_BSH;  // log2 of the rows per bucket
 constexpr int BROWS = 1 << BSH;
@@ -1110,17 +1169,20 @@ __global__ __launch_bounds__(1024) void bucket_scan_global(const long long *__re
 
 // Entries beyond `cap` (a selection that did not hold; the caller checks
 // row[n] and rebuilds) are dropped rather than written.
+// W: the FP64 r of every entry goes along (weighted CSR).
+template <bool W>
 __global__ __launch_bounds__(SEG_THREADS) void bucket_scatter(EdgeSrc s, int nbk, const long long *__restrict__ off,
                                                               const long long *__restrict__ bstart,
                                                               unsigned *__restrict__ ent, float *__restrict__ entv,
                                                               const double *__restrict__ z, int t, long long cap)
 {
-    // per-bucket: this segment's output base (bstart + off) and a cursor
-    extern __shared__ long long sbase[];
-    unsigned *sc = (unsigned *)(sbase + nbk);
+    // per-bucket: this segment's output base (bstart + off; below cap, which
+    // is at most INT32_MAX, whenever anything is written) and a cursor
+    extern __shared__ unsigned sbase[];
+    unsigned *sc = sbase + nbk;
     const int g = blockIdx.x, nsrc = gridDim.x;
     for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) {
-        sbase[b] = bstart[b] + off[(long long)b * nsrc + g];
+        sbase[b] = (unsigned)(bstart[b] + off[(long long)b * nsrc + g]);
         sc[b] = 0u;
     }
     __syncthreads();
@@ -1140,11 +1202,11 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_scatter(EdgeSrc s, int nbk
         for (int u = 0; u < SEG_UNROLL; u++) {
             if (!keep[u]) continue;
             const int bf = e[u].x >> BSH, br = e[u].y >> BSH;
-            const long long pf = sbase[bf] + atomicAdd(&sc[bf], 1u);
-            const long long pr = sbase[br] + atomicAdd(&sc[br], 1u);
+            const unsigned pf = sbase[bf] + atomicAdd(&sc[bf], 1u);
+            const unsigned pr = sbase[br] + atomicAdd(&sc[br], 1u);
             ent[pf] = ((unsigned)(e[u].x & (BROWS - 1)) << 25) | (unsigned)e[u].y;
             ent[pr] = ((unsigned)(e[u].y & (BROWS - 1)) << 25) | (unsigned)e[u].x;
-            if (entv) {
+            if (W) {
                 const float r = (float)r64_pair(z, t, e[u].x, e[u].y);
                 entv[pf] = r;
                 entv[pr] = r;
@@ -1160,26 +1222,126 @@ constexpr int ROWS_THREADS = 1024;
 constexpr int ROWS_UNROLL = 8;  // bucket entries per thread in flight
 constexpr int SORT_MAX = 64 * 16;  // longest row sorted in registers
 
+// LDS of one bitmap wave of bucket_rows: a bitmap over the columns [0, n) in
+// groups of 128 (16 bytes) and one 32-bit prefix count per group.
+__host__ __device__ inline long long rows_groups(int n) { return ((long long)n + 127) / 128; }
+
+// Place the d = z1 - a entries of one row (columns in ctmp, unsorted, no
+// repeats) in column order by rank: set one bit per column, prefix-count the
+// groups (each lane a contiguous run of `per` groups, `per` odd so the
+// 16-byte reads of a wave spread over the banks), then write every entry at
+// its rank; the bits are cleared again by the entries.  Rows of at most
+// 64 * HOLD entries are held in registers across the three passes.
+__device__ __forceinline__ void bitmap_prefix(const uint4 *bits4, unsigned *gpre, long long ng, int per, int lane)
+{
+    const long long g0 = (long long)lane * per;
+    unsigned s = 0;
+    for (int q = 0; q < per; q++) {
+        const long long g = g0 + q;
+        if (g < ng) {
+            const uint4 v = bits4[g];
+            gpre[g] = s;
+            s += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+        }
+    }
+    unsigned incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    const unsigned base = incl - s;
+    if (base)
+        for (int q = 0; q < per; q++) {
+            const long long g = g0 + q;
+            if (g < ng) atomicAdd(&gpre[g], base);
+        }
+}
+
+// rank of column c among the set bits
+__device__ __forceinline__ unsigned bitmap_rank(const uint4 *bits4, const unsigned *gpre, int c)
+{
+    const int g = c >> 7;
+    const uint4 v = bits4[g];
+    const int w = (c >> 5) & 3;
+    unsigned pos = gpre[g];
+    pos += w > 0 ? __popc(v.x) : 0u;
+    pos += w > 1 ? __popc(v.y) : 0u;
+    pos += w > 2 ? __popc(v.z) : 0u;
+    const unsigned word = w == 0 ? v.x : (w == 1 ? v.y : (w == 2 ? v.z : v.w));
+    return pos + __popc(word & ((1u << (c & 31)) - 1u));
+}
+
+template <int HOLD>
+__device__ __forceinline__ void rank_row_bitmap(const int *__restrict__ ctmp, const float *__restrict__ vtmp,
+                                                long long a, long long z1, int lane, uint4 *bits4, unsigned *gpre,
+                                                long long ng, int per, int *__restrict__ col,
+                                                float *__restrict__ val)
+{
+    unsigned *bits = (unsigned *)bits4;
+    if (z1 - a <= 64 * HOLD) {
+        int hc[HOLD];
+#pragma unroll
+        for (int r = 0; r < HOLD; r++) {
+            const long long e = a + r * 64 + lane;
+            hc[r] = e < z1 ? ctmp[e] : -1;
+        }
+#pragma unroll
+        for (int r = 0; r < HOLD; r++)
+            if (hc[r] >= 0) atomicOr(&bits[hc[r] >> 5], 1u << (hc[r] & 31));
+        __builtin_amdgcn_wave_barrier();
+        bitmap_prefix(bits4, gpre, ng, per, lane);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < HOLD; r++)
+            if (hc[r] >= 0) {
+                const unsigned pos = bitmap_rank(bits4, gpre, hc[r]);
+                col[a + pos] = hc[r];
+                if (val) val[a + pos] = vtmp[a + r * 64 + lane];
+            }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < HOLD; r++)
+            if (hc[r] >= 0) bits[hc[r] >> 5] = 0u;
+    } else {
+        for (long long e = a + lane; e < z1; e += 64) {
+            const int c = ctmp[e];
+            atomicOr(&bits[c >> 5], 1u << (c & 31));
+        }
+        __builtin_amdgcn_wave_barrier();
+        bitmap_prefix(bits4, gpre, ng, per, lane);
+        __builtin_amdgcn_wave_barrier();
+        for (long long e = a + lane; e < z1; e += 64) {
+            const int c = ctmp[e];
+            const unsigned pos = bitmap_rank(bits4, gpre, c);
+            col[a + pos] = c;
+            if (val) val[a + pos] = vtmp[e];
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (long long e = a + lane; e < z1; e += 64) bits[ctmp[e] >> 5] = 0u;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 // One work-group per bucket: row counts -> row offsets, scatter into the
-// rows, then each wave sorts whole rows: rows up to SORT_MAX entries by a
-// register bitonic sort, longer ones (waves wv < nbm) by LDS-bitmap rank.
-// W: weights travel with the columns (64-bit sort keys).  Without them the
-// kernel fits two work-groups per CU, so one's phases overlap the other's.
+// rows, then the rows are sorted: rows of more than `dbit` entries by bitmap
+// rank on the first nbm waves (each with its own bitmap), the others by a
+// register bitonic sort on every wave.  W: weights travel with the columns.
 template <bool W>
 __global__ __launch_bounds__(ROWS_THREADS, W ? 4 : 8) void bucket_rows(const long long *__restrict__ bstart,
-                                                                       const unsigned *__restrict__ ent,
-                                                                       const float *__restrict__ entv, int n, int nbk,
-                                                                       int nbm, int *__restrict__ row,
-                                                                       int *__restrict__ ctmp, float *__restrict__ vtmp,
-                                                                       int *__restrict__ col, float *__restrict__ val,
-                                                                       long long cap)
+                                                               const unsigned *__restrict__ ent,
+                                                               const float *__restrict__ entv, int n, int nbk,
+                                                               int nbm, int dbit, int *__restrict__ row,
+                                                               int *__restrict__ ctmp, float *__restrict__ vtmp,
+                                                               int *__restrict__ col, float *__restrict__ val,
+                                                               long long cap)
 {
     if (!W) {
         entv = nullptr;
         vtmp = nullptr;
         val = nullptr;
     }
-    extern __shared__ uint32_t sm[];
+    extern __shared__ __align__(16) uint4 sm4[];
     __shared__ unsigned rc[BROWS], rb[BROWS + 1];
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     constexpr int NWAVE = ROWS_THREADS / 64;
@@ -1187,6 +1349,12 @@ __global__ __launch_bounds__(ROWS_THREADS, W ? 4 : 8) void bucket_rows(const lon
         if (b == nbk - 1 && tid == 0) row[n] = (int)min(bstart[nbk], (long long)INT_MAX);
         return;
     }
+    const long long ng = rows_groups(n);
+    const int per = (int)(((ng + 63) / 64) | 1);
+    uint4 *bits4 = sm4 + (long long)wv * (ng + (ng + 3) / 4);  // rows_wave_u4
+    unsigned *gpre = (unsigned *)(bits4 + ng);
+    if (wv < nbm)
+        for (long long q = lane; q < ng; q += 64) bits4[q] = make_uint4(0u, 0u, 0u, 0u);
     const long long e0 = bstart[b], e1 = bstart[b + 1];
     if (tid < BROWS) rc[tid] = 0u;
     __syncthreads();
@@ -1202,13 +1370,26 @@ __global__ __launch_bounds__(ROWS_THREADS, W ? 4 : 8) void bucket_rows(const lon
             if (x[u] != ~0u) atomicAdd(&rc[x[u] >> 25], 1u);
     }
     __syncthreads();
-    if (tid == 0) {
-        unsigned run = 0;
-        for (int r = 0; r < BROWS; r++) {
-            rb[r] = run;
-            run += rc[r];
+    if (tid < 64) {
+        // exclusive scan of the BROWS row counts by one wave
+        constexpr int PL = (BROWS + 63) / 64;
+        unsigned own = 0;
+#pragma unroll
+        for (int q = 0; q < PL; q++) own += tid * PL + q < BROWS ? rc[tid * PL + q] : 0u;
+        unsigned incl = own;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned y = __shfl_up(incl, o);
+            if (tid >= o) incl += y;
         }
-        rb[BROWS] = run;
+        unsigned run = incl - own;
+#pragma unroll
+        for (int q = 0; q < PL; q++)
+            if (tid * PL + q < BROWS) {
+                rb[tid * PL + q] = run;
+                run += rc[tid * PL + q];
+            }
+        if (tid == 63) rb[BROWS] = incl;
     }
     __syncthreads();
     const int r0 = b * BROWS;
@@ -1235,63 +1416,168 @@ __global__ __launch_bounds__(ROWS_THREADS, W ? 4 : 8) void bucket_rows(const lon
         }
     }
     __syncthreads();
-    // rows that fit in registers: wave wv takes rows wv, wv + NWAVE, ...
+    // short rows on every wave: wave wv takes rows wv, wv + NWAVE, ...
     for (int lr = wv; lr < BROWS && r0 + lr < n; lr += NWAVE) {
         const long long a = e0 + rb[lr];
         const int d = (int)(rb[lr + 1] - rb[lr]);
-        if (d == 0 || d > SORT_MAX) continue;
+        if (d == 0 || d > dbit) continue;
         if (d <= 64) pgdev::sort_row_regs<1>(ctmp, vtmp, a, d, lane, col, val);
         else if (d <= 128) pgdev::sort_row_regs<2>(ctmp, vtmp, a, d, lane, col, val);
         else if (d <= 256) pgdev::sort_row_regs<4>(ctmp, vtmp, a, d, lane, col, val);
         else if (d <= 512) pgdev::sort_row_regs<8>(ctmp, vtmp, a, d, lane, col, val);
         else pgdev::sort_row_regs<16>(ctmp, vtmp, a, d, lane, col, val);
     }
-    // longer rows: waves wv < nbm, one LDS bitmap over [0, n) each
-    if (wv >= nbm || rb[BROWS] <= (unsigned)SORT_MAX) return;
-    const int nw = (n + 31) >> 5;
-    uint32_t *bits = sm + (long long)wv * 2 * nw;
-    uint32_t *pre = bits + nw;
-    bool clear = false;
+    // long rows on the bitmap waves wv < nbm
+    if (wv >= nbm) return;
     for (int lr = wv; lr < BROWS && r0 + lr < n; lr += nbm) {
         const long long a = e0 + rb[lr], z1 = e0 + rb[lr + 1];
-        if (z1 - a <= SORT_MAX) continue;
-        if (!clear) {
-            for (int q = lane; q < nw; q += 64) bits[q] = 0u;
-            clear = true;
+        if (z1 - a <= dbit) continue;
+        rank_row_bitmap<W ? 8 : 16>(ctmp, vtmp, a, z1, lane, bits4, gpre, ng, per, col, val);
+    }
+}
+
+// Unweighted rows of a bucket from one LDS bitmap per row over the columns
+// [0, n): every entry sets its bit, a row's degree is the popcount of its
+// bitmap and its columns are read back in ascending order, so the rows come
+// out sorted with one read of the entries and one write of the columns.  One
+// wave per row of the bucket; work-groups loop over buckets, and the bitmap
+// is zero between buckets (the read-back clears what it finds).  Each lane
+// owns a run of `per` 128-column groups of a row (`per` odd: the 16-byte
+// reads of a wave spread over the LDS banks).
+constexpr int BITS_THREADS = 64 * BROWS;
+constexpr int BITS_UNROLL = 8;
+
+__global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long *__restrict__ bstart,
+                                                                  const unsigned *__restrict__ ent, int n, int nbk,
+                                                                  int *__restrict__ row, int *__restrict__ col,
+                                                                  long long cap)
+{
+    extern __shared__ __align__(16) uint4 bm4[];  // [BROWS][ng]
+    __shared__ unsigned rdeg[BROWS];
+    const int tid = threadIdx.x, lane = tid & 63, r = tid >> 6;
+    if (bstart[nbk] > cap) {  // nothing was scattered; row[n] reports the total
+        if (blockIdx.x == 0 && tid == 0) row[n] = (int)min(bstart[nbk], (long long)INT_MAX);
+        return;
+    }
+    const int ng = (n + 127) >> 7;
+    const int per = ((ng + 63) / 64) | 1;
+    unsigned *bm = (unsigned *)bm4;
+    uint4 *rb4 = bm4 + (long long)r * ng;
+    for (int q = lane; q < ng; q += 64) rb4[q] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+    const int g0 = lane * per, g1 = min(ng, g0 + per);
+    for (int b = blockIdx.x; b < nbk; b += gridDim.x) {
+        const long long e0 = bstart[b], e1 = bstart[b + 1];
+        for (long long k0 = e0 + tid; k0 < e1; k0 += (long long)BITS_THREADS * BITS_UNROLL) {
+            unsigned x[BITS_UNROLL];
+#pragma unroll
+            for (int u = 0; u < BITS_UNROLL; u++) {
+                const long long k = k0 + (long long)u * BITS_THREADS;
+                x[u] = k < e1 ? ent[k] : ~0u;
+            }
+#pragma unroll
+            for (int u = 0; u < BITS_UNROLL; u++)
+                if (x[u] != ~0u) {
+                    const unsigned c = x[u] & COLMASK;
+                    atomicOr(&bm[(x[u] >> 25) * (unsigned)ng * 4u + (c >> 5)], 1u << (c & 31));
+                }
         }
-        __builtin_amdgcn_wave_barrier();
-        for (long long e = a + lane; e < z1; e += 64) {
-            const int c = ctmp[e];
-            atomicOr(&bits[c >> 5], 1u << (c & 31));
+        __syncthreads();
+        // this wave's row: per-lane popcounts, one wave scan
+        unsigned s = 0;
+        for (int g = g0; g < g1; g++) {
+            const uint4 v = rb4[g];
+            s += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
         }
-        __builtin_amdgcn_wave_barrier();
-        // prefix popcount over words: per-lane contiguous ranges + wave scan
-        const int per = (nw + 63) / 64;
-        const int w0 = lane * per, w1 = min(nw, w0 + per);
-        int sacc = 0;
-        for (int q = w0; q < w1; q++) sacc += __popc(bits[q]);
-        int incl = sacc;
+        unsigned incl = s;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
-            int v = __shfl_up(incl, o);
-            if (lane >= o) incl += v;
+            const unsigned y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
         }
-        int run = incl - sacc;
-        for (int q = w0; q < w1; q++) {
-            pre[q] = run;
-            run += __popc(bits[q]);
+        if (lane == 63) rdeg[r] = incl;
+        __syncthreads();
+        unsigned before = 0;
+        for (int q = 0; q < r; q++) before += rdeg[q];
+        const int gr = b * BROWS + r;
+        if (lane == 0 && gr < n) row[gr] = (int)(e0 + before);
+        if (b == nbk - 1 && tid == 0) row[n] = (int)e1;
+        // read the lane's s columns back in order (one per step, moving to
+        // the next non-empty group when the current one is spent), then
+        // clear the lane's groups
+        int *out = col + (e0 + before + (incl - s));
+        int g = g0;
+        unsigned long long lo = 0, hi = 0;
+        for (unsigned it = 0; it < s; it++) {
+            while (!(lo | hi)) {
+                const uint4 v = rb4[g++];
+                lo = ((unsigned long long)v.y << 32) | v.x;
+                hi = ((unsigned long long)v.w << 32) | v.z;
+            }
+            int c;
+            if (lo) {
+                c = __builtin_ctzll(lo);
+                lo &= lo - 1ull;
+            } else {
+                c = 64 + __builtin_ctzll(hi);
+                hi &= hi - 1ull;
+            }
+            *out++ = (g - 1) * 128 + c;
         }
-        __builtin_amdgcn_wave_barrier();
-        for (long long e = a + lane; e < z1; e += 64) {
-            const int c = ctmp[e];
-            const int pos = pre[c >> 5] + __popc(bits[c >> 5] & ((1u << (c & 31)) - 1u));
-            col[a + pos] = c;
-            if (val) val[a + pos] = vtmp[e];
-        }
-        __builtin_amdgcn_wave_barrier();
-        for (long long e = a + lane; e < z1; e += 64) bits[ctmp[e] >> 5] = 0u;  // leave the bitmap clear
-        __builtin_amdgcn_wave_barrier();
+        if (s)
+            for (int q = g0; q < g1; q++) rb4[q] = make_uint4(0u, 0u, 0u, 0u);
+        __syncthreads();
     }
+}
+
+// LDS of bucket_rows_bits (0 when the bitmaps of a bucket do not fit)
+size_t rows_bits_lds(int n, size_t lds_per_block)
+{
+    const size_t b = (size_t)BROWS * (size_t)((n + 127) >> 7) * 16;
+    return b + 64 <= std::min<size_t>(lds_per_block, 160 * 1024) ? b : 0;
+}
+
+// its grid: as many work-groups as the LDS of every CU holds (at most four
+// per CU), no more than there are buckets
+unsigned rows_bits_grid(const pg_ctx *ctx, int n, int nbk)
+{
+    const size_t l = std::max<size_t>(rows_bits_lds(n, ctx->lds_per_block), 1);
+    const long long per_cu = std::max<long long>(1, std::min<long long>(4, (long long)(160 * 1024 / (l + 64))));
+    return (unsigned)std::max<long long>(1, std::min<long long>(nbk, (long long)ctx->num_cu * per_cu));
+}
+
+// Bitmap waves and the smallest row length they take: as many waves as fit
+// half the LDS (two work-groups per CU; at least one wave when a bitmap fits
+// at all, else nbm = 0 and the build is refused), and rows longer than dbit,
+// where dbit is the longest row for which the register bitonic sort costs
+// less than the bitmap's fixed per-row scan (per groups per lane).
+struct RowsPlan {
+    int nbm, dbit;
+    size_t lds;
+};
+__host__ inline long long rows_wave_u4(int n) { return rows_groups(n) + (rows_groups(n) + 3) / 4; }
+RowsPlan rows_plan(int n, size_t lds_per_block)
+{
+    RowsPlan p;
+    const long long wb = rows_wave_u4(n) * 16;
+    const long long lds = (long long)std::min<size_t>(lds_per_block, 160 * 1024);
+    const long long budget = lds / 2 - 1024;
+    p.nbm = (int)std::max<long long>(0, std::min<long long>(ROWS_THREADS / 64, budget / wb));
+    if (p.nbm == 0 && wb + 1024 <= lds) p.nbm = 1;
+    const long long per = ((rows_groups(n) + 63) / 64) | 1;
+    // VALU work per lane: bitonic ~ 4 P log2(64P)(log2(64P)+1)/2, bitmap
+    // ~ 10 per + 20 P; the first P where the bitmap wins sets dbit = 32 P
+    p.dbit = SORT_MAX;
+    for (int P = 2; P <= 16; P <<= 1) {
+        const int lg = 6 + (P == 2 ? 1 : P == 4 ? 2 : P == 8 ? 3 : 4);
+        const long long bitonic = 4LL * P * lg * (lg + 1) / 2, bitmap = 10 * per + 20LL * P;
+        if (bitmap < bitonic) {
+            p.dbit = 32 * P;
+            break;
+        }
+    }
+    p.lds = (size_t)p.nbm * (size_t)wb;
+    return p;
 }
 
 // float nearest to x on the far side (outward rounding of window bounds)
@@ -1659,6 +1945,14 @@ __device__ unsigned long long radix_select(long long c, long long m, KeyF key, W
 
 // Exact selection (U CorMat.cpp:693-773 on FP64 r): need = k - definite
 // candidates, all with r > t* plus the earliest ties in row-major order.
+// The selected set is every candidate whose (key, (i, j)) is at most the
+// need-th smallest such pair (key = r64 descending), found in one work-group:
+// a pass for the key range, a 4096-bin histogram of the keys' offsets above
+// the smallest, and the bin holding the need-th key; its members (a handful
+// for a window of near-uniform r) are gathered into LDS and ranked by
+// comparison.  A bin of more than SEL_CAP members takes the byte-wise radix
+// select instead.
+constexpr int SEL_NB = 4096, SEL_CAP = 2048;
 __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ cand, const double *__restrict__ r64,
                                                       const float *__restrict__ rfast,
                                                       const unsigned long long *__restrict__ cand_cnt, long long cand_cap,
@@ -1667,72 +1961,236 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
                                                       double *__restrict__ tstar, unsigned long long *__restrict__ flips,
                                                       int *__restrict__ err)
 {
-    __shared__ unsigned hist[256];
-    __shared__ unsigned long long sh[2];
-    __shared__ unsigned long long s_def;
-    if (threadIdx.x == 0) s_def = 0;
+    __shared__ unsigned hist[SEL_NB];
+    __shared__ unsigned long long mkey[SEL_CAP];
+    __shared__ unsigned mq[SEL_CAP];
+    __shared__ unsigned long long sh[2], wtot[16];
+    __shared__ unsigned long long s_def, s_k1, s_k2;
+    __shared__ double s_ts;
+    __shared__ unsigned s_m;
+    __shared__ int s_bin;
+    __shared__ long long s_rank;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int RB = 8;
+    if (tid == 0) {
+        s_def = 0;
+        s_m = 0;
+        s_bin = 0;
+        s_rank = 1;
+        s_k1 = 0;
+        s_k2 = 0;
+        s_ts = 0.0;
+    }
     __syncthreads();
     unsigned long long dpart = 0;
-    for (int g = threadIdx.x; g < dn; g += blockDim.x) dpart += dsrc[g];
+    for (int g = tid; g < dn; g += blockDim.x) dpart += dsrc[g];
     atomicAdd(&s_def, dpart);
     __syncthreads();
     const long long c = (long long)min((unsigned long long)cand_cap, *cand_cnt);
     const long long need = k - (long long)s_def;
     if (need < 1 || need > c || (long long)*cand_cnt > cand_cap) {
-        if (threadIdx.x == 0) *err = 1;
+        if (tid == 0) *err = 1;
         return;
     }
-    long long ties_left = 0;
-    const unsigned long long k1 = radix_select(
-        c, need, [&](long long q) { return key_r_desc(r64[q]); }, [](long long) { return true; }, hist, sh,
-        &ties_left);
-    // ties at t*: ordered by (i, j) only when some of them are left out (with
-    // FP64 r an exact tie is rare, and then all of the few are wanted)
-    unsigned long long k2 = ~0ull;
-    {
-        // ties at k1, and t* (every key k1 carries the same r64)
-        unsigned long long nt = 0;
-        for (long long q = threadIdx.x; q < c; q += blockDim.x)
-            if (key_r_desc(r64[q]) == k1) {
-                nt++;
-                tstar[0] = r64[q];
+    auto ij_of = [&](long long q) {
+        const int2 e = cand[q];
+        return ((unsigned long long)(unsigned)e.x << 32) | (unsigned)e.y;
+    };
+    // key range
+    unsigned long long kmin = ~0ull, kmax = 0ull;
+    for (long long q0 = tid; q0 < c; q0 += RB * 1024) {
+        unsigned long long kk[RB];
+#pragma unroll
+        for (int u = 0; u < RB; u++) {
+            const long long q = q0 + u * 1024;
+            kk[u] = q < c ? key_r_desc(r64[q]) : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < RB; u++)
+            if (q0 + u * 1024 < c) {
+                kmin = kk[u] < kmin ? kk[u] : kmin;
+                kmax = kk[u] > kmax ? kk[u] : kmax;
             }
-        if (threadIdx.x == 0) sh[0] = 0;
+    }
+    for (int b = tid; b < SEL_NB; b += 1024) hist[b] = 0u;
+    if (tid == 0) {
+        sh[0] = ~0ull;
+        sh[1] = 0ull;
+    }
+    __syncthreads();
+    atomicMin(&sh[0], kmin);
+    atomicMax(&sh[1], kmax);
+    __syncthreads();
+    kmin = sh[0];
+    kmax = sh[1];
+    int shift = 0;
+    while (((kmax - kmin) >> shift) >= (unsigned long long)SEL_NB) shift++;
+    // histogram of (key - kmin) >> shift
+    for (long long q0 = tid; q0 < c; q0 += RB * 1024) {
+        unsigned long long kk[RB];
+#pragma unroll
+        for (int u = 0; u < RB; u++) {
+            const long long q = q0 + u * 1024;
+            kk[u] = q < c ? key_r_desc(r64[q]) : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < RB; u++)
+            if (q0 + u * 1024 < c) atomicAdd(&hist[(kk[u] - kmin) >> shift], 1u);
+    }
+    __syncthreads();
+    // bin holding the need-th smallest key: thread t owns bins 4t .. 4t+3
+    {
+        const unsigned h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2], h3 = hist[4 * tid + 3];
+        const unsigned long long own = (unsigned long long)h0 + h1 + h2 + h3;
+        unsigned long long incl = own;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned long long y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) wtot[wv] = incl;
         __syncthreads();
-        atomicAdd(&sh[0], nt);
-        __syncthreads();
-        const long long ntie = (long long)sh[0];
-        __syncthreads();
-        if (ntie != ties_left) {
-            long long dummy = 0;
-            k2 = radix_select(
-                c, ties_left,
-                [&](long long q) { return ((unsigned long long)(unsigned)cand[q].x << 32) | (unsigned)cand[q].y; },
-                [&](long long q) { return key_r_desc(r64[q]) == k1; }, hist, sh, &dummy);
+        unsigned long long wbase = 0;
+        for (int w = 0; w < wv; w++) wbase += wtot[w];
+        const unsigned long long before = wbase + incl - own;
+        if (before < (unsigned long long)need && (unsigned long long)need <= before + own) {
+            const unsigned hh[4] = {h0, h1, h2, h3};
+            unsigned long long cum = before;
+            int d = 0;
+            for (int q = 0; q < 4; q++) {
+                if (cum + hh[q] >= (unsigned long long)need) {
+                    d = q;
+                    break;
+                }
+                cum += hh[q];
+            }
+            s_bin = 4 * tid + d;
+            s_rank = need - (long long)cum;  // 1-based rank inside the bin
         }
     }
     __syncthreads();
-    const double ts = tstar[0];
-    unsigned long long fl = 0;
-    const int lane = threadIdx.x & 63;
-    for (long long q0 = 0; q0 < c; q0 += blockDim.x) {  // uniform trip count: ballots below
-        const long long q = q0 + threadIdx.x;
-        bool sel = false;
-        int2 e = make_int2(0, 0);
-        if (q < c) {
-            const double rq = r64[q];
-            e = cand[q];
-            const unsigned long long kq = key_r_desc(rq);
-            const unsigned long long k2q = ((unsigned long long)(unsigned)e.x << 32) | (unsigned)e.y;
-            sel = kq < k1 || (kq == k1 && k2q <= k2);
-            fl += ((double)rfast[q] > ts) != (rq > ts);
+    const int bsel = s_bin;
+    const long long rin = s_rank;
+    const unsigned msel = hist[bsel];
+    if (msel <= (unsigned)SEL_CAP) {
+        // members of the bin into LDS
+        for (long long q0 = tid; q0 < c; q0 += RB * 1024) {
+            unsigned long long kk[RB];
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const long long q = q0 + u * 1024;
+                kk[u] = q < c ? key_r_desc(r64[q]) : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const long long q = q0 + u * 1024;
+                if (q < c && (int)((kk[u] - kmin) >> shift) == bsel) {
+                    const unsigned p = atomicAdd(&s_m, 1u);
+                    mkey[p] = kk[u];
+                    mq[p] = (unsigned)q;
+                }
+            }
         }
-        const unsigned long long m = __ballot(sel);
-        unsigned long long base = 0;
-        if (lane == 0 && m) base = atomicAdd(extra_cnt, (unsigned long long)__popcll(m));
-        base = __shfl(base, 0);
-        if (sel) extra[base + lane_rank(m)] = e;
+        __syncthreads();
+        // the member of rank rin by (key, (i, j))
+        for (unsigned x = tid; x < msel; x += 1024) {
+            const unsigned long long kx = mkey[x];
+            long long ijx = -1;
+            unsigned rank = 0;
+            for (unsigned y = 0; y < msel; y++) {
+                const unsigned long long ky = mkey[y];
+                if (ky < kx) {
+                    rank++;
+                } else if (ky == kx && y != x) {
+                    if (ijx < 0) ijx = (long long)ij_of(mq[x]);
+                    rank += ij_of(mq[y]) < (unsigned long long)ijx;
+                }
+            }
+            if ((long long)rank == rin - 1) {
+                s_k1 = kx;
+                s_k2 = ij_of(mq[x]);
+                s_ts = r64[mq[x]];
+                tstar[0] = s_ts;
+            }
+        }
+    } else {
+        // crowded bin: byte-wise radix select over all candidates
+        long long ties_left = 0;
+        const unsigned long long k1 = radix_select(
+            c, need, [&](long long q) { return key_r_desc(r64[q]); }, [](long long) { return true; }, hist, sh,
+            &ties_left);
+        const unsigned long long k2 = radix_select(
+            c, ties_left, [&](long long q) { return ij_of(q); },
+            [&](long long q) { return key_r_desc(r64[q]) == k1; }, hist, sh, &ties_left);
+        if (tid == 0) {
+            s_k1 = k1;
+            s_k2 = k2;
+        }
+        for (long long q = tid; q < c; q += 1024)
+            if (key_r_desc(r64[q]) == k1) {
+                s_ts = r64[q];
+                tstar[0] = r64[q];
+            }
     }
+    __syncthreads();
+    const unsigned long long k1 = s_k1, k2 = s_k2;
+    const double ts = s_ts;
+    // the selected candidates: this thread's count (and decision flips),
+    // one block scan and one reservation, then the writes
+    auto selected = [&](long long q, const int2 &e, double rq) {
+        const unsigned long long kq = key_r_desc(rq);
+        const unsigned long long k2q = ((unsigned long long)(unsigned)e.x << 32) | (unsigned)e.y;
+        return kq < k1 || (kq == k1 && k2q <= k2);
+    };
+    unsigned long long fl = 0, nsel = 0;
+    for (long long q0 = tid; q0 < c; q0 += RB * 1024) {
+        double rr[RB];
+        float rf[RB];
+        int2 ee[RB];
+#pragma unroll
+        for (int u = 0; u < RB; u++) {
+            const long long q = q0 + u * 1024;
+            rr[u] = q < c ? r64[q] : 0.0;
+            rf[u] = q < c ? rfast[q] : 0.0f;
+            ee[u] = q < c ? cand[q] : make_int2(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < RB; u++)
+            if (q0 + u * 1024 < c) {
+                nsel += selected(q0 + u * 1024, ee[u], rr[u]);
+                fl += ((double)rf[u] > ts) != (rr[u] > ts);
+            }
+    }
+    unsigned long long incl = nsel;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) wtot[wv] = incl;
+    __syncthreads();
+    unsigned long long base = 0, all = 0;
+    for (int w = 0; w < 16; w++) {
+        base += w < wv ? wtot[w] : 0ull;
+        all += wtot[w];
+    }
+    if (tid == 0) sh[0] = atomicAdd(extra_cnt, all);
+    __syncthreads();
+    unsigned long long p = sh[0] + base + incl - nsel;
+    if (nsel)
+        for (long long q0 = tid; q0 < c; q0 += RB * 1024) {
+            double rr[RB];
+            int2 ee[RB];
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const long long q = q0 + u * 1024;
+                rr[u] = q < c ? r64[q] : 0.0;
+                ee[u] = q < c ? cand[q] : make_int2(0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < RB; u++)
+                if (q0 + u * 1024 < c && selected(q0 + u * 1024, ee[u], rr[u])) extra[p++] = ee[u];
+        }
     atomicAdd(flips, fl);
 }
 
@@ -1849,8 +2307,7 @@ pg_status check_opts(pg_ctx *ctx, const float *bold, int32_t n, int32_t t, const
         return pg_fail(ctx, PG_ETHRESH, "sparsity %g%% outside (0, 100]", (double)opts->thr);
     if (!smode && !(opts->thr >= -1.0f && opts->thr <= 1.0f))
         return pg_fail(ctx, PG_ETHRESH, "r threshold %g outside [-1, 1]", (double)opts->thr);
-    const int nw = (n + 31) / 32;
-    if ((size_t)nw * 8 + 64 > ctx->lds_per_block)
+    if (rows_plan(n, ctx->lds_per_block).nbm == 0)
         return pg_fail(ctx, PG_ENOTSUP, "n=%d voxels exceeds the row-sort LDS bitmap", n);
     return PG_OK;
 }
@@ -2108,10 +2565,9 @@ struct CorrRun {
     {
         if (nnz > INT32_MAX) return pg_fail(ctx, PG_ENOTSUP, "nnz %lld exceeds int32 CSR", (long long)nnz);
         const size_t nnzs = (size_t)std::max<long long>(nnz, 1);
-        const int nw = (n + 31) / 32;
         unsigned *bcnt, *ent;
         long long *boff, *bstart, *btot;
-        int *ctmp;
+        int *ctmp = nullptr;
         float *entv = nullptr, *vtmp = nullptr;
         PG_TRY(pg_bufT(ctx, "corr.bcnt", (size_t)nsrc * nbk, &bcnt));
         PG_TRY(pg_bufT(ctx, "corr.boff", (size_t)nsrc * nbk, &boff));
@@ -2119,8 +2575,9 @@ struct CorrRun {
         PG_TRY(pg_bufT(ctx, "corr.btot", (size_t)nbk + 1, &btot));
         PG_TRY(pg_bufT(ctx, "corr.ent", nnzs, &ent));
         PG_TRY(pg_bufT(ctx, "corr.row", (size_t)n + 2, &row));
-        PG_TRY(pg_bufT(ctx, "corr.ctmp", nnzs, &ctmp));
         PG_TRY(pg_bufT(ctx, "corr.col", nnzs, &col));
+        const size_t lbits = weighted ? 0 : rows_bits_lds(n, ctx->lds_per_block);
+        if (!lbits) PG_TRY(pg_bufT(ctx, "corr.ctmp", nnzs, &ctmp));
         val = nullptr;
         if (weighted) {
             PG_TRY(pg_bufT(ctx, "corr.entv", nnzs, &entv));
@@ -2132,24 +2589,34 @@ struct CorrRun {
             bucket_count<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(src, nbk, bcnt);
             bucket_scan_local<<<(unsigned)nbk, 256, 0, st>>>(bcnt, nsrc, boff, btot);
             bucket_scan_global<<<1, 1024, 0, st>>>(btot, nbk, bstart);
-            const size_t shs = (size_t)nbk * (8 + 4);  // bases + cursors
+            const size_t shs = (size_t)nbk * (4 + 4);  // bases + cursors
+            const void *scatter_fn = weighted ? (const void *)bucket_scatter<true> : (const void *)bucket_scatter<false>;
             if (shs > 64 * 1024)
-                PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_scatter,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
-            bucket_scatter<<<(unsigned)nsrc, SEG_THREADS, shs, st>>>(src, nbk, boff, bstart, ent, entv, z64, t,
-                                                                     (long long)nnzs);
-            int nbm = 4;
-            while (nbm > 1 && (size_t)nbm * 8 * nw + 2048 > ctx->lds_per_block) nbm--;
-            const size_t shr = (size_t)nbm * 8 * nw;
-            const void *rows_fn = weighted ? (const void *)bucket_rows<true> : (const void *)bucket_rows<false>;
-            PG_HIP(ctx, hipFuncSetAttribute(rows_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shr));
+                PG_HIP(ctx, hipFuncSetAttribute(scatter_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
             if (weighted)
-                bucket_rows<true><<<(unsigned)nbk, ROWS_THREADS, shr, st>>>(bstart, ent, entv, n, nbk, nbm, row, ctmp,
-                                                                            vtmp, col, val, (long long)nnzs);
+                bucket_scatter<true><<<(unsigned)nsrc, SEG_THREADS, shs, st>>>(src, nbk, boff, bstart, ent, entv, z64,
+                                                                               t, (long long)nnzs);
             else
-                bucket_rows<false><<<(unsigned)nbk, ROWS_THREADS, shr, st>>>(bstart, ent, nullptr, n, nbk, nbm, row,
-                                                                             ctmp, nullptr, col, nullptr,
-                                                                             (long long)nnzs);
+                bucket_scatter<false><<<(unsigned)nsrc, SEG_THREADS, shs, st>>>(src, nbk, boff, bstart, ent, nullptr,
+                                                                                z64, t, (long long)nnzs);
+            if (lbits) {
+                PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows_bits,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lbits));
+                bucket_rows_bits<<<rows_bits_grid(ctx, n, nbk), BITS_THREADS, lbits, st>>>(bstart, ent, n, nbk, row,
+                                                                                         col, (long long)nnzs);
+                PG_KERNEL_CHECK(ctx);
+                return PG_OK;
+            }
+            const RowsPlan rp = rows_plan(n, ctx->lds_per_block);
+            if (rp.nbm == 0) return pg_fail(ctx, PG_ENOTSUP, "n = %d: the row bitmap does not fit LDS", n);
+            const void *rows_fn = weighted ? (const void *)bucket_rows<true> : (const void *)bucket_rows<false>;
+            PG_HIP(ctx, hipFuncSetAttribute(rows_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rp.lds));
+            if (weighted)
+                bucket_rows<true><<<(unsigned)nbk, ROWS_THREADS, rp.lds, st>>>(
+                    bstart, ent, entv, n, nbk, rp.nbm, rp.dbit, row, ctmp, vtmp, col, val, (long long)nnzs);
+            else
+                bucket_rows<false><<<(unsigned)nbk, ROWS_THREADS, rp.lds, st>>>(
+                    bstart, ent, nullptr, n, nbk, rp.nbm, rp.dbit, row, ctmp, nullptr, col, nullptr, (long long)nnzs);
             PG_KERNEL_CHECK(ctx);
         } else {
             PG_HIP(ctx, hipMemsetAsync(row, 0, 4, st));
@@ -2846,7 +3313,8 @@ __global__ __launch_bounds__(256) void cohort_corr_check(const SelCtl *__restric
 // workspace sized for the largest subject, reused by every subject in turn.
 struct PgCohortCorr {
     struct Subj {
-        int n = 0, t = 0, ldp = 0, nr = 0, npad = 0, nc = 0, nbk = 0, nbm = 1, wx = 1;
+        int n = 0, t = 0, ldp = 0, nr = 0, npad = 0, nc = 0, nbk = 0, nbm = 1, dbit = SORT_MAX, wx = 1;
+        size_t rows_lds = 0, bits_lds = 0;
         long long m1 = 0, kwant = 0, nst = 0, stride = 1, stl_off = 0, seg_cap = 0, ccap = 0;
         double f = 0.0, band = 0.0;
         bool graph = false;  // built in the graph (else outside it)
@@ -2883,7 +3351,8 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
     }
     P->sup = gram_super(ctx, max_n0, max_t0);
     const long long nwg = P->nwg;
-    size_t max_zt = 1, max_panel = 1;
+    size_t max_zt = 1, max_panel = 1, max_rows_lds = 16, max_bits_lds = 16;
+    long long max_ctmp = 1;
     long long max_seg = 64, max_ccap = 1, max_dpart = 1, max_nnz = 1, max_bsrc = 1, max_nbk = 1;
     int max_t = 2, max_n = 1;
     std::vector<int2> stl_all;
@@ -2905,7 +3374,7 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
         S.stride = S.nst >= 8 ? std::max<long long>(2, std::min<long long>(50, S.nst / 16)) : 1;
         // the single pass needs a sample of the supertiles and T within the
         // LDS-staged z-score; anything else is built outside the graph
-        S.graph = S.kwant > 0 && S.n > 1 && S.stride > 1 && S.t <= ZS_TMAX;
+        S.graph = S.kwant > 0 && 2 * S.kwant <= INT32_MAX && S.n > 1 && S.stride > 1 && S.t <= ZS_TMAX;
         if (!S.graph) continue;
         const long long msamp = sampled_pairs(stl, S.stride, S.nr, S.nc, g2::RT, g2::TM, g2::TN, S.n, P->sup);
         const double p = (double)S.kwant / (double)S.m1;
@@ -2923,9 +3392,19 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
         S.wx = (int)std::max<long long>(1, pg_cdiv(S.seg_cap, WIN_CHUNK));
         S.band = error_band(PREC_BF16X3, S.t);
         S.nbk = (int)pg_cdiv(S.n, BROWS);
-        const int nw = (S.n + 31) / 32;
-        S.nbm = 4;
-        while (S.nbm > 1 && (size_t)S.nbm * 8 * nw + 2048 > ctx->lds_per_block) S.nbm--;
+        S.bits_lds = rows_bits_lds(S.n, ctx->lds_per_block);
+        if (S.bits_lds) {
+            max_bits_lds = std::max(max_bits_lds, S.bits_lds);
+        } else {
+            const RowsPlan rp = rows_plan(S.n, ctx->lds_per_block);
+            if (rp.nbm == 0)
+                return pg_fail(ctx, PG_ENOTSUP, "subject %d: n = %d: the row bitmap does not fit LDS", s, S.n);
+            S.nbm = rp.nbm;
+            S.dbit = rp.dbit;
+            S.rows_lds = rp.lds;
+            max_rows_lds = std::max(max_rows_lds, rp.lds);
+            max_ctmp = std::max(max_ctmp, 2 * S.kwant);
+        }
         max_zt = std::max(max_zt, (size_t)S.n * S.t);
         max_panel = std::max(max_panel, (size_t)S.npad * S.ldp);
         max_seg = std::max(max_seg, S.seg_cap);
@@ -2965,7 +3444,7 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
     PG_TRY(pg_bufT(ctx, "coh.bstart", (size_t)max_nbk + 1, &P->bstart));
     PG_TRY(pg_bufT(ctx, "coh.btot", (size_t)max_nbk + 1, &P->btot));
     PG_TRY(pg_bufT(ctx, "coh.ent", (size_t)max_nnz, &P->ent));
-    PG_TRY(pg_bufT(ctx, "coh.ctmp", (size_t)max_nnz, &P->ctmp));
+    PG_TRY(pg_bufT(ctx, "coh.ctmp", (size_t)max_ctmp, &P->ctmp));
     // dynamic LDS limits for the largest subject, set before any capture
     const size_t shz = (size_t)max_t * (ZV + 1) * 4;
     PG_HIP(ctx, hipFuncSetAttribute((const void *)zscore_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shz));
@@ -2974,17 +3453,14 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
                                     (int)(stage + NBIN * 4)));
     PG_HIP(ctx, hipFuncSetAttribute((const void *)gram2_kernel<MODE_EMIT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)stage));
-    const size_t shs = (size_t)max_nbk * 12;
+    const size_t shs = (size_t)max_nbk * 8;
     if (shs > 64 * 1024)
-        PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)shs));
-    {
-        const int nw = (max_n + 31) / 32;
-        int nbm = 4;
-        while (nbm > 1 && (size_t)nbm * 8 * nw + 2048 > ctx->lds_per_block) nbm--;
-        PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)((size_t)nbm * 8 * nw)));
-    }
+        PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_scatter<false>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
+    PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)max_rows_lds));
+    PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows_bits, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)max_bits_lds));
     *out = guard.release();
     return PG_OK;
 }
@@ -3074,11 +3550,14 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     bucket_count<<<(unsigned)nsrcb, SEG_THREADS, (size_t)S.nbk * 4, st>>>(es, S.nbk, P->bcnt);
     bucket_scan_local<<<(unsigned)S.nbk, 256, 0, st>>>(P->bcnt, nsrcb, P->boff, P->btot);
     bucket_scan_global<<<1, 1024, 0, st>>>(P->btot, S.nbk, P->bstart);
-    bucket_scatter<<<(unsigned)nsrcb, SEG_THREADS, (size_t)S.nbk * 12, st>>>(es, S.nbk, P->boff, P->bstart, P->ent,
-                                                                            nullptr, P->z64, S.t, nnz);
-    const int nw = (S.n + 31) / 32;
-    bucket_rows<false><<<(unsigned)S.nbk, ROWS_THREADS, (size_t)S.nbm * 8 * nw, st>>>(
-        P->bstart, P->ent, nullptr, S.n, S.nbk, S.nbm, row, P->ctmp, nullptr, col, nullptr, nnz);
+    bucket_scatter<false><<<(unsigned)nsrcb, SEG_THREADS, (size_t)S.nbk * 8, st>>>(es, S.nbk, P->boff, P->bstart,
+                                                                                   P->ent, nullptr, P->z64, S.t, nnz);
+    if (S.bits_lds)
+        bucket_rows_bits<<<rows_bits_grid(ctx, S.n, S.nbk), BITS_THREADS, S.bits_lds, st>>>(P->bstart, P->ent, S.n,
+                                                                                          S.nbk, row, col, nnz);
+    else
+        bucket_rows<false><<<(unsigned)S.nbk, ROWS_THREADS, S.rows_lds, st>>>(
+            P->bstart, P->ent, nullptr, S.n, S.nbk, S.nbm, S.dbit, row, P->ctmp, nullptr, col, nullptr, nnz);
     cohort_corr_check<<<1, 256, 0, st>>>(P->ctl, c, P->seg_cnt, (int)nwg, S.seg_cap, S.ccap, S.kwant, row, S.n,
                                          status);
     PG_KERNEL_CHECK(ctx);

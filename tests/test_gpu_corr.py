@@ -108,6 +108,25 @@ def test_smode_ties_row_major(ctx, prec):
     assert info.t_star == tstar
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_smode_tie_group_larger_than_a_bin(ctx, prec):
+    # two series, 200 copies each: 39,800 pairs at r = 1 and 40,000 tied at
+    # r(A, B); 75% sparsity cuts the tied group, whose members all share one
+    # selection bin (more than the selection gathers into LDS)
+    rng = np.random.default_rng(5)
+    a, b = rng.standard_normal((2, 120)).astype(np.float32)
+    x = np.empty((120, 400), np.float32)
+    x[:, :200] = a[:, None]
+    x[:, 200:] = b[:, None]
+    z64, _ = O.zscore(x)
+    orow, ocol, _, tstar = O.corr_csr_s(z64, 75.0)
+    say(f"tie group prec={prec} nnz={orow[-1]}")
+    info, row, col, _ = _build(ctx, x, 75.0, sparsity=True, prec=prec)
+    np.testing.assert_array_equal(row, orow)
+    np.testing.assert_array_equal(col, ocol)
+    assert info.t_star == tstar
+
+
 def _sparsity_splitting_a_tie(z64, n):
     """A sparsity percent whose k-th largest r is tied and the cut splits the tie."""
     iu = np.triu_indices(n, 1)

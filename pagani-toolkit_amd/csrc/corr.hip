@@ -98,9 +98,6 @@ constexpr int B_BYTES = TN * KC * 2;              // one array of the column sli
 constexpr int STAGE = 2 * A_BYTES + 2 * B_BYTES;  // 24 KB
 constexpr int DMA_WAVE = STAGE / 1024 / 4;        // 1 KB wave loads per wave per stage (6)
 static_assert(DMA_WAVE * 4 * 1024 == STAGE, "stage must split into 1 KB wave loads");
-// hits of one wave's block staged in LDS before the global writes: a quarter
-// of the ring slot that is free during the epilogue, pairs then r~
-constexpr int STAGE_HITS = STAGE / 4 / 12;  // 512
 }  // namespace g2
 
 __host__ __device__ __forceinline__ int g2_swz(int i) { return ((i >> 2) ^ (i >> 3)) & 1; }
@@ -685,29 +682,6 @@ __device__ __forceinline__ void g2_write(const GramArgs &a, const f32x16 (&acc)[
             }
 }
 
-// Pass 2 into LDS: lane hits in order from its slots pe (edges) and pc
-// (rechecks, placed after every edge) of the staging area.
-template <bool BOUNDED, bool SPLIT>
-__device__ __forceinline__ void g2_stage(const f32x16 (&acc)[g2::BMW][g2::BNW], float lo, float up, int ib, int jb,
-                                         int nn, int2 *spair, float *sval, unsigned pe, unsigned pc)
-{
-#pragma unroll
-    for (int bm = 0; bm < g2::BMW; bm++)
-#pragma unroll
-        for (int bn = 0; bn < g2::BNW; bn++)
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const float v = acc[bm][bn][q];
-                const int i = ib + bm * 32 + (q >> 2) * 8 + (q & 3), j = jb + bn * 32;
-                bool hit = v >= lo;
-                if (!BOUNDED) hit = hit & (j < nn) & (i < j);
-                if (!hit) continue;
-                const unsigned p = (SPLIT && !(v > up)) ? pc++ : pe++;
-                spair[p] = make_int2(i, j);
-                sval[p] = v;
-            }
-}
-
 // Epilogue of one wave's 128 x 64 block: pass 1 counts per lane (packed: low
 // 16 bits edges v > upper, high 16 bits rechecks lower <= v <= upper; each at
 // most 8192 per wave); one reservation per wave; pass 2 has each lane write
@@ -715,7 +689,7 @@ __device__ __forceinline__ void g2_stage(const f32x16 (&acc)[g2::BMW][g2::BNW], 
 // inside the matrix and above the diagonal.
 template <bool BOUNDED, bool SPLIT>
 __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f32x16 (&acc)[g2::BMW][g2::BNW], int i0,
-                                        int j0, int lane, int2 *myseg, unsigned long long *s_seg, char *stage)
+                                        int j0, int lane, int2 *myseg, unsigned long long *s_seg)
 {
     const int ib0 = i0 + (lane >> 5) * 4, jb0 = j0 + (lane & 31);
     unsigned cnt = 0;
@@ -763,28 +737,6 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
     // r~ of every segment entry is kept exactly when there is no band (the
     // sparsity single pass)
     float *segr = SPLIT ? nullptr : a.seg_r + (long long)blockIdx.x * a.seg_cap;
-    const unsigned te = tot & 0xFFFFu, tc = tot >> 16;
-    if (fits && te + tc <= (unsigned)g2::STAGE_HITS) {
-        // a few hits: stage them in LDS in the same order, then write each
-        // output run with whole-wave stores
-        int2 *spair = (int2 *)stage;
-        float *sval = (float *)(stage + g2::STAGE_HITS * 8);
-        g2_stage<BOUNDED, SPLIT>(acc, lo, up, ib, jb, nn, spair, sval, ex & 0xFFFFu, te + (ex >> 16));
-        __builtin_amdgcn_wave_barrier();
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        const unsigned long long be0 = be - (ex & 0xFFFFu), bc0 = bc - (ex >> 16);
-        for (unsigned k = lane; k < te; k += 64) {
-            myseg[be0 + k] = spair[k];
-            if (!SPLIT) segr[be0 + k] = sval[k];
-        }
-        if (SPLIT)
-            for (unsigned k = lane; k < tc; k += 64) {
-                a.band[bc0 + k] = spair[te + k];
-                a.band_r[bc0 + k] = sval[te + k];
-            }
-        __builtin_amdgcn_wave_barrier();
-        return;
-    }
     if (fits) g2_write<BOUNDED, SPLIT, false>(a, acc, lo, up, ib, jb, nn, myseg, segr, be, bc);
     else g2_write<BOUNDED, SPLIT, true>(a, acc, lo, up, ib, jb, nn, myseg, segr, be, bc);
 }
@@ -795,8 +747,7 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
 template <int MODE>
 __device__ __forceinline__ void g2_tile_epilogue(const GramArgs &a, const f32x16 (&acc)[g2::BMW][g2::BNW], int R,
                                                  int C, int wm, int wn, int lane, float lower, bool split,
-                                                 int2 *myseg, unsigned long long *s_seg, uint32_t *s_hist,
-                                                 char *stage)
+                                                 int2 *myseg, unsigned long long *s_seg, uint32_t *s_hist)
 {
     const int i0 = R * g2::TM + wm * g2::WR, j0 = C * g2::TN + wn * g2::WC;
     const bool interior = C * g2::TN >= R * g2::TM + g2::TM && (C + 1) * g2::TN <= a.n;
@@ -817,11 +768,11 @@ __device__ __forceinline__ void g2_tile_epilogue(const GramArgs &a, const f32x16
                     }
                 }
     } else if (split) {
-        if (interior) g2_emit<true, true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage);
-        else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage);
+        if (interior) g2_emit<true, true>(a, lower, acc, i0, j0, lane, myseg, s_seg);
+        else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, s_seg);
     } else {
-        if (interior) g2_emit<true, false>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage);
-        else g2_emit<false, false>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage);
+        if (interior) g2_emit<true, false>(a, lower, acc, i0, j0, lane, myseg, s_seg);
+        else g2_emit<false, false>(a, lower, acc, i0, j0, lane, myseg, s_seg);
     }
 }
 
@@ -917,17 +868,7 @@ __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
             slot = slot + 1 == NSTAGE ? 0 : slot + 1;
         }
 
-        // the slot of the tile's last chunk is free once every wave has
-        // multiplied it (the next tile's chunks 0 and 1 are in the other two)
-        char *stage = nullptr;
-        if (MODE == MODE_EMIT) {
-            __builtin_amdgcn_s_barrier();
-            asm volatile("" ::: "memory");
-            int fs = slot + NSTAGE - 1;
-            fs -= fs >= NSTAGE ? NSTAGE : 0;
-            stage = smem + fs * STAGE + w * (STAGE / 4);
-        }
-        g2_tile_epilogue<MODE>(a, acc, R, C, wm, wn, lane, lower, split, myseg, &s_seg, s_hist, stage);
+        g2_tile_epilogue<MODE>(a, acc, R, C, wm, wn, lane, lower, split, myseg, &s_seg, s_hist);
         R = R1;
         C = C1;
         have = have1;

@@ -75,6 +75,24 @@ def test_smode_bit_exact(ctx, prec, weighted, s):
     assert info.nnz == 2 * O.sparsity_k(640, s)
 
 
+@pytest.mark.parametrize("t", [30, 40, 48])
+def test_bf16x3_chunk_counts(ctx, t):
+    # T = 30 fills the two 16-deep chunks the ring starts with (no copy is
+    # issued inside a tile), T = 40 and 48 three (one copy in flight across
+    # the tile boundary): r-mode and sparsity against the oracle
+    x = _data(900, t, 9, 17)
+    z64, _ = O.zscore(x)
+    say(f"chunks t={t}")
+    info, row, col, val = _build(ctx, x, 0.35, prec=1)
+    orow, ocol, oval = O.corr_csr_r(z64, 0.35, False)
+    _assert_same(row, col, val, orow, ocol, None)
+    info, row, col, _ = _build(ctx, x, 2.0, sparsity=True, prec=1)
+    orow, ocol, _, tstar = O.corr_csr_s(z64, 2.0)
+    np.testing.assert_array_equal(row, orow)
+    np.testing.assert_array_equal(col, ocol)
+    assert info.t_star == tstar
+
+
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("s", [1.0, 3.0])
 def test_smode_single_pass_large(ctx, prec, s):

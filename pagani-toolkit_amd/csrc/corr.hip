@@ -98,6 +98,9 @@ constexpr int B_BYTES = TN * KC * 2;              // one array of the column sli
 constexpr int STAGE = 2 * A_BYTES + 2 * B_BYTES;  // 24 KB
 constexpr int DMA_WAVE = STAGE / 1024 / 4;        // 1 KB wave loads per wave per stage (6)
 static_assert(DMA_WAVE * 4 * 1024 == STAGE, "stage must split into 1 KB wave loads");
+// hits of one wave's block staged in LDS before the global writes: a quarter
+// of the ring slot that is free during the epilogue, pairs then r~
+constexpr int STAGE_HITS = STAGE / 4 / 12;  // 512
 }  // namespace g2
 
 __host__ __device__ __forceinline__ int g2_swz(int i) { return ((i >> 2) ^ (i >> 3)) & 1; }
@@ -288,7 +291,11 @@ struct TileWalk {
             k += L;
             const long long s = (xcd + (kk / sup2) * NXCD) * sstep;
             const int q = (int)(kk % sup2);
-            const int2 sc = st[s];
+            // read through the constant address space: a scalar load, which
+            // does not make the wave wait for the ring's copies in flight as
+            // a vector load's vmcnt wait would (the list is read-only here)
+            const __attribute__((address_space(4))) int *stc = (const __attribute__((address_space(4))) int *)st;
+            const int2 sc = make_int2(stc[2 * s], stc[2 * s + 1]);
             const int r = sc.x * sup + q / sup, c = sc.y * sup + q % sup;
             if (r < nr && c < nc && c >= rt * r) {
                 R = r;
@@ -648,6 +655,7 @@ __device__ __forceinline__ void g2_dma(const GramArgs &a, char *dst, int R, int 
 // s_waitcnt vmcnt(n) (gfx9 encoding, other counters left alone)
 #define G2_WAIT_VM(n) __builtin_amdgcn_s_waitcnt((0x3F70) | ((n) & 15) | ((((n) >> 4) & 3) << 14))
 
+
 // Pass 2 of the epilogue: each lane writes its hits in order from its slots
 // be (edges) and bc (rechecks); CHECK guards every write against capacity.
 template <bool BOUNDED, bool SPLIT, bool CHECK>
@@ -682,14 +690,117 @@ __device__ __forceinline__ void g2_write(const GramArgs &a, const f32x16 (&acc)[
             }
 }
 
+// LDS byte address of a pointer into the kernel's LDS
+__device__ __forceinline__ unsigned g2_lds_addr(const void *p)
+{
+    return (unsigned)(size_t)(const __attribute__((address_space(3))) char *)p;
+}
+
+// The staging area's LDS accesses are written in inline assembly: the
+// compiler cannot tell the staging slot from the slots the ring's copies
+// are still filling, and would wait for every copy (vmcnt(0)) before each
+// access; no copy targets the staging slot while it is in use.
+__device__ __forceinline__ void g2_lds_put(unsigned a8, unsigned a4, int i, int j, float v)
+{
+    const unsigned long long ij = ((unsigned long long)(unsigned)j << 32) | (unsigned)i;
+    asm volatile("ds_write_b64 %0, %1\n\tds_write_b32 %2, %3" ::"v"(a8), "v"(ij), "v"(a4), "v"(v) : "memory");
+}
+// the work-group's segment cursor, reserved the same way (the compiler would
+// otherwise wait for every copy in flight before the atomic)
+__device__ __forceinline__ unsigned long long g2_lds_add(unsigned long long *p, unsigned long long v)
+{
+    unsigned long long old;
+    asm volatile("ds_add_rtn_u64 %0, %1, %2\n\ts_waitcnt lgkmcnt(0)" : "=v"(old) : "v"(g2_lds_addr(p)), "v"(v) : "memory");
+    return old;
+}
+// histogram bins likewise (no result, so no wait at all)
+__device__ __forceinline__ void g2_lds_inc(uint32_t *p)
+{
+    asm volatile("ds_add_u32 %0, %1" ::"v"(g2_lds_addr(p)), "v"(1u) : "memory");
+}
+__device__ __forceinline__ void g2_lds_get(unsigned a8, unsigned a4, unsigned long long &ij, float &v)
+{
+    asm volatile("ds_read_b64 %0, %2\n\tds_read_b32 %1, %3\n\ts_waitcnt lgkmcnt(0)"
+                 : "=v"(ij), "=v"(v)
+                 : "v"(a8), "v"(a4)
+                 : "memory");
+}
+
+// Pass 2 into LDS: the lane's hits in order, edges from index pe and rechecks
+// from index pc (placed after every edge of the wave), of which those in
+// [base, base + STAGE_HITS) are written at index - base.  spair / sval: LDS
+// byte addresses of the staged pairs and values.
+template <bool BOUNDED, bool SPLIT>
+__device__ __forceinline__ void g2_stage(const f32x16 (&acc)[g2::BMW][g2::BNW], float lo, float up, int ib, int jb,
+                                         int nn, unsigned spair, unsigned sval, unsigned pe, unsigned pc, unsigned base)
+{
+#pragma unroll
+    for (int bm = 0; bm < g2::BMW; bm++)
+#pragma unroll
+        for (int bn = 0; bn < g2::BNW; bn++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const float v = acc[bm][bn][q];
+                const int i = ib + bm * 32 + (q >> 2) * 8 + (q & 3), j = jb + bn * 32;
+                bool hit = v >= lo;
+                if (!BOUNDED) hit = hit & (j < nn) & (i < j);
+                if (!hit) continue;
+                const unsigned p = ((SPLIT && !(v > up)) ? pc++ : pe++) - base;
+                if (p < (unsigned)g2::STAGE_HITS) g2_lds_put(spair + p * 8u, sval + p * 4u, i, j, v);
+            }
+}
+
+// A wave's staged hits waiting for their global stores.  The stores are
+// issued at the next tile's first step, after its wait for the ring's copy
+// and before its barrier: a wait for a copy also waits for every store issued
+// before it, so stores issued in the epilogue would hold the next tile's
+// first wait for their full latency, while these have a whole step of MFMAs
+// to complete before the next wait.
+struct G2Pending {
+    unsigned te, tc;       // edges, then rechecks, staged
+    unsigned be0, bc0;     // their positions in the segment and the recheck list
+    unsigned spair, sval;  // LDS byte addresses of the staged pairs and r~
+    bool on;
+};
+
+// The stores of a wave's staged hits (the staging slot's next copy is issued
+// only after the work-group barrier that follows).
+template <bool SPLIT>
+__device__ __forceinline__ void g2_flush(const GramArgs &a, const G2Pending &pd, int lane, int2 *myseg)
+{
+    float *segr = SPLIT ? nullptr : a.seg_r + (long long)blockIdx.x * a.seg_cap;
+    const unsigned long long cap = (unsigned long long)a.seg_cap, bcap = (unsigned long long)a.band_cap;
+    for (unsigned k = lane; k < pd.te + pd.tc; k += 64) {
+        unsigned long long ij;
+        float v;
+        g2_lds_get(pd.spair + k * 8u, pd.sval + k * 4u, ij, v);
+        const int2 e = make_int2((int)(unsigned)ij, (int)(unsigned)(ij >> 32));
+        if (k < pd.te) {
+            const unsigned long long p = (unsigned long long)pd.be0 + k;
+            if (p < cap) {
+                myseg[p] = e;
+                if (!SPLIT) segr[p] = v;
+            }
+        } else {
+            const unsigned long long p = (unsigned long long)pd.bc0 + (k - pd.te);
+            if (p < bcap) {
+                a.band[p] = e;
+                a.band_r[p] = v;
+            }
+        }
+    }
+}
+
 // Epilogue of one wave's 128 x 64 block: pass 1 counts per lane (packed: low
 // 16 bits edges v > upper, high 16 bits rechecks lower <= v <= upper; each at
-// most 8192 per wave); one reservation per wave; pass 2 has each lane write
-// its own hits in order from its exclusive prefix.  BOUNDED: every pair is
-// inside the matrix and above the diagonal.
+// most 8192 per wave); one reservation per wave; then the wave's hits (edges,
+// then rechecks) are staged in LDS in output order and left to g2_flush, or,
+// beyond STAGE_HITS of them, each lane writes its own (g2_write).  BOUNDED:
+// every pair is inside the matrix and above the diagonal.
 template <bool BOUNDED, bool SPLIT>
 __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f32x16 (&acc)[g2::BMW][g2::BNW], int i0,
-                                        int j0, int lane, int2 *myseg, unsigned long long *s_seg)
+                                        int j0, int lane, int2 *myseg, unsigned long long *s_seg, char *stage,
+                                        G2Pending &pd)
 {
     const int ib0 = i0 + (lane >> 5) * 4, jb0 = j0 + (lane & 31);
     unsigned cnt = 0;
@@ -718,27 +829,40 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
     if (tot == 0) return;
     unsigned long long be = 0, bc = 0;
     if (lane == 0) {
-        if (tot & 0xFFFFu) be = atomicAdd(s_seg, (unsigned long long)(tot & 0xFFFFu));
+        if (tot & 0xFFFFu) be = g2_lds_add(s_seg, (unsigned long long)(tot & 0xFFFFu));
         if (tot >> 16) bc = atomicAdd(a.band_cnt, (unsigned long long)(tot >> 16));
     }
     const unsigned ex = x - cnt;
-    be = __shfl(be, 0);
-    bc = __shfl(bc, 0);
-    // every write of this wave and tile fits its buffer: no per-value checks
-    const bool fits = be + (tot & 0xFFFFu) <= (unsigned long long)a.seg_cap &&
-                      bc + (tot >> 16) <= (unsigned long long)a.band_cap;
-    be += ex & 0xFFFFu;
-    bc += ex >> 16;
-    // opaque copies: pass 2 recomputes its predicates instead of the compiler
-    // keeping pass 1's 128 of them live
+    const unsigned be0 = (unsigned)__shfl(be, 0), bc0 = (unsigned)__shfl(bc, 0);
+    // opaque copies: the staging pass recomputes its predicates instead of
+    // the compiler keeping the count pass's 128 of them live
     float lo = lower, up = a.upper;
     int ib = ib0, jb = jb0, nn = a.n;
     asm volatile("" : "+v"(lo), "+v"(up), "+v"(ib), "+v"(jb), "+v"(nn));
+    const unsigned te = tot & 0xFFFFu, tt = te + (tot >> 16);
+    const unsigned seg_cap = (unsigned)a.seg_cap, band_cap = (unsigned)a.band_cap;
     // r~ of every segment entry is kept exactly when there is no band (the
     // sparsity single pass)
     float *segr = SPLIT ? nullptr : a.seg_r + (long long)blockIdx.x * a.seg_cap;
-    if (fits) g2_write<BOUNDED, SPLIT, false>(a, acc, lo, up, ib, jb, nn, myseg, segr, be, bc);
-    else g2_write<BOUNDED, SPLIT, true>(a, acc, lo, up, ib, jb, nn, myseg, segr, be, bc);
+    if (tt > (unsigned)g2::STAGE_HITS) {
+        // more hits than the staging area holds: each lane writes its own
+        const bool fits = (unsigned long long)be0 + te <= (unsigned long long)a.seg_cap &&
+                          (unsigned long long)bc0 + (tt - te) <= (unsigned long long)a.band_cap;
+        const unsigned long long bl = be0 + (ex & 0xFFFFu), cl = bc0 + (ex >> 16);
+        if (fits) g2_write<BOUNDED, SPLIT, false>(a, acc, lo, up, ib, jb, nn, myseg, segr, bl, cl);
+        else g2_write<BOUNDED, SPLIT, true>(a, acc, lo, up, ib, jb, nn, myseg, segr, bl, cl);
+        return;
+    }
+    const unsigned spair = g2_lds_addr(stage), sval = spair + g2::STAGE_HITS * 8;
+    g2_stage<BOUNDED, SPLIT>(acc, lo, up, ib, jb, nn, spair, sval, ex & 0xFFFFu, te + (ex >> 16), 0u);
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    pd.te = te;
+    pd.tc = tt - te;
+    pd.be0 = be0;
+    pd.bc0 = bc0;
+    pd.spair = spair;
+    pd.sval = sval;
+    pd.on = true;
 }
 
 // Epilogue of one wave's block of tile (R, C): lane holds column lane%32 and
@@ -747,7 +871,8 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
 template <int MODE>
 __device__ __forceinline__ void g2_tile_epilogue(const GramArgs &a, const f32x16 (&acc)[g2::BMW][g2::BNW], int R,
                                                  int C, int wm, int wn, int lane, float lower, bool split,
-                                                 int2 *myseg, unsigned long long *s_seg, uint32_t *s_hist)
+                                                 int2 *myseg, unsigned long long *s_seg, uint32_t *s_hist,
+                                                 char *stage, G2Pending &pd)
 {
     const int i0 = R * g2::TM + wm * g2::WR, j0 = C * g2::TN + wn * g2::WC;
     const bool interior = C * g2::TN >= R * g2::TM + g2::TM && (C + 1) * g2::TN <= a.n;
@@ -764,19 +889,21 @@ __device__ __forceinline__ void g2_tile_epilogue(const GramArgs &a, const f32x16
                     if (interior || ((j < a.n) && (i < j))) {
                         int b = (int)((v + 1.0f) * (NBIN * 0.5f));
                         b = b < 0 ? 0 : (b >= NBIN ? NBIN - 1 : b);
-                        atomicAdd(&s_hist[b], 1u);
+                        g2_lds_inc(&s_hist[b]);
                     }
                 }
     } else if (split) {
-        if (interior) g2_emit<true, true>(a, lower, acc, i0, j0, lane, myseg, s_seg);
-        else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, s_seg);
+        if (interior) g2_emit<true, true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
+        else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
     } else {
-        if (interior) g2_emit<true, false>(a, lower, acc, i0, j0, lane, myseg, s_seg);
-        else g2_emit<false, false>(a, lower, acc, i0, j0, lane, myseg, s_seg);
+        if (interior) g2_emit<true, false>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
+        else g2_emit<false, false>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
     }
 }
 
-template <int MODE>
+// SPLIT (emit mode): hits above a.upper are edges, the others go to the
+// recheck list; without it every hit is an edge stored with its r~.
+template <int MODE, bool SPLIT = false>
 __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
 {
     constexpr int TM = g2::TM, TN = g2::TN, WR = g2::WR, WC = g2::WC, BMW = g2::BMW, BNW = g2::BNW;
@@ -793,7 +920,7 @@ __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
         for (int b = tid; b < NBIN; b += THREADS) s_hist[b] = 0;
     const float lower = a.lower_dev ? *a.lower_dev : a.lower;
     // upper below -1: every hit is an edge (no recheck band)
-    const bool split = a.upper >= -1.0f;
+    constexpr bool split = SPLIT;
 
     // per-lane operand offsets inside a stage (row l % 32, half l / 32)
     const int lr = lane & 31, lh = lane >> 5;
@@ -813,6 +940,8 @@ __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
         g2_dma(a, smem + STAGE, R, C, 1);
     }
     int slot = 0;
+    G2Pending pend;
+    pend.on = false;
 
     while (have) {
         f32x16 acc[BMW][BNW];
@@ -824,10 +953,18 @@ __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
                 for (int q = 0; q < 16; q++) acc[bm][bn][q] = 0.0f;
 
         for (int kc = 0; kc < K; kc++) {
-            // chunk kc has landed once at most the next chunk's loads are
-            // outstanding (vector memory operations complete in order)
+            // chunk kc has landed once at most the operations issued after
+            // it are outstanding (vector memory operations complete in
+            // order): the next chunk's copies, and for chunks 0 and 1 the
+            // previous epilogue's stores
             if (kc + 1 < K || have1) G2_WAIT_VM(DMA_WAVE);
             else G2_WAIT_VM(0);
+            // the last tile's staged hits leave before the barrier after
+            // which their slot is refilled
+            if (MODE == MODE_EMIT && kc == 0 && pend.on) {
+                g2_flush<SPLIT>(a, pend, lane, myseg);
+                pend.on = false;
+            }
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");  // no LDS access moves across the barrier
             // the slot two ahead was last read one step ago by every wave
@@ -868,13 +1005,25 @@ __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
             slot = slot + 1 == NSTAGE ? 0 : slot + 1;
         }
 
-        g2_tile_epilogue<MODE>(a, acc, R, C, wm, wn, lane, lower, split, myseg, &s_seg, s_hist);
+        // the slot of the tile's last chunk is free once every wave has
+        // multiplied it (the next tile's chunks 0 and 1 are in the other two)
+        char *stage = nullptr;
+        if (MODE == MODE_EMIT) {
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            int fs = slot + NSTAGE - 1;
+            fs -= fs >= NSTAGE ? NSTAGE : 0;
+            stage = smem + fs * STAGE + w * (STAGE / 4);
+        }
+        g2_tile_epilogue<MODE>(a, acc, R, C, wm, wn, lane, lower, split, myseg, &s_seg, s_hist, stage, pend);
         R = R1;
         C = C1;
         have = have1;
         if (have) have1 = walk.next(R1, C1);
     }
+    if (MODE == MODE_EMIT && pend.on) g2_flush<SPLIT>(a, pend, lane, myseg);
     G2_WAIT_VM(0);
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // the histogram's inline-asm LDS adds
     __syncthreads();
     if (MODE == MODE_HIST) {
         uint32_t *out = a.hist + (long long)blockIdx.x * NBIN;
@@ -2402,7 +2551,9 @@ struct CorrRun {
         size_t shm = stage + (mode == MODE_HIST ? NBIN * 4 : 0);
         const void *fn;
         if (g2path)
-            fn = mode == MODE_HIST ? (const void *)gram2_kernel<MODE_HIST> : (const void *)gram2_kernel<MODE_EMIT>;
+            fn = mode == MODE_HIST ? (const void *)gram2_kernel<MODE_HIST>
+                                   : (ga.upper >= -1.0f ? (const void *)gram2_kernel<MODE_EMIT, true>
+                                                        : (const void *)gram2_kernel<MODE_EMIT, false>);
         else
             fn = mode == MODE_HIST ? (const void *)gram_kernel<PREC_F32, MODE_HIST>
                                    : (const void *)gram_kernel<PREC_F32, MODE_EMIT>;
@@ -2411,7 +2562,8 @@ struct CorrRun {
         PG_HIP(ctx, hipEventRecord(ctx->ev[e0], st));
         if (g2path) {
             if (mode == MODE_HIST) gram2_kernel<MODE_HIST><<<(unsigned)nwg_hist, THREADS, shm, st>>>(ga);
-            else gram2_kernel<MODE_EMIT><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
+            else if (ga.upper >= -1.0f) gram2_kernel<MODE_EMIT, true><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
+            else gram2_kernel<MODE_EMIT, false><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
         } else {
             if (mode == MODE_HIST) gram_kernel<PREC_F32, MODE_HIST><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
             else gram_kernel<PREC_F32, MODE_EMIT><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
@@ -3392,7 +3544,7 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
     const size_t stage = (size_t)g2::NSTAGE * g2::STAGE;
     PG_HIP(ctx, hipFuncSetAttribute((const void *)gram2_kernel<MODE_HIST>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)(stage + NBIN * 4)));
-    PG_HIP(ctx, hipFuncSetAttribute((const void *)gram2_kernel<MODE_EMIT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    PG_HIP(ctx, hipFuncSetAttribute((const void *)gram2_kernel<MODE_EMIT, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)stage));
     const size_t shs = (size_t)max_nbk * 8;
     if (shs > 64 * 1024)
@@ -3457,7 +3609,7 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     ga.band_r = P->band_r;
     ga.band_cap = 0;
     ga.band_cnt = P->counters + 0;
-    gram2_kernel<MODE_EMIT><<<(unsigned)nwg, THREADS, stage, st>>>(ga);
+    gram2_kernel<MODE_EMIT, false><<<(unsigned)nwg, THREADS, stage, st>>>(ga);
     const int hb = (int)std::min<long long>(256, nwg);
     fill_u32<<<(unsigned)pg_cdiv(2 * NBIN2, 256), 256, 0, st>>>((uint32_t *)P->h2t, 2 * NBIN2);
     emitted_hist<<<hb, 1024, 0, st>>>(P->seg_r, S.seg_cap, P->seg_cnt, (int)nwg, P->ctl, P->h2);

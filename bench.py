@@ -36,11 +36,11 @@ HBM_PEAK = 8.0e12       # bytes/s
 # 64 lanes x 2.4 GHz = 39.3e12 ops/s, 1.5 ops per update (v_add_f32 +
 # half a v_min3_f32)
 FW_UPDATES_PEAK = 39.3e12 / 1.5
-# HBM bytes per launch of the emitting BF16x3 Gram kernel at this workload,
-# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch), each
-# scaled by what the counter reads for a 1 GiB device copy (FETCH_SIZE x2,
-# WRITE_SIZE x1); profiles/gram2_emit_n58000_t240_counters.md
-GRAM_EMIT_TRAFFIC = 1943480.69 * 1024 * 2 + 277847.88 * 1024
+# HBM bytes per launch of the emitting BF16x3 Gram kernel at this workload
+# (tuned supertile edge 4), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
+# (KiB per dispatch), each scaled by what the counter reads for a 1 GiB device
+# copy (FETCH_SIZE x2, WRITE_SIZE x1); profiles/gram2_emit_n58000_t240_counters.md
+GRAM_EMIT_TRAFFIC = 2757234.8 * 1024 * 2 + 247585.9 * 1024
 
 METRIC = "per-subject wall time N=58k; corr TFLOP/s+MFMA%; BC GTEPS; BF16x3 vs FP32"
 

@@ -2373,6 +2373,10 @@ long long sampled_pairs(const std::vector<int2> &stl, long long stride, int nr, 
                 if (r >= nr || c >= nc || c < rt * r) continue;
                 const long long c0 = (long long)c * tn, c1 = std::min<long long>(n, c0 + tn);
                 const long long i1 = std::min<long long>(n, (long long)r * tm + tm);
+                if (c0 >= i1) {  // every row of the tile is below every column
+                    if (c1 > c0 && i1 > (long long)r * tm) tot += (i1 - (long long)r * tm) * (c1 - c0);
+                    continue;
+                }
                 for (long long i = (long long)r * tm; i < i1; i++) {
                     const long long lo = std::max(i + 1, c0);
                     if (c1 > lo) tot += c1 - lo;
@@ -2419,6 +2423,41 @@ float thr_effective(float thr, bool weighted) { return weighted ? (float)((doubl
 
 // One construction's geometry, device buffers and launch helpers, shared by
 // pg_corr_build and the pg_corr_emit / pg_corr_select pair.
+// What the host checks after the single sparsity pass, gathered on the
+// device so that one small copy into pinned memory brings it back.
+struct SpReport {
+    unsigned long long seg_max;      // the fullest segment's count
+    unsigned long long counters[16]; // the pass's counters
+    SelCtl ctl;
+    double tstar;
+    int rn;                          // row[n] of the CSR built
+    int pad;
+};
+static_assert(sizeof(SpReport) <= PG_REPORT_BYTES, "report exceeds the pinned buffer");
+
+__global__ __launch_bounds__(256) void sp_report(const unsigned long long *__restrict__ seg_cnt, int nseg,
+                                                 const unsigned long long *__restrict__ counters,
+                                                 const SelCtl *__restrict__ ctl, const double *__restrict__ tstar,
+                                                 const int *__restrict__ rown, SpReport *__restrict__ out)
+{
+    __shared__ unsigned long long part[256];
+    unsigned long long m = 0;
+    for (int g = threadIdx.x; g < nseg; g += 256) m = seg_cnt[g] > m ? seg_cnt[g] : m;
+    part[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o && part[threadIdx.x + o] > part[threadIdx.x]) part[threadIdx.x] = part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 16) out->counters[threadIdx.x] = counters[threadIdx.x];
+    if (threadIdx.x == 0) {
+        out->seg_max = part[0];
+        out->ctl = *ctl;
+        out->tstar = *tstar;
+        out->rn = rown ? *rown : -1;
+    }
+}
+
 struct CorrRun {
     pg_ctx *ctx = nullptr;
     hipStream_t st = nullptr;
@@ -2510,7 +2549,14 @@ struct CorrRun {
         supertiles(nr, nc, rt, sup, stl);
         int2 *dstl;
         PG_TRY(pg_bufT(ctx, "corr.stl", stl.size() + 1, &dstl));
-        PG_HIP(ctx, hipMemcpyAsync(dstl, stl.data(), stl.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+        // the list depends on the geometry only: copied again only when it or
+        // its buffer changed
+        const long long key[5] = {nr, nc, rt, sup, (long long)stl.size()};
+        if (ctx->stl_dev != (void *)dstl || std::memcmp(ctx->stl_key, key, sizeof key) != 0) {
+            PG_HIP(ctx, hipMemcpyAsync(dstl, stl.data(), stl.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+            std::memcpy(ctx->stl_key, key, sizeof key);
+            ctx->stl_dev = (void *)dstl;
+        }
         nst = (long long)stl.size();
         nwg = (long long)ctx->num_cu * PG_GRAM_WG_PER_CU;
         nwg = std::max<long long>(NXCD, nwg / NXCD * NXCD);
@@ -2984,15 +3030,20 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             PG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
             PG_TRY(R.launch_csr(es, 2 * kwant));
             PG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
-            SelCtl hc;
-            int rn = -1;
-            PG_HIP(ctx, hipMemcpyAsync(R.hseg.data(), R.seg_cnt, nwg * 8, hipMemcpyDeviceToHost, st));
-            PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof R.hcnt, hipMemcpyDeviceToHost, st));
-            PG_HIP(ctx, hipMemcpyAsync(&hc, ctl, sizeof hc, hipMemcpyDeviceToHost, st));
-            PG_HIP(ctx, hipMemcpyAsync(&inf.t_star, tstar_d, 8, hipMemcpyDeviceToHost, st));
-            if (n > 0) PG_HIP(ctx, hipMemcpyAsync(&rn, R.row + n, 4, hipMemcpyDeviceToHost, st));
+            // every check's inputs gathered on the device, one small copy back
+            SpReport *rep_d;
+            PG_TRY(pg_bufT(ctx, "corr.report", 1, &rep_d));
+            sp_report<<<1, 256, 0, st>>>(R.seg_cnt, (int)nwg, counters, ctl, tstar_d, n > 0 ? R.row + n : nullptr,
+                                         rep_d);
+            PG_KERNEL_CHECK(ctx);
+            PG_HIP(ctx, hipMemcpyAsync(ctx->h_report, rep_d, sizeof(SpReport), hipMemcpyDeviceToHost, st));
             PG_HIP(ctx, hipStreamSynchronize(st));
-            const long long mx = R.max_segment();
+            const SpReport &rep = *(const SpReport *)ctx->h_report;
+            std::memcpy(hcnt, rep.counters, sizeof R.hcnt);
+            const SelCtl hc = rep.ctl;
+            const int rn = rep.rn;
+            inf.t_star = rep.tstar;
+            const long long mx = (long long)rep.seg_max;
             done = hc.ok == 1 && mx <= R.seg_cap && (long long)hcnt[0] <= ccap && ((int *)hcnt)[8] == 0 &&
                    (long long)hcnt[3] + (long long)hcnt[1] == kwant && (long long)rn == 2 * kwant;
             if (done) {

@@ -3,6 +3,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <string>
 
 pg_status pg_fail(pg_ctx *ctx, pg_status st, const char *fmt, ...)
 {
@@ -108,12 +109,20 @@ pg_status pg_create(pg_ctx **out, int device)
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         ctx->num_cu = prop.multiProcessorCount;
         ctx->lds_per_block = prop.sharedMemPerBlock;
+        const std::string a = prop.gcnArchName;
+        ctx->arch = a.substr(0, a.find(':'));
     }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return PG_EDEVICE;
     }
     for (auto &e : ctx->ev) (void)hipEventCreate(&e);
+    if (hipHostMalloc(&ctx->h_report, PG_REPORT_BYTES, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->h_report = nullptr;
+        pg_destroy(ctx);
+        return PG_ENOMEM;
+    }
     // the allocation-free entry points' scratch address on this device
     if (pg_static_scratch(ctx, &ctx->static_scratch) != PG_OK) {
         pg_destroy(ctx);
@@ -133,6 +142,7 @@ void pg_destroy(pg_ctx *ctx)
         if (kv.second.p) (void)hipFree(kv.second.p);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->h_report) (void)hipHostFree(ctx->h_report);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -32,6 +32,12 @@ struct pg_ctx {
     // launch shapes forced by the autotuner while it times candidates (0: none)
     int force_gram_super = 0, force_bc_batches = 0, force_fw_block = 0;
     bool tuning = false;             // inside a search: no nested search
+    std::string arch;                // gcnArchName without features, read once
+    void *h_report = nullptr;        // pinned host buffer for small result read-backs
+    // the Gram supertile list last copied to the "corr.stl" buffer and its key
+    // (row tiles, column tiles, tile aspect, supertile edge, length)
+    void *stl_dev = nullptr;
+    long long stl_key[5] = {-1, -1, -1, -1, -1};
 
     // result of the last pg_corr_build
     int32_t cn = 0, ct = 0;
@@ -49,6 +55,9 @@ struct pg_ctx {
 };
 
 pg_status pg_fail(pg_ctx *ctx, pg_status st, const char *fmt, ...);
+
+// size of pg_ctx::h_report
+constexpr size_t PG_REPORT_BYTES = 4096;
 
 #define PG_HIP(ctx, expr)                                                                  \
     do {                                                                                   \

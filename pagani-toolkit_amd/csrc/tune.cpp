@@ -79,6 +79,7 @@ void mkdirs(const std::string &path)
 
 std::string pg_arch(pg_ctx *ctx)
 {
+    if (!ctx->arch.empty()) return ctx->arch;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) {
         (void)hipGetLastError();

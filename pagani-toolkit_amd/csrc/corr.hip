@@ -113,6 +113,27 @@ __host__ __device__ __forceinline__ long long g2_index(int i, int l, int npad)
 }
 
 
+// Zero fill of one or two buffers of 32-bit words (grid over the larger).
+__global__ void fill_u32(uint32_t *__restrict__ p, long long count, uint32_t *__restrict__ q = nullptr,
+                         long long qcount = 0)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) p[k] = 0u;
+    if (k < qcount) q[k] = 0u;
+}
+
+// Clears on the stream as one fill kernel: hipMemsetAsync of a small buffer
+// goes through the runtime's blit kernels, several microseconds of dispatch
+// each.  Sizes in bytes, multiples of 4.
+inline pg_status clear_dev(pg_ctx *ctx, hipStream_t st, void *p, size_t bytes, void *q = nullptr, size_t qbytes = 0)
+{
+    const long long c = (long long)(bytes / 4), d = q ? (long long)(qbytes / 4) : 0;
+    const long long m = std::max(c, d);
+    if (m > 0) fill_u32<<<(unsigned)pg_cdiv(m, 256), 256, 0, st>>>((uint32_t *)p, c, (uint32_t *)q, d);
+    PG_KERNEL_CHECK(ctx);
+    return PG_OK;
+}
+
 // ---------------------------------------------------------------- z-score
 // U CorMat.cu:34-53: mean and sqrt(sum (x-mu)^2) in FP64 with each product
 // rounded (no FMA), z = (x-mu)/norm.  Zero-variance voxels get z = 0 (the
@@ -180,7 +201,9 @@ __global__ void zscore_write(const float *__restrict__ x, int layout, int n, int
 
 // Fused form for T up to ZS_TMAX: a block stages the whole T x ZV tile of its
 // voxels in LDS (coalesced), ZV threads form the sequential FP64 sums from
-// LDS, then all threads write z64 and the panels.
+// LDS, then all threads write z64 and the panels.  With panels the grid
+// covers the npad padded rows (npad a multiple of ZV) and the blocks write
+// the padding zeros too.
 constexpr int ZS_TMAX = 360;
 
 constexpr int ZV = 32;  // voxels per block of zscore_fused (LDS [t][ZV+1] floats)
@@ -217,14 +240,22 @@ __global__ __launch_bounds__(256) void zscore_fused(const float *__restrict__ x,
         ssd[tid] = sqrt(s2);
     }
     __syncthreads();
-    for (int e = tid; e < ZV * t; e += 256) {
-        const int ii = e / t, l = e % t;
+    // with panels, every element of rows [0, npad) x [0, ldp) is written
+    // (zeros outside [0, n) x [0, t)), so the panels need no clearing first
+    const bool panels = p32 || phi;
+    const int w = panels ? ldp : t, rows = panels ? npad : n;
+    for (int e = tid; e < ZV * w; e += 256) {
+        const int ii = e / w, l = e % w;
         const int i = i0 + ii;
-        if (i >= n) continue;
-        const double sdv = ssd[ii];
-        const double z = sdv > 0.0 ? ((double)zt[l * LD + ii] - smu[ii]) / sdv : 0.0;
-        z64[(long long)i * t + l] = z;
-        const float zf = (float)z;
+        if (i >= rows) continue;
+        const bool in = i < n && l < t;
+        float zf = 0.0f;
+        if (in) {
+            const double sdv = ssd[ii];
+            const double z = sdv > 0.0 ? ((double)zt[l * LD + ii] - smu[ii]) / sdv : 0.0;
+            z64[(long long)i * t + l] = z;
+            zf = (float)z;
+        }
         if (p32) p32[(long long)i * ldp + l] = zf;
         if (phi) {
             const long long q = g2_index(i, l, npad);
@@ -756,10 +787,14 @@ __device__ __forceinline__ void g2_stage(const f32x16 (&acc)[g2::BMW][g2::BNW], 
 // before it, so stores issued in the epilogue would hold the next tile's
 // first wait for their full latency, while these have a whole step of MFMAs
 // to complete before the next wait.
+// Without a band (SPLIT false) the hits are staged as 8-byte records by
+// g2_emit_ballot: r~ bits in the low word, the pair relative to the wave's
+// block origin (i0, j0) as (i - i0) << 6 | (j - j0) in the high word.
 struct G2Pending {
     unsigned te, tc;       // edges, then rechecks, staged
     unsigned be0, bc0;     // their positions in the segment and the recheck list
-    unsigned spair, sval;  // LDS byte addresses of the staged pairs and r~
+    unsigned spair, sval;  // LDS byte addresses of the staged pairs and r~ (records: spair)
+    int i0, j0;            // records: the block origin
     bool on;
 };
 
@@ -770,6 +805,19 @@ __device__ __forceinline__ void g2_flush(const GramArgs &a, const G2Pending &pd,
 {
     float *segr = SPLIT ? nullptr : a.seg_r + (long long)blockIdx.x * a.seg_cap;
     const unsigned long long cap = (unsigned long long)a.seg_cap, bcap = (unsigned long long)a.band_cap;
+    if (!SPLIT) {
+        for (unsigned k = lane; k < pd.te; k += 64) {
+            unsigned long long rec;
+            asm volatile("ds_read_b64 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(rec) : "v"(pd.spair + k * 8u) : "memory");
+            const unsigned idx = (unsigned)(rec >> 32);
+            const unsigned long long p = (unsigned long long)pd.be0 + k;
+            if (p < cap) {
+                myseg[p] = make_int2(pd.i0 + (int)(idx >> 6), pd.j0 + (int)(idx & 63u));
+                segr[p] = __uint_as_float((unsigned)rec);
+            }
+        }
+        return;
+    }
     for (unsigned k = lane; k < pd.te + pd.tc; k += 64) {
         unsigned long long ij;
         float v;
@@ -844,8 +892,9 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
     // r~ of every segment entry is kept exactly when there is no band (the
     // sparsity single pass)
     float *segr = SPLIT ? nullptr : a.seg_r + (long long)blockIdx.x * a.seg_cap;
-    if (tt > (unsigned)g2::STAGE_HITS) {
-        // more hits than the staging area holds: each lane writes its own
+    if (!SPLIT || tt > (unsigned)g2::STAGE_HITS) {
+        // more hits than the staging area holds (or the band-free form, whose
+        // staging is g2_emit_ballot's): each lane writes its own
         const bool fits = (unsigned long long)be0 + te <= (unsigned long long)a.seg_cap &&
                           (unsigned long long)bc0 + (tt - te) <= (unsigned long long)a.band_cap;
         const unsigned long long bl = be0 + (ex & 0xFFFFu), cl = bc0 + (ex >> 16);
@@ -862,6 +911,80 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
     pd.bc0 = bc0;
     pd.spair = spair;
     pd.sval = sval;
+    pd.on = true;
+}
+
+// Epilogue without a band (every hit is a segment entry with its r~), in one
+// pass: per accumulator element, the wave's ballot of hits; a lane with a hit
+// stages it at the wave's running count plus the hits of lower lanes (mbcnt),
+// the count advancing by the ballot's popcount (scalar).  One compare per
+// element when no lane hits, no counting pass, and one 8-byte LDS write per
+// element with hits (the record of G2Pending).  The staging order differs from
+// g2_emit's (element-major), which the segment does not care about.  More than
+// STAGE_RECS hits: the staged ones are dropped and g2_emit writes them all
+// directly.
+template <bool BOUNDED>
+__device__ __forceinline__ void g2_emit_ballot(const GramArgs &a, float lower, const f32x16 (&acc)[g2::BMW][g2::BNW],
+                                               int i0, int j0, int lane, int2 *myseg, unsigned long long *s_seg,
+                                               char *stage, G2Pending &pd)
+{
+    constexpr unsigned STAGE_RECS = g2::STAGE / 4 / 8;  // 768 records per wave
+    const int ib0 = i0 + (lane >> 5) * 4, jb0 = j0 + (lane & 31);
+    // the lane's part of the record index: row (lane / 32) * 4, column lane % 32
+    const unsigned lidx = ((unsigned)(lane >> 5) << 8) | (unsigned)(lane & 31);
+    const unsigned srec = g2_lds_addr(stage);
+    unsigned nst = 0;  // hits so far (wave-uniform); those past STAGE_RECS are not staged
+#pragma unroll
+    for (int bm = 0; bm < g2::BMW; bm++)
+#pragma unroll
+        for (int bn = 0; bn < g2::BNW; bn++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const float v = acc[bm][bn][q];
+                bool hit = v >= lower;
+                if (!BOUNDED) {
+                    const int i = ib0 + bm * 32 + (q >> 2) * 8 + (q & 3), j = jb0 + bn * 32;
+                    hit = hit & (j < a.n) & (i < j);
+                }
+                const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
+                if (m != 0ull) {
+                    if (hit) {
+                        const unsigned p = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((unsigned)m, nst));
+                        // the element's part of the index, materialised inside
+                        // the branch (else every element's is formed up front);
+                        // the lane's and the element's bits are disjoint
+                        unsigned ec = (unsigned)(((bm * 32 + (q >> 2) * 8 + (q & 3)) << 6) | (bn * 32));
+                        asm volatile("" : "+s"(ec));
+                        const unsigned long long rec =
+                            ((unsigned long long)(lidx | ec) << 32) | (unsigned long long)__float_as_uint(v);
+                        if (p < STAGE_RECS)
+                            asm volatile("ds_write_b64 %0, %1" ::"v"(srec + p * 8u), "v"(rec) : "memory");
+                    }
+                    nst += (unsigned)__popcll(m);
+                }
+            }
+    if (nst > STAGE_RECS) {
+        // opaque copies: the direct path recomputes its predicates instead of
+        // the compiler keeping this pass's 128 masks live
+        float lo2 = lower;
+        int i02 = i0, j02 = j0;
+        asm volatile("" : "+v"(lo2), "+v"(i02), "+v"(j02));
+        g2_emit<BOUNDED, false>(a, lo2, acc, i02, j02, lane, myseg, s_seg, stage, pd);
+        return;
+    }
+    if (nst == 0) return;
+    unsigned long long be = 0;
+    if (lane == 0) be = g2_lds_add(s_seg, (unsigned long long)nst);
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    pd.te = nst;
+    pd.tc = 0;
+    pd.be0 = (unsigned)__shfl(be, 0);
+    pd.bc0 = 0;
+    pd.spair = srec;
+    pd.sval = 0;
+    pd.i0 = i0;
+    pd.j0 = j0;
     pd.on = true;
 }
 
@@ -896,8 +1019,8 @@ __device__ __forceinline__ void g2_tile_epilogue(const GramArgs &a, const f32x16
         if (interior) g2_emit<true, true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
         else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
     } else {
-        if (interior) g2_emit<true, false>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
-        else g2_emit<false, false>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
+        if (interior) g2_emit_ballot<true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
+        else g2_emit_ballot<false>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
     }
 }
 
@@ -1122,6 +1245,87 @@ __global__ void accept_kernel(const int2 *__restrict__ pairs, const float *__res
     }
 }
 
+// The same FP64 r, RCK_P pairs per wave: the wave walks its pairs' rows
+// RCK_T time points at a time; in load m a lane reads element
+// lane % RCK_T of pair m * (64 / RCK_T) + lane / RCK_T, so every load covers
+// whole RCK_T * 8-byte runs of a few rows (a thread-per-pair loop touches 64
+// rows per load) and a pair's t points take t / RCK_T dependent rounds.  The
+// rounded products go through LDS and lane p < RCK_P adds pair p's in time
+// order: the same products and the same sequential sum as r64_pair.  Row
+// offsets i * t in 32 bits (n * t < 2^32, checked by the caller).  The loop
+// bounds are uniform over the block.
+constexpr int RCK_P = 64, RCK_T = 1024 / RCK_P, RCK_M = 16;
+__global__ __launch_bounds__(256) void recheck_rows(const int2 *__restrict__ pairs,
+                                                    const unsigned long long *__restrict__ cnt, long long cap,
+                                                    const double *__restrict__ z, int t, double *__restrict__ r_out,
+                                                    const float *__restrict__ rfast,
+                                                    unsigned long long *__restrict__ errmax)
+{
+#pragma clang fp contract(off)
+    __shared__ double prod[4][RCK_P][RCK_T + 1];
+    constexpr int PPL = 64 / RCK_T;  // pairs per load
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lt = lane % RCK_T, lp = lane / RCK_T;
+    const long long c = (long long)min((unsigned long long)cap, *cnt);
+    double err = 0.0;
+    for (long long b0 = (long long)blockIdx.x * 4 * RCK_P; b0 < c; b0 += (long long)gridDim.x * 4 * RCK_P) {
+        const long long q = b0 + w * RCK_P + lane;
+        const int2 e = lane < RCK_P && q < c ? pairs[q] : make_int2(0, 0);
+        // row offsets of the pairs this lane loads for
+        unsigned oa[RCK_M], ob[RCK_M];
+#pragma unroll
+        for (int m = 0; m < RCK_M; m++) {
+            const int src = m * PPL + lp;
+            oa[m] = (unsigned)__shfl(e.x, src) * (unsigned)t;
+            ob[m] = (unsigned)__shfl(e.y, src) * (unsigned)t;
+        }
+        double sum = 0.0;
+        for (int l0 = 0; l0 < t; l0 += RCK_T) {
+            const int l = l0 + lt;
+            const bool in = l < t;
+            double x[RCK_M], y[RCK_M];
+#pragma unroll
+            for (int m = 0; m < RCK_M; m++) {
+                x[m] = in ? z[oa[m] + (unsigned)l] : 0.0;
+                y[m] = in ? z[ob[m] + (unsigned)l] : 0.0;
+            }
+#pragma unroll
+            for (int m = 0; m < RCK_M; m++) prod[w][m * PPL + lp][lt] = x[m] * y[m];
+            __syncthreads();
+            if (lane < RCK_P) {
+                const int kt = min(RCK_T, t - l0);
+                for (int k = 0; k < kt; k++) {
+                    const double pk = prod[w][lane][k];
+                    sum += pk;
+                }
+            }
+            __syncthreads();
+        }
+        if (lane < RCK_P && q < c) {
+            const double r = sum > 1.0 ? 1.0 : (sum < -1.0 ? -1.0 : sum);
+            r_out[q] = r;
+            if (rfast) err = fmax(err, fabs((double)rfast[q] - r));
+        }
+    }
+    if (!rfast) return;
+    for (int o = 32; o > 0; o >>= 1) err = fmax(err, __shfl_xor(err, o));
+    if (lane == 0 && err > 0.0) atomicMax(errmax, (unsigned long long)__double_as_longlong(err));
+}
+
+// FP64 r of the first min(cap, *cnt) listed pairs (recheck_rows when the row
+// offsets fit 32 bits; grid of at most 8 work-groups per CU, bound by cap).
+inline void launch_recheck(const pg_ctx *ctx, hipStream_t st, const int2 *pairs, const unsigned long long *cnt,
+                           long long cap, const double *z, int n, int t, double *r_out, const float *rfast,
+                           unsigned long long *errmax)
+{
+    if (cap <= 0) return;
+    if ((unsigned long long)n * (unsigned long long)t < (1ull << 32)) {
+        const long long g = std::min<long long>(pg_cdiv(cap, 4 * RCK_P), 8LL * std::max(ctx->num_cu, 1));
+        recheck_rows<<<(unsigned)g, 256, 0, st>>>(pairs, cnt, cap, z, t, r_out, rfast, errmax);
+    } else {
+        recheck_kernel<<<(unsigned)pg_cdiv(cap, 256), 256, 0, st>>>(pairs, cnt, cap, z, t, r_out, rfast, errmax);
+    }
+}
+
 // ------------------------------------------------------------- CSR build
 
 struct EdgeSrc {
@@ -1142,11 +1346,6 @@ __device__ __forceinline__ float src_bound(const EdgeSrc &s)
     return s.keep_above_dev ? *s.keep_above_dev : s.keep_above;
 }
 
-__device__ __forceinline__ bool src_keep(const EdgeSrc &s, int g, long long k, float ka)
-{
-    return !(s.seg_r && g < s.nseg && s.seg_r[(long long)g * s.seg_cap + k] <= ka);
-}
-
 // entries of source g, clamped to what its buffer holds
 __device__ __forceinline__ long long src_count(const EdgeSrc &s, int g)
 {
@@ -1158,7 +1357,7 @@ __device__ __forceinline__ int2 src_edge(const EdgeSrc &s, int g, long long k)
 }
 
 // Two-level counting sort into CSR.  Both directions of every kept edge go to
-// the bucket of 128 rows holding their source row: per-segment bucket counts
+// the bucket of BROWS rows holding their source row: per-segment bucket counts
 // in LDS, a 2-D exclusive scan (segment-major inside each bucket) that fixes
 // every segment's output range, a scatter of packed 32-bit entries
 // (row-in-bucket << 25 | column), then one work-group per bucket derives the
@@ -1170,8 +1369,58 @@ constexpr int BSH = PG_CSR_BSH;  // log2 of the rows per bucket
 constexpr int BROWS = 1 << BSH;
 constexpr unsigned COLMASK = (1u << 25) - 1u;
 
-constexpr int SEG_THREADS = 1024;  // one work-group per segment: 16 waves in flight
-constexpr int SEG_UNROLL = 4;      // segment entries per thread in flight
+#ifndef PG_SEG_UNROLL
+#define PG_SEG_UNROLL 8
+#endif
+#ifndef PG_SEG_PIPE
+#define PG_SEG_PIPE 1
+#endif
+constexpr int SEG_THREADS = 1024;          // one work-group per segment: 16 waves in flight
+constexpr int SEG_UNROLL = PG_SEG_UNROLL;  // segment entries per thread per batch
+
+// One thread's batch of source g: entries k0 + u * SEG_THREADS, their r~ when
+// the source is filtered (else NaN, which every keep test passes).
+struct SegBatch {
+    int2 e[SEG_UNROLL];
+    float r[SEG_UNROLL];
+    bool in[SEG_UNROLL];
+    __device__ __forceinline__ void fetch(const EdgeSrc &s, int g, long long k0, long long c)
+    {
+        const bool filt = s.seg_r && g < s.nseg;
+#pragma unroll
+        for (int u = 0; u < SEG_UNROLL; u++) {
+            const long long k = k0 + (long long)u * SEG_THREADS;
+            in[u] = k < c;
+            e[u] = in[u] ? src_edge(s, g, k) : make_int2(0, 0);
+            r[u] = in[u] && filt ? s.seg_r[(long long)g * s.seg_cap + k] : __int_as_float(0x7fc00000);
+        }
+    }
+    __device__ __forceinline__ bool keep(int u, float ka) const { return in[u] && !(r[u] <= ka); }
+};
+
+// f(batch) over this thread's batches of source g; pipelined: the next
+// batch's loads are issued before f works on the current one.
+template <class F>
+__device__ __forceinline__ void seg_batches(const EdgeSrc &s, int g, long long c, F &&f)
+{
+    const long long step = (long long)SEG_THREADS * SEG_UNROLL;
+#if PG_SEG_PIPE
+    SegBatch a;
+    a.fetch(s, g, threadIdx.x, c);
+    for (long long k0 = threadIdx.x; k0 < c; k0 += step) {
+        SegBatch b;
+        b.fetch(s, g, k0 + step, c);
+        f(a);
+        a = b;
+    }
+#else
+    for (long long k0 = threadIdx.x; k0 < c; k0 += step) {
+        SegBatch a;
+        a.fetch(s, g, k0, c);
+        f(a);
+    }
+#endif
+}
 
 __global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, unsigned *__restrict__ cnt)
 {
@@ -1181,24 +1430,14 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, 
     const int g = blockIdx.x;
     const long long c = src_count(s, g);
     const float ka = src_bound(s);
-    // SEG_UNROLL entries per thread per pass: every load is issued before the
-    // first LDS atomic that needs one
-    for (long long k0 = threadIdx.x; k0 < c; k0 += (long long)SEG_THREADS * SEG_UNROLL) {
-        int2 e[SEG_UNROLL];
-        bool keep[SEG_UNROLL];
-#pragma unroll
-        for (int u = 0; u < SEG_UNROLL; u++) {
-            const long long k = k0 + (long long)u * SEG_THREADS;
-            keep[u] = k < c && src_keep(s, g, k, ka);
-            e[u] = k < c ? src_edge(s, g, k) : make_int2(0, 0);
-        }
+    seg_batches(s, g, c, [&](const SegBatch &a) {
 #pragma unroll
         for (int u = 0; u < SEG_UNROLL; u++)
-            if (keep[u]) {
-                atomicAdd(&sc[e[u].x >> BSH], 1u);
-                atomicAdd(&sc[e[u].y >> BSH], 1u);
+            if (a.keep(u, ka)) {
+                atomicAdd(&sc[a.e[u].x >> BSH], 1u);
+                atomicAdd(&sc[a.e[u].y >> BSH], 1u);
             }
-    }
+    });
     __syncthreads();
     for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) cnt[(long long)b * gridDim.x + g] = sc[b];
 }
@@ -1279,30 +1518,23 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_scatter(EdgeSrc s, int nbk
     const long long c = src_count(s, g);
     const float ka = src_bound(s);
     if (bstart[nbk] > cap) return;
-    for (long long k0 = threadIdx.x; k0 < c; k0 += (long long)SEG_THREADS * SEG_UNROLL) {
-        int2 e[SEG_UNROLL];
-        bool keep[SEG_UNROLL];
+    seg_batches(s, g, c, [&](const SegBatch &a) {
 #pragma unroll
         for (int u = 0; u < SEG_UNROLL; u++) {
-            const long long k = k0 + (long long)u * SEG_THREADS;
-            keep[u] = k < c && src_keep(s, g, k, ka);
-            e[u] = k < c ? src_edge(s, g, k) : make_int2(0, 0);
-        }
-#pragma unroll
-        for (int u = 0; u < SEG_UNROLL; u++) {
-            if (!keep[u]) continue;
-            const int bf = e[u].x >> BSH, br = e[u].y >> BSH;
+            if (!a.keep(u, ka)) continue;
+            const int2 e = a.e[u];
+            const int bf = e.x >> BSH, br = e.y >> BSH;
             const unsigned pf = sbase[bf] + atomicAdd(&sc[bf], 1u);
             const unsigned pr = sbase[br] + atomicAdd(&sc[br], 1u);
-            ent[pf] = ((unsigned)(e[u].x & (BROWS - 1)) << 25) | (unsigned)e[u].y;
-            ent[pr] = ((unsigned)(e[u].y & (BROWS - 1)) << 25) | (unsigned)e[u].x;
+            ent[pf] = ((unsigned)(e.x & (BROWS - 1)) << 25) | (unsigned)e.y;
+            ent[pr] = ((unsigned)(e.y & (BROWS - 1)) << 25) | (unsigned)e.x;
             if (W) {
-                const float r = (float)r64_pair(z, t, e[u].x, e[u].y);
+                const float r = (float)r64_pair(z, t, e.x, e.y);
                 entv[pf] = r;
                 entv[pr] = r;
             }
         }
-    }
+    });
 }
 
 // Bitonic sort of one row held lane-major in a wave (element e = lane*P + r),
@@ -2061,7 +2293,7 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
     __shared__ int s_bin;
     __shared__ long long s_rank;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    constexpr int RB = 8;
+    constexpr int RB = 16;
     if (tid == 0) {
         s_def = 0;
         s_m = 0;
@@ -2232,8 +2464,16 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
         const unsigned long long k2q = ((unsigned long long)(unsigned)e.x << 32) | (unsigned)e.y;
         return kq < k1 || (kq == k1 && k2q <= k2);
     };
-    unsigned long long fl = 0, nsel = 0;
-    for (long long q0 = tid; q0 < c; q0 += RB * 1024) {
+    // one pass: per batch, each wave reserves its selected entries' slots
+    // with one LDS atomic and writes them (the list's order is free: the
+    // CSR build sorts every row), then one global reservation of the total
+    if (tid == 0) s_m = 0;
+    __syncthreads();
+    const unsigned long long base0 = *extra_cnt;
+    unsigned long long fl = 0;
+    // trip count uniform over the block (the scan below spans whole waves)
+    for (long long qb = 0; qb < c; qb += RB * 1024) {
+        const long long q0 = qb + tid;
         double rr[RB];
         float rf[RB];
         int2 ee[RB];
@@ -2244,43 +2484,32 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
             rf[u] = q < c ? rfast[q] : 0.0f;
             ee[u] = q < c ? cand[q] : make_int2(0, 0);
         }
+        unsigned sel = 0, ns = 0;
 #pragma unroll
         for (int u = 0; u < RB; u++)
             if (q0 + u * 1024 < c) {
-                nsel += selected(q0 + u * 1024, ee[u], rr[u]);
+                const unsigned b = selected(q0 + u * 1024, ee[u], rr[u]) ? 1u : 0u;
+                sel |= b << u;
+                ns += b;
                 fl += ((double)rf[u] > ts) != (rr[u] > ts);
             }
-    }
-    unsigned long long incl = nsel;
+        unsigned incl = ns;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned long long y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-    }
-    if (lane == 63) wtot[wv] = incl;
-    __syncthreads();
-    unsigned long long base = 0, all = 0;
-    for (int w = 0; w < 16; w++) {
-        base += w < wv ? wtot[w] : 0ull;
-        all += wtot[w];
-    }
-    if (tid == 0) sh[0] = atomicAdd(extra_cnt, all);
-    __syncthreads();
-    unsigned long long p = sh[0] + base + incl - nsel;
-    if (nsel)
-        for (long long q0 = tid; q0 < c; q0 += RB * 1024) {
-            double rr[RB];
-            int2 ee[RB];
-#pragma unroll
-            for (int u = 0; u < RB; u++) {
-                const long long q = q0 + u * 1024;
-                rr[u] = q < c ? r64[q] : 0.0;
-                ee[u] = q < c ? cand[q] : make_int2(0, 0);
-            }
-#pragma unroll
-            for (int u = 0; u < RB; u++)
-                if (q0 + u * 1024 < c && selected(q0 + u * 1024, ee[u], rr[u])) extra[p++] = ee[u];
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
         }
+        const unsigned wall = __shfl(incl, 63);
+        unsigned wb = 0;
+        if (lane == 63 && wall) wb = atomicAdd(&s_m, wall);
+        wb = __shfl(wb, 63);
+        unsigned long long p = base0 + wb + incl - ns;
+#pragma unroll
+        for (int u = 0; u < RB; u++)
+            if ((sel >> u) & 1u) extra[p++] = ee[u];
+    }
+    __syncthreads();
+    if (tid == 0) atomicAdd(extra_cnt, (unsigned long long)s_m);
     atomicAdd(flips, fl);
 }
 
@@ -2509,24 +2738,29 @@ struct CorrRun {
         PG_TRY(pg_bufT(ctx, "corr.mu", n + 1, &mu));
         PG_TRY(pg_bufT(ctx, "corr.sd", n + 1, &sd));
         PG_TRY(pg_bufT(ctx, "corr.z64", (size_t)n * nt + 1, &z64));
+        // the fused z-score writes the panels' padding; the two-kernel form
+        // (long series) does not, so the panels are cleared first
+        const bool fused = n > 0 && t <= ZS_TMAX;
         if (prec == PREC_F32) {
             PG_TRY(pg_buf(ctx, "corr.p32", (size_t)npad * ldp * 4, &p0));
-            PG_HIP(ctx, hipMemsetAsync(p0, 0, (size_t)npad * ldp * 4, st));
+            if (!fused) PG_HIP(ctx, hipMemsetAsync(p0, 0, (size_t)npad * ldp * 4, st));
         } else {
             PG_TRY(pg_buf(ctx, "corr.phi", (size_t)npad * ldp * 2, &p0));
             PG_TRY(pg_buf(ctx, "corr.plo", (size_t)npad * ldp * 2, &p1));
-            PG_HIP(ctx, hipMemsetAsync(p0, 0, (size_t)npad * ldp * 2, st));
-            PG_HIP(ctx, hipMemsetAsync(p1, 0, (size_t)npad * ldp * 2, st));
+            if (!fused) {
+                PG_HIP(ctx, hipMemsetAsync(p0, 0, (size_t)npad * ldp * 2, st));
+                PG_HIP(ctx, hipMemsetAsync(p1, 0, (size_t)npad * ldp * 2, st));
+            }
         }
         const float *dbold = nullptr;
         PG_TRY(pg_in(ctx, bold, (size_t)n * nt, "corr.bold", &dbold));
 
         PG_HIP(ctx, hipEventRecord(ctx->ev[0], st));
-        if (n > 0 && t <= ZS_TMAX) {
+        if (fused) {
             const size_t shz = (size_t)t * (ZV + 1) * 4;
             PG_HIP(ctx, hipFuncSetAttribute((const void *)zscore_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
                                             (int)shz));
-            zscore_fused<<<(unsigned)pg_cdiv(n, ZV), 256, shz, st>>>(
+            zscore_fused<<<(unsigned)pg_cdiv(npad, ZV), 256, shz, st>>>(
                 dbold, opts->layout, n, t, z64, prec == PREC_F32 ? (float *)p0 : nullptr,
                 prec == PREC_BF16X3 ? (uint16_t *)p0 : nullptr, prec == PREC_BF16X3 ? (uint16_t *)p1 : nullptr, ldp,
                 npad);
@@ -2633,8 +2867,8 @@ struct CorrRun {
         else seg_r = nullptr;
         PG_TRY(pg_bufT(ctx, "corr.band", (size_t)band_cap, &bandl));
         PG_TRY(pg_bufT(ctx, "corr.bandr", (size_t)band_cap, &band_r));
-        PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
-        PG_HIP(ctx, hipMemsetAsync(seg_cnt, 0, nwg * sizeof(unsigned long long), st));
+        PG_TRY(clear_dev(ctx, st, counters, 16 * sizeof(unsigned long long), seg_cnt,
+                         nwg * sizeof(unsigned long long)));
         ga.seg = seg;
         ga.seg_r = seg_r;
         ga.seg_cap = seg_cap;
@@ -2828,7 +3062,7 @@ pg_status fine_hist(CorrRun &R, const SelCtl *ctl, unsigned long long **h2t_out)
     unsigned long long *h2t;
     PG_TRY(pg_bufT(ctx, "corr.h2", (size_t)hb * NBIN2, &h2));
     PG_TRY(pg_bufT(ctx, "corr.h2t", NBIN2, &h2t));
-    PG_HIP(ctx, hipMemsetAsync(h2t, 0, NBIN2 * sizeof(unsigned long long), st));
+    PG_TRY(clear_dev(ctx, st, h2t, NBIN2 * sizeof(unsigned long long)));
     emitted_hist<<<hb, 1024, 0, st>>>(R.seg_r, R.seg_cap, R.seg_cnt, (int)R.nwg, ctl, h2);
     hist_sum<<<dim3(NBIN2 / 256, 16), 256, 0, st>>>(h2, hb, NBIN2, h2t);
     PG_KERNEL_CHECK(ctx);
@@ -2856,15 +3090,15 @@ pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Windo
     PG_TRY(pg_bufT(ctx, "corr.candr", (size_t)ccap, &win->cand_r));
     PG_TRY(pg_bufT(ctx, "corr.cand64", (size_t)ccap, &win->cand64));
     PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)ccap, &win->extra));
-    PG_HIP(ctx, hipMemsetAsync(R.counters, 0, 16 * sizeof(unsigned long long), st));
+    PG_TRY(clear_dev(ctx, st, R.counters, 16 * sizeof(unsigned long long)));
     const unsigned wx = (unsigned)std::max<long long>(1, pg_cdiv(R.seg_cap, WIN_CHUNK));
     win->nd = (int)(wx * R.nwg);
     PG_TRY(pg_bufT(ctx, "corr.dpart", (size_t)win->nd, &win->dpart));
     window_kernel<<<dim3(wx, (unsigned)R.nwg), 256, 0, st>>>(R.seg, R.seg_r, R.seg_cap, R.seg_cnt, (int)R.nwg, ctl,
                                                               win->cand, win->cand_r, ccap, R.counters + 0,
                                                               win->dpart);
-    recheck_kernel<<<(unsigned)pg_cdiv(ccap, 256), 256, 0, st>>>(win->cand, R.counters + 0, ccap, R.z64, R.t,
-                                                                  win->cand64, win->cand_r, R.counters + 5);
+    launch_recheck(ctx, st, win->cand, R.counters + 0, ccap, R.z64, R.n, R.t, win->cand64, win->cand_r,
+                   R.counters + 5);
     PG_KERNEL_CHECK(ctx);
     return PG_OK;
 }
@@ -2960,8 +3194,8 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)std::max<long long>(nband, 1), &band64));
         PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)std::max<long long>(nband, 1), &extra));
         if (nband > 0) {
-            recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(R.bandl, counters + 0, R.band_cap, R.z64, t,
-                                                                          band64, R.band_r, counters + 5);
+            launch_recheck(ctx, st, R.bandl, counters + 0, std::min(nband, R.band_cap), R.z64, n, t, band64, R.band_r,
+                           counters + 5);
             accept_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(
                 R.bandl, R.band_r, band64, counters + 0, R.band_cap, (double)thr_eff, extra, extra_cnt, flips);
         }
@@ -2976,8 +3210,8 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
     } else if (kwant == 0 || n < 2) {
         // ---------------------------------------------------- nothing asked
         PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-        PG_HIP(ctx, hipMemsetAsync(counters, 0, 16 * sizeof(unsigned long long), st));
-        PG_HIP(ctx, hipMemsetAsync(R.seg_cnt, 0, nwg * sizeof(unsigned long long), st));
+        PG_TRY(clear_dev(ctx, st, counters, 16 * sizeof(unsigned long long), R.seg_cnt,
+                         nwg * sizeof(unsigned long long)));
         PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
         PG_TRY(pg_bufT(ctx, "corr.extra", 1, &extra));
         PG_TRY(pg_bufT(ctx, "corr.seg", 1, &R.seg));
@@ -3079,8 +3313,8 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)std::max<long long>(nband, 1), &band64));
             PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)std::max<long long>(nband, 1), &extra));
             if (nband > 0)
-                recheck_kernel<<<(unsigned)pg_cdiv(nband, 256), 256, 0, st>>>(R.bandl, counters + 0, R.band_cap, R.z64,
-                                                                              t, band64, R.band_r, counters + 5);
+                launch_recheck(ctx, st, R.bandl, counters + 0, std::min(nband, R.band_cap), R.z64, n, t, band64,
+                               R.band_r, counters + 5);
             select_kernel<<<1, 1024, 0, st>>>(R.bandl, band64, R.band_r, counters + 0, R.band_cap, kwant, R.seg_cnt,
                                               (int)nwg, extra, extra_cnt, tstar_d, flips, (int *)(counters + 4));
             PG_KERNEL_CHECK(ctx);
@@ -3256,7 +3490,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
             R.seg_cap = 64;
             PG_TRY(pg_bufT(ctx, "corr.seg", (size_t)(nwg * R.seg_cap), &R.seg));
             PG_TRY(pg_bufT(ctx, "corr.segr", (size_t)(nwg * R.seg_cap), &R.seg_r));
-            PG_HIP(ctx, hipMemsetAsync(R.seg_cnt, 0, nwg * sizeof(unsigned long long), st));
+            PG_TRY(clear_dev(ctx, st, R.seg_cnt, nwg * sizeof(unsigned long long)));
             hc.lower = 2.0f;
             hc.ok = 1;
             PG_HIP(ctx, hipMemcpyAsync(E.ctl, &hc, sizeof hc, hipMemcpyHostToDevice, st));
@@ -3352,7 +3586,7 @@ extern "C" pg_status pg_corr_select(pg_ctx *ctx, float thr, pg_corr_info *info)
         if (kwant == 0 || n < 2) {
             Window win;
             PG_TRY(pg_bufT(ctx, "corr.extra", 1, &win.extra));
-            PG_HIP(ctx, hipMemsetAsync(c, 0, 16 * sizeof(unsigned long long), st));
+            PG_TRY(clear_dev(ctx, st, c, 16 * sizeof(unsigned long long)));
             EdgeSrc es = R.edges(win.extra, c + 1, 1);
             es.seg_r = R.seg_r;
             es.keep_above = 2.0f;  // no r~ passes: no segment entry is kept
@@ -3419,12 +3653,6 @@ extern "C" pg_status pg_corr_select(pg_ctx *ctx, float thr, pg_corr_info *info)
 // ------------------------------------------- captured cohort: construction
 
 namespace {
-
-__global__ void fill_u32(uint32_t *__restrict__ p, long long count)
-{
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < count) p[k] = 0u;
-}
 
 // Every assumption of the single sparsity pass that pg_corr_build checks on
 // the host after it, checked on the device; a failed one sets a bit of
@@ -3622,11 +3850,9 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     const auto &S = P->subj[(size_t)s];
     const long long nwg = P->nwg;
     if (!S.graph) return PG_OK;
-    const long long panel = (long long)S.npad * S.ldp;  // bf16 elements per array
-    fill_u32<<<(unsigned)pg_cdiv(panel / 2, 256), 256, 0, st>>>((uint32_t *)P->phi, panel / 2);
-    fill_u32<<<(unsigned)pg_cdiv(panel / 2, 256), 256, 0, st>>>((uint32_t *)P->plo, panel / 2);
+    // the z-score writes the panels whole, padding included
     const size_t shz = (size_t)S.t * (ZV + 1) * 4;
-    zscore_fused<<<(unsigned)pg_cdiv(S.n, ZV), 256, shz, st>>>(dbold, PG_BOLD_TIME_MAJOR, S.n, S.t, P->z64, nullptr,
+    zscore_fused<<<(unsigned)pg_cdiv(S.npad, ZV), 256, shz, st>>>(dbold, PG_BOLD_TIME_MAJOR, S.n, S.t, P->z64, nullptr,
                                                                P->phi, P->plo, S.ldp, S.npad);
     GramArgs ga;
     std::memset(&ga, 0, sizeof ga);
@@ -3671,8 +3897,7 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     window_kernel<<<dim3((unsigned)S.wx, (unsigned)nwg), 256, 0, st>>>(P->seg, P->seg_r, S.seg_cap, P->seg_cnt,
                                                                        (int)nwg, P->ctl, P->cand, P->cand_r, S.ccap,
                                                                        c + 0, P->dpart);
-    recheck_kernel<<<(unsigned)pg_cdiv(S.ccap, 256), 256, 0, st>>>(P->cand, c + 0, S.ccap, P->z64, S.t, P->cand64,
-                                                                    P->cand_r, c + 5);
+    launch_recheck(ctx, st, P->cand, c + 0, S.ccap, P->z64, S.n, S.t, P->cand64, P->cand_r, c + 5);
     const int nd = (int)(S.wx * nwg);
     select_kernel<<<1, 1024, 0, st>>>(P->cand, P->cand64, P->cand_r, c + 0, S.ccap, S.kwant, P->dpart, nd, P->extra,
                                       c + 1, P->tstar, c + 2, (int *)(c + 4));

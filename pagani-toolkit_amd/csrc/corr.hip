@@ -101,6 +101,8 @@ static_assert(DMA_WAVE * 4 * 1024 == STAGE, "stage must split into 1 KB wave loa
 // hits of one wave's block staged in LDS before the global writes: a quarter
 // of the ring slot that is free during the epilogue, pairs then r~
 constexpr int STAGE_HITS = STAGE / 4 / 12;  // 512
+constexpr int RING_RECS = STAGE / 4 / 8;     // 768 8-byte records per wave in the free ring slot
+constexpr int DEDIC_RECS = 255;              // records per wave in the dedicated area
 }  // namespace g2
 
 __host__ __device__ __forceinline__ int g2_swz(int i) { return ((i >> 2) ^ (i >> 3)) & 1; }
@@ -923,12 +925,11 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
 // g2_emit's (element-major), which the segment does not care about.  More than
 // STAGE_RECS hits: the staged ones are dropped and g2_emit writes them all
 // directly.
-template <bool BOUNDED>
+template <bool BOUNDED, unsigned STAGE_RECS>
 __device__ __forceinline__ void g2_emit_ballot(const GramArgs &a, float lower, const f32x16 (&acc)[g2::BMW][g2::BNW],
                                                int i0, int j0, int lane, int2 *myseg, unsigned long long *s_seg,
                                                char *stage, G2Pending &pd)
 {
-    constexpr unsigned STAGE_RECS = g2::STAGE / 4 / 8;  // 768 records per wave
     const int ib0 = i0 + (lane >> 5) * 4, jb0 = j0 + (lane & 31);
     // the lane's part of the record index: row (lane / 32) * 4, column lane % 32
     const unsigned lidx = ((unsigned)(lane >> 5) << 8) | (unsigned)(lane & 31);
@@ -991,7 +992,7 @@ __device__ __forceinline__ void g2_emit_ballot(const GramArgs &a, float lower, c
 // Epilogue of one wave's block of tile (R, C): lane holds column lane%32 and
 // rows (q/4)*8 + (lane/32)*4 + q%4 of each 32 x 32 block.  Histogram mode bins
 // every pair i < j < n; emit mode writes the hits (g2_emit).
-template <int MODE>
+template <int MODE, bool DEDIC>
 __device__ __forceinline__ void g2_tile_epilogue(const GramArgs &a, const f32x16 (&acc)[g2::BMW][g2::BNW], int R,
                                                  int C, int wm, int wn, int lane, float lower, bool split,
                                                  int2 *myseg, unsigned long long *s_seg, uint32_t *s_hist,
@@ -1019,14 +1020,15 @@ __device__ __forceinline__ void g2_tile_epilogue(const GramArgs &a, const f32x16
         if (interior) g2_emit<true, true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
         else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
     } else {
-        if (interior) g2_emit_ballot<true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
-        else g2_emit_ballot<false>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
+        constexpr unsigned cap = DEDIC ? g2::DEDIC_RECS : g2::RING_RECS;
+        if (interior) g2_emit_ballot<true, cap>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
+        else g2_emit_ballot<false, cap>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
     }
 }
 
 // SPLIT (emit mode): hits above a.upper are edges, the others go to the
 // recheck list; without it every hit is an edge stored with its r~.
-template <int MODE, bool SPLIT = false>
+template <int MODE, bool SPLIT = false, bool DEDIC = false>
 __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
 {
     constexpr int TM = g2::TM, TN = g2::TN, WR = g2::WR, WC = g2::WC, BMW = g2::BMW, BNW = g2::BNW;
@@ -1035,6 +1037,10 @@ __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
     extern __shared__ __align__(16) char smem[];
     uint32_t *s_hist = (uint32_t *)(smem + NSTAGE * STAGE);
     __shared__ unsigned long long s_seg;
+    // DEDIC (band-free emit only): hits staged in an area of their own, so
+    // the epilogue needs no barrier for a ring slot to come free
+    constexpr bool dedic = DEDIC && !SPLIT;
+    __shared__ unsigned long long s_rec[4][g2::DEDIC_RECS];
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w >> 1, wn = w & 1;
@@ -1131,14 +1137,16 @@ __global__ __launch_bounds__(THREADS, 2) void gram2_kernel(GramArgs a)
         // the slot of the tile's last chunk is free once every wave has
         // multiplied it (the next tile's chunks 0 and 1 are in the other two)
         char *stage = nullptr;
-        if (MODE == MODE_EMIT) {
+        if (MODE == MODE_EMIT && dedic) {
+            stage = (char *)s_rec[w];
+        } else if (MODE == MODE_EMIT) {
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
             int fs = slot + NSTAGE - 1;
             fs -= fs >= NSTAGE ? NSTAGE : 0;
             stage = smem + fs * STAGE + w * (STAGE / 4);
         }
-        g2_tile_epilogue<MODE>(a, acc, R, C, wm, wn, lane, lower, split, myseg, &s_seg, s_hist, stage, pend);
+        g2_tile_epilogue<MODE, dedic>(a, acc, R, C, wm, wn, lane, lower, split, myseg, &s_seg, s_hist, stage, pend);
         R = R1;
         C = C1;
         have = have1;
@@ -1325,6 +1333,12 @@ inline void launch_recheck(const pg_ctx *ctx, hipStream_t st, const int2 *pairs,
         recheck_kernel<<<(unsigned)pg_cdiv(cap, 256), 256, 0, st>>>(pairs, cnt, cap, z, t, r_out, rfast, errmax);
     }
 }
+
+// Whether a band-free emit with hit fraction d stages in the dedicated area:
+// a wave's block holds 8192 pairs, so about 8192 d hits; at most 160 keeps
+// six standard deviations under the area's 255 records (a block beyond it
+// writes its hits directly, correct but slower).
+inline bool dedic_fits(double d) { return d * (g2::WR * g2::WC) <= 160.0; }
 
 // ------------------------------------------------------------- CSR build
 
@@ -2707,6 +2721,10 @@ struct CorrRun {
     int *row = nullptr, *col = nullptr;
     float *val = nullptr;
     bool ran_hist = false, ran_emit = false;
+    // band-free emits: stage hits in the dedicated LDS area (no epilogue
+    // barrier), chosen when a wave's 128 x 64 block is expected to hold well
+    // under its DEDIC_RECS records (else the free ring slot's 768)
+    bool dedic = false;
     long long gram_passes = 0;
     std::vector<unsigned long long> hseg;
     unsigned long long hcnt[16] = {};
@@ -2831,9 +2849,10 @@ struct CorrRun {
         size_t shm = stage + (mode == MODE_HIST ? NBIN * 4 : 0);
         const void *fn;
         if (g2path)
-            fn = mode == MODE_HIST ? (const void *)gram2_kernel<MODE_HIST>
-                                   : (ga.upper >= -1.0f ? (const void *)gram2_kernel<MODE_EMIT, true>
-                                                        : (const void *)gram2_kernel<MODE_EMIT, false>);
+            fn = mode == MODE_HIST        ? (const void *)gram2_kernel<MODE_HIST>
+                 : ga.upper >= -1.0f      ? (const void *)gram2_kernel<MODE_EMIT, true>
+                 : dedic                  ? (const void *)gram2_kernel<MODE_EMIT, false, true>
+                                          : (const void *)gram2_kernel<MODE_EMIT, false>;
         else
             fn = mode == MODE_HIST ? (const void *)gram_kernel<PREC_F32, MODE_HIST>
                                    : (const void *)gram_kernel<PREC_F32, MODE_EMIT>;
@@ -2843,6 +2862,7 @@ struct CorrRun {
         if (g2path) {
             if (mode == MODE_HIST) gram2_kernel<MODE_HIST><<<(unsigned)nwg_hist, THREADS, shm, st>>>(ga);
             else if (ga.upper >= -1.0f) gram2_kernel<MODE_EMIT, true><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
+            else if (dedic) gram2_kernel<MODE_EMIT, false, true><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
             else gram2_kernel<MODE_EMIT, false><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
         } else {
             if (mode == MODE_HIST) gram_kernel<PREC_F32, MODE_HIST><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
@@ -3247,7 +3267,9 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             R.ga.lower_dev = &ctl->lower;
             R.ga.upper = -4.0f;  // every hit is a segment entry
             const double expect = std::min(1.0, f * 1.2 + 2.0 * w) * (double)m1;
+            R.dedic = dedic_fits(expect / std::max<double>((double)m1, 1.0));
             PG_TRY(R.emit_launch(pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096, 1 << 10, true));
+            R.dedic = false;
             R.ga.lower_dev = nullptr;
             unsigned long long *h2t;
             PG_TRY(fine_hist(R, ctl, &h2t));
@@ -3394,6 +3416,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
         // segment sizes from a sampled histogram (about 2% of the supertiles)
         // when there are enough supertiles to sample, else a fixed guess
         long long seg0 = pg_cdiv(std::max<long long>((long long)(m1 * 0.03), 1 << 20), nwg) + 4096;
+        double dens = 1.0;  // expected hit fraction (unknown without the sampled pass)
         const long long stride = R.nst >= 8 ? std::max<long long>(2, std::min<long long>(50, R.nst / 16)) : 1;
         if (stride > 1 && n > 1) {
             std::vector<unsigned long long> h;
@@ -3406,11 +3429,14 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
             const double frac = msamp > 0 ? (double)above / (double)msamp : 1.0;
             const double expect = std::min(1.0, frac * 1.25 + 0.002) * (double)m1;
             seg0 = pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096;
+            dens = expect / std::max<double>((double)m1, 1.0);
         }
         R.ga.lower = E.lower;
         R.ga.upper = -4.0f;
         PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+        R.dedic = dedic_fits(dens);
         PG_TRY(R.emit_pass(seg0, 1 << 10, true));
+        R.dedic = false;
         PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
         hc.lower = E.lower;
         hc.ok = 1;
@@ -3437,6 +3463,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
             R.ga.upper = -4.0f;
             const double expect = std::min(1.0, f * 1.2 + 2.0 * w) * (double)m1;
             long long cap = pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096;
+            R.dedic = dedic_fits(expect / std::max<double>((double)m1, 1.0));
             for (int attempt = 0; attempt < 2; attempt++) {
                 PG_TRY(R.emit_launch(cap, 1 << 10, true));
                 PG_HIP(ctx, hipMemcpyAsync(R.hseg.data(), R.seg_cnt, nwg * 8, hipMemcpyDeviceToHost, st));
@@ -3446,6 +3473,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
                 cap = mx + mx / 8 + 4096;
             }
             R.ga.lower_dev = nullptr;
+            R.dedic = false;
             PG_TRY(fine_hist(R, E.ctl, &E.h2t));
             window_bounds<<<1, 1024, 0, st>>>(E.h2t, E.kwant_max, band, E.ctl);
             PG_KERNEL_CHECK(ctx);
@@ -3690,6 +3718,7 @@ struct PgCohortCorr {
         long long m1 = 0, kwant = 0, nst = 0, stride = 1, stl_off = 0, seg_cap = 0, ccap = 0;
         double f = 0.0, band = 0.0;
         bool graph = false;  // built in the graph (else outside it)
+        bool dedic = false;  // Gram hits staged in the dedicated LDS area
     };
     pg_ctx *ctx = nullptr;
     std::vector<Subj> subj;
@@ -3755,6 +3784,7 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
         const double w = 2.0 / NBIN;
         const double expect = std::min(1.0, S.f * 1.2 + 2.0 * w) * (double)S.m1;
         S.seg_cap = (pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096 + 63) / 64 * 64;
+        S.dedic = dedic_fits(expect / std::max<double>((double)S.m1, 1.0));
         // PAGANI_COHORT_SEG_SCALE scales the segment headroom (a value far
         // below 1 makes segments overflow: the subject is then flagged and
         // rebuilt outside the graph; writes past a segment are dropped)
@@ -3825,6 +3855,8 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
                                     (int)(stage + NBIN * 4)));
     PG_HIP(ctx, hipFuncSetAttribute((const void *)gram2_kernel<MODE_EMIT, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)stage));
+    PG_HIP(ctx, hipFuncSetAttribute((const void *)gram2_kernel<MODE_EMIT, false, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage));
     const size_t shs = (size_t)max_nbk * 8;
     if (shs > 64 * 1024)
         PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_scatter<false>,
@@ -3886,7 +3918,8 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     ga.band_r = P->band_r;
     ga.band_cap = 0;
     ga.band_cnt = P->counters + 0;
-    gram2_kernel<MODE_EMIT, false><<<(unsigned)nwg, THREADS, stage, st>>>(ga);
+    if (S.dedic) gram2_kernel<MODE_EMIT, false, true><<<(unsigned)nwg, THREADS, stage, st>>>(ga);
+    else gram2_kernel<MODE_EMIT, false><<<(unsigned)nwg, THREADS, stage, st>>>(ga);
     const int hb = (int)std::min<long long>(256, nwg);
     fill_u32<<<(unsigned)pg_cdiv(2 * NBIN2, 256), 256, 0, st>>>((uint32_t *)P->h2t, 2 * NBIN2);
     emitted_hist<<<hb, 1024, 0, st>>>(P->seg_r, S.seg_cap, P->seg_cnt, (int)nwg, P->ctl, P->h2);

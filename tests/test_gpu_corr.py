@@ -145,6 +145,23 @@ def test_smode_tie_group_larger_than_a_bin(ctx, prec):
     assert info.t_star == tstar
 
 
+def test_smode_dense_cluster_overflows_block_staging(ctx):
+    # 0.5% sparsity expects about 1% of the pairs kept, so the band-free
+    # emit stages hits in its small per-wave area (255 records); a 300-voxel
+    # cluster at r ~ 0.9 fills whole 128 x 64 blocks with hits, which then
+    # take the direct-write path.  The edge set must not change.
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((120, 3000)).astype(np.float32)
+    x[:, :300] += 3.0 * rng.standard_normal((120, 1)).astype(np.float32)
+    z64, _ = O.zscore(x)
+    orow, ocol, _, tstar = O.corr_csr_s(z64, 0.5)
+    say(f"dense cluster nnz={orow[-1]}")
+    info, row, col, _ = _build(ctx, x, 0.5, sparsity=True, prec=1)
+    np.testing.assert_array_equal(row, orow)
+    np.testing.assert_array_equal(col, ocol)
+    assert info.t_star == tstar
+
+
 def _sparsity_splitting_a_tie(z64, n):
     """A sparsity percent whose k-th largest r is tied and the cut splits the tie."""
     iu = np.triu_indices(n, 1)

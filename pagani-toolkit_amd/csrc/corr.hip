@@ -1334,11 +1334,29 @@ inline void launch_recheck(const pg_ctx *ctx, hipStream_t st, const int2 *pairs,
     }
 }
 
+// Sampled histogram pass: every stride-th supertile of the nst listed (at
+// most 1 in PG_SAMPLE_MAX, at least 16 sampled; 1 = no sampled pass).
+#ifndef PG_SAMPLE_MAX
+#define PG_SAMPLE_MAX 100
+#endif
+inline long long sample_stride(long long nst)
+{
+    return nst >= 8 ? std::max<long long>(2, std::min<long long>(PG_SAMPLE_MAX, nst / 16)) : 1;
+}
+
 // Whether a band-free emit with hit fraction d stages in the dedicated area:
 // a wave's block holds 8192 pairs, so about 8192 d hits; at most 160 keeps
 // six standard deviations under the area's 255 records (a block beyond it
 // writes its hits directly, correct but slower).
 inline bool dedic_fits(double d) { return d * (g2::WR * g2::WC) <= 160.0; }
+
+// Device-side decisions of the single-pass sparsity selection, so the pass
+// runs without host round trips.
+struct SelCtl {
+    float lower;   // emit bound L
+    float L2, U2;  // FP64 candidate window on r~
+    int ok;        // cleared by any step whose assumption fails
+};
 
 // ------------------------------------------------------------- CSR build
 
@@ -1436,12 +1454,15 @@ __device__ __forceinline__ void seg_batches(const EdgeSrc &s, int g, long long c
 #endif
 }
 
-__global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, unsigned *__restrict__ cnt)
+// Work-group x counts source g0 + x; counts of source g land in column g of
+// the bucket-major table cnt[nbk][nsrc].
+__global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, int g0, int nsrc,
+                                                            unsigned *__restrict__ cnt)
 {
     extern __shared__ unsigned sc[];
     for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) sc[b] = 0u;
     __syncthreads();
-    const int g = blockIdx.x;
+    const int g = g0 + blockIdx.x;
     const long long c = src_count(s, g);
     const float ka = src_bound(s);
     seg_batches(s, g, c, [&](const SegBatch &a) {
@@ -1453,7 +1474,84 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, 
             }
     });
     __syncthreads();
-    for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) cnt[(long long)b * gridDim.x + g] = sc[b];
+    for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) cnt[(long long)b * nsrc + g] = sc[b];
+}
+
+// The window split and the segments' bucket counts in one pass over the
+// segments (one work-group each, as bucket_count): r~ > U2 (more exactly,
+// not r~ <= U2: the CSR filter's test) are definite edges, counted into the
+// segment's column of cnt (both directions) and its definite total dpart[g];
+// L2 <= r~ <= U2 go to the candidate list, staged in LDS and reserved once
+// per work-group (beyond WC_STAGE, one by one).  Without a window
+// (ctl->ok == 0) nothing is kept.  Replaces window_kernel + bucket_count over
+// the segments; the selected list's counts follow from bucket_count (g0 =
+// nseg) once it exists.
+constexpr int WC_STAGE = 1024;
+__global__ __launch_bounds__(SEG_THREADS) void window_count(EdgeSrc s, const SelCtl *__restrict__ ctl, int nbk,
+                                                            int nsrc, unsigned *__restrict__ cnt,
+                                                            int2 *__restrict__ cand, float *__restrict__ cand_r,
+                                                            long long cand_cap,
+                                                            unsigned long long *__restrict__ cand_cnt,
+                                                            unsigned long long *__restrict__ dpart)
+{
+    extern __shared__ unsigned sc[];  // [nbk]
+    __shared__ int2 s_ce[WC_STAGE];
+    __shared__ float s_cr[WC_STAGE];
+    __shared__ unsigned s_nc;
+    __shared__ unsigned long long s_nd, s_base;
+    const int tid = threadIdx.x, g = blockIdx.x;
+    for (int b = tid; b < nbk; b += SEG_THREADS) sc[b] = 0u;
+    if (tid == 0) {
+        s_nc = 0u;
+        s_nd = 0ull;
+    }
+    __syncthreads();
+    const long long c = ctl->ok ? src_count(s, g) : 0;
+    const float lo = ctl->L2, hi = ctl->U2;
+    unsigned nd = 0;
+    seg_batches(s, g, c, [&](const SegBatch &a) {
+#pragma unroll
+        for (int u = 0; u < SEG_UNROLL; u++) {
+            if (!a.in[u]) continue;
+            const float r = a.r[u];
+            const int2 e = a.e[u];
+            if (!(r <= hi)) {
+                atomicAdd(&sc[e.x >> BSH], 1u);
+                atomicAdd(&sc[e.y >> BSH], 1u);
+                nd++;
+            } else if (r >= lo) {
+                const unsigned p = atomicAdd(&s_nc, 1u);
+                if (p < (unsigned)WC_STAGE) {
+                    s_ce[p] = e;
+                    s_cr[p] = r;
+                } else {
+                    const unsigned long long q = atomicAdd(cand_cnt, 1ull);
+                    if ((long long)q < cand_cap) {
+                        cand[q] = e;
+                        cand_r[q] = r;
+                    }
+                }
+            }
+        }
+    });
+    unsigned long long ndw = nd;
+    for (int o = 32; o > 0; o >>= 1) ndw += __shfl_xor(ndw, o);
+    if ((tid & 63) == 0 && ndw) atomicAdd(&s_nd, ndw);
+    __syncthreads();
+    for (int b = tid; b < nbk; b += SEG_THREADS) cnt[(long long)b * nsrc + g] = sc[b];
+    const unsigned nl = min(s_nc, (unsigned)WC_STAGE);
+    if (tid == 0) {
+        dpart[g] = s_nd;
+        s_base = nl ? atomicAdd(cand_cnt, (unsigned long long)nl) : 0ull;
+    }
+    __syncthreads();
+    for (unsigned k = tid; k < nl; k += SEG_THREADS) {
+        const unsigned long long q = s_base + k;
+        if ((long long)q < cand_cap) {
+            cand[q] = s_ce[k];
+            cand_r[q] = s_cr[k];
+        }
+    }
 }
 
 // Per bucket b (one block each): tot[b] and the within-bucket exclusive scan
@@ -1934,13 +2032,6 @@ __host__ __device__ inline float round_down_f(double x)
 
 constexpr int NBIN2 = 16384;   // fine histogram over the emitted set
 
-// Device-side decisions of the single-pass sparsity selection, so the pass
-// runs without host round trips.
-struct SelCtl {
-    float lower;   // emit bound L
-    float L2, U2;  // FP64 candidate window on r~
-    int ok;        // cleared by any step whose assumption fails
-};
 
 // Largest bin b of h[0..nb) with sum_{b' >= b} h[b'] >= target (-1 if none),
 // by one block: per-thread bin runs, a suffix scan over threads, one scan of
@@ -2953,8 +3044,9 @@ struct CorrRun {
         return PG_OK;
     }
 
-    // CSR build (bucketed counting sort), launched without a host check
-    pg_status launch_csr(const EdgeSrc &src, long long nnz)
+    // CSR build (bucketed counting sort), launched without a host check.
+    // seg_counted: the segments' bucket counts are in place (window_count).
+    pg_status launch_csr(const EdgeSrc &src, long long nnz, bool seg_counted = false)
     {
         if (nnz > INT32_MAX) return pg_fail(ctx, PG_ENOTSUP, "nnz %lld exceeds int32 CSR", (long long)nnz);
         const size_t nnzs = (size_t)std::max<long long>(nnz, 1);
@@ -2979,7 +3071,10 @@ struct CorrRun {
         }
         if (n > 0) {
             const size_t shb = (size_t)nbk * 4;
-            bucket_count<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(src, nbk, bcnt);
+            if (seg_counted)
+                bucket_count<<<1, SEG_THREADS, shb, st>>>(src, nbk, (int)nwg, nsrc, bcnt);
+            else
+                bucket_count<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(src, nbk, 0, nsrc, bcnt);
             bucket_scan_local<<<(unsigned)nbk, 256, 0, st>>>(bcnt, nsrc, boff, btot);
             bucket_scan_global<<<1, 1024, 0, st>>>(btot, nbk, bstart);
             const size_t shs = (size_t)nbk * (4 + 4);  // bases + cursors
@@ -3102,7 +3197,11 @@ struct Window {
 // counted as definite (per-block counts in dpart), the rest of the window
 // goes to the candidate list (count in counters[0], at most ccap written),
 // whose FP64 r is then computed (largest |r~ - r64| into counters[5]).
-pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Window *win)
+// count_segments: the split also fills the segments' columns of the CSR
+// build's bucket counts (window_count, the definite edges being exactly the
+// segment entries the build keeps: r~ above ctl's U2); launch_csr then counts
+// only the selected list.
+pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Window *win, bool count_segments = false)
 {
     pg_ctx *ctx = R.ctx;
     hipStream_t st = R.st;
@@ -3111,12 +3210,25 @@ pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Windo
     PG_TRY(pg_bufT(ctx, "corr.cand64", (size_t)ccap, &win->cand64));
     PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)ccap, &win->extra));
     PG_TRY(clear_dev(ctx, st, R.counters, 16 * sizeof(unsigned long long)));
-    const unsigned wx = (unsigned)std::max<long long>(1, pg_cdiv(R.seg_cap, WIN_CHUNK));
-    win->nd = (int)(wx * R.nwg);
-    PG_TRY(pg_bufT(ctx, "corr.dpart", (size_t)win->nd, &win->dpart));
-    window_kernel<<<dim3(wx, (unsigned)R.nwg), 256, 0, st>>>(R.seg, R.seg_r, R.seg_cap, R.seg_cnt, (int)R.nwg, ctl,
-                                                              win->cand, win->cand_r, ccap, R.counters + 0,
-                                                              win->dpart);
+    if (count_segments) {
+        win->nd = (int)R.nwg;
+        PG_TRY(pg_bufT(ctx, "corr.dpart", (size_t)win->nd, &win->dpart));
+        unsigned *bcnt;
+        PG_TRY(pg_bufT(ctx, "corr.bcnt", (size_t)R.nsrc * R.nbk, &bcnt));
+        EdgeSrc es = R.edges(nullptr, nullptr, 0);
+        es.seg_r = R.seg_r;
+        PG_HIP(ctx, hipFuncSetAttribute((const void *)window_count, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)((size_t)R.nbk * 4)));
+        window_count<<<(unsigned)R.nwg, SEG_THREADS, (size_t)R.nbk * 4, st>>>(
+            es, ctl, R.nbk, R.nsrc, bcnt, win->cand, win->cand_r, ccap, R.counters + 0, win->dpart);
+    } else {
+        const unsigned wx = (unsigned)std::max<long long>(1, pg_cdiv(R.seg_cap, WIN_CHUNK));
+        win->nd = (int)(wx * R.nwg);
+        PG_TRY(pg_bufT(ctx, "corr.dpart", (size_t)win->nd, &win->dpart));
+        window_kernel<<<dim3(wx, (unsigned)R.nwg), 256, 0, st>>>(R.seg, R.seg_r, R.seg_cap, R.seg_cnt, (int)R.nwg,
+                                                                  ctl, win->cand, win->cand_r, ccap, R.counters + 0,
+                                                                  win->dpart);
+    }
     launch_recheck(ctx, st, win->cand, R.counters + 0, ccap, R.z64, R.n, R.t, win->cand64, win->cand_r,
                    R.counters + 5);
     PG_KERNEL_CHECK(ctx);
@@ -3127,9 +3239,9 @@ pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Windo
 // (select_kernel; t* into tstar_d, errors into counters[4]) and the definite
 // count into counters[3].
 pg_status window_select(CorrRun &R, const SelCtl *ctl, long long ccap, long long kwant, double *tstar_d,
-                        Window *win)
+                        Window *win, bool count_segments = false)
 {
-    PG_TRY(window_candidates(R, ctl, ccap, win));
+    PG_TRY(window_candidates(R, ctl, ccap, win, count_segments));
     hipStream_t st = R.st;
     unsigned long long *c = R.counters;
     select_kernel<<<1, 1024, 0, st>>>(win->cand, win->cand64, win->cand_r, c + 0, ccap, kwant, win->dpart, win->nd,
@@ -3245,7 +3357,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         const double w = 2.0 / NBIN;
         bool done = false;
         // about 2% of the supertiles, when there are enough to sample
-        const long long stride = nst >= 8 ? std::max<long long>(2, std::min<long long>(50, nst / 16)) : 1;
+        const long long stride = sample_stride(nst);
         if (stride > 1) {
             // Single pass, no host round trip until the end:
             // (1) sampled histogram -> a conservative lower bound L on the
@@ -3278,13 +3390,13 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
             const long long ccap = std::min<long long>(1 << 22, std::max<long long>(1 << 16, kwant / 4));
             Window win;
-            PG_TRY(window_select(R, ctl, ccap, kwant, tstar_d, &win));
+            PG_TRY(window_select(R, ctl, ccap, kwant, tstar_d, &win, true));
             extra = win.extra;
             EdgeSrc es = R.edges(extra, extra_cnt, ccap);
             es.seg_r = R.seg_r;
             es.keep_above_dev = &ctl->U2;
             PG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
-            PG_TRY(R.launch_csr(es, 2 * kwant));
+            PG_TRY(R.launch_csr(es, 2 * kwant, true));
             PG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
             // every check's inputs gathered on the device, one small copy back
             SpReport *rep_d;
@@ -3417,7 +3529,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
         // when there are enough supertiles to sample, else a fixed guess
         long long seg0 = pg_cdiv(std::max<long long>((long long)(m1 * 0.03), 1 << 20), nwg) + 4096;
         double dens = 1.0;  // expected hit fraction (unknown without the sampled pass)
-        const long long stride = R.nst >= 8 ? std::max<long long>(2, std::min<long long>(50, R.nst / 16)) : 1;
+        const long long stride = sample_stride(R.nst);
         if (stride > 1 && n > 1) {
             std::vector<unsigned long long> h;
             PG_TRY(R.hist_pass(stride, h));
@@ -3448,8 +3560,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
         inf.s_thresh = opts->thr;
         const double w = 2.0 / NBIN;
         bool ok = false;
-        const long long stride =
-            R.nst >= 8 ? std::max<long long>(2, std::min<long long>(50, R.nst / 16)) : 1;
+        const long long stride = sample_stride(R.nst);
         if (E.kwant_max > 0 && n > 1 && stride > 1) {
             // the single pass's sampled bound for the largest k
             const long long msamp = sampled_pairs(R.stl, stride, R.nr, R.nc, R.rt, R.tm, R.tn, n, R.sup);
@@ -3772,7 +3883,7 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
         S.nst = (long long)stl.size();
         S.stl_off = (long long)stl_all.size();
         stl_all.insert(stl_all.end(), stl.begin(), stl.end());
-        S.stride = S.nst >= 8 ? std::max<long long>(2, std::min<long long>(50, S.nst / 16)) : 1;
+        S.stride = sample_stride(S.nst);
         // the single pass needs a sample of the supertiles and T within the
         // LDS-staged z-score; anything else is built outside the graph
         S.graph = S.kwant > 0 && 2 * S.kwant <= INT32_MAX && S.n > 1 && S.stride > 1 && S.t <= ZS_TMAX;
@@ -3857,6 +3968,8 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
                                     (int)stage));
     PG_HIP(ctx, hipFuncSetAttribute((const void *)gram2_kernel<MODE_EMIT, false, true>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage));
+    PG_HIP(ctx, hipFuncSetAttribute((const void *)window_count, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)((size_t)max_nbk * 4)));
     const size_t shs = (size_t)max_nbk * 8;
     if (shs > 64 * 1024)
         PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_scatter<false>,
@@ -3927,15 +4040,8 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     window_bounds<<<1, 1024, 0, st>>>(P->h2t, S.kwant, S.band, P->ctl);
     fill_u32<<<1, 32, 0, st>>>((uint32_t *)P->counters, 32);
     unsigned long long *c = P->counters;
-    window_kernel<<<dim3((unsigned)S.wx, (unsigned)nwg), 256, 0, st>>>(P->seg, P->seg_r, S.seg_cap, P->seg_cnt,
-                                                                       (int)nwg, P->ctl, P->cand, P->cand_r, S.ccap,
-                                                                       c + 0, P->dpart);
-    launch_recheck(ctx, st, P->cand, c + 0, S.ccap, P->z64, S.n, S.t, P->cand64, P->cand_r, c + 5);
-    const int nd = (int)(S.wx * nwg);
-    select_kernel<<<1, 1024, 0, st>>>(P->cand, P->cand64, P->cand_r, c + 0, S.ccap, S.kwant, P->dpart, nd, P->extra,
-                                      c + 1, P->tstar, c + 2, (int *)(c + 4));
-    definite_sum<<<1, 1024, 0, st>>>(P->dpart, nd, c + 3);
-    // CSR straight into the subject's outputs
+    // the segments as edge sources (r~ above the window kept), then the
+    // window split with their bucket counts (window_count)
     EdgeSrc es;
     std::memset(&es, 0, sizeof es);
     es.seg = P->seg;
@@ -3948,8 +4054,16 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     es.seg_r = P->seg_r;
     es.keep_above_dev = &P->ctl->U2;
     const int nsrcb = (int)nwg + 1;
+    window_count<<<(unsigned)nwg, SEG_THREADS, (size_t)S.nbk * 4, st>>>(es, P->ctl, S.nbk, nsrcb, P->bcnt, P->cand,
+                                                                        P->cand_r, S.ccap, c + 0, P->dpart);
+    launch_recheck(ctx, st, P->cand, c + 0, S.ccap, P->z64, S.n, S.t, P->cand64, P->cand_r, c + 5);
+    const int nd = (int)nwg;
+    select_kernel<<<1, 1024, 0, st>>>(P->cand, P->cand64, P->cand_r, c + 0, S.ccap, S.kwant, P->dpart, nd, P->extra,
+                                      c + 1, P->tstar, c + 2, (int *)(c + 4));
+    definite_sum<<<1, 1024, 0, st>>>(P->dpart, nd, c + 3);
+    // CSR straight into the subject's outputs; the selected list's counts
     const long long nnz = 2 * S.kwant;
-    bucket_count<<<(unsigned)nsrcb, SEG_THREADS, (size_t)S.nbk * 4, st>>>(es, S.nbk, P->bcnt);
+    bucket_count<<<1, SEG_THREADS, (size_t)S.nbk * 4, st>>>(es, S.nbk, (int)nwg, nsrcb, P->bcnt);
     bucket_scan_local<<<(unsigned)S.nbk, 256, 0, st>>>(P->bcnt, nsrcb, P->boff, P->btot);
     bucket_scan_global<<<1, 1024, 0, st>>>(P->btot, S.nbk, P->bstart);
     bucket_scatter<false><<<(unsigned)nsrcb, SEG_THREADS, (size_t)S.nbk * 8, st>>>(es, S.nbk, P->boff, P->bstart,

@@ -3356,7 +3356,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         // ---------------------------------------------------- sparsity
         const double w = 2.0 / NBIN;
         bool done = false;
-        // about 2% of the supertiles, when there are enough to sample
+        // about 1% of the supertiles, when there are enough to sample
         const long long stride = sample_stride(nst);
         if (stride > 1) {
             // Single pass, no host round trip until the end:
@@ -3525,7 +3525,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
         // every pair the loosest threshold could keep, with its r~
         const float thr_eff = thr_effective(opts->thr, opts->weighted != 0);
         E.lower = round_down_f((double)thr_eff - band);
-        // segment sizes from a sampled histogram (about 2% of the supertiles)
+        // segment sizes from a sampled histogram (about 1% of the supertiles)
         // when there are enough supertiles to sample, else a fixed guess
         long long seg0 = pg_cdiv(std::max<long long>((long long)(m1 * 0.03), 1 << 20), nwg) + 4096;
         double dens = 1.0;  // expected hit fraction (unknown without the sampled pass)

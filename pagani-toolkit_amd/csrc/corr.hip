@@ -1210,13 +1210,44 @@ __device__ __forceinline__ double r64_pair(const double *__restrict__ z, int t, 
     return s > 1.0 ? 1.0 : (s < -1.0 ? -1.0 : s);
 }
 
+// Device-side decisions of the single-pass sparsity selection, so the pass
+// runs without host round trips.
+struct SelCtl {
+    float lower;   // emit bound L
+    float L2, U2;  // FP64 candidate window on r~
+    int ok;        // cleared by any step whose assumption fails
+};
+
+// Selection histogram: SEL_NB bins of the candidates' FP64 r over the window
+// widened by the band (r64 lies in [L2 - band, U2 + band]); bin 0 holds the
+// largest r.  Built by the recheck, read by select_kernel.
+constexpr int SEL_NB = 4096, SEL_CAP = 2048;
+struct SelHist {
+    unsigned *h;  // SEL_NB counts, zeroed before the recheck; nullptr: none
+    const SelCtl *ctl;
+    double band;
+};
+__device__ __forceinline__ void sel_range(const SelHist &s, double &top, double &scale)
+{
+    const double lo = (double)s.ctl->L2 - s.band, hi = (double)s.ctl->U2 + s.band;
+    top = hi;
+    scale = hi > lo ? (double)SEL_NB / (hi - lo) : 1.0;
+}
+// clamp(floor((top - r) * scale)): non-decreasing as r decreases, i.e. in the
+// selection key (r descending), so the bins cut the key order into ranges
+__device__ __forceinline__ int sel_bin(double r, double top, double scale)
+{
+    const double x = (top - r) * scale;
+    return x < 1.0 ? 0 : (x >= (double)(SEL_NB - 1) ? SEL_NB - 1 : (int)x);
+}
+
 // FP64 r of every listed pair; with rfast, also the largest |r~ - r64| over
 // them into *errmax (bit pattern of a non-negative double, whose unsigned
 // order is its numeric order), one atomic per wave.
 __global__ void recheck_kernel(const int2 *__restrict__ pairs, const unsigned long long *__restrict__ cnt,
                                long long cap, const double *__restrict__ z, int t,
                                double *__restrict__ r_out, const float *__restrict__ rfast,
-                               unsigned long long *__restrict__ errmax)
+                               unsigned long long *__restrict__ errmax, SelHist sh)
 {
     long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long c = (long long)min((unsigned long long)cap, *cnt);
@@ -1226,6 +1257,11 @@ __global__ void recheck_kernel(const int2 *__restrict__ pairs, const unsigned lo
         const double r = r64_pair(z, t, p.x, p.y);
         r_out[k] = r;
         if (rfast) err = fabs((double)rfast[k] - r);
+        if (sh.h) {
+            double top, scale;
+            sel_range(sh, top, scale);
+            atomicAdd(&sh.h[sel_bin(r, top, scale)], 1u);
+        }
     }
     if (!rfast) return;
     for (int o = 32; o > 0; o >>= 1) err = fmax(err, __shfl_xor(err, o));
@@ -1267,7 +1303,7 @@ __global__ __launch_bounds__(256) void recheck_rows(const int2 *__restrict__ pai
                                                     const unsigned long long *__restrict__ cnt, long long cap,
                                                     const double *__restrict__ z, int t, double *__restrict__ r_out,
                                                     const float *__restrict__ rfast,
-                                                    unsigned long long *__restrict__ errmax)
+                                                    unsigned long long *__restrict__ errmax, SelHist sh)
 {
 #pragma clang fp contract(off)
     __shared__ double prod[4][RCK_P][RCK_T + 1];
@@ -1312,6 +1348,11 @@ __global__ __launch_bounds__(256) void recheck_rows(const int2 *__restrict__ pai
             const double r = sum > 1.0 ? 1.0 : (sum < -1.0 ? -1.0 : sum);
             r_out[q] = r;
             if (rfast) err = fmax(err, fabs((double)rfast[q] - r));
+            if (sh.h) {
+                double top, scale;
+                sel_range(sh, top, scale);
+                atomicAdd(&sh.h[sel_bin(r, top, scale)], 1u);
+            }
         }
     }
     if (!rfast) return;
@@ -1321,16 +1362,18 @@ __global__ __launch_bounds__(256) void recheck_rows(const int2 *__restrict__ pai
 
 // FP64 r of the first min(cap, *cnt) listed pairs (recheck_rows when the row
 // offsets fit 32 bits; grid of at most 8 work-groups per CU, bound by cap).
+// sh.h: also the selection histogram of the r (zeroed by the caller).
 inline void launch_recheck(const pg_ctx *ctx, hipStream_t st, const int2 *pairs, const unsigned long long *cnt,
                            long long cap, const double *z, int n, int t, double *r_out, const float *rfast,
-                           unsigned long long *errmax)
+                           unsigned long long *errmax, SelHist sh = SelHist{nullptr, nullptr, 0.0})
 {
     if (cap <= 0) return;
     if ((unsigned long long)n * (unsigned long long)t < (1ull << 32)) {
         const long long g = std::min<long long>(pg_cdiv(cap, 4 * RCK_P), 8LL * std::max(ctx->num_cu, 1));
-        recheck_rows<<<(unsigned)g, 256, 0, st>>>(pairs, cnt, cap, z, t, r_out, rfast, errmax);
+        recheck_rows<<<(unsigned)g, 256, 0, st>>>(pairs, cnt, cap, z, t, r_out, rfast, errmax, sh);
     } else {
-        recheck_kernel<<<(unsigned)pg_cdiv(cap, 256), 256, 0, st>>>(pairs, cnt, cap, z, t, r_out, rfast, errmax);
+        recheck_kernel<<<(unsigned)pg_cdiv(cap, 256), 256, 0, st>>>(pairs, cnt, cap, z, t, r_out, rfast, errmax,
+                                                                     sh);
     }
 }
 
@@ -1349,14 +1392,6 @@ inline long long sample_stride(long long nst)
 // six standard deviations under the area's 255 records (a block beyond it
 // writes its hits directly, correct but slower).
 inline bool dedic_fits(double d) { return d * (g2::WR * g2::WC) <= 160.0; }
-
-// Device-side decisions of the single-pass sparsity selection, so the pass
-// runs without host round trips.
-struct SelCtl {
-    float lower;   // emit bound L
-    float L2, U2;  // FP64 candidate window on r~
-    int ok;        // cleared by any step whose assumption fails
-};
 
 // ------------------------------------------------------------- CSR build
 
@@ -2379,14 +2414,13 @@ __device__ unsigned long long radix_select(long long c, long long m, KeyF key, W
 // for a window of near-uniform r) are gathered into LDS and ranked by
 // comparison.  A bin of more than SEL_CAP members takes the byte-wise radix
 // select instead.
-constexpr int SEL_NB = 4096, SEL_CAP = 2048;
 __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ cand, const double *__restrict__ r64,
                                                       const float *__restrict__ rfast,
                                                       const unsigned long long *__restrict__ cand_cnt, long long cand_cap,
                                                       long long k, const unsigned long long *__restrict__ dsrc, int dn,
                                                       int2 *__restrict__ extra, unsigned long long *__restrict__ extra_cnt,
                                                       double *__restrict__ tstar, unsigned long long *__restrict__ flips,
-                                                      int *__restrict__ err)
+                                                      int *__restrict__ err, SelHist shh)
 {
     __shared__ unsigned hist[SEL_NB];
     __shared__ unsigned long long mkey[SEL_CAP];
@@ -2423,48 +2457,56 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
         const int2 e = cand[q];
         return ((unsigned long long)(unsigned)e.x << 32) | (unsigned)e.y;
     };
-    // key range
+    // the bins: from the recheck's histogram of the r (sel_bin), else a pass
+    // for the key range and a histogram of (key - kmin) >> shift
     unsigned long long kmin = ~0ull, kmax = 0ull;
-    for (long long q0 = tid; q0 < c; q0 += RB * 1024) {
-        unsigned long long kk[RB];
-#pragma unroll
-        for (int u = 0; u < RB; u++) {
-            const long long q = q0 + u * 1024;
-            kk[u] = q < c ? key_r_desc(r64[q]) : 0ull;
-        }
-#pragma unroll
-        for (int u = 0; u < RB; u++)
-            if (q0 + u * 1024 < c) {
-                kmin = kk[u] < kmin ? kk[u] : kmin;
-                kmax = kk[u] > kmax ? kk[u] : kmax;
-            }
-    }
-    for (int b = tid; b < SEL_NB; b += 1024) hist[b] = 0u;
-    if (tid == 0) {
-        sh[0] = ~0ull;
-        sh[1] = 0ull;
-    }
-    __syncthreads();
-    atomicMin(&sh[0], kmin);
-    atomicMax(&sh[1], kmax);
-    __syncthreads();
-    kmin = sh[0];
-    kmax = sh[1];
     int shift = 0;
-    while (((kmax - kmin) >> shift) >= (unsigned long long)SEL_NB) shift++;
-    // histogram of (key - kmin) >> shift
-    for (long long q0 = tid; q0 < c; q0 += RB * 1024) {
-        unsigned long long kk[RB];
+    double htop = 0.0, hscale = 1.0;
+    if (shh.h) {
+        sel_range(shh, htop, hscale);
+        for (int b = tid; b < SEL_NB; b += 1024) hist[b] = shh.h[b];
+        __syncthreads();
+    } else {
+        for (long long q0 = tid; q0 < c; q0 += RB * 1024) {
+            unsigned long long kk[RB];
 #pragma unroll
-        for (int u = 0; u < RB; u++) {
-            const long long q = q0 + u * 1024;
-            kk[u] = q < c ? key_r_desc(r64[q]) : 0ull;
+            for (int u = 0; u < RB; u++) {
+                const long long q = q0 + u * 1024;
+                kk[u] = q < c ? key_r_desc(r64[q]) : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < RB; u++)
+                if (q0 + u * 1024 < c) {
+                    kmin = kk[u] < kmin ? kk[u] : kmin;
+                    kmax = kk[u] > kmax ? kk[u] : kmax;
+                }
         }
+        for (int b = tid; b < SEL_NB; b += 1024) hist[b] = 0u;
+        if (tid == 0) {
+            sh[0] = ~0ull;
+            sh[1] = 0ull;
+        }
+        __syncthreads();
+        atomicMin(&sh[0], kmin);
+        atomicMax(&sh[1], kmax);
+        __syncthreads();
+        kmin = sh[0];
+        kmax = sh[1];
+        while (((kmax - kmin) >> shift) >= (unsigned long long)SEL_NB) shift++;
+        // histogram of (key - kmin) >> shift
+        for (long long q0 = tid; q0 < c; q0 += RB * 1024) {
+            unsigned long long kk[RB];
 #pragma unroll
-        for (int u = 0; u < RB; u++)
-            if (q0 + u * 1024 < c) atomicAdd(&hist[(kk[u] - kmin) >> shift], 1u);
+            for (int u = 0; u < RB; u++) {
+                const long long q = q0 + u * 1024;
+                kk[u] = q < c ? key_r_desc(r64[q]) : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < RB; u++)
+                if (q0 + u * 1024 < c) atomicAdd(&hist[(kk[u] - kmin) >> shift], 1u);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // bin holding the need-th smallest key: thread t owns bins 4t .. 4t+3
     {
         const unsigned h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2], h3 = hist[4 * tid + 3];
@@ -2503,15 +2545,18 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
         // members of the bin into LDS
         for (long long q0 = tid; q0 < c; q0 += RB * 1024) {
             unsigned long long kk[RB];
+            double rr[RB];
 #pragma unroll
             for (int u = 0; u < RB; u++) {
                 const long long q = q0 + u * 1024;
-                kk[u] = q < c ? key_r_desc(r64[q]) : 0ull;
+                rr[u] = q < c ? r64[q] : 0.0;
+                kk[u] = key_r_desc(rr[u]);
             }
 #pragma unroll
             for (int u = 0; u < RB; u++) {
                 const long long q = q0 + u * 1024;
-                if (q < c && (int)((kk[u] - kmin) >> shift) == bsel) {
+                const int bq = shh.h ? sel_bin(rr[u], htop, hscale) : (int)((kk[u] - kmin) >> shift);
+                if (q < c && bq == bsel) {
                     const unsigned p = atomicAdd(&s_m, 1u);
                     mkey[p] = kk[u];
                     mq[p] = (unsigned)q;
@@ -3191,6 +3236,7 @@ struct Window {
     double *cand64 = nullptr;
     unsigned long long *dpart = nullptr;
     int nd = 0;  // entries of dpart
+    SelHist sh{nullptr, nullptr, 0.0};  // the candidates' selection histogram
 };
 
 // Split the kept segment entries by ctl's window [L2, U2]: r~ > U2 are
@@ -3209,7 +3255,10 @@ pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Windo
     PG_TRY(pg_bufT(ctx, "corr.candr", (size_t)ccap, &win->cand_r));
     PG_TRY(pg_bufT(ctx, "corr.cand64", (size_t)ccap, &win->cand64));
     PG_TRY(pg_bufT(ctx, "corr.extra", (size_t)ccap, &win->extra));
-    PG_TRY(clear_dev(ctx, st, R.counters, 16 * sizeof(unsigned long long)));
+    PG_TRY(pg_bufT(ctx, "corr.selhist", (size_t)SEL_NB, &win->sh.h));
+    win->sh.ctl = ctl;
+    win->sh.band = R.band;
+    PG_TRY(clear_dev(ctx, st, R.counters, 16 * sizeof(unsigned long long), win->sh.h, SEL_NB * sizeof(unsigned)));
     if (count_segments) {
         win->nd = (int)R.nwg;
         PG_TRY(pg_bufT(ctx, "corr.dpart", (size_t)win->nd, &win->dpart));
@@ -3230,7 +3279,7 @@ pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Windo
                                                                   win->dpart);
     }
     launch_recheck(ctx, st, win->cand, R.counters + 0, ccap, R.z64, R.n, R.t, win->cand64, win->cand_r,
-                   R.counters + 5);
+                   R.counters + 5, win->sh);
     PG_KERNEL_CHECK(ctx);
     return PG_OK;
 }
@@ -3245,7 +3294,7 @@ pg_status window_select(CorrRun &R, const SelCtl *ctl, long long ccap, long long
     hipStream_t st = R.st;
     unsigned long long *c = R.counters;
     select_kernel<<<1, 1024, 0, st>>>(win->cand, win->cand64, win->cand_r, c + 0, ccap, kwant, win->dpart, win->nd,
-                                      win->extra, c + 1, tstar_d, c + 2, (int *)(c + 4));
+                                      win->extra, c + 1, tstar_d, c + 2, (int *)(c + 4), win->sh);
     definite_sum<<<1, 1024, 0, st>>>(win->dpart, win->nd, c + 3);
     PG_KERNEL_CHECK(R.ctx);
     return PG_OK;
@@ -3450,7 +3499,8 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
                 launch_recheck(ctx, st, R.bandl, counters + 0, std::min(nband, R.band_cap), R.z64, n, t, band64,
                                R.band_r, counters + 5);
             select_kernel<<<1, 1024, 0, st>>>(R.bandl, band64, R.band_r, counters + 0, R.band_cap, kwant, R.seg_cnt,
-                                              (int)nwg, extra, extra_cnt, tstar_d, flips, (int *)(counters + 4));
+                                              (int)nwg, extra, extra_cnt, tstar_d, flips, (int *)(counters + 4),
+                                              SelHist{nullptr, nullptr, 0.0});
             PG_KERNEL_CHECK(ctx);
             PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof R.hcnt, hipMemcpyDeviceToHost, st));
             PG_HIP(ctx, hipMemcpyAsync(&inf.t_star, tstar_d, 8, hipMemcpyDeviceToHost, st));
@@ -3842,6 +3892,7 @@ struct PgCohortCorr {
     unsigned long long *htot = nullptr, *seg_cnt = nullptr, *counters = nullptr, *h2t = nullptr, *dpart = nullptr;
     float *seg_r = nullptr, *cand_r = nullptr, *band_r = nullptr;
     SelCtl *ctl = nullptr;
+    unsigned *selhist = nullptr;  // the candidates' selection histogram
     unsigned *bcnt = nullptr, *ent = nullptr;
     long long *boff = nullptr, *bstart = nullptr, *btot = nullptr;
     int *ctmp = nullptr;
@@ -3952,6 +4003,7 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
     PG_TRY(pg_bufT(ctx, "coh.extra", (size_t)max_ccap, &P->extra));
     PG_TRY(pg_bufT(ctx, "coh.dpart", (size_t)max_dpart, &P->dpart));
     PG_TRY(pg_bufT(ctx, "coh.tstar", 2, &P->tstar));
+    PG_TRY(pg_bufT(ctx, "coh.selhist", (size_t)SEL_NB, &P->selhist));
     PG_TRY(pg_bufT(ctx, "coh.bcnt", (size_t)max_bsrc, &P->bcnt));
     PG_TRY(pg_bufT(ctx, "coh.boff", (size_t)max_bsrc, &P->boff));
     PG_TRY(pg_bufT(ctx, "coh.bstart", (size_t)max_nbk + 1, &P->bstart));
@@ -4038,7 +4090,8 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     emitted_hist<<<hb, 1024, 0, st>>>(P->seg_r, S.seg_cap, P->seg_cnt, (int)nwg, P->ctl, P->h2);
     hist_sum<<<dim3(NBIN2 / 256, 16), 256, 0, st>>>(P->h2, hb, NBIN2, P->h2t);
     window_bounds<<<1, 1024, 0, st>>>(P->h2t, S.kwant, S.band, P->ctl);
-    fill_u32<<<1, 32, 0, st>>>((uint32_t *)P->counters, 32);
+    fill_u32<<<SEL_NB / 256, 256, 0, st>>>((uint32_t *)P->counters, 32, P->selhist, SEL_NB);
+    const SelHist shh{P->selhist, P->ctl, S.band};
     unsigned long long *c = P->counters;
     // the segments as edge sources (r~ above the window kept), then the
     // window split with their bucket counts (window_count)
@@ -4056,10 +4109,10 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     const int nsrcb = (int)nwg + 1;
     window_count<<<(unsigned)nwg, SEG_THREADS, (size_t)S.nbk * 4, st>>>(es, P->ctl, S.nbk, nsrcb, P->bcnt, P->cand,
                                                                         P->cand_r, S.ccap, c + 0, P->dpart);
-    launch_recheck(ctx, st, P->cand, c + 0, S.ccap, P->z64, S.n, S.t, P->cand64, P->cand_r, c + 5);
+    launch_recheck(ctx, st, P->cand, c + 0, S.ccap, P->z64, S.n, S.t, P->cand64, P->cand_r, c + 5, shh);
     const int nd = (int)nwg;
     select_kernel<<<1, 1024, 0, st>>>(P->cand, P->cand64, P->cand_r, c + 0, S.ccap, S.kwant, P->dpart, nd, P->extra,
-                                      c + 1, P->tstar, c + 2, (int *)(c + 4));
+                                      c + 1, P->tstar, c + 2, (int *)(c + 4), shh);
     definite_sum<<<1, 1024, 0, st>>>(P->dpart, nd, c + 3);
     // CSR straight into the subject's outputs; the selected list's counts
     const long long nnz = 2 * S.kwant;

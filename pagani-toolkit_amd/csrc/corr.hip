@@ -103,6 +103,9 @@ static_assert(DMA_WAVE * 4 * 1024 == STAGE, "stage must split into 1 KB wave loa
 constexpr int STAGE_HITS = STAGE / 4 / 12;  // 512
 constexpr int RING_RECS = STAGE / 4 / 8;     // 768 8-byte records per wave in the free ring slot
 constexpr int DEDIC_RECS = 255;              // records per wave in the dedicated area
+#ifndef PG_G2_EPI
+#define PG_G2_EPI 2
+#endif
 }  // namespace g2
 
 __host__ __device__ __forceinline__ int g2_swz(int i) { return ((i >> 2) ^ (i >> 3)) & 1; }
@@ -220,12 +223,14 @@ __global__ __launch_bounds__(256) void zscore_fused(const float *__restrict__ x,
     __shared__ double smu[ZV], ssd[ZV];
     constexpr int LD = ZV + 1;
     const int i0 = blockIdx.x * ZV, tid = threadIdx.x;
-    for (int e = tid; e < ZV * t; e += 256) {
-        int l, ii;
-        if (layout == 0) { l = e / ZV; ii = e % ZV; }
-        else { ii = e / t; l = e % t; }
-        const int i = i0 + ii;
-        zt[l * LD + ii] = i < n ? (layout == 0 ? x[(long long)l * n + i] : x[(long long)i * t + l]) : 0.0f;
+    if (layout == 0) {  // time-major: ZV consecutive voxels of one time point per 32 lanes
+        const int ii = tid & (ZV - 1), i = i0 + ii;
+        for (int l = tid / ZV; l < t; l += 256 / ZV) zt[l * LD + ii] = i < n ? x[(long long)l * n + i] : 0.0f;
+    } else {  // voxel-major: one voxel's series per wave and pass
+        for (int ii = tid >> 6; ii < ZV; ii += 4) {
+            const int i = i0 + ii;
+            for (int l = tid & 63; l < t; l += 64) zt[l * LD + ii] = i < n ? x[(long long)i * t + l] : 0.0f;
+        }
     }
     __syncthreads();
     if (tid < ZV) {
@@ -243,27 +248,38 @@ __global__ __launch_bounds__(256) void zscore_fused(const float *__restrict__ x,
     }
     __syncthreads();
     // with panels, every element of rows [0, npad) x [0, ldp) is written
-    // (zeros outside [0, n) x [0, t)), so the panels need no clearing first
+    // (zeros outside [0, n) x [0, t)), so the panels need no clearing first.
+    // Each wave takes rows ii = wave, wave + 4, ...; each lane the time
+    // points 2 m, 2 m + 1 for m = lane, lane + 64, ... (ldp is even, and in
+    // the bf16 panels the two share a 32-bit word).
     const bool panels = p32 || phi;
     const int w = panels ? ldp : t, rows = panels ? npad : n;
-    for (int e = tid; e < ZV * w; e += 256) {
-        const int ii = e / w, l = e % w;
+    const int lane = tid & 63;
+    for (int ii = tid >> 6; ii < ZV; ii += 4) {
         const int i = i0 + ii;
-        if (i >= rows) continue;
-        const bool in = i < n && l < t;
-        float zf = 0.0f;
-        if (in) {
-            const double sdv = ssd[ii];
-            const double z = sdv > 0.0 ? ((double)zt[l * LD + ii] - smu[ii]) / sdv : 0.0;
-            z64[(long long)i * t + l] = z;
-            zf = (float)z;
-        }
-        if (p32) p32[(long long)i * ldp + l] = zf;
-        if (phi) {
-            const long long q = g2_index(i, l, npad);
-            const uint16_t h = f2bf(zf);
-            phi[q] = h;
-            plo[q] = f2bf(zf - bf2f(h));
+        if (i >= rows) break;
+        const double sdv = ssd[ii], mu = smu[ii];
+        for (int l = 2 * lane; l < w; l += 128) {
+            float zf[2] = {0.0f, 0.0f};
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (i < n && l + h < t) {
+                    const double z = sdv > 0.0 ? ((double)zt[(l + h) * LD + ii] - mu) / sdv : 0.0;
+                    z64[(long long)i * t + l + h] = z;
+                    zf[h] = (float)z;
+                }
+            }
+            if (p32) {
+                p32[(long long)i * ldp + l] = zf[0];
+                if (l + 1 < w) p32[(long long)i * ldp + l + 1] = zf[1];
+            }
+            if (phi) {
+                const long long q = g2_index(i, l, npad);  // l even: q even, q + 1 holds l + 1
+                const uint16_t h0 = f2bf(zf[0]), h1 = f2bf(zf[1]);
+                const uint16_t g0 = f2bf(zf[0] - bf2f(h0)), g1 = f2bf(zf[1] - bf2f(h1));
+                *(uint32_t *)(phi + q) = (uint32_t)h0 | ((uint32_t)h1 << 16);
+                *(uint32_t *)(plo + q) = (uint32_t)g0 | ((uint32_t)g1 << 16);
+            }
         }
     }
 }
@@ -989,6 +1005,142 @@ __device__ __forceinline__ void g2_emit_ballot(const GramArgs &a, float lower, c
     pd.on = true;
 }
 
+// One accumulator's 16 elements as a lane mask, element q at bit 15 - q:
+// per element a compare and an add-with-carry (m = 2 m + hit).
+__device__ __forceinline__ unsigned g2_mask16(const f32x16 &a, float lower)
+{
+    unsigned m = 0;
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        asm volatile("v_cmp_le_f32 vcc, %1, %2\n\tv_addc_co_u32 %0, vcc, %0, %0, vcc"
+                     : "+v"(m)
+                     : "s"(lower), "v"(a[q])
+                     : "vcc");
+    return m;
+}
+
+// Element q of an accumulator, q per lane: a select tree on q's bits, in
+// assembly (the compiler turns the same tree written in C++ into a scratch
+// array indexed by q).
+__device__ __forceinline__ float g2_pick16(const f32x16 &a, unsigned q)
+{
+    float t0, t1, t2, t3, t4, t5, t6, t7, r;
+    unsigned b;
+    asm volatile(
+        "v_and_b32 %[b], 1, %[q]\n\t"
+        "v_cmp_ne_u32 vcc, 0, %[b]\n\t"
+        "v_cndmask_b32 %[t0], %[a0], %[a1], vcc\n\t"
+        "v_cndmask_b32 %[t1], %[a2], %[a3], vcc\n\t"
+        "v_cndmask_b32 %[t2], %[a4], %[a5], vcc\n\t"
+        "v_cndmask_b32 %[t3], %[a6], %[a7], vcc\n\t"
+        "v_cndmask_b32 %[t4], %[a8], %[a9], vcc\n\t"
+        "v_cndmask_b32 %[t5], %[a10], %[a11], vcc\n\t"
+        "v_cndmask_b32 %[t6], %[a12], %[a13], vcc\n\t"
+        "v_cndmask_b32 %[t7], %[a14], %[a15], vcc\n\t"
+        "v_and_b32 %[b], 2, %[q]\n\t"
+        "v_cmp_ne_u32 vcc, 0, %[b]\n\t"
+        "v_cndmask_b32 %[t0], %[t0], %[t1], vcc\n\t"
+        "v_cndmask_b32 %[t1], %[t2], %[t3], vcc\n\t"
+        "v_cndmask_b32 %[t2], %[t4], %[t5], vcc\n\t"
+        "v_cndmask_b32 %[t3], %[t6], %[t7], vcc\n\t"
+        "v_and_b32 %[b], 4, %[q]\n\t"
+        "v_cmp_ne_u32 vcc, 0, %[b]\n\t"
+        "v_cndmask_b32 %[t0], %[t0], %[t1], vcc\n\t"
+        "v_cndmask_b32 %[t1], %[t2], %[t3], vcc\n\t"
+        "v_and_b32 %[b], 8, %[q]\n\t"
+        "v_cmp_ne_u32 vcc, 0, %[b]\n\t"
+        "v_cndmask_b32 %[r], %[t0], %[t1], vcc"
+        : [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3), [t4] "=&v"(t4), [t5] "=&v"(t5),
+          [t6] "=&v"(t6), [t7] "=&v"(t7), [r] "=&v"(r), [b] "=&v"(b)
+        : [q] "v"(q), [a0] "v"(a[0]), [a1] "v"(a[1]), [a2] "v"(a[2]), [a3] "v"(a[3]), [a4] "v"(a[4]),
+          [a5] "v"(a[5]), [a6] "v"(a[6]), [a7] "v"(a[7]), [a8] "v"(a[8]), [a9] "v"(a[9]), [a10] "v"(a[10]),
+          [a11] "v"(a[11]), [a12] "v"(a[12]), [a13] "v"(a[13]), [a14] "v"(a[14]), [a15] "v"(a[15])
+        : "vcc");
+    return r;
+}
+
+// The band-free epilogue by accumulator (16 elements at a time): each lane's
+// hit mask, its exclusive prefix over the wave from five bit-planes of the
+// per-lane counts (ballot + mbcnt, no LDS), and each lane writes its own hits
+// (a loop over its mask's bits: as many LDS record writes per accumulator as
+// the most hits any lane has in it, about two, instead of one per element
+// with a hit).  Records, staging area and overflow as g2_emit_ballot.
+template <bool BOUNDED, unsigned STAGE_RECS>
+__device__ __forceinline__ void g2_emit_blocks(const GramArgs &a, float lower, const f32x16 (&acc)[g2::BMW][g2::BNW],
+                                               int i0, int j0, int lane, int2 *myseg, unsigned long long *s_seg,
+                                               char *stage, G2Pending &pd)
+{
+    const int ib0 = i0 + (lane >> 5) * 4, jb0 = j0 + (lane & 31);
+    const unsigned lidx = ((unsigned)(lane >> 5) << 8) | (unsigned)(lane & 31);
+    const unsigned srec = g2_lds_addr(stage);
+    unsigned nst = 0;  // hits so far (wave-uniform)
+#pragma unroll
+    for (int bm = 0; bm < g2::BMW; bm++)
+#pragma unroll
+        for (int bn = 0; bn < g2::BNW; bn++) {
+            unsigned m;
+            if (BOUNDED) {
+                m = g2_mask16(acc[bm][bn], lower);
+            } else {
+                m = 0;
+                const int j = jb0 + bn * 32;
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const int i = ib0 + bm * 32 + (q >> 2) * 8 + (q & 3);
+                    const bool hit = (acc[bm][bn][q] >= lower) & (j < a.n) & (i < j);
+                    m = 2 * m + (hit ? 1u : 0u);
+                }
+            }
+            const unsigned cnt = (unsigned)__popc(m);
+            unsigned excl = 0, tot = 0;
+#pragma unroll
+            for (int b = 0; b < 5; b++) {
+                const unsigned long long mb = __builtin_amdgcn_ballot_w64((cnt >> b) & 1u);
+                if (mb) {
+                    excl += __builtin_amdgcn_mbcnt_hi((unsigned)(mb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mb, 0u))
+                            << b;
+                    tot += (unsigned)__popcll(mb) << b;
+                }
+            }
+            if (tot == 0) continue;
+            if (nst + tot <= STAGE_RECS) {
+                unsigned p = nst + excl;
+                const unsigned eb = ((unsigned)(bm * 32) << 6) | (unsigned)(bn * 32);
+                while (m) {
+                    const unsigned k = 31u - (unsigned)__builtin_clz(m);  // highest set bit: element 15 - k
+                    m &= ~(1u << k);
+                    const unsigned q = 15u - k;
+                    const float v = g2_pick16(acc[bm][bn], q);
+                    const unsigned idx = lidx | eb | ((q & 12u) << 7) | ((q & 3u) << 6);
+                    const unsigned long long rec = ((unsigned long long)idx << 32) | (unsigned long long)__float_as_uint(v);
+                    asm volatile("ds_write_b64 %0, %1" ::"v"(srec + p * 8u), "v"(rec) : "memory");
+                    p++;
+                }
+            }
+            nst += tot;
+        }
+    if (nst > STAGE_RECS) {
+        float lo2 = lower;
+        int i02 = i0, j02 = j0;
+        asm volatile("" : "+v"(lo2), "+v"(i02), "+v"(j02));
+        g2_emit<BOUNDED, false>(a, lo2, acc, i02, j02, lane, myseg, s_seg, stage, pd);
+        return;
+    }
+    if (nst == 0) return;
+    unsigned long long be = 0;
+    if (lane == 0) be = g2_lds_add(s_seg, (unsigned long long)nst);
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    pd.te = nst;
+    pd.tc = 0;
+    pd.be0 = (unsigned)__shfl(be, 0);
+    pd.bc0 = 0;
+    pd.spair = srec;
+    pd.sval = 0;
+    pd.i0 = i0;
+    pd.j0 = j0;
+    pd.on = true;
+}
+
 // Epilogue of one wave's block of tile (R, C): lane holds column lane%32 and
 // rows (q/4)*8 + (lane/32)*4 + q%4 of each 32 x 32 block.  Histogram mode bins
 // every pair i < j < n; emit mode writes the hits (g2_emit).
@@ -1021,8 +1173,13 @@ __device__ __forceinline__ void g2_tile_epilogue(const GramArgs &a, const f32x16
         else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
     } else {
         constexpr unsigned cap = DEDIC ? g2::DEDIC_RECS : g2::RING_RECS;
+#if PG_G2_EPI == 2
+        if (interior) g2_emit_blocks<true, cap>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
+        else g2_emit_blocks<false, cap>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
+#else
         if (interior) g2_emit_ballot<true, cap>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
         else g2_emit_ballot<false, cap>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
+#endif
     }
 }
 

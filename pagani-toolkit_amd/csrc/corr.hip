@@ -1060,11 +1060,11 @@ __device__ __forceinline__ float g2_pick16(const f32x16 &a, unsigned q)
 }
 
 // The band-free epilogue by accumulator (16 elements at a time): each lane's
-// hit mask, its exclusive prefix over the wave from five bit-planes of the
-// per-lane counts (ballot + mbcnt, no LDS), and each lane writes its own hits
-// (a loop over its mask's bits: as many LDS record writes per accumulator as
-// the most hits any lane has in it, about two, instead of one per element
-// with a hit).  Records, staging area and overflow as g2_emit_ballot.
+// hit mask, then rounds in which every lane with hits left writes its next
+// one at the wave's count plus the round's mbcnt (as many LDS record writes
+// per accumulator as the most hits any lane has in it, about two, instead of
+// one per element with a hit).  Records, staging area and overflow as
+// g2_emit_ballot.
 template <bool BOUNDED, unsigned STAGE_RECS>
 __device__ __forceinline__ void g2_emit_blocks(const GramArgs &a, float lower, const f32x16 (&acc)[g2::BMW][g2::BNW],
                                                int i0, int j0, int lane, int2 *myseg, unsigned long long *s_seg,
@@ -1091,33 +1091,25 @@ __device__ __forceinline__ void g2_emit_blocks(const GramArgs &a, float lower, c
                     m = 2 * m + (hit ? 1u : 0u);
                 }
             }
-            const unsigned cnt = (unsigned)__popc(m);
-            unsigned excl = 0, tot = 0;
-#pragma unroll
-            for (int b = 0; b < 5; b++) {
-                const unsigned long long mb = __builtin_amdgcn_ballot_w64((cnt >> b) & 1u);
-                if (mb) {
-                    excl += __builtin_amdgcn_mbcnt_hi((unsigned)(mb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mb, 0u))
-                            << b;
-                    tot += (unsigned)__popcll(mb) << b;
-                }
-            }
-            if (tot == 0) continue;
-            if (nst + tot <= STAGE_RECS) {
-                unsigned p = nst + excl;
-                const unsigned eb = ((unsigned)(bm * 32) << 6) | (unsigned)(bn * 32);
-                while (m) {
+            // round r: every lane with more than r hits writes its next one;
+            // positions from the round's ballot (mbcnt), the count by popcount
+            const unsigned eb = ((unsigned)(bm * 32) << 6) | (unsigned)(bn * 32);
+            while (true) {
+                const unsigned long long act = __builtin_amdgcn_ballot_w64(m != 0u);
+                if (act == 0ull) break;
+                if (m) {
+                    const unsigned p = __builtin_amdgcn_mbcnt_hi((unsigned)(act >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)act, nst));
                     const unsigned k = 31u - (unsigned)__builtin_clz(m);  // highest set bit: element 15 - k
                     m &= ~(1u << k);
                     const unsigned q = 15u - k;
                     const float v = g2_pick16(acc[bm][bn], q);
                     const unsigned idx = lidx | eb | ((q & 12u) << 7) | ((q & 3u) << 6);
                     const unsigned long long rec = ((unsigned long long)idx << 32) | (unsigned long long)__float_as_uint(v);
-                    asm volatile("ds_write_b64 %0, %1" ::"v"(srec + p * 8u), "v"(rec) : "memory");
-                    p++;
+                    if (p < STAGE_RECS) asm volatile("ds_write_b64 %0, %1" ::"v"(srec + p * 8u), "v"(rec) : "memory");
                 }
+                nst += (unsigned)__popcll(act);
             }
-            nst += tot;
         }
     if (nst > STAGE_RECS) {
         float lo2 = lower;

@@ -103,9 +103,6 @@ static_assert(DMA_WAVE * 4 * 1024 == STAGE, "stage must split into 1 KB wave loa
 constexpr int STAGE_HITS = STAGE / 4 / 12;  // 512
 constexpr int RING_RECS = STAGE / 4 / 8;     // 768 8-byte records per wave in the free ring slot
 constexpr int DEDIC_RECS = 255;              // records per wave in the dedicated area
-#ifndef PG_G2_EPI
-#define PG_G2_EPI 2
-#endif
 }  // namespace g2
 
 __host__ __device__ __forceinline__ int g2_swz(int i) { return ((i >> 2) ^ (i >> 3)) & 1; }
@@ -806,7 +803,7 @@ __device__ __forceinline__ void g2_stage(const f32x16 (&acc)[g2::BMW][g2::BNW], 
 // first wait for their full latency, while these have a whole step of MFMAs
 // to complete before the next wait.
 // Without a band (SPLIT false) the hits are staged as 8-byte records by
-// g2_emit_ballot: r~ bits in the low word, the pair relative to the wave's
+// g2_emit_blocks: r~ bits in the low word, the pair relative to the wave's
 // block origin (i0, j0) as (i - i0) << 6 | (j - j0) in the high word.
 struct G2Pending {
     unsigned te, tc;       // edges, then rechecks, staged
@@ -912,7 +909,7 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
     float *segr = SPLIT ? nullptr : a.seg_r + (long long)blockIdx.x * a.seg_cap;
     if (!SPLIT || tt > (unsigned)g2::STAGE_HITS) {
         // more hits than the staging area holds (or the band-free form, whose
-        // staging is g2_emit_ballot's): each lane writes its own
+        // staging is g2_emit_blocks'): each lane writes its own
         const bool fits = (unsigned long long)be0 + te <= (unsigned long long)a.seg_cap &&
                           (unsigned long long)bc0 + (tt - te) <= (unsigned long long)a.band_cap;
         const unsigned long long bl = be0 + (ex & 0xFFFFu), cl = bc0 + (ex >> 16);
@@ -929,79 +926,6 @@ __device__ __forceinline__ void g2_emit(const GramArgs &a, float lower, const f3
     pd.bc0 = bc0;
     pd.spair = spair;
     pd.sval = sval;
-    pd.on = true;
-}
-
-// Epilogue without a band (every hit is a segment entry with its r~), in one
-// pass: per accumulator element, the wave's ballot of hits; a lane with a hit
-// stages it at the wave's running count plus the hits of lower lanes (mbcnt),
-// the count advancing by the ballot's popcount (scalar).  One compare per
-// element when no lane hits, no counting pass, and one 8-byte LDS write per
-// element with hits (the record of G2Pending).  The staging order differs from
-// g2_emit's (element-major), which the segment does not care about.  More than
-// STAGE_RECS hits: the staged ones are dropped and g2_emit writes them all
-// directly.
-template <bool BOUNDED, unsigned STAGE_RECS>
-__device__ __forceinline__ void g2_emit_ballot(const GramArgs &a, float lower, const f32x16 (&acc)[g2::BMW][g2::BNW],
-                                               int i0, int j0, int lane, int2 *myseg, unsigned long long *s_seg,
-                                               char *stage, G2Pending &pd)
-{
-    const int ib0 = i0 + (lane >> 5) * 4, jb0 = j0 + (lane & 31);
-    // the lane's part of the record index: row (lane / 32) * 4, column lane % 32
-    const unsigned lidx = ((unsigned)(lane >> 5) << 8) | (unsigned)(lane & 31);
-    const unsigned srec = g2_lds_addr(stage);
-    unsigned nst = 0;  // hits so far (wave-uniform); those past STAGE_RECS are not staged
-#pragma unroll
-    for (int bm = 0; bm < g2::BMW; bm++)
-#pragma unroll
-        for (int bn = 0; bn < g2::BNW; bn++)
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const float v = acc[bm][bn][q];
-                bool hit = v >= lower;
-                if (!BOUNDED) {
-                    const int i = ib0 + bm * 32 + (q >> 2) * 8 + (q & 3), j = jb0 + bn * 32;
-                    hit = hit & (j < a.n) & (i < j);
-                }
-                const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
-                if (m != 0ull) {
-                    if (hit) {
-                        const unsigned p = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                                     __builtin_amdgcn_mbcnt_lo((unsigned)m, nst));
-                        // the element's part of the index, materialised inside
-                        // the branch (else every element's is formed up front);
-                        // the lane's and the element's bits are disjoint
-                        unsigned ec = (unsigned)(((bm * 32 + (q >> 2) * 8 + (q & 3)) << 6) | (bn * 32));
-                        asm volatile("" : "+s"(ec));
-                        const unsigned long long rec =
-                            ((unsigned long long)(lidx | ec) << 32) | (unsigned long long)__float_as_uint(v);
-                        if (p < STAGE_RECS)
-                            asm volatile("ds_write_b64 %0, %1" ::"v"(srec + p * 8u), "v"(rec) : "memory");
-                    }
-                    nst += (unsigned)__popcll(m);
-                }
-            }
-    if (nst > STAGE_RECS) {
-        // opaque copies: the direct path recomputes its predicates instead of
-        // the compiler keeping this pass's 128 masks live
-        float lo2 = lower;
-        int i02 = i0, j02 = j0;
-        asm volatile("" : "+v"(lo2), "+v"(i02), "+v"(j02));
-        g2_emit<BOUNDED, false>(a, lo2, acc, i02, j02, lane, myseg, s_seg, stage, pd);
-        return;
-    }
-    if (nst == 0) return;
-    unsigned long long be = 0;
-    if (lane == 0) be = g2_lds_add(s_seg, (unsigned long long)nst);
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    pd.te = nst;
-    pd.tc = 0;
-    pd.be0 = (unsigned)__shfl(be, 0);
-    pd.bc0 = 0;
-    pd.spair = srec;
-    pd.sval = 0;
-    pd.i0 = i0;
-    pd.j0 = j0;
     pd.on = true;
 }
 
@@ -1063,8 +987,10 @@ __device__ __forceinline__ float g2_pick16(const f32x16 &a, unsigned q)
 // hit mask, then rounds in which every lane with hits left writes its next
 // one at the wave's count plus the round's mbcnt (as many LDS record writes
 // per accumulator as the most hits any lane has in it, about two, instead of
-// one per element with a hit).  Records, staging area and overflow as
-// g2_emit_ballot.
+// one per element with a hit).  8-byte records (G2Pending) in the staging
+// area; more than STAGE_RECS hits: the staged ones are dropped and g2_emit
+// writes them all directly.  The staging order (accumulator-major) differs
+// from g2_emit's, which the segment does not care about.
 template <bool BOUNDED, unsigned STAGE_RECS>
 __device__ __forceinline__ void g2_emit_blocks(const GramArgs &a, float lower, const f32x16 (&acc)[g2::BMW][g2::BNW],
                                                int i0, int j0, int lane, int2 *myseg, unsigned long long *s_seg,
@@ -1165,13 +1091,8 @@ __device__ __forceinline__ void g2_tile_epilogue(const GramArgs &a, const f32x16
         else g2_emit<false, true>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
     } else {
         constexpr unsigned cap = DEDIC ? g2::DEDIC_RECS : g2::RING_RECS;
-#if PG_G2_EPI == 2
         if (interior) g2_emit_blocks<true, cap>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
         else g2_emit_blocks<false, cap>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
-#else
-        if (interior) g2_emit_ballot<true, cap>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
-        else g2_emit_ballot<false, cap>(a, lower, acc, i0, j0, lane, myseg, s_seg, stage, pd);
-#endif
     }
 }
 

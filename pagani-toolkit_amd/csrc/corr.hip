@@ -1495,7 +1495,7 @@ __device__ __forceinline__ int2 src_edge(const EdgeSrc &s, int g, long long k)
 
 // Two-level counting sort into CSR.  Both directions of every kept edge go to
 // the bucket of BROWS rows holding their source row: per-segment bucket counts
-// in LDS, a 2-D exclusive scan (segment-major inside each bucket) that fixes
+// in LDS, a 2-D exclusive scan (over the segments inside each bucket) that fixes
 // every segment's output range, a scatter of packed 32-bit entries
 // (row-in-bucket << 25 | column), then one work-group per bucket derives the
 // row offsets and sorts each row by LDS-bitmap rank.  No global atomics.
@@ -1559,8 +1559,8 @@ __device__ __forceinline__ void seg_batches(const EdgeSrc &s, int g, long long c
 #endif
 }
 
-// Work-group x counts source g0 + x; counts of source g land in column g of
-// the bucket-major table cnt[nbk][nsrc].
+// Work-group x counts source g0 + x; counts of source g land in row g of the
+// segment-major table cnt[nsrc][nbk].
 __global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, int g0, int nsrc,
                                                             unsigned *__restrict__ cnt)
 {
@@ -1579,13 +1579,13 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, 
             }
     });
     __syncthreads();
-    for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) cnt[(long long)b * nsrc + g] = sc[b];
+    for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) cnt[(long long)g * nbk + b] = sc[b];
 }
 
 // The window split and the segments' bucket counts in one pass over the
 // segments (one work-group each, as bucket_count): r~ > U2 (more exactly,
 // not r~ <= U2: the CSR filter's test) are definite edges, counted into the
-// segment's column of cnt (both directions) and its definite total dpart[g];
+// segment's row of cnt (both directions) and its definite total dpart[g];
 // L2 <= r~ <= U2 go to the candidate list, staged in LDS and reserved once
 // per work-group (beyond WC_STAGE, one by one).  Without a window
 // (ctl->ok == 0) nothing is kept.  Replaces window_kernel + bucket_count over
@@ -1643,7 +1643,7 @@ __global__ __launch_bounds__(SEG_THREADS) void window_count(EdgeSrc s, const Sel
     for (int o = 32; o > 0; o >>= 1) ndw += __shfl_xor(ndw, o);
     if ((tid & 63) == 0 && ndw) atomicAdd(&s_nd, ndw);
     __syncthreads();
-    for (int b = tid; b < nbk; b += SEG_THREADS) cnt[(long long)b * nsrc + g] = sc[b];
+    for (int b = tid; b < nbk; b += SEG_THREADS) cnt[(long long)g * nbk + b] = sc[b];
     const unsigned nl = min(s_nc, (unsigned)WC_STAGE);
     if (tid == 0) {
         dpart[g] = s_nd;
@@ -1659,32 +1659,38 @@ __global__ __launch_bounds__(SEG_THREADS) void window_count(EdgeSrc s, const Sel
     }
 }
 
-// Per bucket b (one block each): tot[b] and the within-bucket exclusive scan
-// over segments, off[b][g] = sum_{g' < g} cnt[b][g'] (counts bucket-major).
-__global__ __launch_bounds__(256) void bucket_scan_local(const unsigned *__restrict__ cnt, int nsrc,
-                                                         long long *__restrict__ off, long long *__restrict__ tot)
+// Per bucket b: tot[b] and the within-bucket exclusive scan over segments,
+// off[g][b] = sum_{g' < g} cnt[g'][b] (counts and offsets segment-major, so
+// the count kernels write and the scatter reads whole rows).  One block per
+// 32 consecutive buckets: lane b % 32 of each wave sums its bucket over one
+// of SCAN_SG runs of segments (32 buckets' counts of a segment are one
+// 128-byte read), the partial sums are scanned per bucket, then written back.
+constexpr int SCAN_BK = 32, SCAN_SG = 32;
+__global__ __launch_bounds__(SCAN_BK * SCAN_SG) void bucket_scan_local(const unsigned *__restrict__ cnt, int nsrc,
+                                                                         int nbk, long long *__restrict__ off,
+                                                                         long long *__restrict__ tot)
 {
-    __shared__ long long part[256];
-    const int b = blockIdx.x, tid = threadIdx.x;
-    const unsigned *c = cnt + (long long)b * nsrc;
-    const int per = (nsrc + 255) / 256;
-    const int g0 = tid * per, g1 = min(nsrc, g0 + per);
-    long long s = 0;
-    for (int g = g0; g < g1; g++) s += c[g];
-    part[tid] = s;
+    __shared__ long long part[SCAN_SG][SCAN_BK];
+    const int tid = threadIdx.x, bl = tid & (SCAN_BK - 1), sg = tid / SCAN_BK;
+    const int b = blockIdx.x * SCAN_BK + bl;
+    const int per = (nsrc + SCAN_SG - 1) / SCAN_SG;
+    const int g0 = sg * per, g1 = min(nsrc, g0 + per);
+    long long sum = 0;
+    if (b < nbk) {
+#pragma unroll 8
+        for (int g = g0; g < g1; g++) sum += cnt[(long long)g * nbk + b];
+    }
+    part[sg][bl] = sum;
     __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        long long v = tid >= o ? part[tid - o] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+    long long run = 0;
+    for (int q = 0; q < sg; q++) run += part[q][bl];
+    if (b < nbk) {
+        for (int g = g0; g < g1; g++) {
+            off[(long long)g * nbk + b] = run;
+            run += cnt[(long long)g * nbk + b];
+        }
+        if (sg == SCAN_SG - 1) tot[b] = run;
     }
-    long long run = part[tid] - s;
-    for (int g = g0; g < g1; g++) {
-        off[(long long)b * nsrc + g] = run;
-        run += c[g];
-    }
-    if (tid == 255) tot[b] = part[255];
 }
 
 // bstart = exclusive scan of the bucket totals; then off[b][g] += bstart[b].
@@ -1728,7 +1734,7 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_scatter(EdgeSrc s, int nbk
     unsigned *sc = sbase + nbk;
     const int g = blockIdx.x, nsrc = gridDim.x;
     for (int b = threadIdx.x; b < nbk; b += SEG_THREADS) {
-        sbase[b] = (unsigned)(bstart[b] + off[(long long)b * nsrc + g]);
+        sbase[b] = (unsigned)(bstart[b] + off[(long long)g * nbk + b]);
         sc[b] = 0u;
     }
     __syncthreads();
@@ -3190,7 +3196,8 @@ struct CorrRun {
                 bucket_count<<<1, SEG_THREADS, shb, st>>>(src, nbk, (int)nwg, nsrc, bcnt);
             else
                 bucket_count<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(src, nbk, 0, nsrc, bcnt);
-            bucket_scan_local<<<(unsigned)nbk, 256, 0, st>>>(bcnt, nsrc, boff, btot);
+            bucket_scan_local<<<(unsigned)pg_cdiv(nbk, SCAN_BK), SCAN_BK * SCAN_SG, 0, st>>>(bcnt, nsrc, nbk, boff,
+                                                                                          btot);
             bucket_scan_global<<<1, 1024, 0, st>>>(btot, nbk, bstart);
             const size_t shs = (size_t)nbk * (4 + 4);  // bases + cursors
             const void *scatter_fn = weighted ? (const void *)bucket_scatter<true> : (const void *)bucket_scatter<false>;
@@ -3313,7 +3320,7 @@ struct Window {
 // counted as definite (per-block counts in dpart), the rest of the window
 // goes to the candidate list (count in counters[0], at most ccap written),
 // whose FP64 r is then computed (largest |r~ - r64| into counters[5]).
-// count_segments: the split also fills the segments' columns of the CSR
+// count_segments: the split also fills the segments' rows of the CSR
 // build's bucket counts (window_count, the definite edges being exactly the
 // segment entries the build keeps: r~ above ctl's U2); launch_csr then counts
 // only the selected list.
@@ -4187,7 +4194,8 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     // CSR straight into the subject's outputs; the selected list's counts
     const long long nnz = 2 * S.kwant;
     bucket_count<<<1, SEG_THREADS, (size_t)S.nbk * 4, st>>>(es, S.nbk, (int)nwg, nsrcb, P->bcnt);
-    bucket_scan_local<<<(unsigned)S.nbk, 256, 0, st>>>(P->bcnt, nsrcb, P->boff, P->btot);
+    bucket_scan_local<<<(unsigned)pg_cdiv(S.nbk, SCAN_BK), SCAN_BK * SCAN_SG, 0, st>>>(P->bcnt, nsrcb, S.nbk,
+                                                                                     P->boff, P->btot);
     bucket_scan_global<<<1, 1024, 0, st>>>(P->btot, S.nbk, P->bstart);
     bucket_scatter<false><<<(unsigned)nsrcb, SEG_THREADS, (size_t)S.nbk * 8, st>>>(es, S.nbk, P->boff, P->bstart,
                                                                                    P->ent, nullptr, P->z64, S.t, nnz);

@@ -2496,7 +2496,8 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
                                                       long long k, const unsigned long long *__restrict__ dsrc, int dn,
                                                       int2 *__restrict__ extra, unsigned long long *__restrict__ extra_cnt,
                                                       double *__restrict__ tstar, unsigned long long *__restrict__ flips,
-                                                      int *__restrict__ err, SelHist shh)
+                                                      int *__restrict__ err, SelHist shh,
+                                                      unsigned long long *__restrict__ def_out)
 {
     __shared__ unsigned hist[SEL_NB];
     __shared__ unsigned long long mkey[SEL_CAP];
@@ -2523,6 +2524,7 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
     for (int g = tid; g < dn; g += blockDim.x) dpart += dsrc[g];
     atomicAdd(&s_def, dpart);
     __syncthreads();
+    if (def_out && tid == 0) *def_out = s_def;  // the definite count, for the host's check
     const long long c = (long long)min((unsigned long long)cand_cap, *cand_cnt);
     const long long need = k - (long long)s_def;
     if (need < 1 || need > c || (long long)*cand_cnt > cand_cap) {
@@ -3371,8 +3373,7 @@ pg_status window_select(CorrRun &R, const SelCtl *ctl, long long ccap, long long
     hipStream_t st = R.st;
     unsigned long long *c = R.counters;
     select_kernel<<<1, 1024, 0, st>>>(win->cand, win->cand64, win->cand_r, c + 0, ccap, kwant, win->dpart, win->nd,
-                                      win->extra, c + 1, tstar_d, c + 2, (int *)(c + 4), win->sh);
-    definite_sum<<<1, 1024, 0, st>>>(win->dpart, win->nd, c + 3);
+                                      win->extra, c + 1, tstar_d, c + 2, (int *)(c + 4), win->sh, c + 3);
     PG_KERNEL_CHECK(R.ctx);
     return PG_OK;
 }
@@ -3577,7 +3578,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
                                R.band_r, counters + 5);
             select_kernel<<<1, 1024, 0, st>>>(R.bandl, band64, R.band_r, counters + 0, R.band_cap, kwant, R.seg_cnt,
                                               (int)nwg, extra, extra_cnt, tstar_d, flips, (int *)(counters + 4),
-                                              SelHist{nullptr, nullptr, 0.0});
+                                              SelHist{nullptr, nullptr, 0.0}, nullptr);
             PG_KERNEL_CHECK(ctx);
             PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof R.hcnt, hipMemcpyDeviceToHost, st));
             PG_HIP(ctx, hipMemcpyAsync(&inf.t_star, tstar_d, 8, hipMemcpyDeviceToHost, st));
@@ -4189,8 +4190,7 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     launch_recheck(ctx, st, P->cand, c + 0, S.ccap, P->z64, S.n, S.t, P->cand64, P->cand_r, c + 5, shh);
     const int nd = (int)nwg;
     select_kernel<<<1, 1024, 0, st>>>(P->cand, P->cand64, P->cand_r, c + 0, S.ccap, S.kwant, P->dpart, nd, P->extra,
-                                      c + 1, P->tstar, c + 2, (int *)(c + 4), shh);
-    definite_sum<<<1, 1024, 0, st>>>(P->dpart, nd, c + 3);
+                                      c + 1, P->tstar, c + 2, (int *)(c + 4), shh, c + 3);
     // CSR straight into the subject's outputs; the selected list's counts
     const long long nnz = 2 * S.kwant;
     bucket_count<<<1, SEG_THREADS, (size_t)S.nbk * 4, st>>>(es, S.nbk, (int)nwg, nsrcb, P->bcnt);

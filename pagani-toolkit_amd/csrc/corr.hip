@@ -1303,6 +1303,15 @@ __device__ __forceinline__ void sel_range(const SelHist &s, double &top, double 
     top = hi;
     scale = hi > lo ? (double)SEL_NB / (hi - lo) : 1.0;
 }
+// The cut a selection found: the selected candidates are those whose
+// (key, (i, j)) is at most (k1, k2); ok = 0 when the selection failed.
+struct SelCut {
+    unsigned long long k1, k2;
+    double ts;
+    long long c;
+    int ok;
+};
+
 // clamp(floor((top - r) * scale)): non-decreasing as r decreases, i.e. in the
 // selection key (r descending), so the bins cut the key order into ranges
 __device__ __forceinline__ int sel_bin(double r, double top, double scale)
@@ -2497,7 +2506,7 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
                                                       int2 *__restrict__ extra, unsigned long long *__restrict__ extra_cnt,
                                                       double *__restrict__ tstar, unsigned long long *__restrict__ flips,
                                                       int *__restrict__ err, SelHist shh,
-                                                      unsigned long long *__restrict__ def_out)
+                                                      unsigned long long *__restrict__ def_out, SelCut *__restrict__ cut)
 {
     __shared__ unsigned hist[SEL_NB];
     __shared__ unsigned long long mkey[SEL_CAP];
@@ -2525,6 +2534,7 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
     atomicAdd(&s_def, dpart);
     __syncthreads();
     if (def_out && tid == 0) *def_out = s_def;  // the definite count, for the host's check
+    if (cut && tid == 0) cut->ok = 0;
     const long long c = (long long)min((unsigned long long)cand_cap, *cand_cnt);
     const long long need = k - (long long)s_def;
     if (need < 1 || need > c || (long long)*cand_cnt > cand_cap) {
@@ -2685,6 +2695,16 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
     __syncthreads();
     const unsigned long long k1 = s_k1, k2 = s_k2;
     const double ts = s_ts;
+    if (cut) {  // the selected candidates are written by select_write
+        if (tid == 0) {
+            cut->k1 = k1;
+            cut->k2 = k2;
+            cut->ts = ts;
+            cut->c = c;
+            cut->ok = 1;
+        }
+        return;
+    }
     // the selected candidates: this thread's count (and decision flips),
     // one block scan and one reservation, then the writes
     auto selected = [&](long long q, const int2 &e, double rq) {
@@ -2739,6 +2759,47 @@ __global__ __launch_bounds__(1024) void select_kernel(const int2 *__restrict__ c
     __syncthreads();
     if (tid == 0) atomicAdd(extra_cnt, (unsigned long long)s_m);
     atomicAdd(flips, fl);
+}
+
+// The selected candidates of a select_kernel run that left its cut in *cut:
+// every candidate whose (key, (i, j)) is at most (k1, k2), appended to extra
+// with one reservation per wave (the list's order is free: the CSR build
+// sorts every row); decision flips against t* into *flips.  Grid-stride over
+// the candidates, trip count uniform over each block.
+__global__ __launch_bounds__(256) void select_write(const int2 *__restrict__ cand, const double *__restrict__ r64,
+                                                   const float *__restrict__ rfast, const SelCut *__restrict__ cut,
+                                                   int2 *__restrict__ extra,
+                                                   unsigned long long *__restrict__ extra_cnt,
+                                                   unsigned long long *__restrict__ flips)
+{
+    if (!cut->ok) return;
+    const long long c = cut->c;
+    const unsigned long long k1 = cut->k1, k2 = cut->k2;
+    const double ts = cut->ts;
+    const int lane = threadIdx.x & 63;
+    unsigned long long fl = 0;
+    for (long long qb = (long long)blockIdx.x * 256; qb < c; qb += (long long)gridDim.x * 256) {
+        const long long q = qb + threadIdx.x;
+        bool sel = false;
+        int2 e = make_int2(0, 0);
+        if (q < c) {
+            e = cand[q];
+            const double rq = r64[q];
+            const unsigned long long kq = key_r_desc(rq);
+            const unsigned long long k2q = ((unsigned long long)(unsigned)e.x << 32) | (unsigned)e.y;
+            sel = kq < k1 || (kq == k1 && k2q <= k2);
+            fl += ((double)rfast[q] > ts) != (rq > ts);
+        }
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(sel);
+        if (m == 0ull) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(extra_cnt, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0);
+        if (sel)
+            extra[base + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = e;
+    }
+    for (int o = 32; o > 0; o >>= 1) fl += __shfl_xor(fl, o);
+    if (lane == 0 && fl) atomicAdd(flips, fl);
 }
 
 // out = sum of part[0..np) (one block)
@@ -3372,8 +3433,12 @@ pg_status window_select(CorrRun &R, const SelCtl *ctl, long long ccap, long long
     PG_TRY(window_candidates(R, ctl, ccap, win, count_segments));
     hipStream_t st = R.st;
     unsigned long long *c = R.counters;
+    SelCut *cut;
+    PG_TRY(pg_bufT(R.ctx, "corr.selcut", 1, &cut));
     select_kernel<<<1, 1024, 0, st>>>(win->cand, win->cand64, win->cand_r, c + 0, ccap, kwant, win->dpart, win->nd,
-                                      win->extra, c + 1, tstar_d, c + 2, (int *)(c + 4), win->sh, c + 3);
+                                      win->extra, c + 1, tstar_d, c + 2, (int *)(c + 4), win->sh, c + 3, cut);
+    select_write<<<(unsigned)std::min<long long>(pg_cdiv(ccap, 256), 4LL * std::max(R.ctx->num_cu, 1)), 256, 0, st>>>(
+        win->cand, win->cand64, win->cand_r, cut, win->extra, c + 1, c + 2);
     PG_KERNEL_CHECK(R.ctx);
     return PG_OK;
 }
@@ -3578,7 +3643,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
                                R.band_r, counters + 5);
             select_kernel<<<1, 1024, 0, st>>>(R.bandl, band64, R.band_r, counters + 0, R.band_cap, kwant, R.seg_cnt,
                                               (int)nwg, extra, extra_cnt, tstar_d, flips, (int *)(counters + 4),
-                                              SelHist{nullptr, nullptr, 0.0}, nullptr);
+                                              SelHist{nullptr, nullptr, 0.0}, nullptr, nullptr);
             PG_KERNEL_CHECK(ctx);
             PG_HIP(ctx, hipMemcpyAsync(hcnt, counters, sizeof R.hcnt, hipMemcpyDeviceToHost, st));
             PG_HIP(ctx, hipMemcpyAsync(&inf.t_star, tstar_d, 8, hipMemcpyDeviceToHost, st));
@@ -3971,6 +4036,7 @@ struct PgCohortCorr {
     float *seg_r = nullptr, *cand_r = nullptr, *band_r = nullptr;
     SelCtl *ctl = nullptr;
     unsigned *selhist = nullptr;  // the candidates' selection histogram
+    SelCut *selcut = nullptr;     // and the cut found on it
     unsigned *bcnt = nullptr, *ent = nullptr;
     long long *boff = nullptr, *bstart = nullptr, *btot = nullptr;
     int *ctmp = nullptr;
@@ -4082,6 +4148,7 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
     PG_TRY(pg_bufT(ctx, "coh.dpart", (size_t)max_dpart, &P->dpart));
     PG_TRY(pg_bufT(ctx, "coh.tstar", 2, &P->tstar));
     PG_TRY(pg_bufT(ctx, "coh.selhist", (size_t)SEL_NB, &P->selhist));
+    PG_TRY(pg_bufT(ctx, "coh.selcut", 1, &P->selcut));
     PG_TRY(pg_bufT(ctx, "coh.bcnt", (size_t)max_bsrc, &P->bcnt));
     PG_TRY(pg_bufT(ctx, "coh.boff", (size_t)max_bsrc, &P->boff));
     PG_TRY(pg_bufT(ctx, "coh.bstart", (size_t)max_nbk + 1, &P->bstart));
@@ -4190,7 +4257,9 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     launch_recheck(ctx, st, P->cand, c + 0, S.ccap, P->z64, S.n, S.t, P->cand64, P->cand_r, c + 5, shh);
     const int nd = (int)nwg;
     select_kernel<<<1, 1024, 0, st>>>(P->cand, P->cand64, P->cand_r, c + 0, S.ccap, S.kwant, P->dpart, nd, P->extra,
-                                      c + 1, P->tstar, c + 2, (int *)(c + 4), shh, c + 3);
+                                      c + 1, P->tstar, c + 2, (int *)(c + 4), shh, c + 3, P->selcut);
+    select_write<<<(unsigned)std::min<long long>(pg_cdiv(S.ccap, 256), 4LL * std::max(ctx->num_cu, 1)), 256, 0, st>>>(
+        P->cand, P->cand64, P->cand_r, P->selcut, P->extra, c + 1, c + 2);
     // CSR straight into the subject's outputs; the selected list's counts
     const long long nnz = 2 * S.kwant;
     bucket_count<<<1, SEG_THREADS, (size_t)S.nbk * 4, st>>>(es, S.nbk, (int)nwg, nsrcb, P->bcnt);

@@ -1597,7 +1597,7 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, 
 // segment's row of cnt (both directions) and its definite total dpart[g];
 // L2 <= r~ <= U2 go to the candidate list, staged in LDS and reserved once
 // per work-group (beyond WC_STAGE, one by one).  Without a window
-// (ctl->ok == 0) nothing is kept.  Replaces window_kernel + bucket_count over
+// (ctl->ok == 0) nothing is kept.  Does the work of window_kernel + bucket_count over
 // the segments; the selected list's counts follow from bucket_count (g0 =
 // nseg) once it exists.
 constexpr int WC_STAGE = 1024;

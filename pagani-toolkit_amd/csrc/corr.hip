@@ -220,13 +220,35 @@ __global__ __launch_bounds__(256) void zscore_fused(const float *__restrict__ x,
     __shared__ double smu[ZV], ssd[ZV];
     constexpr int LD = ZV + 1;
     const int i0 = blockIdx.x * ZV, tid = threadIdx.x;
+    // every load of the thread issued before the first LDS store (a loop
+    // that stores each value as it arrives waits for each load in turn)
     if (layout == 0) {  // time-major: ZV consecutive voxels of one time point per 32 lanes
-        const int ii = tid & (ZV - 1), i = i0 + ii;
-        for (int l = tid / ZV; l < t; l += 256 / ZV) zt[l * LD + ii] = i < n ? x[(long long)l * n + i] : 0.0f;
+        constexpr int LS = 256 / ZV, LN = (ZS_TMAX + LS - 1) / LS;
+        const int ii = tid & (ZV - 1), i = i0 + ii, l0 = tid / ZV;
+        float v[LN];
+#pragma unroll
+        for (int k = 0; k < LN; k++) {
+            const int l = l0 + k * LS;
+            v[k] = i < n && l < t ? x[(long long)l * n + i] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < LN; k++)
+            if (l0 + k * LS < t) zt[(l0 + k * LS) * LD + ii] = v[k];
     } else {  // voxel-major: one voxel's series per wave and pass
+        constexpr int LN = (ZS_TMAX + 63) / 64;
+        const int lane = tid & 63;
+#pragma unroll
         for (int ii = tid >> 6; ii < ZV; ii += 4) {
             const int i = i0 + ii;
-            for (int l = tid & 63; l < t; l += 64) zt[l * LD + ii] = i < n ? x[(long long)i * t + l] : 0.0f;
+            float v[LN];
+#pragma unroll
+            for (int k = 0; k < LN; k++) {
+                const int l = lane + 64 * k;
+                v[k] = i < n && l < t ? x[(long long)i * t + l] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < LN; k++)
+                if (lane + 64 * k < t) zt[(lane + 64 * k) * LD + ii] = v[k];
         }
     }
     __syncthreads();

@@ -45,6 +45,13 @@ const char *pg_status_string(pg_status s);
 int pg_abi_version(void);
 /* Wait for all work queued by this context. */
 pg_status pg_synchronize(pg_ctx *ctx);
+/* Stage timing of pg_corr_build / pg_corr_emit / pg_corr_select (the ms_*
+ * fields of pg_corr_info).  No counterpart in the reference: each timing
+ * event leaves the device idle for a few microseconds, so callers that do not
+ * read the stage times (batch pipelines, timing loops) can drop them.
+ * level 2 (default): every stage; 1: only ms_gram_hist and ms_gram_emit (the
+ * Gram kernels); 0: none.  Fields of untimed stages are 0. */
+pg_status pg_set_timing(pg_ctx *ctx, int32_t level);
 
 /* ------------------------------------------------------------ construction */
 

@@ -115,6 +115,15 @@ __host__ __device__ __forceinline__ long long g2_index(int i, int l, int npad)
 }
 
 
+// Stage-timing event idx of pg_corr_build on stream st, recorded when the
+// context's timing level asks for it (events 6..9 bracket the Gram kernels,
+// kept at level 1; the others mark the stage boundaries, level 2).  Each
+// recorded event costs the device a few microseconds of idle time.
+inline hipError_t stage_event(pg_ctx *ctx, int idx, hipStream_t st)
+{
+    return ctx->timing >= (idx >= 6 ? 1 : 2) ? hipEventRecord(ctx->ev[idx], st) : hipSuccess;
+}
+
 // Zero fill of one or two buffers of 32-bit words (grid over the larger).
 __global__ void fill_u32(uint32_t *__restrict__ p, long long count, uint32_t *__restrict__ q = nullptr,
                          long long qcount = 0)
@@ -3070,7 +3079,7 @@ struct CorrRun {
         const float *dbold = nullptr;
         PG_TRY(pg_in(ctx, bold, (size_t)n * nt, "corr.bold", &dbold));
 
-        PG_HIP(ctx, hipEventRecord(ctx->ev[0], st));
+        PG_HIP(ctx, stage_event(ctx, 0, st));
         if (fused) {
             const size_t shz = (size_t)t * (ZV + 1) * 4;
             PG_HIP(ctx, hipFuncSetAttribute((const void *)zscore_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -3089,7 +3098,7 @@ struct CorrRun {
                                              prec == PREC_BF16X3 ? (uint16_t *)p1 : nullptr, ldp, npad);
             PG_KERNEL_CHECK(ctx);
         }
-        PG_HIP(ctx, hipEventRecord(ctx->ev[1], st));
+        PG_HIP(ctx, stage_event(ctx, 1, st));
 
         // Gram launch geometry: PG_GRAM_WG_PER_CU work-groups per CU, a
         // multiple of the XCD count (see TileWalk); supertiles (SR, SC)
@@ -3155,7 +3164,7 @@ struct CorrRun {
                                    : (const void *)gram_kernel<PREC_F32, MODE_EMIT>;
         PG_HIP(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         const int e0 = mode == MODE_HIST ? 6 : 8;
-        PG_HIP(ctx, hipEventRecord(ctx->ev[e0], st));
+        PG_HIP(ctx, stage_event(ctx, e0, st));
         if (g2path) {
             if (mode == MODE_HIST) gram2_kernel<MODE_HIST><<<(unsigned)nwg_hist, THREADS, shm, st>>>(ga);
             else if (ga.upper >= -1.0f) gram2_kernel<MODE_EMIT, true><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
@@ -3166,7 +3175,7 @@ struct CorrRun {
             else gram_kernel<PREC_F32, MODE_EMIT><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
         }
         PG_KERNEL_CHECK(ctx);
-        PG_HIP(ctx, hipEventRecord(ctx->ev[e0 + 1], st));
+        PG_HIP(ctx, stage_event(ctx, e0 + 1, st));
         if (mode == MODE_HIST) ran_hist = true;
         else ran_emit = true;
         if (ga.sstep <= 1) gram_passes++;
@@ -3322,9 +3331,9 @@ struct CorrRun {
     // launch, wait, and check row[n]
     pg_status finish_csr(const EdgeSrc &src, long long nnz)
     {
-        PG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+        PG_HIP(ctx, stage_event(ctx, 4, st));
         PG_TRY(launch_csr(src, nnz));
-        PG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+        PG_HIP(ctx, stage_event(ctx, 5, st));
         PG_HIP(ctx, hipStreamSynchronize(st));
         int rn = 0;
         PG_HIP(ctx, hipMemcpy(&rn, row + n, 4, hipMemcpyDeviceToHost));
@@ -3350,18 +3359,23 @@ struct CorrRun {
     // Stage times from the events, the result kept in the context.
     void finish(pg_corr_info &inf)
     {
+        // stages not timed at the context's timing level report 0
         float ms = 0.0f, ms_hist = 0.0f;
-        (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
-        inf.ms_zscore = ms;
-        (void)hipEventElapsedTime(&ms_hist, ctx->ev[1], ctx->ev[2]);
-        (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
-        inf.ms_gram = ms + ms_hist;
-        (void)hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]);
-        inf.ms_select = ms;
-        (void)hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
-        inf.ms_csr = ms;
-        if (ran_hist && hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess) inf.ms_gram_hist = ms;
-        if (ran_emit && hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]) == hipSuccess) inf.ms_gram_emit = ms;
+        if (ctx->timing >= 2) {
+            (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+            inf.ms_zscore = ms;
+            (void)hipEventElapsedTime(&ms_hist, ctx->ev[1], ctx->ev[2]);
+            (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+            inf.ms_gram = ms + ms_hist;
+            (void)hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]);
+            inf.ms_select = ms;
+            (void)hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
+            inf.ms_csr = ms;
+        }
+        if (ctx->timing >= 1 && ran_hist && hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess)
+            inf.ms_gram_hist = ms;
+        if (ctx->timing >= 1 && ran_emit && hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]) == hipSuccess)
+            inf.ms_gram_emit = ms;
         inf.gram_passes = gram_passes;
         ctx->cn = n;
         ctx->ct = t;
@@ -3528,10 +3542,10 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         // ---------------------------------------------------- r threshold
         R.ga.upper = round_up_f((double)thr_eff + band);
         R.ga.lower = round_down_f((double)thr_eff - band);
-        PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+        PG_HIP(ctx, stage_event(ctx, 2, st));
         const long long seg0 = pg_cdiv(std::max<long long>((long long)(m1 * 0.03), 1 << 20), nwg) + 4096;
         PG_TRY(R.emit_pass(seg0, 1 << 22, false));
-        PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+        PG_HIP(ctx, stage_event(ctx, 3, st));
         long long definite = 0;
         for (auto c : R.hseg) definite += (long long)c;
         const long long nband = (long long)hcnt[0];
@@ -3555,10 +3569,10 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
         PG_TRY(R.finish_csr(R.edges(extra, extra_cnt, std::max<long long>(nband, 1)), inf.nnz));
     } else if (kwant == 0 || n < 2) {
         // ---------------------------------------------------- nothing asked
-        PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+        PG_HIP(ctx, stage_event(ctx, 2, st));
         PG_TRY(clear_dev(ctx, st, counters, 16 * sizeof(unsigned long long), R.seg_cnt,
                          nwg * sizeof(unsigned long long)));
-        PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+        PG_HIP(ctx, stage_event(ctx, 3, st));
         PG_TRY(pg_bufT(ctx, "corr.extra", 1, &extra));
         PG_TRY(pg_bufT(ctx, "corr.seg", 1, &R.seg));
         R.seg_cap = 0;
@@ -3589,7 +3603,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             PG_TRY(pg_bufT(ctx, "corr.selctl", 1, &ctl));
             PG_TRY(R.hist_launch(stride));
             sample_bound<<<1, 1024, 0, st>>>(R.htot, f, ctl);
-            PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+            PG_HIP(ctx, stage_event(ctx, 2, st));
             R.ga.lower_dev = &ctl->lower;
             R.ga.upper = -4.0f;  // every hit is a segment entry
             const double expect = std::min(1.0, f * 1.2 + 2.0 * w) * (double)m1;
@@ -3601,7 +3615,7 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             PG_TRY(fine_hist(R, ctl, &h2t));
             window_bounds<<<1, 1024, 0, st>>>(h2t, kwant, band, ctl);
             PG_KERNEL_CHECK(ctx);
-            PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+            PG_HIP(ctx, stage_event(ctx, 3, st));
             const long long ccap = std::min<long long>(1 << 22, std::max<long long>(1 << 16, kwant / 4));
             Window win;
             PG_TRY(window_select(R, ctl, ccap, kwant, tstar_d, &win, true));
@@ -3609,9 +3623,9 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             EdgeSrc es = R.edges(extra, extra_cnt, ccap);
             es.seg_r = R.seg_r;
             es.keep_above_dev = &ctl->U2;
-            PG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+            PG_HIP(ctx, stage_event(ctx, 4, st));
             PG_TRY(R.launch_csr(es, 2 * kwant, true));
-            PG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+            PG_HIP(ctx, stage_event(ctx, 5, st));
             // every check's inputs gathered on the device, one small copy back
             SpReport *rep_d;
             PG_TRY(pg_bufT(ctx, "corr.report", 1, &rep_d));
@@ -3653,9 +3667,9 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             const double hi_edge = (bstar + 1) * w - 1.0 + 1e-6;
             R.ga.upper = round_up_f(hi_edge + 2.0 * band);
             R.ga.lower = round_down_f(lo_edge - 2.0 * band);
-            PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+            PG_HIP(ctx, stage_event(ctx, 2, st));
             PG_TRY(R.emit_pass(pg_cdiv(kwant, nwg) * 3 / 2 + 4096, 1 << 22, false));
-            PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+            PG_HIP(ctx, stage_event(ctx, 3, st));
             const long long nband = (long long)hcnt[0];
             double *band64;
             PG_TRY(pg_bufT(ctx, "corr.band64", (size_t)std::max<long long>(nband, 1), &band64));
@@ -3760,11 +3774,11 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
         }
         R.ga.lower = E.lower;
         R.ga.upper = -4.0f;
-        PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+        PG_HIP(ctx, stage_event(ctx, 2, st));
         R.dedic = dedic_fits(dens);
         PG_TRY(R.emit_pass(seg0, 1 << 10, true));
         R.dedic = false;
-        PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+        PG_HIP(ctx, stage_event(ctx, 3, st));
         hc.lower = E.lower;
         hc.ok = 1;
         PG_HIP(ctx, hipMemcpyAsync(E.ctl, &hc, sizeof hc, hipMemcpyHostToDevice, st));
@@ -3784,7 +3798,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
             const double f = std::min(1.0, 1.15 * p + 6.0 * std::sqrt(p * (1.0 - p) / meff) + 1.0 / meff);
             PG_TRY(R.hist_launch(stride));
             sample_bound<<<1, 1024, 0, st>>>(R.htot, f, E.ctl);
-            PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+            PG_HIP(ctx, stage_event(ctx, 2, st));
             R.ga.lower_dev = &E.ctl->lower;
             R.ga.upper = -4.0f;
             const double expect = std::min(1.0, f * 1.2 + 2.0 * w) * (double)m1;
@@ -3827,7 +3841,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
             hc.lower = E.lower;
             hc.ok = 1;
             PG_HIP(ctx, hipMemcpyAsync(E.ctl, &hc, sizeof hc, hipMemcpyHostToDevice, st));
-            PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+            PG_HIP(ctx, stage_event(ctx, 2, st));
             R.ga.lower = E.lower;
             R.ga.upper = -4.0f;
             PG_TRY(R.emit_pass(pg_cdiv(above + (long long)h[bstar], nwg) * 3 / 2 + 4096, 1 << 10, true));
@@ -3840,7 +3854,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
         }
         if (E.kwant_max == 0 || n < 2) {
             // nothing to keep: empty segments, no Gram pass
-            PG_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+            PG_HIP(ctx, stage_event(ctx, 2, st));
             R.seg_cap = 64;
             PG_TRY(pg_bufT(ctx, "corr.seg", (size_t)(nwg * R.seg_cap), &R.seg));
             PG_TRY(pg_bufT(ctx, "corr.segr", (size_t)(nwg * R.seg_cap), &R.seg_r));
@@ -3849,7 +3863,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
             hc.ok = 1;
             PG_HIP(ctx, hipMemcpyAsync(E.ctl, &hc, sizeof hc, hipMemcpyHostToDevice, st));
         }
-        PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+        PG_HIP(ctx, stage_event(ctx, 3, st));
     }
     PG_HIP(ctx, hipStreamSynchronize(st));
     long long kept = 0;
@@ -3857,14 +3871,18 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
     for (auto c : R.hseg) kept += (long long)c;
     inf.band_pairs = kept;  // pairs kept for the selections
     float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
-    inf.ms_zscore = ms;
-    float ms_hist = 0.0f;
-    (void)hipEventElapsedTime(&ms_hist, ctx->ev[1], ctx->ev[2]);
-    (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
-    inf.ms_gram = ms + ms_hist;
-    if (R.ran_hist && hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess) inf.ms_gram_hist = ms;
-    if (R.ran_emit && hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]) == hipSuccess) inf.ms_gram_emit = ms;
+    if (ctx->timing >= 2) {
+        (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+        inf.ms_zscore = ms;
+        float ms_hist = 0.0f;
+        (void)hipEventElapsedTime(&ms_hist, ctx->ev[1], ctx->ev[2]);
+        (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+        inf.ms_gram = ms + ms_hist;
+    }
+    if (ctx->timing >= 1 && R.ran_hist && hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess)
+        inf.ms_gram_hist = ms;
+    if (ctx->timing >= 1 && R.ran_emit && hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]) == hipSuccess)
+        inf.ms_gram_emit = ms;
     inf.gram_passes = R.gram_passes;
     inf.nnz = -1;
     E.base = inf;
@@ -3896,7 +3914,7 @@ extern "C" pg_status pg_corr_select(pg_ctx *ctx, float thr, pg_corr_info *info)
     hc.lower = E.lower;
     hc.ok = 1;
     long long ccap = 1 << 16;
-    PG_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+    PG_HIP(ctx, stage_event(ctx, 3, st));
     if (E.kind == PG_THR_R) {
         if (!(thr >= -1.0f && thr <= 1.0f)) return pg_fail(ctx, PG_ETHRESH, "r threshold %g outside [-1, 1]", (double)thr);
         const float thr_eff = thr_effective(thr, R.weighted);
@@ -3959,9 +3977,9 @@ extern "C" pg_status pg_corr_select(pg_ctx *ctx, float thr, pg_corr_info *info)
                 EdgeSrc es = R.edges(win.extra, c + 1, ccap);
                 es.seg_r = R.seg_r;
                 es.keep_above_dev = &E.ctl->U2;
-                PG_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+                PG_HIP(ctx, stage_event(ctx, 4, st));
                 PG_TRY(R.launch_csr(es, 2 * kwant));
-                PG_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+                PG_HIP(ctx, stage_event(ctx, 5, st));
                 SelCtl hw;
                 int rn = -1;
                 PG_HIP(ctx, hipMemcpyAsync(R.hcnt, c, sizeof R.hcnt, hipMemcpyDeviceToHost, st));
@@ -3990,8 +4008,8 @@ extern "C" pg_status pg_corr_select(pg_ctx *ctx, float thr, pg_corr_info *info)
     }
     PG_TRY(check_band(ctx, inf));
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]) == hipSuccess) inf.ms_select = ms;
-    if (hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]) == hipSuccess) inf.ms_csr = ms;
+    if (ctx->timing >= 2 && hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]) == hipSuccess) inf.ms_select = ms;
+    if (ctx->timing >= 2 && hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]) == hipSuccess) inf.ms_csr = ms;
     ctx->cn = n;
     ctx->ct = R.t;
     ctx->cnnz = inf.nnz;

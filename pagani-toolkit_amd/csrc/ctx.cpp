@@ -154,4 +154,12 @@ pg_status pg_synchronize(pg_ctx *ctx)
     return PG_OK;
 }
 
+pg_status pg_set_timing(pg_ctx *ctx, int32_t level)
+{
+    if (!ctx) return PG_EINVAL;
+    if (level < 0 || level > 2) return pg_fail(ctx, PG_EINVAL, "timing level %d outside 0..2", (int)level);
+    ctx->timing = level;
+    return PG_OK;
+}
+
 }  // extern "C"

@@ -103,6 +103,7 @@ _SIGS = {
     "pg_status_string": (C.c_char_p, [C.c_int]),
     "pg_abi_version": (C.c_int, []),
     "pg_synchronize": (C.c_int, [_VP]),
+    "pg_set_timing": (C.c_int, [_VP, C.c_int32]),
     "pg_corr_build": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.POINTER(_Opts), C.POINTER(_Info)]),
     "pg_corr_emit": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.POINTER(_Opts), C.POINTER(_Info)]),
     "pg_corr_select": (C.c_int, [_VP, C.c_float, C.POINTER(_Info)]),
@@ -217,6 +218,11 @@ class Context:
 
     def synchronize(self):
         self._check(self._lib.pg_synchronize(self._h))
+
+    def set_timing(self, level: int):
+        """Stage timing of corr_build: 2 every stage (default), 1 the Gram
+        kernels only, 0 none (untimed stages report 0)."""
+        self._check(self._lib.pg_set_timing(self._h, int(level)))
 
     # ------------------------------------------------------- construction
     def corr_build(self, bold, n: int, t: int, thr: float, *, sparsity: bool = False,

@@ -13,13 +13,16 @@
 // z-scores, sequential sum, no FMA).  The MFMA pass yields r~ with a proven
 // bound |r~ - r64| <= band; only pairs with |r~ - thr| <= band are recomputed in
 // FP64, so the edge set equals the FP64 one exactly.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "pg_internal.hpp"
@@ -114,6 +117,21 @@ __host__ __device__ __forceinline__ long long g2_index(int i, int l, int npad)
     return ((long long)c * npad + i) * 16 + ((h ^ g2_swz(i)) << 3) + (l & 7);
 }
 
+
+// The dynamic LDS limit of a kernel raised to at least `bytes` on the
+// context's device; calls that would not raise it are skipped (the attribute
+// is per device and process, and only ever raised here).
+inline hipError_t set_shared_limit(pg_ctx *ctx, const void *fn, size_t bytes)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, size_t> set;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t &cur = set[{ctx->device, fn}];
+    if (cur >= bytes) return hipSuccess;
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) cur = bytes;
+    return e;
+}
 
 // Stage-timing event idx of pg_corr_build on stream st, recorded when the
 // context's timing level asks for it (events 6..9 bracket the Gram kernels,
@@ -3082,8 +3100,7 @@ struct CorrRun {
         PG_HIP(ctx, stage_event(ctx, 0, st));
         if (fused) {
             const size_t shz = (size_t)t * (ZV + 1) * 4;
-            PG_HIP(ctx, hipFuncSetAttribute((const void *)zscore_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                            (int)shz));
+            PG_HIP(ctx, set_shared_limit(ctx, (const void *)zscore_fused, (int)shz));
             zscore_fused<<<(unsigned)pg_cdiv(npad, ZV), 256, shz, st>>>(
                 dbold, opts->layout, n, t, z64, prec == PREC_F32 ? (float *)p0 : nullptr,
                 prec == PREC_BF16X3 ? (uint16_t *)p0 : nullptr, prec == PREC_BF16X3 ? (uint16_t *)p1 : nullptr, ldp,
@@ -3162,20 +3179,15 @@ struct CorrRun {
         else
             fn = mode == MODE_HIST ? (const void *)gram_kernel<PREC_F32, MODE_HIST>
                                    : (const void *)gram_kernel<PREC_F32, MODE_EMIT>;
-        PG_HIP(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        PG_HIP(ctx, set_shared_limit(ctx, fn, shm));
+        // the kernel's own start and stop events (ev[e0], ev[e0 + 1]) ride on
+        // its dispatch: no separate event packets around it
         const int e0 = mode == MODE_HIST ? 6 : 8;
-        PG_HIP(ctx, stage_event(ctx, e0, st));
-        if (g2path) {
-            if (mode == MODE_HIST) gram2_kernel<MODE_HIST><<<(unsigned)nwg_hist, THREADS, shm, st>>>(ga);
-            else if (ga.upper >= -1.0f) gram2_kernel<MODE_EMIT, true><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
-            else if (dedic) gram2_kernel<MODE_EMIT, false, true><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
-            else gram2_kernel<MODE_EMIT, false><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
-        } else {
-            if (mode == MODE_HIST) gram_kernel<PREC_F32, MODE_HIST><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
-            else gram_kernel<PREC_F32, MODE_EMIT><<<(unsigned)nwg, THREADS, shm, st>>>(ga);
-        }
-        PG_KERNEL_CHECK(ctx);
-        PG_HIP(ctx, stage_event(ctx, e0 + 1, st));
+        hipEvent_t e_start = ctx->timing >= 1 ? ctx->ev[e0] : nullptr;
+        hipEvent_t e_stop = ctx->timing >= 1 ? ctx->ev[e0 + 1] : nullptr;
+        const unsigned grid = (unsigned)(g2path && mode == MODE_HIST ? nwg_hist : nwg);
+        void *args[] = {&ga};
+        PG_HIP(ctx, hipExtLaunchKernel(fn, dim3(grid), dim3(THREADS), args, shm, st, e_start, e_stop, 0));
         if (mode == MODE_HIST) ran_hist = true;
         else ran_emit = true;
         if (ga.sstep <= 1) gram_passes++;
@@ -3296,7 +3308,7 @@ struct CorrRun {
             const size_t shs = (size_t)nbk * (4 + 4);  // bases + cursors
             const void *scatter_fn = weighted ? (const void *)bucket_scatter<true> : (const void *)bucket_scatter<false>;
             if (shs > 64 * 1024)
-                PG_HIP(ctx, hipFuncSetAttribute(scatter_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
+                PG_HIP(ctx, set_shared_limit(ctx, scatter_fn, shs));
             if (weighted)
                 bucket_scatter<true><<<(unsigned)nsrc, SEG_THREADS, shs, st>>>(src, nbk, boff, bstart, ent, entv, z64,
                                                                                t, (long long)nnzs);
@@ -3304,8 +3316,7 @@ struct CorrRun {
                 bucket_scatter<false><<<(unsigned)nsrc, SEG_THREADS, shs, st>>>(src, nbk, boff, bstart, ent, nullptr,
                                                                                 z64, t, (long long)nnzs);
             if (lbits) {
-                PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows_bits,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lbits));
+                PG_HIP(ctx, set_shared_limit(ctx, (const void *)bucket_rows_bits, (int)lbits));
                 bucket_rows_bits<<<rows_bits_grid(ctx, n, nbk), BITS_THREADS, lbits, st>>>(bstart, ent, n, nbk, row,
                                                                                          col, (long long)nnzs);
                 PG_KERNEL_CHECK(ctx);
@@ -3314,7 +3325,7 @@ struct CorrRun {
             const RowsPlan rp = rows_plan(n, ctx->lds_per_block);
             if (rp.nbm == 0) return pg_fail(ctx, PG_ENOTSUP, "n = %d: the row bitmap does not fit LDS", n);
             const void *rows_fn = weighted ? (const void *)bucket_rows<true> : (const void *)bucket_rows<false>;
-            PG_HIP(ctx, hipFuncSetAttribute(rows_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rp.lds));
+            PG_HIP(ctx, set_shared_limit(ctx, rows_fn, rp.lds));
             if (weighted)
                 bucket_rows<true><<<(unsigned)nbk, ROWS_THREADS, rp.lds, st>>>(
                     bstart, ent, entv, n, nbk, rp.nbm, rp.dbit, row, ctmp, vtmp, col, val, (long long)nnzs);
@@ -3442,8 +3453,7 @@ pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Windo
         PG_TRY(pg_bufT(ctx, "corr.bcnt", (size_t)R.nsrc * R.nbk, &bcnt));
         EdgeSrc es = R.edges(nullptr, nullptr, 0);
         es.seg_r = R.seg_r;
-        PG_HIP(ctx, hipFuncSetAttribute((const void *)window_count, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)((size_t)R.nbk * 4)));
+        PG_HIP(ctx, set_shared_limit(ctx, (const void *)window_count, (int)((size_t)R.nbk * 4)));
         window_count<<<(unsigned)R.nwg, SEG_THREADS, (size_t)R.nbk * 4, st>>>(
             es, ctl, R.nbk, R.nsrc, bcnt, win->cand, win->cand_r, ccap, R.counters + 0, win->dpart);
     } else {
@@ -4197,24 +4207,17 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
     PG_TRY(pg_bufT(ctx, "coh.ctmp", (size_t)max_ctmp, &P->ctmp));
     // dynamic LDS limits for the largest subject, set before any capture
     const size_t shz = (size_t)max_t * (ZV + 1) * 4;
-    PG_HIP(ctx, hipFuncSetAttribute((const void *)zscore_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shz));
+    PG_HIP(ctx, set_shared_limit(ctx, (const void *)zscore_fused, (int)shz));
     const size_t stage = (size_t)g2::NSTAGE * g2::STAGE;
-    PG_HIP(ctx, hipFuncSetAttribute((const void *)gram2_kernel<MODE_HIST>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(stage + NBIN * 4)));
-    PG_HIP(ctx, hipFuncSetAttribute((const void *)gram2_kernel<MODE_EMIT, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)stage));
-    PG_HIP(ctx, hipFuncSetAttribute((const void *)gram2_kernel<MODE_EMIT, false, true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage));
-    PG_HIP(ctx, hipFuncSetAttribute((const void *)window_count, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)((size_t)max_nbk * 4)));
+    PG_HIP(ctx, set_shared_limit(ctx, (const void *)gram2_kernel<MODE_HIST>, (int)(stage + NBIN * 4)));
+    PG_HIP(ctx, set_shared_limit(ctx, (const void *)gram2_kernel<MODE_EMIT, false>, stage));
+    PG_HIP(ctx, set_shared_limit(ctx, (const void *)gram2_kernel<MODE_EMIT, false, true>, stage));
+    PG_HIP(ctx, set_shared_limit(ctx, (const void *)window_count, (int)((size_t)max_nbk * 4)));
     const size_t shs = (size_t)max_nbk * 8;
     if (shs > 64 * 1024)
-        PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_scatter<false>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shs));
-    PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)max_rows_lds));
-    PG_HIP(ctx, hipFuncSetAttribute((const void *)bucket_rows_bits, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)max_bits_lds));
+        PG_HIP(ctx, set_shared_limit(ctx, (const void *)bucket_scatter<false>, (int)shs));
+    PG_HIP(ctx, set_shared_limit(ctx, (const void *)bucket_rows<false>, (int)max_rows_lds));
+    PG_HIP(ctx, set_shared_limit(ctx, (const void *)bucket_rows_bits, (int)max_bits_lds));
     *out = guard.release();
     return PG_OK;
 }
@@ -4330,8 +4333,7 @@ pg_status pg_zscore64(pg_ctx *ctx, const float *dbold, int layout, int n, int t,
     hipStream_t st = ctx->stream;
     if (t <= ZS_TMAX) {
         const size_t shz = (size_t)t * (ZV + 1) * 4;
-        PG_HIP(ctx, hipFuncSetAttribute((const void *)zscore_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)shz));
+        PG_HIP(ctx, set_shared_limit(ctx, (const void *)zscore_fused, (int)shz));
         zscore_fused<<<(unsigned)pg_cdiv(n, ZV), 256, shz, st>>>(dbold, layout, n, t, z64, nullptr, nullptr,
                                                                   nullptr, 0, 0);
     } else {

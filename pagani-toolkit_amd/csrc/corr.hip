@@ -2262,9 +2262,16 @@ constexpr double FINE_TOP = 1.0 + 1e-6;  // upper end of the fine histogram
 
 // Candidate window from the fine histogram of the emitted r~: the bin holding
 // the k-th largest, widened by twice the error band on each side.
+// clr_a[0..na), clr_b[0..nb): buffers the next steps need zeroed (the
+// candidate split's counters and the selection histogram), cleared here
+// instead of by a fill launch of their own.
 __global__ __launch_bounds__(1024) void window_bounds(const unsigned long long *__restrict__ h2, long long kwant,
-                                                      double band, SelCtl *__restrict__ ctl)
+                                                      double band, SelCtl *__restrict__ ctl,
+                                                      unsigned long long *__restrict__ clr_a = nullptr, int na = 0,
+                                                      unsigned *__restrict__ clr_b = nullptr, int nb = 0)
 {
+    for (int k = threadIdx.x; k < na; k += 1024) clr_a[k] = 0ull;
+    for (int k = threadIdx.x; k < nb; k += 1024) clr_b[k] = 0u;
     unsigned long long tot = 0;
     const int b2 = top_bin_reaching(h2, NBIN2, (double)kwant, &tot);
     if (threadIdx.x == 0) {
@@ -2283,11 +2290,19 @@ __global__ __launch_bounds__(1024) void window_bounds(const unsigned long long *
 
 // Histogram of r~ over the kept segment entries, bins over [lo, lo + NBIN2/scale).
 // Segments start at multiples of 4 floats (seg_cap % 4 == 0): float4 loads.
+// tot: the NBIN2 totals that hist_sum accumulates next, zeroed here (each
+// work-group a slice) instead of by a fill launch of their own.
 __global__ __launch_bounds__(1024) void emitted_hist(const float *__restrict__ seg_r, long long seg_cap,
                                                      const unsigned long long *__restrict__ seg_cnt, int nseg,
-                                                     const SelCtl *__restrict__ ctl, uint32_t *__restrict__ out)
+                                                     const SelCtl *__restrict__ ctl, uint32_t *__restrict__ out,
+                                                     unsigned long long *__restrict__ tot)
 {
     __shared__ uint32_t h[NBIN2];
+    {
+        const int per = (NBIN2 + (int)gridDim.x - 1) / (int)gridDim.x;
+        const int b0 = blockIdx.x * per, b1 = min(NBIN2, b0 + per);
+        for (int b = b0 + threadIdx.x; b < b1; b += 1024) tot[b] = 0ull;
+    }
     const float lo = ctl->lower;
     const float scale = (float)(NBIN2 / (FINE_TOP - (double)lo));
     for (int b = threadIdx.x; b < NBIN2; b += 1024) h[b] = 0;
@@ -3196,7 +3211,9 @@ struct CorrRun {
 
     // One emitting Gram pass without a host check (the caller checks
     // seg_cnt); store_r keeps r~ of every segment entry.
-    pg_status emit_launch(long long seg_cap0, long long band_cap0, bool store_r)
+    // clear: zero the counters first (the recheck-list count, read by
+    // emit_pass); every work-group writes its own segment count.
+    pg_status emit_launch(long long seg_cap0, long long band_cap0, bool store_r, bool clear = true)
     {
         seg_cap = (seg_cap0 + 63) / 64 * 64;  // segments 256-byte aligned (vector loads)
         band_cap = band_cap0;
@@ -3205,8 +3222,7 @@ struct CorrRun {
         else seg_r = nullptr;
         PG_TRY(pg_bufT(ctx, "corr.band", (size_t)band_cap, &bandl));
         PG_TRY(pg_bufT(ctx, "corr.bandr", (size_t)band_cap, &band_r));
-        PG_TRY(clear_dev(ctx, st, counters, 16 * sizeof(unsigned long long), seg_cnt,
-                         nwg * sizeof(unsigned long long)));
+        if (clear) PG_TRY(clear_dev(ctx, st, counters, 16 * sizeof(unsigned long long)));
         ga.seg = seg;
         ga.seg_r = seg_r;
         ga.seg_cap = seg_cap;
@@ -3217,6 +3233,7 @@ struct CorrRun {
         ga.band_cnt = counters + 0;
         ga.sstep = 0;
         if (n > 1) PG_TRY(launch_gram(MODE_EMIT));
+        else PG_TRY(clear_dev(ctx, st, seg_cnt, nwg * sizeof(unsigned long long)));
         return PG_OK;
     }
 
@@ -3409,8 +3426,7 @@ pg_status fine_hist(CorrRun &R, const SelCtl *ctl, unsigned long long **h2t_out)
     unsigned long long *h2t;
     PG_TRY(pg_bufT(ctx, "corr.h2", (size_t)hb * NBIN2, &h2));
     PG_TRY(pg_bufT(ctx, "corr.h2t", NBIN2, &h2t));
-    PG_TRY(clear_dev(ctx, st, h2t, NBIN2 * sizeof(unsigned long long)));
-    emitted_hist<<<hb, 1024, 0, st>>>(R.seg_r, R.seg_cap, R.seg_cnt, (int)R.nwg, ctl, h2);
+    emitted_hist<<<hb, 1024, 0, st>>>(R.seg_r, R.seg_cap, R.seg_cnt, (int)R.nwg, ctl, h2, h2t);
     hist_sum<<<dim3(NBIN2 / 256, 16), 256, 0, st>>>(h2, hb, NBIN2, h2t);
     PG_KERNEL_CHECK(ctx);
     *h2t_out = h2t;
@@ -3434,7 +3450,10 @@ struct Window {
 // build's bucket counts (window_count, the definite edges being exactly the
 // segment entries the build keeps: r~ above ctl's U2); launch_csr then counts
 // only the selected list.
-pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Window *win, bool count_segments = false)
+// cleared: the counters and the selection histogram were zeroed already
+// (window_bounds).
+pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Window *win, bool count_segments = false,
+                            bool cleared = false)
 {
     pg_ctx *ctx = R.ctx;
     hipStream_t st = R.st;
@@ -3445,7 +3464,8 @@ pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Windo
     PG_TRY(pg_bufT(ctx, "corr.selhist", (size_t)SEL_NB, &win->sh.h));
     win->sh.ctl = ctl;
     win->sh.band = R.band;
-    PG_TRY(clear_dev(ctx, st, R.counters, 16 * sizeof(unsigned long long), win->sh.h, SEL_NB * sizeof(unsigned)));
+    if (!cleared)
+        PG_TRY(clear_dev(ctx, st, R.counters, 16 * sizeof(unsigned long long), win->sh.h, SEL_NB * sizeof(unsigned)));
     if (count_segments) {
         win->nd = (int)R.nwg;
         PG_TRY(pg_bufT(ctx, "corr.dpart", (size_t)win->nd, &win->dpart));
@@ -3474,9 +3494,9 @@ pg_status window_candidates(CorrRun &R, const SelCtl *ctl, long long ccap, Windo
 // (select_kernel; t* into tstar_d, errors into counters[4]) and the definite
 // count into counters[3].
 pg_status window_select(CorrRun &R, const SelCtl *ctl, long long ccap, long long kwant, double *tstar_d,
-                        Window *win, bool count_segments = false)
+                        Window *win, bool count_segments = false, bool cleared = false)
 {
-    PG_TRY(window_candidates(R, ctl, ccap, win, count_segments));
+    PG_TRY(window_candidates(R, ctl, ccap, win, count_segments, cleared));
     hipStream_t st = R.st;
     unsigned long long *c = R.counters;
     SelCut *cut;
@@ -3618,17 +3638,20 @@ extern "C" pg_status pg_corr_build(pg_ctx *ctx, const float *bold, int32_t n, in
             R.ga.upper = -4.0f;  // every hit is a segment entry
             const double expect = std::min(1.0, f * 1.2 + 2.0 * w) * (double)m1;
             R.dedic = dedic_fits(expect / std::max<double>((double)m1, 1.0));
-            PG_TRY(R.emit_launch(pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096, 1 << 10, true));
+            // the counters are zeroed by window_bounds below, before their first use
+            PG_TRY(R.emit_launch(pg_cdiv((long long)expect, nwg) * 5 / 4 + 4096, 1 << 10, true, false));
             R.dedic = false;
             R.ga.lower_dev = nullptr;
             unsigned long long *h2t;
             PG_TRY(fine_hist(R, ctl, &h2t));
-            window_bounds<<<1, 1024, 0, st>>>(h2t, kwant, band, ctl);
+            unsigned *selhist;
+            PG_TRY(pg_bufT(ctx, "corr.selhist", (size_t)SEL_NB, &selhist));
+            window_bounds<<<1, 1024, 0, st>>>(h2t, kwant, band, ctl, counters, 16, selhist, SEL_NB);
             PG_KERNEL_CHECK(ctx);
             PG_HIP(ctx, stage_event(ctx, 3, st));
             const long long ccap = std::min<long long>(1 << 22, std::max<long long>(1 << 16, kwant / 4));
             Window win;
-            PG_TRY(window_select(R, ctl, ccap, kwant, tstar_d, &win, true));
+            PG_TRY(window_select(R, ctl, ccap, kwant, tstar_d, &win, true, true));
             extra = win.extra;
             EdgeSrc es = R.edges(extra, extra_cnt, ccap);
             es.seg_r = R.seg_r;
@@ -4259,7 +4282,8 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     gram2_kernel<MODE_HIST><<<(unsigned)P->nwg_hist, THREADS, stage + NBIN * 4, st>>>(ga);
     hist_reduce<<<NBIN / 32, 256, 0, st>>>(P->hist, (int)P->nwg_hist, P->htot);
     sample_bound<<<1, 1024, 0, st>>>(P->htot, S.f, P->ctl);
-    fill_u32<<<1, 32, 0, st>>>((uint32_t *)P->counters, 32);
+    // (the counters are zeroed by window_bounds before their first use; the
+    // band-free emit writes every segment count itself)
     ga.sstep = 0;
     ga.lower_dev = &P->ctl->lower;
     ga.upper = -4.0f;  // every hit is a segment entry, with its r~
@@ -4274,11 +4298,9 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     if (S.dedic) gram2_kernel<MODE_EMIT, false, true><<<(unsigned)nwg, THREADS, stage, st>>>(ga);
     else gram2_kernel<MODE_EMIT, false><<<(unsigned)nwg, THREADS, stage, st>>>(ga);
     const int hb = (int)std::min<long long>(256, nwg);
-    fill_u32<<<(unsigned)pg_cdiv(2 * NBIN2, 256), 256, 0, st>>>((uint32_t *)P->h2t, 2 * NBIN2);
-    emitted_hist<<<hb, 1024, 0, st>>>(P->seg_r, S.seg_cap, P->seg_cnt, (int)nwg, P->ctl, P->h2);
+    emitted_hist<<<hb, 1024, 0, st>>>(P->seg_r, S.seg_cap, P->seg_cnt, (int)nwg, P->ctl, P->h2, P->h2t);
     hist_sum<<<dim3(NBIN2 / 256, 16), 256, 0, st>>>(P->h2, hb, NBIN2, P->h2t);
-    window_bounds<<<1, 1024, 0, st>>>(P->h2t, S.kwant, S.band, P->ctl);
-    fill_u32<<<SEL_NB / 256, 256, 0, st>>>((uint32_t *)P->counters, 32, P->selhist, SEL_NB);
+    window_bounds<<<1, 1024, 0, st>>>(P->h2t, S.kwant, S.band, P->ctl, P->counters, 16, P->selhist, SEL_NB);
     const SelHist shh{P->selhist, P->ctl, S.band};
     unsigned long long *c = P->counters;
     // the segments as edge sources (r~ above the window kept), then the

@@ -49,8 +49,8 @@ pg_status pg_synchronize(pg_ctx *ctx);
  * fields of pg_corr_info).  No counterpart in the reference: each timing
  * event leaves the device idle for a few microseconds, so callers that do not
  * read the stage times (batch pipelines, timing loops) can drop them.
- * level 2 (default): every stage; 1: only ms_gram_hist and ms_gram_emit (the
- * Gram kernels); 0: none.  Fields of untimed stages are 0. */
+ * level 2 (default): every stage; 1: only ms_gram_emit (the emitting Gram
+ * kernel); 0: none.  Fields of untimed stages are 0. */
 pg_status pg_set_timing(pg_ctx *ctx, int32_t level);
 
 /* ------------------------------------------------------------ construction */

@@ -134,12 +134,13 @@ inline hipError_t set_shared_limit(pg_ctx *ctx, const void *fn, size_t bytes)
 }
 
 // Stage-timing event idx of pg_corr_build on stream st, recorded when the
-// context's timing level asks for it (events 6..9 bracket the Gram kernels,
-// kept at level 1; the others mark the stage boundaries, level 2).  Each
-// recorded event costs the device a few microseconds of idle time.
+// context's timing level asks for it (events 8 and 9 bracket the emitting
+// Gram kernel, kept at level 1; the others, level 2).  Each recorded event
+// costs the device a few microseconds of idle time.
+inline bool stage_timed(const pg_ctx *ctx, int idx) { return ctx->timing >= (idx >= 8 ? 1 : 2); }
 inline hipError_t stage_event(pg_ctx *ctx, int idx, hipStream_t st)
 {
-    return ctx->timing >= (idx >= 6 ? 1 : 2) ? hipEventRecord(ctx->ev[idx], st) : hipSuccess;
+    return stage_timed(ctx, idx) ? hipEventRecord(ctx->ev[idx], st) : hipSuccess;
 }
 
 // Zero fill of one or two buffers of 32-bit words (grid over the larger).
@@ -3198,8 +3199,8 @@ struct CorrRun {
         // the kernel's own start and stop events (ev[e0], ev[e0 + 1]) ride on
         // its dispatch: no separate event packets around it
         const int e0 = mode == MODE_HIST ? 6 : 8;
-        hipEvent_t e_start = ctx->timing >= 1 ? ctx->ev[e0] : nullptr;
-        hipEvent_t e_stop = ctx->timing >= 1 ? ctx->ev[e0 + 1] : nullptr;
+        hipEvent_t e_start = stage_timed(ctx, e0) ? ctx->ev[e0] : nullptr;
+        hipEvent_t e_stop = stage_timed(ctx, e0) ? ctx->ev[e0 + 1] : nullptr;
         const unsigned grid = (unsigned)(g2path && mode == MODE_HIST ? nwg_hist : nwg);
         void *args[] = {&ga};
         PG_HIP(ctx, hipExtLaunchKernel(fn, dim3(grid), dim3(THREADS), args, shm, st, e_start, e_stop, 0));
@@ -3400,7 +3401,7 @@ struct CorrRun {
             (void)hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
             inf.ms_csr = ms;
         }
-        if (ctx->timing >= 1 && ran_hist && hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess)
+        if (ctx->timing >= 2 && ran_hist && hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess)
             inf.ms_gram_hist = ms;
         if (ctx->timing >= 1 && ran_emit && hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]) == hipSuccess)
             inf.ms_gram_emit = ms;
@@ -3912,7 +3913,7 @@ extern "C" pg_status pg_corr_emit(pg_ctx *ctx, const float *bold, int32_t n, int
         (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
         inf.ms_gram = ms + ms_hist;
     }
-    if (ctx->timing >= 1 && R.ran_hist && hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess)
+    if (ctx->timing >= 2 && R.ran_hist && hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess)
         inf.ms_gram_hist = ms;
     if (ctx->timing >= 1 && R.ran_emit && hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]) == hipSuccess)
         inf.ms_gram_emit = ms;

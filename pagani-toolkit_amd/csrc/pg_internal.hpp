@@ -28,7 +28,7 @@ struct pg_ctx {
     std::map<std::string, PgBuf> bufs;  // named grow-only device buffers
     hipEvent_t ev[12] = {};
     // stage-timing events of pg_corr_build (pg_set_timing): 2 every stage,
-    // 1 the Gram kernels only (events 6..9), 0 none
+    // 1 the emitting Gram kernel only (events 8, 9), 0 none
     int timing = 2;
     void *static_scratch = nullptr;  // this device's fixed scratch array (pg_static_scratch)
     PgEmit *emit = nullptr;          // pg_corr_emit state, owned

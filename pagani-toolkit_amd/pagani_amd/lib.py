@@ -220,8 +220,8 @@ class Context:
         self._check(self._lib.pg_synchronize(self._h))
 
     def set_timing(self, level: int):
-        """Stage timing of corr_build: 2 every stage (default), 1 the Gram
-        kernels only, 0 none (untimed stages report 0)."""
+        """Stage timing of corr_build: 2 every stage (default), 1 the
+        emitting Gram kernel only, 0 none (untimed stages report 0)."""
         self._check(self._lib.pg_set_timing(self._h, int(level)))
 
     # ------------------------------------------------------- construction

@@ -197,8 +197,8 @@ def test_voxel_major_layout_and_zero_variance(ctx, prec):
 
 
 def test_timing_levels_change_no_result(ctx):
-    # level 1 keeps only the Gram kernels' events, level 0 none; the edge set
-    # is the same at every level and untimed stages report 0
+    # level 1 keeps only the emitting Gram kernel's events, level 0 none; the
+    # edge set is the same at every level and untimed stages report 0
     x = _data(3000, 120, 8, 31)
     base, row0, col0, _ = _build(ctx, x, 1.0, sparsity=True)
     assert base.ms_zscore > 0 and base.ms_csr > 0 and base.ms_gram_emit > 0
@@ -206,6 +206,7 @@ def test_timing_levels_change_no_result(ctx):
         ctx.set_timing(1)
         i1, row1, col1, _ = _build(ctx, x, 1.0, sparsity=True)
         assert i1.ms_zscore == 0 and i1.ms_select == 0 and i1.ms_csr == 0 and i1.ms_gram == 0
+        assert i1.ms_gram_hist == 0
         assert i1.ms_gram_emit > 0
         ctx.set_timing(0)
         i0, row2, col2, _ = _build(ctx, x, 1.0, sparsity=True)

@@ -2046,10 +2046,21 @@ __global__ __launch_bounds__(ROWS_THREADS, W ? 4 : 8) void bucket_rows(const lon
 // out sorted with one read of the entries and one write of the columns.  One
 // wave per row of the bucket; work-groups loop over buckets, and the bitmap
 // is zero between buckets (the read-back clears what it finds).  Each lane
-// owns a run of `per` 128-column groups of a row (`per` odd: the 16-byte
-// reads of a wave spread over the LDS banks).
+// owns the run of 128-column groups [lane * ng / 64, (lane + 1) * ng / 64)
+// of a row.  The first BITS_PF entries per thread of a work-group's next
+// bucket are loaded before the read-back of the current one, so their HBM
+// latency is spent under it; the loads are clamped to the bucket's last
+// entry (a repeated entry sets a bit that is already set) and only the
+// bit-sets are masked.
 constexpr int BITS_THREADS = 64 * BROWS;
 constexpr int BITS_UNROLL = 8;
+constexpr int BITS_PF = 12;
+
+__device__ __forceinline__ void bits_set(unsigned *bm, unsigned ng, unsigned x)
+{
+    const unsigned c = x & COLMASK;
+    atomicOr(&bm[(x >> 25) * ng * 4u + (c >> 5)], 1u << (c & 31));
+}
 
 __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long *__restrict__ bstart,
                                                                   const unsigned *__restrict__ ent, int n, int nbk,
@@ -2064,28 +2075,37 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
         return;
     }
     const int ng = (n + 127) >> 7;
-    const int per = ((ng + 63) / 64) | 1;
     unsigned *bm = (unsigned *)bm4;
     uint4 *rb4 = bm4 + (long long)r * ng;
     for (int q = lane; q < ng; q += 64) rb4[q] = make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
-    const int g0 = lane * per, g1 = min(ng, g0 + per);
-    for (int b = blockIdx.x; b < nbk; b += gridDim.x) {
-        const long long e0 = bstart[b], e1 = bstart[b + 1];
-        for (long long k0 = e0 + tid; k0 < e1; k0 += (long long)BITS_THREADS * BITS_UNROLL) {
-            unsigned x[BITS_UNROLL];
+    const int g0 = (lane * ng) >> 6, g1 = ((lane + 1) * ng) >> 6;
+    unsigned x[BITS_PF];
+    long long e0 = 0, e1 = 0;
+    auto fetch = [&](int bb) {
+        e0 = bstart[bb];
+        e1 = bstart[bb + 1];
+        if (e1 > e0) {
 #pragma unroll
-            for (int u = 0; u < BITS_UNROLL; u++) {
-                const long long k = k0 + (long long)u * BITS_THREADS;
-                x[u] = k < e1 ? ent[k] : ~0u;
-            }
+            for (int u = 0; u < BITS_PF; u++) x[u] = ent[min(e0 + tid + (long long)u * BITS_THREADS, e1 - 1)];
+        }
+    };
+    if (blockIdx.x < nbk) fetch(blockIdx.x);
+    for (int b = blockIdx.x; b < nbk; b += gridDim.x) {
+#pragma unroll
+        for (int u = 0; u < BITS_PF; u++)
+            if (e0 + tid + (long long)u * BITS_THREADS < e1) bits_set(bm, (unsigned)ng, x[u]);
+        for (long long k0 = e0 + tid + (long long)BITS_PF * BITS_THREADS; k0 < e1;
+             k0 += (long long)BITS_THREADS * BITS_UNROLL) {
+            unsigned y[BITS_UNROLL];
+#pragma unroll
+            for (int u = 0; u < BITS_UNROLL; u++) y[u] = ent[min(k0 + (long long)u * BITS_THREADS, e1 - 1)];
 #pragma unroll
             for (int u = 0; u < BITS_UNROLL; u++)
-                if (x[u] != ~0u) {
-                    const unsigned c = x[u] & COLMASK;
-                    atomicOr(&bm[(x[u] >> 25) * (unsigned)ng * 4u + (c >> 5)], 1u << (c & 31));
-                }
+                if (k0 + (long long)u * BITS_THREADS < e1) bits_set(bm, (unsigned)ng, y[u]);
         }
+        const long long c0 = e0, c1 = e1;
+        if (b + (int)gridDim.x < nbk) fetch(b + gridDim.x);
         __syncthreads();
         // this wave's row: per-lane popcounts, one wave scan
         unsigned s = 0;
@@ -2104,12 +2124,13 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
         unsigned before = 0;
         for (int q = 0; q < r; q++) before += rdeg[q];
         const int gr = b * BROWS + r;
-        if (lane == 0 && gr < n) row[gr] = (int)(e0 + before);
-        if (b == nbk - 1 && tid == 0) row[n] = (int)e1;
+        if (lane == 0 && gr < n) row[gr] = (int)(c0 + before);
+        if (b == nbk - 1 && tid == 0) row[n] = (int)c1;
         // read the lane's s columns back in order (one per step, moving to
-        // the next non-empty group when the current one is spent), then
-        // clear the lane's groups
-        int *out = col + (e0 + before + (incl - s));
+        // the next non-empty group when the current one is spent; the step
+        // takes the lowest bit of the low or high half without a branch),
+        // then clear the lane's groups
+        int *out = col + (c0 + before + (incl - s));
         int g = g0;
         unsigned long long lo = 0, hi = 0;
         for (unsigned it = 0; it < s; it++) {
@@ -2118,14 +2139,12 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
                 lo = ((unsigned long long)v.y << 32) | v.x;
                 hi = ((unsigned long long)v.w << 32) | v.z;
             }
-            int c;
-            if (lo) {
-                c = __builtin_ctzll(lo);
-                lo &= lo - 1ull;
-            } else {
-                c = 64 + __builtin_ctzll(hi);
-                hi &= hi - 1ull;
-            }
+            const bool inlo = lo != 0ull;
+            const unsigned long long w = inlo ? lo : hi;
+            const int c = __builtin_ctzll(w) + (inlo ? 0 : 64);
+            const unsigned long long w2 = w & (w - 1ull);
+            hi = inlo ? hi : w2;
+            lo = inlo ? w2 : 0ull;
             *out++ = (g - 1) * 128 + c;
         }
         if (s)

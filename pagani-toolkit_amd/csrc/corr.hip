@@ -1651,7 +1651,7 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, 
 // the segments; the selected list's counts follow from bucket_count (g0 =
 // nseg) once it exists.
 constexpr int WC_STAGE = 1024;
-__global__ __launch_bounds__(SEG_THREADS) void window_count(EdgeSrc s, const SelCtl *__restrict__ ctl, int nbk,
+__global__ __launch_bounds__(SEG_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void window_count(EdgeSrc s, const SelCtl *__restrict__ ctl, int nbk,
                                                             int nsrc, unsigned *__restrict__ cnt,
                                                             int2 *__restrict__ cand, float *__restrict__ cand_r,
                                                             long long cand_cap,
@@ -2055,6 +2055,24 @@ __global__ __launch_bounds__(ROWS_THREADS, W ? 4 : 8) void bucket_rows(const lon
 constexpr int BITS_THREADS = 64 * BROWS;
 constexpr int BITS_UNROLL = 8;
 constexpr int BITS_PF = 12;
+// longest run of groups per lane: the BROWS bitmaps fit 160 KB of LDS only
+// while ng <= 64 * BITS_RUN
+constexpr int BITS_RUN = 10;
+static_assert((size_t)BROWS * 16 * (64 * BITS_RUN + 1) + 64 > 160 * 1024, "a lane's run can exceed BITS_RUN groups");
+
+// Inclusive sum over the 64 lanes of a wave (DPP: shifts within each row of
+// 16 lanes, then the row totals broadcast into the rows above); every lane
+// of the wave must be active.
+__device__ __forceinline__ unsigned wave_incl_sum(unsigned x)
+{
+    x += (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false);  // row_shr:1
+    x += (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false);  // row_shr:2
+    x += (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);  // row_shr:4
+    x += (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);  // row_shr:8
+    x += (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15 into rows 1, 3
+    x += (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31 into rows 2, 3
+    return x;
+}
 
 __device__ __forceinline__ void bits_set(unsigned *bm, unsigned ng, unsigned x)
 {
@@ -2107,18 +2125,17 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
         const long long c0 = e0, c1 = e1;
         if (b + (int)gridDim.x < nbk) fetch(b + gridDim.x);
         __syncthreads();
-        // this wave's row: per-lane popcounts, one wave scan
+        // this wave's row: per-lane popcounts (the run's reads all issued
+        // before the first is used; reads past the run are clamped into the
+        // row and not counted), one wave scan
         unsigned s = 0;
-        for (int g = g0; g < g1; g++) {
-            const uint4 v = rb4[g];
-            s += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
-        }
-        unsigned incl = s;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned y = __shfl_up(incl, o);
-            if (lane >= o) incl += y;
+        for (int q = 0; q < BITS_RUN; q++) {
+            const uint4 v = rb4[min(g0 + q, ng - 1)];
+            const unsigned c = __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+            s += g0 + q < g1 ? c : 0u;
         }
+        const unsigned incl = wave_incl_sum(s);
         if (lane == 63) rdeg[r] = incl;
         __syncthreads();
         unsigned before = 0;
@@ -2133,11 +2150,13 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
         int *out = col + (c0 + before + (incl - s));
         int g = g0;
         unsigned long long lo = 0, hi = 0;
+        uint4 nx = rb4[min(g, ng - 1)];  // the next group, read one ahead
         for (unsigned it = 0; it < s; it++) {
             while (!(lo | hi)) {
-                const uint4 v = rb4[g++];
-                lo = ((unsigned long long)v.y << 32) | v.x;
-                hi = ((unsigned long long)v.w << 32) | v.z;
+                lo = ((unsigned long long)nx.y << 32) | nx.x;
+                hi = ((unsigned long long)nx.w << 32) | nx.z;
+                g++;
+                nx = rb4[min(g, ng - 1)];
             }
             const bool inlo = lo != 0ull;
             const unsigned long long w = inlo ? lo : hi;

@@ -2041,17 +2041,20 @@ __global__ __launch_bounds__(ROWS_THREADS, W ? 4 : 8) void bucket_rows(const lon
 }
 
 // Unweighted rows of a bucket from one LDS bitmap per row over the columns
-// [0, n): every entry sets its bit, a row's degree is the popcount of its
-// bitmap and its columns are read back in ascending order, so the rows come
-// out sorted with one read of the entries and one write of the columns.  One
-// wave per row of the bucket; work-groups loop over buckets, and the bitmap
-// is zero between buckets (the read-back clears what it finds).  Each lane
-// owns the run of 128-column groups [lane * ng / 64, (lane + 1) * ng / 64)
-// of a row.  The first BITS_PF entries per thread of a work-group's next
-// bucket are loaded before the read-back of the current one, so their HBM
-// latency is spent under it; the loads are clamped to the bucket's last
-// entry (a repeated entry sets a bit that is already set) and only the
-// bit-sets are masked.
+// [0, n): every entry sets its bit and a row's degree is the popcount of its
+// bitmap, so the rows come out sorted with one read of the entries and one
+// write of the columns.  Work-groups loop over buckets; the bitmap is zero
+// between buckets.  Wave r counts row r: each lane owns the run of 128-column
+// groups [lane * ng / 64, (lane + 1) * ng / 64) of the row.  Then, with RANK
+// (the bitmaps and one 32-bit base per group fit LDS), every group's base
+// (its first column's place in the bucket's output) goes to LDS and every
+// entry writes its column at base + the set bits below it in its group, so
+// the bucket's work is spread evenly over the work-group; without RANK each
+// lane reads its run's columns back in order, one per step.  The first
+// BITS_PF entries per thread of a work-group's next bucket are loaded before
+// the current one is written out, so their HBM latency is spent under it;
+// the loads are clamped to the bucket's last entry (a repeated entry sets a
+// bit that is already set) and only the uses are masked.
 constexpr int BITS_THREADS = 64 * BROWS;
 constexpr int BITS_UNROLL = 8;
 constexpr int BITS_PF = 12;
@@ -2080,12 +2083,29 @@ __device__ __forceinline__ void bits_set(unsigned *bm, unsigned ng, unsigned x)
     atomicOr(&bm[(x >> 25) * ng * 4u + (c >> 5)], 1u << (c & 31));
 }
 
+// Entry x's column, written at its place in the bucket's output (colb = the
+// bucket's first column slot): its group's base plus the set bits below it.
+__device__ __forceinline__ void bits_put(const uint4 *bm4, const unsigned *gbase, unsigned ng, unsigned x, int *colb)
+{
+    const unsigned c = x & COLMASK;
+    const unsigned gi = (x >> 25) * ng + (c >> 7);
+    const uint4 v = bm4[gi];
+    const unsigned w = (c >> 5) & 3u, m = (1u << (c & 31)) - 1u;
+    unsigned pos = gbase[gi];
+    pos += __popc(v.x & (w > 0u ? ~0u : m));
+    pos += __popc(v.y & (w > 1u ? ~0u : (w == 1u ? m : 0u)));
+    pos += __popc(v.z & (w > 2u ? ~0u : (w == 2u ? m : 0u)));
+    pos += __popc(v.w & (w == 3u ? m : 0u));
+    colb[pos] = (int)c;
+}
+
+template <bool RANK>
 __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long *__restrict__ bstart,
                                                                   const unsigned *__restrict__ ent, int n, int nbk,
                                                                   int *__restrict__ row, int *__restrict__ col,
                                                                   long long cap)
 {
-    extern __shared__ __align__(16) uint4 bm4[];  // [BROWS][ng]
+    extern __shared__ __align__(16) uint4 bm4[];  // [BROWS][ng], then (RANK) the group bases [BROWS][ng]
     __shared__ unsigned rdeg[BROWS];
     const int tid = threadIdx.x, lane = tid & 63, r = tid >> 6;
     if (bstart[nbk] > cap) {  // nothing was scattered; row[n] reports the total
@@ -2094,46 +2114,59 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
     }
     const int ng = (n + 127) >> 7;
     unsigned *bm = (unsigned *)bm4;
+    unsigned *gbase = (unsigned *)(bm4 + (long long)BROWS * ng);
     uint4 *rb4 = bm4 + (long long)r * ng;
     for (int q = lane; q < ng; q += 64) rb4[q] = make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
     const int g0 = (lane * ng) >> 6, g1 = ((lane + 1) * ng) >> 6;
-    unsigned x[BITS_PF];
+    // this bucket's first entries (x) and the next bucket's (xn); bucket
+    // sizes are below cap <= INT32_MAX, so offsets within one are 32-bit
+    unsigned x[BITS_PF], xn[BITS_PF];
     long long e0 = 0, e1 = 0;
-    auto fetch = [&](int bb) {
-        e0 = bstart[bb];
-        e1 = bstart[bb + 1];
+    if (blockIdx.x < nbk) {
+        e0 = bstart[blockIdx.x];
+        e1 = bstart[blockIdx.x + 1];
         if (e1 > e0) {
+            const unsigned *xb = ent + e0;
+            const unsigned last = (unsigned)(e1 - e0) - 1u;
 #pragma unroll
-            for (int u = 0; u < BITS_PF; u++) x[u] = ent[min(e0 + tid + (long long)u * BITS_THREADS, e1 - 1)];
+            for (int u = 0; u < BITS_PF; u++) x[u] = xb[min((unsigned)(tid + u * BITS_THREADS), last)];
         }
-    };
-    if (blockIdx.x < nbk) fetch(blockIdx.x);
+    }
     for (int b = blockIdx.x; b < nbk; b += gridDim.x) {
+        const unsigned len = (unsigned)(e1 - e0);
 #pragma unroll
         for (int u = 0; u < BITS_PF; u++)
-            if (e0 + tid + (long long)u * BITS_THREADS < e1) bits_set(bm, (unsigned)ng, x[u]);
-        for (long long k0 = e0 + tid + (long long)BITS_PF * BITS_THREADS; k0 < e1;
-             k0 += (long long)BITS_THREADS * BITS_UNROLL) {
+            if ((unsigned)(tid + u * BITS_THREADS) < len) bits_set(bm, (unsigned)ng, x[u]);
+        for (unsigned k0 = (unsigned)(tid + BITS_PF * BITS_THREADS); k0 < len; k0 += BITS_THREADS * BITS_UNROLL) {
             unsigned y[BITS_UNROLL];
 #pragma unroll
-            for (int u = 0; u < BITS_UNROLL; u++) y[u] = ent[min(k0 + (long long)u * BITS_THREADS, e1 - 1)];
+            for (int u = 0; u < BITS_UNROLL; u++) y[u] = ent[e0 + min(k0 + (unsigned)(u * BITS_THREADS), len - 1u)];
 #pragma unroll
             for (int u = 0; u < BITS_UNROLL; u++)
-                if (k0 + (long long)u * BITS_THREADS < e1) bits_set(bm, (unsigned)ng, y[u]);
+                if (k0 + (unsigned)(u * BITS_THREADS) < len) bits_set(bm, (unsigned)ng, y[u]);
         }
         const long long c0 = e0, c1 = e1;
-        if (b + (int)gridDim.x < nbk) fetch(b + gridDim.x);
+        if (b + (int)gridDim.x < nbk) {
+            e0 = bstart[b + gridDim.x];
+            e1 = bstart[b + gridDim.x + 1];
+            if (e1 > e0) {
+                const unsigned *xb = ent + e0;
+                const unsigned last = (unsigned)(e1 - e0) - 1u;
+#pragma unroll
+                for (int u = 0; u < BITS_PF; u++) xn[u] = xb[min((unsigned)(tid + u * BITS_THREADS), last)];
+            }
+        }
         __syncthreads();
         // this wave's row: per-lane popcounts (the run's reads all issued
         // before the first is used; reads past the run are clamped into the
         // row and not counted), one wave scan
-        unsigned s = 0;
+        unsigned s = 0, cq[BITS_RUN];
 #pragma unroll
         for (int q = 0; q < BITS_RUN; q++) {
             const uint4 v = rb4[min(g0 + q, ng - 1)];
-            const unsigned c = __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
-            s += g0 + q < g1 ? c : 0u;
+            cq[q] = g0 + q < g1 ? __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w) : 0u;
+            s += cq[q];
         }
         const unsigned incl = wave_incl_sum(s);
         if (lane == 63) rdeg[r] = incl;
@@ -2143,49 +2176,109 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
         const int gr = b * BROWS + r;
         if (lane == 0 && gr < n) row[gr] = (int)(c0 + before);
         if (b == nbk - 1 && tid == 0) row[n] = (int)c1;
-        // read the lane's s columns back in order (one per step, moving to
-        // the next non-empty group when the current one is spent; the step
-        // takes the lowest bit of the low or high half without a branch),
-        // then clear the lane's groups
-        int *out = col + (c0 + before + (incl - s));
-        int g = g0;
-        unsigned long long lo = 0, hi = 0;
-        uint4 nx = rb4[min(g, ng - 1)];  // the next group, read one ahead
-        for (unsigned it = 0; it < s; it++) {
-            while (!(lo | hi)) {
-                lo = ((unsigned long long)nx.y << 32) | nx.x;
-                hi = ((unsigned long long)nx.w << 32) | nx.z;
-                g++;
-                nx = rb4[min(g, ng - 1)];
+        if (RANK) {
+            unsigned base = before + incl - s;
+            unsigned *gb = gbase + (long long)r * ng;
+#pragma unroll
+            for (int q = 0; q < BITS_RUN; q++)
+                if (g0 + q < g1) {
+                    gb[g0 + q] = base;
+                    base += cq[q];
+                }
+            __syncthreads();
+            int *colb = col + c0;
+#pragma unroll
+            for (int u = 0; u < BITS_PF; u++)
+                if ((unsigned)(tid + u * BITS_THREADS) < len) bits_put(bm4, gbase, (unsigned)ng, x[u], colb);
+            for (unsigned k0 = (unsigned)(tid + BITS_PF * BITS_THREADS); k0 < len;
+                 k0 += BITS_THREADS * BITS_UNROLL) {
+                unsigned y[BITS_UNROLL];
+#pragma unroll
+                for (int u = 0; u < BITS_UNROLL; u++) y[u] = ent[c0 + min(k0 + (unsigned)(u * BITS_THREADS), len - 1u)];
+#pragma unroll
+                for (int u = 0; u < BITS_UNROLL; u++)
+                    if (k0 + (unsigned)(u * BITS_THREADS) < len) bits_put(bm4, gbase, (unsigned)ng, y[u], colb);
             }
-            const bool inlo = lo != 0ull;
-            const unsigned long long w = inlo ? lo : hi;
-            const int c = __builtin_ctzll(w) + (inlo ? 0 : 64);
-            const unsigned long long w2 = w & (w - 1ull);
-            hi = inlo ? hi : w2;
-            lo = inlo ? w2 : 0ull;
-            *out++ = (g - 1) * 128 + c;
+            __syncthreads();  // every rank read before the clears
+        } else {
+            // read the lane's s columns back in order (one per step, moving
+            // to the next non-empty group when the current one is spent; the
+            // step takes the lowest bit of the low or high half without a
+            // branch)
+            int *out = col + (c0 + before + (incl - s));
+            int g = g0;
+            unsigned long long lo = 0, hi = 0;
+            uint4 nx = rb4[min(g, ng - 1)];  // the next group, read one ahead
+            for (unsigned it = 0; it < s; it++) {
+                while (!(lo | hi)) {
+                    lo = ((unsigned long long)nx.y << 32) | nx.x;
+                    hi = ((unsigned long long)nx.w << 32) | nx.z;
+                    g++;
+                    nx = rb4[min(g, ng - 1)];
+                }
+                const bool inlo = lo != 0ull;
+                const unsigned long long w = inlo ? lo : hi;
+                const int c = __builtin_ctzll(w) + (inlo ? 0 : 64);
+                const unsigned long long w2 = w & (w - 1ull);
+                hi = inlo ? hi : w2;
+                lo = inlo ? w2 : 0ull;
+                *out++ = (g - 1) * 128 + c;
+            }
         }
         if (s)
             for (int q = g0; q < g1; q++) rb4[q] = make_uint4(0u, 0u, 0u, 0u);
         __syncthreads();
+#pragma unroll
+        for (int u = 0; u < BITS_PF; u++) x[u] = xn[u];
     }
 }
 
-// LDS of bucket_rows_bits (0 when the bitmaps of a bucket do not fit)
-size_t rows_bits_lds(int n, size_t lds_per_block)
+// LDS of bucket_rows_bits<RANK> (0 when it does not fit): the bitmaps of a
+// bucket, 16 bytes per 128 columns and row, and with RANK 4 bytes more per
+// group for its base
+size_t rows_bits_lds(int n, size_t lds_per_block, bool rank)
 {
-    const size_t b = (size_t)BROWS * (size_t)((n + 127) >> 7) * 16;
+    const size_t b = (size_t)BROWS * (size_t)((n + 127) >> 7) * (rank ? 20 : 16);
     return b + 64 <= std::min<size_t>(lds_per_block, 160 * 1024) ? b : 0;
+}
+
+// The row kernel's variant and LDS: RANK where its bases fit, else the
+// read-back where the bitmaps fit, else lds = 0.
+struct BitsPlan {
+    bool rank = false;
+    size_t lds = 0;
+};
+BitsPlan rows_bits_plan(int n, size_t lds_per_block)
+{
+    BitsPlan p;
+    p.lds = rows_bits_lds(n, lds_per_block, true);
+    p.rank = p.lds != 0;
+    if (!p.rank) p.lds = rows_bits_lds(n, lds_per_block, false);
+    return p;
+}
+
+const void *rows_bits_fn(bool rank)
+{
+    return rank ? (const void *)bucket_rows_bits<true> : (const void *)bucket_rows_bits<false>;
 }
 
 // its grid: as many work-groups as the LDS of every CU holds (at most four
 // per CU), no more than there are buckets
 unsigned rows_bits_grid(const pg_ctx *ctx, int n, int nbk)
 {
-    const size_t l = std::max<size_t>(rows_bits_lds(n, ctx->lds_per_block), 1);
+    const size_t l = std::max<size_t>(rows_bits_plan(n, ctx->lds_per_block).lds, 1);
     const long long per_cu = std::max<long long>(1, std::min<long long>(4, (long long)(160 * 1024 / (l + 64))));
     return (unsigned)std::max<long long>(1, std::min<long long>(nbk, (long long)ctx->num_cu * per_cu));
+}
+
+void launch_rows_bits(const pg_ctx *ctx, const BitsPlan &bp, hipStream_t st, const long long *bstart,
+                      const unsigned *ent, int n, int nbk, int *row, int *col, long long cap)
+{
+    const unsigned grid = rows_bits_grid(ctx, n, nbk);
+    if (bp.rank)
+        bucket_rows_bits<true><<<grid, BITS_THREADS, bp.lds, st>>>(bstart, ent, n, nbk, row, col, cap);
+    else
+        bucket_rows_bits<false><<<grid, BITS_THREADS, bp.lds, st>>>(bstart, ent, n, nbk, row, col, cap);
 }
 
 // Bitmap waves and the smallest row length they take: as many waves as fit
@@ -3344,7 +3437,8 @@ struct CorrRun {
         PG_TRY(pg_bufT(ctx, "corr.ent", nnzs, &ent));
         PG_TRY(pg_bufT(ctx, "corr.row", (size_t)n + 2, &row));
         PG_TRY(pg_bufT(ctx, "corr.col", nnzs, &col));
-        const size_t lbits = weighted ? 0 : rows_bits_lds(n, ctx->lds_per_block);
+        const BitsPlan bp = weighted ? BitsPlan() : rows_bits_plan(n, ctx->lds_per_block);
+        const size_t lbits = bp.lds;
         if (!lbits) PG_TRY(pg_bufT(ctx, "corr.ctmp", nnzs, &ctmp));
         val = nullptr;
         if (weighted) {
@@ -3372,9 +3466,8 @@ struct CorrRun {
                 bucket_scatter<false><<<(unsigned)nsrc, SEG_THREADS, shs, st>>>(src, nbk, boff, bstart, ent, nullptr,
                                                                                 z64, t, (long long)nnzs);
             if (lbits) {
-                PG_HIP(ctx, set_shared_limit(ctx, (const void *)bucket_rows_bits, (int)lbits));
-                bucket_rows_bits<<<rows_bits_grid(ctx, n, nbk), BITS_THREADS, lbits, st>>>(bstart, ent, n, nbk, row,
-                                                                                         col, (long long)nnzs);
+                PG_HIP(ctx, set_shared_limit(ctx, rows_bits_fn(bp.rank), (int)lbits));
+                launch_rows_bits(ctx, bp, st, bstart, ent, n, nbk, row, col, (long long)nnzs);
                 PG_KERNEL_CHECK(ctx);
                 return PG_OK;
             }
@@ -4131,6 +4224,7 @@ struct PgCohortCorr {
     struct Subj {
         int n = 0, t = 0, ldp = 0, nr = 0, npad = 0, nc = 0, nbk = 0, nbm = 1, dbit = SORT_MAX, wx = 1;
         size_t rows_lds = 0, bits_lds = 0;
+        bool bits_rank = false;
         long long m1 = 0, kwant = 0, nst = 0, stride = 1, stl_off = 0, seg_cap = 0, ccap = 0;
         double f = 0.0, band = 0.0;
         bool graph = false;  // built in the graph (else outside it)
@@ -4212,7 +4306,9 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
         S.wx = (int)std::max<long long>(1, pg_cdiv(S.seg_cap, WIN_CHUNK));
         S.band = error_band(PREC_BF16X3, S.t);
         S.nbk = (int)pg_cdiv(S.n, BROWS);
-        S.bits_lds = rows_bits_lds(S.n, ctx->lds_per_block);
+        const BitsPlan bp = rows_bits_plan(S.n, ctx->lds_per_block);
+        S.bits_lds = bp.lds;
+        S.bits_rank = bp.rank;
         if (S.bits_lds) {
             max_bits_lds = std::max(max_bits_lds, S.bits_lds);
         } else {
@@ -4279,7 +4375,8 @@ pg_status pg_cohort_corr_plan(pg_ctx *ctx, int nsub, const int32_t *n, const int
     if (shs > 64 * 1024)
         PG_HIP(ctx, set_shared_limit(ctx, (const void *)bucket_scatter<false>, (int)shs));
     PG_HIP(ctx, set_shared_limit(ctx, (const void *)bucket_rows<false>, (int)max_rows_lds));
-    PG_HIP(ctx, set_shared_limit(ctx, (const void *)bucket_rows_bits, (int)max_bits_lds));
+    PG_HIP(ctx, set_shared_limit(ctx, rows_bits_fn(false), (int)max_bits_lds));
+    PG_HIP(ctx, set_shared_limit(ctx, rows_bits_fn(true), (int)max_bits_lds));
     *out = guard.release();
     return PG_OK;
 }
@@ -4372,12 +4469,15 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     bucket_scan_global<<<1, 1024, 0, st>>>(P->btot, S.nbk, P->bstart);
     bucket_scatter<false><<<(unsigned)nsrcb, SEG_THREADS, (size_t)S.nbk * 8, st>>>(es, S.nbk, P->boff, P->bstart,
                                                                                    P->ent, nullptr, P->z64, S.t, nnz);
-    if (S.bits_lds)
-        bucket_rows_bits<<<rows_bits_grid(ctx, S.n, S.nbk), BITS_THREADS, S.bits_lds, st>>>(P->bstart, P->ent, S.n,
-                                                                                          S.nbk, row, col, nnz);
-    else
+    if (S.bits_lds) {
+        BitsPlan bp;
+        bp.rank = S.bits_rank;
+        bp.lds = S.bits_lds;
+        launch_rows_bits(ctx, bp, st, P->bstart, P->ent, S.n, S.nbk, row, col, nnz);
+    } else {
         bucket_rows<false><<<(unsigned)S.nbk, ROWS_THREADS, S.rows_lds, st>>>(
             P->bstart, P->ent, nullptr, S.n, S.nbk, S.nbm, S.dbit, row, P->ctmp, nullptr, col, nullptr, nnz);
+    }
     cohort_corr_check<<<1, 256, 0, st>>>(P->ctl, c, P->seg_cnt, (int)nwg, S.seg_cap, S.ccap, S.kwant, row, S.n,
                                          status);
     PG_KERNEL_CHECK(ctx);

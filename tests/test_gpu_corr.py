@@ -258,6 +258,31 @@ def test_plain_workload_n5000(ctx, prec):
         f"gram_ms={info.ms_gram:.3f} csr_ms={info.ms_csr:.3f}")
 
 
+@pytest.mark.parametrize("n", [65_000, 66_000])
+def test_unweighted_rows_match_weighted_rows(ctx, n):
+    """Both unweighted row sorts against the weighted one, at full row length.
+
+    The unweighted rows are sorted in LDS bitmaps: at n = 65,000 the bitmaps
+    and one base per 128 columns fit 160 KB and every entry is placed by its
+    rank; at n = 66,000 only the bitmaps fit and the columns are read back
+    lane by lane.  The weighted build sorts its rows with another kernel, so
+    the same selection must come out with identical rows and columns, each
+    row strictly ascending.
+    """
+    x = _data(n, 24, 40, 9)
+    say(f"row sorts n={n}")
+    info, row, col, _ = _build(ctx, x, 0.2, sparsity=True)
+    winfo, wrow, wcol, _ = _build(ctx, x, 0.2, sparsity=True, weighted=True)
+    assert info.nnz == winfo.nnz == 2 * O.sparsity_k(n, 0.2)
+    np.testing.assert_array_equal(row, wrow)
+    np.testing.assert_array_equal(col, wcol)
+    nnz = int(row[-1])
+    inner = np.ones(nnz - 1, dtype=bool)
+    b = row[1:-1].astype(np.int64)
+    inner[b[(b > 0) & (b < nnz)] - 1] = False
+    assert (np.diff(col.astype(np.int64))[inner] > 0).all()
+
+
 _MULTI = {}
 
 

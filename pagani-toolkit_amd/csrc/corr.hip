@@ -2083,9 +2083,9 @@ __device__ __forceinline__ void bits_set(unsigned *bm, unsigned ng, unsigned x)
     atomicOr(&bm[(x >> 25) * ng * 4u + (c >> 5)], 1u << (c & 31));
 }
 
-// Entry x's column, written at its place in the bucket's output (colb = the
-// bucket's first column slot): its group's base plus the set bits below it.
-__device__ __forceinline__ void bits_put(const uint4 *bm4, const unsigned *gbase, unsigned ng, unsigned x, int *colb)
+// Entry x's place in the bucket's output: its group's base plus the set
+// bits below its column in the group.
+__device__ __forceinline__ unsigned bits_rank(const uint4 *bm4, const unsigned *gbase, unsigned ng, unsigned x)
 {
     const unsigned c = x & COLMASK;
     const unsigned gi = (x >> 25) * ng + (c >> 7);
@@ -2096,7 +2096,7 @@ __device__ __forceinline__ void bits_put(const uint4 *bm4, const unsigned *gbase
     pos += __popc(v.y & (w > 1u ? ~0u : (w == 1u ? m : 0u)));
     pos += __popc(v.z & (w > 2u ? ~0u : (w == 2u ? m : 0u)));
     pos += __popc(v.w & (w == 3u ? m : 0u));
-    colb[pos] = (int)c;
+    return pos;
 }
 
 template <bool RANK>
@@ -2187,19 +2187,58 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
                 }
             __syncthreads();
             int *colb = col + c0;
+            const unsigned words = (unsigned)(BROWS * ng * 4);
+            if (len <= min((unsigned)(BITS_PF * BITS_THREADS), words)) {
+                // every rank first; then the bitmaps' LDS takes the bucket's
+                // columns at their places, and the bucket goes out in whole
+                // 256-byte wave stores (stores straight to the ranks leave
+                // most L2 write requests carrying one or two columns); the
+                // bitmaps are zero again when the bucket is done
+                unsigned pos[BITS_PF];
 #pragma unroll
-            for (int u = 0; u < BITS_PF; u++)
-                if ((unsigned)(tid + u * BITS_THREADS) < len) bits_put(bm4, gbase, (unsigned)ng, x[u], colb);
-            for (unsigned k0 = (unsigned)(tid + BITS_PF * BITS_THREADS); k0 < len;
-                 k0 += BITS_THREADS * BITS_UNROLL) {
-                unsigned y[BITS_UNROLL];
+                for (int u = 0; u < BITS_PF; u++)
+                    pos[u] = (unsigned)(tid + u * BITS_THREADS) < len ? bits_rank(bm4, gbase, (unsigned)ng, x[u]) : 0u;
+                __syncthreads();  // every rank read before the bitmaps are overwritten
 #pragma unroll
-                for (int u = 0; u < BITS_UNROLL; u++) y[u] = ent[c0 + min(k0 + (unsigned)(u * BITS_THREADS), len - 1u)];
+                for (int u = 0; u < BITS_PF; u++)
+                    if ((unsigned)(tid + u * BITS_THREADS) < len) bm[pos[u]] = x[u] & COLMASK;
+                const unsigned l4 = (len + 3u) >> 2;
+                if ((unsigned)tid < 4u * l4 - len) bm[len + tid] = 0u;
+                for (unsigned q = l4 + tid; q < (unsigned)(BROWS * ng); q += BITS_THREADS)
+                    bm4[q] = make_uint4(0u, 0u, 0u, 0u);
+                __syncthreads();  // the staged columns complete
+                unsigned v[BITS_PF];
+                const unsigned last = len ? len - 1u : 0u;
 #pragma unroll
-                for (int u = 0; u < BITS_UNROLL; u++)
-                    if (k0 + (unsigned)(u * BITS_THREADS) < len) bits_put(bm4, gbase, (unsigned)ng, y[u], colb);
+                for (int u = 0; u < BITS_PF; u++) v[u] = bm[min((unsigned)(tid + u * BITS_THREADS), last)];
+#pragma unroll
+                for (int u = 0; u < BITS_PF; u++) {
+                    const unsigned k = (unsigned)(tid + u * BITS_THREADS);
+                    if (k < len) {
+                        colb[k] = (int)v[u];
+                        bm[k] = 0u;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < BITS_PF; u++)
+                    if ((unsigned)(tid + u * BITS_THREADS) < len)
+                        colb[bits_rank(bm4, gbase, (unsigned)ng, x[u])] = (int)(x[u] & COLMASK);
+                for (unsigned k0 = (unsigned)(tid + BITS_PF * BITS_THREADS); k0 < len;
+                     k0 += BITS_THREADS * BITS_UNROLL) {
+                    unsigned y[BITS_UNROLL];
+#pragma unroll
+                    for (int u = 0; u < BITS_UNROLL; u++)
+                        y[u] = ent[c0 + min(k0 + (unsigned)(u * BITS_THREADS), len - 1u)];
+#pragma unroll
+                    for (int u = 0; u < BITS_UNROLL; u++)
+                        if (k0 + (unsigned)(u * BITS_THREADS) < len)
+                            colb[bits_rank(bm4, gbase, (unsigned)ng, y[u])] = (int)(y[u] & COLMASK);
+                }
+                __syncthreads();  // every rank read before the clears
+                if (s)
+                    for (int q = g0; q < g1; q++) rb4[q] = make_uint4(0u, 0u, 0u, 0u);
             }
-            __syncthreads();  // every rank read before the clears
         } else {
             // read the lane's s columns back in order (one per step, moving
             // to the next non-empty group when the current one is spent; the
@@ -2224,9 +2263,9 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
                 lo = inlo ? w2 : 0ull;
                 *out++ = (g - 1) * 128 + c;
             }
+            if (s)
+                for (int q = g0; q < g1; q++) rb4[q] = make_uint4(0u, 0u, 0u, 0u);
         }
-        if (s)
-            for (int q = g0; q < g1; q++) rb4[q] = make_uint4(0u, 0u, 0u, 0u);
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < BITS_PF; u++) x[u] = xn[u];

@@ -41,7 +41,7 @@ STAGE_STEPS = 2  # untimed steps after the timed ones, every stage timed
 # (tuned supertile edge 4), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
 # (KiB per dispatch), each scaled by what the counter reads for a 1 GiB device
 # copy (FETCH_SIZE x2, WRITE_SIZE x1); profiles/gram2_emit_n58000_t240_counters.md
-GRAM_EMIT_TRAFFIC = 2718961.8 * 1024 * 2 + 253292.5 * 1024
+GRAM_EMIT_TRAFFIC = 2683291.0 * 1024 * 2 + 253290.9 * 1024
 
 METRIC = "per-subject wall time N=58k; corr TFLOP/s+MFMA%; BC GTEPS; BF16x3 vs FP32"
 

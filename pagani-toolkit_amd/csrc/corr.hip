@@ -2047,8 +2047,9 @@ __global__ __launch_bounds__(ROWS_THREADS, W ? 4 : 8) void bucket_rows(const lon
 // between buckets.  Wave r counts row r: each lane owns the run of 128-column
 // groups [lane * ng / 64, (lane + 1) * ng / 64) of the row.  Then, with RANK
 // (the bitmaps and one 32-bit base per group fit LDS), every group's base
-// (its first column's place in the bucket's output) goes to LDS and every
-// entry writes its column at base + the set bits below it in its group, so
+// (its first column's place in its row) goes to LDS with the row's count,
+// and every entry's place is its row's start (a scan of the row counts in
+// lanes 0..15) + its group's base + the set bits below it in the group, so
 // the bucket's work is spread evenly over the work-group; without RANK each
 // lane reads its run's columns back in order, one per step.  The first
 // BITS_PF entries per thread of a work-group's next bucket are loaded before
@@ -2083,7 +2084,7 @@ __device__ __forceinline__ void bits_set(unsigned *bm, unsigned ng, unsigned x)
     atomicOr(&bm[(x >> 25) * ng * 4u + (c >> 5)], 1u << (c & 31));
 }
 
-// Entry x's place in the bucket's output: its group's base plus the set
+// Entry x's place in its row: its group's base within the row plus the set
 // bits below its column in the group.
 __device__ __forceinline__ unsigned bits_rank(const uint4 *bm4, const unsigned *gbase, unsigned ng, unsigned x)
 {
@@ -2121,7 +2122,7 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
     const int g0 = (lane * ng) >> 6, g1 = ((lane + 1) * ng) >> 6;
     // this bucket's first entries (x) and the next bucket's (xn); bucket
     // sizes are below cap <= INT32_MAX, so offsets within one are 32-bit
-    unsigned x[BITS_PF], xn[BITS_PF];
+    unsigned x[BITS_PF] = {}, xn[BITS_PF] = {};
     long long e0 = 0, e1 = 0;
     if (blockIdx.x < nbk) {
         e0 = bstart[blockIdx.x];
@@ -2170,14 +2171,8 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
         }
         const unsigned incl = wave_incl_sum(s);
         if (lane == 63) rdeg[r] = incl;
-        __syncthreads();
-        unsigned before = 0;
-        for (int q = 0; q < r; q++) before += rdeg[q];
-        const int gr = b * BROWS + r;
-        if (lane == 0 && gr < n) row[gr] = (int)(c0 + before);
-        if (b == nbk - 1 && tid == 0) row[n] = (int)c1;
-        if (RANK) {
-            unsigned base = before + incl - s;
+        if (RANK) {  // group bases within the row, before the row starts are known
+            unsigned base = incl - s;
             unsigned *gb = gbase + (long long)r * ng;
 #pragma unroll
             for (int q = 0; q < BITS_RUN; q++)
@@ -2185,7 +2180,16 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
                     gb[g0 + q] = base;
                     base += cq[q];
                 }
-            __syncthreads();
+        }
+        __syncthreads();
+        // row q's start within the bucket in lane q of every wave
+        const unsigned rd = lane < BROWS ? rdeg[lane] : 0u;
+        const unsigned rstart = wave_incl_sum(rd) - rd;
+        const unsigned before = (unsigned)__builtin_amdgcn_readlane((int)rstart, r);
+        const int gr = b * BROWS + r;
+        if (lane == 0 && gr < n) row[gr] = (int)(c0 + before);
+        if (b == nbk - 1 && tid == 0) row[n] = (int)c1;
+        if (RANK) {
             int *colb = col + c0;
             const unsigned words = (unsigned)(BROWS * ng * 4);
             if (len <= min((unsigned)(BITS_PF * BITS_THREADS), words)) {
@@ -2196,8 +2200,11 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
                 // bitmaps are zero again when the bucket is done
                 unsigned pos[BITS_PF];
 #pragma unroll
-                for (int u = 0; u < BITS_PF; u++)
-                    pos[u] = (unsigned)(tid + u * BITS_THREADS) < len ? bits_rank(bm4, gbase, (unsigned)ng, x[u]) : 0u;
+                for (int u = 0; u < BITS_PF; u++) {
+                    const unsigned rs = (unsigned)__shfl((int)rstart, (int)(x[u] >> 25));
+                    pos[u] = (unsigned)(tid + u * BITS_THREADS) < len ? rs + bits_rank(bm4, gbase, (unsigned)ng, x[u])
+                                                                      : 0u;
+                }
                 __syncthreads();  // every rank read before the bitmaps are overwritten
 #pragma unroll
                 for (int u = 0; u < BITS_PF; u++)
@@ -2221,9 +2228,11 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
                 }
             } else {
 #pragma unroll
-                for (int u = 0; u < BITS_PF; u++)
+                for (int u = 0; u < BITS_PF; u++) {
+                    const unsigned rs = (unsigned)__shfl((int)rstart, (int)(x[u] >> 25));
                     if ((unsigned)(tid + u * BITS_THREADS) < len)
-                        colb[bits_rank(bm4, gbase, (unsigned)ng, x[u])] = (int)(x[u] & COLMASK);
+                        colb[rs + bits_rank(bm4, gbase, (unsigned)ng, x[u])] = (int)(x[u] & COLMASK);
+                }
                 for (unsigned k0 = (unsigned)(tid + BITS_PF * BITS_THREADS); k0 < len;
                      k0 += BITS_THREADS * BITS_UNROLL) {
                     unsigned y[BITS_UNROLL];
@@ -2231,9 +2240,11 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
                     for (int u = 0; u < BITS_UNROLL; u++)
                         y[u] = ent[c0 + min(k0 + (unsigned)(u * BITS_THREADS), len - 1u)];
 #pragma unroll
-                    for (int u = 0; u < BITS_UNROLL; u++)
+                    for (int u = 0; u < BITS_UNROLL; u++) {
+                        const unsigned rs = (unsigned)__shfl((int)rstart, (int)(y[u] >> 25));
                         if (k0 + (unsigned)(u * BITS_THREADS) < len)
-                            colb[bits_rank(bm4, gbase, (unsigned)ng, y[u])] = (int)(y[u] & COLMASK);
+                            colb[rs + bits_rank(bm4, gbase, (unsigned)ng, y[u])] = (int)(y[u] & COLMASK);
+                    }
                 }
                 __syncthreads();  // every rank read before the clears
                 if (s)

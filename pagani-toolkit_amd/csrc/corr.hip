@@ -1648,8 +1648,8 @@ __global__ __launch_bounds__(SEG_THREADS) void bucket_count(EdgeSrc s, int nbk, 
 // L2 <= r~ <= U2 go to the candidate list, staged in LDS and reserved once
 // per work-group (beyond WC_STAGE, one by one).  Without a window
 // (ctl->ok == 0) nothing is kept.  Does the work of window_kernel + bucket_count over
-// the segments; the selected list's counts follow from bucket_count (g0 =
-// nseg) once it exists.
+// the segments; work-group 0 clears the selected list's row (row nseg), which
+// select_write counts as it writes the list.
 constexpr int WC_STAGE = 1024;
 __global__ __launch_bounds__(SEG_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void window_count(EdgeSrc s, const SelCtl *__restrict__ ctl, int nbk,
                                                             int nsrc, unsigned *__restrict__ cnt,
@@ -1665,6 +1665,8 @@ __global__ __launch_bounds__(SEG_THREADS) __attribute__((amdgpu_waves_per_eu(8))
     __shared__ unsigned long long s_nd, s_base;
     const int tid = threadIdx.x, g = blockIdx.x;
     for (int b = tid; b < nbk; b += SEG_THREADS) sc[b] = 0u;
+    if (g == 0)  // the selected list's row, counted by select_write
+        for (int b = tid; b < nbk; b += SEG_THREADS) cnt[(long long)gridDim.x * nbk + b] = 0u;
     if (tid == 0) {
         s_nc = 0u;
         s_nd = 0ull;
@@ -3016,7 +3018,8 @@ __global__ __launch_bounds__(256) void select_write(const int2 *__restrict__ can
                                                    const float *__restrict__ rfast, const SelCut *__restrict__ cut,
                                                    int2 *__restrict__ extra,
                                                    unsigned long long *__restrict__ extra_cnt,
-                                                   unsigned long long *__restrict__ flips)
+                                                   unsigned long long *__restrict__ flips,
+                                                   unsigned *__restrict__ bcnt_row)
 {
     if (!cut->ok) return;
     const long long c = cut->c;
@@ -3041,8 +3044,13 @@ __global__ __launch_bounds__(256) void select_write(const int2 *__restrict__ can
         unsigned long long base = 0;
         if (lane == 0) base = atomicAdd(extra_cnt, (unsigned long long)__popcll(m));
         base = __shfl(base, 0);
-        if (sel)
+        if (sel) {
             extra[base + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = e;
+            if (bcnt_row) {  // the list's CSR bucket counts, both directions
+                atomicAdd(&bcnt_row[e.x >> BSH], 1u);
+                atomicAdd(&bcnt_row[e.y >> BSH], 1u);
+            }
+        }
     }
     for (int o = 32; o > 0; o >>= 1) fl += __shfl_xor(fl, o);
     if (lane == 0 && fl) atomicAdd(flips, fl);
@@ -3471,7 +3479,8 @@ struct CorrRun {
     }
 
     // CSR build (bucketed counting sort), launched without a host check.
-    // seg_counted: the segments' bucket counts are in place (window_count).
+    // seg_counted: every source's bucket counts are in place (the segments'
+    // from window_count, the selected list's from select_write).
     pg_status launch_csr(const EdgeSrc &src, long long nnz, bool seg_counted = false)
     {
         if (nnz > INT32_MAX) return pg_fail(ctx, PG_ENOTSUP, "nnz %lld exceeds int32 CSR", (long long)nnz);
@@ -3498,10 +3507,7 @@ struct CorrRun {
         }
         if (n > 0) {
             const size_t shb = (size_t)nbk * 4;
-            if (seg_counted)
-                bucket_count<<<1, SEG_THREADS, shb, st>>>(src, nbk, (int)nwg, nsrc, bcnt);
-            else
-                bucket_count<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(src, nbk, 0, nsrc, bcnt);
+            if (!seg_counted) bucket_count<<<(unsigned)nsrc, SEG_THREADS, shb, st>>>(src, nbk, 0, nsrc, bcnt);
             bucket_scan_local<<<(unsigned)pg_cdiv(nbk, SCAN_BK), SCAN_BK * SCAN_SG, 0, st>>>(bcnt, nsrc, nbk, boff,
                                                                                           btot);
             bucket_scan_global<<<1, 1024, 0, st>>>(btot, nbk, bstart);
@@ -3685,8 +3691,14 @@ pg_status window_select(CorrRun &R, const SelCtl *ctl, long long ccap, long long
     PG_TRY(pg_bufT(R.ctx, "corr.selcut", 1, &cut));
     select_kernel<<<1, 1024, 0, st>>>(win->cand, win->cand64, win->cand_r, c + 0, ccap, kwant, win->dpart, win->nd,
                                       win->extra, c + 1, tstar_d, c + 2, (int *)(c + 4), win->sh, c + 3, cut);
+    unsigned *bcnt_row = nullptr;  // with the segments' counts, the list's row of them
+    if (count_segments) {
+        unsigned *bcnt;
+        PG_TRY(pg_bufT(R.ctx, "corr.bcnt", (size_t)R.nsrc * R.nbk, &bcnt));
+        bcnt_row = bcnt + (size_t)R.nwg * R.nbk;
+    }
     select_write<<<(unsigned)std::min<long long>(pg_cdiv(ccap, 256), 4LL * std::max(R.ctx->num_cu, 1)), 256, 0, st>>>(
-        win->cand, win->cand64, win->cand_r, cut, win->extra, c + 1, c + 2);
+        win->cand, win->cand64, win->cand_r, cut, win->extra, c + 1, c + 2, bcnt_row);
     PG_KERNEL_CHECK(R.ctx);
     return PG_OK;
 }
@@ -4510,10 +4522,9 @@ pg_status pg_cohort_corr_enqueue(PgCohortCorr *P, int s, const float *dbold, int
     select_kernel<<<1, 1024, 0, st>>>(P->cand, P->cand64, P->cand_r, c + 0, S.ccap, S.kwant, P->dpart, nd, P->extra,
                                       c + 1, P->tstar, c + 2, (int *)(c + 4), shh, c + 3, P->selcut);
     select_write<<<(unsigned)std::min<long long>(pg_cdiv(S.ccap, 256), 4LL * std::max(ctx->num_cu, 1)), 256, 0, st>>>(
-        P->cand, P->cand64, P->cand_r, P->selcut, P->extra, c + 1, c + 2);
-    // CSR straight into the subject's outputs; the selected list's counts
+        P->cand, P->cand64, P->cand_r, P->selcut, P->extra, c + 1, c + 2, P->bcnt + (size_t)nwg * S.nbk);
+    // CSR straight into the subject's outputs (every source's counts in place)
     const long long nnz = 2 * S.kwant;
-    bucket_count<<<1, SEG_THREADS, (size_t)S.nbk * 4, st>>>(es, S.nbk, (int)nwg, nsrcb, P->bcnt);
     bucket_scan_local<<<(unsigned)pg_cdiv(S.nbk, SCAN_BK), SCAN_BK * SCAN_SG, 0, st>>>(P->bcnt, nsrcb, S.nbk,
                                                                                      P->boff, P->btot);
     bucket_scan_global<<<1, 1024, 0, st>>>(P->btot, S.nbk, P->bstart);

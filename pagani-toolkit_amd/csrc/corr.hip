@@ -1754,30 +1754,33 @@ __global__ __launch_bounds__(SCAN_BK * SCAN_SG) void bucket_scan_local(const uns
     }
 }
 
-// bstart = exclusive scan of the bucket totals; then off[b][g] += bstart[b].
+// bstart = exclusive scan of the bucket totals (64-bit: a selection that did
+// not hold can count more than INT32_MAX entries); one wave scan per wave,
+// then each wave adds the totals of the waves before it.
 __global__ __launch_bounds__(1024) void bucket_scan_global(const long long *__restrict__ tot, int nbk,
                                                            long long *__restrict__ bstart)
 {
-    __shared__ long long part[1024];
-    const int tid = threadIdx.x;
+    __shared__ long long wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int per = (nbk + 1023) / 1024;
     const int b0 = tid * per, b1 = min(nbk, b0 + per);
     long long s = 0;
     for (int b = b0; b < b1; b++) s += tot[b];
-    part[tid] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        long long v = tid >= o ? part[tid - o] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+    long long incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
     }
-    long long run = part[tid] - s;
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    long long run = incl - s;
+    for (int q = 0; q < w; q++) run += wsum[q];
     for (int b = b0; b < b1; b++) {
         bstart[b] = run;
         run += tot[b];
     }
-    if (tid == 1023) bstart[nbk] = part[1023];
+    if (tid == 1023) bstart[nbk] = run;
 }
 
 // Entries beyond `cap` (a selection that did not hold; the caller checks
@@ -2065,6 +2068,11 @@ constexpr int BITS_PF = 12;
 // while ng <= 64 * BITS_RUN
 constexpr int BITS_RUN = 10;
 static_assert((size_t)BROWS * 16 * (64 * BITS_RUN + 1) + 64 > 160 * 1024, "a lane's run can exceed BITS_RUN groups");
+// with RANK the bitmaps and the group bases (20 bytes per group and row) fit
+// only while ng <= 64 * BITS_RUN_RANK
+constexpr int BITS_RUN_RANK = 8;
+static_assert((size_t)BROWS * 20 * (64 * BITS_RUN_RANK + 1) + 64 > 160 * 1024,
+              "a lane's run can exceed BITS_RUN_RANK groups");
 
 // Inclusive sum over the 64 lanes of a wave (DPP: shifts within each row of
 // 16 lanes, then the row totals broadcast into the rows above); every lane
@@ -2164,9 +2172,10 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
         // this wave's row: per-lane popcounts (the run's reads all issued
         // before the first is used; reads past the run are clamped into the
         // row and not counted), one wave scan
-        unsigned s = 0, cq[BITS_RUN];
+        constexpr int RUN = RANK ? BITS_RUN_RANK : BITS_RUN;
+        unsigned s = 0, cq[RUN];
 #pragma unroll
-        for (int q = 0; q < BITS_RUN; q++) {
+        for (int q = 0; q < RUN; q++) {
             const uint4 v = rb4[min(g0 + q, ng - 1)];
             cq[q] = g0 + q < g1 ? __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w) : 0u;
             s += cq[q];
@@ -2177,7 +2186,7 @@ __global__ __launch_bounds__(BITS_THREADS) void bucket_rows_bits(const long long
             unsigned base = incl - s;
             unsigned *gb = gbase + (long long)r * ng;
 #pragma unroll
-            for (int q = 0; q < BITS_RUN; q++)
+            for (int q = 0; q < RUN; q++)
                 if (g0 + q < g1) {
                     gb[g0 + q] = base;
                     base += cq[q];

@@ -1292,7 +1292,15 @@ __global__ __launch_bounds__(256) void hist_reduce(const uint32_t *__restrict__ 
     const int bl = threadIdx.x & 31, rg = threadIdx.x >> 5;
     const int b = blockIdx.x * 32 + bl;
     unsigned long long s = 0;
-    for (int g = rg; g < nwg; g += 8) s += h[(long long)g * NBIN + b];
+    int g = rg;
+    for (; g + 8 * 15 < nwg; g += 8 * 16) {  // 16 rows in flight per thread
+        unsigned v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = h[(long long)(g + 8 * u) * NBIN + b];
+#pragma unroll
+        for (int u = 0; u < 16; u++) s += v[u];
+    }
+    for (; g < nwg; g += 8) s += h[(long long)g * NBIN + b];
     part[rg][bl] = s;
     __syncthreads();
     if (rg == 0) {

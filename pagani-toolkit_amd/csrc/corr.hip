@@ -1734,7 +1734,7 @@ __global__ __launch_bounds__(SEG_THREADS) __attribute__((amdgpu_waves_per_eu(8))
 // 32 consecutive buckets: lane b % 32 of each wave sums its bucket over one
 // of SCAN_SG runs of segments (32 buckets' counts of a segment are one
 // 128-byte read), the partial sums are scanned per bucket, then written back.
-constexpr int SCAN_BK = 32, SCAN_SG = 32;
+constexpr int SCAN_BK = 32, SCAN_SG = 32, SCAN_PER = 24;
 __global__ __launch_bounds__(SCAN_BK * SCAN_SG) void bucket_scan_local(const unsigned *__restrict__ cnt, int nsrc,
                                                                          int nbk, long long *__restrict__ off,
                                                                          long long *__restrict__ tot)
@@ -1744,8 +1744,18 @@ __global__ __launch_bounds__(SCAN_BK * SCAN_SG) void bucket_scan_local(const uns
     const int b = blockIdx.x * SCAN_BK + bl;
     const int per = (nsrc + SCAN_SG - 1) / SCAN_SG;
     const int g0 = sg * per, g1 = min(nsrc, g0 + per);
+    // runs of at most SCAN_PER segments are held in registers between the two
+    // passes (all their reads in flight at once); longer runs read twice
+    const bool held = per <= SCAN_PER;
+    unsigned v[SCAN_PER];
     long long sum = 0;
-    if (b < nbk) {
+    if (held) {
+#pragma unroll
+        for (int u = 0; u < SCAN_PER; u++)
+            v[u] = b < nbk && g0 + u < g1 ? cnt[(long long)(g0 + u) * nbk + b] : 0u;
+#pragma unroll
+        for (int u = 0; u < SCAN_PER; u++) sum += v[u];
+    } else if (b < nbk) {
 #pragma unroll 8
         for (int g = g0; g < g1; g++) sum += cnt[(long long)g * nbk + b];
     }
@@ -1754,9 +1764,18 @@ __global__ __launch_bounds__(SCAN_BK * SCAN_SG) void bucket_scan_local(const uns
     long long run = 0;
     for (int q = 0; q < sg; q++) run += part[q][bl];
     if (b < nbk) {
-        for (int g = g0; g < g1; g++) {
-            off[(long long)g * nbk + b] = run;
-            run += cnt[(long long)g * nbk + b];
+        if (held) {
+#pragma unroll
+            for (int u = 0; u < SCAN_PER; u++)
+                if (g0 + u < g1) {
+                    off[(long long)(g0 + u) * nbk + b] = run;
+                    run += v[u];
+                }
+        } else {
+            for (int g = g0; g < g1; g++) {
+                off[(long long)g * nbk + b] = run;
+                run += cnt[(long long)g * nbk + b];
+            }
         }
         if (sg == SCAN_SG - 1) tot[b] = run;
     }
@@ -2539,7 +2558,15 @@ __global__ void hist_sum(const uint32_t *__restrict__ h, int nrow, int nbin, uns
     const int rpb = (nrow + gridDim.y - 1) / gridDim.y;
     const int g0 = blockIdx.y * rpb, g1 = min(nrow, g0 + rpb);
     unsigned long long s = 0;
-    for (int g = g0; g < g1; g++) s += h[(long long)g * nbin + b];
+    int g = g0;
+    for (; g + 15 < g1; g += 16) {  // 16 rows in flight per thread
+        unsigned v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = h[(long long)(g + u) * nbin + b];
+#pragma unroll
+        for (int u = 0; u < 16; u++) s += v[u];
+    }
+    for (; g < g1; g++) s += h[(long long)g * nbin + b];
     if (s) atomicAdd(&out[b], s);
 }
 

@@ -2427,22 +2427,29 @@ constexpr int NBIN2 = 16384;   // fine histogram over the emitted set
 // the crossing run.
 __device__ int top_bin_reaching(const unsigned long long *h, int nb, double target, unsigned long long *total)
 {
-    __shared__ unsigned long long sfx[1024];
+    __shared__ unsigned long long wsum[16];
     __shared__ int s_bin;
-    const int tid = threadIdx.x, per = nb / 1024;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, per = nb / 1024;
     unsigned long long own = 0;
+#pragma unroll 16
     for (int q = 0; q < per; q++) own += h[tid * per + q];
-    sfx[tid] = own;
+    // prefix sums over the threads: a wave scan, then the waves' totals
+    unsigned long long incl = own;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
     if (tid == 0) s_bin = -1;
     __syncthreads();
-    // inclusive suffix sums: sfx[t] = sum over threads t' >= t
-    for (int o = 1; o < 1024; o <<= 1) {
-        const unsigned long long y = tid + o < 1024 ? sfx[tid + o] : 0ull;
-        __syncthreads();
-        sfx[tid] += y;
-        __syncthreads();
+    unsigned long long before = 0, all = 0;
+    for (int q = 0; q < 16; q++) {
+        const unsigned long long v = wsum[q];
+        all += v;
+        before += q < w ? v : 0ull;
     }
-    const unsigned long long above = sfx[tid] - own;  // threads after this one
+    const unsigned long long above = all - before - incl;  // threads after this one
     if ((double)(above + own) >= target && (double)above < target) {
         unsigned long long cum = above;
         for (int q = per - 1; q >= 0; q--) {
@@ -2455,7 +2462,7 @@ __device__ int top_bin_reaching(const unsigned long long *h, int nb, double targ
     }
     if (target <= 0.0 && tid == 1023) s_bin = nb - 1;
     __syncthreads();
-    if (total) *total = sfx[0];
+    if (total) *total = all;
     const int r = s_bin;
     __syncthreads();  // shared state is reused by the next call
     return r;
